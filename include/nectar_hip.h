@@ -1,0 +1,276 @@
+/*
+ * nectar_hip.h — C ABI of the MI355X path-tracing library (libnectar_hip.so).
+ *
+ * The library replaces the device half of NectarRender's render pass.  The
+ * pybind11 module `_pathtracer` (nectarrender_amd/csrc/bindings.cpp) lowers the
+ * reference's Scene/Hittable/Material/Camera objects into the plain structs
+ * below and calls these entry points; tests call them through ctypes.
+ *
+ * Conventions
+ *   - every entry point returns NR_OK (0) or an NR_ERR_* code; the message of
+ *     the last failure on the calling thread is available from nr_last_error().
+ *     Nothing aborts the process (the reference aborts on CUDA errors,
+ *     csrc/core/include/core/cuda.h:25-35).
+ *   - all pointers in descriptors are host pointers; the library copies what it
+ *     needs during the call, so descriptors may be freed on return.
+ *   - images and AOV layers are planar CHW; AOV pixel index = y*W + x, channel
+ *     plane offset = c*H*W (csrc/data/include/data/utils.h:18-20).
+ *   - H = camera resolution[0], W = camera resolution[1]
+ *     (csrc/engine/include/engine/camera.h:176-177, 357-358).
+ */
+#ifndef NECTAR_HIP_H
+#define NECTAR_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NR_ABI_VERSION 1
+
+enum nr_status {
+    NR_OK = 0,
+    NR_ERR_INVALID = 1,   /* bad argument / descriptor */
+    NR_ERR_HIP = 2,       /* HIP runtime failure */
+    NR_ERR_STATE = 3,     /* call order (e.g. render before scene upload) */
+    NR_ERR_LIMIT = 4      /* a size limit of this library was exceeded */
+};
+
+/* HittableType, csrc/hittable/include/hittable/hittable.h:13-15 */
+enum nr_hittable_type {
+    NR_HIT_NULL = 0,
+    NR_HIT_QUAD = 1,
+    NR_HIT_SPHERE = 2,
+    NR_HIT_CUBE = 3,
+    NR_HIT_MESH = 4,
+    NR_HIT_CONSTANT_MEDIUM = 5,
+    NR_HIT_OBJECT_LIGHT = 6
+};
+
+/* MaterialType, csrc/material/include/material/material.h:374-376 */
+enum nr_material_type {
+    NR_MAT_NULL = 0,
+    NR_MAT_LAMBERTIAN = 1,
+    NR_MAT_PBR = 2,
+    NR_MAT_DIELECTRIC = 3,
+    NR_MAT_EMISSIVE = 4,
+    NR_MAT_ISOTROPIC = 5
+};
+
+/* SkylightType, csrc/light/include/skylight.h:131 */
+enum nr_sky_type { NR_SKY_NULL = 0, NR_SKY_SIMPLE = 1, NR_SKY_HDRI = 2 };
+
+/* LayerType, csrc/data/include/data/render_layers.h:11-14 */
+enum nr_layer {
+    NR_LAYER_BEAUTY = 0,
+    NR_LAYER_DIFFUSE = 1,
+    NR_LAYER_SPECULAR = 2,
+    NR_LAYER_OBJECT_NORMAL = 3,
+    NR_LAYER_WORLD_NORMAL = 4,
+    NR_LAYER_SHADOW = 5,
+    NR_LAYER_DEPTH = 6,
+    NR_LAYER_EMISSION = 7,
+    NR_LAYER_OBJECT_ID = 8
+};
+
+/* Estimator. PARITY reproduces trace_full_kernel draw for draw
+ * (csrc/engine/src/trace.cu:47-79).  FAST adds next-event estimation to the
+ * HDRI (importance sampled) and ObjectLights with power-heuristic MIS; it is
+ * checked by convergence, not per pixel. */
+enum nr_mode { NR_MODE_PARITY = 0, NR_MODE_FAST = 1 };
+
+/* One hittable.  Transform(position, rotation_deg, scale) semantics of
+ * csrc/core/include/core/transform.h:25-34; spheres use rotation 0, scale 1
+ * (Transform(position), transform.h:17-23). */
+typedef struct nr_object_desc {
+    int32_t type;          /* nr_hittable_type */
+    int32_t bound_type;    /* CONSTANT_MEDIUM / OBJECT_LIGHT: type of the bound object */
+    int32_t material;      /* index into nr_scene_desc.materials */
+    int32_t mesh;          /* index into nr_scene_desc.meshes when (bound_)type is MESH, else -1 */
+    float position[3];
+    float rotation[3];     /* Euler degrees, R = Rz * Ry * Rx (matrix.h:94-96) */
+    float scale[3];
+    float radius;          /* sphere radius as given to Hittable.sphere (FMIN is added inside) */
+    float density;         /* constant medium density */
+} nr_object_desc;
+
+/* TextureView (csrc/material/include/material/texture.h:49-91). */
+typedef struct nr_texture_desc {
+    int32_t image;         /* index into nr_scene_desc.images; -1 = constant colour */
+    float color[3];
+    float scale;           /* uv scale of image textures */
+} nr_texture_desc;
+
+typedef struct nr_image_desc {
+    int32_t C, H, W;       /* C >= 3 */
+    const uint8_t* data;   /* planar CHW uint8 */
+} nr_image_desc;
+
+/* One material.  tex[] slots: LAMBERTIAN {albedo}; PBR {albedo, roughness,
+ * metallic, emission, normal}; EMISSIVE {texture}; ISOTROPIC {texture}. */
+typedef struct nr_material_desc {
+    int32_t type;          /* nr_material_type */
+    nr_texture_desc tex[5];
+    float tint[3];         /* dielectric */
+    float ior;             /* dielectric */
+    float brightness;      /* emissive */
+} nr_material_desc;
+
+/* MeshVertex (csrc/hittable/include/hittable/mesh.h:10-16). */
+typedef struct nr_mesh_desc {
+    int32_t n_vertices;
+    int32_t n_triangles;
+    const float* vertices;     /* n_vertices x 11: position[3] normal[3] tangent[3] uv[2] */
+    const uint32_t* triangles; /* n_triangles x 3 vertex indices */
+} nr_mesh_desc;
+
+typedef struct nr_sky_desc {
+    int32_t type;          /* nr_sky_type */
+    float start[3], end[3];/* SIMPLE */
+    float rotation;        /* HDRI, degrees about +y */
+    float intensity;       /* HDRI */
+    int32_t C, H, W;       /* HDRI, C >= 3 */
+    const float* data;     /* HDRI planar CHW fp32 */
+} nr_sky_desc;
+
+/* Objects are listed in registration order: hittables then lights
+ * (csrc/scene/include/scene.h:31-39); object_id = position in this list.
+ * materials[0] is the registry's default material (registry.h:23-26). */
+typedef struct nr_scene_desc {
+    int32_t n_objects;
+    const nr_object_desc* objects;
+    int32_t n_materials;
+    const nr_material_desc* materials;
+    int32_t n_images;
+    const nr_image_desc* images;
+    int32_t n_meshes;
+    const nr_mesh_desc* meshes;
+    nr_sky_desc sky;
+} nr_scene_desc;
+
+/* CameraParams (csrc/engine/include/engine/camera.h:68-136). */
+typedef struct nr_camera_desc {
+    float resolution[2];   /* (H, W) as the reference stores it */
+    float position[3];
+    float rotation[3];     /* Euler degrees */
+    uint32_t samples_per_pixel;
+    float focal_length;
+    float focus_distance;
+    float aperture;
+    float sensor_width;
+    float shutter_speed;
+} nr_camera_desc;
+
+/* One call renders sample passes [sample_begin, sample_end) of a view and
+ * folds each into the view's running-mean AOVs exactly like
+ * RenderEngine::render (csrc/engine/src/engine.cpp:42-74).  TraceConfig
+ * fields: csrc/engine/include/engine/trace.h:10-19. */
+typedef struct nr_render_desc {
+    uint32_t sample_begin;
+    uint32_t sample_end;
+    uint32_t ray_depth;
+    uint32_t seed;
+    int32_t do_rr;
+    uint32_t rr_start_bounce;
+    float rr_survival;
+    int32_t mode;          /* nr_mode */
+} nr_render_desc;
+
+/* Closest-hit record (HitRecord, csrc/hittable/include/hittable/hit_record.h).
+ * prim = original triangle index for meshes, -1 otherwise. 64 bytes. */
+typedef struct nr_hit {
+    int32_t object_id;     /* -1 = miss */
+    int32_t material_id;
+    int32_t prim;
+    int32_t front_face;
+    float t;
+    float p[3];
+    float n[3];
+    float tangent[3];
+    float uv[2];
+} nr_hit;
+
+/* StreamConfig (csrc/data/include/data/stream.h:13-30), minus the OIDN knobs. */
+typedef struct nr_post_desc {
+    int32_t linear_to_gamma;
+    int32_t apply_white_balance;
+    float wb_temperature;
+    float wb_tint;
+    int32_t apply_tonemapping;
+    int32_t tm_method;     /* 0 REINHARD, 1 REINHARD_EXTENDED, 2 ACES */
+    float tm_white_point;
+    float tm_alpha;
+    int32_t fix_reinhard;  /* 0 keeps the reference's Reinhard no-op (stream.cu:7-11) */
+} nr_post_desc;
+
+typedef struct nr_ctx nr_ctx;
+
+/* Version of this ABI (NR_ABI_VERSION). */
+int nr_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* nr_last_error(void);
+
+/* Create a context on HIP device `device`.  Replaces the allocations of
+ * RenderEngine::RenderEngine (csrc/engine/src/engine.cpp:7-34). NULL on error. */
+nr_ctx* nr_create(int device);
+void nr_destroy(nr_ctx* ctx);
+
+/* Lower and upload a scene: top-level and per-mesh binned-SAH BVHs, object,
+ * material and texture tables, sky.  Replaces Scene::build
+ * (csrc/scene/src/scene.cpp:68-76) and the registries it drives
+ * (csrc/hittable/include/hittable/registry.h:149-263,
+ *  csrc/material/include/material/registry.h:85-95). */
+int nr_scene_upload(nr_ctx* ctx, const nr_scene_desc* scene);
+
+/* Set the camera of a view (0 <= view < 16) and build its sample pattern.
+ * Replaces Camera::__construct (csrc/engine/include/engine/camera.h:296-305).
+ * (Re)allocates and zeroes the view's AOV layers when the resolution changes. */
+int nr_camera_set(nr_ctx* ctx, int view, const nr_camera_desc* cam, uint32_t seed);
+
+/* Zero the running-mean AOVs of a view (RenderEngine::reset, engine.cpp:216-220). */
+int nr_aov_reset(nr_ctx* ctx, int view);
+
+/* Render sample passes of a view.  Replaces the loop body of
+ * RenderEngine::render (engine.cpp:42-74): trace_full (trace.cu:81-94) +
+ * RenderLayers::accumulate (render_layers.h:131-140).  Asynchronous on the
+ * context's stream; nr_sync waits. */
+int nr_render(nr_ctx* ctx, int view, const nr_render_desc* desc);
+
+/* Copy one AOV layer (C x H x W fp32) of a view to host memory. */
+int nr_aov_read(nr_ctx* ctx, int view, int layer, float* host_chw, size_t n_floats);
+
+/* Device pointer of a view's AOV layer (C x H x W fp32), NULL if absent. */
+float* nr_aov_device_ptr(nr_ctx* ctx, int view, int layer);
+
+/* Closest hit of n world-space rays (origin[3], direction[3] per ray) against
+ * the uploaded scene: SceneGraph::hit (csrc/scene/include/graph.h:16-66). */
+int nr_trace_closest(nr_ctx* ctx, const float* rays, int32_t n, nr_hit* hits);
+
+/* Same with device-resident rays/hits, enqueued on the context stream. */
+int nr_trace_closest_device(nr_ctx* ctx, const float* d_rays, int32_t n, nr_hit* d_hits);
+
+/* White balance -> tonemap -> gamma -> uint8 CHW of a view's BEAUTY layer
+ * (process_stream_kernel, csrc/data/src/stream.cu:146-159).  If out_f32 is
+ * non-NULL the display-referred float image (before x255) is written too. */
+int nr_post(nr_ctx* ctx, int view, const nr_post_desc* desc, uint8_t* out_u8, float* out_f32);
+
+/* Wait for all work of the context. */
+int nr_sync(nr_ctx* ctx);
+
+/* Per-kernel GPU time (ms) of the last nr_render / nr_trace_closest call,
+ * measured with HIP events on the context stream: [0] whole call,
+ * [1] traversal kernels, [2] shading kernels, [3] generate+accumulate.
+ * Returns the number of values written. */
+int nr_last_timings(nr_ctx* ctx, float* ms, int n);
+
+/* Enable/disable event timing (off by default; it adds event records). */
+int nr_set_timing(nr_ctx* ctx, int enabled);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NECTAR_HIP_H */

@@ -1,0 +1,1089 @@
+// nectar_hip.hip — MI355X wavefront path tracer behind the C ABI of
+// include/nectar_hip.h.
+//
+// One sample pass of a view is a wavefront pipeline over all its pixels:
+//   k_generate      camera rays (camera.h:186-278) for every pixel
+//   per bounce:
+//     k_counts      clears the queue counters of this bounce
+//     k_extend      closest hit (SceneGraph::hit, graph.h:16-66) of every live
+//                   path; misses add the sky and retire; hits are compacted
+//                   into one queue per material type (wave ballot + one atomic
+//                   per wave and type)
+//     k_shade<M>    one kernel per material type present: emission, scatter,
+//                   BRDF sampling / mixture light sampling (brdf.cu:4-35),
+//                   bounce-0 AOVs, Russian roulette (trace.cu:63-78); survivors
+//                   are compacted into the next live queue
+//   k_accumulate    running-mean fold of the pass into the view's AOVs
+//                   (accumulate_samples_kernel, data.cu:28-40)
+// Path state lives in HBM as SoA float4 arrays indexed by pixel; queues hold
+// pixel indices.  The per-path Philox stream is (seed, pixel + s*W*H, draw id)
+// exactly as trace.cu:58 keys it, so every path makes the reference's draws.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/nectar_hip.h"
+#include "nr_bvh.h"
+#include "nr_device.h"
+#include "nr_types.h"
+
+// ---------------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+static int fail(int code, const std::string& msg) { g_err = msg; return code; }
+#define HIPCK(expr)                                                                      \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(NR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+// counters layout
+enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_N = 16 };
+
+struct PathPtrs {
+    float4* o;
+    float4* d;
+    float4* atten;
+    float4* L;
+    float4* diff;
+    float4* nrm;
+    uint32_t* draw;
+    GHit* hit;
+    int32_t* qact;      // 2 * cap
+    int32_t* qmat;      // NR_MAT_TYPES * cap
+    int32_t* counts;    // CNT_N
+    uint32_t cap;
+};
+
+struct TraceArgs {
+    uint32_t seed;
+    uint32_t sWH;       // (uint32)(s * W * H), the subsequence offset of pass s
+    uint32_t bounce;
+    uint32_t depth;
+    int32_t do_rr;
+    uint32_t rr_start;
+    float rr_survival;
+};
+
+// ---------------------------------------------------------------------------
+// wave-level queue append: one atomic per wave, lanes get consecutive slots
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wave_append(bool pred, int32_t* counter) {
+    unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return -1;
+    int lane = __lane_id();
+    int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    return pred ? base + __popcll(below) : -1;
+}
+
+// ---------------------------------------------------------------------------
+// camera (engine/include/engine/camera.h:186-278)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Ray camera_ray(const GCamera& c, uint32_t x, uint32_t y, uint32_t pixel, uint32_t s,
+                                          uint32_t seed, Gen& g) {
+    float bx = c.pattern[2 * s], by = c.pattern[2 * s + 1];
+    f2 shift = pcg_vec2((float)pixel, seed);
+    float ox = fmodf(bx + shift.x, 1.0f) - 0.5f;
+    float oy = fmodf(by + shift.y, 1.0f) - 0.5f;
+    float u = ((float)x + ox - (c.res[0] - 1.0f) * 0.5f) * (c.uvw[0] / c.res[0]);
+    float v = ((float)y + oy - (c.res[1] - 1.0f) * 0.5f) * (c.uvw[1] / c.res[1]);
+    float w = -c.focal_length;
+    f3 center = mv(c.R, mk3(u, v, w));
+    float focus_scale = c.focus_distance / c.focal_length;
+    f3 pos = ld3(c.pos);
+    f3 focus = pos + center * focus_scale;
+    f3 origin = pos;
+    if (!(c.defocus_angle <= 0.0f)) {
+        f3 du = mk3(c.R[0], c.R[3], c.R[6]) * c.defocus_radius;
+        f3 dv = mk3(c.R[1], c.R[4], c.R[7]) * c.defocus_radius;
+        float lx, ly;
+        while (true) {                                                           // vector.h:303-311
+            lx = g.rf(-1.0f, 1.0f);
+            ly = g.rf(-1.0f, 1.0f);
+            if (lx * lx + ly * ly < 1.0f) break;
+        }
+        origin = pos + (du * lx + dv * ly);
+    }
+    if (c.shutter_time > 0.0f) (void)g.uniform();                                // ray time draw
+    return mkray(origin, normalize(focus - origin));
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(128) k_trace_closest(GScene S, const float* __restrict__ rays, int n,
+                                                       nr_hit* __restrict__ hits, int32_t* __restrict__ err) {
+    extern __shared__ int2 smem[];
+    int tid = threadIdx.x;
+    Stack st{smem + (tid >> 6) * S.stack_depth * 64, tid & 63, S.stack_depth};
+    int i = blockIdx.x * blockDim.x + tid;
+    if (i >= n) return;
+    const float* q = rays + 6 * (size_t)i;
+    Ray r = mkray(mk3(q[0], q[1], q[2]), mk3(q[3], q[4], q[5]));
+    HRec rec;
+    hrec_clear(rec);
+    bool of = false;
+    int oi = scene_closest(S, r, rec, st, of);
+    if (of) atomicOr(err, 1);
+    nr_hit h;
+    if (oi < 0) {
+        memset(&h, 0, sizeof(h));
+        h.object_id = -1;
+        h.prim = -1;
+    } else {
+        const GObject& o = S.objects[oi];
+        h.object_id = o.object_id;
+        h.material_id = o.material;
+        h.prim = rec.prim;
+        h.front_face = rec.front ? 1 : 0;
+        h.t = rec.t;
+        h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
+        h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
+        h.tangent[0] = rec.tg.x; h.tangent[1] = rec.tg.y; h.tangent[2] = rec.tg.z;
+        h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
+    }
+    hits[i] = h;
+}
+
+__global__ void __launch_bounds__(256) k_generate(GCamera cam, PathPtrs P, uint32_t n, uint32_t s, uint32_t seed,
+                                                  uint32_t sWH) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) P.counts[CNT_ACT + 0] = (int32_t)n;
+    if (i >= n) return;
+    uint32_t x = i % cam.W, y = i / cam.W;
+    Gen g{seed, i + sWH, 0u};
+    Ray r = camera_ray(cam, x, y, i, s, seed, g);
+    P.o[i] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
+    P.d[i] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
+    P.atten[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.diff[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.draw[i] = g.draw;
+    P.qact[i] = (int32_t)i;
+}
+
+__global__ void k_counts(int32_t* counts, uint32_t bounce) {
+    int t = threadIdx.x;
+    if (t < NR_MAT_TYPES) counts[CNT_MAT + t] = 0;
+    if (t == 0) counts[CNT_ACT + ((bounce + 1) & 1)] = 0;
+}
+
+__global__ void __launch_bounds__(128) k_extend(GScene S, PathPtrs P, TraceArgs A) {
+    extern __shared__ int2 smem[];
+    int tid = threadIdx.x;
+    Stack st{smem + (tid >> 6) * S.stack_depth * 64, tid & 63, S.stack_depth};
+    const int in = A.bounce & 1;
+    const int n = P.counts[CNT_ACT + in];
+    int i = blockIdx.x * blockDim.x + tid;
+    int path = -1;
+    int mtype = 0;
+    if (i < n) {
+        path = P.qact[(size_t)in * P.cap + i];
+        float4 o4 = P.o[path], d4 = P.d[path];
+        Ray r = mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
+        HRec rec;
+        hrec_clear(rec);
+        bool of = false;
+        int oi = scene_closest(S, r, rec, st, of);
+        if (of) atomicOr(&P.counts[CNT_ERR], 1);
+        if (oi < 0) {
+            float4 a4 = P.atten[path], l4 = P.L[path];
+            f3 L = mk3(l4.x, l4.y, l4.z) + mk3(a4.x, a4.y, a4.z) * sky_sample(S.sky, r.d);
+            P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
+        } else {
+            const GObject& ob = S.objects[oi];
+            GHit h;
+            h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
+            h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
+            h.tg[0] = rec.tg.x; h.tg[1] = rec.tg.y; h.tg[2] = rec.tg.z;
+            h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
+            h.material = ob.material;
+            h.front_face = rec.front ? 1 : 0;
+            h.object_id = ob.object_id;
+            h.prim = rec.prim;
+            h.t = rec.t;
+            P.hit[path] = h;
+            mtype = ob.pad[0];
+        }
+    }
+#pragma unroll
+    for (int t = 1; t < NR_MAT_TYPES; t++) {
+        int slot = wave_append(mtype == t, &P.counts[CNT_MAT + t]);
+        if (mtype == t) P.qmat[(size_t)t * P.cap + slot] = path;
+    }
+}
+
+// material evaluate (material.h:59-67, 152-180, 351-357)
+template <int M>
+__device__ __forceinline__ f3 mat_eval(const GScene& S, const GMaterial& m, f3 n, f3 sn, f2 uv, f3 view, f3 l) {
+    if (M == NR_MAT_LAMBERTIAN) {
+        float c = dot(n, normalize(l));
+        if (c <= 0.0f) return s3(0.0f);
+        return tex_sample(S, m.tex[0], uv) * (c / NR_PI);
+    } else if (M == NR_MAT_PBR) {
+        float n_dot_l = dot(sn, l);
+        float n_dot_v = dot(sn, view);
+        if (n_dot_l <= 0.0f || n_dot_v <= 0.0f) return s3(0.0f);
+        f3 albedo = tex_sample(S, m.tex[0], uv);
+        float roughness = tex_sample(S, m.tex[1], uv).x;
+        float metallic = tex_sample(S, m.tex[2], uv).x;
+        float alpha = fmaxf(roughness * roughness, 1e-3f);
+        f3 h = normalize(view + l);
+        float n_dot_h = fmaxf(dot(sn, h), 0.0f);
+        float v_dot_h = fmaxf(dot(view, h), 0.0f);
+        f3 F0 = s3(0.04f) * (1.0f - metallic) + albedo * metallic;
+        f3 F = F0 + (s3(1.0f) - F0) * powf(1.0f - v_dot_h, 5.0f);
+        float D = ggx_D(n_dot_h, alpha);
+        float G = smith_g1(n_dot_l, alpha) * smith_g1(n_dot_v, alpha);
+        f3 spec = F * (D * G / fmaxf(4.0f * n_dot_v * n_dot_l, 1e-4f));
+        f3 diff = (albedo * (1.0f - metallic)) * (1.0f / NR_PI);
+        return (diff + spec) * n_dot_l;
+    } else {
+        return tex_sample(S, m.tex[0], uv) * (1.0f / NR_PI4);
+    }
+}
+
+template <int M>
+__global__ void __launch_bounds__(256) k_shade(GScene S, PathPtrs P, TraceArgs A) {
+    const int n = P.counts[CNT_MAT + M];
+    const int out = (A.bounce + 1) & 1;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool cont = false;
+    int path = -1;
+    if (i < n) {
+        path = P.qmat[(size_t)M * P.cap + i];
+        GHit h = P.hit[path];
+        float4 d4 = P.d[path], a4 = P.atten[path], l4 = P.L[path];
+        f3 din = mk3(d4.x, d4.y, d4.z);
+        f3 atten = mk3(a4.x, a4.y, a4.z);
+        f3 L = mk3(l4.x, l4.y, l4.z);
+        f3 p = ld3(h.p), nn = ld3(h.n), tg = ld3(h.tg);
+        f2 uv; uv.x = h.uv[0]; uv.y = h.uv[1];
+        bool front = h.front_face != 0;
+        Gen g{A.seed, (uint32_t)path + A.sWH, P.draw[path]};
+        const GMaterial& m = S.materials[h.material];
+        // emission (material.h:182-188, 306-312)
+        f3 em = s3(0.0f);
+        if (M == NR_MAT_PBR && front) em = tex_sample(S, m.tex[3], uv);
+        if (M == NR_MAT_EMISSIVE && front) em = tex_sample(S, m.tex[0], uv) * m.brightness;
+        L = L + atten * em;
+        bool alive = false;
+        f3 nd = din;
+        if (M == NR_MAT_DIELECTRIC) {                                           // material.h:220-244
+            float ri = front ? (1.0f / m.ior) : m.ior;
+            f3 unit = normalize(din);
+            float cos_theta = fminf(dot(-unit, nn), 1.0f);
+            float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+            bool refl = (double)(ri * sin_theta) > 1.0;
+            if (!refl) {
+                float r0 = (1.0f - ri) / (1.0f + ri);
+                r0 = r0 * r0;
+                float refl_p = r0 + (1.0f - r0) * powf((1.0f - cos_theta), 5.0f);
+                refl = refl_p > g.uniform();
+            }
+            nd = refl ? reflect(unit, nn) : refract(unit, nn, ri);
+            atten = atten * ld3(m.tint);
+            alive = true;
+        } else if (M != NR_MAT_EMISSIVE) {
+            ONB uvw;
+            f3 view = s3(0.0f), sn = nn;
+            float alpha = 0.0f, sp = 0.0f;
+            if (M == NR_MAT_LAMBERTIAN) {
+                uvw.build(nn);
+            } else if (M == NR_MAT_PBR) {                                       // material.h:127-150
+                sn = pbr_shading_normal(S, m, nn, tg, uv);
+                f3 albedo = tex_sample(S, m.tex[0], uv);
+                float metallic = tex_sample(S, m.tex[2], uv).x;
+                f3 F0 = s3(0.04f) * (1.0f - metallic) + albedo * metallic;
+                sp = fminf(0.9f, fmaxf(0.1f, fmaxf(F0.x, fmaxf(F0.y, F0.z))));
+                view = normalize(-din);
+                float rough = tex_sample(S, m.tex[1], uv).x;
+                alpha = fmaxf(rough * rough, 1e-3f);
+                uvw.build(sn);
+            }
+            f3 dir;
+            float pdf;
+            if (S.first_light >= 0) {                                          // brdf.cu:16-23
+                const GObject& Lo = S.objects[S.first_light];
+                if (g.rf() < 0.5f) {
+                    if (M == NR_MAT_LAMBERTIAN) dir = uvw.xf(cosine_dir(g));
+                    else if (M == NR_MAT_PBR) dir = pbr_pdf_generate(uvw, view, alpha, sp, g);
+                    else dir = unit_vector(g);
+                } else {
+                    dir = light_random(Lo, p, g);
+                }
+                float mp;
+                if (M == NR_MAT_LAMBERTIAN) mp = cosine_pdf_value(uvw, dir);
+                else if (M == NR_MAT_PBR) mp = pbr_pdf_value(uvw, view, alpha, sp, dir);
+                else mp = 1.0f / NR_PI4;
+                pdf = 0.5f * mp + 0.5f * light_pdf(Lo, p, dir);
+            } else {
+                if (M == NR_MAT_LAMBERTIAN) { dir = uvw.xf(cosine_dir(g)); pdf = cosine_pdf_value(uvw, dir); }
+                else if (M == NR_MAT_PBR) { dir = pbr_pdf_generate(uvw, view, alpha, sp, g); pdf = pbr_pdf_value(uvw, view, alpha, sp, dir); }
+                else { dir = unit_vector(g); pdf = 1.0f / NR_PI4; }
+            }
+            if (!(pdf <= 0.0f)) {
+                f3 brdf = mat_eval<M>(S, m, nn, sn, uv, normalize(-din), dir);
+                atten = atten * divs(brdf, pdf);
+                nd = dir;
+                alive = true;
+            }
+        }
+        if (alive) {
+            if (A.bounce == 0) {                                                // trace.cu:67-70
+                f3 dv = s3(0.0f) + atten, nv = s3(0.0f) + nn;
+                P.diff[path] = make_float4(dv.x, dv.y, dv.z, 0.0f);
+                P.nrm[path] = make_float4(nv.x, nv.y, nv.z, 0.0f);
+            }
+            if (A.do_rr && A.bounce >= A.rr_start && A.bounce + 1 < A.depth) {   // trace.cu:72-77
+                float survival = fmaxf(atten.x, fmaxf(atten.y, atten.z));
+                survival = fminf(survival, A.rr_survival);
+                if (g.uniform() > survival) alive = false;
+                else atten = atten * s3(1.0f / survival);
+            }
+        }
+        P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
+        if (alive && A.bounce + 1 < A.depth) {
+            P.atten[path] = make_float4(atten.x, atten.y, atten.z, 0.0f);
+            P.o[path] = make_float4(p.x, p.y, p.z, 0.0f);
+            P.d[path] = make_float4(nd.x, nd.y, nd.z, 0.0f);
+            P.draw[path] = g.draw;
+            cont = true;
+        }
+    }
+    int slot = wave_append(cont, &P.counts[CNT_ACT + out]);
+    if (cont) P.qact[(size_t)out * P.cap + slot] = path;
+}
+
+__global__ void __launch_bounds__(256) k_accumulate(PathPtrs P, float* beauty, float* diffuse, float* normal,
+                                                    uint32_t n, uint32_t s) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float fn = (float)s;
+    float a = (fn - 1.0f) / fn, b = 1.0f / fn;
+    size_t HW = n;
+    float4 l4 = P.L[i], d4 = P.diff[i], n4 = P.nrm[i];
+    if (beauty) {
+        beauty[i] = beauty[i] * a + l4.x * b;
+        beauty[HW + i] = beauty[HW + i] * a + l4.y * b;
+        beauty[2 * HW + i] = beauty[2 * HW + i] * a + l4.z * b;
+    }
+    if (diffuse) {
+        diffuse[i] = diffuse[i] * a + d4.x * b;
+        diffuse[HW + i] = diffuse[HW + i] * a + d4.y * b;
+        diffuse[2 * HW + i] = diffuse[2 * HW + i] * a + d4.z * b;
+    }
+    if (normal) {
+        normal[i] = normal[i] * a + n4.x * b;
+        normal[HW + i] = normal[HW + i] * a + n4.y * b;
+        normal[2 * HW + i] = normal[2 * HW + i] * a + n4.z * b;
+    }
+}
+
+struct PostArgs {
+    float ratio[3];
+    float tint;
+    int32_t wb, tm, tm_method, gamma, fix_reinhard;
+    float white_point, alpha;
+};
+
+__device__ __forceinline__ float clamp01(float x) { return fminf(1.0f, fmaxf(0.0f, x)); }
+
+// process_stream_kernel (data/src/stream.cu:146-159)
+__global__ void __launch_bounds__(256) k_post(const float* __restrict__ img, uint32_t HW, PostArgs A,
+                                              uint8_t* __restrict__ out8, float* __restrict__ outf) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= HW) return;
+    f3 c = mk3(img[i], img[HW + i], img[2 * (size_t)HW + i]);
+    if (A.wb) {
+        c = c * ld3(A.ratio);
+        c = mk3(c.x * (1.0f + A.tint * 0.5f), c.y * (1.0f - A.tint), c.z * (1.0f + A.tint * 0.5f));
+    }
+    if (A.tm) {
+        f3 col = c;
+        if (A.tm_method == 0) {
+            if (A.fix_reinhard) col = divv(col, adds(col, 1.0f));
+        } else if (A.tm_method == 1) {
+            float inv_w2 = 1.0f / (A.white_point * A.white_point);
+            f3 num = col * (s3(1.0f) + col * inv_w2);
+            col = divv(num, s3(1.0f) + col);
+        } else {
+            const float a = 2.51f, b = 0.03f, cc = 2.43f, dd = 0.59f, e = 0.14f;
+            f3 num = col * adds(col * a, b);
+            f3 den = adds(col * adds(col * cc, dd), e);
+            col = mk3(clamp01(num.x / den.x), clamp01(num.y / den.y), clamp01(num.z / den.z));
+        }
+        c = c * (1.0f - A.alpha) + col * A.alpha;
+        c = mk3(clamp01(c.x), clamp01(c.y), clamp01(c.z));
+    }
+    if (A.gamma) c = mk3(c.x > 0.0f ? sqrtf(c.x) : 0.0f, c.y > 0.0f ? sqrtf(c.y) : 0.0f, c.z > 0.0f ? sqrtf(c.z) : 0.0f);
+    float v[3] = {c.x, c.y, c.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float x = v[k];
+        if (x != x) x = 0.0f;
+        x = fmaxf(0.0f, fminf(1.0f, x));
+        if (outf) outf[k * (size_t)HW + i] = x;
+        if (out8) out8[k * (size_t)HW + i] = (uint8_t)(x * 255.0f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host math for scene lowering (same operation order as the device side and
+// the reference: core/include/core/matrix.h, transform.h, camera.h:124-136)
+// ---------------------------------------------------------------------------
+namespace host {
+struct M3 { float m[9]; };
+static M3 ident() { M3 r{}; r.m[0] = r.m[4] = r.m[8] = 1.0f; return r; }
+static M3 mul(const M3& a, const M3& b) {
+    M3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            float acc = 0.0f;
+            for (int k = 0; k < 3; k++) acc += a.m[3 * i + k] * b.m[3 * k + j];
+            r.m[3 * i + j] = acc;
+        }
+    return r;
+}
+static M3 rx(float a) { float c = cosf(a), s = sinf(a); M3 r = ident(); r.m[4] = c; r.m[5] = -s; r.m[7] = s; r.m[8] = c; return r; }
+static M3 ry(float a) { float c = cosf(a), s = sinf(a); M3 r = ident(); r.m[0] = c; r.m[2] = s; r.m[6] = -s; r.m[8] = c; return r; }
+static M3 rz(float a) { float c = cosf(a), s = sinf(a); M3 r = ident(); r.m[0] = c; r.m[1] = -s; r.m[3] = s; r.m[4] = c; return r; }
+static float deg2rad(float d) { return (float)((double)d * 0.01745329); }
+static M3 euler(float x, float y, float z) { return mul(mul(rz(z), ry(y)), rx(x)); }
+
+static GXform xform(const float* p, const float* rot_deg, const float* sc, bool position_only) {
+    GXform x{};
+    M3 R = position_only ? euler(0.0f, 0.0f, 0.0f) : euler(deg2rad(rot_deg[0]), deg2rad(rot_deg[1]), deg2rad(rot_deg[2]));
+    std::memcpy(x.R, R.m, sizeof(x.R));
+    for (int k = 0; k < 3; k++) {
+        x.pos[k] = p[k];
+        x.scale[k] = position_only ? 1.0f : sc[k];
+        x.inv_s[k] = position_only ? 1.0f : 1.0f / ((sc[k] + 1e-8f) + 1e-8f);
+    }
+    return x;
+}
+
+static nrb::Box transformed(const nrb::Box& b, const GXform& x) {               // aabb.h:73-96
+    float mn[3] = {1e32f, 1e32f, 1e32f}, mx[3] = {-1e32f, -1e32f, -1e32f};
+    for (int c = 0; c < 8; c++) {
+        f3 p = mk3((c & 1) ? b.hi[0] : b.lo[0], (c & 2) ? b.hi[1] : b.lo[1], (c & 4) ? b.hi[2] : b.lo[2]);
+        f3 w = mv(x.R, p * ld3(x.scale)) + ld3(x.pos);
+        mn[0] = fminf(mn[0], w.x); mn[1] = fminf(mn[1], w.y); mn[2] = fminf(mn[2], w.z);
+        mx[0] = fmaxf(mx[0], w.x); mx[1] = fmaxf(mx[1], w.y); mx[2] = fmaxf(mx[2], w.z);
+    }
+    nrb::Box r = nrb::box_points(mn, mx);
+    for (int k = 0; k < 3; k++)                                                  // AABB::buffer(EPS)
+        if (r.hi[k] - r.lo[k] < 1e-4f) { float pad = 1e-4f / 2.0f; r.lo[k] -= pad; r.hi[k] += pad; }
+    return r;
+}
+
+// SampleGenerator::generate_n_rooks (camera.h:45-61); the reference's host
+// compiler evaluates the Vector2 constructor arguments right to left, so the
+// second coordinate's jitter is drawn first.
+static std::vector<float> n_rooks(uint32_t n, uint32_t seed) {
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::mt19937 rng(seed);
+    std::shuffle(perm.begin(), perm.end(), rng);
+    std::uniform_real_distribution<float> jitter(0.0f, 1.0f);
+    std::vector<float> out(2 * (size_t)n);
+    for (uint32_t i = 0; i < n; i++) {
+        float jy = jitter(rng);
+        float jx = jitter(rng);
+        out[2 * i] = (i + jx) / (float)n;
+        out[2 * i + 1] = (perm[i] + jy) / (float)n;
+    }
+    return out;
+}
+}  // namespace host
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct View {
+    bool valid = false;
+    nr_camera_desc desc{};
+    uint32_t seed = 0;
+    GCamera cam{};
+    float* d_pattern = nullptr;
+    uint32_t H = 0, W = 0;
+    float* layer[9] = {nullptr};
+};
+
+struct nr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool has_scene = false;
+    GScene scene{};
+    std::vector<void*> scene_mem;
+    bool mat_present[NR_MAT_TYPES] = {false};
+    View views[NR_MAX_VIEWS];
+    // path state
+    PathPtrs P{};
+    std::vector<void*> path_mem;
+    uint32_t path_cap = 0;
+    int32_t* d_err = nullptr;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    float last_ms[8] = {0};
+    int n_last = 0;
+};
+
+static void free_list(std::vector<void*>& v) {
+    for (void* p : v) if (p) (void)hipFree(p);
+    v.clear();
+}
+
+template <typename T>
+static int upload(nr_ctx* c, const T* host, size_t n, T** dev) {
+    *dev = nullptr;
+    if (n == 0) return NR_OK;
+    HIPCK(hipMalloc((void**)dev, n * sizeof(T)));
+    c->scene_mem.push_back(*dev);
+    HIPCK(hipMemcpyAsync(*dev, host, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return NR_OK;
+}
+
+static size_t stack_lds_bytes(int depth, int block) { return (size_t)depth * 64 * sizeof(int2) * (block / 64); }
+
+extern "C" {
+
+int nr_abi_version(void) { return NR_ABI_VERSION; }
+const char* nr_last_error(void) { return g_err.c_str(); }
+
+nr_ctx* nr_create(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { g_err = "no HIP device visible"; return nullptr; }
+    if (device < 0 || device >= n) { g_err = "device index out of range"; return nullptr; }
+    if (hipSetDevice(device) != hipSuccess) { g_err = "hipSetDevice failed"; return nullptr; }
+    auto* c = new nr_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_err = "hipStreamCreate failed";
+        delete c;
+        return nullptr;
+    }
+    if (hipMalloc((void**)&c->d_err, sizeof(int32_t) * 4) != hipSuccess) {
+        g_err = "hipMalloc failed";
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return nullptr;
+    }
+    (void)hipMemsetAsync(c->d_err, 0, sizeof(int32_t) * 4, c->stream);
+    return c;
+}
+
+void nr_destroy(nr_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_list(c->scene_mem);
+    free_list(c->path_mem);
+    for (auto& v : c->views) {
+        if (v.d_pattern) (void)hipFree(v.d_pattern);
+        for (auto* l : v.layer) if (l) (void)hipFree(l);
+    }
+    for (auto e : c->ev) (void)hipEventDestroy(e);
+    if (c->d_err) (void)hipFree(c->d_err);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
+    if (!c || !d) return fail(NR_ERR_INVALID, "null context or scene");
+    HIPCK(hipSetDevice(c->device));
+    HIPCK(hipStreamSynchronize(c->stream));
+    free_list(c->scene_mem);
+    c->has_scene = false;
+    for (auto& b : c->mat_present) b = false;
+    GScene S{};
+    // images
+    std::vector<GImage> imgs(d->n_images);
+    for (int i = 0; i < d->n_images; i++) {
+        const nr_image_desc& im = d->images[i];
+        if (im.C < 3 || im.H <= 0 || im.W <= 0 || !im.data) return fail(NR_ERR_INVALID, "image needs C >= 3 and data");
+        uint8_t* p = nullptr;
+        int e = upload(c, im.data, (size_t)im.C * im.H * im.W, &p);
+        if (e) return e;
+        imgs[i] = GImage{p, im.C, im.H, im.W, 0};
+    }
+    // materials
+    if (d->n_materials <= 0) return fail(NR_ERR_INVALID, "scene needs at least the default material");
+    std::vector<GMaterial> mats(d->n_materials);
+    for (int i = 0; i < d->n_materials; i++) {
+        const nr_material_desc& md = d->materials[i];
+        if (md.type < 1 || md.type > 5) return fail(NR_ERR_INVALID, "bad material type");
+        GMaterial& g = mats[i];
+        std::memset(&g, 0, sizeof(g));
+        g.type = md.type;
+        for (int k = 0; k < 5; k++) {
+            if (md.tex[k].image >= d->n_images) return fail(NR_ERR_INVALID, "texture image index out of range");
+            g.tex[k].image = md.tex[k].image;
+            for (int a = 0; a < 3; a++) g.tex[k].color[a] = md.tex[k].color[a];
+            g.tex[k].scale = md.tex[k].scale;
+        }
+        for (int a = 0; a < 3; a++) g.tint[a] = md.tint[a];
+        g.ior = md.ior;
+        g.brightness = md.brightness;
+    }
+    // meshes
+    std::vector<GMesh> meshes(d->n_meshes);
+    std::vector<nrb::Box> mesh_local(d->n_meshes);
+    int max_mesh_depth = 0;
+    for (int mi = 0; mi < d->n_meshes; mi++) {
+        const nr_mesh_desc& md = d->meshes[mi];
+        if (md.n_vertices < 0 || md.n_triangles < 0) return fail(NR_ERR_INVALID, "bad mesh sizes");
+        for (int64_t k = 0; k < 3 * (int64_t)md.n_triangles; k++)
+            if (md.triangles[k] >= (uint32_t)md.n_vertices) return fail(NR_ERR_INVALID, "triangle index out of range");
+        std::vector<nrb::Box> tb(md.n_triangles);
+        for (int t = 0; t < md.n_triangles; t++) {                             // mesh.h:320-336
+            const float* a = md.vertices + 11 * (size_t)md.triangles[3 * t];
+            const float* b = md.vertices + 11 * (size_t)md.triangles[3 * t + 1];
+            const float* cc = md.vertices + 11 * (size_t)md.triangles[3 * t + 2];
+            float mn[3], mx[3];
+            for (int k = 0; k < 3; k++) {
+                mn[k] = fminf(a[k], fminf(b[k], cc[k]));
+                mx[k] = fmaxf(a[k], fmaxf(b[k], cc[k]));
+            }
+            tb[t] = nrb::box_points(mn, mx);
+        }
+        nrb::Tree T;
+        nrb::Builder().build(tb, T);
+        std::vector<GNode> nodes;
+        GMesh gm{};
+        nrb::to_pairs(T, nodes, gm.root_ref, gm.root_min, gm.root_max);
+        gm.depth = T.depth;
+        max_mesh_depth = std::max(max_mesh_depth, T.depth);
+        std::vector<GTri> tris(md.n_triangles);
+        std::vector<uint32_t> idx(3 * (size_t)md.n_triangles);
+        for (int s = 0; s < md.n_triangles; s++) {
+            int src = T.order[s];
+            GTri& g = tris[s];
+            for (int k = 0; k < 3; k++) {
+                uint32_t vi = md.triangles[3 * src + k];
+                idx[3 * (size_t)s + k] = vi;
+                float* dst = k == 0 ? g.v0 : (k == 1 ? g.v1 : g.v2);
+                for (int a = 0; a < 3; a++) dst[a] = md.vertices[11 * (size_t)vi + a];
+            }
+            g.orig = src;
+            g.pad0 = g.pad1 = 0;
+        }
+        std::vector<GVertex> verts(md.n_vertices);
+        float mn[3] = {1e32f, 1e32f, 1e32f}, mx[3] = {-1e32f, -1e32f, -1e32f};
+        for (int v = 0; v < md.n_vertices; v++) {
+            const float* f = md.vertices + 11 * (size_t)v;
+            GVertex& g = verts[v];
+            for (int a = 0; a < 3; a++) { g.normal[a] = f[3 + a]; g.tangent[a] = f[6 + a]; }
+            g.uv[0] = f[9]; g.uv[1] = f[10];
+            g.pad[0] = g.pad[1] = g.pad[2] = g.pad[3] = 0.0f;
+            for (int a = 0; a < 3; a++) { mn[a] = fminf(mn[a], f[a]); mx[a] = fmaxf(mx[a], f[a]); }
+        }
+        if (md.n_vertices > 0) mesh_local[mi] = nrb::box_points(mn, mx);        // mesh.h:218-237
+        else mesh_local[mi] = nrb::box_none();
+        GNode* dn; GTri* dt; uint32_t* di; GVertex* dv;
+        int e = upload(c, nodes.data(), nodes.size(), &dn);
+        if (!e) e = upload(c, tris.data(), tris.size(), &dt);
+        if (!e) e = upload(c, idx.data(), idx.size(), &di);
+        if (!e) e = upload(c, verts.data(), verts.size(), &dv);
+        if (e) return e;
+        gm.nodes = dn; gm.tris = dt; gm.idx = di; gm.verts = dv;
+        meshes[mi] = gm;
+    }
+    // objects: registration order -> world boxes -> top-level tree
+    int n = d->n_objects;
+    std::vector<GObject> reg(n);
+    std::vector<nrb::Box> wb(n);
+    for (int i = 0; i < n; i++) {
+        const nr_object_desc& od = d->objects[i];
+        if (od.type < 1 || od.type > 6) return fail(NR_ERR_INVALID, "bad hittable type");
+        int geom = (od.type == NR_HIT_CONSTANT_MEDIUM || od.type == NR_HIT_OBJECT_LIGHT) ? od.bound_type : od.type;
+        if (geom < 1 || geom > 4) return fail(NR_ERR_INVALID, "bound object must be quad, sphere, cube or mesh");
+        if (geom == NR_HIT_MESH && (od.mesh < 0 || od.mesh >= d->n_meshes)) return fail(NR_ERR_INVALID, "mesh index out of range");
+        if (od.material < 0 || od.material >= d->n_materials) return fail(NR_ERR_INVALID, "material index out of range");
+        GObject& o = reg[i];
+        std::memset(&o, 0, sizeof(o));
+        o.xf = host::xform(od.position, od.rotation, od.scale, geom == NR_HIT_SPHERE);
+        o.type = od.type;
+        o.bound_type = od.bound_type;
+        o.material = od.material;
+        o.object_id = i;
+        o.mesh = geom == NR_HIT_MESH ? od.mesh : -1;
+        o.radius = od.radius + 1e-8f;
+        o.neg_inv_density = -1.0f / (od.density + 1e-8f);
+        o.pad[0] = mats[od.material].type;
+        c->mat_present[mats[od.material].type] = true;
+        nrb::Box lb;
+        if (geom == NR_HIT_QUAD || geom == NR_HIT_CUBE) { float a[3] = {-0.5f, -0.5f, -0.5f}, b[3] = {0.5f, 0.5f, 0.5f}; lb = nrb::box_points(a, b); }
+        else if (geom == NR_HIT_SPHERE) { float a[3] = {-o.radius, -o.radius, -o.radius}, b[3] = {o.radius, o.radius, o.radius}; lb = nrb::box_points(a, b); }
+        else lb = mesh_local[od.mesh];
+        wb[i] = host::transformed(lb, o.xf);
+    }
+    nrb::Tree T;
+    nrb::Builder().build(wb, T);
+    std::vector<GNode> tnodes;
+    nrb::to_pairs(T, tnodes, S.root_ref, S.root_min, S.root_max);
+    std::vector<GObject> objs(n);
+    S.first_light = -1;
+    for (int s = 0; s < n; s++) {
+        objs[s] = reg[T.order[s]];
+        if (S.first_light < 0 && objs[s].type == NR_HIT_OBJECT_LIGHT) S.first_light = s;   // registry.h:240-244
+    }
+    S.n_objects = n;
+    S.stack_depth = T.depth + max_mesh_depth + 4;
+    {
+        GNode* dn; GObject* dob; GMesh* dm; GMaterial* dmat; GImage* dimg;
+        int e = upload(c, tnodes.data(), tnodes.size(), &dn);
+        if (!e) e = upload(c, objs.data(), objs.size(), &dob);
+        if (!e) e = upload(c, meshes.data(), meshes.size(), &dm);
+        if (!e) e = upload(c, mats.data(), mats.size(), &dmat);
+        if (!e) e = upload(c, imgs.data(), imgs.size(), &dimg);
+        if (e) return e;
+        S.nodes = dn; S.objects = dob; S.meshes = dm; S.materials = dmat; S.images = dimg;
+    }
+    // sky (skylight.h)
+    S.sky.type = d->sky.type;
+    for (int a = 0; a < 3; a++) { S.sky.start[a] = d->sky.start[a]; S.sky.end[a] = d->sky.end[a]; }
+    {
+        host::M3 R = host::ry(host::deg2rad(d->sky.rotation));
+        std::memcpy(S.sky.rot, R.m, sizeof(S.sky.rot));
+    }
+    S.sky.intensity = d->sky.intensity;
+    S.sky.data = nullptr;
+    if (d->sky.type == NR_SKY_HDRI && d->sky.data) {
+        if (d->sky.C < 3 || d->sky.H <= 0 || d->sky.W <= 0) return fail(NR_ERR_INVALID, "HDRI needs C >= 3");
+        S.sky.H = d->sky.H; S.sky.W = d->sky.W;
+        float* p;
+        // only the first three planes are read (skylight.h:106-114)
+        int e = upload(c, d->sky.data, (size_t)3 * d->sky.H * d->sky.W, &p);
+        if (e) return e;
+        S.sky.data = p;
+    }
+    // cube face transforms (shapes.h:192-223)
+    const float fp[6][3] = {{0, 0, 0.5f}, {0, 0, -0.5f}, {0.5f, 0, 0}, {-0.5f, 0, 0}, {0, 0.5f, 0}, {0, -0.5f, 0}};
+    const float fr[6][3] = {{90, 0, 0}, {-90, 0, 0}, {0, 0, -90}, {0, 0, 90}, {0, 0, 0}, {180, 0, 0}};
+    const float one[3] = {1.0f, 1.0f, 1.0f};
+    for (int i = 0; i < 6; i++) S.cube_faces[i] = host::xform(fp[i], fr[i], one, false);
+    HIPCK(hipStreamSynchronize(c->stream));
+    c->scene = S;
+    c->has_scene = true;
+    return NR_OK;
+}
+
+int nr_camera_set(nr_ctx* c, int view, const nr_camera_desc* cd, uint32_t seed) {
+    if (!c || !cd) return fail(NR_ERR_INVALID, "null argument");
+    if (view < 0 || view >= NR_MAX_VIEWS) return fail(NR_ERR_INVALID, "view index out of range");
+    if (cd->resolution[0] < 1.0f || cd->resolution[1] < 1.0f) return fail(NR_ERR_INVALID, "resolution must be positive");
+    if (cd->samples_per_pixel < 1) return fail(NR_ERR_INVALID, "samples_per_pixel must be >= 1");
+    HIPCK(hipSetDevice(c->device));
+    View& v = c->views[view];
+    uint32_t H = (uint32_t)cd->resolution[0], W = (uint32_t)cd->resolution[1];
+    if ((uint64_t)H * W > (1ull << 31)) return fail(NR_ERR_LIMIT, "view larger than 2^31 pixels");
+    HIPCK(hipStreamSynchronize(c->stream));
+    GCamera& g = v.cam;                                                          // camera.h:124-136
+    host::M3 R = host::euler(host::deg2rad(cd->rotation[0]), host::deg2rad(cd->rotation[1]), host::deg2rad(cd->rotation[2]));
+    std::memcpy(g.R, R.m, sizeof(g.R));
+    for (int a = 0; a < 3; a++) g.pos[a] = cd->position[a];
+    g.shutter_time = 1.0f / (cd->shutter_speed + 1e-8f);
+    float aperture_radius = cd->aperture / 2.0f;
+    g.defocus_angle = 2.0f * atanf(aperture_radius / (cd->focus_distance + 1e-8f));
+    g.defocus_radius = cd->focus_distance * tanf(g.defocus_angle / 2.0f);
+    float aspect = cd->resolution[1] / cd->resolution[0];
+    g.uvw[0] = cd->sensor_width; g.uvw[1] = -cd->sensor_width * aspect; g.uvw[2] = 0.0f;
+    g.res[0] = cd->resolution[0]; g.res[1] = cd->resolution[1];
+    g.focal_length = cd->focal_length; g.focus_distance = cd->focus_distance;
+    g.H = H; g.W = W;
+    bool pattern_dirty = !v.valid || v.seed != seed || v.desc.samples_per_pixel != cd->samples_per_pixel;
+    if (pattern_dirty) {
+        std::vector<float> pat = host::n_rooks(cd->samples_per_pixel, seed);
+        if (v.d_pattern) HIPCK(hipFree(v.d_pattern));
+        v.d_pattern = nullptr;
+        HIPCK(hipMalloc((void**)&v.d_pattern, pat.size() * sizeof(float)));
+        HIPCK(hipMemcpy(v.d_pattern, pat.data(), pat.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    g.pattern = v.d_pattern;
+    g.n_pattern = cd->samples_per_pixel;
+    if (!v.valid || v.H != H || v.W != W) {
+        for (auto& l : v.layer) { if (l) HIPCK(hipFree(l)); l = nullptr; }
+        const int layers[3] = {NR_LAYER_BEAUTY, NR_LAYER_DIFFUSE, NR_LAYER_WORLD_NORMAL};
+        for (int L : layers) {
+            HIPCK(hipMalloc((void**)&v.layer[L], sizeof(float) * 3 * (size_t)H * W));
+            HIPCK(hipMemset(v.layer[L], 0, sizeof(float) * 3 * (size_t)H * W));
+        }
+    }
+    v.H = H; v.W = W;
+    v.desc = *cd;
+    v.seed = seed;
+    v.valid = true;
+    return NR_OK;
+}
+
+int nr_aov_reset(nr_ctx* c, int view) {
+    if (!c || view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_INVALID, "bad view");
+    HIPCK(hipSetDevice(c->device));
+    View& v = c->views[view];
+    for (auto* l : v.layer)
+        if (l) HIPCK(hipMemsetAsync(l, 0, sizeof(float) * 3 * (size_t)v.H * v.W, c->stream));
+    return NR_OK;
+}
+
+static int ensure_paths(nr_ctx* c, uint32_t n) {
+    if (n <= c->path_cap) return NR_OK;
+    HIPCK(hipStreamSynchronize(c->stream));
+    free_list(c->path_mem);
+    c->path_cap = 0;
+    PathPtrs& P = c->P;
+    auto al = [&](void** p, size_t bytes) -> int {
+        HIPCK(hipMalloc(p, bytes));
+        c->path_mem.push_back(*p);
+        return NR_OK;
+    };
+    int e = al((void**)&P.o, sizeof(float4) * n);
+    if (!e) e = al((void**)&P.d, sizeof(float4) * n);
+    if (!e) e = al((void**)&P.atten, sizeof(float4) * n);
+    if (!e) e = al((void**)&P.L, sizeof(float4) * n);
+    if (!e) e = al((void**)&P.diff, sizeof(float4) * n);
+    if (!e) e = al((void**)&P.nrm, sizeof(float4) * n);
+    if (!e) e = al((void**)&P.draw, sizeof(uint32_t) * n);
+    if (!e) e = al((void**)&P.hit, sizeof(GHit) * n);
+    if (!e) e = al((void**)&P.qact, sizeof(int32_t) * 2 * (size_t)n);
+    if (!e) e = al((void**)&P.qmat, sizeof(int32_t) * NR_MAT_TYPES * (size_t)n);
+    if (!e) e = al((void**)&P.counts, sizeof(int32_t) * CNT_N);
+    if (e) { free_list(c->path_mem); return e; }
+    HIPCK(hipMemsetAsync(P.counts, 0, sizeof(int32_t) * CNT_N, c->stream));
+    P.cap = n;
+    c->path_cap = n;
+    return NR_OK;
+}
+
+static hipEvent_t ev_at(nr_ctx* c, size_t i) {
+    while (c->ev.size() <= i) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev.push_back(e);
+    }
+    return c->ev[i];
+}
+
+int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
+    if (!c || !d) return fail(NR_ERR_INVALID, "null argument");
+    if (!c->has_scene) return fail(NR_ERR_STATE, "nr_render before nr_scene_upload");
+    if (view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_STATE, "view has no camera");
+    View& v = c->views[view];
+    if (d->sample_end > v.cam.n_pattern) return fail(NR_ERR_INVALID, "sample index beyond samples_per_pixel");
+    if (d->sample_begin < 1 && d->sample_end > d->sample_begin)
+        return fail(NR_ERR_INVALID, "sample passes start at 1 (running mean of passes 1..n-1)");
+    if (d->mode != NR_MODE_PARITY) return fail(NR_ERR_INVALID, "only NR_MODE_PARITY is implemented");
+    HIPCK(hipSetDevice(c->device));
+    uint32_t N = v.H * v.W;
+    int e = ensure_paths(c, N);
+    if (e) return e;
+    const GScene& S = c->scene;
+    const int TB = 128;
+    size_t lds = stack_lds_bytes(S.stack_depth, TB);
+    if (lds > 160 * 1024) return fail(NR_ERR_LIMIT, "BVH too deep for the LDS traversal stack");
+    dim3 gN((N + 255) / 256), gT((N + TB - 1) / TB);
+    size_t ei = 0;
+    float t_ext = 0.0f, t_shade = 0.0f, t_other = 0.0f;
+    std::vector<std::pair<size_t, int>> marks;   // (event index of start, category)
+    auto mark = [&](int cat) {
+        if (!c->timing) return;
+        hipEvent_t a = ev_at(c, ei);
+        (void)hipEventRecord(a, c->stream);
+        marks.push_back({ei, cat});
+        ei++;
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) { e0 = ev_at(c, 1000); e1 = ev_at(c, 1001); (void)hipEventRecord(e0, c->stream); }
+    for (uint32_t s = d->sample_begin; s < d->sample_end; s++) {
+        uint32_t sWH = (uint32_t)((uint64_t)s * (uint64_t)v.W * (uint64_t)v.H);
+        mark(2);
+        hipLaunchKernelGGL(k_generate, gN, dim3(256), 0, c->stream, v.cam, c->P, N, s, d->seed, sWH);
+        for (uint32_t b = 0; b < d->ray_depth; b++) {
+            TraceArgs A{d->seed, sWH, b, d->ray_depth, d->do_rr, d->rr_start_bounce, d->rr_survival};
+            hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->P.counts, b);
+            mark(0);
+            hipLaunchKernelGGL(k_extend, gT, dim3(TB), lds, c->stream, S, c->P, A);
+            mark(1);
+            if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL(k_shade<NR_MAT_LAMBERTIAN>, gN, dim3(256), 0, c->stream, S, c->P, A);
+            if (c->mat_present[NR_MAT_PBR]) hipLaunchKernelGGL(k_shade<NR_MAT_PBR>, gN, dim3(256), 0, c->stream, S, c->P, A);
+            if (c->mat_present[NR_MAT_DIELECTRIC]) hipLaunchKernelGGL(k_shade<NR_MAT_DIELECTRIC>, gN, dim3(256), 0, c->stream, S, c->P, A);
+            if (c->mat_present[NR_MAT_EMISSIVE]) hipLaunchKernelGGL(k_shade<NR_MAT_EMISSIVE>, gN, dim3(256), 0, c->stream, S, c->P, A);
+            if (c->mat_present[NR_MAT_ISOTROPIC]) hipLaunchKernelGGL(k_shade<NR_MAT_ISOTROPIC>, gN, dim3(256), 0, c->stream, S, c->P, A);
+        }
+        mark(2);
+        hipLaunchKernelGGL(k_accumulate, gN, dim3(256), 0, c->stream, c->P, v.layer[NR_LAYER_BEAUTY],
+                           v.layer[NR_LAYER_DIFFUSE], v.layer[NR_LAYER_WORLD_NORMAL], N, s);
+    }
+    HIPCK(hipGetLastError());
+    if (c->timing) {
+        mark(3);
+        (void)hipEventRecord(e1, c->stream);
+        HIPCK(hipEventSynchronize(e1));
+        for (size_t k = 0; k + 1 < marks.size(); k++) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, c->ev[marks[k].first], c->ev[marks[k + 1].first]);
+            if (marks[k].second == 0) t_ext += ms;
+            else if (marks[k].second == 1) t_shade += ms;
+            else t_other += ms;
+        }
+        float tot = 0.0f;
+        (void)hipEventElapsedTime(&tot, e0, e1);
+        c->last_ms[0] = tot; c->last_ms[1] = t_ext; c->last_ms[2] = t_shade; c->last_ms[3] = t_other;
+        c->n_last = 4;
+        int32_t err = 0;
+        HIPCK(hipMemcpy(&err, c->P.counts + CNT_ERR, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (err) return fail(NR_ERR_LIMIT, "traversal stack overflow");
+    }
+    return NR_OK;
+}
+
+int nr_aov_read(nr_ctx* c, int view, int layer, float* host, size_t n_floats) {
+    if (!c || !host || view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_INVALID, "bad view");
+    if (layer < 0 || layer > 8 || !c->views[view].layer[layer]) return fail(NR_ERR_INVALID, "layer not allocated");
+    View& v = c->views[view];
+    size_t need = 3 * (size_t)v.H * v.W;
+    if (n_floats < need) return fail(NR_ERR_INVALID, "host buffer too small");
+    HIPCK(hipSetDevice(c->device));
+    HIPCK(hipMemcpyAsync(host, v.layer[layer], need * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    int32_t err = 0;
+    if (c->P.counts) {
+        HIPCK(hipMemcpy(&err, c->P.counts + CNT_ERR, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (err) return fail(NR_ERR_LIMIT, "traversal stack overflow during render");
+    }
+    return NR_OK;
+}
+
+float* nr_aov_device_ptr(nr_ctx* c, int view, int layer) {
+    if (!c || view < 0 || view >= NR_MAX_VIEWS || layer < 0 || layer > 8) return nullptr;
+    return c->views[view].layer[layer];
+}
+
+int nr_trace_closest_device(nr_ctx* c, const float* d_rays, int32_t n, nr_hit* d_hits) {
+    if (!c) return fail(NR_ERR_INVALID, "null context");
+    if (!c->has_scene) return fail(NR_ERR_STATE, "trace before scene upload");
+    if (n < 0) return fail(NR_ERR_INVALID, "negative ray count");
+    if (n == 0) return NR_OK;
+    HIPCK(hipSetDevice(c->device));
+    const int TB = 128;
+    size_t lds = stack_lds_bytes(c->scene.stack_depth, TB);
+    if (lds > 160 * 1024) return fail(NR_ERR_LIMIT, "BVH too deep for the LDS traversal stack");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) { e0 = ev_at(c, 1000); e1 = ev_at(c, 1001); (void)hipEventRecord(e0, c->stream); }
+    hipLaunchKernelGGL(k_trace_closest, dim3((n + TB - 1) / TB), dim3(TB), lds, c->stream, c->scene, d_rays, n, d_hits, c->d_err);
+    HIPCK(hipGetLastError());
+    if (c->timing) {
+        (void)hipEventRecord(e1, c->stream);
+        HIPCK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        c->last_ms[0] = ms; c->last_ms[1] = ms; c->last_ms[2] = 0.0f; c->last_ms[3] = 0.0f;
+        c->n_last = 4;
+    }
+    return NR_OK;
+}
+
+int nr_trace_closest(nr_ctx* c, const float* rays, int32_t n, nr_hit* hits) {
+    if (!c || (!rays && n > 0) || (!hits && n > 0)) return fail(NR_ERR_INVALID, "null argument");
+    if (!c->has_scene) return fail(NR_ERR_STATE, "trace before scene upload");
+    if (n <= 0) return n == 0 ? NR_OK : fail(NR_ERR_INVALID, "negative ray count");
+    HIPCK(hipSetDevice(c->device));
+    float* dr = nullptr;
+    nr_hit* dh = nullptr;
+    HIPCK(hipMalloc((void**)&dr, sizeof(float) * 6 * (size_t)n));
+    if (hipMalloc((void**)&dh, sizeof(nr_hit) * (size_t)n) != hipSuccess) { (void)hipFree(dr); return fail(NR_ERR_HIP, "hipMalloc hits"); }
+    int rc = NR_OK;
+    int32_t zero = 0, err = 0;
+    if (hipMemcpyAsync(dr, rays, sizeof(float) * 6 * (size_t)n, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(c->d_err, &zero, sizeof(int32_t), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = fail(NR_ERR_HIP, "copy rays");
+    if (rc == NR_OK) rc = nr_trace_closest_device(c, dr, n, dh);
+    if (rc == NR_OK && (hipMemcpyAsync(hits, dh, sizeof(nr_hit) * (size_t)n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                        hipMemcpyAsync(&err, c->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                        hipStreamSynchronize(c->stream) != hipSuccess))
+        rc = fail(NR_ERR_HIP, "copy hits");
+    (void)hipFree(dr);
+    (void)hipFree(dh);
+    if (rc == NR_OK && err) rc = fail(NR_ERR_LIMIT, "traversal stack overflow");
+    return rc;
+}
+
+static float host_kelvin_channel(float kelvin, int ch) {                         // stream.cu:65-86
+    float temp = kelvin / 100.0f;
+    float r, g, b;
+    if (temp <= 66.0f) { r = 255.0f; g = 99.4708025861f * logf(temp) - 161.1195681661f; }
+    else { r = 329.698727446f * powf(temp - 60.0f, -0.1332047592f); g = 288.1221695283f * powf(temp - 60.0f, -0.0755148492f); }
+    if (temp >= 66.0f) b = 255.0f;
+    else if (temp <= 19.0f) b = 0.0f;
+    else b = 138.5177312231f * logf(temp - 10.0f) - 305.0447927307f;
+    float v = ch == 0 ? r : (ch == 1 ? g : b);
+    return fminf(1.0f, fmaxf(0.0f, v / 255.0f));
+}
+
+int nr_post(nr_ctx* c, int view, const nr_post_desc* d, uint8_t* out8, float* outf) {
+    if (!c || !d || view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_INVALID, "bad view");
+    View& v = c->views[view];
+    HIPCK(hipSetDevice(c->device));
+    uint32_t HW = v.H * v.W;
+    PostArgs A{};
+    float tgt_k = fmaxf(1000.0f, fminf(d->wb_temperature, 40000.0f));
+    for (int k = 0; k < 3; k++)
+        A.ratio[k] = host_kelvin_channel(tgt_k, k) / fmaxf(host_kelvin_channel(6500.0f, k), 1e-4f);
+    A.tint = d->wb_tint * 0.0025f;
+    A.wb = d->apply_white_balance; A.tm = d->apply_tonemapping; A.tm_method = d->tm_method;
+    A.gamma = d->linear_to_gamma; A.fix_reinhard = d->fix_reinhard;
+    A.white_point = d->tm_white_point; A.alpha = d->tm_alpha;
+    uint8_t* d8 = nullptr;
+    float* df = nullptr;
+    if (out8) HIPCK(hipMalloc((void**)&d8, 3 * (size_t)HW));
+    if (outf && hipMalloc((void**)&df, 3 * sizeof(float) * (size_t)HW) != hipSuccess) {
+        if (d8) (void)hipFree(d8);
+        return fail(NR_ERR_HIP, "hipMalloc post");
+    }
+    hipLaunchKernelGGL(k_post, dim3((HW + 255) / 256), dim3(256), 0, c->stream, v.layer[NR_LAYER_BEAUTY], HW, A, d8, df);
+    int rc = NR_OK;
+    if (hipGetLastError() != hipSuccess) rc = fail(NR_ERR_HIP, "k_post launch");
+    if (rc == NR_OK && out8 && hipMemcpyAsync(out8, d8, 3 * (size_t)HW, hipMemcpyDeviceToHost, c->stream) != hipSuccess) rc = fail(NR_ERR_HIP, "copy post u8");
+    if (rc == NR_OK && outf && hipMemcpyAsync(outf, df, 3 * sizeof(float) * (size_t)HW, hipMemcpyDeviceToHost, c->stream) != hipSuccess) rc = fail(NR_ERR_HIP, "copy post f32");
+    if (hipStreamSynchronize(c->stream) != hipSuccess && rc == NR_OK) rc = fail(NR_ERR_HIP, "post sync");
+    if (d8) (void)hipFree(d8);
+    if (df) (void)hipFree(df);
+    return rc;
+}
+
+int nr_sync(nr_ctx* c) {
+    if (!c) return fail(NR_ERR_INVALID, "null context");
+    HIPCK(hipSetDevice(c->device));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return NR_OK;
+}
+
+int nr_last_timings(nr_ctx* c, float* ms, int n) {
+    if (!c || !ms) return 0;
+    int k = std::min(n, c->n_last);
+    for (int i = 0; i < k; i++) ms[i] = c->last_ms[i];
+    return k;
+}
+
+int nr_set_timing(nr_ctx* c, int enabled) {
+    if (!c) return fail(NR_ERR_INVALID, "null context");
+    c->timing = enabled != 0;
+    return NR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,675 @@
+// nr_device.h — device arithmetic of the MI355X path tracer.
+//
+// Every routine keeps the reference's operation order so that, compiled with
+// -ffp-contract=off (IEEE +,-,*,/,sqrt on gfx950), results match the
+// reference's arithmetic bit for bit; only transcendentals (sin, cos, atan2,
+// acos, log, pow) come from the ROCm device library.  File:line citations are
+// relative to nectar_render/csrc/ of the reference.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "nr_types.h"
+
+#define NR_EPS 1e-4f
+#define NR_FMIN 1e-8f
+#define NR_FMAX 1e32f
+#define NR_PI 3.1415926535897932385f
+#define NR_PI2 (NR_PI * 2.0f)
+#define NR_PI4 (NR_PI2 * NR_PI2)
+
+#define NR_HD __host__ __device__ __forceinline__
+#define NR_D __device__ __forceinline__
+
+struct f3 { float x, y, z; };
+struct f2 { float x, y; };
+
+NR_HD f3 mk3(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+NR_HD f3 s3(float a) { return mk3(a, a, a); }
+NR_HD f3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
+NR_HD f3 operator+(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+NR_HD f3 operator-(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+NR_HD f3 operator*(f3 a, f3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+NR_HD f3 operator*(f3 a, float t) { return mk3(a.x * t, a.y * t, a.z * t); }
+NR_HD f3 operator-(f3 a) { return mk3(-a.x, -a.y, -a.z); }
+NR_HD f3 adds(f3 a, float t) { return mk3(a.x + t, a.y + t, a.z + t); }
+// Vector / float multiplies by the reciprocal (core/vector.h:251-254)
+NR_HD f3 divs(f3 a, float t) { return a * (1.0f / t); }
+// Vector / Vector adds FMIN to the divisor (core/vector.h:82-88)
+NR_HD f3 divv(f3 a, f3 b) { return mk3(a.x / (b.x + NR_FMIN), a.y / (b.y + NR_FMIN), a.z / (b.z + NR_FMIN)); }
+NR_HD float dot(f3 u, f3 v) { return u.x * v.x + u.y * v.y + u.z * v.z; }
+NR_HD f3 cross(f3 u, f3 v) { return mk3(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x); }
+NR_HD float len2(f3 v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
+NR_HD float vlength(f3 v) { return sqrtf(len2(v)); }
+NR_HD f3 normalize(f3 v) { return divs(v, vlength(v)); }
+NR_HD float comp(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+NR_HD f3 reflect(f3 v, f3 n) { return v - n * (2.0f * dot(v, n)); }            // vector.h:469-474
+NR_HD f3 refract(f3 uv, f3 n, float ior) {                                     // vector.h:476-485
+    float cos_theta = fminf(dot(-uv, n), 1.0f);
+    f3 perp = (uv + n * cos_theta) * ior;
+    float k = (float)(1.0 - (double)len2(perp));
+    return perp + n * (-sqrtf(fabsf(k)));
+}
+NR_HD f3 mv(const float* R, f3 v) {                                            // matrix.h:16-22
+    return mk3(R[0] * v.x + R[1] * v.y + R[2] * v.z,
+               R[3] * v.x + R[4] * v.y + R[5] * v.z,
+               R[6] * v.x + R[7] * v.y + R[8] * v.z);
+}
+NR_HD f3 mtv(const float* R, f3 v) {                                           // R^T * v
+    return mk3(R[0] * v.x + R[3] * v.y + R[6] * v.z,
+               R[1] * v.x + R[4] * v.y + R[7] * v.z,
+               R[2] * v.x + R[5] * v.y + R[8] * v.z);
+}
+
+struct Ray { f3 o, d, inv; };
+NR_HD Ray mkray(f3 o, f3 d) { Ray r; r.o = o; r.d = d; r.inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z); return r; }
+NR_HD f3 at(const Ray& r, float t) { return r.o + r.d * t; }
+NR_HD Ray to_object(const Ray& r, const GXform& x) {                           // ray.h:33-37
+    f3 o = mtv(x.R, r.o - ld3(x.pos)) * ld3(x.inv_s);
+    f3 d = mtv(x.R, r.d) * ld3(x.inv_s);
+    return mkray(o, d);
+}
+
+// AABB::hit_t (bvh/aabb.h:135-157): entry distance or FMAX
+NR_HD float box_t(const float* mn, const float* mx, const Ray& r, float tmin, float tmax) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float o = comp(r.o, a), iv = comp(r.inv, a);
+        float t0 = (mn[a] - o) * iv;
+        float t1 = (mx[a] - o) * iv;
+        if (t0 < t1) {
+            if (t0 > tmin) tmin = t0;
+            if (t1 < tmax) tmax = t1;
+        } else {
+            if (t1 > tmin) tmin = t1;
+            if (t0 < tmax) tmax = t0;
+        }
+        if (tmax <= tmin) return NR_FMAX;
+    }
+    return tmin;
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 per draw (core/random.h:50-55 with rocRAND's init semantics):
+// key = seed, counter = (draw/4, 0, subsequence, 0), word draw%4,
+// u = 2^-32 + word * 2^-32 in (0, 1].
+// ---------------------------------------------------------------------------
+NR_D uint32_t philox_word(uint32_t seed, uint32_t subseq, uint32_t draw) {
+    uint32_t c0 = draw >> 2, c1 = 0u, c2 = subseq, c3 = 0u;
+    uint32_t k0 = seed, k1 = 0u;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    uint32_t w = draw & 3u;
+    return w == 0 ? c0 : (w == 1 ? c1 : (w == 2 ? c2 : c3));
+}
+
+struct Gen {
+    uint32_t seed, subseq, draw;
+    NR_D float uniform() {
+        uint32_t w = philox_word(seed, subseq, draw);
+        draw += 1u;
+        const float inv = 2.3283064e-10f;
+        return inv + ((float)w * inv);
+    }
+    NR_D float rf(float mn, float mx) { return uniform() * (mx - mn) + mn; }
+    NR_D float rf() { return uniform() * (1.0f - 0.0f) + 0.0f; }
+};
+
+// ---------------------------------------------------------------------------
+// PCG hash (random/include/hash.h:16-191)
+// ---------------------------------------------------------------------------
+__constant__ static uint32_t NR_HASH_TABLE[64] = {
+    0x9E3779B9u, 0x85EBCA6Bu, 0xC2B2AE35u, 0x27D4EB2Fu, 0x165667B1u, 0x68D2A5B5u, 0xBF58476Du, 0x94D049BBu,
+    0xA3B195B5u, 0x1B873593u, 0xCC9E2D51u, 0x6C62272Eu, 0xE654025Bu, 0x2B5AD5B5u, 0x6C696E65u, 0x736F6D65u,
+    0x646F6D65u, 0x70736575u, 0x6E646F6Du, 0xFF51AFD7u, 0xC4CEA4D6u, 0x45D9F3Bu,  0x3335B369u, 0xA341316Cu,
+    0x52784632u, 0x68656C6Cu, 0x6F20776Fu, 0x726C6421u, 0xB5297A4Du, 0x68E31DA4u, 0x1B56C4E9u, 0xD2A98B26u,
+    0x637B2B53u, 0x4F6E4F35u, 0x62DC5AB7u, 0x5A17A2C3u, 0x6EF305DAu, 0x2C9F410Du, 0x3771A6A7u, 0x15C3A7C4u,
+    0x8F43CD0Bu, 0xA9DE8FA3u, 0xEBF7A401u, 0x58399E71u, 0x42B91A0Du, 0x4B608671u, 0xB41B21C7u, 0x2D5A73B7u,
+    0xE50B4C1Fu, 0x95C48A21u, 0x37A07BA9u, 0xC0F4A537u, 0xA8FA763Bu, 0xD08BE5EDu, 0x4A9C5EB3u, 0x7B14EEA7u,
+    0xCC3A6547u, 0x9B89D66Du, 0xE15E5E3Bu, 0x4F3D8A5Fu, 0xD4E0A97Bu, 0x3C6EF372u, 0xA54FF53Au, 0x510E527Fu,
+};
+#define NR_HASH_SCALAR (1.0f / 4294967296.0f)   // 1.0f / (float)0xFFFFFFFF, hash.h:16
+
+NR_D uint32_t hmix(uint32_t v, uint32_t ci) { v ^= (v >> 16u) * NR_HASH_TABLE[ci & 63u]; return v ^ (v >> 13u); }
+NR_D uint32_t pcg_hash(uint32_t i, uint32_t seed, uint32_t stream) {          // hash.h:47-61
+    uint32_t s = (seed * 1759274928u) ^ (stream * 2246822519u);
+    s = i ^ s;
+    s = hmix(s, stream * 3u);
+    s = hmix(s, stream * 3u + 1u);
+    s = hmix(s, stream * 3u + 2u);
+    uint32_t state = s * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+NR_D float pcg_f(float f, uint32_t seed) { return (float)pcg_hash(__float_as_uint(f), seed, 0u) * NR_HASH_SCALAR; }
+NR_D float pcg_f3(f3 a, uint32_t seed) {                                       // hash.h:92-105
+    uint32_t s = seed * 1759274928u;
+    s ^= pcg_hash(__float_as_uint(a.x), (seed + 0u) * 2654435761u, 0u) ^ 668265263u;
+    s ^= pcg_hash(__float_as_uint(a.y), (seed + 1u) * 2246822519u, 0u) ^ 374761393u;
+    s ^= pcg_hash(__float_as_uint(a.z), (seed + 2u) * 3266489917u, 0u) ^ 2654435761u;
+    uint32_t state = s * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (float)((word >> 22u) ^ word) * NR_HASH_SCALAR;
+}
+NR_D float pcg_ray(const Ray& r) {                                             // hash.h:107-124
+    float a = pcg_f3(r.o, 0u);
+    float b = pcg_f3(r.d, 3266489917u);
+    return (a + b) * 0.5f;
+}
+NR_D uint32_t f2u_sat(float x) {
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+NR_D f2 pcg_vec2(float f, uint32_t seed) {                                     // hash.h:182-191
+    f2 r;
+    r.x = pcg_f(f, seed);
+    uint32_t s = seed ^ f2u_sat(r.x * 4294967296.0f);
+    r.y = pcg_f((float)(s * 747796405u + 2891336453u), 0u);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// ONB (core/onb.h:5-26) and direction sampling (core/vector.h:489-519)
+// ---------------------------------------------------------------------------
+struct ONB {
+    f3 u, v, w;
+    NR_D void build(f3 n) {
+        w = normalize(n);
+        f3 a = (fabsf(w.x) > 0.9f) ? mk3(0.0f, 1.0f, 0.0f) : mk3(1.0f, 0.0f, 0.0f);
+        v = normalize(cross(w, a));
+        u = cross(w, v);
+    }
+    NR_D f3 xf(f3 p) const { return (u * p.x + v * p.y) + w * p.z; }
+};
+NR_D f3 cosine_dir(Gen& g) {
+    float r1 = g.rf();
+    float r2 = g.rf();
+    float phi = NR_PI2 * r1;
+    return mk3(cosf(phi) * sqrtf(r2), sinf(phi) * sqrtf(r2), sqrtf(1.0f - r2));
+}
+NR_D f3 unit_vector(Gen& g) {
+    while (true) {
+        float a = g.rf(-1.0f, 1.0f);
+        float b = g.rf(-1.0f, 1.0f);
+        float c = g.rf(-1.0f, 1.0f);
+        f3 p = mk3(a, b, c);
+        float l = len2(p);
+        if (1e-160 < (double)l && l <= 1.0f) return divs(p, sqrtf(l));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// hit record in flight (HitRecord, hittable/hit_record.h)
+// ---------------------------------------------------------------------------
+struct HRec {
+    f3 p, n, tg;
+    f2 uv;
+    float t;
+    bool front;
+    int32_t prim;
+};
+NR_D void hrec_clear(HRec& r) {
+    r.p = s3(0.0f); r.n = s3(0.0f); r.tg = s3(0.0f); r.uv.x = 0.0f; r.uv.y = 0.0f;
+    r.t = 0.0f; r.front = false; r.prim = -1;
+}
+
+// HitRecord::to_world_space (hit_record.h:24-51)
+NR_D void to_world(HRec& rec, const GXform& x, const Ray& obj_ray, const Ray& world_ray, bool bias) {
+    rec.p = mv(x.R, rec.p * ld3(x.scale)) + ld3(x.pos);
+    rec.tg = normalize(mv(x.R, rec.tg * ld3(x.scale)));
+    rec.front = dot(obj_ray.d, rec.n) < 0.0f;
+    rec.n = rec.front ? rec.n : -rec.n;
+    rec.n = normalize(mv(x.R, rec.n * ld3(x.inv_s)));
+    rec.front = dot(world_ray.d, rec.n) < 0.0f;
+    rec.n = rec.front ? rec.n : -rec.n;
+    rec.tg = normalize(rec.tg - rec.n * dot(rec.tg, rec.n));
+    if (vlength(rec.tg) < 1e-8f)
+        rec.tg = normalize(cross(rec.n, fabsf(rec.n.x) > 0.9f ? mk3(0.0f, 1.0f, 0.0f) : mk3(1.0f, 0.0f, 0.0f)));
+    if (bias) rec.p = rec.p + rec.n * NR_EPS;
+    rec.t = dot(rec.p - world_ray.o, world_ray.d) / len2(world_ray.d);
+}
+
+// Quad constants (hittable/primitives.h:8-16): u = (1,0,0), v = (0,0,-1),
+// n = cross(u, v) = (-0, 1, 0), normal = n, w = n / dot(n, n) = n, area 1.
+NR_D f3 quad_u() { return mk3(1.0f, 0.0f, 0.0f); }
+NR_D f3 quad_v() { return mk3(0.0f, 0.0f, -1.0f); }
+NR_D f3 quad_n() { return cross(quad_u(), quad_v()); }
+
+// Quad::hit (primitives.h:19-32, 73-88)
+NR_D bool quad_hit(const Ray& r, float tmin, float tmax, HRec& rec) {
+    f3 nrm = normalize(quad_n());
+    float denom = dot(nrm, r.d);
+    if (fabsf(denom) < NR_FMIN) return false;
+    float t = -dot(nrm, r.o) / denom;
+    if (!(tmin <= t && t <= tmax)) return false;
+    rec.t = t;
+    rec.p = at(r, t);
+    rec.n = nrm;
+    rec.tg = quad_u();
+    f3 n = quad_n();
+    f3 w = divs(n, dot(n, n));
+    float a = dot(w, cross(rec.p, quad_v()));
+    float b = dot(w, cross(quad_u(), rec.p));
+    rec.uv.x = a + 0.5f;
+    rec.uv.y = b + 0.5f;
+    return ((-0.5f < a) && (a < 0.5f)) & ((-0.5f < b) && (b < 0.5f));
+}
+
+// Sphere::hit (hittable/shapes.h:25-58)
+NR_D bool sphere_hit(const Ray& r, float tmin, float tmax, HRec& rec, float radius, const GXform& x) {
+    f3 oc = -r.o;
+    float a = len2(r.d);
+    float h = dot(r.d, oc);
+    float c = len2(oc) - radius * radius;
+    float disc = h * h - a * c;
+    if (disc < 0.0f) return false;
+    float sq = sqrtf(disc);
+    float root = (h - sq) / a;
+    if (!(tmin < root && root < tmax)) {
+        root = (h + sq) / a;
+        if (!(tmin < root && root < tmax)) return false;
+    }
+    rec.t = root;
+    rec.p = at(r, root);
+    f3 nn = divv(rec.p, adds(ld3(x.scale) * radius, NR_FMIN));
+    rec.n = nn;
+    rec.tg = normalize(mk3(-rec.p.z, 0.0f, rec.p.x));
+    float theta = acosf(-nn.y);
+    float phi = atan2f(-nn.z, nn.x) + NR_PI;
+    rec.uv.x = phi / NR_PI2;
+    rec.uv.y = theta / NR_PI;
+    return true;
+}
+
+// Cube::hit (hittable/shapes.h:142-173)
+NR_D bool cube_hit(const GScene& S, const Ray& r, float tmin, float tmax, HRec& rec) {
+    bool any = false;
+    int face = -1;
+    Ray face_ray = r;
+    for (int i = 0; i < 6; i++) {
+        const GXform& fx = S.cube_faces[i];
+        Ray fr = to_object(r, fx);
+        HRec fre;
+        hrec_clear(fre);
+        if (quad_hit(fr, tmin, tmax, fre)) {
+            any = true; face = i; face_ray = fr; rec = fre; tmax = fre.t;
+        }
+    }
+    if (any) to_world(rec, S.cube_faces[face], face_ray, r, false);
+    return any;
+}
+
+// Möller–Trumbore (hittable/mesh.h:338-365)
+NR_D bool tri_hit(const Ray& r, f3 v0, f3 v1, f3 v2, float tmin, float tmax, float& ot, float& ou, float& ov) {
+    f3 e1 = v1 - v0, e2 = v2 - v0;
+    f3 h = cross(r.d, e2);
+    float a = dot(e1, h);
+    if (fabsf(a) < NR_FMIN) return false;
+    float f = 1.0f / a;
+    f3 s = r.o - v0;
+    float u = f * dot(s, h);
+    if (u < 0.0f || u > 1.0f) return false;
+    f3 q = cross(s, e1);
+    float v = f * dot(r.d, q);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float t = f * dot(e2, q);
+    if (!(tmin <= t && t <= tmax)) return false;
+    ot = t; ou = u; ov = v;
+    return true;
+}
+
+// LDS traversal stack: one column per lane, entries (ref, entry t)
+struct Stack {
+    int2* base;     // base[level * 64 + lane]
+    int lane;
+    int cap;
+    NR_D void push(int& sp, int32_t ref, float t) {
+        if (sp < cap) { base[sp * 64 + lane] = make_int2(ref, __float_as_int(t)); }
+        sp++;
+    }
+    NR_D void pop(int& sp, int32_t& ref, float& t) {
+        sp--;
+        int2 e = base[sp * 64 + lane];
+        ref = e.x; t = __int_as_float(e.y);
+    }
+};
+
+NR_D void load_node(const GNode* nodes, int32_t i, float* lmn, float* lmx, float* rmn, float* rmx, int32_t& l, int32_t& r) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    float4 a = p[0];
+    float4 b = p[1];
+    float4 c = p[2];
+    float4 d = p[3];
+    lmn[0] = a.x; lmn[1] = a.y; lmn[2] = a.z; lmx[0] = a.w;
+    lmx[1] = b.x; lmx[2] = b.y; rmn[0] = b.z; rmn[1] = b.w;
+    rmn[2] = c.x; rmx[0] = c.y; rmx[1] = c.z; rmx[2] = c.w;
+    l = __float_as_int(d.x); r = __float_as_int(d.y);
+}
+
+// Mesh::hit traversal (hittable/mesh.h:239-292) in near-first order with the
+// child entry distance kept on the stack: popping (ref, t) and testing
+// t < tmax equals the reference's re-test of the popped node's box, because
+// tmin never changes and tmax only shrinks during a traversal.
+// Returns the BVH slot of the closest triangle (or -1), its t, u, v.
+// `overflow` is set if the stack would exceed its capacity.
+NR_D int32_t mesh_closest(const GMesh& M, const Ray& r, float tmin, float tmax, Stack& st, int sp0,
+                          float& bt, float& bu, float& bv, bool& overflow) {
+    int32_t best = -1;
+    int sp = sp0;
+    if (sp0 + 1 > st.cap) { overflow = true; return -1; }
+    st.push(sp, M.root_ref, box_t(M.root_min, M.root_max, r, tmin, tmax));
+    while (sp > sp0) {
+        int32_t ref; float te;
+        st.pop(sp, ref, te);
+        if (!(te < tmax)) continue;
+        if (ref < 0) {
+            int32_t slot = ~ref;
+            const float4* tp = reinterpret_cast<const float4*>(M.tris + slot);
+            float4 a = tp[0], b = tp[1], c = tp[2];
+            float t, u, v;
+            if (tri_hit(r, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), mk3(b.z, b.w, c.x), tmin, tmax, t, u, v)) {
+                best = slot; tmax = t; bt = t; bu = u; bv = v;
+            }
+        } else {
+            float lmn[3], lmx[3], rmn[3], rmx[3]; int32_t l, rr;
+            load_node(M.nodes, ref, lmn, lmx, rmn, rmx, l, rr);
+            float tl = box_t(lmn, lmx, r, tmin, tmax);
+            float tr = box_t(rmn, rmx, r, tmin, tmax);
+            bool left_near = tl < tr;
+            int32_t near_i = left_near ? l : rr, far_i = left_near ? rr : l;
+            float near_t = left_near ? tl : tr, far_t = left_near ? tr : tl;
+            if (sp + 2 > st.cap) { overflow = true; break; }
+            if (far_t < tmax) st.push(sp, far_i, far_t);
+            st.push(sp, near_i, near_t);
+        }
+    }
+    return best;
+}
+
+// interpolate_hit (hittable/mesh.h:367-383)
+NR_D void mesh_interp(const GMesh& M, int32_t slot, const Ray& r, float t, float u, float v, HRec& rec) {
+    const uint32_t* ix = M.idx + 3 * (size_t)slot;
+    const GVertex& a = M.verts[ix[0]];
+    const GVertex& b = M.verts[ix[1]];
+    const GVertex& c = M.verts[ix[2]];
+    float w = 1.0f - u - v;
+    rec.t = t;
+    rec.p = at(r, t);
+    rec.uv.x = a.uv[0] * w + b.uv[0] * u + c.uv[0] * v;
+    rec.uv.y = a.uv[1] * w + b.uv[1] * u + c.uv[1] * v;
+    rec.n = normalize((ld3(a.normal) * w + ld3(b.normal) * u) + ld3(c.normal) * v);
+    rec.tg = normalize((ld3(a.tangent) * w + ld3(b.tangent) * u) + ld3(c.tangent) * v);
+    rec.prim = M.tris[slot].orig;
+}
+
+NR_D bool mesh_hit(const GScene& S, int mesh, const Ray& r, float tmin, float tmax, HRec& rec,
+                   Stack& st, int sp0, bool& overflow) {
+    const GMesh& M = S.meshes[mesh];
+    float t, u, v;
+    int32_t slot = mesh_closest(M, r, tmin, tmax, st, sp0, t, u, v, overflow);
+    if (slot < 0) return false;
+    mesh_interp(M, slot, r, t, u, v, rec);
+    return true;
+}
+
+// Hittable::hit of a primitive type (hittable/hittable.h:332-352)
+NR_D bool prim_hit(const GScene& S, int type, const GObject& o, const Ray& r, float tmin, float tmax, HRec& rec,
+                   Stack& st, int sp0, bool& overflow) {
+    switch (type) {
+        case 1: return quad_hit(r, tmin, tmax, rec);
+        case 2: return sphere_hit(r, tmin, tmax, rec, o.radius, o.xf);
+        case 3: return cube_hit(S, r, tmin, tmax, rec);
+        case 4: return mesh_hit(S, o.mesh, r, tmin, tmax, rec, st, sp0, overflow);
+    }
+    return false;
+}
+
+// ConstantMedium::hit (hittable/volumes.h:14-45)
+NR_D bool medium_hit(const GScene& S, const GObject& o, const Ray& r, float tmin, float tmax, HRec& rec,
+                     Stack& st, int sp0, bool& overflow) {
+    HRec r1, r2;
+    hrec_clear(r1);
+    hrec_clear(r2);
+    if (!prim_hit(S, o.bound_type, o, r, -NR_FMAX, NR_FMAX, r1, st, sp0, overflow)) return false;
+    if (!prim_hit(S, o.bound_type, o, r, r1.t + NR_EPS, NR_FMAX, r2, st, sp0, overflow)) return false;
+    if (r1.t < tmin) r1.t = tmin;
+    if (r2.t > tmax) r2.t = tmax;
+    if (r1.t >= r2.t) return false;
+    if (r1.t < 0.0f) r1.t = 0.0f;
+    float ray_length = vlength(r.d);
+    float inside = (r2.t - r1.t) * ray_length;
+    float hit_distance = o.neg_inv_density * logf(pcg_ray(r));
+    if (hit_distance > inside) return false;
+    rec.p = at(r, r1.t + hit_distance / ray_length);
+    rec.n = mk3(1.0f, 0.0f, 0.0f);
+    rec.front = true;
+    return true;
+}
+
+// Hittable::hit_test (hittable/hittable.h:324-330)
+NR_D bool hit_test(const GScene& S, const GObject& o, const Ray& r, HRec& rec, Stack& st, int sp0, bool& overflow) {
+    Ray orr = to_object(r, o.xf);
+    bool h;
+    if (o.type == 6)          // ObjectLight (lights.h:11-20)
+        h = prim_hit(S, o.bound_type, o, orr, -NR_FMAX, NR_FMAX, rec, st, sp0, overflow);
+    else if (o.type == 5)
+        h = medium_hit(S, o, orr, NR_EPS, NR_FMAX, rec, st, sp0, overflow);
+    else
+        h = prim_hit(S, o.type, o, orr, NR_EPS, NR_FMAX, rec, st, sp0, overflow);
+    to_world(rec, o.xf, orr, r, true);
+    return h;
+}
+
+// SceneGraph::hit (scene/include/graph.h:16-66).  Returns the object table
+// index of the closest hit or -1; rec receives the world-space record.
+NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, bool& overflow) {
+    float tmin = NR_EPS, tmax = NR_FMAX;
+    int32_t best = -1;
+    int sp = 0;
+    if (st.cap < 1 || S.n_objects == 0) return -1;
+    st.push(sp, S.root_ref, box_t(S.root_min, S.root_max, r, tmin, tmax));
+    while (sp > 0) {
+        int32_t ref; float te;
+        st.pop(sp, ref, te);
+        if (!(te < tmax)) continue;
+        if (ref < 0) {
+            int32_t oi = ~ref;
+            const GObject& o = S.objects[oi];
+            HRec tmp;
+            hrec_clear(tmp);
+            if (hit_test(S, o, r, tmp, st, sp, overflow)) {
+                if (tmin < tmp.t && tmp.t < tmax) {
+                    best = oi;
+                    rec = tmp;
+                    tmax = tmp.t;
+                }
+            }
+            if (overflow) break;
+        } else {
+            float lmn[3], lmx[3], rmn[3], rmx[3]; int32_t l, rr;
+            load_node(S.nodes, ref, lmn, lmx, rmn, rmx, l, rr);
+            float tl = box_t(lmn, lmx, r, tmin, tmax);
+            float tr = box_t(rmn, rmx, r, tmin, tmax);
+            bool left_near = tl < tr;
+            int32_t near_i = left_near ? l : rr, far_i = left_near ? rr : l;
+            float near_t = left_near ? tl : tr, far_t = left_near ? tr : tl;
+            if (sp + 2 > st.cap) { overflow = true; break; }
+            if (far_t < tmax) st.push(sp, far_i, far_t);
+            st.push(sp, near_i, near_t);
+        }
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// textures (material/texture.h:49-91)
+// ---------------------------------------------------------------------------
+NR_D f3 tex_sample(const GScene& S, const GTex& t, f2 uv) {
+    if (t.image < 0) return ld3(t.color);
+    const GImage& im = S.images[t.image];
+    float sc = 1.0f / fmaxf(NR_FMIN, t.scale);
+    float su = fmodf(uv.x * sc, 1.0f), sv = fmodf(uv.y * sc, 1.0f);
+    float u = su < 0.0f ? 0.0f : (su > 1.0f ? 1.0f : su);
+    float v = 1.0f - (sv < 0.0f ? 0.0f : (sv > 1.0f ? 1.0f : sv));
+    int i = (int)(u * (float)im.W); if (i >= im.W) i = im.W - 1;
+    int j = (int)(v * (float)im.H); if (j >= im.H) j = im.H - 1;
+    size_t px = (size_t)j * im.W + i, hw = (size_t)im.H * im.W;
+    f3 c = mk3((float)im.data[px], (float)im.data[hw + px], (float)im.data[2 * hw + px]);
+    return divs(c, 255.0f);
+}
+
+// ---------------------------------------------------------------------------
+// sky (light/include/skylight.h:33-122)
+// ---------------------------------------------------------------------------
+NR_D f3 sky_sample(const GSky& sk, f3 d) {
+    if (sk.type == 1) {
+        float a = (float)(0.5 * ((double)d.y + 1.0));
+        float oma = (float)(1.0 - (double)a);
+        return ld3(sk.start) * oma + ld3(sk.end) * a;
+    }
+    if (sk.type == 2) {
+        if (!sk.data) return s3(0.0f);
+        f3 dir = mv(sk.rot, normalize(d));
+        float u = atan2f(dir.z, dir.x) / NR_PI2 + 0.5f;
+        float v = acosf(fminf(fmaxf(dir.y, -1.0f), 1.0f)) / NR_PI;
+        int W = sk.W, H = sk.H;
+        float fx = u * (float)W - 0.5f;
+        float fy = v * (float)H - 0.5f;
+        int x0 = (int)floorf(fx), y0 = (int)floorf(fy);
+        int x1 = x0 + 1, y1 = y0 + 1;
+        float tx = fx - (float)x0, ty = fy - (float)y0;
+        x0 = ((x0 % W) + W) % W;
+        x1 = ((x1 % W) + W) % W;
+        y0 = (int)fmaxf(0.0f, fminf((float)(H - 1), (float)y0));
+        y1 = (int)fmaxf(0.0f, fminf((float)(H - 1), (float)y1));
+        size_t hw = (size_t)H * W;
+        size_t i00 = (size_t)y0 * W + x0, i10 = (size_t)y0 * W + x1, i01 = (size_t)y1 * W + x0, i11 = (size_t)y1 * W + x1;
+        f3 c00 = mk3(sk.data[i00], sk.data[hw + i00], sk.data[2 * hw + i00]);
+        f3 c10 = mk3(sk.data[i10], sk.data[hw + i10], sk.data[2 * hw + i10]);
+        f3 c01 = mk3(sk.data[i01], sk.data[hw + i01], sk.data[2 * hw + i01]);
+        f3 c11 = mk3(sk.data[i11], sk.data[hw + i11], sk.data[2 * hw + i11]);
+        f3 top = c00 * (1.0f - tx) + c10 * tx;
+        f3 bot = c01 * (1.0f - tx) + c11 * tx;
+        return (top * (1.0f - ty) + bot * ty) * sk.intensity;
+    }
+    return s3(0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// PDFs (engine/include/engine/pdf.h)
+// ---------------------------------------------------------------------------
+NR_D float ggx_D(float n_dot_h, float alpha) {                                // pdf.h:100-104
+    float a2 = alpha * alpha;
+    float d = n_dot_h * n_dot_h * (a2 - 1.0f) + 1.0f;
+    return a2 / fmaxf(NR_PI * d * d, 1e-8f);
+}
+NR_D float smith_g1(float nx, float alpha) {                                   // pdf.h:106-111
+    float a2 = alpha * alpha;
+    return 2.0f * nx / (nx + sqrtf(a2 + (1.0f - a2) * nx * nx));
+}
+NR_D float pbr_pdf_value(const ONB& uvw, f3 view, float alpha, float sp, f3 d) {   // pdf.h:135-148
+    float diffuse_pdf = fmaxf(0.0f, dot(normalize(d), uvw.w) / NR_PI);
+    f3 h = normalize(view + d);
+    float n_dot_h = fmaxf(dot(uvw.w, h), 0.0f);
+    float v_dot_h = fmaxf(dot(view, h), 1e-4f);
+    float spec_pdf = (ggx_D(n_dot_h, alpha) * n_dot_h) / (4.0f * v_dot_h);
+    return (1.0f - sp) * diffuse_pdf + sp * spec_pdf;
+}
+NR_D f3 pbr_pdf_generate(const ONB& uvw, f3 view, float alpha, float sp, Gen& g) { // pdf.h:150-166
+    if (g.rf() < sp) {
+        float u1 = g.rf();
+        float u2 = g.rf();
+        float cos_theta = sqrtf((1.0f - u1) / (1.0f + (alpha * alpha - 1.0f) * u1));
+        float sin_theta = sqrtf(fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
+        float phi = NR_PI2 * u2;
+        f3 h = uvw.xf(mk3(sin_theta * cosf(phi), sin_theta * sinf(phi), cos_theta));
+        return normalize(h * (2.0f * dot(view, h)) - view);
+    }
+    return uvw.xf(cosine_dir(g));
+}
+NR_D float cosine_pdf_value(const ONB& uvw, f3 d) {                           // pdf.h:26-29
+    float c = dot(normalize(d), uvw.w);
+    return fmaxf(0.0f, c / NR_PI);
+}
+
+// ---------------------------------------------------------------------------
+// light sampling of the first ObjectLight (hittable.h:356-386)
+// ---------------------------------------------------------------------------
+NR_D float quad_light_pdf(const GObject& o, f3 origin, f3 dir) {              // primitives.h:34-49
+    HRec rec;
+    hrec_clear(rec);
+    Ray wr = mkray(origin, dir);
+    Ray orr = to_object(wr, o.xf);
+    bool h = quad_hit(orr, NR_EPS, NR_FMAX, rec);
+    to_world(rec, o.xf, orr, wr, true);
+    if (!h) return 0.0f;
+    float dist2 = rec.t * rec.t * len2(dir);
+    float cosine = fabsf(dot(dir, rec.n) / vlength(dir));
+    if (cosine < 1e-8f) return 0.0f;
+    f3 n = quad_n();
+    float world_area = vlength(n) * o.xf.scale[0] * o.xf.scale[2];
+    return dist2 / (cosine * world_area);
+}
+NR_D f3 quad_light_random(const GObject& o, f3 origin, Gen& g) {              // primitives.h:52-61
+    f3 corner = quad_u() * -0.5f - quad_v() * 0.5f;
+    float r1 = g.rf();
+    float r2 = g.rf();
+    f3 p = (corner + quad_u() * r1) + quad_v() * r2;
+    p = mv(o.xf.R, p * ld3(o.xf.scale)) + ld3(o.xf.pos);
+    return p - origin;
+}
+NR_D float sphere_light_pdf(const GObject& o, f3 origin, f3 dir) {            // shapes.h:60-76
+    HRec rec;
+    hrec_clear(rec);
+    if (!sphere_hit(mkray(origin, dir), NR_EPS, NR_FMAX, rec, o.radius, o.xf)) return 0.0f;
+    float rad_sq = o.radius * o.radius;
+    float dist2 = len2(ld3(o.xf.pos) - origin);
+    float cos_theta_max = sqrtf(1.0f - rad_sq / dist2);
+    float solid_angle = NR_PI2 * (1.0f - cos_theta_max);
+    return 1.0f / solid_angle;
+}
+NR_D f3 sphere_light_random(const GObject& o, f3 origin, Gen& g) {            // shapes.h:78-108
+    f3 direction = ld3(o.xf.pos) - origin;
+    float dist2 = len2(direction);
+    ONB uvw;
+    uvw.build(direction);
+    float r1 = g.rf();
+    float r2 = g.rf();
+    float rad_sq = o.radius * o.radius;
+    float z = 1.0f + r2 * (sqrtf(1.0f - rad_sq / dist2) - 1.0f);
+    float phi = NR_PI2 * r1;
+    float x = cosf(phi) * sqrtf(1.0f - z * z);
+    float y = sinf(phi) * sqrtf(1.0f - z * z);
+    return uvw.xf(mk3(x, y, z));
+}
+NR_D int light_geom(const GObject& L) {
+    return L.type == 6 ? L.bound_type : (L.type == 5 ? -1 : L.type);
+}
+NR_D float light_pdf(const GObject& L, f3 origin, f3 dir) {
+    int t = light_geom(L);
+    if (t == 1) return quad_light_pdf(L, origin, dir);
+    if (t == 2) return sphere_light_pdf(L, origin, dir);
+    return 0.0f;
+}
+NR_D f3 light_random(const GObject& L, f3 origin, Gen& g) {
+    int t = light_geom(L);
+    if (t == 1) return quad_light_random(L, origin, g);
+    if (t == 2) return sphere_light_random(L, origin, g);
+    if (t == 4) return s3(0.0f);
+    return mk3(1.0f, 0.0f, 0.0f);
+}
+
+// PBR::get_shading_normal (material/material.h:113-125)
+NR_D f3 pbr_shading_normal(const GScene& S, const GMaterial& m, f3 n, f3 tg, f2 uv) {
+    f3 tn = tex_sample(S, m.tex[4], uv);
+    f3 map_n = normalize(mk3(tn.x * 2.0f - 1.0f, tn.y * 2.0f - 1.0f, tn.z * 2.0f - 1.0f));
+    f3 t = normalize(tg - n * dot(tg, n));
+    f3 b = cross(n, t);
+    return normalize((t * map_n.x + b * map_n.y) + n * map_n.z);
+}

@@ -1,0 +1,136 @@
+"""Loader and thin Python wrapper of libnectar_hip.so (include/nectar_hip.h).
+
+The library is built in-tree (``python -c "import __graft_entry__ as g; g.build()"``).
+There is no fallback: if the library is missing or no GPU is visible every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnectar_hip.so")
+_lib = None
+
+
+class NectarError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NectarError(f"HIP library not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "nr_abi_version": (C.c_int, []),
+        "nr_last_error": (C.c_char_p, []),
+        "nr_create": (vp, [C.c_int]),
+        "nr_destroy": (None, [vp]),
+        "nr_scene_upload": (C.c_int, [vp, C.POINTER(abi.NrSceneDesc)]),
+        "nr_camera_set": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrCameraDesc), C.c_uint32]),
+        "nr_aov_reset": (C.c_int, [vp, C.c_int]),
+        "nr_render": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrRenderDesc)]),
+        "nr_aov_read": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_size_t]),
+        "nr_aov_device_ptr": (vp, [vp, C.c_int, C.c_int]),
+        "nr_trace_closest": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int32, C.POINTER(abi.NrHit)]),
+        "nr_trace_closest_device": (C.c_int, [vp, vp, C.c_int32, vp]),
+        "nr_post": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrPostDesc), C.POINTER(C.c_uint8), C.POINTER(C.c_float)]),
+        "nr_sync": (C.c_int, [vp]),
+        "nr_last_timings": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int]),
+        "nr_set_timing": (C.c_int, [vp, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    if L.nr_abi_version() != 1:
+        raise NectarError("ABI version mismatch")
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != abi.NR_OK:
+        raise NectarError(f"nectar_hip error {rc}: {lib().nr_last_error().decode()}")
+
+
+class Context:
+    """One HIP device context: a scene, up to 16 views, their AOVs."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        self._h = L.nr_create(device)
+        if not self._h:
+            raise NectarError(f"nr_create failed: {L.nr_last_error().decode()}")
+        self._scene_keep = None
+        self.views = {}
+
+    def close(self):
+        if self._h:
+            lib().nr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scene: abi.SceneDesc):
+        d = scene.to_c()
+        self._scene_keep = scene
+        _check(lib().nr_scene_upload(self._h, C.byref(d)))
+
+    def set_camera(self, view: int, cam: abi.NrCameraDesc, seed: int):
+        _check(lib().nr_camera_set(self._h, view, C.byref(cam), seed))
+        self.views[view] = (int(cam.resolution[0]), int(cam.resolution[1]))
+
+    def reset(self, view: int = 0):
+        _check(lib().nr_aov_reset(self._h, view))
+
+    def render(self, view: int, desc: abi.NrRenderDesc):
+        _check(lib().nr_render(self._h, view, C.byref(desc)))
+
+    def sync(self):
+        _check(lib().nr_sync(self._h))
+
+    def read(self, view: int = 0, layer: int = abi.LAYER_BEAUTY) -> np.ndarray:
+        H, W = self.views[view]
+        out = np.empty((3, H, W), np.float32)
+        _check(lib().nr_aov_read(self._h, view, layer, out.ctypes.data_as(C.POINTER(C.c_float)), out.size))
+        return out
+
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        hits = np.empty(r.shape[0], abi.HIT_DTYPE)
+        _check(lib().nr_trace_closest(self._h, r.ctypes.data_as(C.POINTER(C.c_float)), r.shape[0],
+                                      hits.ctypes.data_as(C.POINTER(abi.NrHit))))
+        return hits
+
+    def trace_closest_device(self, d_rays: int, n: int, d_hits: int):
+        _check(lib().nr_trace_closest_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits)))
+
+    def post(self, view: int, desc: abi.NrPostDesc):
+        H, W = self.views[view]
+        u8 = np.empty((3, H, W), np.uint8)
+        f = np.empty((3, H, W), np.float32)
+        _check(lib().nr_post(self._h, view, C.byref(desc), u8.ctypes.data_as(C.POINTER(C.c_uint8)),
+                             f.ctypes.data_as(C.POINTER(C.c_float))))
+        return u8, f
+
+    def set_timing(self, on: bool):
+        _check(lib().nr_set_timing(self._h, int(on)))
+
+    def timings(self):
+        buf = (C.c_float * 8)()
+        k = lib().nr_last_timings(self._h, buf, 8)
+        return list(buf[:k])

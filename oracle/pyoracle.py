@@ -1,0 +1,204 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/libnr_oracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker / the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from nectarrender_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libnr_oracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    fp = C.POINTER(C.c_float)
+    sig = {
+        "nro_scene_build": (vp, [C.POINTER(abi.NrSceneDesc)]),
+        "nro_scene_free": (None, [vp]),
+        "nro_scene_first_light": (C.c_int, [vp]),
+        "nro_scene_mesh_depth": (C.c_int, [vp, C.c_int]),
+        "nro_trace_closest": (C.c_int, [vp, fp, C.c_int, C.POINTER(abi.NrHit), C.c_int]),
+        "nro_trace_counts": (C.c_int, [vp, fp, C.c_int, C.POINTER(C.c_int64)]),
+        "nro_camera_build": (vp, [C.POINTER(abi.NrCameraDesc), C.c_uint32]),
+        "nro_camera_free": (None, [vp]),
+        "nro_camera_ray": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, fp]),
+        "nro_nrooks": (C.c_int, [C.c_uint32, C.c_uint32, fp]),
+        "nro_render_pass": (C.c_int, [vp, vp, C.POINTER(abi.NrRenderDesc), C.c_uint32, C.POINTER(C.c_int32),
+                                      C.c_int, fp, fp, fp, C.POINTER(C.c_int64), C.c_int]),
+        "nro_render": (C.c_int, [vp, vp, C.POINTER(abi.NrRenderDesc), fp, fp, fp, C.c_int]),
+        "nro_accumulate": (None, [fp, fp, C.c_size_t, C.c_uint32]),
+        "nro_post": (C.c_int, [fp, C.c_int, C.c_int, C.POINTER(abi.NrPostDesc), fp, C.POINTER(C.c_uint8)]),
+        "nro_philox_block": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "nro_philox_uniform": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32]),
+        "nro_pcg_hash_int": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
+        "nro_pcg_float": (C.c_float, [C.c_float, C.c_uint32, C.c_uint32]),
+        "nro_pcg_float3": (C.c_float, [fp, C.c_uint32, C.c_uint32]),
+        "nro_pcg_float_ray": (C.c_float, [fp, fp]),
+        "nro_pcg_vector2": (None, [C.c_float, C.c_uint32, fp]),
+        "nro_bvh_build": (C.c_int, [fp, C.c_int, C.POINTER(C.c_int32), fp, C.POINTER(C.c_int32)]),
+        "nro_sky_sample": (C.c_int, [vp, fp, fp]),
+        "nro_ggx_distribution": (C.c_float, [C.c_float, C.c_float]),
+        "nro_pdf_pbr_value": (C.c_float, [fp, fp, C.c_float, C.c_float, fp]),
+        "nro_pdf_pbr_generate": (C.c_int, [fp, fp, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32, fp]),
+        "nro_pdf_cosine_value": (C.c_float, [fp, fp]),
+        "nro_pdf_cosine_generate": (C.c_int, [fp, C.c_uint32, C.c_uint32, C.c_uint32, fp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def f3(v):
+    return np.ascontiguousarray(v, np.float32)
+
+
+class OracleScene:
+    def __init__(self, scene: abi.SceneDesc):
+        self._keep = scene
+        d = scene.to_c()
+        self.h = lib().nro_scene_build(C.byref(d))
+        if not self.h:
+            raise RuntimeError("oracle scene build failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().nro_scene_free(self.h)
+            self.h = None
+
+    def trace_closest(self, rays: np.ndarray, threads: int = 0) -> np.ndarray:
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        hits = np.empty(r.shape[0], abi.HIT_DTYPE)
+        rc = lib().nro_trace_closest(self.h, _fp(r), r.shape[0], hits.ctypes.data_as(C.POINTER(abi.NrHit)), threads)
+        if rc:
+            raise RuntimeError("oracle traversal failed")
+        return hits
+
+    def trace_counts(self, rays: np.ndarray):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        c = np.zeros(3, np.int64)
+        lib().nro_trace_counts(self.h, _fp(r), r.shape[0], c.ctypes.data_as(C.POINTER(C.c_int64)))
+        return c
+
+    def first_light(self):
+        return lib().nro_scene_first_light(self.h)
+
+    def sky(self, d):
+        out = np.zeros(3, np.float32)
+        lib().nro_sky_sample(self.h, _fp(f3(d)), _fp(out))
+        return out
+
+
+class OracleCamera:
+    def __init__(self, cam: abi.NrCameraDesc, seed: int):
+        self.desc = cam
+        self.H, self.W = int(cam.resolution[0]), int(cam.resolution[1])
+        self.h = lib().nro_camera_build(C.byref(cam), seed)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().nro_camera_free(self.h)
+            self.h = None
+
+    def ray(self, x, y, s, seed):
+        out = np.zeros(7, np.float32)
+        rc = lib().nro_camera_ray(self.h, x, y, s, seed, _fp(out))
+        if rc:
+            raise ValueError("sample index beyond the pattern")
+        return out
+
+
+def render_pass(scene: OracleScene, cam: OracleCamera, desc: abi.NrRenderDesc, s: int, pixels=None, threads=0,
+                counts=False):
+    n = cam.H * cam.W
+    b = np.zeros((3, cam.H, cam.W), np.float32)
+    d = np.zeros_like(b)
+    nn = np.zeros_like(b)
+    px = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
+    cnt = np.zeros(5, np.int64)
+    rc = lib().nro_render_pass(scene.h, cam.h, C.byref(desc), s,
+                               None if px is None else px.ctypes.data_as(C.POINTER(C.c_int32)),
+                               0 if px is None else px.size, _fp(b), _fp(d), _fp(nn),
+                               cnt.ctypes.data_as(C.POINTER(C.c_int64)) if counts else None, threads)
+    if rc:
+        raise RuntimeError("oracle render pass failed")
+    return (b, d, nn, cnt) if counts else (b, d, nn)
+
+
+def render(scene: OracleScene, cam: OracleCamera, desc: abi.NrRenderDesc, threads=0):
+    b = np.zeros((3, cam.H, cam.W), np.float32)
+    d = np.zeros_like(b)
+    nn = np.zeros_like(b)
+    rc = lib().nro_render(scene.h, cam.h, C.byref(desc), _fp(b), _fp(d), _fp(nn), threads)
+    if rc:
+        raise RuntimeError("oracle render failed")
+    return b, d, nn
+
+
+def accumulate(avg: np.ndarray, sample: np.ndarray, s: int):
+    a = np.ascontiguousarray(avg, np.float32)
+    lib().nro_accumulate(_fp(a), _fp(np.ascontiguousarray(sample, np.float32)), a.size, s)
+    return a
+
+
+def post(chw: np.ndarray, desc: abi.NrPostDesc):
+    img = np.ascontiguousarray(chw, np.float32)
+    _, H, W = img.shape
+    f = np.empty_like(img)
+    u8 = np.empty(img.shape, np.uint8)
+    lib().nro_post(_fp(img), H, W, C.byref(desc), _fp(f), u8.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return u8, f
+
+
+def philox_uniform(seed, subseq, offset):
+    return lib().nro_philox_uniform(seed, subseq, offset)
+
+
+def philox_block(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().nro_philox_block(c, k, o)
+    return list(o)
+
+
+def nrooks(n, seed):
+    out = np.zeros(2 * n, np.float32)
+    lib().nro_nrooks(n, seed, _fp(out))
+    return out.reshape(n, 2)
+
+
+def bvh_build(boxes: np.ndarray):
+    b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+    n = b.shape[0]
+    nodes = np.zeros((2 * n, 3), np.int32)
+    nb = np.zeros((2 * n, 6), np.float32)
+    order = np.zeros(n, np.int32)
+    k = lib().nro_bvh_build(_fp(b), n, nodes.ctypes.data_as(C.POINTER(C.c_int32)), _fp(nb),
+                            order.ctypes.data_as(C.POINTER(C.c_int32)))
+    return nodes[:k], nb[:k], order

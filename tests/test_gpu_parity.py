@@ -1,0 +1,132 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Closest-hit records are compared bit for bit (IEEE +,-,*,/,sqrt on both sides,
+same BVH, same near-first order); sphere uv and medium hits go through acos /
+atan2 / log, so those fields get a 1e-5 relative tolerance.  Rendered images
+are compared per pixel at 1e-4 relative: transcendental last-ulp differences
+can flip a branch (Fresnel, roulette, rejection loops) in a few paths, so the
+share of pixels beyond the tolerance must stay below 0.5%.
+"""
+import numpy as np
+import pytest
+
+from nectarrender_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-4
+
+
+def _hits_compare(g, o, approx_mask):
+    assert np.array_equal(g["object_id"], o["object_id"]), "closest object differs"
+    assert np.array_equal(g["prim"], o["prim"]), "closest triangle differs"
+    hit = g["object_id"] >= 0
+    exact = hit & ~approx_mask
+    for f in ("t", "p", "n", "tangent"):
+        a, b = g[f][exact], o[f][exact]
+        same = (a == b) | (np.isnan(a) & np.isnan(b))
+        assert same.all(), f"field {f} not bit-exact on {np.count_nonzero(~same)} values"
+    a, b = g["front_face"][hit], o["front_face"][hit]
+    assert np.array_equal(a, b)
+    approx = hit & approx_mask
+    if approx.any():
+        for f in ("t", "p"):
+            a, b = g[f][approx], o[f][approx]
+            assert np.allclose(a, b, rtol=1e-5, atol=1e-6, equal_nan=True), f
+    a, b = g["uv"][hit], o["uv"][hit]
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-6, equal_nan=True)
+
+
+def _approx_mask(scene, hits):
+    """Hits whose record passes through a transcendental: constant media (log)."""
+    media = {i for i, o in enumerate(scene.objects) if o["type"] == abi.HIT_CONSTANT_MEDIUM}
+    return np.isin(hits["object_id"], list(media)) if media else np.zeros(hits.shape, bool)
+
+
+@pytest.mark.parametrize("name", ["config1", "cornell", "mixed", "icosphere"])
+def test_closest_hit_parity(ctx, oracle, name):
+    if name == "config1":
+        s, _, _ = scenes.config1()
+        rays = scenes.random_rays(200_000, 1, center=(0, 0.5, 0), spread=4.0)
+    elif name == "cornell":
+        s, _, _ = scenes.cornell()
+        rays = scenes.random_rays(200_000, 2, spread=1.2)
+    elif name == "mixed":
+        s, _, _ = scenes.mixed_scene()
+        rays = scenes.random_rays(200_000, 3, center=(0, 0.3, 0), spread=2.5)
+    else:
+        v, t = scenes.displaced_icosphere(freq=48)
+        s = abi.SceneDesc()
+        m = s.add_mesh(v, t)
+        s.mesh(m, (0.1, 0, 0), (10, 20, 30), (1, 1.2, 0.9), abi.Material.lambertian())
+        rays = scenes.random_rays(200_000, 4, spread=2.0)
+    ctx.upload(s)
+    g = ctx.trace_closest(rays)
+    o = oracle.OracleScene(s).trace_closest(rays)
+    assert (g["object_id"] >= 0).mean() > 0.2
+    _hits_compare(g, o, _approx_mask(s, o))
+
+
+def test_closest_hit_edge_cases(ctx, oracle):
+    """Rays through exact edges/vertices of a mesh, grazing and axis-parallel rays, rays from inside."""
+    P, T = scenes.geodesic_sphere(4)
+    s = abi.SceneDesc()
+    m = s.add_mesh(scenes.mesh_vertices(P.astype(np.float32), T), T)
+    s.mesh(m, (0, 0, 0), (0, 0, 0), (1, 1, 1), abi.Material.lambertian())
+    s.quad((0, -1, 0), (0, 0, 0), (4, 1, 4), abi.Material.lambertian())
+    s.sphere((2, 0, 0), 0.5, abi.Material.lambertian())
+    verts = P.astype(np.float32)
+    mids = ((verts[T[:, 0]] + verts[T[:, 1]]) * 0.5).astype(np.float32)
+    tg = np.concatenate([verts, mids])
+    o = np.tile(np.array([[0.05, 0.02, 3.0]], np.float32), (tg.shape[0], 1))
+    rays = [np.concatenate([o, tg - o], 1)]
+    ax = np.array([[0, 0, 5, 0, 0, -1], [5, 0, 0, -1, 0, 0], [0, 5, 0, 0, -1, 0], [2, 3, 0, 0, -1, 0],
+                   [0, 0, 0, 1, 0, 0], [0, 0, 0, 0, 1, 0], [0, -0.999, 0, 1, 0, 0], [2, 0, 0, 0, 0, 1]], np.float32)
+    rays.append(ax)
+    r = np.concatenate(rays).astype(np.float32)
+    ctx.upload(s)
+    g = ctx.trace_closest(r)
+    oo = oracle.OracleScene(s).trace_closest(r)
+    _hits_compare(g, oo, np.zeros(g.shape, bool))
+
+
+def _render_both(ctx, oracle, scene, cam, rp, s0, s1):
+    desc = abi.render_desc(s0, s1, rp["depth"], rp["seed"])
+    ctx.upload(scene)
+    ctx.set_camera(0, cam, rp["seed"])
+    ctx.reset(0)
+    ctx.render(0, desc)
+    g = [ctx.read(0, L) for L in (abi.LAYER_BEAUTY, abi.LAYER_DIFFUSE, abi.LAYER_WORLD_NORMAL)]
+    o = oracle.render(oracle.OracleScene(scene), oracle.OracleCamera(cam, rp["seed"]), desc)
+    return g, o
+
+
+def _frac_beyond(a, b, rtol=RTOL, atol=1e-6):
+    bad = ~np.isclose(a, b, rtol=rtol, atol=atol, equal_nan=True)
+    return bad.any(axis=0).mean()
+
+
+@pytest.mark.parametrize("name", ["config1", "cornell", "mixed"])
+def test_render_parity(ctx, oracle, name):
+    if name == "config1":
+        s, cam, rp = scenes.config1(spp=5)
+        cam.resolution[0] = cam.resolution[1] = 160
+    elif name == "cornell":
+        s, cam, rp = scenes.cornell(spp=5, res=128)
+    else:
+        s, cam, rp = scenes.mixed_scene(spp=5, res=128)
+    g, o = _render_both(ctx, oracle, s, cam, rp, 1, 5)
+    for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
+        f = _frac_beyond(g[k], o[k])
+        assert f < 5e-3, f"{name_}: {f:.4%} of pixels beyond {RTOL}"
+    # most pixels bit-exact or within float noise
+    assert _frac_beyond(g[0], o[0], rtol=1e-6) < 0.05
+
+
+def test_render_parity_c2_geometry(ctx, oracle):
+    """The 1M-triangle configuration's geometry and HDRI at reduced resolution (fewer pixels, same scene)."""
+    s, cam, rp = scenes.config2(spp=3)
+    cam.resolution[0], cam.resolution[1] = 90, 160
+    g, o = _render_both(ctx, oracle, s, cam, rp, 1, 3)
+    assert _frac_beyond(g[0], o[0]) < 5e-3
+    assert _frac_beyond(g[2], o[2]) < 5e-3
