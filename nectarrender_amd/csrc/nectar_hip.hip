@@ -246,7 +246,7 @@ __device__ __forceinline__ f3 mat_eval(const GScene& S, const GMaterial& m, f3 n
         float n_dot_h = fmaxf(dot(sn, h), 0.0f);
         float v_dot_h = fmaxf(dot(view, h), 0.0f);
         f3 F0 = s3(0.04f) * (1.0f - metallic) + albedo * metallic;
-        f3 F = F0 + (s3(1.0f) - F0) * powf(1.0f - v_dot_h, 5.0f);
+        f3 F = F0 + (s3(1.0f) - F0) * nrm::powf_(1.0f - v_dot_h, 5.0f);
         float D = ggx_D(n_dot_h, alpha);
         float G = smith_g1(n_dot_l, alpha) * smith_g1(n_dot_v, alpha);
         f3 spec = F * (D * G / fmaxf(4.0f * n_dot_v * n_dot_l, 1e-4f));
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(256) k_shade(GScene S, PathPtrs P, TraceArgs A
             if (!refl) {
                 float r0 = (1.0f - ri) / (1.0f + ri);
                 r0 = r0 * r0;
-                float refl_p = r0 + (1.0f - r0) * powf((1.0f - cos_theta), 5.0f);
+                float refl_p = r0 + (1.0f - r0) * nrm::powf_((1.0f - cos_theta), 5.0f);
                 refl = refl_p > g.uniform();
             }
             nd = refl ? reflect(unit, nn) : refract(unit, nn, ri);

@@ -1,13 +1,14 @@
 // nr_device.h — device arithmetic of the MI355X path tracer.
 //
-// Every routine keeps the reference's operation order so that, compiled with
-// -ffp-contract=off (IEEE +,-,*,/,sqrt on gfx950), results match the
-// reference's arithmetic bit for bit; only transcendentals (sin, cos, atan2,
-// acos, log, pow) come from the ROCm device library.  File:line citations are
-// relative to nectar_render/csrc/ of the reference.
+// Every routine keeps the reference's operation order; compiled with
+// -ffp-contract=off (IEEE +,-,*,/,sqrt on gfx950) and with the transcendentals
+// of nr_mathfn.h, results are bit-identical to the same arithmetic on the
+// host.  File:line citations are relative to nectar_render/csrc/ of the
+// reference.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "nr_mathfn.h"
 #include "nr_types.h"
 
 #define NR_EPS 1e-4f
@@ -191,7 +192,7 @@ NR_D f3 cosine_dir(Gen& g) {
     float r1 = g.rf();
     float r2 = g.rf();
     float phi = NR_PI2 * r1;
-    return mk3(cosf(phi) * sqrtf(r2), sinf(phi) * sqrtf(r2), sqrtf(1.0f - r2));
+    return mk3(nrm::cosf_(phi) * sqrtf(r2), nrm::sinf_(phi) * sqrtf(r2), sqrtf(1.0f - r2));
 }
 NR_D f3 unit_vector(Gen& g) {
     while (true) {
@@ -280,8 +281,8 @@ NR_D bool sphere_hit(const Ray& r, float tmin, float tmax, HRec& rec, float radi
     f3 nn = divv(rec.p, adds(ld3(x.scale) * radius, NR_FMIN));
     rec.n = nn;
     rec.tg = normalize(mk3(-rec.p.z, 0.0f, rec.p.x));
-    float theta = acosf(-nn.y);
-    float phi = atan2f(-nn.z, nn.x) + NR_PI;
+    float theta = nrm::acosf_(-nn.y);
+    float phi = nrm::atan2f_(-nn.z, nn.x) + NR_PI;
     rec.uv.x = phi / NR_PI2;
     rec.uv.y = theta / NR_PI;
     return true;
@@ -444,7 +445,7 @@ NR_D bool medium_hit(const GScene& S, const GObject& o, const Ray& r, float tmin
     if (r1.t < 0.0f) r1.t = 0.0f;
     float ray_length = vlength(r.d);
     float inside = (r2.t - r1.t) * ray_length;
-    float hit_distance = o.neg_inv_density * logf(pcg_ray(r));
+    float hit_distance = o.neg_inv_density * nrm::logf_(pcg_ray(r));
     if (hit_distance > inside) return false;
     rec.p = at(r, r1.t + hit_distance / ray_length);
     rec.n = mk3(1.0f, 0.0f, 0.0f);
@@ -536,8 +537,8 @@ NR_D f3 sky_sample(const GSky& sk, f3 d) {
     if (sk.type == 2) {
         if (!sk.data) return s3(0.0f);
         f3 dir = mv(sk.rot, normalize(d));
-        float u = atan2f(dir.z, dir.x) / NR_PI2 + 0.5f;
-        float v = acosf(fminf(fmaxf(dir.y, -1.0f), 1.0f)) / NR_PI;
+        float u = nrm::atan2f_(dir.z, dir.x) / NR_PI2 + 0.5f;
+        float v = nrm::acosf_(fminf(fmaxf(dir.y, -1.0f), 1.0f)) / NR_PI;
         int W = sk.W, H = sk.H;
         float fx = u * (float)W - 0.5f;
         float fy = v * (float)H - 0.5f;
@@ -588,7 +589,7 @@ NR_D f3 pbr_pdf_generate(const ONB& uvw, f3 view, float alpha, float sp, Gen& g)
         float cos_theta = sqrtf((1.0f - u1) / (1.0f + (alpha * alpha - 1.0f) * u1));
         float sin_theta = sqrtf(fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
         float phi = NR_PI2 * u2;
-        f3 h = uvw.xf(mk3(sin_theta * cosf(phi), sin_theta * sinf(phi), cos_theta));
+        f3 h = uvw.xf(mk3(sin_theta * nrm::cosf_(phi), sin_theta * nrm::sinf_(phi), cos_theta));
         return normalize(h * (2.0f * dot(view, h)) - view);
     }
     return uvw.xf(cosine_dir(g));
@@ -644,8 +645,8 @@ NR_D f3 sphere_light_random(const GObject& o, f3 origin, Gen& g) {            //
     float rad_sq = o.radius * o.radius;
     float z = 1.0f + r2 * (sqrtf(1.0f - rad_sq / dist2) - 1.0f);
     float phi = NR_PI2 * r1;
-    float x = cosf(phi) * sqrtf(1.0f - z * z);
-    float y = sinf(phi) * sqrtf(1.0f - z * z);
+    float x = nrm::cosf_(phi) * sqrtf(1.0f - z * z);
+    float y = nrm::sinf_(phi) * sqrtf(1.0f - z * z);
     return uvw.xf(mk3(x, y, z));
 }
 NR_D int light_geom(const GObject& L) {

@@ -26,6 +26,7 @@
  * --use_fast_math (CMakeLists.txt:110-112); this file uses glibc libm.
  */
 #include "nr_oracle.h"
+#include "../nectarrender_amd/csrc/nr_mathfn.h"
 
 #include <algorithm>
 #include <cmath>
@@ -465,8 +466,8 @@ static inline V3 random_cosine_direction(Gen& g) {                              
     float r1 = g.random_float();
     float r2 = g.random_float();
     float phi = PI2 * r1;
-    float x = cosf(phi) * sqrtf(r2);
-    float y = sinf(phi) * sqrtf(r2);
+    float x = nrm::cosf_(phi) * sqrtf(r2);
+    float y = nrm::sinf_(phi) * sqrtf(r2);
     float z = sqrtf(1.0f - r2);
     return {x, y, z};
 }
@@ -590,8 +591,8 @@ static bool sphere_hit(const Ray& r, Interval rt, HitRec& rec, float radius, con
     V3 norm = divv(rec.p, adds(muls(x.scale, radius), FMIN));
     rec.n = norm;
     rec.tangent = normalize(V3{-rec.p.z, 0.0f, rec.p.x});
-    float theta = acosf(-norm.y);                                                        // shapes.h:19-23
-    float phi = atan2f(-norm.z, norm.x) + PI;
+    float theta = nrm::acosf_(-norm.y);                                                        // shapes.h:19-23
+    float phi = nrm::atan2f_(-norm.z, norm.x) + PI;
     rec.uv = {phi / PI2, theta / PI};
     return true;
 }
@@ -696,7 +697,7 @@ static bool medium_hit(const Scene& S, const Object& o, const Ray& r, Interval r
     if (r1.t < 0.0f) r1.t = 0.0f;
     float ray_length = length(r.d);
     float inside = (r2.t - r1.t) * ray_length;
-    float hit_distance = o.neg_inv_density * logf(pcg_float_ray(r));
+    float hit_distance = o.neg_inv_density * nrm::logf_(pcg_float_ray(r));
     if (hit_distance > inside) return false;
     rec.p = at(r, r1.t + hit_distance / ray_length);
     rec.n = {1.0f, 0.0f, 0.0f};
@@ -808,8 +809,8 @@ static V3 sphere_random(const Object& o, V3 origin, Gen& g) {                   
     float rad_sq = o.radius * o.radius;
     float z = 1.0f + r2 * (sqrtf(1.0f - rad_sq / dist2) - 1.0f);
     float phi = PI2 * r1;
-    float x = cosf(phi) * sqrtf(1.0f - z * z);
-    float y = sinf(phi) * sqrtf(1.0f - z * z);
+    float x = nrm::cosf_(phi) * sqrtf(1.0f - z * z);
+    float y = nrm::sinf_(phi) * sqrtf(1.0f - z * z);
     return uvw.transform({x, y, z});
 }
 static float light_pdf_value(const Object& L, V3 origin, V3 dir) {
@@ -896,7 +897,7 @@ struct Pdf {
                     float cos_theta = sqrtf((1.0f - u1) / (1.0f + (alpha * alpha - 1.0f) * u1));
                     float sin_theta = sqrtf(fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
                     float phi = PI2 * u2;
-                    V3 h = uvw.transform({sin_theta * cosf(phi), sin_theta * sinf(phi), cos_theta});
+                    V3 h = uvw.transform({sin_theta * nrm::cosf_(phi), sin_theta * nrm::sinf_(phi), cos_theta});
                     return normalize(sub(muls(h, 2.0f * dot(view, h)), view));
                 }
                 return uvw.transform(random_cosine_direction(g));
@@ -957,7 +958,7 @@ static bool mat_scatter(const Scene& S, const Material& m, const HitRec& rec, co
             if (!refl) {
                 float r0 = (1.0f - ri) / (1.0f + ri);                                    // material.h:262-269
                 r0 = r0 * r0;
-                float refl_p = r0 + (1.0f - r0) * powf((1.0f - cos_theta), 5.0f);
+                float refl_p = r0 + (1.0f - r0) * nrm::powf_((1.0f - cos_theta), 5.0f);
                 refl = refl_p > g.uniform();
             }
             dir = refl ? reflect(unit, rec.n) : refract(unit, rec.n, ri);
@@ -994,7 +995,7 @@ static V3 mat_evaluate(const Scene& S, const Material& m, const HitRec& rec, V3 
             float n_dot_h = fmaxf(dot(n, h), 0.0f);
             float v_dot_h = fmaxf(dot(view, h), 0.0f);
             V3 F0 = add(muls(v3(0.04f), 1.0f - metallic), muls(albedo, metallic));
-            V3 F = add(F0, muls(sub(v3(1.0f), F0), powf(1.0f - v_dot_h, 5.0f)));
+            V3 F = add(F0, muls(sub(v3(1.0f), F0), nrm::powf_(1.0f - v_dot_h, 5.0f)));
             float D = ggx_D(n_dot_h, alpha);
             float G = smith_g1(n_dot_l, alpha) * smith_g1(n_dot_v, alpha);
             V3 spec = muls(F, D * G / fmaxf(4.0f * n_dot_v * n_dot_l, 1e-4f));
@@ -1034,8 +1035,8 @@ static V3 sky_sample(const Scene& S, const Ray& r) {
         if (S.hdri.empty()) return v3(0.0f);
         M3 rot = rot_y(deg2rad(sk.rotation));
         V3 dir = mv(rot, normalize(r.d));
-        float u = atan2f(dir.z, dir.x) / PI2 + 0.5f;
-        float v = acosf(fminf(fmaxf(dir.y, -1.0f), 1.0f)) / PI;
+        float u = nrm::atan2f_(dir.z, dir.x) / PI2 + 0.5f;
+        float v = nrm::acosf_(fminf(fmaxf(dir.y, -1.0f), 1.0f)) / PI;
         int W = sk.W, H = sk.H;
         float fx = u * (float)W - 0.5f;
         float fy = v * (float)H - 0.5f;
@@ -1552,6 +1553,21 @@ int nro_pdf_cosine_generate(const float* n, uint32_t seed, uint32_t subseq, uint
     V3 d = pdf_cosine({n[0], n[1], n[2]}).generate(g);
     out3[0] = d.x; out3[1] = d.y; out3[2] = d.z;
     return (int)(g.draw - offset);
+}
+
+// deterministic transcendentals shared with the HIP kernels (nr_mathfn.h):
+// which = 0 sin, 1 cos, 2 acos, 3 atan2(x, y), 4 log, 5 pow(x, y)
+void nro_mathfn(int which, const float* x, const float* y, float* out, int n) {
+    for (int i = 0; i < n; i++) {
+        switch (which) {
+            case 0: out[i] = nrm::sinf_(x[i]); break;
+            case 1: out[i] = nrm::cosf_(x[i]); break;
+            case 2: out[i] = nrm::acosf_(x[i]); break;
+            case 3: out[i] = nrm::atan2f_(x[i], y[i]); break;
+            case 4: out[i] = nrm::logf_(x[i]); break;
+            case 5: out[i] = nrm::powf_(x[i], y[i]); break;
+        }
+    }
 }
 
 }  // extern "C"

@@ -84,6 +84,9 @@ int nro_pdf_pbr_generate(const float* n3, const float* view3, float roughness, f
 float nro_pdf_cosine_value(const float* n3, const float* dir3);
 int nro_pdf_cosine_generate(const float* n3, uint32_t seed, uint32_t subseq, uint32_t offset, float* out3);
 
+/* deterministic transcendentals (nr_mathfn.h): 0 sin 1 cos 2 acos 3 atan2(x,y) 4 log 5 pow(x,y) */
+void nro_mathfn(int which, const float* x, const float* y, float* out, int n);
+
 #ifdef __cplusplus
 }
 #endif

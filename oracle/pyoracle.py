@@ -61,6 +61,7 @@ def lib():
         "nro_pdf_pbr_generate": (C.c_int, [fp, fp, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32, fp]),
         "nro_pdf_cosine_value": (C.c_float, [fp, fp]),
         "nro_pdf_cosine_generate": (C.c_int, [fp, C.c_uint32, C.c_uint32, C.c_uint32, fp]),
+        "nro_mathfn": (None, [C.c_int, fp, fp, fp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -202,3 +203,14 @@ def bvh_build(boxes: np.ndarray):
     k = lib().nro_bvh_build(_fp(b), n, nodes.ctypes.data_as(C.POINTER(C.c_int32)), _fp(nb),
                             order.ctypes.data_as(C.POINTER(C.c_int32)))
     return nodes[:k], nb[:k], order
+
+
+MATHFN = {"sin": 0, "cos": 1, "acos": 2, "atan2": 3, "log": 4, "pow": 5}
+
+
+def mathfn(name, x, y=None):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, np.float32)
+    out = np.empty_like(x)
+    lib().nro_mathfn(MATHFN[name], _fp(x), _fp(y), _fp(out), x.size)
+    return out

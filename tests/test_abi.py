@@ -1,0 +1,60 @@
+"""The C-ABI library loads and exports every entry point include/nectar_hip.h declares (CPU only)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "nectar_hip.h")
+LIB = os.path.join(ROOT, "nectarrender_amd", "libnectar_hip.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nr_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    for n in ("nr_create", "nr_destroy", "nr_scene_upload", "nr_camera_set", "nr_render", "nr_trace_closest",
+              "nr_aov_read", "nr_post", "nr_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(LIB), "run __graft_entry__.build() first"
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (\w+)$", out, flags=re.M))
+    missing = [n for n in declared() if n not in exported]
+    assert not missing, missing
+
+
+def test_library_loads_and_reports_version():
+    L = C.CDLL(LIB)
+    assert L.nr_abi_version() == 1
+    L.nr_last_error.restype = C.c_char_p
+    assert isinstance(L.nr_last_error(), bytes)
+
+
+def test_null_arguments_fail_cleanly_without_device():
+    """Error paths reached before any HIP call: status codes + message, no abort."""
+    from nectarrender_amd import abi, native
+    L = native.lib()
+    assert L.nr_scene_upload(None, None) == 1
+    assert b"null" in L.nr_last_error()
+    assert L.nr_render(None, 0, None) == 1
+    assert L.nr_trace_closest(None, None, 0, None) == 1
+    assert L.nr_post(None, 0, None, None, None) == 1
+    assert L.nr_set_timing(None, 1) == 1
+    assert L.nr_sync(None) == 1
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is visible")
+def test_create_without_gpu_returns_null_with_message():
+    from nectarrender_amd import native
+    L = native.lib()
+    assert not L.nr_create(0)
+    assert L.nr_last_error()
