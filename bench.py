@@ -1,0 +1,123 @@
+"""Benchmark: Mpaths/s of the path-tracing sample pass on one MI355X.
+
+Workload: 1920x1080 view (resolution (1080, 1920) in the reference's (H, W)
+order) of a 1,003,520-triangle displaced icosphere (PBR metal, roughness 0.4)
+on a Lambertian quad under a procedural 2048x1024 fp32 HDRI with a 1e4-radiance
+0.5 degree sun; depth 12, Russian roulette from bounce 3, seed 0; synthetic,
+seeded scene data.  A step is one sample pass over all 2,073,600 pixels
+(one path per pixel); K steps render sample passes 1..K, the 64-spp image is
+K = 63 (the reference renders passes 1..n-1, engine.cpp:43).
+
+Output: one JSON line with the contract fields plus
+  roofline      traversal (k_extend) bytes / traversal time against HBM peak,
+                with the algorithmic byte model counted by the oracle
+  cpu_baseline  the CPU oracle (same estimator) on this host's cores, on
+                full 1080p passes for ~10 s
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpaths/s @1080p; secs to RMSE≤0.01 vs 4096spp ref (plain vs denoised)"
+W, H = 1920, 1080
+PATHS_PER_STEP = W * H
+# Traversal work per path of this workload, counted by oracle/traversal_work.py
+# on every 61st pixel over sample passes 1..4 (135,976 paths): 1.7945 segments,
+# 23.873 interior nodes (64 B), 1.8217 triangle tests (48 B), 1.9960 top-level
+# objects (128 B), 40 B ray/hit per segment.
+TRAVERSAL_BYTES_PER_PATH = 1942.58
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(budget_s: float = 10.0, max_passes: int = 8):
+    """The oracle's render pass over the same workload, timed on the host cores."""
+    from nectarrender_amd import abi, scenes
+    from oracle import pyoracle as po
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    s, cam, rp = scenes.config2(spp=64)
+    os_ = po.OracleScene(s)
+    oc = po.OracleCamera(cam, rp["seed"])
+    d = abi.render_desc(1, 64, rp["depth"], rp["seed"])
+    t0 = time.perf_counter()
+    n = 0
+    while n < max_passes and (time.perf_counter() - t0) < budget_s:
+        po.render_pass(os_, oc, d, 1 + n, threads=threads)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": round(n * PATHS_PER_STEP / dt / 1e6, 3), "unit": "Mpaths/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} full 1920x1080 sample passes ({n * PATHS_PER_STEP} paths) of the same scene, "
+                      f"{dt:.1f} s, OpenMP over 16x16 tiles"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=63)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
+              f"tiles/samples with no exchange beyond a final sum, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    K, Wm = args.steps, args.warmup
+
+    from nectarrender_amd import abi, native, scenes
+
+    scene, cam, rp = scenes.config2(spp=max(64, K + 1, Wm + 1))
+    ctx = native.Context(0)
+    ctx.upload(scene)
+    ctx.set_camera(0, cam, rp["seed"])
+    # warmup passes, then a clean accumulation for the timed passes
+    if Wm > 0:
+        ctx.render(0, abi.render_desc(1, 1 + Wm, rp["depth"], rp["seed"]))
+    ctx.sync()
+    ctx.reset(0)
+    ctx.set_timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.render(0, abi.render_desc(1, 1 + K, rp["depth"], rp["seed"]))
+    ctx.sync()
+    t1 = time.perf_counter()
+    tm = ctx.timings()       # [total, traversal, shading, generate+accumulate] ms, HIP events
+    ctx.set_timing(False)
+    img = ctx.read(0)
+    ctx.close()
+    dt = t1 - t0
+    paths = K * PATHS_PER_STEP
+    value = paths / dt / 1e6
+    n_extend = K * rp["depth"]
+    trav_s = tm[1] / 1e3
+    achieved = TRAVERSAL_BYTES_PER_PATH * paths / trav_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_extend", "launches": n_extend, "avg_launch_ms": round(tm[1] / n_extend, 4),
+            "bytes_per_launch": round(TRAVERSAL_BYTES_PER_PATH * paths / n_extend)}
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": K,
+        "warmup": Wm, "ms_per_step": round(dt / K * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "1920x1080 (H=1080, W=1920), 1,003,520-triangle displaced icosphere PBR metal "
+                               "rough 0.4 on Lambertian quad, 2048x1024 fp32 HDRI with 1e4 sun, depth 12, "
+                               "parity estimator, one path per pixel per step",
+                   "spp_passes": K, "ray_depth": rp["depth"], "paths_per_step": PATHS_PER_STEP},
+        "gpu_ms": {"total": round(tm[0], 3), "traversal": round(tm[1], 3), "shading": round(tm[2], 3),
+                   "generate_accumulate": round(tm[3], 3)},
+        "image_mean": float(img.mean()),
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
