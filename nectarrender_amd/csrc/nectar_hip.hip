@@ -539,7 +539,8 @@ struct nr_ctx {
     int32_t* d_err = nullptr;
     // timing
     bool timing = false;
-    std::vector<hipEvent_t> ev;
+    std::vector<hipEvent_t> ev;             // per-launch marks
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     float last_ms[8] = {0};
     int n_last = 0;
 };
@@ -578,6 +579,12 @@ nr_ctx* nr_create(int device) {
         delete c;
         return nullptr;
     }
+    if (hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+        g_err = "hipEventCreate failed";
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return nullptr;
+    }
     if (hipMalloc((void**)&c->d_err, sizeof(int32_t) * 4) != hipSuccess) {
         g_err = "hipMalloc failed";
         (void)hipStreamDestroy(c->stream);
@@ -599,6 +606,8 @@ void nr_destroy(nr_ctx* c) {
         for (auto* l : v.layer) if (l) (void)hipFree(l);
     }
     for (auto e : c->ev) (void)hipEventDestroy(e);
+    if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
+    if (c->ev_end) (void)hipEventDestroy(c->ev_end);
     if (c->d_err) (void)hipFree(c->d_err);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -907,8 +916,8 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
         marks.push_back({ei, cat});
         ei++;
     };
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->timing) { e0 = ev_at(c, 1000); e1 = ev_at(c, 1001); (void)hipEventRecord(e0, c->stream); }
+    hipEvent_t e0 = c->ev_begin, e1 = c->ev_end;
+    if (c->timing) (void)hipEventRecord(e0, c->stream);
     for (uint32_t s = d->sample_begin; s < d->sample_end; s++) {
         uint32_t sWH = (uint32_t)((uint64_t)s * (uint64_t)v.W * (uint64_t)v.H);
         mark(2);
@@ -983,8 +992,8 @@ int nr_trace_closest_device(nr_ctx* c, const float* d_rays, int32_t n, nr_hit* d
     const int TB = 128;
     size_t lds = stack_lds_bytes(c->scene.stack_depth, TB);
     if (lds > 160 * 1024) return fail(NR_ERR_LIMIT, "BVH too deep for the LDS traversal stack");
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->timing) { e0 = ev_at(c, 1000); e1 = ev_at(c, 1001); (void)hipEventRecord(e0, c->stream); }
+    hipEvent_t e0 = c->ev_begin, e1 = c->ev_end;
+    if (c->timing) (void)hipEventRecord(e0, c->stream);
     hipLaunchKernelGGL(k_trace_closest, dim3((n + TB - 1) / TB), dim3(TB), lds, c->stream, c->scene, d_rays, n, d_hits, c->d_err);
     HIPCK(hipGetLastError());
     if (c->timing) {
