@@ -47,7 +47,15 @@ static int fail(int code, const std::string& msg) { g_err = msg; return code; }
     } while (0)
 
 // counters layout
-enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_N = 16 };
+// counters (one int each): live queues [0,2), material queues [2,8), error flag,
+// per-XCD chunk cursors of the persistent traversal kernels [16,24)
+enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_FETCH = 16, CNT_N = 32 };
+constexpr int LDS_STACK_LEVELS = 16;   // traversal stack levels kept in LDS
+
+struct StackCfg {
+    int2* spill;            // levels >= LDS_STACK_LEVELS, one column per persistent thread
+    size_t stride;          // persistent threads in the grid
+};
 
 struct PathPtrs {
     float4* o;
@@ -89,6 +97,37 @@ __device__ __forceinline__ int wave_append(bool pred, int32_t* counter) {
     return pred ? base + __popcll(below) : -1;
 }
 
+// Persistent traversal kernels take work in chunks of 64 items (one per lane).
+// Chunks are XCD-affine: the workgroups of XCD x (blockIdx % 8 under the
+// round-robin dispatch of workgroups to the 8 XCDs) first drain their own
+// contiguous eighth of the chunks, so neighbouring rays share an L2, then steal
+// from the next XCD's range.  Wave-uniform; -1 when all work is taken.
+__device__ __forceinline__ int fetch_chunk(int32_t* cursors, int n_chunks, int& x, int& tried) {
+    while (tried < 8) {
+        int lo = (int)(((int64_t)x * n_chunks) >> 3), hi = (int)(((int64_t)(x + 1) * n_chunks) >> 3);
+        int c = 0;
+        if (__lane_id() == 0) c = atomicAdd(&cursors[x], 1);
+        c = __shfl(c, 0);
+        if (lo + c < hi) return lo + c;
+        x = (x + 1) & 7;
+        tried++;
+    }
+    return -1;
+}
+
+__device__ __forceinline__ Stack make_stack(int2* smem, const GScene& S, const StackCfg& SC) {
+    int tid = threadIdx.x;
+    Stack st;
+    st.lds = smem + (tid >> 6) * LDS_STACK_LEVELS * 64;
+    st.spill = SC.spill;
+    st.lane = tid & 63;
+    st.lds_cap = LDS_STACK_LEVELS;
+    st.cap = S.stack_depth;
+    st.slot = (size_t)blockIdx.x * blockDim.x + tid;
+    st.spill_stride = SC.stride;
+    return st;
+}
+
 // ---------------------------------------------------------------------------
 // camera (engine/include/engine/camera.h:186-278)
 // ---------------------------------------------------------------------------
@@ -125,105 +164,130 @@ __device__ __forceinline__ Ray camera_ray(const GCamera& c, uint32_t x, uint32_t
 // kernels
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(128) k_trace_closest(GScene S, const float* __restrict__ rays, int n,
-                                                       nr_hit* __restrict__ hits, int32_t* __restrict__ err) {
+                                                       nr_hit* __restrict__ hits, int32_t* __restrict__ counts,
+                                                       StackCfg SC) {
     extern __shared__ int2 smem[];
-    int tid = threadIdx.x;
-    Stack st{smem + (tid >> 6) * S.stack_depth * 64, tid & 63, S.stack_depth};
-    int i = blockIdx.x * blockDim.x + tid;
-    if (i >= n) return;
-    const float* q = rays + 6 * (size_t)i;
-    Ray r = mkray(mk3(q[0], q[1], q[2]), mk3(q[3], q[4], q[5]));
-    HRec rec;
-    hrec_clear(rec);
-    bool of = false;
-    int oi = scene_closest(S, r, rec, st, of);
-    if (of) atomicOr(err, 1);
-    nr_hit h;
-    if (oi < 0) {
-        memset(&h, 0, sizeof(h));
-        h.object_id = -1;
-        h.prim = -1;
-    } else {
-        const GObject& o = S.objects[oi];
-        h.object_id = o.object_id;
-        h.material_id = o.material;
-        h.prim = rec.prim;
-        h.front_face = rec.front ? 1 : 0;
-        h.t = rec.t;
-        h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
-        h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
-        h.tangent[0] = rec.tg.x; h.tangent[1] = rec.tg.y; h.tangent[2] = rec.tg.z;
-        h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
-    }
-    hits[i] = h;
-}
-
-__global__ void __launch_bounds__(256) k_generate(GCamera cam, PathPtrs P, uint32_t n, uint32_t s, uint32_t seed,
-                                                  uint32_t sWH) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) P.counts[CNT_ACT + 0] = (int32_t)n;
-    if (i >= n) return;
-    uint32_t x = i % cam.W, y = i / cam.W;
-    Gen g{seed, i + sWH, 0u};
-    Ray r = camera_ray(cam, x, y, i, s, seed, g);
-    P.o[i] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
-    P.d[i] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
-    P.atten[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.diff[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.draw[i] = g.draw;
-    P.qact[i] = (int32_t)i;
-}
-
-__global__ void k_counts(int32_t* counts, uint32_t bounce) {
-    int t = threadIdx.x;
-    if (t < NR_MAT_TYPES) counts[CNT_MAT + t] = 0;
-    if (t == 0) counts[CNT_ACT + ((bounce + 1) & 1)] = 0;
-}
-
-__global__ void __launch_bounds__(128) k_extend(GScene S, PathPtrs P, TraceArgs A) {
-    extern __shared__ int2 smem[];
-    int tid = threadIdx.x;
-    Stack st{smem + (tid >> 6) * S.stack_depth * 64, tid & 63, S.stack_depth};
-    const int in = A.bounce & 1;
-    const int n = P.counts[CNT_ACT + in];
-    int i = blockIdx.x * blockDim.x + tid;
-    int path = -1;
-    int mtype = 0;
-    if (i < n) {
-        path = P.qact[(size_t)in * P.cap + i];
-        float4 o4 = P.o[path], d4 = P.d[path];
-        Ray r = mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
+    Stack st = make_stack(smem, S, SC);
+    const int n_chunks = (n + 63) >> 6;
+    int x = blockIdx.x & 7, tried = 0;
+    while (true) {
+        int chunk = fetch_chunk(counts + CNT_FETCH, n_chunks, x, tried);
+        if (chunk < 0) break;
+        int i = chunk * 64 + st.lane;
+        if (i >= n) continue;
+        const float* q = rays + 6 * (size_t)i;
+        Ray r = mkray(mk3(q[0], q[1], q[2]), mk3(q[3], q[4], q[5]));
         HRec rec;
         hrec_clear(rec);
         bool of = false;
         int oi = scene_closest(S, r, rec, st, of);
-        if (of) atomicOr(&P.counts[CNT_ERR], 1);
+        if (of) atomicOr(&counts[CNT_ERR], 1);
+        nr_hit h;
         if (oi < 0) {
-            float4 a4 = P.atten[path], l4 = P.L[path];
-            f3 L = mk3(l4.x, l4.y, l4.z) + mk3(a4.x, a4.y, a4.z) * sky_sample(S.sky, r.d);
-            P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
+            memset(&h, 0, sizeof(h));
+            h.object_id = -1;
+            h.prim = -1;
         } else {
-            const GObject& ob = S.objects[oi];
-            GHit h;
+            const GObject& o = S.objects[oi];
+            h.object_id = o.object_id;
+            h.material_id = o.material;
+            h.prim = rec.prim;
+            h.front_face = rec.front ? 1 : 0;
+            h.t = rec.t;
             h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
             h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
-            h.tg[0] = rec.tg.x; h.tg[1] = rec.tg.y; h.tg[2] = rec.tg.z;
+            h.tangent[0] = rec.tg.x; h.tangent[1] = rec.tg.y; h.tangent[2] = rec.tg.z;
             h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
-            h.material = ob.material;
-            h.front_face = rec.front ? 1 : 0;
-            h.object_id = ob.object_id;
-            h.prim = rec.prim;
-            h.t = rec.t;
-            P.hit[path] = h;
-            mtype = ob.pad[0];
         }
+        hits[i] = h;
     }
-#pragma unroll
-    for (int t = 1; t < NR_MAT_TYPES; t++) {
-        int slot = wave_append(mtype == t, &P.counts[CNT_MAT + t]);
-        if (mtype == t) P.qmat[(size_t)t * P.cap + slot] = path;
+}
+
+// Primary rays in 8x8 pixel tiles (one tile per wave) so a wave's rays are
+// coherent; the live queue is filled by wave compaction, counts[CNT_ACT] must
+// be zero on entry.
+__global__ void __launch_bounds__(256) k_generate(GCamera cam, PathPtrs P, uint32_t s, uint32_t seed, uint32_t sWH) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t tiles_x = (cam.W + 7) >> 3;
+    uint32_t tile = t >> 6, within = t & 63;
+    uint32_t x = (tile % tiles_x) * 8 + (within & 7), y = (tile / tiles_x) * 8 + (within >> 3);
+    bool valid = x < cam.W && y < cam.H;
+    uint32_t i = y * cam.W + x;
+    if (valid) {
+        Gen g{seed, i + sWH, 0u};
+        Ray r = camera_ray(cam, x, y, i, s, seed, g);
+        P.o[i] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
+        P.d[i] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
+        P.atten[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.diff[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.draw[i] = g.draw;
+    }
+    int slot = wave_append(valid, &P.counts[CNT_ACT + 0]);
+    if (valid) P.qact[slot] = (int32_t)i;
+}
+
+// Clears the counters a bounce appends to (material queues, next live queue)
+// and the chunk cursors of its traversal; bounce = -1 clears the first live
+// queue before k_generate.
+__global__ void k_counts(int32_t* counts, int32_t bounce) {
+    int t = threadIdx.x;
+    if (bounce < 0) {
+        if (t == 0) counts[CNT_ACT + 0] = 0;
+        return;
+    }
+    if (t < NR_MAT_TYPES) counts[CNT_MAT + t] = 0;
+    if (t < 8) counts[CNT_FETCH + t] = 0;
+    if (t == 0) counts[CNT_ACT + ((bounce + 1) & 1)] = 0;
+}
+
+__global__ void __launch_bounds__(128) k_extend(GScene S, PathPtrs P, TraceArgs A, StackCfg SC) {
+    extern __shared__ int2 smem[];
+    Stack st = make_stack(smem, S, SC);
+    const int in = A.bounce & 1;
+    const int n = P.counts[CNT_ACT + in];
+    const int n_chunks = (n + 63) >> 6;
+    int x = blockIdx.x & 7, tried = 0;
+    while (true) {
+        int chunk = fetch_chunk(P.counts + CNT_FETCH, n_chunks, x, tried);
+        if (chunk < 0) break;
+        int i = chunk * 64 + st.lane;
+        int path = -1;
+        int mtype = 0;
+        if (i < n) {
+            path = P.qact[(size_t)in * P.cap + i];
+            float4 o4 = P.o[path], d4 = P.d[path];
+            Ray r = mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
+            HRec rec;
+            hrec_clear(rec);
+            bool of = false;
+            int oi = scene_closest(S, r, rec, st, of);
+            if (of) atomicOr(&P.counts[CNT_ERR], 1);
+            if (oi < 0) {
+                float4 a4 = P.atten[path], l4 = P.L[path];
+                f3 L = mk3(l4.x, l4.y, l4.z) + mk3(a4.x, a4.y, a4.z) * sky_sample(S.sky, r.d);
+                P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
+            } else {
+                const GObject& ob = S.objects[oi];
+                GHit h;
+                h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
+                h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
+                h.tg[0] = rec.tg.x; h.tg[1] = rec.tg.y; h.tg[2] = rec.tg.z;
+                h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
+                h.material = ob.material;
+                h.front_face = rec.front ? 1 : 0;
+                h.object_id = ob.object_id;
+                h.prim = rec.prim;
+                h.t = rec.t;
+                P.hit[path] = h;
+                mtype = ob.pad[0];
+            }
+        }
+        for (int t = 1; t < NR_MAT_TYPES; t++) {
+            int slot = wave_append(mtype == t, &P.counts[CNT_MAT + t]);
+            if (mtype == t) P.qmat[(size_t)t * P.cap + slot] = path;
+        }
     }
 }
 
@@ -261,7 +325,9 @@ template <int M>
 __global__ void __launch_bounds__(256) k_shade(GScene S, PathPtrs P, TraceArgs A) {
     const int n = P.counts[CNT_MAT + M];
     const int out = (A.bounce + 1) & 1;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
+    int i = base + (threadIdx.x & 63);
     bool cont = false;
     int path = -1;
     if (i < n) {
@@ -367,6 +433,7 @@ __global__ void __launch_bounds__(256) k_shade(GScene S, PathPtrs P, TraceArgs A
     }
     int slot = wave_append(cont, &P.counts[CNT_ACT + out]);
     if (cont) P.qact[(size_t)out * P.cap + slot] = path;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_accumulate(PathPtrs P, float* beauty, float* diffuse, float* normal,
@@ -536,7 +603,13 @@ struct nr_ctx {
     PathPtrs P{};
     std::vector<void*> path_mem;
     uint32_t path_cap = 0;
-    int32_t* d_err = nullptr;
+    int32_t* d_err = nullptr;           // counters of nr_trace_closest (CNT_N ints)
+    // persistent launch shapes
+    int n_cu = 0;
+    int trav_grid = 0;                  // workgroups of 128 for k_extend / k_trace_closest
+    int shade_grid = 0;                 // workgroups of 256 for k_shade
+    int2* spill = nullptr;              // traversal stack levels beyond LDS
+    size_t spill_levels = 0;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;             // per-launch marks
@@ -560,7 +633,20 @@ static int upload(nr_ctx* c, const T* host, size_t n, T** dev) {
     return NR_OK;
 }
 
-static size_t stack_lds_bytes(int depth, int block) { return (size_t)depth * 64 * sizeof(int2) * (block / 64); }
+static size_t trav_lds_bytes() { return (size_t)LDS_STACK_LEVELS * 64 * sizeof(int2) * (128 / 64); }
+
+// spill columns for stack levels beyond LDS, one per persistent traversal thread
+static int ensure_spill(nr_ctx* c, int stack_depth) {
+    size_t levels = stack_depth > LDS_STACK_LEVELS ? (size_t)(stack_depth - LDS_STACK_LEVELS) : 0;
+    if (levels <= c->spill_levels && (c->spill || levels == 0)) return NR_OK;
+    if (c->spill) HIPCK(hipFree(c->spill));
+    c->spill = nullptr;
+    c->spill_levels = 0;
+    if (levels == 0) return NR_OK;
+    HIPCK(hipMalloc((void**)&c->spill, levels * (size_t)c->trav_grid * 128 * sizeof(int2)));
+    c->spill_levels = levels;
+    return NR_OK;
+}
 
 extern "C" {
 
@@ -585,13 +671,24 @@ nr_ctx* nr_create(int device) {
         delete c;
         return nullptr;
     }
-    if (hipMalloc((void**)&c->d_err, sizeof(int32_t) * 4) != hipSuccess) {
+    if (hipMalloc((void**)&c->d_err, sizeof(int32_t) * CNT_N) != hipSuccess) {
         g_err = "hipMalloc failed";
         (void)hipStreamDestroy(c->stream);
         delete c;
         return nullptr;
     }
-    (void)hipMemsetAsync(c->d_err, 0, sizeof(int32_t) * 4, c->stream);
+    (void)hipMemsetAsync(c->d_err, 0, sizeof(int32_t) * CNT_N, c->stream);
+    hipDeviceProp_t prop;
+    int nb = 0;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend, 128, trav_lds_bytes()) != hipSuccess || nb < 1) {
+        g_err = "device query failed";
+        nr_destroy(c);
+        return nullptr;
+    }
+    c->n_cu = prop.multiProcessorCount;
+    c->trav_grid = ((nb * c->n_cu + 7) / 8) * 8;
+    c->shade_grid = c->n_cu * 8;
     return c;
 }
 
@@ -609,6 +706,7 @@ void nr_destroy(nr_ctx* c) {
     if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
     if (c->ev_end) (void)hipEventDestroy(c->ev_end);
     if (c->d_err) (void)hipFree(c->d_err);
+    if (c->spill) (void)hipFree(c->spill);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -754,6 +852,10 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
     }
     S.n_objects = n;
     S.stack_depth = T.depth + max_mesh_depth + 4;
+    {
+        int e = ensure_spill(c, S.stack_depth);
+        if (e) return e;
+    }
     {
         GNode* dn; GObject* dob; GMesh* dm; GMaterial* dmat; GImage* dimg;
         int e = upload(c, tnodes.data(), tnodes.size(), &dn);
@@ -902,10 +1004,10 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     int e = ensure_paths(c, N);
     if (e) return e;
     const GScene& S = c->scene;
-    const int TB = 128;
-    size_t lds = stack_lds_bytes(S.stack_depth, TB);
-    if (lds > 160 * 1024) return fail(NR_ERR_LIMIT, "BVH too deep for the LDS traversal stack");
-    dim3 gN((N + 255) / 256), gT((N + TB - 1) / TB);
+    StackCfg SC{c->spill, (size_t)c->trav_grid * 128};
+    const size_t lds = trav_lds_bytes();
+    const uint32_t gen_threads = ((v.W + 7) / 8) * ((v.H + 7) / 8) * 64;
+    dim3 gGen((gen_threads + 255) / 256), gN((N + 255) / 256);
     size_t ei = 0;
     float t_ext = 0.0f, t_shade = 0.0f, t_other = 0.0f;
     std::vector<std::pair<size_t, int>> marks;   // (event index of start, category)
@@ -921,18 +1023,20 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     for (uint32_t s = d->sample_begin; s < d->sample_end; s++) {
         uint32_t sWH = (uint32_t)((uint64_t)s * (uint64_t)v.W * (uint64_t)v.H);
         mark(2);
-        hipLaunchKernelGGL(k_generate, gN, dim3(256), 0, c->stream, v.cam, c->P, N, s, d->seed, sWH);
+        hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->P.counts, -1);
+        hipLaunchKernelGGL(k_generate, gGen, dim3(256), 0, c->stream, v.cam, c->P, s, d->seed, sWH);
         for (uint32_t b = 0; b < d->ray_depth; b++) {
             TraceArgs A{d->seed, sWH, b, d->ray_depth, d->do_rr, d->rr_start_bounce, d->rr_survival};
-            hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->P.counts, b);
+            hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->P.counts, (int32_t)b);
             mark(0);
-            hipLaunchKernelGGL(k_extend, gT, dim3(TB), lds, c->stream, S, c->P, A);
+            hipLaunchKernelGGL(k_extend, dim3(c->trav_grid), dim3(128), lds, c->stream, S, c->P, A, SC);
             mark(1);
-            if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL(k_shade<NR_MAT_LAMBERTIAN>, gN, dim3(256), 0, c->stream, S, c->P, A);
-            if (c->mat_present[NR_MAT_PBR]) hipLaunchKernelGGL(k_shade<NR_MAT_PBR>, gN, dim3(256), 0, c->stream, S, c->P, A);
-            if (c->mat_present[NR_MAT_DIELECTRIC]) hipLaunchKernelGGL(k_shade<NR_MAT_DIELECTRIC>, gN, dim3(256), 0, c->stream, S, c->P, A);
-            if (c->mat_present[NR_MAT_EMISSIVE]) hipLaunchKernelGGL(k_shade<NR_MAT_EMISSIVE>, gN, dim3(256), 0, c->stream, S, c->P, A);
-            if (c->mat_present[NR_MAT_ISOTROPIC]) hipLaunchKernelGGL(k_shade<NR_MAT_ISOTROPIC>, gN, dim3(256), 0, c->stream, S, c->P, A);
+            dim3 gS(c->shade_grid);
+            if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL(k_shade<NR_MAT_LAMBERTIAN>, gS, dim3(256), 0, c->stream, S, c->P, A);
+            if (c->mat_present[NR_MAT_PBR]) hipLaunchKernelGGL(k_shade<NR_MAT_PBR>, gS, dim3(256), 0, c->stream, S, c->P, A);
+            if (c->mat_present[NR_MAT_DIELECTRIC]) hipLaunchKernelGGL(k_shade<NR_MAT_DIELECTRIC>, gS, dim3(256), 0, c->stream, S, c->P, A);
+            if (c->mat_present[NR_MAT_EMISSIVE]) hipLaunchKernelGGL(k_shade<NR_MAT_EMISSIVE>, gS, dim3(256), 0, c->stream, S, c->P, A);
+            if (c->mat_present[NR_MAT_ISOTROPIC]) hipLaunchKernelGGL(k_shade<NR_MAT_ISOTROPIC>, gS, dim3(256), 0, c->stream, S, c->P, A);
         }
         mark(2);
         hipLaunchKernelGGL(k_accumulate, gN, dim3(256), 0, c->stream, c->P, v.layer[NR_LAYER_BEAUTY],
@@ -989,12 +1093,12 @@ int nr_trace_closest_device(nr_ctx* c, const float* d_rays, int32_t n, nr_hit* d
     if (n < 0) return fail(NR_ERR_INVALID, "negative ray count");
     if (n == 0) return NR_OK;
     HIPCK(hipSetDevice(c->device));
-    const int TB = 128;
-    size_t lds = stack_lds_bytes(c->scene.stack_depth, TB);
-    if (lds > 160 * 1024) return fail(NR_ERR_LIMIT, "BVH too deep for the LDS traversal stack");
+    StackCfg SC{c->spill, (size_t)c->trav_grid * 128};
     hipEvent_t e0 = c->ev_begin, e1 = c->ev_end;
     if (c->timing) (void)hipEventRecord(e0, c->stream);
-    hipLaunchKernelGGL(k_trace_closest, dim3((n + TB - 1) / TB), dim3(TB), lds, c->stream, c->scene, d_rays, n, d_hits, c->d_err);
+    hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->d_err, 0);
+    hipLaunchKernelGGL(k_trace_closest, dim3(c->trav_grid), dim3(128), trav_lds_bytes(), c->stream, c->scene, d_rays, n,
+                       d_hits, c->d_err, SC);
     HIPCK(hipGetLastError());
     if (c->timing) {
         (void)hipEventRecord(e1, c->stream);
@@ -1019,11 +1123,11 @@ int nr_trace_closest(nr_ctx* c, const float* rays, int32_t n, nr_hit* hits) {
     int rc = NR_OK;
     int32_t zero = 0, err = 0;
     if (hipMemcpyAsync(dr, rays, sizeof(float) * 6 * (size_t)n, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        hipMemcpyAsync(c->d_err, &zero, sizeof(int32_t), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        hipMemcpyAsync(c->d_err + CNT_ERR, &zero, sizeof(int32_t), hipMemcpyHostToDevice, c->stream) != hipSuccess)
         rc = fail(NR_ERR_HIP, "copy rays");
     if (rc == NR_OK) rc = nr_trace_closest_device(c, dr, n, dh);
     if (rc == NR_OK && (hipMemcpyAsync(hits, dh, sizeof(nr_hit) * (size_t)n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-                        hipMemcpyAsync(&err, c->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                        hipMemcpyAsync(&err, c->d_err + CNT_ERR, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                         hipStreamSynchronize(c->stream) != hipSuccess))
         rc = fail(NR_ERR_HIP, "copy hits");
     (void)hipFree(dr);

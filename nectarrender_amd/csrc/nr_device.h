@@ -325,19 +325,28 @@ NR_D bool tri_hit(const Ray& r, f3 v0, f3 v1, f3 v2, float tmin, float tmax, flo
     return true;
 }
 
-// LDS traversal stack: one column per lane, entries (ref, entry t)
+// Traversal stack: the first lds_cap levels live in LDS (one column per lane,
+// conflict-free), deeper levels spill to a per-thread column of global memory.
+// Entries are (ref, entry t).  cap bounds the total depth.
 struct Stack {
-    int2* base;     // base[level * 64 + lane]
+    int2* lds;          // lds[level * 64 + lane]
+    int2* spill;        // spill[(level - lds_cap) * spill_stride + slot]
     int lane;
+    int lds_cap;
     int cap;
+    size_t slot;
+    size_t spill_stride;
     NR_D void push(int& sp, int32_t ref, float t) {
-        if (sp < cap) { base[sp * 64 + lane] = make_int2(ref, __float_as_int(t)); }
+        int2 e = make_int2(ref, __float_as_int(t));
+        if (sp < lds_cap) lds[sp * 64 + lane] = e;
+        else if (sp < cap) spill[(size_t)(sp - lds_cap) * spill_stride + slot] = e;
         sp++;
     }
     NR_D void pop(int& sp, int32_t& ref, float& t) {
         sp--;
-        int2 e = base[sp * 64 + lane];
-        ref = e.x; t = __int_as_float(e.y);
+        int2 e = sp < lds_cap ? lds[sp * 64 + lane] : spill[(size_t)(sp - lds_cap) * spill_stride + slot];
+        ref = e.x;
+        t = __int_as_float(e.y);
     }
 };
 
