@@ -50,7 +50,19 @@ static int fail(int code, const std::string& msg) { g_err = msg; return code; }
 // counters (one int each): live queues [0,2), material queues [2,8), error flag,
 // per-XCD chunk cursors of the persistent traversal kernels [16,24)
 enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_FETCH = 16, CNT_N = 32 };
-constexpr int LDS_STACK_LEVELS = 16;   // traversal stack levels kept in LDS
+#ifndef NR_LDS_LEVELS
+#define NR_LDS_LEVELS 16
+#endif
+constexpr int LDS_STACK_LEVELS = NR_LDS_LEVELS;   // traversal stack levels kept in LDS
+// register budget of the traversal kernels: minimum waves per SIMD (0 = compiler's choice)
+#ifndef NR_TRAV_OCC
+#define NR_TRAV_OCC 0
+#endif
+#if NR_TRAV_OCC > 0
+#define NR_TRAV_ATTR __attribute__((amdgpu_waves_per_eu(NR_TRAV_OCC, 8)))
+#else
+#define NR_TRAV_ATTR
+#endif
 
 struct StackCfg {
     int2* spill;            // levels >= LDS_STACK_LEVELS, one column per persistent thread
@@ -105,8 +117,12 @@ __device__ __forceinline__ int wave_append(bool pred, int32_t* counter) {
 __device__ __forceinline__ int fetch_chunk(int32_t* cursors, int n_chunks, int& x, int& tried) {
     while (tried < 8) {
         int lo = (int)(((int64_t)x * n_chunks) >> 3), hi = (int)(((int64_t)(x + 1) * n_chunks) >> 3);
-        int c = 0;
-        if (__lane_id() == 0) c = atomicAdd(&cursors[x], 1);
+        int c = hi - lo;
+        if (__lane_id() == 0) {
+            // a plain load first: once a range is drained, skip its atomic
+            int seen = __atomic_load_n(&cursors[x], __ATOMIC_RELAXED);
+            if (seen < hi - lo) c = atomicAdd(&cursors[x], 1);
+        }
         c = __shfl(c, 0);
         if (lo + c < hi) return lo + c;
         x = (x + 1) & 7;
@@ -163,10 +179,11 @@ __device__ __forceinline__ Ray camera_ray(const GCamera& c, uint32_t x, uint32_t
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(128) k_trace_closest(GScene S, const float* __restrict__ rays, int n,
+__global__ void __launch_bounds__(128) NR_TRAV_ATTR k_trace_closest(const GScene* __restrict__ Sp, const float* __restrict__ rays, int n,
                                                        nr_hit* __restrict__ hits, int32_t* __restrict__ counts,
                                                        StackCfg SC) {
     extern __shared__ int2 smem[];
+    const GScene& S = *Sp;
     Stack st = make_stack(smem, S, SC);
     const int n_chunks = (n + 63) >> 6;
     int x = blockIdx.x & 7, tried = 0;
@@ -203,29 +220,32 @@ __global__ void __launch_bounds__(128) k_trace_closest(GScene S, const float* __
     }
 }
 
-// Primary rays in 8x8 pixel tiles (one tile per wave) so a wave's rays are
-// coherent; the live queue is filled by wave compaction, counts[CNT_ACT] must
-// be zero on entry.
-__global__ void __launch_bounds__(256) k_generate(GCamera cam, PathPtrs P, uint32_t s, uint32_t seed, uint32_t sWH) {
+// Primary rays in 8x8 pixel tiles so a wave's 64 rays are coherent.  Thread t
+// takes pixel (x, y) of tile order and queue slot q(x, y): bands of 8 rows,
+// tiles left to right within a band, rows within a tile (partial tiles at the
+// right and bottom edges are packed densely).
+__global__ void __launch_bounds__(256) k_generate(GCamera cam, PathPtrs P, uint32_t n, uint32_t s, uint32_t seed,
+                                                  uint32_t sWH) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) P.counts[CNT_ACT + 0] = (int32_t)n;
     uint32_t tiles_x = (cam.W + 7) >> 3;
     uint32_t tile = t >> 6, within = t & 63;
     uint32_t x = (tile % tiles_x) * 8 + (within & 7), y = (tile / tiles_x) * 8 + (within >> 3);
-    bool valid = x < cam.W && y < cam.H;
+    if (x >= cam.W || y >= cam.H) return;
     uint32_t i = y * cam.W + x;
-    if (valid) {
-        Gen g{seed, i + sWH, 0u};
-        Ray r = camera_ray(cam, x, y, i, s, seed, g);
-        P.o[i] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
-        P.d[i] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
-        P.atten[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-        P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        P.diff[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        P.nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        P.draw[i] = g.draw;
-    }
-    int slot = wave_append(valid, &P.counts[CNT_ACT + 0]);
-    if (valid) P.qact[slot] = (int32_t)i;
+    uint32_t band = y >> 3, bh = min(8u, cam.H - band * 8);
+    uint32_t tx = x >> 3, tw = min(8u, cam.W - tx * 8);
+    uint32_t q = band * 8 * cam.W + tx * 8 * bh + (y & 7) * tw + (x & 7);
+    Gen g{seed, i + sWH, 0u};
+    Ray r = camera_ray(cam, x, y, i, s, seed, g);
+    P.o[i] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
+    P.d[i] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
+    P.atten[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.diff[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.draw[i] = g.draw;
+    P.qact[q] = (int32_t)i;
 }
 
 // Clears the counters a bounce appends to (material queues, next live queue)
@@ -242,8 +262,9 @@ __global__ void k_counts(int32_t* counts, int32_t bounce) {
     if (t == 0) counts[CNT_ACT + ((bounce + 1) & 1)] = 0;
 }
 
-__global__ void __launch_bounds__(128) k_extend(GScene S, PathPtrs P, TraceArgs A, StackCfg SC) {
+__global__ void __launch_bounds__(128) NR_TRAV_ATTR k_extend(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A, StackCfg SC) {
     extern __shared__ int2 smem[];
+    const GScene& S = *Sp;
     Stack st = make_stack(smem, S, SC);
     const int in = A.bounce & 1;
     const int n = P.counts[CNT_ACT + in];
@@ -322,7 +343,8 @@ __device__ __forceinline__ f3 mat_eval(const GScene& S, const GMaterial& m, f3 n
 }
 
 template <int M>
-__global__ void __launch_bounds__(256) k_shade(GScene S, PathPtrs P, TraceArgs A) {
+__global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A) {
+    const GScene& S = *Sp;
     const int n = P.counts[CNT_MAT + M];
     const int out = (A.bounce + 1) & 1;
     const int stride = gridDim.x * blockDim.x;
@@ -596,6 +618,7 @@ struct nr_ctx {
     hipStream_t stream = nullptr;
     bool has_scene = false;
     GScene scene{};
+    GScene* d_scene = nullptr;          // device copy of `scene`
     std::vector<void*> scene_mem;
     bool mat_present[NR_MAT_TYPES] = {false};
     View views[NR_MAX_VIEWS];
@@ -889,6 +912,12 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
     const float fr[6][3] = {{90, 0, 0}, {-90, 0, 0}, {0, 0, -90}, {0, 0, 90}, {0, 0, 0}, {180, 0, 0}};
     const float one[3] = {1.0f, 1.0f, 1.0f};
     for (int i = 0; i < 6; i++) S.cube_faces[i] = host::xform(fp[i], fr[i], one, false);
+    {
+        GScene* ds;
+        int e = upload(c, &S, 1, &ds);
+        if (e) return e;
+        c->d_scene = ds;
+    }
     HIPCK(hipStreamSynchronize(c->stream));
     c->scene = S;
     c->has_scene = true;
@@ -1003,7 +1032,6 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     uint32_t N = v.H * v.W;
     int e = ensure_paths(c, N);
     if (e) return e;
-    const GScene& S = c->scene;
     StackCfg SC{c->spill, (size_t)c->trav_grid * 128};
     const size_t lds = trav_lds_bytes();
     const uint32_t gen_threads = ((v.W + 7) / 8) * ((v.H + 7) / 8) * 64;
@@ -1023,20 +1051,19 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     for (uint32_t s = d->sample_begin; s < d->sample_end; s++) {
         uint32_t sWH = (uint32_t)((uint64_t)s * (uint64_t)v.W * (uint64_t)v.H);
         mark(2);
-        hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->P.counts, -1);
-        hipLaunchKernelGGL(k_generate, gGen, dim3(256), 0, c->stream, v.cam, c->P, s, d->seed, sWH);
+        hipLaunchKernelGGL(k_generate, gGen, dim3(256), 0, c->stream, v.cam, c->P, N, s, d->seed, sWH);
         for (uint32_t b = 0; b < d->ray_depth; b++) {
             TraceArgs A{d->seed, sWH, b, d->ray_depth, d->do_rr, d->rr_start_bounce, d->rr_survival};
             hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->P.counts, (int32_t)b);
             mark(0);
-            hipLaunchKernelGGL(k_extend, dim3(c->trav_grid), dim3(128), lds, c->stream, S, c->P, A, SC);
+            hipLaunchKernelGGL(k_extend, dim3(c->trav_grid), dim3(128), lds, c->stream, c->d_scene, c->P, A, SC);
             mark(1);
             dim3 gS(c->shade_grid);
-            if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL(k_shade<NR_MAT_LAMBERTIAN>, gS, dim3(256), 0, c->stream, S, c->P, A);
-            if (c->mat_present[NR_MAT_PBR]) hipLaunchKernelGGL(k_shade<NR_MAT_PBR>, gS, dim3(256), 0, c->stream, S, c->P, A);
-            if (c->mat_present[NR_MAT_DIELECTRIC]) hipLaunchKernelGGL(k_shade<NR_MAT_DIELECTRIC>, gS, dim3(256), 0, c->stream, S, c->P, A);
-            if (c->mat_present[NR_MAT_EMISSIVE]) hipLaunchKernelGGL(k_shade<NR_MAT_EMISSIVE>, gS, dim3(256), 0, c->stream, S, c->P, A);
-            if (c->mat_present[NR_MAT_ISOTROPIC]) hipLaunchKernelGGL(k_shade<NR_MAT_ISOTROPIC>, gS, dim3(256), 0, c->stream, S, c->P, A);
+            if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL(k_shade<NR_MAT_LAMBERTIAN>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
+            if (c->mat_present[NR_MAT_PBR]) hipLaunchKernelGGL(k_shade<NR_MAT_PBR>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
+            if (c->mat_present[NR_MAT_DIELECTRIC]) hipLaunchKernelGGL(k_shade<NR_MAT_DIELECTRIC>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
+            if (c->mat_present[NR_MAT_EMISSIVE]) hipLaunchKernelGGL(k_shade<NR_MAT_EMISSIVE>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
+            if (c->mat_present[NR_MAT_ISOTROPIC]) hipLaunchKernelGGL(k_shade<NR_MAT_ISOTROPIC>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
         }
         mark(2);
         hipLaunchKernelGGL(k_accumulate, gN, dim3(256), 0, c->stream, c->P, v.layer[NR_LAYER_BEAUTY],
@@ -1097,7 +1124,7 @@ int nr_trace_closest_device(nr_ctx* c, const float* d_rays, int32_t n, nr_hit* d
     hipEvent_t e0 = c->ev_begin, e1 = c->ev_end;
     if (c->timing) (void)hipEventRecord(e0, c->stream);
     hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->d_err, 0);
-    hipLaunchKernelGGL(k_trace_closest, dim3(c->trav_grid), dim3(128), trav_lds_bytes(), c->stream, c->scene, d_rays, n,
+    hipLaunchKernelGGL(k_trace_closest, dim3(c->trav_grid), dim3(128), trav_lds_bytes(), c->stream, c->d_scene, d_rays, n,
                        d_hits, c->d_err, SC);
     HIPCK(hipGetLastError());
     if (c->timing) {

@@ -462,25 +462,64 @@ NR_D bool medium_hit(const GScene& S, const GObject& o, const Ray& r, float tmin
     return true;
 }
 
-// Hittable::hit_test (hittable/hittable.h:324-330)
-NR_D bool hit_test(const GScene& S, const GObject& o, const Ray& r, HRec& rec, Stack& st, int sp0, bool& overflow) {
+// Object test of the top-level traversal (Hittable::hit_test, hittable.h:324-330):
+// the world-space distance of the
+// object's closest hit (Hittable::hit_test + HitRecord::to_world_space), and
+// for plain meshes the triangle slot, barycentrics and object-space t so the
+// full record can be rebuilt without a second traversal.
+// Analytic objects, media and lights: out of line so their uniform loads (cube
+// face transforms, medium parameters) are not hoisted into the registers of the
+// mesh traversal loop.  Fills rec in world space.
+__device__ __attribute__((noinline)) bool object_hit_generic(const GScene& S, const GObject& o, const Ray& r,
+                                                             HRec& rec, Stack& st, int sp0, bool& overflow) {
     Ray orr = to_object(r, o.xf);
+    hrec_clear(rec);
     bool h;
-    if (o.type == 6)          // ObjectLight (lights.h:11-20)
+    if (o.type == 6)
         h = prim_hit(S, o.bound_type, o, orr, -NR_FMAX, NR_FMAX, rec, st, sp0, overflow);
     else if (o.type == 5)
         h = medium_hit(S, o, orr, NR_EPS, NR_FMAX, rec, st, sp0, overflow);
     else
         h = prim_hit(S, o.type, o, orr, NR_EPS, NR_FMAX, rec, st, sp0, overflow);
-    to_world(rec, o.xf, orr, r, true);
+    if (h) to_world(rec, o.xf, orr, r, true);
     return h;
 }
 
+NR_D bool object_t(const GScene& S, const GObject& o, const Ray& r, Stack& st, int sp0, bool& overflow,
+                   float& tw, int32_t& slot, float& u, float& v, float& tobj) {
+    HRec rec;
+    bool h;
+    slot = -1;
+    if (o.type == 4 || o.type == 1 || o.type == 2) {   // mesh, quad, sphere inline
+        Ray orr = to_object(r, o.xf);
+        hrec_clear(rec);
+        if (o.type == 4) {
+            const GMesh& M = S.meshes[o.mesh];
+            slot = mesh_closest(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow);
+            h = slot >= 0;
+            if (h) mesh_interp(M, slot, orr, tobj, u, v, rec);
+        } else if (o.type == 1) {
+            h = quad_hit(orr, NR_EPS, NR_FMAX, rec);
+        } else {
+            h = sphere_hit(orr, NR_EPS, NR_FMAX, rec, o.radius, o.xf);
+        }
+        if (h) to_world(rec, o.xf, orr, r, true);
+    } else {
+        h = object_hit_generic(S, o, r, rec, st, sp0, overflow);
+    }
+    if (!h) return false;
+    tw = rec.t;
+    return true;
+}
+
 // SceneGraph::hit (scene/include/graph.h:16-66).  Returns the object table
-// index of the closest hit or -1; rec receives the world-space record.
+// index of the closest hit or -1; rec receives the world-space record.  Only
+// (object, t, triangle slot, u, v) stay live during the traversal; the record
+// of the winner is rebuilt at the end by the same deterministic arithmetic.
 NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, bool& overflow) {
     float tmin = NR_EPS, tmax = NR_FMAX;
-    int32_t best = -1;
+    int32_t best = -1, bslot = -1;
+    float bu = 0.0f, bv = 0.0f, btobj = 0.0f;
     int sp = 0;
     if (st.cap < 1 || S.n_objects == 0) return -1;
     st.push(sp, S.root_ref, box_t(S.root_min, S.root_max, r, tmin, tmax));
@@ -490,14 +529,12 @@ NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, 
         if (!(te < tmax)) continue;
         if (ref < 0) {
             int32_t oi = ~ref;
-            const GObject& o = S.objects[oi];
-            HRec tmp;
-            hrec_clear(tmp);
-            if (hit_test(S, o, r, tmp, st, sp, overflow)) {
-                if (tmin < tmp.t && tmp.t < tmax) {
-                    best = oi;
-                    rec = tmp;
-                    tmax = tmp.t;
+            float tw, u, v, tobj;
+            int32_t slot;
+            if (object_t(S, S.objects[oi], r, st, sp, overflow, tw, slot, u, v, tobj)) {
+                if (tmin < tw && tw < tmax) {
+                    best = oi; bslot = slot; bu = u; bv = v; btobj = tobj;
+                    tmax = tw;
                 }
             }
             if (overflow) break;
@@ -512,6 +549,17 @@ NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, 
             if (sp + 2 > st.cap) { overflow = true; break; }
             if (far_t < tmax) st.push(sp, far_i, far_t);
             st.push(sp, near_i, near_t);
+        }
+    }
+    if (best >= 0 && !overflow) {
+        const GObject& o = S.objects[best];
+        if (o.type == 4) {
+            Ray orr = to_object(r, o.xf);
+            hrec_clear(rec);
+            mesh_interp(S.meshes[o.mesh], bslot, orr, btobj, bu, bv, rec);
+            to_world(rec, o.xf, orr, r, true);
+        } else {
+            object_hit_generic(S, o, r, rec, st, 0, overflow);
         }
     }
     return best;

@@ -14,7 +14,7 @@ import numpy as np
 from . import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libnectar_hip.so")
+LIB_PATH = os.environ.get("NECTAR_HIP_LIB") or os.path.join(_HERE, "libnectar_hip.so")
 _lib = None
 
 
