@@ -51,7 +51,7 @@ static int fail(int code, const std::string& msg) { g_err = msg; return code; }
 // per-XCD chunk cursors of the persistent traversal kernels [16,24)
 enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_FETCH = 16, CNT_N = 32 };
 #ifndef NR_LDS_LEVELS
-#define NR_LDS_LEVELS 16
+#define NR_LDS_LEVELS 8
 #endif
 constexpr int LDS_STACK_LEVELS = NR_LDS_LEVELS;   // traversal stack levels kept in LDS
 // register budget of the traversal kernels: minimum waves per SIMD (0 = compiler's choice)
@@ -262,18 +262,27 @@ __global__ void k_counts(int32_t* counts, int32_t bounce) {
     if (t == 0) counts[CNT_ACT + ((bounce + 1) & 1)] = 0;
 }
 
+#ifndef NR_PERSISTENT
+#define NR_PERSISTENT 0
+#endif
 __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_extend(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A, StackCfg SC) {
     extern __shared__ int2 smem[];
     const GScene& S = *Sp;
     Stack st = make_stack(smem, S, SC);
     const int in = A.bounce & 1;
     const int n = P.counts[CNT_ACT + in];
+#if NR_PERSISTENT
     const int n_chunks = (n + 63) >> 6;
     int x = blockIdx.x & 7, tried = 0;
     while (true) {
         int chunk = fetch_chunk(P.counts + CNT_FETCH, n_chunks, x, tried);
         if (chunk < 0) break;
         int i = chunk * 64 + st.lane;
+#else
+    {
+        if ((int)(blockIdx.x * blockDim.x) >= n) return;
+        int i = blockIdx.x * blockDim.x + threadIdx.x;
+#endif
         int path = -1;
         int mtype = 0;
         if (i < n) {
@@ -633,6 +642,8 @@ struct nr_ctx {
     int shade_grid = 0;                 // workgroups of 256 for k_shade
     int2* spill = nullptr;              // traversal stack levels beyond LDS
     size_t spill_levels = 0;
+    int2* spill_np = nullptr;           // same, one column per path (one-ray-per-thread grid)
+    size_t spill_np_elems = 0;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;             // per-launch marks
@@ -730,6 +741,7 @@ void nr_destroy(nr_ctx* c) {
     if (c->ev_end) (void)hipEventDestroy(c->ev_end);
     if (c->d_err) (void)hipFree(c->d_err);
     if (c->spill) (void)hipFree(c->spill);
+    if (c->spill_np) (void)hipFree(c->spill_np);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1032,7 +1044,22 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     uint32_t N = v.H * v.W;
     int e = ensure_paths(c, N);
     if (e) return e;
+#if NR_PERSISTENT
     StackCfg SC{c->spill, (size_t)c->trav_grid * 128};
+#else
+    StackCfg SC{c->spill_np, (size_t)N};
+    {
+        size_t lv = c->scene.stack_depth > LDS_STACK_LEVELS ? (size_t)(c->scene.stack_depth - LDS_STACK_LEVELS) : 0;
+        if (lv * N > c->spill_np_elems) {
+            if (c->spill_np) HIPCK(hipFree(c->spill_np));
+            c->spill_np = nullptr;
+            c->spill_np_elems = 0;
+            HIPCK(hipMalloc((void**)&c->spill_np, lv * N * sizeof(int2)));
+            c->spill_np_elems = lv * N;
+        }
+        SC.spill = c->spill_np;
+    }
+#endif
     const size_t lds = trav_lds_bytes();
     const uint32_t gen_threads = ((v.W + 7) / 8) * ((v.H + 7) / 8) * 64;
     dim3 gGen((gen_threads + 255) / 256), gN((N + 255) / 256);
@@ -1056,7 +1083,11 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
             TraceArgs A{d->seed, sWH, b, d->ray_depth, d->do_rr, d->rr_start_bounce, d->rr_survival};
             hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->P.counts, (int32_t)b);
             mark(0);
+#if NR_PERSISTENT
             hipLaunchKernelGGL(k_extend, dim3(c->trav_grid), dim3(128), lds, c->stream, c->d_scene, c->P, A, SC);
+#else
+            hipLaunchKernelGGL(k_extend, dim3((N + 127) / 128), dim3(128), lds, c->stream, c->d_scene, c->P, A, SC);
+#endif
             mark(1);
             dim3 gS(c->shade_grid);
             if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL(k_shade<NR_MAT_LAMBERTIAN>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);

@@ -470,7 +470,15 @@ NR_D bool medium_hit(const GScene& S, const GObject& o, const Ray& r, float tmin
 // Analytic objects, media and lights: out of line so their uniform loads (cube
 // face transforms, medium parameters) are not hoisted into the registers of the
 // mesh traversal loop.  Fills rec in world space.
-__device__ __attribute__((noinline)) bool object_hit_generic(const GScene& S, const GObject& o, const Ray& r,
+#ifndef NR_GENERIC_INLINE
+#define NR_GENERIC_INLINE 1
+#endif
+#if NR_GENERIC_INLINE
+#define NR_GENERIC_ATTR __forceinline__
+#else
+#define NR_GENERIC_ATTR __attribute__((noinline))
+#endif
+__device__ NR_GENERIC_ATTR bool object_hit_generic(const GScene& S, const GObject& o, const Ray& r,
                                                              HRec& rec, Stack& st, int sp0, bool& overflow) {
     Ray orr = to_object(r, o.xf);
     hrec_clear(rec);
