@@ -87,14 +87,14 @@ def main():
     ctx.render(0, abi.render_desc(1, 1 + K, rp["depth"], rp["seed"]))
     ctx.sync()
     t1 = time.perf_counter()
-    tm = ctx.timings()       # [total, traversal, shading, generate+accumulate] ms, HIP events
+    tm = ctx.timings()       # [total, traversal, shading, generate+accumulate] ms (HIP events), iterations
     ctx.set_timing(False)
     img = ctx.read(0)
     ctx.close()
     dt = t1 - t0
     paths = K * PATHS_PER_STEP
     value = paths / dt / 1e6
-    n_extend = K * rp["depth"]
+    n_extend = int(tm[4])    # one k_extend launch per streaming iteration
     trav_s = tm[1] / 1e3
     achieved = TRAVERSAL_BYTES_PER_PATH * paths / trav_s / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -262,7 +262,8 @@ int nr_sync(nr_ctx* ctx);
 
 /* Per-kernel GPU time (ms) of the last nr_render / nr_trace_closest call,
  * measured with HIP events on the context stream: [0] whole call,
- * [1] traversal kernels, [2] shading kernels, [3] generate+accumulate.
+ * [1] traversal kernels, [2] shading kernels, [3] generate+accumulate,
+ * [4] streaming iterations (= traversal launches) of nr_render.
  * Returns the number of values written. */
 int nr_last_timings(nr_ctx* ctx, float* ms, int n);
 

@@ -1,23 +1,27 @@
 // nectar_hip.hip — MI355X wavefront path tracer behind the C ABI of
 // include/nectar_hip.h.
 //
-// One sample pass of a view is a wavefront pipeline over all its pixels:
-//   k_generate      camera rays (camera.h:186-278) for every pixel
-//   per bounce:
-//     k_counts      clears the queue counters of this bounce
+// The sample passes of a view run as a streaming wavefront pipeline with one
+// path slot per pixel:
+//   k_start         first camera ray (camera.h:186-278) of every pixel
+//   per iteration:
+//     k_counts      clears the queue counters the iteration appends to
 //     k_extend      closest hit (SceneGraph::hit, graph.h:16-66) of every live
-//                   path; misses add the sky and retire; hits are compacted
-//                   into one queue per material type (wave ballot + one atomic
-//                   per wave and type)
+//                   path; misses add the sky and end the path; hits are
+//                   compacted into one queue per material type (wave ballot +
+//                   one atomic per wave and type)
 //     k_shade<M>    one kernel per material type present: emission, scatter,
 //                   BRDF sampling / mixture light sampling (brdf.cu:4-35),
 //                   bounce-0 AOVs, Russian roulette (trace.cu:63-78); survivors
 //                   are compacted into the next live queue
-//   k_accumulate    running-mean fold of the pass into the view's AOVs
-//                   (accumulate_samples_kernel, data.cu:28-40)
+//     k_regen       folds every ended path's sample into the running-mean AOVs
+//                   (accumulate_samples_kernel, data.cu:28-40) and starts the
+//                   pixel's next sample in the same iteration
+// The host reads the live count once per iteration to size the traversal grid.
 // Path state lives in HBM as SoA float4 arrays indexed by pixel; queues hold
 // pixel indices.  The per-path Philox stream is (seed, pixel + s*W*H, draw id)
-// exactly as trace.cu:58 keys it, so every path makes the reference's draws.
+// exactly as trace.cu:58 keys it, so every path makes the reference's draws,
+// and each pixel's samples are folded in order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,9 +51,8 @@ static int fail(int code, const std::string& msg) { g_err = msg; return code; }
     } while (0)
 
 // counters layout
-// counters (one int each): live queues [0,2), material queues [2,8), error flag,
-// per-XCD chunk cursors of the persistent traversal kernels [16,24)
-enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_FETCH = 16, CNT_N = 32 };
+// counters (one int each): live queues [0,2), material queues [2,8), error flag
+enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_N = 16 };
 #ifndef NR_LDS_LEVELS
 #define NR_LDS_LEVELS 8
 #endif
@@ -65,10 +68,12 @@ constexpr int LDS_STACK_LEVELS = NR_LDS_LEVELS;   // traversal stack levels kept
 #endif
 
 struct StackCfg {
-    int2* spill;            // levels >= LDS_STACK_LEVELS, one column per persistent thread
-    size_t stride;          // persistent threads in the grid
+    int2* spill;            // levels >= LDS_STACK_LEVELS, one column per ray of the launch
+    size_t stride;          // rays in the launch
 };
 
+// Path state, one slot per pixel.  o.w carries the path's bounce index and
+// d.w the sample index it is tracing (as integer bits).
 struct PathPtrs {
     float4* o;
     float4* d;
@@ -81,17 +86,19 @@ struct PathPtrs {
     int32_t* qact;      // 2 * cap
     int32_t* qmat;      // NR_MAT_TYPES * cap
     int32_t* counts;    // CNT_N
+    uint8_t* finished;  // 1 = the pixel's path ended, fold it and start its next sample
     uint32_t cap;
 };
 
 struct TraceArgs {
     uint32_t seed;
-    uint32_t sWH;       // (uint32)(s * W * H), the subsequence offset of pass s
-    uint32_t bounce;
+    uint32_t WH;        // pixels of the view: the subsequence of sample s is pixel + s*W*H (mod 2^32)
+    uint32_t in;        // live queue read by this iteration (the other one is filled)
     uint32_t depth;
     int32_t do_rr;
     uint32_t rr_start;
     float rr_survival;
+    uint32_t s_end;     // passes [s_begin, s_end) are rendered
 };
 
 // ---------------------------------------------------------------------------
@@ -107,28 +114,6 @@ __device__ __forceinline__ int wave_append(bool pred, int32_t* counter) {
     base = __shfl(base, leader);
     unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
     return pred ? base + __popcll(below) : -1;
-}
-
-// Persistent traversal kernels take work in chunks of 64 items (one per lane).
-// Chunks are XCD-affine: the workgroups of XCD x (blockIdx % 8 under the
-// round-robin dispatch of workgroups to the 8 XCDs) first drain their own
-// contiguous eighth of the chunks, so neighbouring rays share an L2, then steal
-// from the next XCD's range.  Wave-uniform; -1 when all work is taken.
-__device__ __forceinline__ int fetch_chunk(int32_t* cursors, int n_chunks, int& x, int& tried) {
-    while (tried < 8) {
-        int lo = (int)(((int64_t)x * n_chunks) >> 3), hi = (int)(((int64_t)(x + 1) * n_chunks) >> 3);
-        int c = hi - lo;
-        if (__lane_id() == 0) {
-            // a plain load first: once a range is drained, skip its atomic
-            int seen = __atomic_load_n(&cursors[x], __ATOMIC_RELAXED);
-            if (seen < hi - lo) c = atomicAdd(&cursors[x], 1);
-        }
-        c = __shfl(c, 0);
-        if (lo + c < hi) return lo + c;
-        x = (x + 1) & 7;
-        tried++;
-    }
-    return -1;
 }
 
 __device__ __forceinline__ Stack make_stack(int2* smem, const GScene& S, const StackCfg& SC) {
@@ -179,145 +164,144 @@ __device__ __forceinline__ Ray camera_ray(const GCamera& c, uint32_t x, uint32_t
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(128) NR_TRAV_ATTR k_trace_closest(const GScene* __restrict__ Sp, const float* __restrict__ rays, int n,
-                                                       nr_hit* __restrict__ hits, int32_t* __restrict__ counts,
-                                                       StackCfg SC) {
+__global__ void __launch_bounds__(128) NR_TRAV_ATTR k_trace_closest(const GScene* __restrict__ Sp,
+                                                                    const float* __restrict__ rays, int n,
+                                                                    nr_hit* __restrict__ hits,
+                                                                    int32_t* __restrict__ counts, StackCfg SC) {
     extern __shared__ int2 smem[];
     const GScene& S = *Sp;
     Stack st = make_stack(smem, S, SC);
-    const int n_chunks = (n + 63) >> 6;
-    int x = blockIdx.x & 7, tried = 0;
-    while (true) {
-        int chunk = fetch_chunk(counts + CNT_FETCH, n_chunks, x, tried);
-        if (chunk < 0) break;
-        int i = chunk * 64 + st.lane;
-        if (i >= n) continue;
-        const float* q = rays + 6 * (size_t)i;
-        Ray r = mkray(mk3(q[0], q[1], q[2]), mk3(q[3], q[4], q[5]));
-        HRec rec;
-        hrec_clear(rec);
-        bool of = false;
-        int oi = scene_closest(S, r, rec, st, of);
-        if (of) atomicOr(&counts[CNT_ERR], 1);
-        nr_hit h;
-        if (oi < 0) {
-            memset(&h, 0, sizeof(h));
-            h.object_id = -1;
-            h.prim = -1;
-        } else {
-            const GObject& o = S.objects[oi];
-            h.object_id = o.object_id;
-            h.material_id = o.material;
-            h.prim = rec.prim;
-            h.front_face = rec.front ? 1 : 0;
-            h.t = rec.t;
-            h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
-            h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
-            h.tangent[0] = rec.tg.x; h.tangent[1] = rec.tg.y; h.tangent[2] = rec.tg.z;
-            h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
-        }
-        hits[i] = h;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* q = rays + 6 * (size_t)i;
+    Ray r = mkray(mk3(q[0], q[1], q[2]), mk3(q[3], q[4], q[5]));
+    HRec rec;
+    hrec_clear(rec);
+    bool of = false;
+    int oi = scene_closest(S, r, rec, st, of);
+    if (of) atomicOr(&counts[CNT_ERR], 1);
+    nr_hit h;
+    if (oi < 0) {
+        memset(&h, 0, sizeof(h));
+        h.object_id = -1;
+        h.prim = -1;
+    } else {
+        const GObject& o = S.objects[oi];
+        h.object_id = o.object_id;
+        h.material_id = o.material;
+        h.prim = rec.prim;
+        h.front_face = rec.front ? 1 : 0;
+        h.t = rec.t;
+        h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
+        h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
+        h.tangent[0] = rec.tg.x; h.tangent[1] = rec.tg.y; h.tangent[2] = rec.tg.z;
+        h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
     }
+    hits[i] = h;
 }
 
-// Primary rays in 8x8 pixel tiles so a wave's 64 rays are coherent.  Thread t
-// takes pixel (x, y) of tile order and queue slot q(x, y): bands of 8 rows,
-// tiles left to right within a band, rows within a tile (partial tiles at the
-// right and bottom edges are packed densely).
-__global__ void __launch_bounds__(256) k_generate(GCamera cam, PathPtrs P, uint32_t n, uint32_t s, uint32_t seed,
-                                                  uint32_t sWH) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) P.counts[CNT_ACT + 0] = (int32_t)n;
+// Streaming sample passes.  Every pixel owns one path slot.  When a pixel's
+// path ends (sky, no scatter, zero pdf, roulette or max depth) it is flagged;
+// k_regen folds the finished sample into the running-mean AOVs exactly as
+// accumulate_samples_kernel does (data.cu:28-40) and starts the pixel's next
+// sample at once, so traversal launches stay full.  A pixel's samples are
+// folded strictly in order, so the AOVs equal the reference's pass-by-pass
+// result bit for bit.
+
+// 8x8 pixel tiles, tiles left to right within bands of 8 rows: thread t of the
+// tiled grid maps to pixel (x, y); partial tiles at the edges are skipped.
+__device__ __forceinline__ bool tile_pixel(const GCamera& cam, uint32_t t, uint32_t& x, uint32_t& y) {
     uint32_t tiles_x = (cam.W + 7) >> 3;
     uint32_t tile = t >> 6, within = t & 63;
-    uint32_t x = (tile % tiles_x) * 8 + (within & 7), y = (tile / tiles_x) * 8 + (within >> 3);
-    if (x >= cam.W || y >= cam.H) return;
+    x = (tile % tiles_x) * 8 + (within & 7);
+    y = (tile / tiles_x) * 8 + (within >> 3);
+    return x < cam.W && y < cam.H;
+}
+
+__device__ __forceinline__ void start_path(const GCamera& cam, PathPtrs& P, uint32_t x, uint32_t y, uint32_t s,
+                                           uint32_t seed, uint32_t WH) {
     uint32_t i = y * cam.W + x;
-    uint32_t band = y >> 3, bh = min(8u, cam.H - band * 8);
-    uint32_t tx = x >> 3, tw = min(8u, cam.W - tx * 8);
-    uint32_t q = band * 8 * cam.W + tx * 8 * bh + (y & 7) * tw + (x & 7);
-    Gen g{seed, i + sWH, 0u};
+    Gen g{seed, i + s * WH, 0u};                                                // trace.cu:58
     Ray r = camera_ray(cam, x, y, i, s, seed, g);
-    P.o[i] = make_float4(r.o.x, r.o.y, r.o.z, 0.0f);
-    P.d[i] = make_float4(r.d.x, r.d.y, r.d.z, 0.0f);
+    P.o[i] = make_float4(r.o.x, r.o.y, r.o.z, __int_as_float(0));
+    P.d[i] = make_float4(r.d.x, r.d.y, r.d.z, __int_as_float((int)s));
     P.atten[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.diff[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.draw[i] = g.draw;
-    P.qact[q] = (int32_t)i;
 }
 
-// Clears the counters a bounce appends to (material queues, next live queue)
-// and the chunk cursors of its traversal; bounce = -1 clears the first live
-// queue before k_generate.
-__global__ void k_counts(int32_t* counts, int32_t bounce) {
+// running-mean fold of one finished sample (accumulate_samples_kernel)
+__device__ __forceinline__ void fold(float* layer, size_t HW, size_t i, float4 v, float a, float b) {
+    if (!layer) return;
+    layer[i] = layer[i] * a + v.x * b;
+    layer[HW + i] = layer[HW + i] * a + v.y * b;
+    layer[2 * HW + i] = layer[2 * HW + i] * a + v.z * b;
+}
+
+// First sample of every pixel, queued in tile order (closed-form dense slot).
+__global__ void __launch_bounds__(256) k_start(GCamera cam, PathPtrs P, uint32_t s, TraceArgs A) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, x, y;
+    if (t == 0) P.counts[CNT_ACT + 0] = (int32_t)(cam.W * cam.H);
+    if (!tile_pixel(cam, t, x, y)) return;
+    uint32_t band = y >> 3, bh = min(8u, cam.H - band * 8);
+    uint32_t tx = x >> 3, tw = min(8u, cam.W - tx * 8);
+    uint32_t q = band * 8 * cam.W + tx * 8 * bh + (y & 7) * tw + (x & 7);
+    start_path(cam, P, x, y, s, A.seed, A.WH);
+    P.finished[y * cam.W + x] = 0;
+    P.qact[q] = (int32_t)(y * cam.W + x);
+}
+
+// Clears the counters one iteration appends to: material queues and the next live queue.
+__global__ void k_counts(int32_t* counts, int32_t in) {
     int t = threadIdx.x;
-    if (bounce < 0) {
-        if (t == 0) counts[CNT_ACT + 0] = 0;
-        return;
-    }
     if (t < NR_MAT_TYPES) counts[CNT_MAT + t] = 0;
-    if (t < 8) counts[CNT_FETCH + t] = 0;
-    if (t == 0) counts[CNT_ACT + ((bounce + 1) & 1)] = 0;
+    if (t == 0) counts[CNT_ACT + (1 - in)] = 0;
 }
 
-#ifndef NR_PERSISTENT
-#define NR_PERSISTENT 0
-#endif
 __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_extend(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A, StackCfg SC) {
     extern __shared__ int2 smem[];
     const GScene& S = *Sp;
     Stack st = make_stack(smem, S, SC);
-    const int in = A.bounce & 1;
-    const int n = P.counts[CNT_ACT + in];
-#if NR_PERSISTENT
-    const int n_chunks = (n + 63) >> 6;
-    int x = blockIdx.x & 7, tried = 0;
-    while (true) {
-        int chunk = fetch_chunk(P.counts + CNT_FETCH, n_chunks, x, tried);
-        if (chunk < 0) break;
-        int i = chunk * 64 + st.lane;
-#else
-    {
-        if ((int)(blockIdx.x * blockDim.x) >= n) return;
-        int i = blockIdx.x * blockDim.x + threadIdx.x;
-#endif
-        int path = -1;
-        int mtype = 0;
-        if (i < n) {
-            path = P.qact[(size_t)in * P.cap + i];
-            float4 o4 = P.o[path], d4 = P.d[path];
-            Ray r = mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
-            HRec rec;
-            hrec_clear(rec);
-            bool of = false;
-            int oi = scene_closest(S, r, rec, st, of);
-            if (of) atomicOr(&P.counts[CNT_ERR], 1);
-            if (oi < 0) {
-                float4 a4 = P.atten[path], l4 = P.L[path];
-                f3 L = mk3(l4.x, l4.y, l4.z) + mk3(a4.x, a4.y, a4.z) * sky_sample(S.sky, r.d);
-                P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
-            } else {
-                const GObject& ob = S.objects[oi];
-                GHit h;
-                h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
-                h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
-                h.tg[0] = rec.tg.x; h.tg[1] = rec.tg.y; h.tg[2] = rec.tg.z;
-                h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
-                h.material = ob.material;
-                h.front_face = rec.front ? 1 : 0;
-                h.object_id = ob.object_id;
-                h.prim = rec.prim;
-                h.t = rec.t;
-                P.hit[path] = h;
-                mtype = ob.pad[0];
-            }
+    const int n = P.counts[CNT_ACT + A.in];
+    if ((int)(blockIdx.x * blockDim.x) >= n) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int path = -1;
+    int mtype = 0;
+    if (i < n) {
+        path = P.qact[(size_t)A.in * P.cap + i];
+        float4 o4 = P.o[path], d4 = P.d[path];
+        Ray r = mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
+        HRec rec;
+        hrec_clear(rec);
+        bool of = false;
+        int oi = scene_closest(S, r, rec, st, of);
+        if (of) atomicOr(&P.counts[CNT_ERR], 1);
+        if (oi < 0) {                                                           // trace.cu:19-22
+            float4 a4 = P.atten[path], l4 = P.L[path];
+            f3 L = mk3(l4.x, l4.y, l4.z) + mk3(a4.x, a4.y, a4.z) * sky_sample(S.sky, r.d);
+            P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
+            P.finished[path] = 1;
+        } else {
+            const GObject& ob = S.objects[oi];
+            GHit h;
+            h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
+            h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
+            h.tg[0] = rec.tg.x; h.tg[1] = rec.tg.y; h.tg[2] = rec.tg.z;
+            h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
+            h.material = ob.material;
+            h.front_face = rec.front ? 1 : 0;
+            h.object_id = ob.object_id;
+            h.prim = rec.prim;
+            h.t = rec.t;
+            P.hit[path] = h;
+            mtype = ob.pad[0];
         }
-        for (int t = 1; t < NR_MAT_TYPES; t++) {
-            int slot = wave_append(mtype == t, &P.counts[CNT_MAT + t]);
-            if (mtype == t) P.qmat[(size_t)t * P.cap + slot] = path;
-        }
+    }
+    for (int t = 1; t < NR_MAT_TYPES; t++) {
+        int slot = wave_append(mtype == t, &P.counts[CNT_MAT + t]);
+        if (mtype == t) P.qmat[(size_t)t * P.cap + slot] = path;
     }
 }
 
@@ -355,7 +339,7 @@ template <int M>
 __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A) {
     const GScene& S = *Sp;
     const int n = P.counts[CNT_MAT + M];
-    const int out = (A.bounce + 1) & 1;
+    const int out = 1 - A.in;
     const int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
     int i = base + (threadIdx.x & 63);
@@ -364,14 +348,16 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
     if (i < n) {
         path = P.qmat[(size_t)M * P.cap + i];
         GHit h = P.hit[path];
-        float4 d4 = P.d[path], a4 = P.atten[path], l4 = P.L[path];
+        float4 o4 = P.o[path], d4 = P.d[path], a4 = P.atten[path], l4 = P.L[path];
+        const uint32_t bounce = (uint32_t)__float_as_int(o4.w);
+        const uint32_t sample = (uint32_t)__float_as_int(d4.w);
         f3 din = mk3(d4.x, d4.y, d4.z);
         f3 atten = mk3(a4.x, a4.y, a4.z);
         f3 L = mk3(l4.x, l4.y, l4.z);
         f3 p = ld3(h.p), nn = ld3(h.n), tg = ld3(h.tg);
         f2 uv; uv.x = h.uv[0]; uv.y = h.uv[1];
         bool front = h.front_face != 0;
-        Gen g{A.seed, (uint32_t)path + A.sWH, P.draw[path]};
+        Gen g{A.seed, (uint32_t)path + sample * A.WH, P.draw[path]};
         const GMaterial& m = S.materials[h.material];
         // emission (material.h:182-188, 306-312)
         f3 em = s3(0.0f);
@@ -441,12 +427,12 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
             }
         }
         if (alive) {
-            if (A.bounce == 0) {                                                // trace.cu:67-70
+            if (bounce == 0) {                                                  // trace.cu:67-70
                 f3 dv = s3(0.0f) + atten, nv = s3(0.0f) + nn;
                 P.diff[path] = make_float4(dv.x, dv.y, dv.z, 0.0f);
                 P.nrm[path] = make_float4(nv.x, nv.y, nv.z, 0.0f);
             }
-            if (A.do_rr && A.bounce >= A.rr_start && A.bounce + 1 < A.depth) {   // trace.cu:72-77
+            if (A.do_rr && bounce >= A.rr_start && bounce + 1 < A.depth) {      // trace.cu:72-77
                 float survival = fmaxf(atten.x, fmaxf(atten.y, atten.z));
                 survival = fminf(survival, A.rr_survival);
                 if (g.uniform() > survival) alive = false;
@@ -454,12 +440,14 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
             }
         }
         P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
-        if (alive && A.bounce + 1 < A.depth) {
+        if (alive && bounce + 1 < A.depth) {
             P.atten[path] = make_float4(atten.x, atten.y, atten.z, 0.0f);
-            P.o[path] = make_float4(p.x, p.y, p.z, 0.0f);
-            P.d[path] = make_float4(nd.x, nd.y, nd.z, 0.0f);
+            P.o[path] = make_float4(p.x, p.y, p.z, __int_as_float((int)(bounce + 1)));
+            P.d[path] = make_float4(nd.x, nd.y, nd.z, d4.w);
             P.draw[path] = g.draw;
             cont = true;
+        } else {
+            P.finished[path] = 1;
         }
     }
     int slot = wave_append(cont, &P.counts[CNT_ACT + out]);
@@ -467,29 +455,44 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
     }
 }
 
-__global__ void __launch_bounds__(256) k_accumulate(PathPtrs P, float* beauty, float* diffuse, float* normal,
-                                                    uint32_t n, uint32_t s) {
+// Fold the finished paths' samples and start their next samples, scanning all
+// pixels in tile order so the new primary rays of a wave are coherent.
+__global__ void __launch_bounds__(256) k_regen(GCamera cam, PathPtrs P, TraceArgs A, float* beauty,
+                                               float* diffuse, float* normal) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, x = 0, y = 0;
+    bool restart = false;
+    uint32_t i = 0;
+    if (tile_pixel(cam, t, x, y)) {
+        i = y * cam.W + x;
+        if (P.finished[i]) {
+            P.finished[i] = 0;
+            uint32_t s = (uint32_t)__float_as_int(P.d[i].w);
+            float fn = (float)s;
+            float a = (fn - 1.0f) / fn, b = 1.0f / fn;
+            size_t HW = (size_t)cam.W * cam.H;
+            fold(beauty, HW, i, P.L[i], a, b);
+            fold(diffuse, HW, i, P.diff[i], a, b);
+            fold(normal, HW, i, P.nrm[i], a, b);
+            if (s + 1 < A.s_end) {
+                start_path(cam, P, x, y, s + 1, A.seed, A.WH);
+                restart = true;
+            }
+        }
+    }
+    int slot = wave_append(restart, &P.counts[CNT_ACT + (1 - A.in)]);
+    if (restart) P.qact[(size_t)(1 - A.in) * P.cap + slot] = (int32_t)i;
+}
+
+// ray_depth 0: every pass adds a zero sample (trace.cu:63 runs no bounce)
+__global__ void __launch_bounds__(256) k_fold_zero(float* beauty, float* diffuse, float* normal, uint32_t n, uint32_t s) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float fn = (float)s;
     float a = (fn - 1.0f) / fn, b = 1.0f / fn;
-    size_t HW = n;
-    float4 l4 = P.L[i], d4 = P.diff[i], n4 = P.nrm[i];
-    if (beauty) {
-        beauty[i] = beauty[i] * a + l4.x * b;
-        beauty[HW + i] = beauty[HW + i] * a + l4.y * b;
-        beauty[2 * HW + i] = beauty[2 * HW + i] * a + l4.z * b;
-    }
-    if (diffuse) {
-        diffuse[i] = diffuse[i] * a + d4.x * b;
-        diffuse[HW + i] = diffuse[HW + i] * a + d4.y * b;
-        diffuse[2 * HW + i] = diffuse[2 * HW + i] * a + d4.z * b;
-    }
-    if (normal) {
-        normal[i] = normal[i] * a + n4.x * b;
-        normal[HW + i] = normal[HW + i] * a + n4.y * b;
-        normal[2 * HW + i] = normal[2 * HW + i] * a + n4.z * b;
-    }
+    float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    fold(beauty, n, i, z, a, b);
+    fold(diffuse, n, i, z, a, b);
+    fold(normal, n, i, z, a, b);
 }
 
 struct PostArgs {
@@ -638,12 +641,11 @@ struct nr_ctx {
     int32_t* d_err = nullptr;           // counters of nr_trace_closest (CNT_N ints)
     // persistent launch shapes
     int n_cu = 0;
-    int trav_grid = 0;                  // workgroups of 128 for k_extend / k_trace_closest
-    int shade_grid = 0;                 // workgroups of 256 for k_shade
+    int shade_grid = 0;                 // workgroups of 256 for k_shade (grid-stride)
     int2* spill = nullptr;              // traversal stack levels beyond LDS
-    size_t spill_levels = 0;
-    int2* spill_np = nullptr;           // same, one column per path (one-ray-per-thread grid)
-    size_t spill_np_elems = 0;
+    size_t spill_elems = 0;
+    int32_t* h_live = nullptr;          // pinned: live-path count read back once per iteration
+    int64_t last_iterations = 0;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;             // per-launch marks
@@ -669,16 +671,20 @@ static int upload(nr_ctx* c, const T* host, size_t n, T** dev) {
 
 static size_t trav_lds_bytes() { return (size_t)LDS_STACK_LEVELS * 64 * sizeof(int2) * (128 / 64); }
 
-// spill columns for stack levels beyond LDS, one per persistent traversal thread
-static int ensure_spill(nr_ctx* c, int stack_depth) {
-    size_t levels = stack_depth > LDS_STACK_LEVELS ? (size_t)(stack_depth - LDS_STACK_LEVELS) : 0;
-    if (levels <= c->spill_levels && (c->spill || levels == 0)) return NR_OK;
-    if (c->spill) HIPCK(hipFree(c->spill));
-    c->spill = nullptr;
-    c->spill_levels = 0;
-    if (levels == 0) return NR_OK;
-    HIPCK(hipMalloc((void**)&c->spill, levels * (size_t)c->trav_grid * 128 * sizeof(int2)));
-    c->spill_levels = levels;
+// Spill columns for the traversal stack levels beyond LDS, one per ray of a
+// launch of n rays (level-major, so a wave's spills are contiguous).
+static int stack_spill(nr_ctx* c, uint32_t n, StackCfg& SC) {
+    size_t lv = c->scene.stack_depth > LDS_STACK_LEVELS ? (size_t)(c->scene.stack_depth - LDS_STACK_LEVELS) : 0;
+    size_t need = lv * (size_t)n;
+    if (need > c->spill_elems) {
+        if (c->spill) HIPCK(hipFree(c->spill));
+        c->spill = nullptr;
+        c->spill_elems = 0;
+        HIPCK(hipMalloc((void**)&c->spill, need * sizeof(int2)));
+        c->spill_elems = need;
+    }
+    SC.spill = c->spill;
+    SC.stride = n;
     return NR_OK;
 }
 
@@ -713,15 +719,17 @@ nr_ctx* nr_create(int device) {
     }
     (void)hipMemsetAsync(c->d_err, 0, sizeof(int32_t) * CNT_N, c->stream);
     hipDeviceProp_t prop;
-    int nb = 0;
-    if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend, 128, trav_lds_bytes()) != hipSuccess || nb < 1) {
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
         g_err = "device query failed";
         nr_destroy(c);
         return nullptr;
     }
+    if (hipHostMalloc((void**)&c->h_live, sizeof(int32_t) * 4, hipHostMallocDefault) != hipSuccess) {
+        g_err = "hipHostMalloc failed";
+        nr_destroy(c);
+        return nullptr;
+    }
     c->n_cu = prop.multiProcessorCount;
-    c->trav_grid = ((nb * c->n_cu + 7) / 8) * 8;
     c->shade_grid = c->n_cu * 8;
     return c;
 }
@@ -741,7 +749,7 @@ void nr_destroy(nr_ctx* c) {
     if (c->ev_end) (void)hipEventDestroy(c->ev_end);
     if (c->d_err) (void)hipFree(c->d_err);
     if (c->spill) (void)hipFree(c->spill);
-    if (c->spill_np) (void)hipFree(c->spill_np);
+    if (c->h_live) (void)hipHostFree(c->h_live);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -888,10 +896,6 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
     S.n_objects = n;
     S.stack_depth = T.depth + max_mesh_depth + 4;
     {
-        int e = ensure_spill(c, S.stack_depth);
-        if (e) return e;
-    }
-    {
         GNode* dn; GObject* dob; GMesh* dm; GMaterial* dmat; GImage* dimg;
         int e = upload(c, tnodes.data(), tnodes.size(), &dn);
         if (!e) e = upload(c, objs.data(), objs.size(), &dob);
@@ -1015,7 +1019,9 @@ static int ensure_paths(nr_ctx* c, uint32_t n) {
     if (!e) e = al((void**)&P.qact, sizeof(int32_t) * 2 * (size_t)n);
     if (!e) e = al((void**)&P.qmat, sizeof(int32_t) * NR_MAT_TYPES * (size_t)n);
     if (!e) e = al((void**)&P.counts, sizeof(int32_t) * CNT_N);
+    if (!e) e = al((void**)&P.finished, sizeof(uint8_t) * (size_t)n);
     if (e) { free_list(c->path_mem); return e; }
+    HIPCK(hipMemsetAsync(P.finished, 0, n, c->stream));
     HIPCK(hipMemsetAsync(P.counts, 0, sizeof(int32_t) * CNT_N, c->stream));
     P.cap = n;
     c->path_cap = n;
@@ -1044,25 +1050,12 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     uint32_t N = v.H * v.W;
     int e = ensure_paths(c, N);
     if (e) return e;
-#if NR_PERSISTENT
-    StackCfg SC{c->spill, (size_t)c->trav_grid * 128};
-#else
-    StackCfg SC{c->spill_np, (size_t)N};
-    {
-        size_t lv = c->scene.stack_depth > LDS_STACK_LEVELS ? (size_t)(c->scene.stack_depth - LDS_STACK_LEVELS) : 0;
-        if (lv * N > c->spill_np_elems) {
-            if (c->spill_np) HIPCK(hipFree(c->spill_np));
-            c->spill_np = nullptr;
-            c->spill_np_elems = 0;
-            HIPCK(hipMalloc((void**)&c->spill_np, lv * N * sizeof(int2)));
-            c->spill_np_elems = lv * N;
-        }
-        SC.spill = c->spill_np;
-    }
-#endif
+    StackCfg SC;
+    e = stack_spill(c, N, SC);
+    if (e) return e;
     const size_t lds = trav_lds_bytes();
     const uint32_t gen_threads = ((v.W + 7) / 8) * ((v.H + 7) / 8) * 64;
-    dim3 gGen((gen_threads + 255) / 256), gN((N + 255) / 256);
+    dim3 gGen((gen_threads + 255) / 256), gN((N + 255) / 256), gS(c->shade_grid);
     size_t ei = 0;
     float t_ext = 0.0f, t_shade = 0.0f, t_other = 0.0f;
     std::vector<std::pair<size_t, int>> marks;   // (event index of start, category)
@@ -1075,30 +1068,41 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     };
     hipEvent_t e0 = c->ev_begin, e1 = c->ev_end;
     if (c->timing) (void)hipEventRecord(e0, c->stream);
-    for (uint32_t s = d->sample_begin; s < d->sample_end; s++) {
-        uint32_t sWH = (uint32_t)((uint64_t)s * (uint64_t)v.W * (uint64_t)v.H);
+    float* lb = v.layer[NR_LAYER_BEAUTY];
+    float* ld = v.layer[NR_LAYER_DIFFUSE];
+    float* ln = v.layer[NR_LAYER_WORLD_NORMAL];
+    c->last_iterations = 0;
+    if (d->sample_end > d->sample_begin && d->ray_depth == 0) {
+        for (uint32_t s = d->sample_begin; s < d->sample_end; s++)
+            hipLaunchKernelGGL(k_fold_zero, gN, dim3(256), 0, c->stream, lb, ld, ln, N, s);
+    } else if (d->sample_end > d->sample_begin) {
+        TraceArgs A{d->seed, (uint32_t)((uint64_t)v.W * v.H), 0u, d->ray_depth, d->do_rr, d->rr_start_bounce,
+                    d->rr_survival, d->sample_end};
         mark(2);
-        hipLaunchKernelGGL(k_generate, gGen, dim3(256), 0, c->stream, v.cam, c->P, N, s, d->seed, sWH);
-        for (uint32_t b = 0; b < d->ray_depth; b++) {
-            TraceArgs A{d->seed, sWH, b, d->ray_depth, d->do_rr, d->rr_start_bounce, d->rr_survival};
-            hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->P.counts, (int32_t)b);
+        hipLaunchKernelGGL(k_start, gGen, dim3(256), 0, c->stream, v.cam, c->P, d->sample_begin, A);
+        int64_t live = N;
+        const int64_t max_iters = (int64_t)(d->sample_end - d->sample_begin) * d->ray_depth + 1;
+        while (live > 0) {
+            if (c->last_iterations++ > max_iters) return fail(NR_ERR_STATE, "streaming loop did not drain");
+            hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->P.counts, (int32_t)A.in);
             mark(0);
-#if NR_PERSISTENT
-            hipLaunchKernelGGL(k_extend, dim3(c->trav_grid), dim3(128), lds, c->stream, c->d_scene, c->P, A, SC);
-#else
-            hipLaunchKernelGGL(k_extend, dim3((N + 127) / 128), dim3(128), lds, c->stream, c->d_scene, c->P, A, SC);
-#endif
+            hipLaunchKernelGGL(k_extend, dim3((uint32_t)((live + 127) / 128)), dim3(128), lds, c->stream, c->d_scene,
+                               c->P, A, SC);
             mark(1);
-            dim3 gS(c->shade_grid);
             if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL(k_shade<NR_MAT_LAMBERTIAN>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
             if (c->mat_present[NR_MAT_PBR]) hipLaunchKernelGGL(k_shade<NR_MAT_PBR>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
             if (c->mat_present[NR_MAT_DIELECTRIC]) hipLaunchKernelGGL(k_shade<NR_MAT_DIELECTRIC>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
             if (c->mat_present[NR_MAT_EMISSIVE]) hipLaunchKernelGGL(k_shade<NR_MAT_EMISSIVE>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
             if (c->mat_present[NR_MAT_ISOTROPIC]) hipLaunchKernelGGL(k_shade<NR_MAT_ISOTROPIC>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
+            mark(2);
+            hipLaunchKernelGGL(k_regen, gGen, dim3(256), 0, c->stream, v.cam, c->P, A, lb, ld, ln);
+            HIPCK(hipGetLastError());
+            HIPCK(hipMemcpyAsync(c->h_live, c->P.counts + CNT_ACT + (1 - A.in), sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, c->stream));
+            HIPCK(hipStreamSynchronize(c->stream));
+            live = *c->h_live;
+            A.in = 1 - A.in;
         }
-        mark(2);
-        hipLaunchKernelGGL(k_accumulate, gN, dim3(256), 0, c->stream, c->P, v.layer[NR_LAYER_BEAUTY],
-                           v.layer[NR_LAYER_DIFFUSE], v.layer[NR_LAYER_WORLD_NORMAL], N, s);
     }
     HIPCK(hipGetLastError());
     if (c->timing) {
@@ -1115,7 +1119,8 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
         float tot = 0.0f;
         (void)hipEventElapsedTime(&tot, e0, e1);
         c->last_ms[0] = tot; c->last_ms[1] = t_ext; c->last_ms[2] = t_shade; c->last_ms[3] = t_other;
-        c->n_last = 4;
+        c->last_ms[4] = (float)c->last_iterations;
+        c->n_last = 5;
         int32_t err = 0;
         HIPCK(hipMemcpy(&err, c->P.counts + CNT_ERR, sizeof(int32_t), hipMemcpyDeviceToHost));
         if (err) return fail(NR_ERR_LIMIT, "traversal stack overflow");
@@ -1151,12 +1156,14 @@ int nr_trace_closest_device(nr_ctx* c, const float* d_rays, int32_t n, nr_hit* d
     if (n < 0) return fail(NR_ERR_INVALID, "negative ray count");
     if (n == 0) return NR_OK;
     HIPCK(hipSetDevice(c->device));
-    StackCfg SC{c->spill, (size_t)c->trav_grid * 128};
+    StackCfg SC;
+    int e = stack_spill(c, (uint32_t)n, SC);
+    if (e) return e;
+    HIPCK(hipMemsetAsync(c->d_err + CNT_ERR, 0, sizeof(int32_t), c->stream));
     hipEvent_t e0 = c->ev_begin, e1 = c->ev_end;
     if (c->timing) (void)hipEventRecord(e0, c->stream);
-    hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->d_err, 0);
-    hipLaunchKernelGGL(k_trace_closest, dim3(c->trav_grid), dim3(128), trav_lds_bytes(), c->stream, c->d_scene, d_rays, n,
-                       d_hits, c->d_err, SC);
+    hipLaunchKernelGGL(k_trace_closest, dim3((n + 127) / 128), dim3(128), trav_lds_bytes(), c->stream, c->d_scene, d_rays,
+                       n, d_hits, c->d_err, SC);
     HIPCK(hipGetLastError());
     if (c->timing) {
         (void)hipEventRecord(e1, c->stream);
