@@ -644,7 +644,8 @@ struct nr_ctx {
     int shade_grid = 0;                 // workgroups of 256 for k_shade (grid-stride)
     int2* spill = nullptr;              // traversal stack levels beyond LDS
     size_t spill_elems = 0;
-    int32_t* h_live = nullptr;          // pinned: live-path count read back once per iteration
+    int32_t* h_live = nullptr;          // pinned: live-path counts of the last two iterations
+    hipEvent_t ev_count[2] = {nullptr, nullptr};
     int64_t last_iterations = 0;
     // timing
     bool timing = false;
@@ -705,7 +706,9 @@ nr_ctx* nr_create(int device) {
         delete c;
         return nullptr;
     }
-    if (hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+    if (hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_count[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_count[1], hipEventDisableTiming) != hipSuccess) {
         g_err = "hipEventCreate failed";
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -746,6 +749,7 @@ void nr_destroy(nr_ctx* c) {
     }
     for (auto e : c->ev) (void)hipEventDestroy(e);
     if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
+    for (auto e : c->ev_count) if (e) (void)hipEventDestroy(e);
     if (c->ev_end) (void)hipEventDestroy(c->ev_end);
     if (c->d_err) (void)hipFree(c->d_err);
     if (c->spill) (void)hipFree(c->spill);
@@ -1080,13 +1084,23 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
                     d->rr_survival, d->sample_end};
         mark(2);
         hipLaunchKernelGGL(k_start, gGen, dim3(256), 0, c->stream, v.cam, c->P, d->sample_begin, A);
-        int64_t live = N;
-        const int64_t max_iters = (int64_t)(d->sample_end - d->sample_begin) * d->ray_depth + 1;
-        while (live > 0) {
-            if (c->last_iterations++ > max_iters) return fail(NR_ERR_STATE, "streaming loop did not drain");
+        // The live count never grows from one iteration to the next (a path
+        // either continues, or ends and restarts at most once), so a count read
+        // back two iterations late is an upper bound for the traversal grid and
+        // the GPU always has the next iteration queued.
+        const int64_t max_iters = (int64_t)(d->sample_end - d->sample_begin) * d->ray_depth + 2;
+        int64_t it = 0;
+        int64_t bound = N;
+        while (true) {
+            if (it >= 2) {
+                HIPCK(hipEventSynchronize(c->ev_count[it & 1]));
+                bound = c->h_live[it & 1];
+                if (bound == 0) break;
+            }
+            if (it > max_iters) return fail(NR_ERR_STATE, "streaming loop did not drain");
             hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->P.counts, (int32_t)A.in);
             mark(0);
-            hipLaunchKernelGGL(k_extend, dim3((uint32_t)((live + 127) / 128)), dim3(128), lds, c->stream, c->d_scene,
+            hipLaunchKernelGGL(k_extend, dim3((uint32_t)((bound + 127) / 128)), dim3(128), lds, c->stream, c->d_scene,
                                c->P, A, SC);
             mark(1);
             if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL(k_shade<NR_MAT_LAMBERTIAN>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
@@ -1097,12 +1111,13 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
             mark(2);
             hipLaunchKernelGGL(k_regen, gGen, dim3(256), 0, c->stream, v.cam, c->P, A, lb, ld, ln);
             HIPCK(hipGetLastError());
-            HIPCK(hipMemcpyAsync(c->h_live, c->P.counts + CNT_ACT + (1 - A.in), sizeof(int32_t),
+            HIPCK(hipMemcpyAsync(c->h_live + (it & 1), c->P.counts + CNT_ACT + (1 - A.in), sizeof(int32_t),
                                  hipMemcpyDeviceToHost, c->stream));
-            HIPCK(hipStreamSynchronize(c->stream));
-            live = *c->h_live;
+            HIPCK(hipEventRecord(c->ev_count[it & 1], c->stream));
             A.in = 1 - A.in;
+            it++;
         }
+        c->last_iterations = it;
     }
     HIPCK(hipGetLastError());
     if (c->timing) {
