@@ -365,39 +365,69 @@ NR_D void load_node(const GNode* nodes, int32_t i, float* lmn, float* lmx, float
 // Mesh::hit traversal (hittable/mesh.h:239-292) in near-first order with the
 // child entry distance kept on the stack: popping (ref, t) and testing
 // t < tmax equals the reference's re-test of the popped node's box, because
-// tmin never changes and tmax only shrinks during a traversal.
+// tmin never changes and tmax only shrinks during a traversal.  The near child
+// stays in registers (the reference pushes it and pops it straight back), only
+// the far child goes to the stack.
+// Wave scheduling: a lane that reaches a leaf holds it until NR_LEAF_BATCH
+// lanes of the wave (or all lanes still working) hold one, then the wave runs
+// the triangle test once for all of them.  Each lane still does its own steps
+// in exactly the reference's order; only the interleaving across lanes changes.
 // Returns the BVH slot of the closest triangle (or -1), its t, u, v.
 // `overflow` is set if the stack would exceed its capacity.
+#ifndef NR_LEAF_BATCH
+#define NR_LEAF_BATCH 1
+#endif
 NR_D int32_t mesh_closest(const GMesh& M, const Ray& r, float tmin, float tmax, Stack& st, int sp0,
                           float& bt, float& bu, float& bv, bool& overflow) {
     int32_t best = -1;
     int sp = sp0;
     if (sp0 + 1 > st.cap) { overflow = true; return -1; }
-    st.push(sp, M.root_ref, box_t(M.root_min, M.root_max, r, tmin, tmax));
-    while (sp > sp0) {
-        int32_t ref; float te;
-        st.pop(sp, ref, te);
-        if (!(te < tmax)) continue;
-        if (ref < 0) {
-            int32_t slot = ~ref;
-            const float4* tp = reinterpret_cast<const float4*>(M.tris + slot);
-            float4 a = tp[0], b = tp[1], c = tp[2];
-            float t, u, v;
-            if (tri_hit(r, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), mk3(b.z, b.w, c.x), tmin, tmax, t, u, v)) {
-                best = slot; tmax = t; bt = t; bu = u; bv = v;
+    int32_t ref = M.root_ref;
+    float te = box_t(M.root_min, M.root_max, r, tmin, tmax);
+    bool active = true;     // the lane still has work
+    bool leaf = false;      // the lane holds a leaf to test
+    while (true) {
+        if (active && !leaf) {
+            if (!(te < tmax)) {                        // culled: take the next entry
+                if (sp > sp0) st.pop(sp, ref, te);
+                else active = false;
+            } else if (ref < 0) {
+                leaf = true;
+            } else {
+                float lmn[3], lmx[3], rmn[3], rmx[3]; int32_t l, rr;
+                load_node(M.nodes, ref, lmn, lmx, rmn, rmx, l, rr);
+                float tl = box_t(lmn, lmx, r, tmin, tmax);
+                float tr = box_t(rmn, rmx, r, tmin, tmax);
+                bool left_near = tl < tr;
+                int32_t near_i = left_near ? l : rr, far_i = left_near ? rr : l;
+                float near_t = left_near ? tl : tr, far_t = left_near ? tr : tl;
+                if (sp + 1 > st.cap) {
+                    overflow = true;
+                    active = false;
+                } else {
+                    if (far_t < tmax) st.push(sp, far_i, far_t);
+                    ref = near_i;
+                    te = near_t;
+                }
             }
-        } else {
-            float lmn[3], lmx[3], rmn[3], rmx[3]; int32_t l, rr;
-            load_node(M.nodes, ref, lmn, lmx, rmn, rmx, l, rr);
-            float tl = box_t(lmn, lmx, r, tmin, tmax);
-            float tr = box_t(rmn, rmx, r, tmin, tmax);
-            bool left_near = tl < tr;
-            int32_t near_i = left_near ? l : rr, far_i = left_near ? rr : l;
-            float near_t = left_near ? tl : tr, far_t = left_near ? tr : tl;
-            if (sp + 2 > st.cap) { overflow = true; break; }
-            if (far_t < tmax) st.push(sp, far_i, far_t);
-            st.push(sp, near_i, near_t);
         }
+        unsigned long long held = __ballot(leaf);
+        unsigned long long walking = __ballot(active && !leaf);
+        if (held != 0ull && (__popcll(held) >= NR_LEAF_BATCH || walking == 0ull)) {
+            if (leaf) {
+                int32_t slot = ~ref;
+                const float4* tp = reinterpret_cast<const float4*>(M.tris + slot);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                float t, u, v;
+                if (tri_hit(r, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), mk3(b.z, b.w, c.x), tmin, tmax, t, u, v)) {
+                    best = slot; tmax = t; bt = t; bu = u; bv = v;
+                }
+                leaf = false;
+                if (sp > sp0) st.pop(sp, ref, te);
+                else active = false;
+            }
+        }
+        if (__ballot(active) == 0ull) break;
     }
     return best;
 }
