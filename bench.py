@@ -33,6 +33,10 @@ PATHS_PER_STEP = W * H
 # 23.873 interior nodes (64 B), 1.8217 triangle tests (48 B), 1.9960 top-level
 # objects (128 B), 40 B ray/hit per segment.
 TRAVERSAL_BYTES_PER_PATH = 1942.58
+# HBM bytes k_extend moves per path on this workload, from rocprofv3 FETCH_SIZE (x 1.99992, the factor
+# a 1 GiB device-to-device copy gives) + WRITE_SIZE (x 1.0) over 10,368,000 paths:
+# profiles/k_extend_c2_1080p.md.
+TRAVERSAL_HBM_BYTES_PER_PATH = 1316.2
 HBM_PEAK_GBS = 8000.0
 
 
@@ -63,6 +67,9 @@ def main():
     ap.add_argument("--steps", type=int, default=63)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (a kernel of known traffic for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
@@ -71,6 +78,14 @@ def main():
     K, Wm = args.steps, args.warmup
 
     from nectarrender_amd import abi, native, scenes
+
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     scene, cam, rp = scenes.config2(spp=max(64, K + 1, Wm + 1))
     ctx = native.Context(0)
@@ -98,7 +113,8 @@ def main():
     trav_s = tm[1] / 1e3
     achieved = TRAVERSAL_BYTES_PER_PATH * paths / trav_s / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": round(TRAVERSAL_HBM_BYTES_PER_PATH * paths / n_extend),
             "kernel": "k_extend", "launches": n_extend, "avg_launch_ms": round(tm[1] / n_extend, 4),
             "bytes_per_launch": round(TRAVERSAL_BYTES_PER_PATH * paths / n_extend)}
     out = {
