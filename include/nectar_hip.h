@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define NR_ABI_VERSION 1
+#define NR_ABI_VERSION 2
 
 enum nr_status {
     NR_OK = 0,
@@ -203,6 +203,7 @@ typedef struct nr_post_desc {
     float tm_white_point;
     float tm_alpha;
     int32_t fix_reinhard;  /* 0 keeps the reference's Reinhard no-op (stream.cu:7-11) */
+    int32_t layer;         /* nr_layer streamed (TransferStream::link, stream.h:84-93); 3-channel layers */
 } nr_post_desc;
 
 typedef struct nr_ctx nr_ctx;
@@ -252,8 +253,9 @@ int nr_trace_closest(nr_ctx* ctx, const float* rays, int32_t n, nr_hit* hits);
 /* Same with device-resident rays/hits, enqueued on the context stream. */
 int nr_trace_closest_device(nr_ctx* ctx, const float* d_rays, int32_t n, nr_hit* d_hits);
 
-/* White balance -> tonemap -> gamma -> uint8 CHW of a view's BEAUTY layer
- * (process_stream_kernel, csrc/data/src/stream.cu:146-159).  If out_f32 is
+/* White balance -> tonemap -> gamma -> uint8 CHW of a view's AOV layer
+ * desc->layer (BEAUTY unless relinked) (process_stream_kernel,
+ * csrc/data/src/stream.cu:146-159).  If out_f32 is
  * non-NULL the display-referred float image (before x255) is written too. */
 int nr_post(nr_ctx* ctx, int view, const nr_post_desc* desc, uint8_t* out_u8, float* out_f32);
 

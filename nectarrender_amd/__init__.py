@@ -1,6 +1,11 @@
 """MI355X-native NectarRender path-tracing hot path.
 
 libnectar_hip.so (csrc/nectar_hip.hip) is the HIP implementation behind the C
-ABI of include/nectar_hip.h; `abi` mirrors that ABI in ctypes, `native` loads
-it, `scenes` builds the benchmark scenes.
+ABI of include/nectar_hip.h.  `_pathtracer` (csrc/bindings.cpp) is the
+drop-in for the reference's pybind11 module of the same name; `abi` mirrors
+the C ABI in ctypes, `native` loads it, `scenes` builds the benchmark scenes.
+
+``import nectarrender_amd.pathtracer`` (or putting this directory on
+``sys.path``) makes the module importable as top-level ``_pathtracer``, the
+name the reference's Python package imports.
 """

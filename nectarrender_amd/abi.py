@@ -88,14 +88,17 @@ assert HIT_DTYPE.itemsize == C.sizeof(NrHit) == 64
 class NrPostDesc(C.Structure):
     _fields_ = [("linear_to_gamma", C.c_int32), ("apply_white_balance", C.c_int32), ("wb_temperature", C.c_float),
                 ("wb_tint", C.c_float), ("apply_tonemapping", C.c_int32), ("tm_method", C.c_int32),
-                ("tm_white_point", C.c_float), ("tm_alpha", C.c_float), ("fix_reinhard", C.c_int32)]
+                ("tm_white_point", C.c_float), ("tm_alpha", C.c_float), ("fix_reinhard", C.c_int32),
+                ("layer", C.c_int32)]
 
 
 def post_desc(linear_to_gamma=True, apply_white_balance=True, wb_temperature=6800.0, wb_tint=1.0,
-              apply_tonemapping=True, tm_method=0, tm_white_point=1.0, tm_alpha=1.0, fix_reinhard=False):
+              apply_tonemapping=True, tm_method=0, tm_white_point=1.0, tm_alpha=1.0, fix_reinhard=False,
+              layer=LAYER_BEAUTY):
     """StreamConfig defaults of csrc/data/include/data/stream.h:13-30."""
     return NrPostDesc(int(linear_to_gamma), int(apply_white_balance), wb_temperature, wb_tint,
-                      int(apply_tonemapping), int(tm_method), tm_white_point, tm_alpha, int(fix_reinhard))
+                      int(apply_tonemapping), int(tm_method), tm_white_point, tm_alpha, int(fix_reinhard),
+                      int(layer))
 
 
 @dataclass

@@ -1231,6 +1231,7 @@ static float host_kelvin_channel(float kelvin, int ch) {                        
 int nr_post(nr_ctx* c, int view, const nr_post_desc* d, uint8_t* out8, float* outf) {
     if (!c || !d || view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_INVALID, "bad view");
     View& v = c->views[view];
+    if (d->layer < 0 || d->layer > 8 || !v.layer[d->layer]) return fail(NR_ERR_INVALID, "post: layer not allocated");
     HIPCK(hipSetDevice(c->device));
     uint32_t HW = v.H * v.W;
     PostArgs A{};
@@ -1248,7 +1249,7 @@ int nr_post(nr_ctx* c, int view, const nr_post_desc* d, uint8_t* out8, float* ou
         if (d8) (void)hipFree(d8);
         return fail(NR_ERR_HIP, "hipMalloc post");
     }
-    hipLaunchKernelGGL(k_post, dim3((HW + 255) / 256), dim3(256), 0, c->stream, v.layer[NR_LAYER_BEAUTY], HW, A, d8, df);
+    hipLaunchKernelGGL(k_post, dim3((HW + 255) / 256), dim3(256), 0, c->stream, v.layer[d->layer], HW, A, d8, df);
     int rc = NR_OK;
     if (hipGetLastError() != hipSuccess) rc = fail(NR_ERR_HIP, "k_post launch");
     if (rc == NR_OK && out8 && hipMemcpyAsync(out8, d8, 3 * (size_t)HW, hipMemcpyDeviceToHost, c->stream) != hipSuccess) rc = fail(NR_ERR_HIP, "copy post u8");

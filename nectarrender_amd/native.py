@@ -52,7 +52,7 @@ def lib() -> C.CDLL:
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
-    if L.nr_abi_version() != 1:
+    if L.nr_abi_version() != 2:
         raise NectarError("ABI version mismatch")
     _lib = L
     return L
