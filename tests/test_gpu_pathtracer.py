@@ -145,8 +145,9 @@ def test_mesh_from_obj_file_renders_like_scenedesc(tmp_path, oracle):
     path = tmp_path / "ico.obj"
     path.write_text("\n".join(lines) + "\n")
     verts, tris = P.io.load_obj(str(path))
-    mesh = Hit.mesh(str(path), V3(0, 0.3, 0), V3(10, 20, 0), V3(0.6), Mat.pbr(albedo=Col(0.9, 0.6, 0.3),
-                                                                             roughness=Col(0.3, 0.3, 0.3)))
+    Tex = P.material.texture.Texture
+    mat = Mat.pbr(albedo=Tex.from_color(Col(0.9, 0.6, 0.3)), roughness=Tex.from_color(Col(0.3, 0.3, 0.3)))
+    mesh = Hit.mesh(str(path), V3(0, 0.3, 0), V3(10, 20, 0), V3(0.6), mat)
     ground = Hit.quad(V3(0, -0.4, 0), V3(0, 0, 0), V3(4.0), Mat.lambertian(Col(0.5, 0.5, 0.5)))
     scene = P.scene.Scene([ground, mesh], [], P.light.Skylight.simple(Col(1, 1, 1), Col(0.5, 0.7, 1.0)))
     cp = P.engine.camera.CameraParams(P.core.vector.Vector2(64, 64), V3(0, 0.5, 2.5), V3(-8, 0, 0), 4, 2.0, 2.5, 0.0,
