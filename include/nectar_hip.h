@@ -240,6 +240,9 @@ int nr_aov_reset(nr_ctx* ctx, int view);
  * context's stream; nr_sync waits. */
 int nr_render(nr_ctx* ctx, int view, const nr_render_desc* desc);
 
+/* Image size of a view (H = resolution[0], W = resolution[1]). */
+int nr_view_size(nr_ctx* ctx, int view, int32_t* H, int32_t* W);
+
 /* Copy one AOV layer (C x H x W fp32) of a view to host memory. */
 int nr_aov_read(nr_ctx* ctx, int view, int layer, float* host_chw, size_t n_floats);
 

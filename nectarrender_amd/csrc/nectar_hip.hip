@@ -36,12 +36,19 @@
 #include "../../include/nectar_hip.h"
 #include "nr_bvh.h"
 #include "nr_device.h"
+#include "nr_error.h"
 #include "nr_types.h"
 
 // ---------------------------------------------------------------------------
 // error plumbing
 // ---------------------------------------------------------------------------
-static thread_local std::string g_err;
+// message of the last failure on this thread, shared by every translation unit
+// of the library (nr_error.h)
+std::string& nr_error_slot() {
+    static thread_local std::string slot;
+    return slot;
+}
+#define g_err nr_error_slot()
 static int fail(int code, const std::string& msg) { g_err = msg; return code; }
 #define HIPCK(expr)                                                                      \
     do {                                                                                 \
@@ -1140,6 +1147,13 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
         HIPCK(hipMemcpy(&err, c->P.counts + CNT_ERR, sizeof(int32_t), hipMemcpyDeviceToHost));
         if (err) return fail(NR_ERR_LIMIT, "traversal stack overflow");
     }
+    return NR_OK;
+}
+
+int nr_view_size(nr_ctx* c, int view, int32_t* H, int32_t* W) {
+    if (!c || !H || !W || view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_INVALID, "bad view");
+    *H = (int32_t)c->views[view].H;
+    *W = (int32_t)c->views[view].W;
     return NR_OK;
 }
 

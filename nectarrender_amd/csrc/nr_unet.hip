@@ -1,0 +1,862 @@
+// nr_unet.hip — learned U-Net denoiser for gfx950 (include/nectar_denoise.h).
+//
+// Tensors are NHWC bf16 ("views": base, channel stride, channel offset), so a
+// skip concatenation is two producers writing disjoint channel ranges of one
+// buffer.  Every 3x3 convolution is an implicit GEMM on the matrix cores:
+//   forward  D[co][pix] = sum_k W[co][k] X[k][pix], k = (tap, ci)
+//   input gradient: the same kernel with the weights flipped and transposed
+//   weight gradient dW[co][tap][ci] = sum_pix G[co][pix] X[ci][pix+tap]
+// using v_mfma_f32_16x16x32_bf16 (16 x 16 output tile, K = 32, fp32
+// accumulation).  Forward/input-gradient tiles read 8 consecutive channels
+// (16 B) per lane straight from HBM/L2; the weight gradient transposes
+// 32-pixel tiles through LDS so the reduction runs over pixels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/nectar_denoise.h"
+#include "nr_error.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
+typedef float f32x4 __attribute__((__vector_size__(4 * sizeof(float))));
+
+int fail(int code, const std::string& m) { nr_error_slot() = m; return code; }
+#define UCK(expr)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) return fail(NR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+__host__ __device__ inline uint16_t f2bf(float f) {          // round to nearest even
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+struct View {             // element (pixel p, channel c) at p[pix * cs + off + c]
+    uint16_t* p;
+    int cs, off;
+};
+
+// ---------------------------------------------------------------------------
+// convolution (forward and input gradient)
+// ---------------------------------------------------------------------------
+struct ConvArgs {
+    const uint16_t* in; int in_cs, in_off, cin_g;   // cin_g groups of 8 input channels
+    int N, H, W;
+    const uint16_t* w; int kg_pad;                  // w[co][kg_pad * 8], kg = tap * cin_g + c8
+    const float* bias;                              // may be null
+    int cout;                                       // channels stored
+    int relu;
+    uint16_t* out; int out_cs, out_off;
+    const uint16_t* mask; int mask_cs, mask_off;    // result *= (mask > 0) when non-null
+    float* out_f32; const float* resid;             // final layer: out_f32[p*4+c] = acc + bias + resid[p*4+c]
+};
+
+constexpr int CONV_WAVES = 4;
+
+template <int TM, int TN>
+__global__ void __launch_bounds__(64 * CONV_WAVES) k_conv(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int P = a.N * a.H * a.W;
+    const int pbase = (blockIdx.x * CONV_WAVES + wave) * 16 * TN;
+    if (pbase >= P) return;
+    const int cobase = blockIdx.y * 16 * TM;
+    const int r = lane & 15, kq = lane >> 4;
+    const int kg_total = 9 * a.cin_g;
+    int py[TN], px[TN];
+    bool pv[TN];
+    for (int n = 0; n < TN; n++) {
+        int p = pbase + n * 16 + r;
+        pv[n] = p < P;
+        int q = pv[n] ? p : 0;
+        px[n] = q % a.W;
+        py[n] = (q / a.W) % a.H;
+    }
+    f32x4 acc[TM][TN];
+    for (int m = 0; m < TM; m++)
+        for (int n = 0; n < TN; n++) acc[m][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int ksteps = a.kg_pad >> 2;
+    for (int ks = 0; ks < ksteps; ks++) {
+        const int g = ks * 4 + kq;
+        bf16x8 bfrag[TN];
+        const bool gv = g < kg_total;
+        const int tap = gv ? g / a.cin_g : 0, c8 = gv ? g - tap * a.cin_g : 0;
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        for (int n = 0; n < TN; n++) {
+            const int yy = py[n] + dy, xx = px[n] + dx;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gv && pv[n] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+                const int q = pbase + n * 16 + r + dy * a.W + dx;
+                v = *reinterpret_cast<const uint4*>(a.in + (size_t)q * a.in_cs + a.in_off + c8 * 8);
+            }
+            bfrag[n] = __builtin_bit_cast(bf16x8, v);
+        }
+        for (int m = 0; m < TM; m++) {
+            const int co = cobase + m * 16 + r;
+            uint4 v = *reinterpret_cast<const uint4*>(a.w + ((size_t)co * a.kg_pad + g) * 8);
+            bf16x8 afrag = __builtin_bit_cast(bf16x8, v);
+            for (int n = 0; n < TN; n++)
+                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag[n], acc[m][n], 0, 0, 0);
+        }
+    }
+    // D[co][pix]: lane holds column pix = r, rows co = 4 * kq + j
+    for (int m = 0; m < TM; m++)
+        for (int n = 0; n < TN; n++) {
+            if (!pv[n]) continue;
+            const int p = pbase + n * 16 + r;
+            const int co0 = cobase + m * 16 + kq * 4;
+            float v[4];
+            for (int j = 0; j < 4; j++) {
+                const int co = co0 + j;
+                float x = acc[m][n][j] + ((a.bias && co < a.cout) ? a.bias[co] : 0.0f);
+                if (a.relu) x = fmaxf(x, 0.0f);
+                if (a.mask && co < a.cout && !(bf2f(a.mask[(size_t)p * a.mask_cs + a.mask_off + co]) > 0.0f)) x = 0.0f;
+                v[j] = x;
+            }
+            if (a.out_f32) {
+                for (int j = 0; j < 4; j++) {
+                    const int co = co0 + j;
+                    if (co < a.cout && co < 4) a.out_f32[(size_t)p * 4 + co] = v[j] + a.resid[(size_t)p * 4 + co];
+                }
+            } else {
+                uint16_t* o = a.out + (size_t)p * a.out_cs + a.out_off + co0;
+                if (co0 + 3 < a.cout) {
+                    uint2 pk;
+                    pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+                    pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                    *reinterpret_cast<uint2*>(o) = pk;
+                } else {
+                    for (int j = 0; j < 4; j++)
+                        if (co0 + j < a.cout) o[j] = f2bf(v[j]);
+                }
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient: one wave per (tap, 16 co, 16 ci) block over a pixel range
+// ---------------------------------------------------------------------------
+struct WgradArgs {
+    const uint16_t* x; int x_cs, x_off, cin;
+    const uint16_t* g; int g_cs, g_off, cout;
+    int N, H, W;
+    int nci;              // ci blocks of 16
+    int chunks_per_wg;    // 32-pixel chunks per workgroup
+    float* dw;            // [cout][9][cin]
+};
+
+__global__ void __launch_bounds__(64) k_wgrad(WgradArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t gT[16 * 40];   // [co][pix], rows padded to 40
+    __shared__ __attribute__((aligned(16))) uint16_t xT[16 * 40];   // [ci][pix]
+    const int lane = threadIdx.x;
+    const int tap = blockIdx.y;
+    const int co0 = (blockIdx.z / a.nci) * 16, ci0 = (blockIdx.z % a.nci) * 16;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int P = a.N * a.H * a.W;
+    const int nchunks = (P + 31) / 32;
+    const int c_beg = blockIdx.x * a.chunks_per_wg;
+    const int c_end = min(nchunks, c_beg + a.chunks_per_wg);
+    const int lp = lane >> 1, half = lane & 1;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c = c_beg; c < c_end; c++) {
+        const int p = c * 32 + lp;
+        uint4 gv = make_uint4(0u, 0u, 0u, 0u), xv = make_uint4(0u, 0u, 0u, 0u);
+        if (p < P) {
+            gv = *reinterpret_cast<const uint4*>(a.g + (size_t)p * a.g_cs + a.g_off + co0 + half * 8);
+            const int px = p % a.W, py = (p / a.W) % a.H;
+            const int yy = py + dy, xx = px + dx;
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                xv = *reinterpret_cast<const uint4*>(a.x + (size_t)(p + dy * a.W + dx) * a.x_cs + a.x_off + ci0 + half * 8);
+        }
+        const uint16_t* gs = reinterpret_cast<const uint16_t*>(&gv);
+        const uint16_t* xs = reinterpret_cast<const uint16_t*>(&xv);
+        for (int j = 0; j < 8; j++) {
+            gT[(half * 8 + j) * 40 + lp] = gs[j];
+            xT[(half * 8 + j) * 40 + lp] = xs[j];
+        }
+        __syncthreads();
+        const int r = lane & 15, kq = lane >> 4;
+        uint4 av = *reinterpret_cast<const uint4*>(&gT[r * 40 + kq * 8]);
+        uint4 bv = *reinterpret_cast<const uint4*>(&xT[r * 40 + kq * 8]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv),
+                                                      acc, 0, 0, 0);
+        __syncthreads();
+    }
+    const int ci = ci0 + (lane & 15);
+    for (int j = 0; j < 4; j++) {
+        const int co = co0 + (lane >> 4) * 4 + j;
+        if (co < a.cout && ci < a.cin) atomicAdd(&a.dw[((size_t)co * 9 + tap) * a.cin + ci], acc[j]);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bias_grad(const uint16_t* g, int g_cs, int g_off, int cout, int cpad, int P,
+                                                   int pix_per_block, float* db) {
+    const int lanes_per_pix = cpad;
+    const int phases = 256 / lanes_per_pix;
+    const int t = threadIdx.x;
+    const int c = t % lanes_per_pix, ph = t / lanes_per_pix;
+    if (ph >= phases) return;
+    const int p0 = blockIdx.x * pix_per_block;
+    const int p1 = min(P, p0 + pix_per_block);
+    float s = 0.0f;
+    for (int p = p0 + ph; p < p1; p += phases) s += bf2f(g[(size_t)p * g_cs + g_off + c]);
+    if (c < cout) atomicAdd(&db[c], s);
+}
+
+// ---------------------------------------------------------------------------
+// pooling, upsampling, input/output transforms, loss, optimizer
+// ---------------------------------------------------------------------------
+// 2x2 average pool; src dims H x W, dst H/2 x W/2
+__global__ void k_pool(View s, View d, int C, int N, int H, int W) {
+    const int Ho = H >> 1, Wo = W >> 1;
+    const size_t total = (size_t)N * Ho * Wo * C;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = (int)(i % C);
+    const size_t q = i / C;
+    const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((size_t)Wo * Ho));
+    const size_t p00 = ((size_t)n * H + 2 * yo) * W + 2 * xo;
+    float v = bf2f(s.p[p00 * s.cs + s.off + c]) + bf2f(s.p[(p00 + 1) * s.cs + s.off + c]) +
+              bf2f(s.p[(p00 + W) * s.cs + s.off + c]) + bf2f(s.p[(p00 + W + 1) * s.cs + s.off + c]);
+    d.p[q * d.cs + d.off + c] = f2bf(0.25f * v);
+}
+
+// d[p] = (skip[p] + 0.25 * gdown[p/2]) * (mask[p] > 0); dims of d: H x W
+__global__ void k_pool_bwd(View gdown, View skip, View mask, View d, int C, int N, int H, int W) {
+    const size_t total = (size_t)N * H * W * C;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = (int)(i % C);
+    const size_t p = i / C;
+    const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
+    const size_t q = ((size_t)n * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1);
+    float v = 0.25f * bf2f(gdown.p[q * gdown.cs + gdown.off + c]);
+    if (skip.p) v += bf2f(skip.p[p * skip.cs + skip.off + c]);
+    if (!(bf2f(mask.p[p * mask.cs + mask.off + c]) > 0.0f)) v = 0.0f;
+    d.p[p * d.cs + d.off + c] = f2bf(v);
+}
+
+// nearest 2x upsample; dst dims H x W, src H/2 x W/2
+__global__ void k_up(View s, View d, int C, int N, int H, int W) {
+    const size_t total = (size_t)N * H * W * C;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = (int)(i % C);
+    const size_t p = i / C;
+    const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
+    const size_t q = ((size_t)n * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1);
+    d.p[p * d.cs + d.off + c] = s.p[q * s.cs + s.off + c];
+}
+
+// d[q] = sum of g over the 2x2 children of q, times (mask[q] > 0); dst dims Ho x Wo
+__global__ void k_up_bwd(View g, View mask, View d, int C, int N, int Ho, int Wo) {
+    const size_t total = (size_t)N * Ho * Wo * C;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = (int)(i % C);
+    const size_t q = i / C;
+    const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((size_t)Wo * Ho));
+    const int W = Wo * 2;
+    const size_t p00 = ((size_t)n * (Ho * 2) + 2 * yo) * W + 2 * xo;
+    float v = bf2f(g.p[p00 * g.cs + g.off + c]) + bf2f(g.p[(p00 + 1) * g.cs + g.off + c]) +
+              bf2f(g.p[(p00 + W) * g.cs + g.off + c]) + bf2f(g.p[(p00 + W + 1) * g.cs + g.off + c]);
+    if (!(bf2f(mask.p[q * mask.cs + mask.off + c]) > 0.0f)) v = 0.0f;
+    d.p[q * d.cs + d.off + c] = f2bf(v);
+}
+
+// planar N x 3 x H x W inputs -> X0 (16 channels, padded dims Hp x Wp) and log1p beauty (4-stride f32)
+__global__ void k_prep(const float* beauty, const float* albedo, const float* normal, int N, int H, int W, int Hp,
+                       int Wp, uint16_t* x0, float* xlog) {
+    const size_t total = (size_t)N * Hp * Wp;
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= total) return;
+    const int x = (int)(p % Wp), y = (int)((p / Wp) % Hp), n = (int)(p / ((size_t)Wp * Hp));
+    float f[16];
+    for (int k = 0; k < 16; k++) f[k] = 0.0f;
+    if (x < W && y < H) {
+        const size_t hw = (size_t)H * W, base = (size_t)n * 3 * hw + (size_t)y * W + x;
+        for (int c = 0; c < 3; c++) {
+            float b = beauty[base + c * hw];
+            f[c] = log1pf(fmaxf(b, 0.0f));          // NaN samples become 0
+            f[3 + c] = log1pf(fmaxf(albedo[base + c * hw], 0.0f));
+            float nv = normal[base + c * hw];
+            f[6 + c] = (nv == nv) ? fminf(fmaxf(nv, -1.0f), 1.0f) : 0.0f;
+        }
+    }
+    uint32_t pk[8];
+    for (int k = 0; k < 8; k++) pk[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+    uint4* o = reinterpret_cast<uint4*>(x0 + p * 16);
+    o[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+    o[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
+    for (int c = 0; c < 4; c++) xlog[p * 4 + c] = c < 3 ? f[c] : 0.0f;
+}
+
+__global__ void k_target(const float* t, int N, int H, int W, float* tlog) {
+    const size_t total = (size_t)N * H * W;
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= total) return;
+    const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
+    const size_t hw = (size_t)H * W, base = (size_t)n * 3 * hw + (size_t)y * W + x;
+    for (int c = 0; c < 4; c++) tlog[p * 4 + c] = c < 3 ? log1pf(fmaxf(t[base + c * hw], 0.0f)) : 0.0f;
+}
+
+// L1 loss on log radiance; gradient into a 16-channel bf16 tensor; per-block partial sums
+__global__ void __launch_bounds__(256) k_loss(const float* y, const float* t, int P, float inv_count, uint16_t* gy,
+                                              float* partial) {
+    __shared__ float red[256];
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float s = 0.0f;
+    if (p < (size_t)P) {
+        uint16_t g[16];
+        for (int c = 0; c < 16; c++) g[c] = 0;
+        for (int c = 0; c < 3; c++) {
+            float d = y[p * 4 + c] - t[p * 4 + c];
+            s += fabsf(d);
+            g[c] = f2bf(d > 0.0f ? inv_count : (d < 0.0f ? -inv_count : 0.0f));
+        }
+        uint32_t pk[8];
+        for (int k = 0; k < 8; k++) pk[k] = (uint32_t)g[2 * k] | ((uint32_t)g[2 * k + 1] << 16);
+        uint4* o = reinterpret_cast<uint4*>(gy + p * 16);
+        o[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+        o[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_output(const float* y, int N, int H, int W, int Hp, int Wp, float* out) {
+    const size_t total = (size_t)N * H * W;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int x = (int)(i % W), yy = (int)((i / W) % H), n = (int)(i / ((size_t)W * H));
+    const size_t p = ((size_t)n * Hp + yy) * Wp + x;
+    const size_t hw = (size_t)H * W, base = (size_t)n * 3 * hw + (size_t)yy * W + x;
+    for (int c = 0; c < 3; c++) out[base + c * hw] = expm1f(fmaxf(y[p * 4 + c], 0.0f));
+}
+
+__global__ void k_adam(float* prm, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
+                       float c1, float c2, float eps) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.0f - b1) * gi;
+    float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    prm[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// fp32 [cout][9][cin] -> forward bf16 [cout_pad][kg_pad*8] (kg = tap*cin_g + c8)
+__global__ void k_pack_fwd(const float* w, int cout, int cin, int cin_g, int cout_pad, int kg_pad, uint16_t* o) {
+    const size_t total = (size_t)cout_pad * kg_pad * 8;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int co = (int)(i / ((size_t)kg_pad * 8));
+    const int k = (int)(i % ((size_t)kg_pad * 8));
+    const int g = k >> 3, j = k & 7;
+    float v = 0.0f;
+    if (co < cout && g < 9 * cin_g) {
+        const int tap = g / cin_g, ci = (g % cin_g) * 8 + j;
+        if (ci < cin) v = w[((size_t)co * 9 + tap) * cin + ci];
+    }
+    o[i] = f2bf(v);
+}
+
+// input-gradient weights: rows ci, groups g' = tap' * cout_g + c8' over the (padded) output channels,
+// element W[co = c8'*8 + j][8 - tap'][ci]
+__global__ void k_pack_bwd(const float* w, int cout, int cin, int cout_g, int cin_pad, int kg_pad, uint16_t* o) {
+    const size_t total = (size_t)cin_pad * kg_pad * 8;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int ci = (int)(i / ((size_t)kg_pad * 8));
+    const int k = (int)(i % ((size_t)kg_pad * 8));
+    const int g = k >> 3, j = k & 7;
+    float v = 0.0f;
+    if (ci < cin && g < 9 * cout_g) {
+        const int tap = g / cout_g, co = (g % cout_g) * 8 + j;
+        if (co < cout) v = w[((size_t)co * 9 + (8 - tap)) * cin + ci];
+    }
+    o[i] = f2bf(v);
+}
+
+inline int up16(int x) { return (x + 15) & ~15; }
+inline int up4(int x) { return (x + 3) & ~3; }
+inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// network description
+// ---------------------------------------------------------------------------
+constexpr int LV = 5;
+constexpr int CH[LV] = {32, 48, 64, 96, 128};
+constexpr int IN_CH = 9, IN_PAD = 16;
+
+struct Layer {
+    int cin, cout;            // actual channels of the parameters
+    int cin_t;                // channels of the input tensor the forward kernel reads (multiple of 8)
+    size_t w_off, b_off;      // offsets into the flat parameter vector
+    int cout_pad, kg_pad;     // forward packing
+    int cin_pad, kgb_pad;     // input-gradient packing
+    uint16_t* wf = nullptr;
+    uint16_t* wb = nullptr;
+};
+
+struct Net {
+    std::vector<Layer> L;     // enc0a..enc4b (10), dec3a..dec0b (8), out (1)
+    size_t n_params = 0;
+    Net() {
+        auto add = [&](int cin, int cout, int cin_t) {
+            Layer l{};
+            l.cin = cin; l.cout = cout; l.cin_t = cin_t;
+            l.w_off = n_params; n_params += (size_t)cout * 9 * cin;
+            l.b_off = n_params; n_params += (size_t)cout;
+            l.cout_pad = up16(cout);
+            l.kg_pad = up4(9 * (cin_t / 8));
+            l.cin_pad = up16(cin);
+            l.kgb_pad = up4(9 * (l.cout_pad / 8));
+            L.push_back(l);
+        };
+        int prev = IN_CH, prev_t = IN_PAD;
+        for (int l = 0; l < LV; l++) {
+            add(prev, CH[l], prev_t);
+            add(CH[l], CH[l], CH[l]);
+            prev = prev_t = CH[l];
+        }
+        for (int l = LV - 2; l >= 0; l--) {
+            add(CH[l] + CH[l + 1], CH[l], CH[l] + CH[l + 1]);
+            add(CH[l], CH[l], CH[l]);
+        }
+        add(CH[0], 3, CH[0]);
+    }
+    Layer& enc(int l, int b) { return L[2 * l + b]; }
+    Layer& dec(int l, int b) { return L[10 + 2 * (LV - 2 - l) + b]; }
+    Layer& out() { return L[18]; }
+};
+
+}  // namespace
+
+struct nr_unet {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Net net;
+    float* prm = nullptr;      // fp32 master parameters
+    float* grad = nullptr;
+    float* m = nullptr;
+    float* v = nullptr;
+    uint16_t* wmem = nullptr;  // packed bf16 weights of all layers
+    int adam_t = 0;
+    // activations for the current shape
+    int N = 0, H = 0, W = 0;
+    void* amem = nullptr;
+    size_t amem_bytes = 0;
+    uint16_t *X[LV] = {}, *A[LV] = {}, *D[LV] = {}, *DA[LV] = {}, *DO[LV] = {}, *E4 = nullptr;
+    uint16_t *gX[LV] = {}, *gA[LV] = {}, *gE[LV] = {}, *gD[LV] = {}, *gDA[LV] = {}, *gDO[LV] = {}, *gY = nullptr;
+    float *xlog = nullptr, *y = nullptr, *tlog = nullptr, *partial = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float last_ms = 0.0f;
+    double last_flops = 0.0;
+};
+
+namespace {
+
+int pack_all(nr_unet* u) {
+    for (Layer& l : u->net.L) {
+        size_t nf = (size_t)l.cout_pad * l.kg_pad * 8;
+        hipLaunchKernelGGL(k_pack_fwd, dim3(blocks(nf, 256)), dim3(256), 0, u->stream, u->prm + l.w_off, l.cout, l.cin,
+                           l.cin_t / 8, l.cout_pad, l.kg_pad, l.wf);
+        size_t nb = (size_t)l.cin_pad * l.kgb_pad * 8;
+        hipLaunchKernelGGL(k_pack_bwd, dim3(blocks(nb, 256)), dim3(256), 0, u->stream, u->prm + l.w_off, l.cout, l.cin,
+                           l.cout_pad / 8, l.cin_pad, l.kgb_pad, l.wb);
+    }
+    UCK(hipGetLastError());
+    return NR_OK;
+}
+
+// (Re)allocate activation and gradient tensors for N images of H x W (multiples of 16).
+int ensure_shape(nr_unet* u, int N, int H, int W) {
+    if (u->N == N && u->H == H && u->W == W) return NR_OK;
+    if (u->amem) { UCK(hipFree(u->amem)); u->amem = nullptr; }
+    size_t off = 0;
+    std::vector<std::pair<void**, size_t>> plan;
+    auto want = [&](void** p, size_t bytes) { plan.push_back({p, bytes}); };
+    for (int l = 0; l < LV; l++) {
+        const size_t P = (size_t)N * (H >> l) * (W >> l);
+        const int cin = l == 0 ? IN_PAD : CH[l - 1];
+        want((void**)&u->X[l], P * cin * 2);
+        want((void**)&u->A[l], P * CH[l] * 2);
+        want((void**)&u->gX[l], P * cin * 2);
+        want((void**)&u->gA[l], P * CH[l] * 2);
+        want((void**)&u->gE[l], P * CH[l] * 2);
+        if (l < LV - 1) {
+            const int cd = CH[l] + CH[l + 1];
+            want((void**)&u->D[l], P * cd * 2);
+            want((void**)&u->gD[l], P * cd * 2);
+            want((void**)&u->DA[l], P * CH[l] * 2);
+            want((void**)&u->DO[l], P * CH[l] * 2);
+            want((void**)&u->gDA[l], P * CH[l] * 2);
+            want((void**)&u->gDO[l], P * CH[l] * 2);
+        } else {
+            want((void**)&u->E4, P * CH[l] * 2);
+        }
+    }
+    const size_t P0 = (size_t)N * H * W;
+    want((void**)&u->gY, P0 * 16 * 2);
+    want((void**)&u->xlog, P0 * 4 * 4);
+    want((void**)&u->y, P0 * 4 * 4);
+    want((void**)&u->tlog, P0 * 4 * 4);
+    want((void**)&u->partial, (P0 / 256 + 1) * 4);
+    for (auto& pr : plan) off += (pr.second + 255) & ~(size_t)255;
+    UCK(hipMalloc(&u->amem, off));
+    u->amem_bytes = off;
+    size_t o = 0;
+    for (auto& pr : plan) {
+        *pr.first = (char*)u->amem + o;
+        o += (pr.second + 255) & ~(size_t)255;
+    }
+    u->N = N; u->H = H; u->W = W;
+    return NR_OK;
+}
+
+double conv_flops(const Layer& l, size_t P) { return 2.0 * P * 9.0 * l.cin * l.cout; }
+
+void launch_conv(nr_unet* u, const Layer& l, bool backward, View in, int cin_groups, int N, int H, int W, View out,
+                 int cout, bool relu, const float* bias, View mask, float* out_f32, const float* resid) {
+    ConvArgs a{};
+    a.in = in.p; a.in_cs = in.cs; a.in_off = in.off; a.cin_g = cin_groups;
+    a.N = N; a.H = H; a.W = W;
+    a.w = backward ? l.wb : l.wf;
+    a.kg_pad = backward ? l.kgb_pad : l.kg_pad;
+    a.bias = bias;
+    a.cout = cout;
+    a.relu = relu;
+    a.out = out.p; a.out_cs = out.cs; a.out_off = out.off;
+    a.mask = mask.p; a.mask_cs = mask.cs; a.mask_off = mask.off;
+    a.out_f32 = out_f32; a.resid = resid;
+    const int cpad = up16(cout);
+    const int P = N * H * W;
+    const int TN = 4;
+    const int pix_per_wg = 16 * TN * CONV_WAVES;
+    dim3 block(64 * CONV_WAVES);
+    // output-channel tiles per wave: 16 * TM divides cpad
+    int TM = (cpad % 64 == 0) ? 4 : (cpad % 48 == 0) ? 3 : (cpad % 32 == 0) ? 2 : 1;
+    dim3 grid((P + pix_per_wg - 1) / pix_per_wg, cpad / (16 * TM));
+    switch (TM) {
+        case 4: hipLaunchKernelGGL((k_conv<4, TN>), grid, block, 0, u->stream, a); break;
+        case 3: hipLaunchKernelGGL((k_conv<3, TN>), grid, block, 0, u->stream, a); break;
+        case 2: hipLaunchKernelGGL((k_conv<2, TN>), grid, block, 0, u->stream, a); break;
+        default: hipLaunchKernelGGL((k_conv<1, TN>), grid, block, 0, u->stream, a); break;
+    }
+}
+
+void launch_wgrad(nr_unet* u, const Layer& l, View x, int cin, View g, int cout, int N, int H, int W) {
+    WgradArgs a{};
+    a.x = x.p; a.x_cs = x.cs; a.x_off = x.off; a.cin = cin;
+    a.g = g.p; a.g_cs = g.cs; a.g_off = g.off; a.cout = cout;
+    a.N = N; a.H = H; a.W = W;
+    a.nci = up16(cin) / 16;
+    const int nco = up16(cout) / 16;
+    const int P = N * H * W;
+    const int nchunks = (P + 31) / 32;
+    // about 2048 workgroups per layer
+    const int per_tap_blocks = 9 * nco * a.nci;
+    int want_x = std::max(1, 2048 / per_tap_blocks);
+    a.chunks_per_wg = std::max(4, (nchunks + want_x - 1) / want_x);
+    a.dw = u->grad + l.w_off;
+    dim3 grid((nchunks + a.chunks_per_wg - 1) / a.chunks_per_wg, 9, nco * a.nci);
+    hipLaunchKernelGGL(k_wgrad, grid, dim3(64), 0, u->stream, a);
+    const int cpad = up16(cout);
+    const int ppb = 4096;
+    hipLaunchKernelGGL(k_bias_grad, dim3(blocks(P, ppb)), dim3(256), 0, u->stream, g.p, g.cs, g.off, cout, cpad, P, ppb,
+                       u->grad + l.b_off);
+}
+
+void forward(nr_unet* u, int N, int H, int W) {
+    Net& n = u->net;
+    const View none{nullptr, 0, 0};
+    for (int l = 0; l < LV; l++) {
+        const int h = H >> l, w = W >> l;
+        const int cin_t = l == 0 ? IN_PAD : CH[l - 1];
+        View x{u->X[l], cin_t, 0}, a{u->A[l], CH[l], 0};
+        View e = l < LV - 1 ? View{u->D[l], CH[l] + CH[l + 1], 0} : View{u->E4, CH[l], 0};
+        Layer& la = n.enc(l, 0);
+        Layer& lb = n.enc(l, 1);
+        launch_conv(u, la, false, x, cin_t / 8, N, h, w, a, CH[l], true, u->prm + la.b_off, none, nullptr, nullptr);
+        launch_conv(u, lb, false, a, CH[l] / 8, N, h, w, e, CH[l], true, u->prm + lb.b_off, none, nullptr, nullptr);
+        if (l < LV - 1) {
+            View xn{u->X[l + 1], CH[l], 0};
+            size_t tot = (size_t)N * (h >> 1) * (w >> 1) * CH[l];
+            hipLaunchKernelGGL(k_pool, dim3(blocks(tot, 256)), dim3(256), 0, u->stream, e, xn, CH[l], N, h, w);
+        }
+    }
+    for (int l = LV - 2; l >= 0; l--) {
+        const int h = H >> l, w = W >> l;
+        const int cd = CH[l] + CH[l + 1];
+        View src = l == LV - 2 ? View{u->E4, CH[l + 1], 0} : View{u->DO[l + 1], CH[l + 1], 0};
+        View up{u->D[l], cd, CH[l]};
+        size_t tot = (size_t)N * h * w * CH[l + 1];
+        hipLaunchKernelGGL(k_up, dim3(blocks(tot, 256)), dim3(256), 0, u->stream, src, up, CH[l + 1], N, h, w);
+        Layer& la = n.dec(l, 0);
+        Layer& lb = n.dec(l, 1);
+        launch_conv(u, la, false, View{u->D[l], cd, 0}, cd / 8, N, h, w, View{u->DA[l], CH[l], 0}, CH[l], true,
+                    u->prm + la.b_off, none, nullptr, nullptr);
+        launch_conv(u, lb, false, View{u->DA[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->DO[l], CH[l], 0}, CH[l], true,
+                    u->prm + lb.b_off, none, nullptr, nullptr);
+    }
+    Layer& lo = n.out();
+    launch_conv(u, lo, false, View{u->DO[0], CH[0], 0}, CH[0] / 8, N, H, W, none, 3, false, u->prm + lo.b_off, none,
+                u->y, u->xlog);
+}
+
+void backward(nr_unet* u, int N, int H, int W) {
+    Net& n = u->net;
+    const View none{nullptr, 0, 0};
+    Layer& lo = n.out();
+    // output layer
+    launch_wgrad(u, lo, View{u->DO[0], CH[0], 0}, CH[0], View{u->gY, 16, 0}, 3, N, H, W);
+    launch_conv(u, lo, true, View{u->gY, 16, 0}, 2, N, H, W, View{u->gDO[0], CH[0], 0}, CH[0], false, nullptr,
+                View{u->DO[0], CH[0], 0}, nullptr, nullptr);
+    // decoder, finest level first
+    for (int l = 0; l < LV - 1; l++) {
+        const int h = H >> l, w = W >> l;
+        const int cd = CH[l] + CH[l + 1];
+        Layer& la = n.dec(l, 0);
+        Layer& lb = n.dec(l, 1);
+        launch_wgrad(u, lb, View{u->DA[l], CH[l], 0}, CH[l], View{u->gDO[l], CH[l], 0}, CH[l], N, h, w);
+        launch_conv(u, lb, true, View{u->gDO[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->gDA[l], CH[l], 0}, CH[l], false,
+                    nullptr, View{u->DA[l], CH[l], 0}, nullptr, nullptr);
+        launch_wgrad(u, la, View{u->D[l], cd, 0}, cd, View{u->gDA[l], CH[l], 0}, CH[l], N, h, w);
+        launch_conv(u, la, true, View{u->gDA[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->gD[l], cd, 0}, cd, false,
+                    nullptr, none, nullptr, nullptr);
+        // upsampled half of the concatenation -> the coarser decoder output (or the bottleneck)
+        View src_mask = l == LV - 2 ? View{u->E4, CH[l + 1], 0} : View{u->DO[l + 1], CH[l + 1], 0};
+        View dst = l == LV - 2 ? View{u->gE[LV - 1], CH[l + 1], 0} : View{u->gDO[l + 1], CH[l + 1], 0};
+        size_t tot = (size_t)N * (h >> 1) * (w >> 1) * CH[l + 1];
+        hipLaunchKernelGGL(k_up_bwd, dim3(blocks(tot, 256)), dim3(256), 0, u->stream, View{u->gD[l], cd, CH[l]},
+                           src_mask, dst, CH[l + 1], N, h >> 1, w >> 1);
+    }
+    // encoder, coarsest level first
+    for (int l = LV - 1; l >= 0; l--) {
+        const int h = H >> l, w = W >> l;
+        const int cin_t = l == 0 ? IN_PAD : CH[l - 1];
+        const int cin = l == 0 ? IN_CH : CH[l - 1];
+        Layer& la = n.enc(l, 0);
+        Layer& lb = n.enc(l, 1);
+        if (l < LV - 1) {
+            const int cd = CH[l] + CH[l + 1];
+            size_t tot = (size_t)N * h * w * CH[l];
+            hipLaunchKernelGGL(k_pool_bwd, dim3(blocks(tot, 256)), dim3(256), 0, u->stream,
+                               View{u->gX[l + 1], CH[l], 0}, View{u->gD[l], cd, 0}, View{u->D[l], cd, 0},
+                               View{u->gE[l], CH[l], 0}, CH[l], N, h, w);
+        }
+        launch_wgrad(u, lb, View{u->A[l], CH[l], 0}, CH[l], View{u->gE[l], CH[l], 0}, CH[l], N, h, w);
+        launch_conv(u, lb, true, View{u->gE[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->gA[l], CH[l], 0}, CH[l], false,
+                    nullptr, View{u->A[l], CH[l], 0}, nullptr, nullptr);
+        launch_wgrad(u, la, View{u->X[l], cin_t, 0}, cin, View{u->gA[l], CH[l], 0}, CH[l], N, h, w);
+        if (l > 0)
+            launch_conv(u, la, true, View{u->gA[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->gX[l], cin_t, 0}, cin, false,
+                        nullptr, none, nullptr, nullptr);
+    }
+}
+
+double net_flops(nr_unet* u, size_t P0) {
+    double f = 0.0;
+    Net& n = u->net;
+    for (int l = 0; l < LV; l++) {
+        size_t P = P0 >> (2 * l);
+        f += conv_flops(n.enc(l, 0), P) + conv_flops(n.enc(l, 1), P);
+        if (l < LV - 1) f += conv_flops(n.dec(l, 0), P) + conv_flops(n.dec(l, 1), P);
+    }
+    return f + conv_flops(n.out(), P0);
+}
+
+int check_dims(int N, int H, int W) {
+    if (N <= 0 || H <= 0 || W <= 0) return fail(NR_ERR_INVALID, "bad image dimensions");
+    if ((size_t)N * H * W * 224 * 2 > ((size_t)1 << 37)) return fail(NR_ERR_LIMIT, "image batch too large");
+    return NR_OK;
+}
+
+int train_common(nr_unet* u, const float* b, const float* al, const float* nm, const float* t, int N, int H, int W,
+                 float* loss) {
+    if (!u || !b || !al || !nm || !t) return fail(NR_ERR_INVALID, "null argument");
+    if (int e = check_dims(N, H, W)) return e;
+    if (H % 16 || W % 16) return fail(NR_ERR_INVALID, "training crops must be multiples of 16");
+    UCK(hipSetDevice(u->device));
+    if (int e = ensure_shape(u, N, H, W)) return e;
+    const size_t P = (size_t)N * H * W;
+    hipLaunchKernelGGL(k_prep, dim3(blocks(P, 256)), dim3(256), 0, u->stream, b, al, nm, N, H, W, H, W, u->X[0], u->xlog);
+    hipLaunchKernelGGL(k_target, dim3(blocks(P, 256)), dim3(256), 0, u->stream, t, N, H, W, u->tlog);
+    forward(u, N, H, W);
+    const unsigned nb = blocks(P, 256);
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, u->stream, u->y, u->tlog, (int)P, 1.0f / (3.0f * (float)P), u->gY,
+                       u->partial);
+    UCK(hipMemsetAsync(u->grad, 0, u->net.n_params * sizeof(float), u->stream));
+    backward(u, N, H, W);
+    UCK(hipGetLastError());
+    std::vector<float> part(nb);
+    UCK(hipMemcpyAsync(part.data(), u->partial, nb * sizeof(float), hipMemcpyDeviceToHost, u->stream));
+    UCK(hipStreamSynchronize(u->stream));
+    double s = 0.0;
+    for (float x : part) s += x;
+    if (loss) *loss = (float)(s / (3.0 * (double)P));
+    return NR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t nr_unet_param_count(void) { return Net().n_params; }
+
+nr_unet* nr_unet_create(int device, uint32_t seed) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) { fail(NR_ERR_HIP, "no HIP device visible"); return nullptr; }
+    if (device < 0 || device >= n) { fail(NR_ERR_INVALID, "bad device"); return nullptr; }
+    auto* u = new nr_unet();
+    u->device = device;
+    auto bad = [&](const char* m) { nr_error_slot() = m; nr_unet_destroy(u); return (nr_unet*)nullptr; };
+    if (hipSetDevice(device) != hipSuccess) return bad("hipSetDevice");
+    if (hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking) != hipSuccess) return bad("stream");
+    if (hipEventCreate(&u->e0) != hipSuccess || hipEventCreate(&u->e1) != hipSuccess) return bad("events");
+    const size_t np = u->net.n_params;
+    if (hipMalloc((void**)&u->prm, np * 4) != hipSuccess || hipMalloc((void**)&u->grad, np * 4) != hipSuccess ||
+        hipMalloc((void**)&u->m, np * 4) != hipSuccess || hipMalloc((void**)&u->v, np * 4) != hipSuccess)
+        return bad("hipMalloc parameters");
+    size_t wbytes = 0;
+    for (Layer& l : u->net.L)
+        wbytes += ((size_t)l.cout_pad * l.kg_pad * 8 + (size_t)l.cin_pad * l.kgb_pad * 8) * 2 + 512;
+    if (hipMalloc((void**)&u->wmem, wbytes) != hipSuccess) return bad("hipMalloc weights");
+    size_t o = 0;
+    for (Layer& l : u->net.L) {
+        l.wf = u->wmem + o; o += ((size_t)l.cout_pad * l.kg_pad * 8 + 127) & ~(size_t)127;
+        l.wb = u->wmem + o; o += ((size_t)l.cin_pad * l.kgb_pad * 8 + 127) & ~(size_t)127;
+    }
+    // He-normal weights, zero biases; the output layer starts small (near-identity residual)
+    std::vector<float> h(np, 0.0f);
+    std::mt19937 rng(seed);
+    for (size_t li = 0; li < u->net.L.size(); li++) {
+        const Layer& l = u->net.L[li];
+        const float sd = std::sqrt(2.0f / (9.0f * (float)l.cin)) * (li + 1 == u->net.L.size() ? 0.1f : 1.0f);
+        std::normal_distribution<float> nd(0.0f, sd);
+        for (size_t i = 0; i < (size_t)l.cout * 9 * l.cin; i++) h[l.w_off + i] = nd(rng);
+    }
+    if (nr_unet_set_params(u, h.data(), np) != NR_OK) { nr_unet_destroy(u); return nullptr; }
+    return u;
+}
+
+void nr_unet_destroy(nr_unet* u) {
+    if (!u) return;
+    (void)hipSetDevice(u->device);
+    if (u->stream) (void)hipStreamSynchronize(u->stream);
+    for (void* p : {(void*)u->prm, (void*)u->grad, (void*)u->m, (void*)u->v, (void*)u->wmem, u->amem})
+        if (p) (void)hipFree(p);
+    if (u->e0) (void)hipEventDestroy(u->e0);
+    if (u->e1) (void)hipEventDestroy(u->e1);
+    if (u->stream) (void)hipStreamDestroy(u->stream);
+    delete u;
+}
+
+int nr_unet_set_params(nr_unet* u, const float* p, size_t n) {
+    if (!u || !p) return fail(NR_ERR_INVALID, "null argument");
+    if (n != u->net.n_params) return fail(NR_ERR_INVALID, "parameter count mismatch");
+    UCK(hipSetDevice(u->device));
+    UCK(hipMemcpyAsync(u->prm, p, n * 4, hipMemcpyHostToDevice, u->stream));
+    UCK(hipMemsetAsync(u->m, 0, n * 4, u->stream));
+    UCK(hipMemsetAsync(u->v, 0, n * 4, u->stream));
+    u->adam_t = 0;
+    if (int e = pack_all(u)) return e;
+    UCK(hipStreamSynchronize(u->stream));
+    return NR_OK;
+}
+
+int nr_unet_get_params(nr_unet* u, float* p, size_t n) {
+    if (!u || !p) return fail(NR_ERR_INVALID, "null argument");
+    if (n != u->net.n_params) return fail(NR_ERR_INVALID, "parameter count mismatch");
+    UCK(hipSetDevice(u->device));
+    UCK(hipMemcpyAsync(p, u->prm, n * 4, hipMemcpyDeviceToHost, u->stream));
+    UCK(hipStreamSynchronize(u->stream));
+    return NR_OK;
+}
+
+int nr_unet_forward(nr_unet* u, const float* b, const float* al, const float* nm, int32_t N, int32_t H, int32_t W,
+                    float* out) {
+    if (!u || !b || !al || !nm || !out) return fail(NR_ERR_INVALID, "null argument");
+    if (int e = check_dims(N, H, W)) return e;
+    UCK(hipSetDevice(u->device));
+    const int Hp = up16(H), Wp = up16(W);
+    if (int e = ensure_shape(u, N, Hp, Wp)) return e;
+    const size_t P = (size_t)N * Hp * Wp;
+    UCK(hipEventRecord(u->e0, u->stream));
+    hipLaunchKernelGGL(k_prep, dim3(blocks(P, 256)), dim3(256), 0, u->stream, b, al, nm, N, H, W, Hp, Wp, u->X[0],
+                       u->xlog);
+    forward(u, N, Hp, Wp);
+    const size_t Po = (size_t)N * H * W;
+    hipLaunchKernelGGL(k_output, dim3(blocks(Po, 256)), dim3(256), 0, u->stream, u->y, N, H, W, Hp, Wp, out);
+    UCK(hipGetLastError());
+    UCK(hipEventRecord(u->e1, u->stream));
+    UCK(hipEventSynchronize(u->e1));
+    UCK(hipEventElapsedTime(&u->last_ms, u->e0, u->e1));
+    u->last_flops = net_flops(u, P);
+    return NR_OK;
+}
+
+int nr_unet_gradients(nr_unet* u, const float* b, const float* al, const float* nm, const float* t, int32_t N,
+                      int32_t H, int32_t W, float* grad, float* loss) {
+    if (!grad) return fail(NR_ERR_INVALID, "null argument");
+    if (int e = train_common(u, b, al, nm, t, N, H, W, loss)) return e;
+    UCK(hipMemcpy(grad, u->grad, u->net.n_params * 4, hipMemcpyDeviceToHost));
+    return NR_OK;
+}
+
+int nr_unet_train_step(nr_unet* u, const float* b, const float* al, const float* nm, const float* t, int32_t N,
+                       int32_t H, int32_t W, float lr, float* loss) {
+    if (int e = train_common(u, b, al, nm, t, N, H, W, loss)) return e;
+    u->adam_t++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, (float)u->adam_t), c2 = 1.0f - std::pow(b2, (float)u->adam_t);
+    const size_t np = u->net.n_params;
+    hipLaunchKernelGGL(k_adam, dim3(blocks(np, 256)), dim3(256), 0, u->stream, u->prm, u->grad, u->m, u->v, np, lr, b1,
+                       b2, c1, c2, 1e-8f);
+    if (int e = pack_all(u)) return e;
+    UCK(hipStreamSynchronize(u->stream));
+    return NR_OK;
+}
+
+int nr_denoise(nr_ctx* ctx, int view, nr_unet* u, float* out) {
+    if (!ctx || !u || !out) return fail(NR_ERR_INVALID, "null argument");
+    const float* b = nr_aov_device_ptr(ctx, view, NR_LAYER_BEAUTY);
+    const float* al = nr_aov_device_ptr(ctx, view, NR_LAYER_DIFFUSE);
+    const float* nm = nr_aov_device_ptr(ctx, view, NR_LAYER_WORLD_NORMAL);
+    if (!b || !al || !nm) return fail(NR_ERR_STATE, "view has no beauty/diffuse/world-normal layers");
+    int32_t H = 0, W = 0;
+    if (nr_view_size(ctx, view, &H, &W) != NR_OK) return NR_ERR_STATE;
+    if (nr_sync(ctx) != NR_OK) return NR_ERR_HIP;
+    return nr_unet_forward(u, b, al, nm, 1, H, W, out);
+}
+
+int nr_unet_last_stats(nr_unet* u, float* ms, double* flops) {
+    if (!u) return fail(NR_ERR_INVALID, "null argument");
+    if (ms) *ms = u->last_ms;
+    if (flops) *flops = u->last_flops;
+    return NR_OK;
+}
+
+}  // extern "C"

@@ -1,4 +1,4 @@
-"""The C-ABI library loads and exports every entry point include/nectar_hip.h declares (CPU only)."""
+"""The C-ABI library loads and exports every entry point include/*.h declares (CPU only)."""
 import ctypes as C
 import os
 import re
@@ -7,12 +7,13 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "nectar_hip.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if h.endswith(".h")]
 LIB = os.path.join(ROOT, "nectarrender_amd", "libnectar_hip.so")
 
 
 def declared():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(nr_[a-z_0-9]+)\s*\(", text)))
 
@@ -20,7 +21,8 @@ def declared():
 def test_header_declares_entry_points():
     names = declared()
     for n in ("nr_create", "nr_destroy", "nr_scene_upload", "nr_camera_set", "nr_render", "nr_trace_closest",
-              "nr_aov_read", "nr_post", "nr_last_error"):
+              "nr_aov_read", "nr_post", "nr_last_error", "nr_view_size", "nr_unet_create", "nr_unet_forward",
+              "nr_unet_train_step", "nr_unet_gradients", "nr_denoise"):
         assert n in names
 
 
@@ -50,6 +52,12 @@ def test_null_arguments_fail_cleanly_without_device():
     assert L.nr_post(None, 0, None, None, None) == 1
     assert L.nr_set_timing(None, 1) == 1
     assert L.nr_sync(None) == 1
+    from nectarrender_amd import denoise
+    D = denoise._lib()
+    assert D.nr_unet_param_count() == 1016131
+    assert D.nr_unet_forward(None, None, None, None, 1, 16, 16, None) == 1
+    assert D.nr_denoise(None, 0, None, None) == 1
+    assert not D.nr_unet_create(0, 0) or os.path.exists("/dev/kfd")
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is visible")
