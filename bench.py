@@ -61,12 +61,48 @@ def cpu_baseline(budget_s: float = 12.0, max_passes: int = 63):
                       f"{dt:.1f} s, OpenMP over 16x16 tiles"}
 
 
+def quality_section(log):
+    """Seconds to RMSE <= 0.01 (display-referred) vs a 4032-pass float64 reference, plain and denoised,
+    on the 1080p three-sphere scene with 500 media/dielectric spheres (parity estimator)."""
+    import torch
+
+    from nectarrender_amd import abi, native, quality, scenes
+    from nectarrender_amd.denoise import WEIGHTS, UNet
+    s, cam, rp = scenes.config5(spp=2049)
+    ctx = native.Context(0)
+    ctx.upload(s)
+    ref_disp, _ = quality.reference(ctx, abi, torch, cam, rp, log=log)
+    unet = UNet(0).load() if os.path.exists(WEIGHTS) else None
+    plain, den = quality.curves(ctx, abi, torch, cam, rp, ref_disp, unet)
+    out = {"workload": "1920x1080 (H=1080, W=1920) three-sphere scene + 500 constant-medium/dielectric spheres, "
+                       "depth 8, white balance 6500 K tint 0, ACES, sqrt gamma",
+           "target_rmse": quality.TARGET, "reference": "64 renders x 63 passes, float64 mean, disjoint seeds",
+           "plain_s": quality.time_to_target(plain),
+           "plain": [[k, round(t, 5), round(e, 5)] for k, t, e in plain]}
+    if unet is not None:
+        pts = [(k, tr + td, e) for k, tr, td, e in den]
+        out["denoised_s"] = quality.time_to_target(pts)
+        out["denoised"] = [[k, round(tr, 5), round(td, 5), round(e, 5)] for k, tr, td, e in den]
+        # denoiser throughput at this size (HIP events around the U-Net launch sequence)
+        ms, flops = unet.last_stats()
+        out["denoiser"] = {"ms": round(ms, 3), "tflops": round(flops / ms / 1e9, 2), "peak_tflops": 2500.0,
+                           "frac": round(flops / ms / 1e9 / 2500.0, 4), "flops_per_frame": flops,
+                           "dtype": "bf16 (fp32 accumulation)"}
+        unet.close()
+    else:
+        out["denoised_s"] = None
+        out["denoised_note"] = "no trained weights at " + WEIGHTS
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=63)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-quality", action="store_true", help="skip the seconds-to-RMSE measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (a kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -77,10 +113,12 @@ def main():
         sys.exit(2)
     K, Wm = args.steps, args.warmup
 
+    if not args.no_quality or args.calibrate_copy:
+        import torch                      # its HIP runtime initialises before the library's
+        torch.cuda.init()
     from nectarrender_amd import abi, native, scenes
 
     if args.calibrate_copy:
-        import torch
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -130,6 +168,8 @@ def main():
         "image_mean": float(img.mean()),
         "roofline": roof,
     }
+    if not args.no_quality:
+        out["time_to_rmse"] = quality_section(lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -262,6 +262,12 @@ int nr_trace_closest_device(nr_ctx* ctx, const float* d_rays, int32_t n, nr_hit*
  * non-NULL the display-referred float image (before x255) is written too. */
 int nr_post(nr_ctx* ctx, int view, const nr_post_desc* desc, uint8_t* out_u8, float* out_f32);
 
+/* The same post-processing of any planar 3 x H x W fp32 device image (a
+ * denoised frame, an accumulated reference); outputs are device pointers,
+ * either may be NULL.  Synchronous. */
+int nr_post_device(nr_ctx* ctx, const float* d_chw, int32_t H, int32_t W, const nr_post_desc* desc,
+                   uint8_t* d_u8, float* d_f32);
+
 /* Wait for all work of the context. */
 int nr_sync(nr_ctx* ctx);
 

@@ -1242,12 +1242,7 @@ static float host_kelvin_channel(float kelvin, int ch) {                        
     return fminf(1.0f, fmaxf(0.0f, v / 255.0f));
 }
 
-int nr_post(nr_ctx* c, int view, const nr_post_desc* d, uint8_t* out8, float* outf) {
-    if (!c || !d || view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_INVALID, "bad view");
-    View& v = c->views[view];
-    if (d->layer < 0 || d->layer > 8 || !v.layer[d->layer]) return fail(NR_ERR_INVALID, "post: layer not allocated");
-    HIPCK(hipSetDevice(c->device));
-    uint32_t HW = v.H * v.W;
+static PostArgs post_args(const nr_post_desc* d) {
     PostArgs A{};
     float tgt_k = fmaxf(1000.0f, fminf(d->wb_temperature, 40000.0f));
     for (int k = 0; k < 3; k++)
@@ -1256,6 +1251,16 @@ int nr_post(nr_ctx* c, int view, const nr_post_desc* d, uint8_t* out8, float* ou
     A.wb = d->apply_white_balance; A.tm = d->apply_tonemapping; A.tm_method = d->tm_method;
     A.gamma = d->linear_to_gamma; A.fix_reinhard = d->fix_reinhard;
     A.white_point = d->tm_white_point; A.alpha = d->tm_alpha;
+    return A;
+}
+
+int nr_post(nr_ctx* c, int view, const nr_post_desc* d, uint8_t* out8, float* outf) {
+    if (!c || !d || view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_INVALID, "bad view");
+    View& v = c->views[view];
+    if (d->layer < 0 || d->layer > 8 || !v.layer[d->layer]) return fail(NR_ERR_INVALID, "post: layer not allocated");
+    HIPCK(hipSetDevice(c->device));
+    uint32_t HW = v.H * v.W;
+    PostArgs A = post_args(d);
     uint8_t* d8 = nullptr;
     float* df = nullptr;
     if (out8) HIPCK(hipMalloc((void**)&d8, 3 * (size_t)HW));
@@ -1272,6 +1277,17 @@ int nr_post(nr_ctx* c, int view, const nr_post_desc* d, uint8_t* out8, float* ou
     if (d8) (void)hipFree(d8);
     if (df) (void)hipFree(df);
     return rc;
+}
+
+int nr_post_device(nr_ctx* c, const float* img, int32_t H, int32_t W, const nr_post_desc* d, uint8_t* d8, float* df) {
+    if (!c || !d || !img) return fail(NR_ERR_INVALID, "null argument");
+    if (H <= 0 || W <= 0 || (uint64_t)H * W > (1ull << 31)) return fail(NR_ERR_INVALID, "bad image size");
+    HIPCK(hipSetDevice(c->device));
+    const uint32_t HW = (uint32_t)H * (uint32_t)W;
+    hipLaunchKernelGGL(k_post, dim3((HW + 255) / 256), dim3(256), 0, c->stream, img, HW, post_args(d), d8, df);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(c->stream));
+    return NR_OK;
 }
 
 int nr_sync(nr_ctx* c) {

@@ -61,6 +61,8 @@ class UNet:
     """U-Net denoiser on HIP device `device` (He-normal init from `seed`)."""
 
     def __init__(self, device: int = 0, seed: int = 0):
+        import torch                      # device buffers are torch tensors; its runtime starts first
+        torch.cuda.init()
         L = _lib()
         self._h = L.nr_unet_create(device, seed)
         if not self._h:

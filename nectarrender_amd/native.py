@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -45,6 +46,8 @@ def lib() -> C.CDLL:
         "nr_trace_closest_device": (C.c_int, [vp, vp, C.c_int32, vp]),
         "nr_post": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrPostDesc), C.POINTER(C.c_uint8), C.POINTER(C.c_float)]),
         "nr_sync": (C.c_int, [vp]),
+        "nr_post_device": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.POINTER(abi.NrPostDesc), vp, vp]),
+        "nr_view_size": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "nr_last_timings": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int]),
         "nr_set_timing": (C.c_int, [vp, C.c_int]),
     }
@@ -68,6 +71,11 @@ class Context:
 
     def __init__(self, device: int = 0):
         L = lib()
+        # torch ships its own HIP runtime; when both share a process, torch's must
+        # initialise first or it finds no device afterwards
+        torch = sys.modules.get("torch")
+        if torch is not None and torch.cuda.is_available():
+            torch.cuda.init()
         self._h = L.nr_create(device)
         if not self._h:
             raise NectarError(f"nr_create failed: {L.nr_last_error().decode()}")
@@ -126,6 +134,16 @@ class Context:
         _check(lib().nr_post(self._h, view, C.byref(desc), u8.ctypes.data_as(C.POINTER(C.c_uint8)),
                              f.ctypes.data_as(C.POINTER(C.c_float))))
         return u8, f
+
+    def aov_device_ptr(self, view: int, layer: int) -> int:
+        p = lib().nr_aov_device_ptr(self._h, view, layer)
+        if not p:
+            raise NectarError(f"view {view} has no layer {layer}")
+        return p
+
+    def post_device(self, d_img: int, H: int, W: int, desc: abi.NrPostDesc, d_u8: int = 0, d_f32: int = 0):
+        _check(lib().nr_post_device(self._h, C.c_void_p(d_img), H, W, C.byref(desc), C.c_void_p(d_u8 or None),
+                                    C.c_void_p(d_f32 or None)))
 
     def set_timing(self, on: bool):
         _check(lib().nr_set_timing(self._h, int(on)))

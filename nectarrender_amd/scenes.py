@@ -260,3 +260,122 @@ def random_rays(n: int, seed: int, center=(0, 0, 0), spread=3.0) -> np.ndarray:
     tgt = (rng.random((n, 3)) - 0.5) * spread + np.asarray(center)
     d = tgt - o
     return np.concatenate([o, d], axis=1).astype(np.float32)
+
+
+def look_rotation(position, target):
+    """Euler degrees (pitch, yaw, 0) whose R = Rz Ry Rx maps -z onto the direction position -> target."""
+    d = np.asarray(target, np.float64) - np.asarray(position, np.float64)
+    d /= np.linalg.norm(d)
+    pitch = np.degrees(np.arcsin(np.clip(d[1], -1.0, 1.0)))
+    yaw = np.degrees(np.arctan2(-d[0], -d[2]))
+    return (float(pitch), float(yaw), 0.0)
+
+
+def config3(n_spheres=10_000, seed=0, scale=1.0, n_views=16):
+    """10,000 spheres, centres U[-10,10]^3, radius U[0.05,0.3]: 40% PBR (rough U[0,1], metallic
+    Bernoulli 0.5, albedo U[0.2,1]^3), 20% dielectric 1.5, 20% Lambertian (albedo U[0.2,0.9]^3),
+    10% emissive (radiance U[1,20]), 10% constant media (sphere bound, density U[0.1,2], albedo 0.8);
+    simple sky.  16 views of 256x256 .. 1920x1080 (sum 10,142,016 pixels) looking at the origin from a
+    ring of radius 25; `scale` shrinks every view (parity tests)."""
+    rng = np.random.default_rng(seed)
+    s = SceneDesc()
+    c = rng.uniform(-10, 10, (n_spheres, 3))
+    r = rng.uniform(0.05, 0.3, n_spheres)
+    kind = rng.random(n_spheres)
+    for i in range(n_spheres):
+        k = kind[i]
+        if k < 0.4:
+            m = Material.pbr(albedo=tuple(rng.uniform(0.2, 1.0, 3)), roughness=float(rng.random()),
+                             metallic=float(rng.random() < 0.5))
+            s.sphere(c[i], r[i], m)
+        elif k < 0.6:
+            s.sphere(c[i], r[i], Material.dielectric((1, 1, 1), 1.5))
+        elif k < 0.8:
+            s.sphere(c[i], r[i], Material.lambertian(tuple(rng.uniform(0.2, 0.9, 3))))
+        elif k < 0.9:
+            s.sphere(c[i], r[i], Material.emissive((1, 1, 1), float(rng.uniform(1, 20))))
+        else:
+            s.constant_medium(SceneDesc.bound(abi.HIT_SPHERE, tuple(c[i]), radius=float(r[i])),
+                              float(rng.uniform(0.1, 2.0)), (0.8, 0.8, 0.8))
+    s.sky_simple((1, 1, 1), (0.5, 0.7, 1.0))
+    sizes = [(256, 256), (320, 256), (384, 288), (448, 320), (512, 384), (576, 432), (640, 480), (720, 540),
+             (800, 600), (960, 640), (1024, 768), (1152, 768), (1280, 800), (1440, 900), (1600, 900), (1920, 1080)]
+    cams = []
+    for v, (w, h) in enumerate(sizes[:n_views]):
+        ang = 2 * np.pi * v / len(sizes)
+        pos = (25 * np.cos(ang), 4.0 * np.sin(3 * ang), 25 * np.sin(ang))
+        res = (max(8, int(round(h * scale))), max(8, int(round(w * scale))))
+        cams.append(camera_desc(res, pos, look_rotation(pos, (0, 0, 0)), 32, 2.0, 25.0, 0.0, 2.0, 1.0))
+    return s, cams, dict(depth=16, seed=0)
+
+
+def config5(n_spheres=500, seed=0, res=(1080, 1920), spp=4096):
+    """The 1080p three-sphere scene (config1's ground, spheres and constant unit sky) plus 500 spheres in
+    the style of config3, half constant media (density U[0.1,2], albedo 0.8) and half dielectric 1.5,
+    radius U[0.05,0.3], spread over the visible region; depth 8."""
+    rng = np.random.default_rng(seed + 5)
+    s = SceneDesc()
+    s.quad((0, 0, 0), (0, 0, 0), (10, 1, 10), Material.lambertian((0.5, 0.5, 0.5)))
+    s.sphere((-1.1, 0.5, 0), 0.5, Material.pbr(albedo=(1.0, 0.6, 0.15), roughness=0.2, metallic=1.0))
+    s.sphere((0, 0.5, 0), 0.5, Material.dielectric((1, 1, 1), 1.5))
+    s.sphere((1.1, 0.5, 0), 0.5, Material.lambertian((0.2, 0.4, 0.8)))
+    for i in range(n_spheres):
+        c = (float(rng.uniform(-3.5, 3.5)), float(rng.uniform(0.05, 2.5)), float(rng.uniform(-5.0, 1.5)))
+        r = float(rng.uniform(0.05, 0.3))
+        if i % 2 == 0:
+            s.constant_medium(SceneDesc.bound(abi.HIT_SPHERE, c, radius=r), float(rng.uniform(0.1, 2.0)),
+                              (0.8, 0.8, 0.8))
+        else:
+            s.sphere(c, r, Material.dielectric((1, 1, 1), 1.5))
+    s.sky_simple((1, 1, 1), (1, 1, 1))
+    # same framing correction as config2 for the reference's res[0]-based x projection
+    cam = camera_desc(res, (0.0, 1.2, 4.8), (-30.0, 20.15, 0.0), spp, 2.0, 4.8, 0.0, 2.0, 1.0)
+    return s, cam, dict(depth=8, seed=0)
+
+
+def training_scene(seed: int, res=256, spp=1024):
+    """Randomised scenes in the style of config1/config5 for the denoiser: a ground quad, 3-40 spheres of
+    every material (some inside constant media), sometimes a small mesh, a simple or HDRI sky, a camera
+    on a ring looking at the middle."""
+    rng = np.random.default_rng(10_000 + seed)
+    s = SceneDesc()
+    g = tuple(rng.uniform(0.2, 0.8, 3))
+    s.quad((0, 0, 0), (0, float(rng.uniform(0, 90)), 0), (12, 1, 12), Material.lambertian(g))
+    n = int(rng.integers(3, 40))
+    for _ in range(n):
+        r = float(rng.uniform(0.08, 0.7))
+        c = (float(rng.uniform(-2.5, 2.5)), r * float(rng.uniform(0.8, 3.0)), float(rng.uniform(-2.5, 2.5)))
+        k = rng.random()
+        if k < 0.3:
+            s.sphere(c, r, Material.pbr(albedo=tuple(rng.uniform(0.2, 1.0, 3)), roughness=float(rng.random()),
+                                        metallic=float(rng.random() < 0.5)))
+        elif k < 0.5:
+            s.sphere(c, r, Material.dielectric(tuple(rng.uniform(0.8, 1.0, 3)), float(rng.uniform(1.3, 1.8))))
+        elif k < 0.75:
+            s.sphere(c, r, Material.lambertian(tuple(rng.uniform(0.1, 0.9, 3))))
+        elif k < 0.82:
+            s.sphere(c, r * 0.4, Material.emissive(tuple(rng.uniform(0.5, 1.0, 3)), float(rng.uniform(2, 20))))
+        else:
+            s.constant_medium(SceneDesc.bound(abi.HIT_SPHERE, c, radius=r), float(rng.uniform(0.1, 3.0)),
+                              tuple(rng.uniform(0.5, 0.95, 3)))
+    if rng.random() < 0.3:
+        P, T = geodesic_sphere(4)
+        mi = s.add_mesh(mesh_vertices((P * 0.5).astype(np.float32), T), T)
+        s.mesh(mi, (float(rng.uniform(-1, 1)), 0.6, float(rng.uniform(-1, 1))), tuple(rng.uniform(0, 90, 3)),
+               tuple(rng.uniform(0.6, 1.4, 3)), Material.pbr(albedo=tuple(rng.uniform(0.3, 1, 3)),
+                                                             roughness=float(rng.uniform(0.1, 0.8)), metallic=0.0))
+    if rng.random() < 0.5:
+        top = tuple(rng.uniform(0.3, 1.2, 3))
+        s.sky_simple(top, top if rng.random() < 0.5 else tuple(rng.uniform(0.3, 1.2, 3)))
+    else:
+        s.sky_hdri(procedural_hdri(128, 64, seed=seed, sigma=0.3, sun_radiance=float(rng.uniform(20, 400)),
+                                   sun_diameter_deg=6.0, sun_elevation_deg=float(rng.uniform(15, 70)),
+                                   sun_azimuth_deg=float(rng.uniform(0, 360))),
+                   rotation=float(rng.uniform(0, 360)), intensity=float(rng.uniform(0.3, 1.0)))
+    ang = rng.uniform(0, 2 * np.pi)
+    dist = rng.uniform(3.0, 6.0)
+    pos = (dist * np.cos(ang), float(rng.uniform(0.4, 2.5)), dist * np.sin(ang))
+    tgt = (float(rng.uniform(-0.5, 0.5)), float(rng.uniform(0.2, 0.8)), float(rng.uniform(-0.5, 0.5)))
+    cam = camera_desc((res, res), pos, look_rotation(pos, tgt), spp, float(rng.uniform(1.5, 3.0)), dist,
+                      float(rng.choice([0.0, 0.0, 0.05])), 2.0, 1.0)
+    return s, cam, dict(depth=8, seed=int(rng.integers(1, 1 << 30)))
