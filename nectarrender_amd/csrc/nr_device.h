@@ -72,21 +72,21 @@ NR_HD Ray to_object(const Ray& r, const GXform& x) {                           /
 
 // AABB::hit_t (bvh/aabb.h:135-157): entry distance or FMAX
 NR_HD float box_t(const float* mn, const float* mx, const Ray& r, float tmin, float tmax) {
+    // The reference leaves the axis loop as soon as tmax <= tmin.  tmin only
+    // grows and tmax only shrinks, so finishing all three axes with selects
+    // and testing once gives the same answer (NaN slab distances never update
+    // either bound, as in the branchy form) without divergent control flow.
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         float o = comp(r.o, a), iv = comp(r.inv, a);
         float t0 = (mn[a] - o) * iv;
         float t1 = (mx[a] - o) * iv;
-        if (t0 < t1) {
-            if (t0 > tmin) tmin = t0;
-            if (t1 < tmax) tmax = t1;
-        } else {
-            if (t1 > tmin) tmin = t1;
-            if (t0 < tmax) tmax = t0;
-        }
-        if (tmax <= tmin) return NR_FMAX;
+        bool c = t0 < t1;
+        float lo = c ? t0 : t1, hi = c ? t1 : t0;
+        tmin = lo > tmin ? lo : tmin;
+        tmax = hi < tmax ? hi : tmax;
     }
-    return tmin;
+    return tmax <= tmin ? NR_FMAX : tmin;
 }
 
 // ---------------------------------------------------------------------------
