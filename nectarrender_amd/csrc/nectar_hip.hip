@@ -171,6 +171,7 @@ __device__ __forceinline__ Ray camera_ray(const GCamera& c, uint32_t x, uint32_t
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
+template <int TY>
 __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_trace_closest(const GScene* __restrict__ Sp,
                                                                     const float* __restrict__ rays, int n,
                                                                     nr_hit* __restrict__ hits,
@@ -185,7 +186,7 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_trace_closest(const GScene
     HRec rec;
     hrec_clear(rec);
     bool of = false;
-    int oi = scene_closest(S, r, rec, st, of);
+    int oi = scene_closest<TY>(S, r, rec, st, of);
     if (of) atomicOr(&counts[CNT_ERR], 1);
     nr_hit h;
     if (oi < 0) {
@@ -267,6 +268,7 @@ __global__ void k_counts(int32_t* counts, int32_t in) {
     if (t == 0) counts[CNT_ACT + (1 - in)] = 0;
 }
 
+template <int TY>
 __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_extend(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A, StackCfg SC) {
     extern __shared__ int2 smem[];
     const GScene& S = *Sp;
@@ -283,7 +285,7 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_extend(const GScene* __res
         HRec rec;
         hrec_clear(rec);
         bool of = false;
-        int oi = scene_closest(S, r, rec, st, of);
+        int oi = scene_closest<TY>(S, r, rec, st, of);
         if (of) atomicOr(&P.counts[CNT_ERR], 1);
         if (oi < 0) {                                                           // trace.cu:19-22
             float4 a4 = P.atten[path], l4 = P.L[path];
@@ -640,6 +642,7 @@ struct nr_ctx {
     GScene* d_scene = nullptr;          // device copy of `scene`
     std::vector<void*> scene_mem;
     bool mat_present[NR_MAT_TYPES] = {false};
+    int obj_types = NR_TY_ALL;      // object types of the scene (NR_TY_* bits): picks the traversal kernel
     View views[NR_MAX_VIEWS];
     // path state
     PathPtrs P{};
@@ -772,6 +775,7 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
     free_list(c->scene_mem);
     c->has_scene = false;
     for (auto& b : c->mat_present) b = false;
+    c->obj_types = 0;
     GScene S{};
     // images
     std::vector<GImage> imgs(d->n_images);
@@ -888,6 +892,8 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
         o.neg_inv_density = -1.0f / (od.density + 1e-8f);
         o.pad[0] = mats[od.material].type;
         c->mat_present[mats[od.material].type] = true;
+        c->obj_types |= od.type == NR_HIT_QUAD ? NR_TY_QUAD : od.type == NR_HIT_SPHERE ? NR_TY_SPHERE
+                      : od.type == NR_HIT_MESH ? NR_TY_MESH : NR_TY_OTHER;
         nrb::Box lb;
         if (geom == NR_HIT_QUAD || geom == NR_HIT_CUBE) { float a[3] = {-0.5f, -0.5f, -0.5f}, b[3] = {0.5f, 0.5f, 0.5f}; lb = nrb::box_points(a, b); }
         else if (geom == NR_HIT_SPHERE) { float a[3] = {-o.radius, -o.radius, -o.radius}, b[3] = {o.radius, o.radius, o.radius}; lb = nrb::box_points(a, b); }
@@ -1107,8 +1113,13 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
             if (it > max_iters) return fail(NR_ERR_STATE, "streaming loop did not drain");
             hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->P.counts, (int32_t)A.in);
             mark(0);
-            hipLaunchKernelGGL(k_extend, dim3((uint32_t)((bound + 127) / 128)), dim3(128), lds, c->stream, c->d_scene,
-                               c->P, A, SC);
+            if (c->obj_types & NR_TY_OTHER)
+                hipLaunchKernelGGL(k_extend<NR_TY_ALL>, dim3((uint32_t)((bound + 127) / 128)), dim3(128), lds,
+                                   c->stream, c->d_scene, c->P, A, SC);
+            else
+                hipLaunchKernelGGL(k_extend<NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH>,
+                                   dim3((uint32_t)((bound + 127) / 128)), dim3(128), lds, c->stream, c->d_scene, c->P,
+                                   A, SC);
             mark(1);
             if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL(k_shade<NR_MAT_LAMBERTIAN>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
             if (c->mat_present[NR_MAT_PBR]) hipLaunchKernelGGL(k_shade<NR_MAT_PBR>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
@@ -1191,8 +1202,12 @@ int nr_trace_closest_device(nr_ctx* c, const float* d_rays, int32_t n, nr_hit* d
     HIPCK(hipMemsetAsync(c->d_err + CNT_ERR, 0, sizeof(int32_t), c->stream));
     hipEvent_t e0 = c->ev_begin, e1 = c->ev_end;
     if (c->timing) (void)hipEventRecord(e0, c->stream);
-    hipLaunchKernelGGL(k_trace_closest, dim3((n + 127) / 128), dim3(128), trav_lds_bytes(), c->stream, c->d_scene, d_rays,
-                       n, d_hits, c->d_err, SC);
+    if (c->obj_types & NR_TY_OTHER)
+        hipLaunchKernelGGL(k_trace_closest<NR_TY_ALL>, dim3((n + 127) / 128), dim3(128), trav_lds_bytes(), c->stream,
+                           c->d_scene, d_rays, n, d_hits, c->d_err, SC);
+    else
+        hipLaunchKernelGGL(k_trace_closest<NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH>, dim3((n + 127) / 128), dim3(128),
+                           trav_lds_bytes(), c->stream, c->d_scene, d_rays, n, d_hits, c->d_err, SC);
     HIPCK(hipGetLastError());
     if (c->timing) {
         (void)hipEventRecord(e1, c->stream);

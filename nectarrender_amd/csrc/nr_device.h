@@ -308,21 +308,21 @@ NR_D bool cube_hit(const GScene& S, const Ray& r, float tmin, float tmax, HRec& 
 
 // Möller–Trumbore (hittable/mesh.h:338-365)
 NR_D bool tri_hit(const Ray& r, f3 v0, f3 v1, f3 v2, float tmin, float tmax, float& ot, float& ou, float& ov) {
+    // Every test of the reference's early-out sequence, evaluated without
+    // branches (a lane's later terms cannot change an earlier rejection).
     f3 e1 = v1 - v0, e2 = v2 - v0;
     f3 h = cross(r.d, e2);
     float a = dot(e1, h);
-    if (fabsf(a) < NR_FMIN) return false;
     float f = 1.0f / a;
     f3 s = r.o - v0;
     float u = f * dot(s, h);
-    if (u < 0.0f || u > 1.0f) return false;
     f3 q = cross(s, e1);
     float v = f * dot(r.d, q);
-    if (v < 0.0f || u + v > 1.0f) return false;
     float t = f * dot(e2, q);
-    if (!(tmin <= t && t <= tmax)) return false;
-    ot = t; ou = u; ov = v;
-    return true;
+    bool ok = !(fabsf(a) < NR_FMIN) && !(u < 0.0f || u > 1.0f) && !(v < 0.0f || u + v > 1.0f) &&
+              (tmin <= t && t <= tmax);
+    if (ok) { ot = t; ou = u; ov = v; }
+    return ok;
 }
 
 // Traversal stack: the first lds_cap levels live in LDS (one column per lane,
@@ -523,26 +523,33 @@ __device__ NR_GENERIC_ATTR bool object_hit_generic(const GScene& S, const GObjec
     return h;
 }
 
+// Object types a traversal kernel is compiled for (a scene picks the smallest
+// set that covers it; the code of absent types, and its registers, drop out).
+enum : int { NR_TY_QUAD = 1, NR_TY_SPHERE = 2, NR_TY_MESH = 4, NR_TY_OTHER = 8, NR_TY_ALL = 15 };
+
+template <int TY>
 NR_D bool object_t(const GScene& S, const GObject& o, const Ray& r, Stack& st, int sp0, bool& overflow,
                    float& tw, int32_t& slot, float& u, float& v, float& tobj) {
     HRec rec;
-    bool h;
+    bool h = false;
     slot = -1;
-    if (o.type == 4 || o.type == 1 || o.type == 2) {   // mesh, quad, sphere inline
+    const bool inl = ((TY & NR_TY_MESH) && o.type == 4) || ((TY & NR_TY_QUAD) && o.type == 1) ||
+                     ((TY & NR_TY_SPHERE) && o.type == 2);
+    if (inl) {   // mesh, quad, sphere inline
         Ray orr = to_object(r, o.xf);
         hrec_clear(rec);
-        if (o.type == 4) {
+        if ((TY & NR_TY_MESH) && o.type == 4) {
             const GMesh& M = S.meshes[o.mesh];
             slot = mesh_closest(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow);
             h = slot >= 0;
             if (h) mesh_interp(M, slot, orr, tobj, u, v, rec);
-        } else if (o.type == 1) {
+        } else if ((TY & NR_TY_QUAD) && o.type == 1) {
             h = quad_hit(orr, NR_EPS, NR_FMAX, rec);
-        } else {
+        } else if (TY & NR_TY_SPHERE) {
             h = sphere_hit(orr, NR_EPS, NR_FMAX, rec, o.radius, o.xf);
         }
         if (h) to_world(rec, o.xf, orr, r, true);
-    } else {
+    } else if (TY & NR_TY_OTHER) {
         h = object_hit_generic(S, o, r, rec, st, sp0, overflow);
     }
     if (!h) return false;
@@ -554,6 +561,7 @@ NR_D bool object_t(const GScene& S, const GObject& o, const Ray& r, Stack& st, i
 // index of the closest hit or -1; rec receives the world-space record.  Only
 // (object, t, triangle slot, u, v) stay live during the traversal; the record
 // of the winner is rebuilt at the end by the same deterministic arithmetic.
+template <int TY>
 NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, bool& overflow) {
     float tmin = NR_EPS, tmax = NR_FMAX;
     int32_t best = -1, bslot = -1;
@@ -569,7 +577,7 @@ NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, 
             int32_t oi = ~ref;
             float tw, u, v, tobj;
             int32_t slot;
-            if (object_t(S, S.objects[oi], r, st, sp, overflow, tw, slot, u, v, tobj)) {
+            if (object_t<TY>(S, S.objects[oi], r, st, sp, overflow, tw, slot, u, v, tobj)) {
                 if (tmin < tw && tw < tmax) {
                     best = oi; bslot = slot; bu = u; bv = v; btobj = tobj;
                     tmax = tw;
@@ -591,12 +599,18 @@ NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, 
     }
     if (best >= 0 && !overflow) {
         const GObject& o = S.objects[best];
-        if (o.type == 4) {
+        if ((TY & NR_TY_MESH) && o.type == 4) {
             Ray orr = to_object(r, o.xf);
             hrec_clear(rec);
             mesh_interp(S.meshes[o.mesh], bslot, orr, btobj, bu, bv, rec);
             to_world(rec, o.xf, orr, r, true);
-        } else {
+        } else if (((TY & NR_TY_QUAD) && o.type == 1) || ((TY & NR_TY_SPHERE) && o.type == 2)) {
+            Ray orr = to_object(r, o.xf);
+            hrec_clear(rec);
+            bool h = o.type == 1 ? quad_hit(orr, NR_EPS, NR_FMAX, rec)
+                                 : sphere_hit(orr, NR_EPS, NR_FMAX, rec, o.radius, o.xf);
+            if (h) to_world(rec, o.xf, orr, r, true);
+        } else if (TY & NR_TY_OTHER) {
             object_hit_generic(S, o, r, rec, st, 0, overflow);
         }
     }
