@@ -94,6 +94,7 @@ struct PathPtrs {
     int32_t* qmat;      // NR_MAT_TYPES * cap
     int32_t* counts;    // CNT_N
     uint8_t* finished;  // 1 = the pixel's path ended, fold it and start its next sample
+    float4* acc;        // running means during a render: beauty [0, cap), diffuse [cap, 2 cap), normal [2 cap, 3 cap)
     uint32_t cap;
 };
 
@@ -121,6 +122,28 @@ __device__ __forceinline__ int wave_append(bool pred, int32_t* counter) {
     base = __shfl(base, leader);
     unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
     return pred ? base + __popcll(below) : -1;
+}
+
+// workgroup-level queue append: one atomic per workgroup (slots are dense, in
+// wave order within the workgroup).  Every thread of the block must call it.
+template <int BLOCK>
+__device__ __forceinline__ int block_append(bool pred, int32_t* counter) {
+    constexpr int NW = BLOCK / 64;
+    __shared__ int wcount[NW];
+    __shared__ int wbase[NW];
+    const int wave = threadIdx.x >> 6, lane = __lane_id();
+    unsigned long long mask = __ballot(pred);
+    if (lane == 0) wcount[wave] = __popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int total = 0;
+        for (int w = 0; w < NW; w++) { wbase[w] = total; total += wcount[w]; }
+        int base = total ? atomicAdd(counter, total) : 0;
+        for (int w = 0; w < NW; w++) wbase[w] += base;
+    }
+    __syncthreads();
+    unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    return pred ? wbase[wave] + __popcll(below) : -1;
 }
 
 __device__ __forceinline__ Stack make_stack(int2* smem, const GScene& S, const StackCfg& SC) {
@@ -240,12 +263,38 @@ __device__ __forceinline__ void start_path(const GCamera& cam, PathPtrs& P, uint
     P.draw[i] = g.draw;
 }
 
-// running-mean fold of one finished sample (accumulate_samples_kernel)
-__device__ __forceinline__ void fold(float* layer, size_t HW, size_t i, float4 v, float a, float b) {
-    if (!layer) return;
-    layer[i] = layer[i] * a + v.x * b;
-    layer[HW + i] = layer[HW + i] * a + v.y * b;
-    layer[2 * HW + i] = layer[2 * HW + i] * a + v.z * b;
+// running-mean fold of one finished sample (accumulate_samples_kernel): the
+// same per-channel arithmetic on a pixel-interleaved copy of the layer
+__device__ __forceinline__ void fold(float4* acc, float4 v, float a, float b) {
+    float4 m = *acc;
+    m.x = m.x * a + v.x * b;
+    m.y = m.y * a + v.y * b;
+    m.z = m.z * a + v.z * b;
+    *acc = m;
+}
+
+// The planar C x H x W layers stay the state between calls; a render folds into
+// pixel-interleaved copies (one 16-byte read-modify-write per layer and pixel)
+// and writes them back at the end.
+__global__ void __launch_bounds__(256) k_pack_acc(const float* __restrict__ b, const float* __restrict__ d,
+                                                  const float* __restrict__ n, float4* __restrict__ acc, uint32_t HW,
+                                                  uint32_t cap) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= HW) return;
+    acc[i] = make_float4(b[i], b[HW + i], b[2 * (size_t)HW + i], 0.0f);
+    acc[cap + i] = make_float4(d[i], d[HW + i], d[2 * (size_t)HW + i], 0.0f);
+    acc[2 * (size_t)cap + i] = make_float4(n[i], n[HW + i], n[2 * (size_t)HW + i], 0.0f);
+}
+
+__global__ void __launch_bounds__(256) k_unpack_acc(const float4* __restrict__ acc, float* __restrict__ b,
+                                                    float* __restrict__ d, float* __restrict__ n, uint32_t HW,
+                                                    uint32_t cap) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= HW) return;
+    float4 x = acc[i], y = acc[cap + i], z = acc[2 * (size_t)cap + i];
+    b[i] = x.x; b[HW + i] = x.y; b[2 * (size_t)HW + i] = x.z;
+    d[i] = y.x; d[HW + i] = y.y; d[2 * (size_t)HW + i] = y.z;
+    n[i] = z.x; n[HW + i] = z.y; n[2 * (size_t)HW + i] = z.z;
 }
 
 // First sample of every pixel, queued in tile order (closed-form dense slot).
@@ -466,8 +515,8 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
 
 // Fold the finished paths' samples and start their next samples, scanning all
 // pixels in tile order so the new primary rays of a wave are coherent.
-__global__ void __launch_bounds__(256) k_regen(GCamera cam, PathPtrs P, TraceArgs A, float* beauty,
-                                               float* diffuse, float* normal) {
+constexpr int REGEN_BLOCK = 1024;
+__global__ void __launch_bounds__(REGEN_BLOCK) k_regen(GCamera cam, PathPtrs P, TraceArgs A) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, x = 0, y = 0;
     bool restart = false;
     uint32_t i = 0;
@@ -475,20 +524,19 @@ __global__ void __launch_bounds__(256) k_regen(GCamera cam, PathPtrs P, TraceArg
         i = y * cam.W + x;
         if (P.finished[i]) {
             P.finished[i] = 0;
-            uint32_t s = (uint32_t)__float_as_int(P.d[i].w);
+            uint32_t s = (uint32_t)__float_as_int(reinterpret_cast<const float*>(P.d + i)[3]);
             float fn = (float)s;
             float a = (fn - 1.0f) / fn, b = 1.0f / fn;
-            size_t HW = (size_t)cam.W * cam.H;
-            fold(beauty, HW, i, P.L[i], a, b);
-            fold(diffuse, HW, i, P.diff[i], a, b);
-            fold(normal, HW, i, P.nrm[i], a, b);
+            fold(P.acc + i, P.L[i], a, b);
+            fold(P.acc + P.cap + i, P.diff[i], a, b);
+            fold(P.acc + 2 * (size_t)P.cap + i, P.nrm[i], a, b);
             if (s + 1 < A.s_end) {
                 start_path(cam, P, x, y, s + 1, A.seed, A.WH);
                 restart = true;
             }
         }
     }
-    int slot = wave_append(restart, &P.counts[CNT_ACT + (1 - A.in)]);
+    int slot = block_append<REGEN_BLOCK>(restart, &P.counts[CNT_ACT + (1 - A.in)]);
     if (restart) P.qact[(size_t)(1 - A.in) * P.cap + slot] = (int32_t)i;
 }
 
@@ -498,10 +546,9 @@ __global__ void __launch_bounds__(256) k_fold_zero(float* beauty, float* diffuse
     if (i >= n) return;
     float fn = (float)s;
     float a = (fn - 1.0f) / fn, b = 1.0f / fn;
-    float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    fold(beauty, n, i, z, a, b);
-    fold(diffuse, n, i, z, a, b);
-    fold(normal, n, i, z, a, b);
+    float* layers[3] = {beauty, diffuse, normal};
+    for (float* l : layers)
+        for (int c = 0; c < 3; c++) l[(size_t)c * n + i] = l[(size_t)c * n + i] * a + 0.0f * b;
 }
 
 struct PostArgs {
@@ -1037,6 +1084,7 @@ static int ensure_paths(nr_ctx* c, uint32_t n) {
     if (!e) e = al((void**)&P.qmat, sizeof(int32_t) * NR_MAT_TYPES * (size_t)n);
     if (!e) e = al((void**)&P.counts, sizeof(int32_t) * CNT_N);
     if (!e) e = al((void**)&P.finished, sizeof(uint8_t) * (size_t)n);
+    if (!e) e = al((void**)&P.acc, sizeof(float4) * 3 * (size_t)n);
     if (e) { free_list(c->path_mem); return e; }
     HIPCK(hipMemsetAsync(P.finished, 0, n, c->stream));
     HIPCK(hipMemsetAsync(P.counts, 0, sizeof(int32_t) * CNT_N, c->stream));
@@ -1096,6 +1144,7 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
         TraceArgs A{d->seed, (uint32_t)((uint64_t)v.W * v.H), 0u, d->ray_depth, d->do_rr, d->rr_start_bounce,
                     d->rr_survival, d->sample_end};
         mark(2);
+        hipLaunchKernelGGL(k_pack_acc, gN, dim3(256), 0, c->stream, lb, ld, ln, c->P.acc, N, c->P.cap);
         hipLaunchKernelGGL(k_start, gGen, dim3(256), 0, c->stream, v.cam, c->P, d->sample_begin, A);
         // The live count never grows from one iteration to the next (a path
         // either continues, or ends and restarts at most once), so a count read
@@ -1127,7 +1176,8 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
             if (c->mat_present[NR_MAT_EMISSIVE]) hipLaunchKernelGGL(k_shade<NR_MAT_EMISSIVE>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
             if (c->mat_present[NR_MAT_ISOTROPIC]) hipLaunchKernelGGL(k_shade<NR_MAT_ISOTROPIC>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
             mark(2);
-            hipLaunchKernelGGL(k_regen, gGen, dim3(256), 0, c->stream, v.cam, c->P, A, lb, ld, ln);
+            hipLaunchKernelGGL(k_regen, dim3((gen_threads + REGEN_BLOCK - 1) / REGEN_BLOCK), dim3(REGEN_BLOCK), 0,
+                               c->stream, v.cam, c->P, A);
             HIPCK(hipGetLastError());
             HIPCK(hipMemcpyAsync(c->h_live + (it & 1), c->P.counts + CNT_ACT + (1 - A.in), sizeof(int32_t),
                                  hipMemcpyDeviceToHost, c->stream));
@@ -1136,6 +1186,8 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
             it++;
         }
         c->last_iterations = it;
+        mark(2);
+        hipLaunchKernelGGL(k_unpack_acc, gN, dim3(256), 0, c->stream, c->P.acc, lb, ld, ln, N, c->P.cap);
     }
     HIPCK(hipGetLastError());
     if (c->timing) {
