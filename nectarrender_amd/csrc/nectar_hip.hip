@@ -146,6 +146,34 @@ __device__ __forceinline__ int block_append(bool pred, int32_t* counter) {
     return pred ? wbase[wave] + __popcll(below) : -1;
 }
 
+// workgroup-level append of each thread's item to queue `key` (1 <= key < NQ,
+// 0 = none): one atomic per workgroup and non-empty queue.
+template <int BLOCK, int NQ>
+__device__ __forceinline__ int block_append_keyed(int key, int32_t* counters) {
+    constexpr int NW = BLOCK / 64;
+    __shared__ int wcount[NQ][NW];
+    __shared__ int wbase[NQ][NW];
+    const int wave = threadIdx.x >> 6, lane = __lane_id();
+    unsigned long long mine = 0ull;
+    for (int q = 1; q < NQ; q++) {
+        unsigned long long m = __ballot(key == q);
+        if (key == q) mine = m;
+        if (lane == 0) wcount[q][wave] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ && threadIdx.x > 0) {
+        const int q = threadIdx.x;
+        int total = 0;
+        for (int w = 0; w < NW; w++) { wbase[q][w] = total; total += wcount[q][w]; }
+        int base = total ? atomicAdd(&counters[q], total) : 0;
+        for (int w = 0; w < NW; w++) wbase[q][w] += base;
+    }
+    __syncthreads();
+    if (key <= 0) return -1;
+    unsigned long long below = lane == 0 ? 0ull : (mine & ((~0ull) >> (64 - lane)));
+    return wbase[key][wave] + __popcll(below);
+}
+
 __device__ __forceinline__ Stack make_stack(int2* smem, const GScene& S, const StackCfg& SC) {
     int tid = threadIdx.x;
     Stack st;
@@ -357,10 +385,8 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_extend(const GScene* __res
             mtype = ob.pad[0];
         }
     }
-    for (int t = 1; t < NR_MAT_TYPES; t++) {
-        int slot = wave_append(mtype == t, &P.counts[CNT_MAT + t]);
-        if (mtype == t) P.qmat[(size_t)t * P.cap + slot] = path;
-    }
+    const int slot = block_append_keyed<128, NR_MAT_TYPES>(mtype, &P.counts[CNT_MAT]);
+    if (mtype > 0) P.qmat[(size_t)mtype * P.cap + slot] = path;
 }
 
 // material evaluate (material.h:59-67, 152-180, 351-357)
@@ -399,8 +425,8 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
     const int n = P.counts[CNT_MAT + M];
     const int out = 1 - A.in;
     const int stride = gridDim.x * blockDim.x;
-    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
-    int i = base + (threadIdx.x & 63);
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {     // workgroup-uniform trip count
+    int i = base + threadIdx.x;
     bool cont = false;
     int path = -1;
     if (i < n) {
@@ -508,7 +534,7 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
             P.finished[path] = 1;
         }
     }
-    int slot = wave_append(cont, &P.counts[CNT_ACT + out]);
+    int slot = block_append<256>(cont, &P.counts[CNT_ACT + out]);
     if (cont) P.qact[(size_t)out * P.cap + slot] = path;
     }
 }
