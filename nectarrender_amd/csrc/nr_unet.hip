@@ -148,6 +148,111 @@ __global__ void __launch_bounds__(64 * CONV_WAVES) k_conv(ConvArgs a) {
         }
 }
 
+// LDS-tiled variant: a workgroup computes 4 rows x 64 pixels x (16 TM) output
+// channels.  For each chunk of 32 input channels the 6 x 66-pixel halo tile is
+// staged in LDS once and feeds all 9 taps (each input element is read from
+// memory ~1.5 times instead of 9); weights are [rows][tap][channels padded to
+// 32], so one K step of 32 is one tap of one chunk.
+struct ConvLdsArgs {
+    const uint16_t* in; int in_cs, in_off, cin_avail;   // channels present in the input tensor
+    int c32;                                           // padded channels per tap of the weights
+    int N, H, W;
+    const uint16_t* w;                                 // [rows][9][c32]
+    const float* bias;
+    int cout;
+    int relu;
+    uint16_t* out; int out_cs, out_off;
+    const uint16_t* mask; int mask_cs, mask_off;
+    float* out_f32; const float* resid;
+};
+
+constexpr int CL_TILE_W = 64, CL_TILE_H = 4;
+
+template <int TM>
+__global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
+    __shared__ uint4 tile[CL_TILE_H + 2][CL_TILE_W + 2][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, kq = lane >> 4;
+    const int x0 = blockIdx.x * CL_TILE_W;
+    const int rowblocks = (a.H + CL_TILE_H - 1) / CL_TILE_H;
+    const int n = blockIdx.y / rowblocks, y0 = (blockIdx.y - n * rowblocks) * CL_TILE_H;
+    const int y = y0 + wave;
+    const int cobase = blockIdx.z * 16 * TM;
+    f32x4 acc[TM][4];
+    for (int m = 0; m < TM; m++)
+        for (int t = 0; t < 4; t++) acc[m][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int nchunks = a.c32 >> 5;
+    constexpr int NV = (CL_TILE_H + 2) * (CL_TILE_W + 2) * 4;
+    for (int c = 0; c < nchunks; c++) {
+        // all of this thread's tile loads in flight at once, then the LDS stores
+        constexpr int NLD = (NV + 255) / 256;
+        uint4 val[NLD];
+#pragma unroll
+        for (int k = 0; k < NLD; k++) {
+            const int v = threadIdx.x + k * 256;
+            const int g = v & 3, px = v >> 2;
+            const int row = px / (CL_TILE_W + 2), col = px - row * (CL_TILE_W + 2);
+            const int yy = y0 - 1 + row, xx = x0 - 1 + col, ch = c * 32 + g * 8;
+            val[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (v < NV && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && ch < a.cin_avail)
+                val[k] = *reinterpret_cast<const uint4*>(a.in + ((size_t)(n * a.H + yy) * a.W + xx) * a.in_cs + a.in_off + ch);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NLD; k++) {
+            const int v = threadIdx.x + k * 256;
+            if (v < NV) (&tile[0][0][0])[v] = val[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap - 3 * (tap / 3);
+            bf16x8 b[4];
+            for (int t = 0; t < 4; t++) b[t] = __builtin_bit_cast(bf16x8, tile[wave + dy][t * 16 + r + dx][kq]);
+            for (int m = 0; m < TM; m++) {
+                const int co = cobase + m * 16 + r;
+                uint4 wv = *reinterpret_cast<const uint4*>(a.w + ((size_t)co * 9 + tap) * a.c32 + c * 32 + kq * 8);
+                bf16x8 af = __builtin_bit_cast(bf16x8, wv);
+                for (int t = 0; t < 4; t++)
+                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b[t], acc[m][t], 0, 0, 0);
+            }
+        }
+    }
+    if (y >= a.H) return;
+    for (int m = 0; m < TM; m++)
+        for (int t = 0; t < 4; t++) {
+            const int x = x0 + t * 16 + r;
+            if (x >= a.W) continue;
+            const size_t p = ((size_t)n * a.H + y) * a.W + x;
+            const int co0 = cobase + m * 16 + kq * 4;
+            float v[4];
+            for (int j = 0; j < 4; j++) {
+                const int co = co0 + j;
+                float xv = acc[m][t][j] + ((a.bias && co < a.cout) ? a.bias[co] : 0.0f);
+                if (a.relu) xv = fmaxf(xv, 0.0f);
+                if (a.mask && co < a.cout && !(bf2f(a.mask[p * a.mask_cs + a.mask_off + co]) > 0.0f)) xv = 0.0f;
+                v[j] = xv;
+            }
+            if (a.out_f32) {
+                for (int j = 0; j < 4; j++) {
+                    const int co = co0 + j;
+                    if (co < a.cout && co < 4) a.out_f32[p * 4 + co] = v[j] + a.resid[p * 4 + co];
+                }
+            } else {
+                uint16_t* o = a.out + p * a.out_cs + a.out_off + co0;
+                if (co0 + 3 < a.cout) {
+                    uint2 pk;
+                    pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+                    pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                    *reinterpret_cast<uint2*>(o) = pk;
+                } else {
+                    for (int j = 0; j < 4; j++)
+                        if (co0 + j < a.cout) o[j] = f2bf(v[j]);
+                }
+            }
+        }
+}
+
 // ---------------------------------------------------------------------------
 // weight gradient: one wave per (tap, 16 co, 16 ci) block over a pixel range
 // ---------------------------------------------------------------------------
@@ -221,62 +326,92 @@ __global__ void __launch_bounds__(256) k_bias_grad(const uint16_t* g, int g_cs, 
 // ---------------------------------------------------------------------------
 // pooling, upsampling, input/output transforms, loss, optimizer
 // ---------------------------------------------------------------------------
+// Pooling and upsampling move 8 channels (16 B) per thread; every tensor's
+// channel count, stride and offset are multiples of 8.
+__device__ inline void ld8(const uint16_t* p, float* f) {
+    uint4 v = *reinterpret_cast<const uint4*>(p);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    for (int k = 0; k < 4; k++) { f[2 * k] = bf2f((uint16_t)(w[k] & 0xFFFFu)); f[2 * k + 1] = bf2f((uint16_t)(w[k] >> 16)); }
+}
+__device__ inline void st8(uint16_t* p, const float* f) {
+    uint32_t w[4];
+    for (int k = 0; k < 4; k++) w[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+    *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 // 2x2 average pool; src dims H x W, dst H/2 x W/2
 __global__ void k_pool(View s, View d, int C, int N, int H, int W) {
-    const int Ho = H >> 1, Wo = W >> 1;
-    const size_t total = (size_t)N * Ho * Wo * C;
+    const int C8 = C >> 3, Ho = H >> 1, Wo = W >> 1;
+    const size_t total = (size_t)N * Ho * Wo * C8;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    const int c = (int)(i % C);
-    const size_t q = i / C;
+    const int c = (int)(i % C8) * 8;
+    const size_t q = i / C8;
     const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((size_t)Wo * Ho));
     const size_t p00 = ((size_t)n * H + 2 * yo) * W + 2 * xo;
-    float v = bf2f(s.p[p00 * s.cs + s.off + c]) + bf2f(s.p[(p00 + 1) * s.cs + s.off + c]) +
-              bf2f(s.p[(p00 + W) * s.cs + s.off + c]) + bf2f(s.p[(p00 + W + 1) * s.cs + s.off + c]);
-    d.p[q * d.cs + d.off + c] = f2bf(0.25f * v);
+    float a[8], b[8], e[8], f[8], o[8];
+    ld8(s.p + p00 * s.cs + s.off + c, a);
+    ld8(s.p + (p00 + 1) * s.cs + s.off + c, b);
+    ld8(s.p + (p00 + W) * s.cs + s.off + c, e);
+    ld8(s.p + (p00 + W + 1) * s.cs + s.off + c, f);
+    for (int k = 0; k < 8; k++) o[k] = 0.25f * (a[k] + b[k] + e[k] + f[k]);
+    st8(d.p + q * d.cs + d.off + c, o);
 }
 
 // d[p] = (skip[p] + 0.25 * gdown[p/2]) * (mask[p] > 0); dims of d: H x W
 __global__ void k_pool_bwd(View gdown, View skip, View mask, View d, int C, int N, int H, int W) {
-    const size_t total = (size_t)N * H * W * C;
+    const int C8 = C >> 3;
+    const size_t total = (size_t)N * H * W * C8;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    const int c = (int)(i % C);
-    const size_t p = i / C;
+    const int c = (int)(i % C8) * 8;
+    const size_t p = i / C8;
     const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
     const size_t q = ((size_t)n * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1);
-    float v = 0.25f * bf2f(gdown.p[q * gdown.cs + gdown.off + c]);
-    if (skip.p) v += bf2f(skip.p[p * skip.cs + skip.off + c]);
-    if (!(bf2f(mask.p[p * mask.cs + mask.off + c]) > 0.0f)) v = 0.0f;
-    d.p[p * d.cs + d.off + c] = f2bf(v);
+    float g[8], sk[8], m[8], o[8];
+    ld8(gdown.p + q * gdown.cs + gdown.off + c, g);
+    ld8(mask.p + p * mask.cs + mask.off + c, m);
+    if (skip.p) ld8(skip.p + p * skip.cs + skip.off + c, sk);
+    for (int k = 0; k < 8; k++) {
+        float v = 0.25f * g[k];
+        if (skip.p) v += sk[k];
+        o[k] = m[k] > 0.0f ? v : 0.0f;
+    }
+    st8(d.p + p * d.cs + d.off + c, o);
 }
 
 // nearest 2x upsample; dst dims H x W, src H/2 x W/2
 __global__ void k_up(View s, View d, int C, int N, int H, int W) {
-    const size_t total = (size_t)N * H * W * C;
+    const int C8 = C >> 3;
+    const size_t total = (size_t)N * H * W * C8;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    const int c = (int)(i % C);
-    const size_t p = i / C;
+    const int c = (int)(i % C8) * 8;
+    const size_t p = i / C8;
     const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
     const size_t q = ((size_t)n * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1);
-    d.p[p * d.cs + d.off + c] = s.p[q * s.cs + s.off + c];
+    *reinterpret_cast<uint4*>(d.p + p * d.cs + d.off + c) = *reinterpret_cast<const uint4*>(s.p + q * s.cs + s.off + c);
 }
 
 // d[q] = sum of g over the 2x2 children of q, times (mask[q] > 0); dst dims Ho x Wo
 __global__ void k_up_bwd(View g, View mask, View d, int C, int N, int Ho, int Wo) {
-    const size_t total = (size_t)N * Ho * Wo * C;
+    const int C8 = C >> 3;
+    const size_t total = (size_t)N * Ho * Wo * C8;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    const int c = (int)(i % C);
-    const size_t q = i / C;
+    const int c = (int)(i % C8) * 8;
+    const size_t q = i / C8;
     const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((size_t)Wo * Ho));
     const int W = Wo * 2;
     const size_t p00 = ((size_t)n * (Ho * 2) + 2 * yo) * W + 2 * xo;
-    float v = bf2f(g.p[p00 * g.cs + g.off + c]) + bf2f(g.p[(p00 + 1) * g.cs + g.off + c]) +
-              bf2f(g.p[(p00 + W) * g.cs + g.off + c]) + bf2f(g.p[(p00 + W + 1) * g.cs + g.off + c]);
-    if (!(bf2f(mask.p[q * mask.cs + mask.off + c]) > 0.0f)) v = 0.0f;
-    d.p[q * d.cs + d.off + c] = f2bf(v);
+    float a[8], b[8], e[8], f[8], m[8], o[8];
+    ld8(g.p + p00 * g.cs + g.off + c, a);
+    ld8(g.p + (p00 + 1) * g.cs + g.off + c, b);
+    ld8(g.p + (p00 + W) * g.cs + g.off + c, e);
+    ld8(g.p + (p00 + W + 1) * g.cs + g.off + c, f);
+    ld8(mask.p + q * mask.cs + mask.off + c, m);
+    for (int k = 0; k < 8; k++) o[k] = m[k] > 0.0f ? (a[k] + b[k] + e[k] + f[k]) : 0.0f;
+    st8(d.p + q * d.cs + d.off + c, o);
 }
 
 // planar N x 3 x H x W inputs -> X0 (16 channels, padded dims Hp x Wp) and log1p beauty (4-stride f32)
@@ -399,6 +534,29 @@ __global__ void k_pack_bwd(const float* w, int cout, int cin, int cout_g, int ci
     o[i] = f2bf(v);
 }
 
+// fp32 [cout][9][cin] -> bf16 [rows][9][c32] (forward: rows = output channels)
+__global__ void k_pack_fwd_tap(const float* w, int cout, int cin, int c32, int rows, uint16_t* o) {
+    const size_t total = (size_t)rows * 9 * c32;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int ci = (int)(i % c32);
+    const int tap = (int)((i / c32) % 9);
+    const int co = (int)(i / ((size_t)9 * c32));
+    o[i] = f2bf(co < cout && ci < cin ? w[((size_t)co * 9 + tap) * cin + ci] : 0.0f);
+}
+
+// input-gradient weights, rows = input channels: element (ci, tap, co) = W[co][8 - tap][ci]
+__global__ void k_pack_bwd_tap(const float* w, int cout, int cin, int c32, int rows, uint16_t* o) {
+    const size_t total = (size_t)rows * 9 * c32;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int co = (int)(i % c32);
+    const int tap = (int)((i / c32) % 9);
+    const int ci = (int)(i / ((size_t)9 * c32));
+    o[i] = f2bf(ci < cin && co < cout ? w[((size_t)co * 9 + (8 - tap)) * cin + ci] : 0.0f);
+}
+
+inline int up32(int x) { return (x + 31) & ~31; }
 inline int up16(int x) { return (x + 15) & ~15; }
 inline int up4(int x) { return (x + 3) & ~3; }
 inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
@@ -416,8 +574,11 @@ struct Layer {
     size_t w_off, b_off;      // offsets into the flat parameter vector
     int cout_pad, kg_pad;     // forward packing
     int cin_pad, kgb_pad;     // input-gradient packing
+    int fc32, bc32;           // tap-major packings: channels per tap (forward, input gradient)
     uint16_t* wf = nullptr;
     uint16_t* wb = nullptr;
+    uint16_t* wf2 = nullptr;
+    uint16_t* wb2 = nullptr;
 };
 
 struct Net {
@@ -433,6 +594,8 @@ struct Net {
             l.kg_pad = up4(9 * (cin_t / 8));
             l.cin_pad = up16(cin);
             l.kgb_pad = up4(9 * (l.cout_pad / 8));
+            l.fc32 = up32(cin_t);
+            l.bc32 = up32(l.cout_pad);
             L.push_back(l);
         };
         int prev = IN_CH, prev_t = IN_PAD;
@@ -486,6 +649,12 @@ int pack_all(nr_unet* u) {
         size_t nb = (size_t)l.cin_pad * l.kgb_pad * 8;
         hipLaunchKernelGGL(k_pack_bwd, dim3(blocks(nb, 256)), dim3(256), 0, u->stream, u->prm + l.w_off, l.cout, l.cin,
                            l.cout_pad / 8, l.cin_pad, l.kgb_pad, l.wb);
+        size_t nf2 = (size_t)l.cout_pad * 9 * l.fc32;
+        hipLaunchKernelGGL(k_pack_fwd_tap, dim3(blocks(nf2, 256)), dim3(256), 0, u->stream, u->prm + l.w_off, l.cout,
+                           l.cin, l.fc32, l.cout_pad, l.wf2);
+        size_t nb2 = (size_t)l.cin_pad * 9 * l.bc32;
+        hipLaunchKernelGGL(k_pack_bwd_tap, dim3(blocks(nb2, 256)), dim3(256), 0, u->stream, u->prm + l.w_off, l.cout,
+                           l.cin, l.bc32, l.cin_pad, l.wb2);
     }
     UCK(hipGetLastError());
     return NR_OK;
@@ -540,11 +709,11 @@ double conv_flops(const Layer& l, size_t P) { return 2.0 * P * 9.0 * l.cin * l.c
 
 void launch_conv(nr_unet* u, const Layer& l, bool backward, View in, int cin_groups, int N, int H, int W, View out,
                  int cout, bool relu, const float* bias, View mask, float* out_f32, const float* resid) {
-    ConvArgs a{};
-    a.in = in.p; a.in_cs = in.cs; a.in_off = in.off; a.cin_g = cin_groups;
+    ConvLdsArgs a{};
+    a.in = in.p; a.in_cs = in.cs; a.in_off = in.off; a.cin_avail = cin_groups * 8;
+    a.c32 = backward ? l.bc32 : l.fc32;
     a.N = N; a.H = H; a.W = W;
-    a.w = backward ? l.wb : l.wf;
-    a.kg_pad = backward ? l.kgb_pad : l.kg_pad;
+    a.w = backward ? l.wb2 : l.wf2;
     a.bias = bias;
     a.cout = cout;
     a.relu = relu;
@@ -552,18 +721,14 @@ void launch_conv(nr_unet* u, const Layer& l, bool backward, View in, int cin_gro
     a.mask = mask.p; a.mask_cs = mask.cs; a.mask_off = mask.off;
     a.out_f32 = out_f32; a.resid = resid;
     const int cpad = up16(cout);
-    const int P = N * H * W;
-    const int TN = 4;
-    const int pix_per_wg = 16 * TN * CONV_WAVES;
-    dim3 block(64 * CONV_WAVES);
-    // output-channel tiles per wave: 16 * TM divides cpad
     int TM = (cpad % 64 == 0) ? 4 : (cpad % 48 == 0) ? 3 : (cpad % 32 == 0) ? 2 : 1;
-    dim3 grid((P + pix_per_wg - 1) / pix_per_wg, cpad / (16 * TM));
+    dim3 grid((W + CL_TILE_W - 1) / CL_TILE_W, N * ((H + CL_TILE_H - 1) / CL_TILE_H), cpad / (16 * TM));
+    dim3 block(256);
     switch (TM) {
-        case 4: hipLaunchKernelGGL((k_conv<4, TN>), grid, block, 0, u->stream, a); break;
-        case 3: hipLaunchKernelGGL((k_conv<3, TN>), grid, block, 0, u->stream, a); break;
-        case 2: hipLaunchKernelGGL((k_conv<2, TN>), grid, block, 0, u->stream, a); break;
-        default: hipLaunchKernelGGL((k_conv<1, TN>), grid, block, 0, u->stream, a); break;
+        case 4: hipLaunchKernelGGL(k_conv_lds<4>, grid, block, 0, u->stream, a); break;
+        case 3: hipLaunchKernelGGL(k_conv_lds<3>, grid, block, 0, u->stream, a); break;
+        case 2: hipLaunchKernelGGL(k_conv_lds<2>, grid, block, 0, u->stream, a); break;
+        default: hipLaunchKernelGGL(k_conv_lds<1>, grid, block, 0, u->stream, a); break;
     }
 }
 
@@ -603,7 +768,7 @@ void forward(nr_unet* u, int N, int H, int W) {
         launch_conv(u, lb, false, a, CH[l] / 8, N, h, w, e, CH[l], true, u->prm + lb.b_off, none, nullptr, nullptr);
         if (l < LV - 1) {
             View xn{u->X[l + 1], CH[l], 0};
-            size_t tot = (size_t)N * (h >> 1) * (w >> 1) * CH[l];
+            size_t tot = (size_t)N * (h >> 1) * (w >> 1) * (CH[l] / 8);
             hipLaunchKernelGGL(k_pool, dim3(blocks(tot, 256)), dim3(256), 0, u->stream, e, xn, CH[l], N, h, w);
         }
     }
@@ -612,7 +777,7 @@ void forward(nr_unet* u, int N, int H, int W) {
         const int cd = CH[l] + CH[l + 1];
         View src = l == LV - 2 ? View{u->E4, CH[l + 1], 0} : View{u->DO[l + 1], CH[l + 1], 0};
         View up{u->D[l], cd, CH[l]};
-        size_t tot = (size_t)N * h * w * CH[l + 1];
+        size_t tot = (size_t)N * h * w * (CH[l + 1] / 8);
         hipLaunchKernelGGL(k_up, dim3(blocks(tot, 256)), dim3(256), 0, u->stream, src, up, CH[l + 1], N, h, w);
         Layer& la = n.dec(l, 0);
         Layer& lb = n.dec(l, 1);
@@ -649,7 +814,7 @@ void backward(nr_unet* u, int N, int H, int W) {
         // upsampled half of the concatenation -> the coarser decoder output (or the bottleneck)
         View src_mask = l == LV - 2 ? View{u->E4, CH[l + 1], 0} : View{u->DO[l + 1], CH[l + 1], 0};
         View dst = l == LV - 2 ? View{u->gE[LV - 1], CH[l + 1], 0} : View{u->gDO[l + 1], CH[l + 1], 0};
-        size_t tot = (size_t)N * (h >> 1) * (w >> 1) * CH[l + 1];
+        size_t tot = (size_t)N * (h >> 1) * (w >> 1) * (CH[l + 1] / 8);
         hipLaunchKernelGGL(k_up_bwd, dim3(blocks(tot, 256)), dim3(256), 0, u->stream, View{u->gD[l], cd, CH[l]},
                            src_mask, dst, CH[l + 1], N, h >> 1, w >> 1);
     }
@@ -662,7 +827,7 @@ void backward(nr_unet* u, int N, int H, int W) {
         Layer& lb = n.enc(l, 1);
         if (l < LV - 1) {
             const int cd = CH[l] + CH[l + 1];
-            size_t tot = (size_t)N * h * w * CH[l];
+            size_t tot = (size_t)N * h * w * (CH[l] / 8);
             hipLaunchKernelGGL(k_pool_bwd, dim3(blocks(tot, 256)), dim3(256), 0, u->stream,
                                View{u->gX[l + 1], CH[l], 0}, View{u->gD[l], cd, 0}, View{u->D[l], cd, 0},
                                View{u->gE[l], CH[l], 0}, CH[l], N, h, w);
@@ -742,12 +907,15 @@ nr_unet* nr_unet_create(int device, uint32_t seed) {
         return bad("hipMalloc parameters");
     size_t wbytes = 0;
     for (Layer& l : u->net.L)
-        wbytes += ((size_t)l.cout_pad * l.kg_pad * 8 + (size_t)l.cin_pad * l.kgb_pad * 8) * 2 + 512;
+        wbytes += ((size_t)l.cout_pad * l.kg_pad * 8 + (size_t)l.cin_pad * l.kgb_pad * 8 +
+                   (size_t)l.cout_pad * 9 * l.fc32 + (size_t)l.cin_pad * 9 * l.bc32) * 2 + 1024;
     if (hipMalloc((void**)&u->wmem, wbytes) != hipSuccess) return bad("hipMalloc weights");
     size_t o = 0;
     for (Layer& l : u->net.L) {
         l.wf = u->wmem + o; o += ((size_t)l.cout_pad * l.kg_pad * 8 + 127) & ~(size_t)127;
         l.wb = u->wmem + o; o += ((size_t)l.cin_pad * l.kgb_pad * 8 + 127) & ~(size_t)127;
+        l.wf2 = u->wmem + o; o += ((size_t)l.cout_pad * 9 * l.fc32 + 127) & ~(size_t)127;
+        l.wb2 = u->wmem + o; o += ((size_t)l.cin_pad * 9 * l.bc32 + 127) & ~(size_t)127;
     }
     // He-normal weights, zero biases; the output layer starts small (near-identity residual)
     std::vector<float> h(np, 0.0f);
