@@ -40,6 +40,11 @@ __host__ __device__ inline uint16_t f2bf(float f) {          // round to nearest
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
 }
+typedef __bf16 bf16x2 __attribute__((__vector_size__(2 * sizeof(__bf16))));
+__device__ inline uint32_t pk_bf16(float lo, float hi) {   // v_cvt_pk_bf16_f32: round to nearest even
+    bf16x2 v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, v);
+}
 __host__ __device__ inline float bf2f(uint16_t h) {
     uint32_t u = (uint32_t)h << 16;
     float f;
@@ -137,8 +142,8 @@ __global__ void __launch_bounds__(64 * CONV_WAVES) k_conv(ConvArgs a) {
                 uint16_t* o = a.out + (size_t)p * a.out_cs + a.out_off + co0;
                 if (co0 + 3 < a.cout) {
                     uint2 pk;
-                    pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-                    pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                    pk.x = pk_bf16(v[0], v[1]);
+                    pk.y = pk_bf16(v[2], v[3]);
                     *reinterpret_cast<uint2*>(o) = pk;
                 } else {
                     for (int j = 0; j < 4; j++)
@@ -152,7 +157,9 @@ __global__ void __launch_bounds__(64 * CONV_WAVES) k_conv(ConvArgs a) {
 // channels.  For each chunk of 32 input channels the 6 x 66-pixel halo tile is
 // staged in LDS once and feeds all 9 taps (each input element is read from
 // memory ~1.5 times instead of 9); weights are [rows][tap][channels padded to
-// 32], so one K step of 32 is one tap of one chunk.
+// 32], so one K step of 32 is one tap of one chunk.  The tile is stored
+// [row][channel group][column]: the 16 lanes that share a channel group read 16
+// consecutive 16-byte columns, one contiguous 256-byte run with no bank conflict.
 struct ConvLdsArgs {
     const uint16_t* in; int in_cs, in_off, cin_avail;   // channels present in the input tensor
     int c32;                                           // padded channels per tap of the weights
@@ -170,7 +177,7 @@ constexpr int CL_TILE_W = 64, CL_TILE_H = 4;
 
 template <int TM>
 __global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
-    __shared__ uint4 tile[CL_TILE_H + 2][CL_TILE_W + 2][4];
+    __shared__ uint4 tile[CL_TILE_H + 2][4][CL_TILE_W + 2];   // [row][8-channel group][column]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, kq = lane >> 4;
     const int x0 = blockIdx.x * CL_TILE_W;
@@ -179,7 +186,9 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
     const int y = y0 + wave;
     const int cobase = blockIdx.z * 16 * TM;
     f32x4 acc[TM][4];
+#pragma unroll
     for (int m = 0; m < TM; m++)
+#pragma unroll
         for (int t = 0; t < 4; t++) acc[m][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const int nchunks = a.c32 >> 5;
     constexpr int NV = (CL_TILE_H + 2) * (CL_TILE_W + 2) * 4;
@@ -201,31 +210,39 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
 #pragma unroll
         for (int k = 0; k < NLD; k++) {
             const int v = threadIdx.x + k * 256;
-            if (v < NV) (&tile[0][0][0])[v] = val[k];
+            const int g = v & 3, px = v >> 2;
+            const int row = px / (CL_TILE_W + 2), col = px - row * (CL_TILE_W + 2);
+            if (v < NV) tile[row][g][col] = val[k];
         }
         __syncthreads();
 #pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap - 3 * (tap / 3);
             bf16x8 b[4];
-            for (int t = 0; t < 4; t++) b[t] = __builtin_bit_cast(bf16x8, tile[wave + dy][t * 16 + r + dx][kq]);
+#pragma unroll
+            for (int t = 0; t < 4; t++) b[t] = __builtin_bit_cast(bf16x8, tile[wave + dy][kq][t * 16 + r + dx]);
+#pragma unroll
             for (int m = 0; m < TM; m++) {
                 const int co = cobase + m * 16 + r;
                 uint4 wv = *reinterpret_cast<const uint4*>(a.w + ((size_t)co * 9 + tap) * a.c32 + c * 32 + kq * 8);
                 bf16x8 af = __builtin_bit_cast(bf16x8, wv);
+#pragma unroll
                 for (int t = 0; t < 4; t++)
                     acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b[t], acc[m][t], 0, 0, 0);
             }
         }
     }
     if (y >= a.H) return;
+#pragma unroll
     for (int m = 0; m < TM; m++)
+#pragma unroll
         for (int t = 0; t < 4; t++) {
             const int x = x0 + t * 16 + r;
             if (x >= a.W) continue;
             const size_t p = ((size_t)n * a.H + y) * a.W + x;
             const int co0 = cobase + m * 16 + kq * 4;
             float v[4];
+#pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int co = co0 + j;
                 float xv = acc[m][t][j] + ((a.bias && co < a.cout) ? a.bias[co] : 0.0f);
@@ -242,8 +259,8 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
                 uint16_t* o = a.out + p * a.out_cs + a.out_off + co0;
                 if (co0 + 3 < a.cout) {
                     uint2 pk;
-                    pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-                    pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                    pk.x = pk_bf16(v[0], v[1]);
+                    pk.y = pk_bf16(v[2], v[3]);
                     *reinterpret_cast<uint2*>(o) = pk;
                 } else {
                     for (int j = 0; j < 4; j++)
@@ -335,7 +352,7 @@ __device__ inline void ld8(const uint16_t* p, float* f) {
 }
 __device__ inline void st8(uint16_t* p, const float* f) {
     uint32_t w[4];
-    for (int k = 0; k < 4; k++) w[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+    for (int k = 0; k < 4; k++) w[k] = pk_bf16(f[2 * k], f[2 * k + 1]);
     *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
@@ -434,7 +451,7 @@ __global__ void k_prep(const float* beauty, const float* albedo, const float* no
         }
     }
     uint32_t pk[8];
-    for (int k = 0; k < 8; k++) pk[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+    for (int k = 0; k < 8; k++) pk[k] = pk_bf16(f[2 * k], f[2 * k + 1]);
     uint4* o = reinterpret_cast<uint4*>(x0 + p * 16);
     o[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
     o[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
