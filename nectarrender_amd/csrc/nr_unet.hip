@@ -1,15 +1,18 @@
 // nr_unet.hip — learned U-Net denoiser for gfx950 (include/nectar_denoise.h).
 //
-// Tensors are NHWC bf16 ("views": base, channel stride, channel offset), so a
-// skip concatenation is two producers writing disjoint channel ranges of one
-// buffer.  Every 3x3 convolution is an implicit GEMM on the matrix cores:
+// Tensors are NHWC bf16 ("views": base, channel stride, channel offset).  Every
+// 3x3 convolution is an implicit GEMM on the matrix cores:
 //   forward  D[co][pix] = sum_k W[co][k] X[k][pix], k = (tap, ci)
 //   input gradient: the same kernel with the weights flipped and transposed
 //   weight gradient dW[co][tap][ci] = sum_pix G[co][pix] X[ci][pix+tap]
 // using v_mfma_f32_16x16x32_bf16 (16 x 16 output tile, K = 32, fp32
-// accumulation).  Forward/input-gradient tiles read 8 consecutive channels
-// (16 B) per lane straight from HBM/L2; the weight gradient transposes
-// 32-pixel tiles through LDS so the reduction runs over pixels.
+// accumulation).  Forward/input-gradient convolutions stage a halo tile of 32
+// input channels in LDS and run all 9 taps from it; the encoder's 2x2 average
+// pool is fused into its second convolution's epilogue and the decoder's skip
+// concatenation + nearest upsample into its first convolution's tile loader, so
+// neither the pooled-away nor the upsampled tensor makes an extra HBM round
+// trip.  The weight gradient transposes 32-pixel tiles through LDS so the
+// reduction runs over pixels.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -60,99 +63,6 @@ struct View {             // element (pixel p, channel c) at p[pix * cs + off + 
 // ---------------------------------------------------------------------------
 // convolution (forward and input gradient)
 // ---------------------------------------------------------------------------
-struct ConvArgs {
-    const uint16_t* in; int in_cs, in_off, cin_g;   // cin_g groups of 8 input channels
-    int N, H, W;
-    const uint16_t* w; int kg_pad;                  // w[co][kg_pad * 8], kg = tap * cin_g + c8
-    const float* bias;                              // may be null
-    int cout;                                       // channels stored
-    int relu;
-    uint16_t* out; int out_cs, out_off;
-    const uint16_t* mask; int mask_cs, mask_off;    // result *= (mask > 0) when non-null
-    float* out_f32; const float* resid;             // final layer: out_f32[p*4+c] = acc + bias + resid[p*4+c]
-};
-
-constexpr int CONV_WAVES = 4;
-
-template <int TM, int TN>
-__global__ void __launch_bounds__(64 * CONV_WAVES) k_conv(ConvArgs a) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int P = a.N * a.H * a.W;
-    const int pbase = (blockIdx.x * CONV_WAVES + wave) * 16 * TN;
-    if (pbase >= P) return;
-    const int cobase = blockIdx.y * 16 * TM;
-    const int r = lane & 15, kq = lane >> 4;
-    const int kg_total = 9 * a.cin_g;
-    int py[TN], px[TN];
-    bool pv[TN];
-    for (int n = 0; n < TN; n++) {
-        int p = pbase + n * 16 + r;
-        pv[n] = p < P;
-        int q = pv[n] ? p : 0;
-        px[n] = q % a.W;
-        py[n] = (q / a.W) % a.H;
-    }
-    f32x4 acc[TM][TN];
-    for (int m = 0; m < TM; m++)
-        for (int n = 0; n < TN; n++) acc[m][n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int ksteps = a.kg_pad >> 2;
-    for (int ks = 0; ks < ksteps; ks++) {
-        const int g = ks * 4 + kq;
-        bf16x8 bfrag[TN];
-        const bool gv = g < kg_total;
-        const int tap = gv ? g / a.cin_g : 0, c8 = gv ? g - tap * a.cin_g : 0;
-        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-        for (int n = 0; n < TN; n++) {
-            const int yy = py[n] + dy, xx = px[n] + dx;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (gv && pv[n] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
-                const int q = pbase + n * 16 + r + dy * a.W + dx;
-                v = *reinterpret_cast<const uint4*>(a.in + (size_t)q * a.in_cs + a.in_off + c8 * 8);
-            }
-            bfrag[n] = __builtin_bit_cast(bf16x8, v);
-        }
-        for (int m = 0; m < TM; m++) {
-            const int co = cobase + m * 16 + r;
-            uint4 v = *reinterpret_cast<const uint4*>(a.w + ((size_t)co * a.kg_pad + g) * 8);
-            bf16x8 afrag = __builtin_bit_cast(bf16x8, v);
-            for (int n = 0; n < TN; n++)
-                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag[n], acc[m][n], 0, 0, 0);
-        }
-    }
-    // D[co][pix]: lane holds column pix = r, rows co = 4 * kq + j
-    for (int m = 0; m < TM; m++)
-        for (int n = 0; n < TN; n++) {
-            if (!pv[n]) continue;
-            const int p = pbase + n * 16 + r;
-            const int co0 = cobase + m * 16 + kq * 4;
-            float v[4];
-            for (int j = 0; j < 4; j++) {
-                const int co = co0 + j;
-                float x = acc[m][n][j] + ((a.bias && co < a.cout) ? a.bias[co] : 0.0f);
-                if (a.relu) x = fmaxf(x, 0.0f);
-                if (a.mask && co < a.cout && !(bf2f(a.mask[(size_t)p * a.mask_cs + a.mask_off + co]) > 0.0f)) x = 0.0f;
-                v[j] = x;
-            }
-            if (a.out_f32) {
-                for (int j = 0; j < 4; j++) {
-                    const int co = co0 + j;
-                    if (co < a.cout && co < 4) a.out_f32[(size_t)p * 4 + co] = v[j] + a.resid[(size_t)p * 4 + co];
-                }
-            } else {
-                uint16_t* o = a.out + (size_t)p * a.out_cs + a.out_off + co0;
-                if (co0 + 3 < a.cout) {
-                    uint2 pk;
-                    pk.x = pk_bf16(v[0], v[1]);
-                    pk.y = pk_bf16(v[2], v[3]);
-                    *reinterpret_cast<uint2*>(o) = pk;
-                } else {
-                    for (int j = 0; j < 4; j++)
-                        if (co0 + j < a.cout) o[j] = f2bf(v[j]);
-                }
-            }
-        }
-}
-
 // LDS-tiled variant: a workgroup computes 4 rows x 64 pixels x (16 TM) output
 // channels.  For each chunk of 32 input channels the 6 x 66-pixel halo tile is
 // staged in LDS once and feeds all 9 taps (each input element is read from
@@ -162,6 +72,7 @@ __global__ void __launch_bounds__(64 * CONV_WAVES) k_conv(ConvArgs a) {
 // consecutive 16-byte columns, one contiguous 256-byte run with no bank conflict.
 struct ConvLdsArgs {
     const uint16_t* in; int in_cs, in_off, cin_avail;   // channels present in the input tensor
+    const uint16_t* up; int up_cs, up_off, up_c0;      // channels >= up_c0: nearest 2x upsample of a (H/2 x W/2) tensor
     int c32;                                           // padded channels per tap of the weights
     int N, H, W;
     const uint16_t* w;                                 // [rows][9][c32]
@@ -169,13 +80,22 @@ struct ConvLdsArgs {
     int cout;
     int relu;
     uint16_t* out; int out_cs, out_off;
+    uint16_t* pool; int pool_cs, pool_off;             // also store the 2x2 average of the stored values (H, W even)
     const uint16_t* mask; int mask_cs, mask_off;
     float* out_f32; const float* resid;
 };
 
+// value of the lane next to this one (lane ^ 1), through DPP quad_perm [1,0,3,2]
+__device__ inline float lane_xor1(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
 constexpr int CL_TILE_W = 64, CL_TILE_H = 4;
 
-template <int TM>
+// UP: the loader reads channels >= up_c0 from the upsampled tensor; POOL: the
+// fused-pool epilogue (forward encoder output).  Both are compile-time so the
+// plain convolutions carry neither's registers.
+template <int TM, bool UP, bool POOL>
 __global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
     __shared__ uint4 tile[CL_TILE_H + 2][4][CL_TILE_W + 2];   // [row][8-channel group][column]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -203,8 +123,13 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
             const int row = px / (CL_TILE_W + 2), col = px - row * (CL_TILE_W + 2);
             const int yy = y0 - 1 + row, xx = x0 - 1 + col, ch = c * 32 + g * 8;
             val[k] = make_uint4(0u, 0u, 0u, 0u);
-            if (v < NV && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && ch < a.cin_avail)
-                val[k] = *reinterpret_cast<const uint4*>(a.in + ((size_t)(n * a.H + yy) * a.W + xx) * a.in_cs + a.in_off + ch);
+            if (v < NV && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && ch < a.cin_avail) {
+                const uint16_t* src =
+                    (UP && ch >= a.up_c0)
+                        ? a.up + ((size_t)(n * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.up_cs + a.up_off + (ch - a.up_c0)
+                        : a.in + ((size_t)(n * a.H + yy) * a.W + xx) * a.in_cs + a.in_off + ch;
+                val[k] = *reinterpret_cast<const uint4*>(src);
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -231,6 +156,62 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
                     acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b[t], acc[m][t], 0, 0, 0);
             }
         }
+    }
+    if constexpr (POOL) {
+        // Forward encoder output: bias + ReLU, bf16 store, and the 2x2 average pool of
+        // the stored values.  The horizontal pair is lanes r, r^1; the vertical pair is
+        // waves 2i, 2i+1 (tile rows are 4-aligned), exchanged through the (no longer
+        // needed) input tile.  H and W are even, so a pair is entirely in or out.
+        const bool rowv = y < a.H;
+        float* xch = reinterpret_cast<float*>(&tile[0][0][0]);   // [2][16][32] floats per m
+#pragma unroll
+        for (int m = 0; m < TM; m++) {
+            const int co0 = cobase + m * 16 + kq * 4;
+            float hs[4][4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int x = x0 + t * 16 + r;
+                float v[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    float xv = acc[m][t][j] + a.bias[co0 + j];
+                    v[j] = a.relu ? fmaxf(xv, 0.0f) : xv;
+                }
+                uint2 pk;
+                pk.x = pk_bf16(v[0], v[1]);
+                pk.y = pk_bf16(v[2], v[3]);
+                if (rowv && x < a.W)
+                    *reinterpret_cast<uint2*>(a.out + (((size_t)n * a.H + y) * a.W + x) * a.out_cs + a.out_off + co0) = pk;
+                const float s[4] = {bf2f((uint16_t)(pk.x & 0xFFFFu)), bf2f((uint16_t)(pk.x >> 16)),
+                                    bf2f((uint16_t)(pk.y & 0xFFFFu)), bf2f((uint16_t)(pk.y >> 16))};
+#pragma unroll
+                for (int j = 0; j < 4; j++) hs[t][j] = s[j] + lane_xor1(s[j]);
+            }
+            __syncthreads();                   // tile reads (or the previous m's exchange) are done
+            if ((wave & 1) && !(r & 1)) {
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) xch[((wave >> 1) * 16 + t * 4 + j) * 32 + (lane >> 1)] = hs[t][j];
+            }
+            __syncthreads();
+            if (!(wave & 1) && !(r & 1) && rowv) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int x = x0 + t * 16 + r;
+                    if (x >= a.W) continue;
+                    float o[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) o[j] = 0.25f * (hs[t][j] + xch[((wave >> 1) * 16 + t * 4 + j) * 32 + (lane >> 1)]);
+                    uint2 pk;
+                    pk.x = pk_bf16(o[0], o[1]);
+                    pk.y = pk_bf16(o[2], o[3]);
+                    const size_t q = ((size_t)n * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
+                    *reinterpret_cast<uint2*>(a.pool + q * a.pool_cs + a.pool_off + co0) = pk;
+                }
+            }
+        }
+        return;
     }
     if (y >= a.H) return;
 #pragma unroll
@@ -275,6 +256,7 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
 // ---------------------------------------------------------------------------
 struct WgradArgs {
     const uint16_t* x; int x_cs, x_off, cin;
+    const uint16_t* up; int up_cs, up_off, up_c0;   // channels >= up_c0: nearest 2x upsample of an H/2 x W/2 tensor
     const uint16_t* g; int g_cs, g_off, cout;
     int N, H, W;
     int nci;              // ci blocks of 16
@@ -302,8 +284,15 @@ __global__ void __launch_bounds__(64) k_wgrad(WgradArgs a) {
             gv = *reinterpret_cast<const uint4*>(a.g + (size_t)p * a.g_cs + a.g_off + co0 + half * 8);
             const int px = p % a.W, py = (p / a.W) % a.H;
             const int yy = py + dy, xx = px + dx;
-            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-                xv = *reinterpret_cast<const uint4*>(a.x + (size_t)(p + dy * a.W + dx) * a.x_cs + a.x_off + ci0 + half * 8);
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) {
+                const int ch = ci0 + half * 8;
+                const int n = p / (a.W * a.H);
+                const uint16_t* src =
+                    (a.up && ch >= a.up_c0)
+                        ? a.up + ((size_t)(n * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.up_cs + a.up_off + (ch - a.up_c0)
+                        : a.x + (size_t)(p + dy * a.W + dx) * a.x_cs + a.x_off + ch;
+                xv = *reinterpret_cast<const uint4*>(src);
+            }
         }
         const uint16_t* gs = reinterpret_cast<const uint16_t*>(&gv);
         const uint16_t* xs = reinterpret_cast<const uint16_t*>(&xv);
@@ -343,8 +332,9 @@ __global__ void __launch_bounds__(256) k_bias_grad(const uint16_t* g, int g_cs, 
 // ---------------------------------------------------------------------------
 // pooling, upsampling, input/output transforms, loss, optimizer
 // ---------------------------------------------------------------------------
-// Pooling and upsampling move 8 channels (16 B) per thread; every tensor's
-// channel count, stride and offset are multiples of 8.
+// The forward pass pools in the encoder convolution's epilogue and upsamples in
+// the decoder convolution's loader; their adjoints below move 8 channels (16 B)
+// per thread (every tensor's channel count, stride and offset are multiples of 8).
 __device__ inline void ld8(const uint16_t* p, float* f) {
     uint4 v = *reinterpret_cast<const uint4*>(p);
     const uint32_t w[4] = {v.x, v.y, v.z, v.w};
@@ -354,25 +344,6 @@ __device__ inline void st8(uint16_t* p, const float* f) {
     uint32_t w[4];
     for (int k = 0; k < 4; k++) w[k] = pk_bf16(f[2 * k], f[2 * k + 1]);
     *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
-}
-
-// 2x2 average pool; src dims H x W, dst H/2 x W/2
-__global__ void k_pool(View s, View d, int C, int N, int H, int W) {
-    const int C8 = C >> 3, Ho = H >> 1, Wo = W >> 1;
-    const size_t total = (size_t)N * Ho * Wo * C8;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int c = (int)(i % C8) * 8;
-    const size_t q = i / C8;
-    const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((size_t)Wo * Ho));
-    const size_t p00 = ((size_t)n * H + 2 * yo) * W + 2 * xo;
-    float a[8], b[8], e[8], f[8], o[8];
-    ld8(s.p + p00 * s.cs + s.off + c, a);
-    ld8(s.p + (p00 + 1) * s.cs + s.off + c, b);
-    ld8(s.p + (p00 + W) * s.cs + s.off + c, e);
-    ld8(s.p + (p00 + W + 1) * s.cs + s.off + c, f);
-    for (int k = 0; k < 8; k++) o[k] = 0.25f * (a[k] + b[k] + e[k] + f[k]);
-    st8(d.p + q * d.cs + d.off + c, o);
 }
 
 // d[p] = (skip[p] + 0.25 * gdown[p/2]) * (mask[p] > 0); dims of d: H x W
@@ -395,19 +366,6 @@ __global__ void k_pool_bwd(View gdown, View skip, View mask, View d, int C, int 
         o[k] = m[k] > 0.0f ? v : 0.0f;
     }
     st8(d.p + p * d.cs + d.off + c, o);
-}
-
-// nearest 2x upsample; dst dims H x W, src H/2 x W/2
-__global__ void k_up(View s, View d, int C, int N, int H, int W) {
-    const int C8 = C >> 3;
-    const size_t total = (size_t)N * H * W * C8;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int c = (int)(i % C8) * 8;
-    const size_t p = i / C8;
-    const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
-    const size_t q = ((size_t)n * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1);
-    *reinterpret_cast<uint4*>(d.p + p * d.cs + d.off + c) = *reinterpret_cast<const uint4*>(s.p + q * s.cs + s.off + c);
 }
 
 // d[q] = sum of g over the 2x2 children of q, times (mask[q] > 0); dst dims Ho x Wo
@@ -518,39 +476,6 @@ __global__ void k_adam(float* prm, const float* g, float* m, float* v, size_t n,
     prm[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// fp32 [cout][9][cin] -> forward bf16 [cout_pad][kg_pad*8] (kg = tap*cin_g + c8)
-__global__ void k_pack_fwd(const float* w, int cout, int cin, int cin_g, int cout_pad, int kg_pad, uint16_t* o) {
-    const size_t total = (size_t)cout_pad * kg_pad * 8;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int co = (int)(i / ((size_t)kg_pad * 8));
-    const int k = (int)(i % ((size_t)kg_pad * 8));
-    const int g = k >> 3, j = k & 7;
-    float v = 0.0f;
-    if (co < cout && g < 9 * cin_g) {
-        const int tap = g / cin_g, ci = (g % cin_g) * 8 + j;
-        if (ci < cin) v = w[((size_t)co * 9 + tap) * cin + ci];
-    }
-    o[i] = f2bf(v);
-}
-
-// input-gradient weights: rows ci, groups g' = tap' * cout_g + c8' over the (padded) output channels,
-// element W[co = c8'*8 + j][8 - tap'][ci]
-__global__ void k_pack_bwd(const float* w, int cout, int cin, int cout_g, int cin_pad, int kg_pad, uint16_t* o) {
-    const size_t total = (size_t)cin_pad * kg_pad * 8;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int ci = (int)(i / ((size_t)kg_pad * 8));
-    const int k = (int)(i % ((size_t)kg_pad * 8));
-    const int g = k >> 3, j = k & 7;
-    float v = 0.0f;
-    if (ci < cin && g < 9 * cout_g) {
-        const int tap = g / cout_g, co = (g % cout_g) * 8 + j;
-        if (co < cout) v = w[((size_t)co * 9 + (8 - tap)) * cin + ci];
-    }
-    o[i] = f2bf(v);
-}
-
 // fp32 [cout][9][cin] -> bf16 [rows][9][c32] (forward: rows = output channels)
 __global__ void k_pack_fwd_tap(const float* w, int cout, int cin, int c32, int rows, uint16_t* o) {
     const size_t total = (size_t)rows * 9 * c32;
@@ -575,7 +500,6 @@ __global__ void k_pack_bwd_tap(const float* w, int cout, int cin, int c32, int r
 
 inline int up32(int x) { return (x + 31) & ~31; }
 inline int up16(int x) { return (x + 15) & ~15; }
-inline int up4(int x) { return (x + 3) & ~3; }
 inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 // ---------------------------------------------------------------------------
@@ -589,11 +513,8 @@ struct Layer {
     int cin, cout;            // actual channels of the parameters
     int cin_t;                // channels of the input tensor the forward kernel reads (multiple of 8)
     size_t w_off, b_off;      // offsets into the flat parameter vector
-    int cout_pad, kg_pad;     // forward packing
-    int cin_pad, kgb_pad;     // input-gradient packing
-    int fc32, bc32;           // tap-major packings: channels per tap (forward, input gradient)
-    uint16_t* wf = nullptr;
-    uint16_t* wb = nullptr;
+    int cout_pad, cin_pad;    // rows of the forward / input-gradient packings (multiples of 16)
+    int fc32, bc32;           // channels per tap of those packings (multiples of 32)
     uint16_t* wf2 = nullptr;
     uint16_t* wb2 = nullptr;
 };
@@ -608,9 +529,7 @@ struct Net {
             l.w_off = n_params; n_params += (size_t)cout * 9 * cin;
             l.b_off = n_params; n_params += (size_t)cout;
             l.cout_pad = up16(cout);
-            l.kg_pad = up4(9 * (cin_t / 8));
             l.cin_pad = up16(cin);
-            l.kgb_pad = up4(9 * (l.cout_pad / 8));
             l.fc32 = up32(cin_t);
             l.bc32 = up32(l.cout_pad);
             L.push_back(l);
@@ -648,7 +567,8 @@ struct nr_unet {
     int N = 0, H = 0, W = 0;
     void* amem = nullptr;
     size_t amem_bytes = 0;
-    uint16_t *X[LV] = {}, *A[LV] = {}, *D[LV] = {}, *DA[LV] = {}, *DO[LV] = {}, *E4 = nullptr;
+    // X: level input, A: first conv, E: encoder output (skip), DA/DO: decoder convs
+    uint16_t *X[LV] = {}, *A[LV] = {}, *E[LV] = {}, *DA[LV] = {}, *DO[LV] = {};
     uint16_t *gX[LV] = {}, *gA[LV] = {}, *gE[LV] = {}, *gD[LV] = {}, *gDA[LV] = {}, *gDO[LV] = {}, *gY = nullptr;
     float *xlog = nullptr, *y = nullptr, *tlog = nullptr, *partial = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -660,12 +580,6 @@ namespace {
 
 int pack_all(nr_unet* u) {
     for (Layer& l : u->net.L) {
-        size_t nf = (size_t)l.cout_pad * l.kg_pad * 8;
-        hipLaunchKernelGGL(k_pack_fwd, dim3(blocks(nf, 256)), dim3(256), 0, u->stream, u->prm + l.w_off, l.cout, l.cin,
-                           l.cin_t / 8, l.cout_pad, l.kg_pad, l.wf);
-        size_t nb = (size_t)l.cin_pad * l.kgb_pad * 8;
-        hipLaunchKernelGGL(k_pack_bwd, dim3(blocks(nb, 256)), dim3(256), 0, u->stream, u->prm + l.w_off, l.cout, l.cin,
-                           l.cout_pad / 8, l.cin_pad, l.kgb_pad, l.wb);
         size_t nf2 = (size_t)l.cout_pad * 9 * l.fc32;
         hipLaunchKernelGGL(k_pack_fwd_tap, dim3(blocks(nf2, 256)), dim3(256), 0, u->stream, u->prm + l.w_off, l.cout,
                            l.cin, l.fc32, l.cout_pad, l.wf2);
@@ -692,16 +606,14 @@ int ensure_shape(nr_unet* u, int N, int H, int W) {
         want((void**)&u->gX[l], P * cin * 2);
         want((void**)&u->gA[l], P * CH[l] * 2);
         want((void**)&u->gE[l], P * CH[l] * 2);
+        want((void**)&u->E[l], P * CH[l] * 2);
         if (l < LV - 1) {
             const int cd = CH[l] + CH[l + 1];
-            want((void**)&u->D[l], P * cd * 2);
             want((void**)&u->gD[l], P * cd * 2);
             want((void**)&u->DA[l], P * CH[l] * 2);
             want((void**)&u->DO[l], P * CH[l] * 2);
             want((void**)&u->gDA[l], P * CH[l] * 2);
             want((void**)&u->gDO[l], P * CH[l] * 2);
-        } else {
-            want((void**)&u->E4, P * CH[l] * 2);
         }
     }
     const size_t P0 = (size_t)N * H * W;
@@ -724,10 +636,16 @@ int ensure_shape(nr_unet* u, int N, int H, int W) {
 
 double conv_flops(const Layer& l, size_t P) { return 2.0 * P * 9.0 * l.cin * l.cout; }
 
+// `up` (may be null): input channels from cin_groups*8 on are the nearest 2x
+// upsample of this H/2 x W/2 tensor.  `pool` (may be null): also write the 2x2
+// average of the (ReLU, bf16) output there.
 void launch_conv(nr_unet* u, const Layer& l, bool backward, View in, int cin_groups, int N, int H, int W, View out,
-                 int cout, bool relu, const float* bias, View mask, float* out_f32, const float* resid) {
+                 int cout, bool relu, const float* bias, View mask, float* out_f32, const float* resid,
+                 View up = View{nullptr, 0, 0}, int up_groups = 0, View pool = View{nullptr, 0, 0}) {
     ConvLdsArgs a{};
-    a.in = in.p; a.in_cs = in.cs; a.in_off = in.off; a.cin_avail = cin_groups * 8;
+    a.in = in.p; a.in_cs = in.cs; a.in_off = in.off; a.cin_avail = (cin_groups + up_groups) * 8;
+    a.up = up.p; a.up_cs = up.cs; a.up_off = up.off; a.up_c0 = cin_groups * 8;
+    a.pool = pool.p; a.pool_cs = pool.cs; a.pool_off = pool.off;
     a.c32 = backward ? l.bc32 : l.fc32;
     a.N = N; a.H = H; a.W = W;
     a.w = backward ? l.wb2 : l.wf2;
@@ -741,17 +659,27 @@ void launch_conv(nr_unet* u, const Layer& l, bool backward, View in, int cin_gro
     int TM = (cpad % 64 == 0) ? 4 : (cpad % 48 == 0) ? 3 : (cpad % 32 == 0) ? 2 : 1;
     dim3 grid((W + CL_TILE_W - 1) / CL_TILE_W, N * ((H + CL_TILE_H - 1) / CL_TILE_H), cpad / (16 * TM));
     dim3 block(256);
+    const int mode = a.pool ? 2 : a.up ? 1 : 0;
+#define NR_CONV_CASE(T)                                                                                   \
+    case T:                                                                                               \
+        if (mode == 2) hipLaunchKernelGGL((k_conv_lds<T, false, true>), grid, block, 0, u->stream, a);     \
+        else if (mode == 1) hipLaunchKernelGGL((k_conv_lds<T, true, false>), grid, block, 0, u->stream, a); \
+        else hipLaunchKernelGGL((k_conv_lds<T, false, false>), grid, block, 0, u->stream, a);              \
+        break;
     switch (TM) {
-        case 4: hipLaunchKernelGGL(k_conv_lds<4>, grid, block, 0, u->stream, a); break;
-        case 3: hipLaunchKernelGGL(k_conv_lds<3>, grid, block, 0, u->stream, a); break;
-        case 2: hipLaunchKernelGGL(k_conv_lds<2>, grid, block, 0, u->stream, a); break;
-        default: hipLaunchKernelGGL(k_conv_lds<1>, grid, block, 0, u->stream, a); break;
+        NR_CONV_CASE(4)
+        NR_CONV_CASE(3)
+        NR_CONV_CASE(2)
+        default: NR_CONV_CASE(1)
     }
+#undef NR_CONV_CASE
 }
 
-void launch_wgrad(nr_unet* u, const Layer& l, View x, int cin, View g, int cout, int N, int H, int W) {
+void launch_wgrad(nr_unet* u, const Layer& l, View x, int cin, View g, int cout, int N, int H, int W,
+                  View up = View{nullptr, 0, 0}, int up_c0 = 0) {
     WgradArgs a{};
     a.x = x.p; a.x_cs = x.cs; a.x_off = x.off; a.cin = cin;
+    a.up = up.p; a.up_cs = up.cs; a.up_off = up.off; a.up_c0 = up_c0;
     a.g = g.p; a.g_cs = g.cs; a.g_off = g.off; a.cout = cout;
     a.N = N; a.H = H; a.W = W;
     a.nci = up16(cin) / 16;
@@ -777,29 +705,21 @@ void forward(nr_unet* u, int N, int H, int W) {
     for (int l = 0; l < LV; l++) {
         const int h = H >> l, w = W >> l;
         const int cin_t = l == 0 ? IN_PAD : CH[l - 1];
-        View x{u->X[l], cin_t, 0}, a{u->A[l], CH[l], 0};
-        View e = l < LV - 1 ? View{u->D[l], CH[l] + CH[l + 1], 0} : View{u->E4, CH[l], 0};
+        View x{u->X[l], cin_t, 0}, a{u->A[l], CH[l], 0}, e{u->E[l], CH[l], 0};
+        View xn = l < LV - 1 ? View{u->X[l + 1], CH[l], 0} : none;
         Layer& la = n.enc(l, 0);
         Layer& lb = n.enc(l, 1);
         launch_conv(u, la, false, x, cin_t / 8, N, h, w, a, CH[l], true, u->prm + la.b_off, none, nullptr, nullptr);
-        launch_conv(u, lb, false, a, CH[l] / 8, N, h, w, e, CH[l], true, u->prm + lb.b_off, none, nullptr, nullptr);
-        if (l < LV - 1) {
-            View xn{u->X[l + 1], CH[l], 0};
-            size_t tot = (size_t)N * (h >> 1) * (w >> 1) * (CH[l] / 8);
-            hipLaunchKernelGGL(k_pool, dim3(blocks(tot, 256)), dim3(256), 0, u->stream, e, xn, CH[l], N, h, w);
-        }
+        launch_conv(u, lb, false, a, CH[l] / 8, N, h, w, e, CH[l], true, u->prm + lb.b_off, none, nullptr, nullptr,
+                    none, 0, xn);
     }
     for (int l = LV - 2; l >= 0; l--) {
         const int h = H >> l, w = W >> l;
-        const int cd = CH[l] + CH[l + 1];
-        View src = l == LV - 2 ? View{u->E4, CH[l + 1], 0} : View{u->DO[l + 1], CH[l + 1], 0};
-        View up{u->D[l], cd, CH[l]};
-        size_t tot = (size_t)N * h * w * (CH[l + 1] / 8);
-        hipLaunchKernelGGL(k_up, dim3(blocks(tot, 256)), dim3(256), 0, u->stream, src, up, CH[l + 1], N, h, w);
+        View coarse = l == LV - 2 ? View{u->E[LV - 1], CH[l + 1], 0} : View{u->DO[l + 1], CH[l + 1], 0};
         Layer& la = n.dec(l, 0);
         Layer& lb = n.dec(l, 1);
-        launch_conv(u, la, false, View{u->D[l], cd, 0}, cd / 8, N, h, w, View{u->DA[l], CH[l], 0}, CH[l], true,
-                    u->prm + la.b_off, none, nullptr, nullptr);
+        launch_conv(u, la, false, View{u->E[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->DA[l], CH[l], 0}, CH[l], true,
+                    u->prm + la.b_off, none, nullptr, nullptr, coarse, CH[l + 1] / 8);
         launch_conv(u, lb, false, View{u->DA[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->DO[l], CH[l], 0}, CH[l], true,
                     u->prm + lb.b_off, none, nullptr, nullptr);
     }
@@ -825,15 +745,15 @@ void backward(nr_unet* u, int N, int H, int W) {
         launch_wgrad(u, lb, View{u->DA[l], CH[l], 0}, CH[l], View{u->gDO[l], CH[l], 0}, CH[l], N, h, w);
         launch_conv(u, lb, true, View{u->gDO[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->gDA[l], CH[l], 0}, CH[l], false,
                     nullptr, View{u->DA[l], CH[l], 0}, nullptr, nullptr);
-        launch_wgrad(u, la, View{u->D[l], cd, 0}, cd, View{u->gDA[l], CH[l], 0}, CH[l], N, h, w);
+        View coarse = l == LV - 2 ? View{u->E[LV - 1], CH[l + 1], 0} : View{u->DO[l + 1], CH[l + 1], 0};
+        launch_wgrad(u, la, View{u->E[l], CH[l], 0}, cd, View{u->gDA[l], CH[l], 0}, CH[l], N, h, w, coarse, CH[l]);
         launch_conv(u, la, true, View{u->gDA[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->gD[l], cd, 0}, cd, false,
                     nullptr, none, nullptr, nullptr);
         // upsampled half of the concatenation -> the coarser decoder output (or the bottleneck)
-        View src_mask = l == LV - 2 ? View{u->E4, CH[l + 1], 0} : View{u->DO[l + 1], CH[l + 1], 0};
         View dst = l == LV - 2 ? View{u->gE[LV - 1], CH[l + 1], 0} : View{u->gDO[l + 1], CH[l + 1], 0};
         size_t tot = (size_t)N * (h >> 1) * (w >> 1) * (CH[l + 1] / 8);
         hipLaunchKernelGGL(k_up_bwd, dim3(blocks(tot, 256)), dim3(256), 0, u->stream, View{u->gD[l], cd, CH[l]},
-                           src_mask, dst, CH[l + 1], N, h >> 1, w >> 1);
+                           coarse, dst, CH[l + 1], N, h >> 1, w >> 1);
     }
     // encoder, coarsest level first
     for (int l = LV - 1; l >= 0; l--) {
@@ -846,7 +766,7 @@ void backward(nr_unet* u, int N, int H, int W) {
             const int cd = CH[l] + CH[l + 1];
             size_t tot = (size_t)N * h * w * (CH[l] / 8);
             hipLaunchKernelGGL(k_pool_bwd, dim3(blocks(tot, 256)), dim3(256), 0, u->stream,
-                               View{u->gX[l + 1], CH[l], 0}, View{u->gD[l], cd, 0}, View{u->D[l], cd, 0},
+                               View{u->gX[l + 1], CH[l], 0}, View{u->gD[l], cd, 0}, View{u->E[l], CH[l], 0},
                                View{u->gE[l], CH[l], 0}, CH[l], N, h, w);
         }
         launch_wgrad(u, lb, View{u->A[l], CH[l], 0}, CH[l], View{u->gE[l], CH[l], 0}, CH[l], N, h, w);
@@ -924,13 +844,10 @@ nr_unet* nr_unet_create(int device, uint32_t seed) {
         return bad("hipMalloc parameters");
     size_t wbytes = 0;
     for (Layer& l : u->net.L)
-        wbytes += ((size_t)l.cout_pad * l.kg_pad * 8 + (size_t)l.cin_pad * l.kgb_pad * 8 +
-                   (size_t)l.cout_pad * 9 * l.fc32 + (size_t)l.cin_pad * 9 * l.bc32) * 2 + 1024;
+        wbytes += ((size_t)l.cout_pad * 9 * l.fc32 + (size_t)l.cin_pad * 9 * l.bc32) * 2 + 512;
     if (hipMalloc((void**)&u->wmem, wbytes) != hipSuccess) return bad("hipMalloc weights");
     size_t o = 0;
     for (Layer& l : u->net.L) {
-        l.wf = u->wmem + o; o += ((size_t)l.cout_pad * l.kg_pad * 8 + 127) & ~(size_t)127;
-        l.wb = u->wmem + o; o += ((size_t)l.cin_pad * l.kgb_pad * 8 + 127) & ~(size_t)127;
         l.wf2 = u->wmem + o; o += ((size_t)l.cout_pad * 9 * l.fc32 + 127) & ~(size_t)127;
         l.wb2 = u->wmem + o; o += ((size_t)l.cin_pad * 9 * l.bc32 + 127) & ~(size_t)127;
     }
