@@ -135,3 +135,37 @@ class UNet:
         ms, fl = C.c_float(), C.c_double()
         _check(_lib().nr_unet_last_stats(self._h, C.byref(ms), C.byref(fl)))
         return ms.value, fl.value
+
+
+def _time_forward(H: int, W: int, reps: int) -> dict:
+    """Median wall time (HIP events inside the library) of one U-Net forward over an H x W
+    frame of synthetic inputs (beauty 0..2, albedo 0..1, normals -1..1), shipped weights."""
+    import torch
+    net = UNet(0).load()
+    g = torch.Generator(device="cuda").manual_seed(0)
+    b = torch.rand((1, 3, H, W), device="cuda", generator=g) * 2
+    a = torch.rand((1, 3, H, W), device="cuda", generator=g)
+    n = torch.rand((1, 3, H, W), device="cuda", generator=g) * 2 - 1
+    out = torch.empty_like(b)
+    for _ in range(3):
+        net.forward(b, a, n, out)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        net.forward(b, a, n, out)
+        ms.append(net.last_stats()[0])
+    flops = net.last_stats()[1]
+    med = sorted(ms)[len(ms) // 2]
+    return {"H": H, "W": W, "forward_ms": round(med, 4), "flops": flops,
+            "tflops": round(flops / med / 1e9, 2)}
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="time the U-Net forward pass on HIP device 0")
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--reps", type=int, default=10)
+    args = ap.parse_args()
+    print(json.dumps(_time_forward(args.height, args.width, args.reps)), flush=True)
