@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define NR_ABI_VERSION 2
+#define NR_ABI_VERSION 3
 
 enum nr_status {
     NR_OK = 0,
@@ -267,6 +267,54 @@ int nr_post(nr_ctx* ctx, int view, const nr_post_desc* desc, uint8_t* out_u8, fl
  * either may be NULL.  Synchronous. */
 int nr_post_device(nr_ctx* ctx, const float* d_chw, int32_t H, int32_t W, const nr_post_desc* desc,
                    uint8_t* d_u8, float* d_f32);
+
+/* ---- interactive mode: viewport preview, picking, selection, readback ---- */
+
+/* RenderEngine::viewport (csrc/engine/src/engine.cpp:76-96) -> trace_viewport
+ * (csrc/engine/src/trace.cu:99-161): one centre ray per pixel (get_ray_center,
+ * camera.h:181-184); a hit is shaded with the material's viewport colour under a
+ * headlight and a directional light, a miss shows the sky and, if show_axis_grid,
+ * the triplanar axis grid (csrc/engine/src/axis_grid.cu).  Overwrites the view's
+ * BEAUTY, DIFFUSE and WORLD_NORMAL layers (aovs.overwrite, engine.cpp:89-91). */
+int nr_viewport(nr_ctx* ctx, int view, int show_axis_grid);
+
+/* hit_test_ray (trace.cu:166-188) as SceneInterface::query_scene calls it
+ * (csrc/scene/include/interface.h:59-70): closest hit of the view camera's
+ * screen-space ray at (su, sv) in [0,1]^2 (camera.h:215-226).  Host output;
+ * object_id = -1 on a miss. */
+int nr_pick(nr_ctx* ctx, int view, float su, float sv, nr_hit* out);
+
+/* selection_mask (csrc/scene/src/scene_interface.cu:3-83): d_mask (H x W device
+ * bytes) = 255 on the pixels within outline_radius of, but not on, object
+ * object_id as seen through the view's screen-space rays; 0 elsewhere. */
+int nr_selection_mask(nr_ctx* ctx, int view, int32_t object_id, int32_t outline_radius, uint8_t* d_mask);
+
+/* An H x W byte device buffer owned by the view (allocated on first use, freed by
+ * nr_destroy; reallocated when the view grows), for callers without a device allocator of
+ * their own: SceneInterface keeps its selection mask here (interface.h:193-199). */
+uint8_t* nr_view_mask_buffer(nr_ctx* ctx, int view);
+
+/* composite_overlay (csrc/data/src/stream.cu:177-227) on a planar 3 x H x W
+ * device uint8 image: blend colour (clamped to [0,1], x255) with the mask as alpha. */
+int nr_overlay_device(nr_ctx* ctx, uint8_t* d_img, int32_t H, int32_t W, const uint8_t* d_mask,
+                      const float color[3]);
+
+/* TransferStream::readback (csrc/data/include/data/stream.h:154-182). */
+typedef struct nr_readback_desc {
+    nr_post_desc post;           /* post->layer is the linked layer */
+    void* denoiser;              /* nr_unet* (nectar_denoise.h): denoise the layer with the
+                                    view's DIFFUSE and WORLD_NORMAL first; NULL = off */
+    const uint8_t* d_overlay;    /* H x W device mask (nr_selection_mask); NULL = none */
+    float overlay_color[3];
+} nr_readback_desc;
+
+/* Linked layer -> [denoise] -> post -> [overlay] -> host_u8 (3 x H x W), on the
+ * view's own readback stream through pinned staging.  It may run on another
+ * thread while nr_render / nr_viewport run on the same context: it reads the
+ * layers as the last completed call left them, and a call that rewrites the
+ * layers waits for a readback in flight.  Not concurrently with nr_camera_set,
+ * nr_scene_upload or nr_destroy. */
+int nr_readback(nr_ctx* ctx, int view, const nr_readback_desc* desc, uint8_t* host_u8);
 
 /* Wait for all work of the context. */
 int nr_sync(nr_ctx* ctx);

@@ -92,6 +92,11 @@ class NrPostDesc(C.Structure):
                 ("layer", C.c_int32)]
 
 
+class NrReadbackDesc(C.Structure):
+    _fields_ = [("post", NrPostDesc), ("denoiser", C.c_void_p), ("d_overlay", C.c_void_p),
+                ("overlay_color", C.c_float * 3)]
+
+
 def post_desc(linear_to_gamma=True, apply_white_balance=True, wb_temperature=6800.0, wb_tint=1.0,
               apply_tonemapping=True, tm_method=0, tm_white_point=1.0, tm_alpha=1.0, fix_reinhard=False,
               layer=LAYER_BEAUTY):

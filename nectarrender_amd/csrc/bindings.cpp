@@ -28,12 +28,14 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <optional>
 #include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
 
+#include "../../include/nectar_denoise.h"
 #include "../../include/nectar_hip.h"
 #include "host/nr_io.h"
 
@@ -437,6 +439,8 @@ struct Camera {
 struct Ctx {
     nr_ctx* h = nullptr;
     std::mutex mu;            // one stream per context: calls are serialised
+    // Readbacks (nr_readback) run beside renders; scene and camera changes exclude them.
+    std::shared_mutex layout_mu;
     explicit Ctx(int device) {
         h = nr_create(device);
         if (!h) throw std::runtime_error(std::string("nectar_hip: ") + nr_last_error());
@@ -570,8 +574,27 @@ struct StreamConfig {                                                        // 
     float tm_white_point = 1.0f, tm_alpha = 1.0f;
 };
 
-// TransferStream (stream.h:47-279): post-processes the linked layer into a
-// host uint8 CHW buffer and returns that buffer's address.
+// The learned U-Net in OIDN's slot (denoise.h:25-83), loaded with the shipped
+// weights on the first denoised readback.
+struct Denoiser {
+    nr_unet* h = nullptr;
+    explicit Denoiser(int device) {
+        h = nr_unet_create(device, 0u);
+        if (!h) throw std::runtime_error(std::string("nr_unet_create: ") + nr_last_error());
+        py::object path = py::module_::import("nectarrender_amd.denoise").attr("WEIGHTS");
+        py::array_t<float, py::array::c_style | py::array::forcecast> w =
+            py::module_::import("numpy").attr("load")(path, py::arg("allow_pickle") = false);
+        if (nr_unet_set_params(h, w.data(), (size_t)w.size()) != NR_OK) {
+            nr_unet_destroy(h);
+            throw std::runtime_error(std::string("U-Net weights: ") + nr_last_error());
+        }
+    }
+    ~Denoiser() { if (h) nr_unet_destroy(h); }
+};
+
+// TransferStream (stream.h:47-279): the linked layer -> [U-Net denoise] -> white
+// balance / tonemap / gamma -> [selection overlay] -> a host uint8 CHW buffer,
+// on the library's readback stream (nr_readback), concurrently with rendering.
 struct TransferStream {
     CtxP ctx;
     int view = 0;
@@ -581,13 +604,14 @@ struct TransferStream {
     std::atomic<StreamState> state{StreamState::INACTIVE};
     std::vector<uint8_t> buffer;
     std::mutex cfg_mu;
+    std::mutex readback_mu;                      // readback_mutex (stream.h:155)
+    std::shared_ptr<Denoiser> denoiser;
+    std::atomic<const uint8_t*> overlay_mask{nullptr};   // device H x W mask (SceneInterface)
+    Color overlay_color = c3(1.0f, 1.0f, 1.0f);
 
     uintptr_t buffer_ptr() { return reinterpret_cast<uintptr_t>(buffer.data()); }
-    uintptr_t readback() {
-        if (state.load() == StreamState::FROZEN) return buffer_ptr();
-        if (!ctx) throw std::runtime_error("TransferStream is not linked to an engine");
-        StreamConfig c;
-        { std::lock_guard<std::mutex> g(cfg_mu); c = cfg; }
+    StreamConfig config() { std::lock_guard<std::mutex> g(cfg_mu); return cfg; }
+    static nr_post_desc post_of(const StreamConfig& c, int layer) {
         nr_post_desc d{};
         d.linear_to_gamma = c.linear_to_gamma;
         d.apply_white_balance = c.apply_white_balance;
@@ -598,27 +622,41 @@ struct TransferStream {
         d.tm_white_point = c.tm_white_point;
         d.tm_alpha = c.tm_alpha;
         d.fix_reinhard = 0;
-        d.layer = linked;
+        d.layer = layer;
+        return d;
+    }
+    // with the GIL held: create the denoiser the configuration asks for
+    void prepare() {
+        if (config().apply_denoising && !denoiser) denoiser = std::make_shared<Denoiser>(0);
+    }
+    uintptr_t readback() {
+        std::lock_guard<std::mutex> rb(readback_mu);
+        if (state.load() == StreamState::FROZEN) return buffer_ptr();
+        if (!ctx) throw std::runtime_error("TransferStream is not linked to an engine");
+        const StreamConfig c = config();
+        nr_readback_desc d{};
+        d.post = post_of(c, linked);
+        d.denoiser = (c.apply_denoising && denoiser) ? denoiser->h : nullptr;
+        d.d_overlay = overlay_mask.load();
+        for (int k = 0; k < 3; k++) d.overlay_color[k] = overlay_color.e[k];
         buffer.resize(C * H * W);
-        std::lock_guard<std::mutex> g(ctx->mu);
-        check(nr_post(ctx->h, view, &d, buffer.data(), nullptr));
+        std::shared_lock<std::shared_mutex> g(ctx->layout_mu);
+        check(nr_readback(ctx->h, view, &d, buffer.data()));
         return buffer_ptr();
     }
     py::array_t<uint8_t> readback_numpy() {
-        readback();
+        prepare();
+        {
+            py::gil_scoped_release r;
+            readback();
+        }
         py::array_t<uint8_t> a({C, H, W});
         std::memcpy(a.mutable_data(), buffer.data(), buffer.size());
         return a;
     }
     py::array_t<float> readback_float() {
         if (!ctx) throw std::runtime_error("TransferStream is not linked to an engine");
-        StreamConfig c;
-        { std::lock_guard<std::mutex> g(cfg_mu); c = cfg; }
-        nr_post_desc d{};
-        d.linear_to_gamma = c.linear_to_gamma; d.apply_white_balance = c.apply_white_balance;
-        d.wb_temperature = c.wb_temperature; d.wb_tint = c.wb_tint; d.apply_tonemapping = c.apply_tonemapping;
-        d.tm_method = (int32_t)c.tm_method; d.tm_white_point = c.tm_white_point; d.tm_alpha = c.tm_alpha;
-        d.layer = linked;
+        const nr_post_desc d = post_of(config(), linked);
         py::array_t<float> a({C, H, W});
         std::lock_guard<std::mutex> g(ctx->mu);
         check(nr_post(ctx->h, view, &d, nullptr, a.mutable_data()));
@@ -657,6 +695,60 @@ void call_with_gil(const F& f, A&&... a) {
     f(std::forward<A>(a)...);
 }
 
+// SceneInterface (csrc/scene/include/interface.h): picking, the selection
+// outline overlaid on the stream, the outliner, and live edits (transforms,
+// added objects, the sky).
+struct SceneNode {                                                           // outliner.h:7-17
+    size_t object_id = 0, parent_id = 0, material_id = 0;
+    bool is_light = false;
+    std::string name, type_name;
+};
+struct SceneOutline { std::vector<SceneNode> nodes; };
+
+const char* hittable_type_name(int t) {                                      // hittable.h:17-28
+    switch (t) {
+        case NR_HIT_QUAD: return "Quad";
+        case NR_HIT_SPHERE: return "Sphere";
+        case NR_HIT_CUBE: return "Cube";
+        case NR_HIT_MESH: return "Mesh";
+        case NR_HIT_CONSTANT_MEDIUM: return "ConstantMedium";
+        case NR_HIT_OBJECT_LIGHT: return "ObjectLight";
+        default: return "Null";
+    }
+}
+
+struct RenderEngine;
+struct SceneInterface {
+    RenderEngine* eng = nullptr;
+    std::mutex mu;                        // interface_mutex
+    HitRecord rec;
+    std::atomic<bool> enabled{false}, teardown_pending{false};
+    Color mask_color = c3(0.98f, 0.75f, 0.17f);                              // SelectionMaskConfig, interface.h:27-30
+    int outline_radius = 3;
+
+    void update();
+    void query_scene(float u, float v);
+    void select_scene_node(const SceneNode& n) {
+        std::lock_guard<std::mutex> g(mu);
+        rec.object_id = (int)n.object_id;
+        rec.material_id = (int)n.material_id;
+        enabled = true;
+    }
+    void enable() { enabled = true; }
+    void disable() { teardown_pending = true; }
+    bool is_enabled() const { return enabled.load(); }
+    SceneOutline outline();
+    HitP picked();
+    Transform get_transform() { std::lock_guard<std::mutex> g(mu); return picked()->xform; }
+    void set_transform(const Transform& x);
+    void add_object(const HitP& h);
+    std::shared_ptr<Skylight> get_skylight();
+    void swap_skylight(const Skylight& sky);
+    void request_skylight_update();
+    Material get_material() { std::lock_guard<std::mutex> g(mu); return *picked()->mat; }
+    void set_material(const Material&);
+};
+
 struct RenderEngine {
     CtxP ctx;
     Camera cam;
@@ -679,9 +771,11 @@ struct RenderEngine {
     std::function<void(uint32_t)> on_frame_finished = [](uint32_t) {};
     std::function<EnginePollResponse()> poll_updates = [] { return EnginePollResponse{}; };
     bool frame_hook = false, poll_hook = false;
+    SceneInterface iface;
 
     RenderEngine(const Camera& camera, uint32_t depth, uint32_t seed_)
         : ctx(std::make_shared<Ctx>(0)), cam(camera), ray_depth(depth), seed(seed_) {
+        iface.eng = this;
         cam.p.build();
         pattern_spp = cam.p.samples_per_pixel;
         nr_camera_desc d = cam.p.desc(pattern_spp);
@@ -706,15 +800,39 @@ struct RenderEngine {
     void set_scene(const Scene& s) {                                         // engine.cpp:117-121
         auto L = lower_scene(s);
         std::lock_guard<std::mutex> g(ctx->mu);
+        std::unique_lock<std::shared_mutex> lay(ctx->layout_mu);
         check(nr_scene_upload(ctx->h, &L->desc));
         scene = std::make_shared<Scene>(s);
         has_scene = true;
     }
 
+    // re-upload the engine's own scene after a live edit
+    void upload_scene() {
+        auto L = lower_scene(*scene);
+        std::lock_guard<std::mutex> g(ctx->mu);
+        std::unique_lock<std::shared_mutex> lay(ctx->layout_mu);
+        check(nr_scene_upload(ctx->h, &L->desc));
+    }
+
     void push_camera() {
         nr_camera_desc d = cam.p.desc(pattern_spp);
         std::lock_guard<std::mutex> g(ctx->mu);
+        std::unique_lock<std::shared_mutex> lay(ctx->layout_mu);
         check(nr_camera_set(ctx->h, 0, &d, seed));
+    }
+
+    ReturnState viewport() {                                                 // engine.cpp:76-96
+        if (!has_scene) throw std::runtime_error("RenderEngine.viewport() called before set_scene()");
+        iface.update();
+        poll_gui_updates();
+        if (requests.stop_pending()) return ReturnState::STOPPED;
+        {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            std::shared_lock<std::shared_mutex> lay(ctx->layout_mu);
+            check(nr_viewport(ctx->h, 0, show_axis_grid.load() ? 1 : 0));
+        }
+        if (requests.restart_pending()) return ReturnState::RESTARTED;
+        return ReturnState::WAITING;
     }
 
     void poll_gui_updates() {                                                // engine.cpp:170-191
@@ -740,6 +858,7 @@ struct RenderEngine {
         if (!has_scene) throw std::runtime_error("RenderEngine.render() called before set_scene()");
         const bool per_frame = frame_hook || poll_hook;
         while (sample_idx < n_samples()) {
+            iface.update();
             poll_gui_updates();
             if (requests.stop_pending()) return ReturnState::STOPPED;
             const uint32_t end = per_frame ? sample_idx + 1 : n_samples();
@@ -754,6 +873,7 @@ struct RenderEngine {
             d.mode = NR_MODE_PARITY;
             {
                 std::lock_guard<std::mutex> g(ctx->mu);
+                std::shared_lock<std::shared_mutex> lay(ctx->layout_mu);
                 check(nr_render(ctx->h, 0, &d));
                 check(nr_sync(ctx->h));
             }
@@ -794,12 +914,11 @@ struct RenderEngine {
                 reset();
                 state = EngineState::RENDERING;
                 call_with_gil(on_render_started);
-                // the rasterised viewport preview (trace_viewport) is not part of
-                // this library: a VIEWPORT engine idles until switched back
-                ReturnState rs = engine_type.load() == EngineType::VIEWPORT ? ReturnState::WAITING : render();
+                ReturnState rs = engine_type.load() == EngineType::VIEWPORT ? viewport() : render();
                 handle(rs);
                 continue;
             }
+            iface.update();
             std::this_thread::sleep_for(std::chrono::milliseconds(16));
         }
         call_with_gil(on_shutdown);
@@ -840,6 +959,126 @@ struct RenderEngine {
         return out;
     }
 };
+
+// --- SceneInterface (interface.h) -------------------------------------------
+void SceneInterface::update() {                                              // interface.h:51-56
+    if (!enabled.load()) return;
+    RenderEngine& e = *eng;
+    TransferStream& st = *e.stream;
+    if (teardown_pending.load()) {                                           // interface.h:202-207
+        st.overlay_mask = nullptr;
+        std::lock_guard<std::mutex> g(mu);
+        rec = HitRecord{};
+        rec.object_id = 0;
+        teardown_pending = false;
+        enabled = false;
+        return;
+    }
+    if (!e.has_scene) return;
+    int obj;
+    { std::lock_guard<std::mutex> g(mu); obj = rec.object_id; }
+    std::lock_guard<std::mutex> g(e.ctx->mu);                                // build_selection_mask, interface.h:193-199
+    std::shared_lock<std::shared_mutex> lay(e.ctx->layout_mu);
+    uint8_t* m = nr_view_mask_buffer(e.ctx->h, 0);
+    if (!m) throw std::runtime_error(std::string("nectar_hip: ") + nr_last_error());
+    check(nr_selection_mask(e.ctx->h, 0, obj, outline_radius, m));
+    st.overlay_color = mask_color;
+    st.overlay_mask = m;
+}
+
+// query_scene (interface.h:59-70): the picked record; a miss leaves the
+// default record (object 0), as the reference's hit_test_ray does
+void SceneInterface::query_scene(float u, float v) {
+    RenderEngine& e = *eng;
+    if (!e.has_scene) throw std::runtime_error("query_scene() called before set_scene()");
+    nr_hit h{};
+    {
+        std::lock_guard<std::mutex> g(e.ctx->mu);
+        check(nr_pick(e.ctx->h, 0, u, v, &h));
+    }
+    std::lock_guard<std::mutex> g(mu);
+    rec = HitRecord{};
+    rec.object_id = 0;
+    if (h.object_id >= 0) {
+        rec.object_id = h.object_id; rec.material_id = h.material_id; rec.prim = h.prim;
+        rec.front_face = h.front_face != 0; rec.t = h.t;
+        rec.p = v3(h.p[0], h.p[1], h.p[2]);
+        rec.n = v3(h.n[0], h.n[1], h.n[2]);
+        rec.tangent = v3(h.tangent[0], h.tangent[1], h.tangent[2]);
+        rec.uv = v2(h.uv[0], h.uv[1]);
+    }
+    enabled = true;
+}
+
+SceneOutline SceneInterface::outline() {                                     // outliner.h:48-66
+    RenderEngine& e = *eng;
+    SceneOutline o;
+    if (!e.has_scene) return o;
+    auto L = lower_scene(*e.scene);
+    for (size_t i = 0; i < e.scene->objects.size(); i++) {
+        SceneNode n;
+        n.object_id = i;
+        n.material_id = (size_t)L->objs[i].material;
+        n.is_light = e.scene->objects[i]->type == NR_HIT_OBJECT_LIGHT;
+        n.name = n.type_name = hittable_type_name(e.scene->objects[i]->type);
+        o.nodes.push_back(n);
+    }
+    return o;
+}
+
+HitP SceneInterface::picked() {                                              // hit_object(), interface.h:187-189
+    RenderEngine& e = *eng;
+    if (!e.has_scene || rec.object_id < 0 || (size_t)rec.object_id >= e.scene->objects.size())
+        throw std::runtime_error("Object index not valid.");
+    return e.scene->objects[(size_t)rec.object_id];
+}
+
+void SceneInterface::set_transform(const Transform& x) {                     // interface.h:127-133
+    RenderEngine& e = *eng;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        picked()->xform = x;
+    }
+    e.upload_scene();
+    e.requests.restart();
+}
+
+void SceneInterface::add_object(const HitP& h) {                             // interface.h:100-105
+    RenderEngine& e = *eng;
+    if (!e.has_scene) throw std::runtime_error("add_object() called before set_scene()");
+    {
+        std::lock_guard<std::mutex> g(mu);
+        e.scene->objects.insert(e.scene->objects.begin() + (long)e.scene->n_hittables, h);
+        e.scene->n_hittables++;
+    }
+    e.upload_scene();
+    e.requests.restart();
+}
+
+std::shared_ptr<Skylight> SceneInterface::get_skylight() {
+    std::lock_guard<std::mutex> g(mu);
+    if (!eng->has_scene) throw std::runtime_error("no scene");
+    return eng->scene->sky;
+}
+
+void SceneInterface::swap_skylight(const Skylight& sky) {                     // interface.h:145-151
+    {
+        std::lock_guard<std::mutex> g(mu);
+        if (!eng->has_scene) throw std::runtime_error("no scene");
+        eng->scene->sky = std::make_shared<Skylight>(sky);
+    }
+    eng->upload_scene();
+    eng->requests.restart();
+}
+
+void SceneInterface::request_skylight_update() {                             // interface.h:140-143
+    if (!eng->has_scene) throw std::runtime_error("no scene");
+    eng->upload_scene();
+    eng->requests.restart();
+}
+
+// set_material (interface.h:155-160) only restarts the render in the reference
+void SceneInterface::set_material(const Material&) { eng->requests.restart(); }
 
 template <class T>
 std::string vec_repr(const char* name, const T& v, int n) {
@@ -1223,7 +1462,11 @@ PYBIND11_MODULE(_pathtracer, m) {
         .def_readwrite("tm_alpha", &StreamConfig::tm_alpha);
 
     py::class_<TransferStream, std::shared_ptr<TransferStream>>(m_data, "TransferStream")
-        .def("readback", [](TransferStream& s) { py::gil_scoped_release r; return s.readback(); })
+        .def("readback", [](TransferStream& s) {
+            s.prepare();
+            py::gil_scoped_release r;
+            return s.readback();
+        })
         .def("readback_numpy", &TransferStream::readback_numpy, "Post-processed uint8 C x H x W copy.")
         .def("readback_float", &TransferStream::readback_float,
              "Display-referred float image (white balance, tonemap, gamma, clamp) before the x255.")
@@ -1240,15 +1483,18 @@ PYBIND11_MODULE(_pathtracer, m) {
         .def("freeze", [](TransferStream& s) { if (s.state.load() == StreamState::ACTIVE) s.state = StreamState::FROZEN; })
         .def("unfreeze", [](TransferStream& s) { if (s.state.load() == StreamState::FROZEN) s.state = StreamState::ACTIVE; })
         .def("is_linked", [](const TransferStream& s) { return (bool)s.ctx; })
-        .def("has_overlay", [](const TransferStream&) { return false; })
-        .def("remove_overlay", [](TransferStream&) {})
+        .def("has_overlay", [](const TransferStream& s) { return s.overlay_mask.load() != nullptr; })
+        .def("remove_overlay", [](TransferStream& s) { s.overlay_mask = nullptr; })
         .def("shape", [](const TransferStream& s) { return std::array<size_t, 3>{s.C, s.H, s.W}; })
         .def("n_pixels", [](const TransferStream& s) { return s.H * s.W; })
         .def("n_elements", [](const TransferStream& s) { return s.C * s.H * s.W; })
         .def("n_bytes", [](const TransferStream& s) { return s.C * s.H * s.W; })
         .def("enable_denoising", [](TransferStream& s) { std::lock_guard<std::mutex> g(s.cfg_mu); s.cfg.apply_denoising = true; })
         .def("disable_denoising", [](TransferStream& s) { std::lock_guard<std::mutex> g(s.cfg_mu); s.cfg.apply_denoising = false; })
-        .def("rebuild_denoiser", [](TransferStream&) {});
+        .def("rebuild_denoiser", [](TransferStream& s) {
+            std::lock_guard<std::mutex> g(s.readback_mu);
+            s.denoiser.reset();                  // recreated with the shipped weights on the next readback
+        });
 
     py::class_<DataObject>(m_data, "DataObject")
         .def("n_pixels", &DataObject::n_pixels)
@@ -1317,6 +1563,35 @@ PYBIND11_MODULE(_pathtracer, m) {
 
     // ---------------------------------------------------------------- scene
     auto m_scene = m.def_submodule("scene", "Scene module.");
+    py::class_<SceneNode>(m_scene, "SceneNode")
+        .def_readonly("object_id", &SceneNode::object_id)
+        .def_readonly("parent_id", &SceneNode::parent_id)
+        .def_readonly("material_id", &SceneNode::material_id)
+        .def_readonly("is_light", &SceneNode::is_light)
+        .def_readonly("name", &SceneNode::name)
+        .def_readonly("type_name", &SceneNode::type_name);
+    py::class_<SceneOutline>(m_scene, "SceneOutline")
+        .def_readonly("nodes", &SceneOutline::nodes);
+    py::class_<SceneInterface>(m_scene, "SceneInterface")
+        .def("set_material", [](SceneInterface& si, const Material& m) { si.set_material(m); }, py::arg("mat"))
+        .def("add_object", [](SceneInterface& si, const HitP& h) { si.add_object(h); }, py::arg("obj"))
+        .def("query_scene", [](SceneInterface& si, float u, float v) { py::gil_scoped_release r; si.query_scene(u, v); })
+        .def("select_scene_node", &SceneInterface::select_scene_node)
+        .def("enable", &SceneInterface::enable)
+        .def("disable", &SceneInterface::disable)
+        .def("is_enabled", &SceneInterface::is_enabled)
+        .def("is_disabled", [](const SceneInterface& si) { return !si.is_enabled(); })
+        .def("get_scene_outline", &SceneInterface::outline)
+        .def("get_transform", &SceneInterface::get_transform)
+        .def("set_transform", [](SceneInterface& si, const Transform& x) { py::gil_scoped_release r; si.set_transform(x); })
+        .def("get_material", &SceneInterface::get_material)
+        .def("get_hit_record", [](SceneInterface& si) { std::lock_guard<std::mutex> g(si.mu); return si.rec; })
+        .def("request_skylight_update", &SceneInterface::request_skylight_update)
+        .def("get_skylight", &SceneInterface::get_skylight)
+        .def("swap_skylight", [](SceneInterface& si, const Skylight& sky) { si.swap_skylight(sky); },
+             py::arg("new_skylight"))
+        .def("update", [](SceneInterface& si) { py::gil_scoped_release r; si.update(); },
+             "Rebuild the selection outline overlay (the engine does this every pass and while idle).");
     py::class_<Scene, std::shared_ptr<Scene>>(m_scene, "Scene")
         .def(py::init([](py::list hittables, py::list lights, std::shared_ptr<Skylight> sky) {
             auto s = std::make_shared<Scene>();
@@ -1455,6 +1730,10 @@ PYBIND11_MODULE(_pathtracer, m) {
         .def("set_axis_grid_visible", [](RenderEngine& e, bool v) { e.show_axis_grid = v; })
         .def("sample_index", [](const RenderEngine& e) { return e.sample_idx; },
              "Index of the next sample pass (1 after a reset).")
+        .def("viewport", [](RenderEngine& e) { py::gil_scoped_release r; (void)e.viewport(); },
+             "One viewport preview frame into the layers (what a VIEWPORT engine renders while idle).")
+        .def("get_scene_interface", [](RenderEngine& e) -> SceneInterface& { return e.iface; },
+             rvp::reference_internal)
         .def("trace_closest", &RenderEngine::trace,
              "Closest hits of world-space rays, an (n, 6) array of origin and direction.");
 #undef NR_HOOK

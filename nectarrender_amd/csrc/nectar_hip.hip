@@ -33,6 +33,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/nectar_denoise.h"
 #include "../../include/nectar_hip.h"
 #include "nr_bvh.h"
 #include "nr_device.h"
@@ -257,6 +258,171 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_trace_closest(const GScene
         h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
     }
     hits[i] = h;
+}
+
+// ---------------------------------------------------------------------------
+// viewport preview, picking and selection (engine/src/trace.cu:99-188,
+// scene/src/scene_interface.cu:3-50)
+// ---------------------------------------------------------------------------
+// 128-thread workgroup b of a 1-D grid covers the 16 x 8 pixel tile b (tiles in
+// row-major order); a 1-D grid keeps blockIdx.x * 128 + tid a unique stack-spill slot
+__device__ __forceinline__ bool tile16x8(uint32_t W, uint32_t H, uint32_t& x, uint32_t& y) {
+    const uint32_t tiles_x = (W + 15) >> 4;
+    x = (blockIdx.x % tiles_x) * 16 + (threadIdx.x & 15);
+    y = (blockIdx.x / tiles_x) * 8 + (threadIdx.x >> 4);
+    return x < W && y < H;
+}
+
+// DeviceCamera::get_ray_center (camera.h:181-184) = get_focus_point(x, y, 0, 0, 0):
+// the offset of sample 0 of pixel 0 under seed 0, the same for every pixel
+__device__ __forceinline__ Ray camera_center_ray(const GCamera& c, uint32_t x, uint32_t y) {
+    float bx = c.pattern[0], by = c.pattern[1];
+    f2 shift = pcg_vec2(0.0f, 0u);
+    float ox = fmodf(bx + shift.x, 1.0f) - 0.5f;
+    float oy = fmodf(by + shift.y, 1.0f) - 0.5f;
+    float u = ((float)x + ox - (c.res[0] - 1.0f) * 0.5f) * (c.uvw[0] / c.res[0]);
+    float v = ((float)y + oy - (c.res[1] - 1.0f) * 0.5f) * (c.uvw[1] / c.res[1]);
+    f3 center = mv(c.R, mk3(u, v, -c.focal_length));
+    f3 pos = ld3(c.pos);
+    f3 focus = pos + center * (c.focus_distance / c.focal_length);
+    return mkray(pos, normalize(focus - pos));
+}
+
+// trace_viewport_kernel: closest hit of the centre ray; a hit is shaded with a
+// headlight and a fixed directional light, a miss shows the sky (and the axis
+// grid).  The AOVs are cleared and then added to, so each value is 0 + v.
+template <int TY>
+__global__ void __launch_bounds__(128) NR_TRAV_ATTR k_viewport(const GScene* __restrict__ Sp, GCamera cam, int show_grid,
+                                                               float* __restrict__ beauty, float* __restrict__ diffuse,
+                                                               float* __restrict__ normal, int32_t* __restrict__ counts,
+                                                               StackCfg SC) {
+    extern __shared__ int2 smem[];
+    const GScene& S = *Sp;
+    Stack st = make_stack(smem, S, SC);
+    uint32_t x, y;
+    if (!tile16x8(cam.W, cam.H, x, y)) return;
+    Ray r = camera_center_ray(cam, x, y);
+    HRec rec;
+    hrec_clear(rec);
+    bool of = false;
+    int oi = scene_closest<TY>(S, r, rec, st, of);
+    if (of) atomicOr(&counts[CNT_ERR], 1);
+    f3 b, dcol = s3(0.0f), n = s3(0.0f);
+    if (oi < 0) {
+        f3 sky = sky_sample(S.sky, r.d);
+        f3 gc;
+        float ga;
+        b = (show_grid && triplanar_grid(r, gc, ga)) ? lerp3(sky, gc, ga) : sky;
+    } else {
+        f3 light_vector = ld3(cam.pos) - rec.p;
+        float light_dist = vlength(light_vector);
+        f3 headlight_dir = normalize(light_vector);
+        float headlight = dot(headlight_dir, rec.n) * 0.5f + 0.5f;
+        headlight *= 1.0f / (light_dist * light_dist);
+        headlight = fminf(0.75f, headlight);
+        float directional = (dot(mk3(1.0f, 1.0f, 1.0f), rec.n) + 1.0f) * 0.5f;
+        directional = fmaxf(0.1f, directional);
+        f3 col = viewport_color(S, S.materials[S.objects[oi].material], rec.uv);
+        b = col * fmaxf(headlight, directional);
+        dcol = col;
+        n = rec.n;
+    }
+    const size_t HW = (size_t)cam.H * cam.W, i = (size_t)y * cam.W + x;
+    beauty[i] = 0.0f + b.x; beauty[HW + i] = 0.0f + b.y; beauty[2 * HW + i] = 0.0f + b.z;
+    diffuse[i] = 0.0f + dcol.x; diffuse[HW + i] = 0.0f + dcol.y; diffuse[2 * HW + i] = 0.0f + dcol.z;
+    normal[i] = 0.0f + n.x; normal[HW + i] = 0.0f + n.y; normal[2 * HW + i] = 0.0f + n.z;
+}
+
+// mask_selected_kernel: 255 where the screen-space ray of the pixel centre hits
+// object `object_id`
+template <int TY>
+__global__ void __launch_bounds__(128) NR_TRAV_ATTR k_select_mask(const GScene* __restrict__ Sp, GCamera cam,
+                                                                  int32_t object_id, uint8_t* __restrict__ mask,
+                                                                  int32_t* __restrict__ counts, StackCfg SC) {
+    extern __shared__ int2 smem[];
+    const GScene& S = *Sp;
+    Stack st = make_stack(smem, S, SC);
+    uint32_t x, y;
+    if (!tile16x8(cam.W, cam.H, x, y)) return;
+    float su = ((float)x + 0.5f) / (float)cam.W;
+    float sv = ((float)y + 0.5f) / (float)cam.H;
+    Ray r = screen_space_ray(cam, su, sv);
+    HRec rec;
+    hrec_clear(rec);
+    bool of = false;
+    int oi = scene_closest<TY>(S, r, rec, st, of);
+    if (of) atomicOr(&counts[CNT_ERR], 1);
+    mask[(size_t)y * cam.W + x] = (oi >= 0 && S.objects[oi].object_id == object_id) ? 255u : 0u;
+}
+
+// build_outline_kernel: 255 on unmasked pixels with a masked pixel within the
+// (2r+1)^2 square around them
+__global__ void __launch_bounds__(128) k_outline(const uint8_t* __restrict__ base, uint8_t* __restrict__ out,
+                                                 uint32_t H, uint32_t W, int radius) {
+    uint32_t x, y;
+    if (!tile16x8(W, H, x, y)) return;
+    const size_t i = (size_t)y * W + x;
+    uint8_t o = 0u;
+    if (base[i] == 0) {
+        for (int dy = -radius; dy <= radius && !o; dy++) {
+            const int ny = (int)y + dy;
+            if (ny < 0 || ny >= (int)H) continue;
+            for (int dx = -radius; dx <= radius; dx++) {
+                const int nx = (int)x + dx;
+                if (nx < 0 || nx >= (int)W) continue;
+                if (base[(size_t)ny * W + nx] != 0) { o = 255u; break; }
+            }
+        }
+    }
+    out[i] = o;
+}
+
+// hit_test_ray_kernel: one screen-space ray
+template <int TY>
+__global__ void __launch_bounds__(64) NR_TRAV_ATTR k_pick(const GScene* __restrict__ Sp, GCamera cam, float su, float sv,
+                                                          nr_hit* __restrict__ out, int32_t* __restrict__ counts,
+                                                          StackCfg SC) {
+    extern __shared__ int2 smem[];
+    const GScene& S = *Sp;
+    Stack st = make_stack(smem, S, SC);
+    if (threadIdx.x != 0) return;
+    Ray r = screen_space_ray(cam, su, sv);
+    HRec rec;
+    hrec_clear(rec);
+    bool of = false;
+    int oi = scene_closest<TY>(S, r, rec, st, of);
+    if (of) atomicOr(&counts[CNT_ERR], 1);
+    nr_hit h;
+    memset(&h, 0, sizeof(h));
+    h.object_id = -1;
+    h.prim = -1;
+    if (oi >= 0) {
+        const GObject& o = S.objects[oi];
+        h.object_id = o.object_id;
+        h.material_id = o.material;
+        h.prim = rec.prim;
+        h.front_face = rec.front ? 1 : 0;
+        h.t = rec.t;
+        h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
+        h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
+        h.tangent[0] = rec.tg.x; h.tangent[1] = rec.tg.y; h.tangent[2] = rec.tg.z;
+        h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
+    }
+    *out = h;
+}
+
+// composite_overlay_kernel (data/src/stream.cu:177-202): blend `rgb` into a
+// planar uint8 image with the mask as alpha
+__global__ void __launch_bounds__(256) k_overlay(uint8_t* __restrict__ img, const uint8_t* __restrict__ mask, size_t HW,
+                                                 uint32_t r, uint32_t g, uint32_t b) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= HW) return;
+    const uint32_t alpha = mask[i];
+    if (alpha == 0u) return;
+    const uint32_t inv = 255u - alpha;
+    img[i] = (uint8_t)((r * alpha + img[i] * inv + 127u) / 255u);
+    img[HW + i] = (uint8_t)((g * alpha + img[HW + i] * inv + 127u) / 255u);
+    img[2 * HW + i] = (uint8_t)((b * alpha + img[2 * HW + i] * inv + 127u) / 255u);
 }
 
 // Streaming sample passes.  Every pixel owns one path slot.  When a pixel's
@@ -705,6 +871,16 @@ struct View {
     float* d_pattern = nullptr;
     uint32_t H = 0, W = 0;
     float* layer[9] = {nullptr};
+    // interactive mode
+    std::mutex layers_mu;                 // layer writes vs. readbacks (nr_readback)
+    std::mutex rb_mu;                     // one readback of this view at a time
+    uint8_t* d_sel_base = nullptr;        // selection mask before the outline pass (H x W)
+    uint8_t* d_mask = nullptr;            // nr_view_mask_buffer (H x W)
+    size_t mask_cap = 0;
+    uint8_t* d_rb_u8 = nullptr;           // readback image (3 x H x W)
+    float* d_rb_den = nullptr;            // denoised layer (3 x H x W)
+    uint8_t* h_rb = nullptr;              // pinned staging (3 x H x W)
+    size_t sel_cap = 0, rb_cap = 0, den_cap = 0;
 };
 
 struct nr_ctx {
@@ -736,6 +912,19 @@ struct nr_ctx {
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     float last_ms[8] = {0};
     int n_last = 0;
+    // readback (nr_readback): its own stream; ev_layers marks the last enqueued
+    // layer write, ev_rb_done the last enqueued readback of the layers
+    hipStream_t rb_stream = nullptr;
+    hipEvent_t ev_layers = nullptr, ev_rb_done = nullptr;
+};
+
+// Enqueue-side guard of every call that rewrites a view's layers: the writes
+// wait for readbacks in flight and are published to later readbacks.
+struct LayerWrite {
+    nr_ctx* c;
+    std::lock_guard<std::mutex> lk;
+    LayerWrite(nr_ctx* c_, View& v) : c(c_), lk(v.layers_mu) { (void)hipStreamWaitEvent(c->stream, c->ev_rb_done, 0); }
+    ~LayerWrite() { (void)hipEventRecord(c->ev_layers, c->stream); }
 };
 
 static void free_list(std::vector<void*>& v) {
@@ -791,7 +980,10 @@ nr_ctx* nr_create(int device) {
     }
     if (hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_count[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_count[1], hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_count[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_layers, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_rb_done, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->rb_stream, hipStreamNonBlocking) != hipSuccess) {
         g_err = "hipEventCreate failed";
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -826,9 +1018,15 @@ void nr_destroy(nr_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     free_list(c->scene_mem);
     free_list(c->path_mem);
+    if (c->rb_stream) (void)hipStreamSynchronize(c->rb_stream);
     for (auto& v : c->views) {
         if (v.d_pattern) (void)hipFree(v.d_pattern);
         for (auto* l : v.layer) if (l) (void)hipFree(l);
+        if (v.d_sel_base) (void)hipFree(v.d_sel_base);
+        if (v.d_mask) (void)hipFree(v.d_mask);
+        if (v.d_rb_u8) (void)hipFree(v.d_rb_u8);
+        if (v.d_rb_den) (void)hipFree(v.d_rb_den);
+        if (v.h_rb) (void)hipHostFree(v.h_rb);
     }
     for (auto e : c->ev) (void)hipEventDestroy(e);
     if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
@@ -837,6 +1035,9 @@ void nr_destroy(nr_ctx* c) {
     if (c->d_err) (void)hipFree(c->d_err);
     if (c->spill) (void)hipFree(c->spill);
     if (c->h_live) (void)hipHostFree(c->h_live);
+    if (c->ev_layers) (void)hipEventDestroy(c->ev_layers);
+    if (c->ev_rb_done) (void)hipEventDestroy(c->ev_rb_done);
+    if (c->rb_stream) (void)hipStreamDestroy(c->rb_stream);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1082,6 +1283,7 @@ int nr_aov_reset(nr_ctx* c, int view) {
     if (!c || view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_INVALID, "bad view");
     HIPCK(hipSetDevice(c->device));
     View& v = c->views[view];
+    LayerWrite w(c, v);
     for (auto* l : v.layer)
         if (l) HIPCK(hipMemsetAsync(l, 0, sizeof(float) * 3 * (size_t)v.H * v.W, c->stream));
     return NR_OK;
@@ -1164,6 +1366,7 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     float* ln = v.layer[NR_LAYER_WORLD_NORMAL];
     c->last_iterations = 0;
     if (d->sample_end > d->sample_begin && d->ray_depth == 0) {
+        LayerWrite w(c, v);
         for (uint32_t s = d->sample_begin; s < d->sample_end; s++)
             hipLaunchKernelGGL(k_fold_zero, gN, dim3(256), 0, c->stream, lb, ld, ln, N, s);
     } else if (d->sample_end > d->sample_begin) {
@@ -1213,6 +1416,7 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
         }
         c->last_iterations = it;
         mark(2);
+        LayerWrite w(c, v);
         hipLaunchKernelGGL(k_unpack_acc, gN, dim3(256), 0, c->stream, c->P.acc, lb, ld, ln, N, c->P.cap);
     }
     HIPCK(hipGetLastError());
@@ -1380,6 +1584,206 @@ int nr_post_device(nr_ctx* c, const float* img, int32_t H, int32_t W, const nr_p
     hipLaunchKernelGGL(k_post, dim3((HW + 255) / 256), dim3(256), 0, c->stream, img, HW, post_args(d), d8, df);
     HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(c->stream));
+    return NR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// interactive mode
+// ---------------------------------------------------------------------------
+static int view_of(nr_ctx* c, int view, View** out) {
+    if (!c) return fail(NR_ERR_INVALID, "null context");
+    if (view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_INVALID, "bad view");
+    *out = &c->views[view];
+    return NR_OK;
+}
+
+// 1-D grid of 128-thread workgroups over the view's 16 x 8 pixel tiles
+static uint32_t tile_blocks(const View& v) { return ((v.W + 15) / 16) * ((v.H + 7) / 8); }
+
+static int check_overflow(nr_ctx* c) {
+    int32_t err = 0;
+    HIPCK(hipMemcpyAsync(&err, c->d_err + CNT_ERR, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (err) return fail(NR_ERR_LIMIT, "traversal stack overflow");
+    return NR_OK;
+}
+
+int nr_viewport(nr_ctx* c, int view, int show_axis_grid) {
+    View* v = nullptr;
+    if (int e = view_of(c, view, &v)) return e;
+    if (!c->has_scene) return fail(NR_ERR_STATE, "nr_viewport before nr_scene_upload");
+    float* lb = v->layer[NR_LAYER_BEAUTY];
+    float* ld = v->layer[NR_LAYER_DIFFUSE];
+    float* ln = v->layer[NR_LAYER_WORLD_NORMAL];
+    if (!lb || !ld || !ln) return fail(NR_ERR_STATE, "view has no beauty/diffuse/world-normal layers");
+    HIPCK(hipSetDevice(c->device));
+    const uint32_t nb = tile_blocks(*v);
+    StackCfg SC;
+    if (int e = stack_spill(c, nb * 128u, SC)) return e;
+    HIPCK(hipMemsetAsync(c->d_err + CNT_ERR, 0, sizeof(int32_t), c->stream));
+    {
+        LayerWrite w(c, *v);
+        if (c->obj_types & NR_TY_OTHER)
+            hipLaunchKernelGGL(k_viewport<NR_TY_ALL>, dim3(nb), dim3(128), trav_lds_bytes(), c->stream, c->d_scene,
+                               v->cam, show_axis_grid != 0, lb, ld, ln, c->d_err, SC);
+        else
+            hipLaunchKernelGGL(k_viewport<NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH>, dim3(nb), dim3(128),
+                               trav_lds_bytes(), c->stream, c->d_scene, v->cam, show_axis_grid != 0, lb, ld, ln,
+                               c->d_err, SC);
+        HIPCK(hipGetLastError());
+    }
+    return check_overflow(c);
+}
+
+int nr_pick(nr_ctx* c, int view, float su, float sv, nr_hit* out) {
+    View* v = nullptr;
+    if (int e = view_of(c, view, &v)) return e;
+    if (!out) return fail(NR_ERR_INVALID, "null output");
+    if (!c->has_scene) return fail(NR_ERR_STATE, "nr_pick before nr_scene_upload");
+    HIPCK(hipSetDevice(c->device));
+    StackCfg SC;
+    if (int e = stack_spill(c, 64u, SC)) return e;
+    nr_hit* dh = nullptr;
+    HIPCK(hipMalloc((void**)&dh, sizeof(nr_hit)));
+    int rc = NR_OK;
+    if (hipMemsetAsync(c->d_err + CNT_ERR, 0, sizeof(int32_t), c->stream) != hipSuccess) rc = fail(NR_ERR_HIP, "memset");
+    if (rc == NR_OK) {
+        if (c->obj_types & NR_TY_OTHER)
+            hipLaunchKernelGGL(k_pick<NR_TY_ALL>, dim3(1), dim3(64), trav_lds_bytes(), c->stream, c->d_scene, v->cam,
+                               su, sv, dh, c->d_err, SC);
+        else
+            hipLaunchKernelGGL(k_pick<NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH>, dim3(1), dim3(64), trav_lds_bytes(),
+                               c->stream, c->d_scene, v->cam, su, sv, dh, c->d_err, SC);
+        if (hipGetLastError() != hipSuccess) rc = fail(NR_ERR_HIP, "k_pick launch");
+    }
+    if (rc == NR_OK && hipMemcpyAsync(out, dh, sizeof(nr_hit), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = fail(NR_ERR_HIP, "copy hit");
+    if (rc == NR_OK) rc = check_overflow(c);
+    (void)hipFree(dh);
+    return rc;
+}
+
+int nr_selection_mask(nr_ctx* c, int view, int32_t object_id, int32_t outline_radius, uint8_t* d_mask) {
+    View* v = nullptr;
+    if (int e = view_of(c, view, &v)) return e;
+    if (!d_mask) return fail(NR_ERR_INVALID, "null mask");
+    if (outline_radius < 0 || outline_radius > 64) return fail(NR_ERR_INVALID, "outline radius outside 0..64");
+    if (!c->has_scene) return fail(NR_ERR_STATE, "nr_selection_mask before nr_scene_upload");
+    HIPCK(hipSetDevice(c->device));
+    const size_t HW = (size_t)v->H * v->W;
+    if (v->sel_cap < HW) {
+        if (v->d_sel_base) HIPCK(hipFree(v->d_sel_base));
+        v->d_sel_base = nullptr;
+        v->sel_cap = 0;
+        HIPCK(hipMalloc((void**)&v->d_sel_base, HW));
+        v->sel_cap = HW;
+    }
+    const uint32_t nb = tile_blocks(*v);
+    StackCfg SC;
+    if (int e = stack_spill(c, nb * 128u, SC)) return e;
+    HIPCK(hipMemsetAsync(c->d_err + CNT_ERR, 0, sizeof(int32_t), c->stream));
+    if (c->obj_types & NR_TY_OTHER)
+        hipLaunchKernelGGL(k_select_mask<NR_TY_ALL>, dim3(nb), dim3(128), trav_lds_bytes(), c->stream, c->d_scene,
+                           v->cam, object_id, v->d_sel_base, c->d_err, SC);
+    else
+        hipLaunchKernelGGL(k_select_mask<NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH>, dim3(nb), dim3(128),
+                           trav_lds_bytes(), c->stream, c->d_scene, v->cam, object_id, v->d_sel_base, c->d_err, SC);
+    hipLaunchKernelGGL(k_outline, dim3(nb), dim3(128), 0, c->stream, v->d_sel_base, d_mask, v->H, v->W,
+                       (int)outline_radius);
+    HIPCK(hipGetLastError());
+    return check_overflow(c);
+}
+
+uint8_t* nr_view_mask_buffer(nr_ctx* c, int view) {
+    View* v = nullptr;
+    if (view_of(c, view, &v) != NR_OK) return nullptr;
+    const size_t HW = (size_t)v->H * v->W;
+    if (v->mask_cap < HW) {
+        if (hipSetDevice(c->device) != hipSuccess) { fail(NR_ERR_HIP, "hipSetDevice"); return nullptr; }
+        if (v->d_mask) (void)hipFree(v->d_mask);
+        v->d_mask = nullptr;
+        v->mask_cap = 0;
+        if (hipMalloc((void**)&v->d_mask, HW) != hipSuccess) { fail(NR_ERR_HIP, "hipMalloc mask"); return nullptr; }
+        if (hipMemset(v->d_mask, 0, HW) != hipSuccess) { fail(NR_ERR_HIP, "hipMemset mask"); return nullptr; }
+        v->mask_cap = HW;
+    }
+    return v->d_mask;
+}
+
+// composite_overlay (stream.cu:214-222): colour channel -> uint8 by truncation
+static void overlay_rgb(const float* color, uint32_t* rgb) {
+    for (int k = 0; k < 3; k++) rgb[k] = (uint32_t)(uint8_t)(fminf(1.0f, fmaxf(0.0f, color[k])) * 255.0f);
+}
+
+int nr_overlay_device(nr_ctx* c, uint8_t* d_img, int32_t H, int32_t W, const uint8_t* d_mask, const float color[3]) {
+    if (!c || !d_img || !d_mask || !color) return fail(NR_ERR_INVALID, "null argument");
+    if (H <= 0 || W <= 0 || (uint64_t)H * W > (1ull << 31)) return fail(NR_ERR_INVALID, "bad image size");
+    HIPCK(hipSetDevice(c->device));
+    uint32_t rgb[3];
+    overlay_rgb(color, rgb);
+    const size_t HW = (size_t)H * W;
+    hipLaunchKernelGGL(k_overlay, dim3((uint32_t)((HW + 255) / 256)), dim3(256), 0, c->stream, d_img, d_mask, HW,
+                       rgb[0], rgb[1], rgb[2]);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(c->stream));
+    return NR_OK;
+}
+
+int nr_readback(nr_ctx* c, int view, const nr_readback_desc* d, uint8_t* host) {
+    View* v = nullptr;
+    if (int e = view_of(c, view, &v)) return e;
+    if (!d || !host) return fail(NR_ERR_INVALID, "null argument");
+    const int layer = d->post.layer;
+    if (layer < 0 || layer > 8 || !v->layer[layer]) return fail(NR_ERR_INVALID, "readback: layer not allocated");
+    if (d->denoiser && (!v->layer[NR_LAYER_DIFFUSE] || !v->layer[NR_LAYER_WORLD_NORMAL]))
+        return fail(NR_ERR_STATE, "denoising needs the DIFFUSE and WORLD_NORMAL layers");
+    std::lock_guard<std::mutex> one(v->rb_mu);
+    HIPCK(hipSetDevice(c->device));
+    const size_t HW = (size_t)v->H * v->W;
+    if (v->rb_cap < HW) {
+        if (v->d_rb_u8) HIPCK(hipFree(v->d_rb_u8));
+        if (v->h_rb) HIPCK(hipHostFree(v->h_rb));
+        v->d_rb_u8 = nullptr; v->h_rb = nullptr; v->rb_cap = 0;
+        HIPCK(hipMalloc((void**)&v->d_rb_u8, 3 * HW));
+        HIPCK(hipHostMalloc((void**)&v->h_rb, 3 * HW, hipHostMallocDefault));
+        v->rb_cap = HW;
+    }
+    if (d->denoiser && v->den_cap < HW) {
+        if (v->d_rb_den) HIPCK(hipFree(v->d_rb_den));
+        v->d_rb_den = nullptr; v->den_cap = 0;
+        HIPCK(hipMalloc((void**)&v->d_rb_den, 3 * HW * sizeof(float)));
+        v->den_cap = HW;
+    }
+    const PostArgs A = post_args(&d->post);
+    {
+        std::lock_guard<std::mutex> lk(v->layers_mu);   // no layer write is enqueued until the reads are
+        const float* src = v->layer[layer];
+        if (d->denoiser) {
+            // the U-Net runs on its own stream and returns when done
+            HIPCK(hipEventSynchronize(c->ev_layers));
+            int rc = nr_unet_forward(static_cast<nr_unet*>(d->denoiser), src, v->layer[NR_LAYER_DIFFUSE],
+                                     v->layer[NR_LAYER_WORLD_NORMAL], 1, (int32_t)v->H, (int32_t)v->W, v->d_rb_den);
+            if (rc != NR_OK) return rc;
+            (void)hipSetDevice(c->device);
+            src = v->d_rb_den;
+        } else {
+            HIPCK(hipStreamWaitEvent(c->rb_stream, c->ev_layers, 0));
+        }
+        hipLaunchKernelGGL(k_post, dim3((uint32_t)((HW + 255) / 256)), dim3(256), 0, c->rb_stream, src, (uint32_t)HW, A,
+                           v->d_rb_u8, (float*)nullptr);
+        HIPCK(hipGetLastError());
+        HIPCK(hipEventRecord(c->ev_rb_done, c->rb_stream));
+    }
+    if (d->d_overlay) {
+        uint32_t rgb[3];
+        overlay_rgb(d->overlay_color, rgb);
+        hipLaunchKernelGGL(k_overlay, dim3((uint32_t)((HW + 255) / 256)), dim3(256), 0, c->rb_stream, v->d_rb_u8,
+                           d->d_overlay, HW, rgb[0], rgb[1], rgb[2]);
+        HIPCK(hipGetLastError());
+    }
+    HIPCK(hipMemcpyAsync(v->h_rb, v->d_rb_u8, 3 * HW, hipMemcpyDeviceToHost, c->rb_stream));
+    HIPCK(hipStreamSynchronize(c->rb_stream));
+    std::memcpy(host, v->h_rb, 3 * HW);
     return NR_OK;
 }
 

@@ -783,3 +783,72 @@ NR_D f3 pbr_shading_normal(const GScene& S, const GMaterial& m, f3 n, f3 tg, f2 
     f3 b = cross(n, t);
     return normalize((t * map_n.x + b * map_n.y) + n * map_n.z);
 }
+
+// ---------------------------------------------------------------------------
+// viewport preview (engine/src/trace.cu:99-143, engine/src/axis_grid.cu) and
+// the screen-space rays of picking and selection (engine/camera.h:181-184, 215-226)
+// ---------------------------------------------------------------------------
+NR_HD float smoothstep_f(float a, float b, float x) {                          // core/cg_std.h:97-103
+    float t = fmaxf(0.0f, fminf(1.0f, (x - a) / (b - a)));
+    return t * t * (3.0f - (2.0f * t));
+}
+NR_HD f3 lerp3(f3 x, f3 y, float w) { return x + (y - x) * w; }               // core/cg_std.h:90-93
+
+NR_HD float grid_mask(float u, float v, float spacing, float width) {          // axis_grid.cu:4-10
+    float du = fabsf(fmodf(u + spacing * 0.5f, spacing) - spacing * 0.5f);
+    float dv = fabsf(fmodf(v + spacing * 0.5f, spacing) - spacing * 0.5f);
+    float d = fminf(du, dv);
+    return 1.0f - smoothstep_f(0.0f, width, d);
+}
+
+// axis_grid (axis_grid.cu:12-51) with max_t = FMAX and line width 1e-3; the
+// grid level's powf(10, level0) is 10^level0 (level0 is a whole number >= 0)
+NR_HD bool axis_grid(const Ray& r, int axis, f3 major, f3& grid_color, float& grid_alpha) {
+    float dir_axis = comp(r.d, axis);
+    if (fabsf(dir_axis) < 1e-6f) return false;
+    float t = -comp(r.o, axis) / dir_axis;
+    if (t <= 0.0f || t >= NR_FMAX) return false;
+    f3 p = at(r, t);
+    p = mk3(fabsf(p.x), fabsf(p.y), fabsf(p.z));
+    float u = comp(p, (axis + 1) % 3), v = comp(p, (axis + 2) % 3);
+    float pixel_size = 1e-3f * t;
+    float level = fmaxf(nrm::log10f_pos(fmaxf(pixel_size, 1e-5f)) + 1.0f, 0.0f);
+    float level0 = floorf(level);
+    float blend = level - level0;
+    float spacing0 = nrm::pow10f_int((int)level0);
+    float spacing1 = spacing0 * 10.0f;
+    float width = pixel_size * 1.5f;
+    float g0 = grid_mask(u, v, spacing0, width);
+    float g1 = grid_mask(u, v, spacing1, width);
+    float mask = fmaxf(g0 * (1.0f - blend), g1);
+    if (mask < 0.01f) return false;
+    float fade = fmaxf(0.0f, 1.0f - t / 100.0f);
+    grid_color = lerp3(s3(0.35f), major, smoothstep_f(0.0f, 1.0f, g1));
+    grid_alpha = mask * fade;
+    return true;
+}
+
+NR_HD bool triplanar_grid(const Ray& r, f3& grid_color, float& grid_alpha) {  // axis_grid.cu:53-70
+    return axis_grid(r, 0, mk3(1.0f, 0.3f, 0.3f), grid_color, grid_alpha) ||
+           axis_grid(r, 1, mk3(0.3f, 1.0f, 0.3f), grid_color, grid_alpha) ||
+           axis_grid(r, 2, mk3(0.3f, 0.3f, 1.0f), grid_color, grid_alpha);
+}
+
+// Material::viewport_color (material/material.h:43-45, 109-111, 218, 289-291, 335-337, 588-596)
+NR_D f3 viewport_color(const GScene& S, const GMaterial& m, f2 uv) {
+    switch (m.type) {
+        case 1: case 2: case 4: case 5: return tex_sample(S, m.tex[0], uv);
+        case 3: return ld3(m.tint);
+        default: return s3(0.0f);
+    }
+}
+
+// DeviceCamera::screen_space_ray (camera.h:215-226)
+NR_HD Ray screen_space_ray(const GCamera& c, float su, float sv) {
+    float x = su * c.res[0];
+    float y = sv * c.res[1];
+    float u = (x - (c.res[0] - 1.0f) * 0.5f) * (c.uvw[0] / c.res[0]);
+    float v = (y - (c.res[1] - 1.0f) * 0.5f) * (c.uvw[1] / c.res[1]);
+    f3 d = normalize(mv(c.R, mk3(u, v, -c.focal_length)));
+    return mkray(ld3(c.pos), d);
+}

@@ -13,6 +13,7 @@
 //   atan     two half-angle reductions t -> t / (1 + sqrt(1 + t^2)), Taylor to t^27
 //   acos     atan2(sqrt(1 - x^2), x) (x^2 and 1 - x^2 are exact in double for float x)
 //   pow      exp(y log x) in double, with the IEEE special cases the path tracer meets
+//   log10    log / ln 10 in double; 10^k for whole k by repeated multiplication
 // Accuracy: < 1 ulp of float against float64 references (tests/test_mathfn.py).
 // Compile with -ffp-contract=off so no step is fused.
 #pragma once
@@ -216,6 +217,17 @@ NRM_FN float powf_(float xf, float yf) {
     if (x == bitsd(0x7FF0000000000000ull)) return (float)(yf > 0.0f ? (neg ? -x : x) : 0.0);
     double v = exp_d(y * log_pos(x));
     return (float)(neg ? -v : v);
+}
+
+// log10 of a positive finite float: log(x) / ln 10 in double, rounded once
+NRM_FN float log10f_pos(float xf) { return (float)(log_pos((double)xf) / 2.302585092994046); }
+
+// 10^k for an integer k >= 0 (a float exponent that is a whole number): exact in
+// double up to 10^22, so the float result is the correctly rounded power there
+NRM_FN float pow10f_int(int k) {
+    double v = 1.0;
+    for (int i = 0; i < k; i++) v = v * 10.0;
+    return (float)v;
 }
 
 NRM_FN double dsqrt(double x) {

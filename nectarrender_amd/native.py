@@ -50,12 +50,17 @@ def lib() -> C.CDLL:
         "nr_view_size": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "nr_last_timings": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int]),
         "nr_set_timing": (C.c_int, [vp, C.c_int]),
+        "nr_viewport": (C.c_int, [vp, C.c_int, C.c_int]),
+        "nr_pick": (C.c_int, [vp, C.c_int, C.c_float, C.c_float, C.POINTER(abi.NrHit)]),
+        "nr_selection_mask": (C.c_int, [vp, C.c_int, C.c_int32, C.c_int32, vp]),
+        "nr_overlay_device": (C.c_int, [vp, vp, C.c_int32, C.c_int32, vp, C.POINTER(C.c_float)]),
+        "nr_readback": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrReadbackDesc), C.POINTER(C.c_uint8)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
-    if L.nr_abi_version() != 2:
+    if L.nr_abi_version() != 3:
         raise NectarError("ABI version mismatch")
     _lib = L
     return L
@@ -144,6 +149,36 @@ class Context:
     def post_device(self, d_img: int, H: int, W: int, desc: abi.NrPostDesc, d_u8: int = 0, d_f32: int = 0):
         _check(lib().nr_post_device(self._h, C.c_void_p(d_img), H, W, C.byref(desc), C.c_void_p(d_u8 or None),
                                     C.c_void_p(d_f32 or None)))
+
+    # interactive mode ------------------------------------------------------
+    def viewport(self, view: int = 0, show_axis_grid: bool = True):
+        """Viewport preview into the view's BEAUTY / DIFFUSE / WORLD_NORMAL layers."""
+        _check(lib().nr_viewport(self._h, view, int(bool(show_axis_grid))))
+
+    def pick(self, view: int, su: float, sv: float):
+        h = abi.NrHit()
+        _check(lib().nr_pick(self._h, view, su, sv, C.byref(h)))
+        return h
+
+    def selection_mask_device(self, view: int, object_id: int, radius: int, d_mask: int):
+        _check(lib().nr_selection_mask(self._h, view, object_id, radius, C.c_void_p(d_mask)))
+
+    def overlay_device(self, d_img: int, H: int, W: int, d_mask: int, color):
+        col = (C.c_float * 3)(*[float(c) for c in color])
+        _check(lib().nr_overlay_device(self._h, C.c_void_p(d_img), H, W, C.c_void_p(d_mask), col))
+
+    def readback(self, view: int, post: abi.NrPostDesc, denoiser=None, d_overlay: int = 0,
+                 overlay_color=(1.0, 1.0, 1.0)) -> np.ndarray:
+        """TransferStream::readback: [denoise] -> post -> [overlay] -> uint8 (3, H, W)."""
+        H, W = self.views[view]
+        d = abi.NrReadbackDesc()
+        d.post = post
+        d.denoiser = denoiser
+        d.d_overlay = d_overlay or None
+        d.overlay_color[:] = [float(c) for c in overlay_color]
+        out = np.empty((3, H, W), np.uint8)
+        _check(lib().nr_readback(self._h, view, C.byref(d), out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
 
     def set_timing(self, on: bool):
         _check(lib().nr_set_timing(self._h, int(on)))

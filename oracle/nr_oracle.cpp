@@ -1465,6 +1465,219 @@ static V3 kelvin_to_rgb(float kelvin) {                                         
 }
 static inline float clamp01(float x) { return fminf(1.0f, fmaxf(0.0f, x)); }
 
+// ---------------------------------------------------------------------------
+// viewport preview (engine/src/trace.cu:99-143, engine/src/axis_grid.cu),
+// picking (trace.cu:166-177) and selection (scene/src/scene_interface.cu:3-50),
+// overlay (data/src/stream.cu:177-202)
+// ---------------------------------------------------------------------------
+static inline float cg_smoothstep(float a, float b, float x) {                             // cg_std.h:97-103
+    float t = fmaxf(0.0f, fminf(1.0f, (x - a) / (b - a)));                                 // saturate, cg_std.h:82-85
+    return t * t * (3.0f - (2.0f * t));
+}
+static inline V3 cg_lerp(V3 x, V3 y, float w) { return add(x, muls(sub(y, x), w)); }      // cg_std.h:90-93
+
+static float grid_mask(float u, float v, float spacing, float width) {                    // axis_grid.cu:4-10
+    float du = fabsf(fmodf(u + spacing * 0.5f, spacing) - spacing * 0.5f);
+    float dv = fabsf(fmodf(v + spacing * 0.5f, spacing) - spacing * 0.5f);
+    float d = fminf(du, dv);
+    return 1.0f - cg_smoothstep(0.0f, width, d);
+}
+
+static bool axis_grid(const Ray& ray, float max_t, int axis, V3 major, V3& gc, float& ga) { // axis_grid.cu:12-51
+    const float line_width = 1e-3f;
+    float dir_axis = get(ray.d, axis);
+    if (fabsf(dir_axis) < 1e-6f) return false;
+    float t = -get(ray.o, axis) / dir_axis;
+    if (t <= 0.0f || t >= max_t) return false;
+    V3 p = at(ray, t);
+    p = {fabsf(p.x), fabsf(p.y), fabsf(p.z)};
+    int u_axis = (axis + 1) % 3, v_axis = (axis + 2) % 3;
+    float u = get(p, u_axis), v = get(p, v_axis);
+    float pixel_size = line_width * t;
+    // log10f and powf(10, level0) through the shared transcendentals (level0 is whole)
+    float level = fmaxf(nrm::log10f_pos(fmaxf(pixel_size, 1e-5f)) + 1.0f, 0.0f);
+    float level0 = floorf(level);
+    float blend = level - level0;
+    float spacing0 = nrm::pow10f_int((int)level0);
+    float spacing1 = spacing0 * 10.0f;
+    float width = pixel_size * 1.5f;
+    float g0 = grid_mask(u, v, spacing0, width);
+    float g1 = grid_mask(u, v, spacing1, width);
+    float mask = fmaxf(g0 * (1.0f - blend), g1);
+    if (mask < 0.01f) return false;
+    float fade = fmaxf(0.0f, 1.0f - t / 100.0f);
+    gc = cg_lerp(v3(0.35f), major, cg_smoothstep(0.0f, 1.0f, g1));
+    ga = mask * fade;
+    return true;
+}
+
+static bool triplanar_grid(const Ray& ray, float max_t, V3& gc, float& ga) {             // axis_grid.cu:53-70
+    bool hit_grid = false;
+    hit_grid = hit_grid || axis_grid(ray, max_t, 0, {1.0f, 0.3f, 0.3f}, gc, ga);
+    hit_grid = hit_grid || axis_grid(ray, max_t, 1, {0.3f, 1.0f, 0.3f}, gc, ga);
+    hit_grid = hit_grid || axis_grid(ray, max_t, 2, {0.3f, 0.3f, 1.0f}, gc, ga);
+    return hit_grid;
+}
+
+static V3 viewport_color(const Scene& S, const Material& m, const HitRec& rec) {          // material.h:588-596
+    switch (m.type) {
+        case NR_MAT_LAMBERTIAN: return tex_sample(S, m.tex[0], rec.uv);                   // material.h:43-45
+        case NR_MAT_PBR: return tex_sample(S, m.tex[0], rec.uv);                          // material.h:109-111
+        case NR_MAT_DIELECTRIC: return m.tint;                                            // material.h:218
+        case NR_MAT_EMISSIVE: return tex_sample(S, m.tex[0], rec.uv);                     // material.h:289-291
+        case NR_MAT_ISOTROPIC: return tex_sample(S, m.tex[0], rec.uv);                    // material.h:335-337
+        default: return v3(0.0f);
+    }
+}
+
+// DeviceCamera::get_ray_center (camera.h:181-184): get_focus_point(x, y, 0, 0, 0)
+static Ray center_ray(const Camera& c, uint32_t x, uint32_t y) {
+    V2 base = c.pattern[0];
+    V2 shift = pcg_vector2(0.0f, 0u);
+    V2 off = {fmodf(base.x + shift.x, 1.0f) - 0.5f, fmodf(base.y + shift.y, 1.0f) - 0.5f};
+    float rx = c.p.resolution[0], ry = c.p.resolution[1];
+    float u = ((float)x + off.x - (rx - 1.0f) * 0.5f) * (c.uvw.x / rx);
+    float v = ((float)y + off.y - (ry - 1.0f) * 0.5f) * (c.uvw.y / ry);
+    V3 center = mv(c.R, {u, v, -c.p.focal_length});
+    V3 pos = {c.p.position[0], c.p.position[1], c.p.position[2]};
+    V3 focus = add(pos, muls(center, c.p.focus_distance / c.p.focal_length));
+    return make_ray(pos, normalize(sub(focus, pos)));
+}
+
+// DeviceCamera::screen_space_ray (camera.h:215-226)
+static Ray screen_space_ray(const Camera& c, float su, float sv) {
+    float x = su * c.p.resolution[0];
+    float y = sv * c.p.resolution[1];
+    float u = (x - (c.p.resolution[0] - 1.0f) * 0.5f) * (c.uvw.x / c.p.resolution[0]);
+    float v = (y - (c.p.resolution[1] - 1.0f) * 0.5f) * (c.uvw.y / c.p.resolution[1]);
+    V3 direction = normalize(mv(c.R, {u, v, -c.p.focal_length}));
+    return make_ray({c.p.position[0], c.p.position[1], c.p.position[2]}, direction);
+}
+
+int nro_viewport(const nro_scene* sc, const nro_camera* cc, int show_axis_grid, float* beauty, float* diffuse,
+                 float* normal, int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+    const Scene& S = sc->s;
+    const Camera& c = cc->c;
+    const size_t H = c.H, W = c.W, HW = H * W;
+    int err = 0;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long y = 0; y < (long)H; y++) {
+        for (size_t x = 0; x < W; x++) {
+            try {
+                Ray ray = center_ray(c, (uint32_t)x, (uint32_t)y);
+                HitRec rec;
+                bool hit = scene_hit(S, ray, Interval{EPS, FMAX}, rec, nullptr);
+                V3 b, dcol = v3(0.0f), n = v3(0.0f);    // sample AOVs start cleared
+                if (!hit) {
+                    V3 gc; float ga;
+                    V3 sky = sky_sample(S, ray);
+                    b = (show_axis_grid && triplanar_grid(ray, FMAX, gc, ga)) ? cg_lerp(sky, gc, ga) : sky;
+                } else {
+                    V3 light_vector = sub({c.p.position[0], c.p.position[1], c.p.position[2]}, rec.p);
+                    float light_dist = length(light_vector);
+                    V3 headlight_dir = normalize(light_vector);
+                    float headlight = dot(headlight_dir, rec.n) * 0.5f + 0.5f;
+                    headlight *= 1.0f / (light_dist * light_dist);
+                    headlight = fminf(0.75f, headlight);
+                    V3 directional_light_dir = {1.0f, 1.0f, 1.0f};
+                    float directional = (dot(directional_light_dir, rec.n) + 1.0f) * 0.5f;
+                    directional = fmaxf(0.1f, directional);
+                    V3 col = viewport_color(S, S.materials[rec.material_id], rec);
+                    b = muls(col, fmaxf(headlight, directional));
+                    dcol = col;
+                    n = rec.n;
+                }
+                const size_t i = (size_t)y * W + x;
+                beauty[i] = 0.0f + b.x; beauty[HW + i] = 0.0f + b.y; beauty[2 * HW + i] = 0.0f + b.z;
+                diffuse[i] = 0.0f + dcol.x; diffuse[HW + i] = 0.0f + dcol.y; diffuse[2 * HW + i] = 0.0f + dcol.z;
+                normal[i] = 0.0f + n.x; normal[HW + i] = 0.0f + n.y; normal[2 * HW + i] = 0.0f + n.z;
+            } catch (...) {
+#pragma omp atomic write
+                err = 1;
+            }
+        }
+    }
+    return err;
+}
+
+int nro_triplanar_grid(const float* ray6, float* out4) {
+    Ray r = make_ray({ray6[0], ray6[1], ray6[2]}, {ray6[3], ray6[4], ray6[5]});
+    V3 gc{0, 0, 0};
+    float ga = 0.0f;
+    bool h = triplanar_grid(r, FMAX, gc, ga);
+    out4[0] = gc.x; out4[1] = gc.y; out4[2] = gc.z; out4[3] = ga;
+    return h ? 1 : 0;
+}
+
+int nro_center_ray(const nro_camera* cc, uint32_t x, uint32_t y, float* out6) {
+    Ray r = center_ray(cc->c, x, y);
+    out6[0] = r.o.x; out6[1] = r.o.y; out6[2] = r.o.z;
+    out6[3] = r.d.x; out6[4] = r.d.y; out6[5] = r.d.z;
+    return 0;
+}
+
+int nro_screen_ray(const nro_camera* cc, float su, float sv, float* out6) {
+    Ray r = screen_space_ray(cc->c, su, sv);
+    out6[0] = r.o.x; out6[1] = r.o.y; out6[2] = r.o.z;
+    out6[3] = r.d.x; out6[4] = r.d.y; out6[5] = r.d.z;
+    return 0;
+}
+
+int nro_pick(const nro_scene* sc, const nro_camera* cc, float su, float sv, nr_hit* out) {
+    Ray r = screen_space_ray(cc->c, su, sv);
+    HitRec rec;
+    bool h = scene_hit(sc->s, r, Interval{EPS, FMAX}, rec, nullptr);
+    fill_hit(rec, h, out);
+    return 0;
+}
+
+int nro_selection_mask(const nro_scene* sc, const nro_camera* cc, int32_t object_id, int radius, uint8_t* out) {
+    const Camera& c = cc->c;
+    const size_t H = c.H, W = c.W;
+    std::vector<uint8_t> base(H * W);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long y = 0; y < (long)H; y++)
+        for (size_t x = 0; x < W; x++) {                                                  // mask_selected_kernel
+            float su = (x + 0.5f) / (float)W;
+            float sv = (y + 0.5f) / (float)H;
+            Ray r = screen_space_ray(c, su, sv);
+            HitRec rec;
+            bool hit = scene_hit(sc->s, r, Interval{EPS, FMAX}, rec, nullptr);
+            base[(size_t)y * W + x] = (hit && rec.object_id == object_id) ? 255u : 0u;
+        }
+    for (size_t y = 0; y < H; y++)
+        for (size_t x = 0; x < W; x++) {                                                  // build_outline_kernel
+            size_t idx = y * W + x;
+            out[idx] = 0u;
+            if (base[idx] != 0) continue;
+            bool found = false;
+            for (int dy = -radius; dy <= radius && !found; dy++)
+                for (int dx = -radius; dx <= radius && !found; dx++) {
+                    long nx = (long)x + dx, ny = (long)y + dy;
+                    if (nx < 0 || ny < 0 || nx >= (long)W || ny >= (long)H) continue;
+                    if (base[(size_t)ny * W + (size_t)nx] != 0) found = true;
+                }
+            if (found) out[idx] = 255u;
+        }
+    return 0;
+}
+
+int nro_overlay(uint8_t* img, const uint8_t* mask, const float* color, int H, int W) {
+    uint32_t rgb[3];
+    for (int k = 0; k < 3; k++) rgb[k] = (uint8_t)(fminf(1.0f, fmaxf(0.0f, color[k])) * 255.0f);  // stream.cu:214-222
+    const size_t HW = (size_t)H * W;
+    for (size_t i = 0; i < HW; i++) {
+        uint32_t alpha = mask[i];
+        if (alpha == 0u) continue;
+        uint32_t inv = 255u - alpha;
+        for (int k = 0; k < 3; k++) img[k * HW + i] = (uint8_t)((rgb[k] * alpha + img[k * HW + i] * inv + 127u) / 255u);
+    }
+    return 0;
+}
+
 int nro_post(const float* chw, int H, int W, const nr_post_desc* d, float* out_f32, uint8_t* out_u8) {
     size_t HW = (size_t)H * W;
     V3 ref = kelvin_to_rgb(6500.0f);

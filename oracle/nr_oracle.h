@@ -59,6 +59,18 @@ void nro_accumulate(float* avg, const float* sample, size_t n, uint32_t current_
 /* post: process_stream_kernel on a 3 x H x W image */
 int nro_post(const float* chw, int H, int W, const nr_post_desc* d, float* out_f32, uint8_t* out_u8);
 
+/* viewport preview (trace_viewport_kernel): overwrites 3 x H x W beauty,
+ * diffuse and normal; screen-space ray, picking, selection mask + outline,
+ * overlay composite (composite_overlay_kernel) */
+int nro_viewport(const nro_scene* s, const nro_camera* c, int show_axis_grid, float* beauty, float* diffuse,
+                 float* normal, int threads);
+int nro_triplanar_grid(const float* ray6, float* out4 /* grid colour rgb, alpha */);
+int nro_center_ray(const nro_camera* c, uint32_t x, uint32_t y, float* out6);
+int nro_screen_ray(const nro_camera* c, float su, float sv, float* out6);
+int nro_pick(const nro_scene* s, const nro_camera* c, float su, float sv, nr_hit* out);
+int nro_selection_mask(const nro_scene* s, const nro_camera* c, int32_t object_id, int radius, uint8_t* out);
+int nro_overlay(uint8_t* chw_u8, const uint8_t* mask, const float* color3, int H, int W);
+
 /* RNG and hashing */
 void nro_philox_block(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float nro_philox_uniform(uint32_t seed, uint32_t subsequence, uint32_t offset);

@@ -62,6 +62,13 @@ def lib():
         "nro_pdf_cosine_value": (C.c_float, [fp, fp]),
         "nro_pdf_cosine_generate": (C.c_int, [fp, C.c_uint32, C.c_uint32, C.c_uint32, fp]),
         "nro_mathfn": (None, [C.c_int, fp, fp, fp, C.c_int]),
+        "nro_viewport": (C.c_int, [vp, vp, C.c_int, fp, fp, fp, C.c_int]),
+        "nro_triplanar_grid": (C.c_int, [fp, fp]),
+        "nro_center_ray": (C.c_int, [vp, C.c_uint32, C.c_uint32, fp]),
+        "nro_screen_ray": (C.c_int, [vp, C.c_float, C.c_float, fp]),
+        "nro_pick": (C.c_int, [vp, vp, C.c_float, C.c_float, C.POINTER(abi.NrHit)]),
+        "nro_selection_mask": (C.c_int, [vp, vp, C.c_int32, C.c_int, C.POINTER(C.c_uint8)]),
+        "nro_overlay": (C.c_int, [C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), fp, C.c_int, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -159,6 +166,57 @@ def render(scene: OracleScene, cam: OracleCamera, desc: abi.NrRenderDesc, thread
     if rc:
         raise RuntimeError("oracle render failed")
     return b, d, nn
+
+
+def viewport(scene: OracleScene, cam: OracleCamera, show_axis_grid: bool, threads=0):
+    """trace_viewport_kernel: (beauty, diffuse, world normal), each 3 x H x W."""
+    b = np.zeros((3, cam.H, cam.W), np.float32)
+    d = np.zeros_like(b)
+    nn = np.zeros_like(b)
+    if lib().nro_viewport(scene.h, cam.h, int(bool(show_axis_grid)), _fp(b), _fp(d), _fp(nn), threads):
+        raise RuntimeError("oracle viewport failed")
+    return b, d, nn
+
+
+def triplanar_grid(ray6):
+    """(hit, rgb, alpha) of the viewport's axis grid for one ray (origin, direction)."""
+    r = np.ascontiguousarray(ray6, np.float32)
+    out = np.zeros(4, np.float32)
+    h = lib().nro_triplanar_grid(_fp(r), _fp(out))
+    return bool(h), out[:3].copy(), float(out[3])
+
+
+def center_ray(cam: OracleCamera, x: int, y: int):
+    out = np.zeros(6, np.float32)
+    lib().nro_center_ray(cam.h, x, y, _fp(out))
+    return out
+
+
+def screen_ray(cam: OracleCamera, su: float, sv: float):
+    out = np.zeros(6, np.float32)
+    lib().nro_screen_ray(cam.h, su, sv, _fp(out))
+    return out
+
+
+def pick(scene: OracleScene, cam: OracleCamera, su: float, sv: float):
+    h = abi.NrHit()
+    lib().nro_pick(scene.h, cam.h, su, sv, C.byref(h))
+    return h
+
+
+def selection_mask(scene: OracleScene, cam: OracleCamera, object_id: int, radius: int):
+    out = np.zeros((cam.H, cam.W), np.uint8)
+    lib().nro_selection_mask(scene.h, cam.h, object_id, radius, out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out
+
+
+def overlay(img_u8: np.ndarray, mask: np.ndarray, color):
+    img = np.ascontiguousarray(img_u8, np.uint8).copy()
+    m = np.ascontiguousarray(mask, np.uint8)
+    col = np.asarray(color, np.float32)
+    _, H, W = img.shape
+    lib().nro_overlay(img.ctypes.data_as(C.POINTER(C.c_uint8)), m.ctypes.data_as(C.POINTER(C.c_uint8)), _fp(col), H, W)
+    return img
 
 
 def accumulate(avg: np.ndarray, sample: np.ndarray, s: int):

@@ -113,6 +113,7 @@ def test_stream_readback_matches_oracle_post(oracle, oracle_cornell):
     cfg = P.data.StreamConfig()
     cfg.tm_method = P.data.TonemapMethod.ACES
     cfg.wb_temperature = 5200.0
+    cfg.apply_denoising = False
     e.stream().update_config(cfg)
     got = e.stream().readback_numpy()
     desc = abi.post_desc(tm_method=2, wb_temperature=5200.0)
@@ -163,3 +164,93 @@ def test_mesh_from_obj_file_renders_like_scenedesc(tmp_path, oracle):
     cam = abi.camera_desc((64, 64), (0, 0.5, 2.5), (-8, 0, 0), 4, 2.0, 2.5, 0.0, 2.0, 1.0)
     ref = oracle.render(oracle.OracleScene(s), oracle.OracleCamera(cam, 3), abi.render_desc(1, 4, 6, 3))
     assert _same(e.layers().beauty.numpy_float(), ref[0])
+
+
+def test_viewport_engine_matches_oracle(oracle):
+    """RenderEngine.viewport (engine.cpp:76-96) fills the layers with the preview shading."""
+    e = _engine()
+    e.set_axis_grid_visible(True)
+    e.viewport()
+    s, cam, _ = scenes.cornell(spp=SPP, res=RES)
+    ref = oracle.viewport(oracle.OracleScene(s), oracle.OracleCamera(cam, 42), True)
+    assert _same(e.layers().beauty.numpy_float(), ref[0])
+    assert _same(e.layers().diffuse.numpy_float(), ref[1])
+    assert _same(e.layers().world_normal.numpy_float(), ref[2])
+
+
+def test_viewport_engine_type_in_idle_loop():
+    e = _engine()
+    e.set_engine_type(P.engine.EngineType.VIEWPORT)
+    frames = []
+    e.on_restarted = lambda: frames.append(1)
+    t = threading.Thread(target=e.idle)
+    t.start()
+    e.requests().start()
+    import time
+    time.sleep(0.5)
+    e.requests().shutdown()
+    t.join(timeout=60)
+    assert not t.is_alive()
+    assert (e.layers().diffuse.numpy_float() != 0).any()     # a viewport frame was drawn
+
+
+def test_scene_interface_pick_outline_overlay(oracle):
+    """query_scene -> picked record; the selection outline (radius 3, colour (0.98, 0.75, 0.17)) is
+    composited over the post-processed stream until disable()."""
+    e = _engine()
+    e.render()
+    si = e.get_scene_interface()
+    s, cam, rp = scenes.cornell(spp=SPP, res=RES)
+    osc, ocam = oracle.OracleScene(s), oracle.OracleCamera(cam, rp["seed"])
+    su, sv = 0.62, 0.80                                   # on the short cube
+    si.query_scene(su, sv)
+    ref_hit = oracle.pick(osc, ocam, su, sv)
+    rec = si.get_hit_record()
+    assert ref_hit.object_id >= 0 and rec.object_id == ref_hit.object_id and rec.t == ref_hit.t
+    assert si.is_enabled()
+    si.update()
+    assert e.stream().has_overlay()
+    cfg = P.data.StreamConfig()
+    cfg.apply_denoising = False
+    e.stream().update_config(cfg)
+    got = e.stream().readback_numpy()
+    base, _ = oracle.post(e.layers().beauty.numpy_float(), abi.post_desc())
+    mask = oracle.selection_mask(osc, ocam, ref_hit.object_id, 3)
+    assert (mask > 0).any()
+    np.testing.assert_array_equal(got, oracle.overlay(base, mask, (0.98, 0.75, 0.17)))
+    si.disable()
+    si.update()
+    assert not e.stream().has_overlay() and si.is_disabled()
+    np.testing.assert_array_equal(e.stream().readback_numpy(), base)
+    nodes = si.get_scene_outline().nodes
+    assert [n.type_name for n in nodes][-1] == "ObjectLight" and len(nodes) == 8
+
+
+def test_scene_interface_set_transform_rerenders(oracle):
+    e = _engine()
+    si = e.get_scene_interface()
+    si.select_scene_node(si.get_scene_outline().nodes[5])      # the short cube
+    x = si.get_transform()
+    moved = P.core.Transform(V3(-0.1, -0.35, 0.25), V3(0.0, 10.0, 0.0), V3(0.3))
+    si.set_transform(moved)
+    e.render()
+    s, cam, rp = scenes.cornell(spp=SPP, res=RES)
+    s.objects[5]["position"] = (-0.1, -0.35, 0.25)
+    s.objects[5]["rotation"] = (0.0, 10.0, 0.0)
+    ref = oracle.render(oracle.OracleScene(s), oracle.OracleCamera(cam, rp["seed"]),
+                        abi.render_desc(1, SPP, rp["depth"], rp["seed"]))
+    assert _same(e.layers().beauty.numpy_float(), ref[0])
+    assert x is not None
+
+
+def test_default_readback_is_denoised():
+    """StreamConfig.apply_denoising defaults to on (stream.h:14): the readback is the U-Net output."""
+    e = _engine()
+    e.render()
+    got = e.stream().readback_numpy()
+    cfg = P.data.StreamConfig()
+    cfg.apply_denoising = False
+    e.stream().update_config(cfg)
+    plain = e.stream().readback_numpy()
+    assert got.shape == plain.shape and not np.array_equal(got, plain)
+    assert np.abs(got.astype(np.int32) - plain.astype(np.int32)).mean() < 40
