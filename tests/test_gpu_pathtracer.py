@@ -253,4 +253,9 @@ def test_default_readback_is_denoised():
     e.stream().update_config(cfg)
     plain = e.stream().readback_numpy()
     assert got.shape == plain.shape and not np.array_equal(got, plain)
-    assert np.abs(got.astype(np.int32) - plain.astype(np.int32)).mean() < 40
+
+    def tv(a):
+        a = a.astype(np.float64)
+        return np.abs(np.diff(a, axis=1)).mean() + np.abs(np.diff(a, axis=2)).mean()
+
+    assert tv(got) < 0.5 * tv(plain)          # 4-pass noise removed
