@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define NR_ABI_VERSION 3
+#define NR_ABI_VERSION 4
 
 enum nr_status {
     NR_OK = 0,
@@ -80,6 +80,12 @@ enum nr_layer {
  * HDRI (importance sampled) and ObjectLights with power-heuristic MIS; it is
  * checked by convergence, not per pixel. */
 enum nr_mode { NR_MODE_PARITY = 0, NR_MODE_FAST = 1 };
+
+/* Where nr_scene_upload builds mesh trees (BVH<T>::build, csrc/hittable/include/bvh/bvh.h:36-55,
+ * for Mesh::build, csrc/hittable/include/hittable/mesh.h:177-216).  Both builders make the same
+ * binned-SAH decisions and produce the same node array; DEVICE runs them on the GPU.  AUTO
+ * (the default) uses the device for meshes of at least 4096 triangles. */
+enum nr_build_mode { NR_BUILD_HOST = 0, NR_BUILD_DEVICE = 1, NR_BUILD_AUTO = 2 };
 
 /* One hittable.  Transform(position, rotation_deg, scale) semantics of
  * csrc/core/include/core/transform.h:25-34; spheres use rotation 0, scale 1
@@ -315,6 +321,26 @@ typedef struct nr_readback_desc {
  * layers waits for a readback in flight.  Not concurrently with nr_camera_set,
  * nr_scene_upload or nr_destroy. */
 int nr_readback(nr_ctx* ctx, int view, const nr_readback_desc* desc, uint8_t* host_u8);
+
+/* ---- scene trees and live edits ---- */
+
+/* Mesh tree builder for later nr_scene_upload calls (enum nr_build_mode). */
+int nr_set_build_mode(nr_ctx* ctx, int mode);
+
+/* The uploaded tree of mesh `mesh`: n_nodes child-pair nodes of 64 bytes (both child boxes
+ * as min[3] max[3] left then right, then the left and right refs: >= 0 node index, < 0 leaf
+ * ~slot; two zero pads) and the original triangle index of every leaf slot.  nodes64 /
+ * slot_orig may be NULL (sizes only); depth = nodes on the longest root-to-leaf path. */
+int nr_mesh_tree(nr_ctx* ctx, int mesh, void* nodes64, int32_t* slot_orig, int32_t* n_nodes, int32_t* n_tris,
+                 int32_t* depth);
+
+/* SceneInterface::set_transform (csrc/scene/include/interface.h:127-133) as a partial rebuild
+ * (the registry's mark_dirty + build_bvh, csrc/hittable/include/hittable/registry.h:216-234):
+ * object `index` (registration order, = nr_hit.object_id) gets Transform(position,
+ * rotation in degrees, scale); only the object records and the top-level tree are rebuilt and
+ * re-uploaded; mesh trees, materials, textures and the sky stay on the device. */
+int nr_object_set_transform(nr_ctx* ctx, int32_t index, const float position[3], const float rotation[3],
+                            const float scale[3]);
 
 /* Wait for all work of the context. */
 int nr_sync(nr_ctx* ctx);

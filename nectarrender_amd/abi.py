@@ -85,6 +85,13 @@ HIT_DTYPE = np.dtype([("object_id", "<i4"), ("material_id", "<i4"), ("prim", "<i
 assert HIT_DTYPE.itemsize == C.sizeof(NrHit) == 64
 
 
+# child-pair tree node (nr_mesh_tree): both child boxes, child refs (>= 0 node, < 0 ~leaf slot), pads
+NODE_DTYPE = np.dtype([("lmin", "<f4", 3), ("lmax", "<f4", 3), ("rmin", "<f4", 3), ("rmax", "<f4", 3),
+                       ("left", "<i4"), ("right", "<i4"), ("pad", "<i4", 2)])
+assert NODE_DTYPE.itemsize == 64
+BUILD_HOST, BUILD_DEVICE, BUILD_AUTO = 0, 1, 2
+
+
 class NrPostDesc(C.Structure):
     _fields_ = [("linear_to_gamma", C.c_int32), ("apply_white_balance", C.c_int32), ("wb_temperature", C.c_float),
                 ("wb_tint", C.c_float), ("apply_tonemapping", C.c_int32), ("tm_method", C.c_int32),

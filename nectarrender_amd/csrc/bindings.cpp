@@ -1033,13 +1033,23 @@ HitP SceneInterface::picked() {                                              // 
     return e.scene->objects[(size_t)rec.object_id];
 }
 
-void SceneInterface::set_transform(const Transform& x) {                     // interface.h:127-133
+// interface.h:127-133: only the object records and the top-level tree are
+// rebuilt on the device (nr_object_set_transform)
+void SceneInterface::set_transform(const Transform& x) {
     RenderEngine& e = *eng;
+    int32_t idx;
     {
         std::lock_guard<std::mutex> g(mu);
         picked()->xform = x;
+        idx = rec.object_id;
     }
-    e.upload_scene();
+    float p[3], r[3], sc[3];
+    for (int k = 0; k < 3; k++) { p[k] = x.position.e[k]; r[k] = x.rotation_deg.e[k]; sc[k] = x.scale.e[k]; }
+    {
+        std::lock_guard<std::mutex> g(e.ctx->mu);
+        std::unique_lock<std::shared_mutex> lay(e.ctx->layout_mu);
+        check(nr_object_set_transform(e.ctx->h, idx, p, r, sc));
+    }
     e.requests.restart();
 }
 

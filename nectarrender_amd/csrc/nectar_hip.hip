@@ -36,6 +36,7 @@
 #include "../../include/nectar_denoise.h"
 #include "../../include/nectar_hip.h"
 #include "nr_bvh.h"
+#include "nr_bvh_gpu.h"
 #include "nr_device.h"
 #include "nr_error.h"
 #include "nr_types.h"
@@ -892,6 +893,13 @@ struct nr_ctx {
     std::vector<void*> scene_mem;
     bool mat_present[NR_MAT_TYPES] = {false};
     int obj_types = NR_TY_ALL;      // object types of the scene (NR_TY_* bits): picks the traversal kernel
+    int build_mode = NR_BUILD_AUTO;
+    // host copies kept for introspection and live edits
+    std::vector<GMesh> h_meshes;
+    std::vector<int32_t> h_mesh_nodes;      // child-pair nodes per mesh
+    std::vector<GObject> reg_objects;       // registration order
+    std::vector<nrb::Box> reg_local;        // local (object-space) boxes
+    int max_mesh_depth = 0;
     View views[NR_MAX_VIEWS];
     // path state
     PathPtrs P{};
@@ -958,6 +966,63 @@ static int stack_spill(nr_ctx* c, uint32_t n, StackCfg& SC) {
     }
     SC.spill = c->spill;
     SC.stride = n;
+    return NR_OK;
+}
+
+// Mesh tree on the GPU (nr_bvh_gpu.h): the same node array as the host build.
+// Node, triangle and index arrays are left in the scene allocations.
+static int build_mesh_on_device(nr_ctx* c, const nr_mesh_desc& md, GMesh& gm, GNode*& dn, GTri*& dt, uint32_t*& di,
+                                int32_t& n_nodes) {
+    const int32_t nt = md.n_triangles;
+    float* dv = nullptr;
+    uint32_t* dtri = nullptr;
+    float* bx = nullptr;
+    int32_t* dord = nullptr;
+    auto release = [&] {
+        if (dv) (void)hipFree(dv);
+        if (dtri) (void)hipFree(dtri);
+        if (bx) (void)hipFree(bx);
+        if (dord) (void)hipFree(dord);
+    };
+    auto hip = [&](hipError_t e, const char* what) -> int {
+        if (e == hipSuccess) return NR_OK;
+        release();
+        return fail(NR_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    if (int e = hip(hipMalloc((void**)&dv, sizeof(float) * 11 * (size_t)md.n_vertices), "hipMalloc vertices")) return e;
+    if (int e = hip(hipMalloc((void**)&dtri, sizeof(uint32_t) * 3 * (size_t)nt), "hipMalloc triangles")) return e;
+    if (int e = hip(hipMalloc((void**)&bx, sizeof(float) * 6 * (size_t)nt), "hipMalloc boxes")) return e;
+    if (int e = hip(hipMalloc((void**)&dord, sizeof(int32_t) * (size_t)nt), "hipMalloc order")) return e;
+    if (int e = hip(hipMemcpyAsync(dv, md.vertices, sizeof(float) * 11 * (size_t)md.n_vertices, hipMemcpyHostToDevice,
+                                   c->stream), "copy vertices")) return e;
+    if (int e = hip(hipMemcpyAsync(dtri, md.triangles, sizeof(uint32_t) * 3 * (size_t)nt, hipMemcpyHostToDevice,
+                                   c->stream), "copy triangles")) return e;
+    float* const boxes[6] = {bx, bx + nt, bx + 2 * (size_t)nt, bx + 3 * (size_t)nt, bx + 4 * (size_t)nt,
+                             bx + 5 * (size_t)nt};
+    nrg::triangle_boxes(c->stream, dv, dtri, nt, boxes);
+    dn = nullptr; dt = nullptr; di = nullptr;
+    if (int e = hip(hipMalloc((void**)&dn, sizeof(GNode) * (size_t)(nt - 1)), "hipMalloc nodes")) return e;
+    c->scene_mem.push_back(dn);
+    if (int e = hip(hipMalloc((void**)&dt, sizeof(GTri) * (size_t)nt), "hipMalloc tris")) return e;
+    c->scene_mem.push_back(dt);
+    if (int e = hip(hipMalloc((void**)&di, sizeof(uint32_t) * 3 * (size_t)nt), "hipMalloc idx")) return e;
+    c->scene_mem.push_back(di);
+    nrg::Result r;
+    std::string err;
+    const float* const cboxes[6] = {boxes[0], boxes[1], boxes[2], boxes[3], boxes[4], boxes[5]};
+    int e = nrg::build(c->stream, nt, cboxes, dn, dord, r, err);
+    if (e != NR_OK) {
+        release();
+        return fail(e, "device tree build: " + err);
+    }
+    nrg::gather_triangles(c->stream, dv, dtri, dord, nt, dt, di);
+    if (int e2 = hip(hipGetLastError(), "gather triangles")) return e2;
+    if (int e2 = hip(hipStreamSynchronize(c->stream), "device tree build")) return e2;
+    release();
+    gm.root_ref = r.root_ref;
+    for (int a = 0; a < 3; a++) { gm.root_min[a] = r.root_lo[a]; gm.root_max[a] = r.root_hi[a]; }
+    gm.depth = r.depth;
+    n_nodes = nt - 1;
     return NR_OK;
 }
 
@@ -1082,6 +1147,7 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
     }
     // meshes
     std::vector<GMesh> meshes(d->n_meshes);
+    std::vector<int32_t> mesh_nodes(d->n_meshes, 0);
     std::vector<nrb::Box> mesh_local(d->n_meshes);
     int max_mesh_depth = 0;
     for (int mi = 0; mi < d->n_meshes; mi++) {
@@ -1089,39 +1155,59 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
         if (md.n_vertices < 0 || md.n_triangles < 0) return fail(NR_ERR_INVALID, "bad mesh sizes");
         for (int64_t k = 0; k < 3 * (int64_t)md.n_triangles; k++)
             if (md.triangles[k] >= (uint32_t)md.n_vertices) return fail(NR_ERR_INVALID, "triangle index out of range");
-        std::vector<nrb::Box> tb(md.n_triangles);
-        for (int t = 0; t < md.n_triangles; t++) {                             // mesh.h:320-336
-            const float* a = md.vertices + 11 * (size_t)md.triangles[3 * t];
-            const float* b = md.vertices + 11 * (size_t)md.triangles[3 * t + 1];
-            const float* cc = md.vertices + 11 * (size_t)md.triangles[3 * t + 2];
-            float mn[3], mx[3];
-            for (int k = 0; k < 3; k++) {
-                mn[k] = fminf(a[k], fminf(b[k], cc[k]));
-                mx[k] = fmaxf(a[k], fmaxf(b[k], cc[k]));
-            }
-            tb[t] = nrb::box_points(mn, mx);
-        }
-        nrb::Tree T;
-        nrb::Builder().build(tb, T);
-        std::vector<GNode> nodes;
+        const bool on_device = md.n_triangles > 1 &&
+                               (c->build_mode == NR_BUILD_DEVICE ||
+                                (c->build_mode == NR_BUILD_AUTO && md.n_triangles >= 4096));
         GMesh gm{};
-        nrb::to_pairs(T, nodes, gm.root_ref, gm.root_min, gm.root_max);
-        gm.depth = T.depth;
-        max_mesh_depth = std::max(max_mesh_depth, T.depth);
-        std::vector<GTri> tris(md.n_triangles);
-        std::vector<uint32_t> idx(3 * (size_t)md.n_triangles);
-        for (int s = 0; s < md.n_triangles; s++) {
-            int src = T.order[s];
-            GTri& g = tris[s];
-            for (int k = 0; k < 3; k++) {
-                uint32_t vi = md.triangles[3 * src + k];
-                idx[3 * (size_t)s + k] = vi;
-                float* dst = k == 0 ? g.v0 : (k == 1 ? g.v1 : g.v2);
-                for (int a = 0; a < 3; a++) dst[a] = md.vertices[11 * (size_t)vi + a];
-            }
-            g.orig = src;
-            g.pad0 = g.pad1 = 0;
+        int32_t n_nodes = 0;
+        GNode* dn = nullptr;
+        GTri* dt = nullptr;
+        uint32_t* di = nullptr;
+        bool built = false;
+        if (on_device) {
+            int e = build_mesh_on_device(c, md, gm, dn, dt, di, n_nodes);
+            if (e == NR_OK) built = true;
+            else if (e != NR_ERR_LIMIT) return e;        // a degenerate cluster the device path refuses: host build
         }
+        if (!built) {
+            std::vector<nrb::Box> tb(md.n_triangles);
+            for (int t = 0; t < md.n_triangles; t++) {                         // mesh.h:320-336
+                const float* a = md.vertices + 11 * (size_t)md.triangles[3 * t];
+                const float* b = md.vertices + 11 * (size_t)md.triangles[3 * t + 1];
+                const float* cc = md.vertices + 11 * (size_t)md.triangles[3 * t + 2];
+                float mn[3], mx[3];
+                for (int k = 0; k < 3; k++) {
+                    mn[k] = fminf(a[k], fminf(b[k], cc[k]));
+                    mx[k] = fmaxf(a[k], fmaxf(b[k], cc[k]));
+                }
+                tb[t] = nrb::box_points(mn, mx);
+            }
+            nrb::Tree T;
+            nrb::Builder().build(tb, T);
+            std::vector<GNode> nodes;
+            nrb::to_pairs(T, nodes, gm.root_ref, gm.root_min, gm.root_max);
+            gm.depth = T.depth;
+            std::vector<GTri> tris(md.n_triangles);
+            std::vector<uint32_t> idx(3 * (size_t)md.n_triangles);
+            for (int s = 0; s < md.n_triangles; s++) {
+                int src = T.order[s];
+                GTri& g = tris[s];
+                for (int k = 0; k < 3; k++) {
+                    uint32_t vi = md.triangles[3 * src + k];
+                    idx[3 * (size_t)s + k] = vi;
+                    float* dst = k == 0 ? g.v0 : (k == 1 ? g.v1 : g.v2);
+                    for (int a = 0; a < 3; a++) dst[a] = md.vertices[11 * (size_t)vi + a];
+                }
+                g.orig = src;
+                g.pad0 = g.pad1 = 0;
+            }
+            int e = upload(c, nodes.data(), nodes.size(), &dn);
+            if (!e) e = upload(c, tris.data(), tris.size(), &dt);
+            if (!e) e = upload(c, idx.data(), idx.size(), &di);
+            if (e) return e;
+            n_nodes = (int32_t)nodes.size();
+        }
+        max_mesh_depth = std::max(max_mesh_depth, gm.depth);
         std::vector<GVertex> verts(md.n_vertices);
         float mn[3] = {1e32f, 1e32f, 1e32f}, mx[3] = {-1e32f, -1e32f, -1e32f};
         for (int v = 0; v < md.n_vertices; v++) {
@@ -1134,19 +1220,17 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
         }
         if (md.n_vertices > 0) mesh_local[mi] = nrb::box_points(mn, mx);        // mesh.h:218-237
         else mesh_local[mi] = nrb::box_none();
-        GNode* dn; GTri* dt; uint32_t* di; GVertex* dv;
-        int e = upload(c, nodes.data(), nodes.size(), &dn);
-        if (!e) e = upload(c, tris.data(), tris.size(), &dt);
-        if (!e) e = upload(c, idx.data(), idx.size(), &di);
-        if (!e) e = upload(c, verts.data(), verts.size(), &dv);
+        GVertex* dv;
+        int e = upload(c, verts.data(), verts.size(), &dv);
         if (e) return e;
         gm.nodes = dn; gm.tris = dt; gm.idx = di; gm.verts = dv;
         meshes[mi] = gm;
+        mesh_nodes[mi] = n_nodes;
     }
     // objects: registration order -> world boxes -> top-level tree
     int n = d->n_objects;
     std::vector<GObject> reg(n);
-    std::vector<nrb::Box> wb(n);
+    std::vector<nrb::Box> wb(n), local(n);
     for (int i = 0; i < n; i++) {
         const nr_object_desc& od = d->objects[i];
         if (od.type < 1 || od.type > 6) return fail(NR_ERR_INVALID, "bad hittable type");
@@ -1172,6 +1256,7 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
         if (geom == NR_HIT_QUAD || geom == NR_HIT_CUBE) { float a[3] = {-0.5f, -0.5f, -0.5f}, b[3] = {0.5f, 0.5f, 0.5f}; lb = nrb::box_points(a, b); }
         else if (geom == NR_HIT_SPHERE) { float a[3] = {-o.radius, -o.radius, -o.radius}, b[3] = {o.radius, o.radius, o.radius}; lb = nrb::box_points(a, b); }
         else lb = mesh_local[od.mesh];
+        local[i] = lb;
         wb[i] = host::transformed(lb, o.xf);
     }
     nrb::Tree T;
@@ -1228,6 +1313,11 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
     HIPCK(hipStreamSynchronize(c->stream));
     c->scene = S;
     c->has_scene = true;
+    c->h_meshes = meshes;
+    c->h_mesh_nodes = mesh_nodes;
+    c->reg_objects = reg;
+    c->reg_local = local;
+    c->max_mesh_depth = max_mesh_depth;
     return NR_OK;
 }
 
@@ -1784,6 +1874,71 @@ int nr_readback(nr_ctx* c, int view, const nr_readback_desc* d, uint8_t* host) {
     HIPCK(hipMemcpyAsync(v->h_rb, v->d_rb_u8, 3 * HW, hipMemcpyDeviceToHost, c->rb_stream));
     HIPCK(hipStreamSynchronize(c->rb_stream));
     std::memcpy(host, v->h_rb, 3 * HW);
+    return NR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// scene trees and live edits
+// ---------------------------------------------------------------------------
+int nr_set_build_mode(nr_ctx* c, int mode) {
+    if (!c) return fail(NR_ERR_INVALID, "null context");
+    if (mode < NR_BUILD_HOST || mode > NR_BUILD_AUTO) return fail(NR_ERR_INVALID, "bad build mode");
+    c->build_mode = mode;
+    return NR_OK;
+}
+
+int nr_mesh_tree(nr_ctx* c, int mesh, void* nodes64, int32_t* slot_orig, int32_t* n_nodes, int32_t* n_tris,
+                 int32_t* depth) {
+    if (!c) return fail(NR_ERR_INVALID, "null context");
+    if (!c->has_scene) return fail(NR_ERR_STATE, "no scene");
+    if (mesh < 0 || mesh >= (int)c->h_meshes.size()) return fail(NR_ERR_INVALID, "mesh index out of range");
+    const GMesh& m = c->h_meshes[mesh];
+    const int32_t nn = c->h_mesh_nodes[mesh];
+    const int32_t nt = nn + 1;
+    if (n_nodes) *n_nodes = nn;
+    if (n_tris) *n_tris = nt;
+    if (depth) *depth = m.depth;
+    HIPCK(hipSetDevice(c->device));
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (nodes64 && nn > 0) HIPCK(hipMemcpy(nodes64, m.nodes, sizeof(GNode) * (size_t)nn, hipMemcpyDeviceToHost));
+    if (slot_orig) {
+        std::vector<GTri> t(nt);
+        HIPCK(hipMemcpy(t.data(), m.tris, sizeof(GTri) * (size_t)nt, hipMemcpyDeviceToHost));
+        for (int32_t k = 0; k < nt; k++) slot_orig[k] = t[k].orig;
+    }
+    return NR_OK;
+}
+
+int nr_object_set_transform(nr_ctx* c, int32_t index, const float position[3], const float rotation[3],
+                            const float scale[3]) {
+    if (!c || !position || !rotation || !scale) return fail(NR_ERR_INVALID, "null argument");
+    if (!c->has_scene) return fail(NR_ERR_STATE, "no scene");
+    const int32_t n = (int32_t)c->reg_objects.size();
+    if (index < 0 || index >= n) return fail(NR_ERR_INVALID, "object index out of range");
+    HIPCK(hipSetDevice(c->device));
+    GObject& o = c->reg_objects[index];
+    const int geom = (o.type == NR_HIT_CONSTANT_MEDIUM || o.type == NR_HIT_OBJECT_LIGHT) ? o.bound_type : o.type;
+    o.xf = host::xform(position, rotation, scale, geom == NR_HIT_SPHERE);
+    std::vector<nrb::Box> wb(n);
+    for (int32_t i = 0; i < n; i++) wb[i] = host::transformed(c->reg_local[i], c->reg_objects[i].xf);
+    nrb::Tree T;
+    nrb::Builder().build(wb, T);
+    std::vector<GNode> tnodes;
+    GScene S = c->scene;
+    nrb::to_pairs(T, tnodes, S.root_ref, S.root_min, S.root_max);
+    std::vector<GObject> objs(n);
+    S.first_light = -1;
+    for (int32_t k = 0; k < n; k++) {
+        objs[k] = c->reg_objects[T.order[k]];
+        if (S.first_light < 0 && objs[k].type == NR_HIT_OBJECT_LIGHT) S.first_light = k;   // registry.h:240-244
+    }
+    S.stack_depth = T.depth + c->max_mesh_depth + 4;
+    // same sizes as the full upload: rewrite the tables in place once the queued work is done
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (!tnodes.empty()) HIPCK(hipMemcpy((void*)S.nodes, tnodes.data(), sizeof(GNode) * tnodes.size(), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy((void*)S.objects, objs.data(), sizeof(GObject) * objs.size(), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(c->d_scene, &S, sizeof(GScene), hipMemcpyHostToDevice));
+    c->scene = S;
     return NR_OK;
 }
 

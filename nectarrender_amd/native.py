@@ -55,12 +55,18 @@ def lib() -> C.CDLL:
         "nr_selection_mask": (C.c_int, [vp, C.c_int, C.c_int32, C.c_int32, vp]),
         "nr_overlay_device": (C.c_int, [vp, vp, C.c_int32, C.c_int32, vp, C.POINTER(C.c_float)]),
         "nr_readback": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrReadbackDesc), C.POINTER(C.c_uint8)]),
+        "nr_view_mask_buffer": (vp, [vp, C.c_int]),
+        "nr_set_build_mode": (C.c_int, [vp, C.c_int]),
+        "nr_mesh_tree": (C.c_int, [vp, C.c_int, vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+        "nr_object_set_transform": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                              C.POINTER(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
-    if L.nr_abi_version() != 3:
+    if L.nr_abi_version() != 4:
         raise NectarError("ABI version mismatch")
     _lib = L
     return L
@@ -179,6 +185,25 @@ class Context:
         out = np.empty((3, H, W), np.uint8)
         _check(lib().nr_readback(self._h, view, C.byref(d), out.ctypes.data_as(C.POINTER(C.c_uint8))))
         return out
+
+    # scene trees and live edits -------------------------------------------
+    def set_build_mode(self, mode: int):
+        """Mesh tree builder of later uploads: abi.BUILD_HOST, BUILD_DEVICE or BUILD_AUTO."""
+        _check(lib().nr_set_build_mode(self._h, mode))
+
+    def mesh_tree(self, mesh: int):
+        """(nodes as abi.NODE_DTYPE, slot -> original triangle, depth) of an uploaded mesh."""
+        nn, nt, dep = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().nr_mesh_tree(self._h, mesh, None, None, C.byref(nn), C.byref(nt), C.byref(dep)))
+        nodes = np.zeros(nn.value, abi.NODE_DTYPE)
+        orig = np.zeros(nt.value, np.int32)
+        _check(lib().nr_mesh_tree(self._h, mesh, nodes.ctypes.data_as(C.c_void_p),
+                                  orig.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(nn), C.byref(nt), C.byref(dep)))
+        return nodes, orig, dep.value
+
+    def set_object_transform(self, index: int, position, rotation, scale):
+        f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])
+        _check(lib().nr_object_set_transform(self._h, index, f3(position), f3(rotation), f3(scale)))
 
     def set_timing(self, on: bool):
         _check(lib().nr_set_timing(self._h, int(on)))

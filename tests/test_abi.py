@@ -23,7 +23,8 @@ def test_header_declares_entry_points():
     for n in ("nr_create", "nr_destroy", "nr_scene_upload", "nr_camera_set", "nr_render", "nr_trace_closest",
               "nr_aov_read", "nr_post", "nr_last_error", "nr_view_size", "nr_unet_create", "nr_unet_forward",
               "nr_unet_train_step", "nr_unet_gradients", "nr_denoise", "nr_viewport", "nr_pick",
-              "nr_selection_mask", "nr_overlay_device", "nr_readback"):
+              "nr_selection_mask", "nr_overlay_device", "nr_readback", "nr_set_build_mode", "nr_mesh_tree",
+              "nr_object_set_transform", "nr_view_mask_buffer"):
         assert n in names
 
 
@@ -37,7 +38,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_loads_and_reports_version():
     L = C.CDLL(LIB)
-    assert L.nr_abi_version() == 3
+    assert L.nr_abi_version() == 4
     L.nr_last_error.restype = C.c_char_p
     assert isinstance(L.nr_last_error(), bytes)
 
@@ -58,6 +59,10 @@ def test_null_arguments_fail_cleanly_without_device():
     assert L.nr_selection_mask(None, 0, 0, 3, None) == 1
     assert L.nr_overlay_device(None, None, 4, 4, None, None) == 1
     assert L.nr_readback(None, 0, None, None) == 1
+    assert not L.nr_view_mask_buffer(None, 0)
+    assert L.nr_set_build_mode(None, 1) == 1
+    assert L.nr_mesh_tree(None, 0, None, None, None, None, None) == 1
+    assert L.nr_object_set_transform(None, 0, None, None, None) == 1
     from nectarrender_amd import denoise
     D = denoise._lib()
     assert D.nr_unet_param_count() == 1016131
