@@ -127,7 +127,14 @@ def main():
 
     scene, cam, rp = scenes.config2(spp=max(64, K + 1, Wm + 1))
     ctx = native.Context(0)
-    ctx.upload(scene)
+    # scene upload (outside the Mpaths/s region): mesh tree on the host, then on the GPU (same tree)
+    build = {}
+    for name, mode in (("host_tree_s", abi.BUILD_HOST), ("device_tree_s", abi.BUILD_DEVICE)):
+        ctx.set_build_mode(mode)
+        tb = time.perf_counter()
+        ctx.upload(scene)
+        ctx.sync()
+        build[name] = round(time.perf_counter() - tb, 4)
     ctx.set_camera(0, cam, rp["seed"])
     # warmup passes, then a clean accumulation for the timed passes
     if Wm > 0:
@@ -166,6 +173,7 @@ def main():
         "gpu_ms": {"total": round(tm[0], 3), "traversal": round(tm[1], 3), "shading": round(tm[2], 3),
                    "generate_accumulate": round(tm[3], 3)},
         "image_mean": float(img.mean()),
+        "scene_upload": dict(build, triangles=1_003_520),
         "roofline": roof,
     }
     if not args.no_quality:
