@@ -22,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -103,6 +105,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-quality", action="store_true", help="skip the seconds-to-RMSE measurement")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="stream groups per render (1..4; default: the library's default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (a kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -136,21 +140,31 @@ def main():
         ctx.sync()
         build[name] = round(time.perf_counter() - tb, 4)
     ctx.set_camera(0, cam, rp["seed"])
+    if args.streams:
+        ctx.set_render_streams(args.streams)
     # warmup passes, then a clean accumulation for the timed passes
     if Wm > 0:
         ctx.render(0, abi.render_desc(1, 1 + Wm, rp["depth"], rp["seed"]))
     ctx.sync()
     ctx.reset(0)
-    ctx.set_timing(True)
     ctx.sync()
     t0 = time.perf_counter()
     ctx.render(0, abi.render_desc(1, 1 + K, rp["depth"], rp["seed"]))
     ctx.sync()
     t1 = time.perf_counter()
+    img = ctx.read(0)
+    # the same passes again with per-kernel HIP events (one stream group, so each
+    # kernel's time is its own) for the breakdown and the k_extend roofline
+    ctx.reset(0)
+    ctx.set_timing(True)
+    ctx.render(0, abi.render_desc(1, 1 + K, rp["depth"], rp["seed"]))
+    ctx.sync()
     tm = ctx.timings()       # [total, traversal, shading, generate+accumulate] ms (HIP events), iterations
     ctx.set_timing(False)
-    img = ctx.read(0)
+    img_t = ctx.read(0)
     ctx.close()
+    if not np.array_equal(img, img_t):
+        raise RuntimeError("the timed and the event-timed renders differ")
     dt = t1 - t0
     paths = K * PATHS_PER_STEP
     value = paths / dt / 1e6
@@ -171,7 +185,8 @@ def main():
                                "parity estimator, one path per pixel per step",
                    "spp_passes": K, "ray_depth": rp["depth"], "paths_per_step": PATHS_PER_STEP},
         "gpu_ms": {"total": round(tm[0], 3), "traversal": round(tm[1], 3), "shading": round(tm[2], 3),
-                   "generate_accumulate": round(tm[3], 3)},
+                   "generate_accumulate": round(tm[3], 3), "note": "event-timed rerun of the same passes on "
+                   "one stream group; `value` is the default two stream groups"},
         "image_mean": float(img.mean()),
         "scene_upload": dict(build, triangles=1_003_520),
         "roofline": roof,

@@ -324,6 +324,13 @@ int nr_readback(nr_ctx* ctx, int view, const nr_readback_desc* desc, uint8_t* ho
 
 /* ---- scene trees and live edits ---- */
 
+/* Number of stream groups nr_render splits a view into (1..4, default 2): bands of
+ * whole 8-pixel tile rows, each a complete wavefront pipeline on its own HIP stream,
+ * so one group's traversal tail overlaps the others' work.  The result does not
+ * depend on it (every pixel's samples are folded in order).  Timing mode
+ * (nr_set_timing) always renders with one group. */
+int nr_set_render_streams(nr_ctx* ctx, int n);
+
 /* Mesh tree builder for later nr_scene_upload calls (enum nr_build_mode). */
 int nr_set_build_mode(nr_ctx* ctx, int mode);
 

@@ -66,6 +66,10 @@ enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_N = 16 };
 #define NR_LDS_LEVELS 8
 #endif
 constexpr int LDS_STACK_LEVELS = NR_LDS_LEVELS;   // traversal stack levels kept in LDS
+#define NR_MAX_GROUPS 4
+#ifndef NR_RENDER_STREAMS
+#define NR_RENDER_STREAMS 2
+#endif
 // register budget of the traversal kernels: minimum waves per SIMD (0 = compiler's choice)
 #ifndef NR_TRAV_OCC
 #define NR_TRAV_OCC 0
@@ -109,6 +113,8 @@ struct TraceArgs {
     uint32_t rr_start;
     float rr_survival;
     uint32_t s_end;     // passes [s_begin, s_end) are rendered
+    uint32_t t0, t1;    // this stream group's range of the tiled pixel enumeration (tile_pixel)
+    uint32_t npix;      // pixels in the group
 };
 
 // ---------------------------------------------------------------------------
@@ -493,13 +499,16 @@ __global__ void __launch_bounds__(256) k_unpack_acc(const float4* __restrict__ a
 }
 
 // First sample of every pixel, queued in tile order (closed-form dense slot).
+// P.qact of a stream group starts at the group's first queue position q0 =
+// t0 / 64 tiles' worth of pixels (groups are whole bands of tiles).
 __global__ void __launch_bounds__(256) k_start(GCamera cam, PathPtrs P, uint32_t s, TraceArgs A) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, x, y;
-    if (t == 0) P.counts[CNT_ACT + 0] = (int32_t)(cam.W * cam.H);
-    if (!tile_pixel(cam, t, x, y)) return;
+    uint32_t t = A.t0 + blockIdx.x * blockDim.x + threadIdx.x, x, y;
+    if (t == A.t0) P.counts[CNT_ACT + 0] = (int32_t)A.npix;
+    if (t >= A.t1 || !tile_pixel(cam, t, x, y)) return;
     uint32_t band = y >> 3, bh = min(8u, cam.H - band * 8);
     uint32_t tx = x >> 3, tw = min(8u, cam.W - tx * 8);
-    uint32_t q = band * 8 * cam.W + tx * 8 * bh + (y & 7) * tw + (x & 7);
+    uint32_t band0 = (A.t0 >> 6) / ((cam.W + 7) >> 3);
+    uint32_t q = (band - band0) * 8 * cam.W + tx * 8 * bh + (y & 7) * tw + (x & 7);
     start_path(cam, P, x, y, s, A.seed, A.WH);
     P.finished[y * cam.W + x] = 0;
     P.qact[q] = (int32_t)(y * cam.W + x);
@@ -710,10 +719,10 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
 // pixels in tile order so the new primary rays of a wave are coherent.
 constexpr int REGEN_BLOCK = 1024;
 __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(GCamera cam, PathPtrs P, TraceArgs A) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, x = 0, y = 0;
+    uint32_t t = A.t0 + blockIdx.x * blockDim.x + threadIdx.x, x = 0, y = 0;
     bool restart = false;
     uint32_t i = 0;
-    if (tile_pixel(cam, t, x, y)) {
+    if (t < A.t1 && tile_pixel(cam, t, x, y)) {
         i = y * cam.W + x;
         if (P.finished[i]) {
             P.finished[i] = 0;
@@ -911,8 +920,15 @@ struct nr_ctx {
     int shade_grid = 0;                 // workgroups of 256 for k_shade (grid-stride)
     int2* spill = nullptr;              // traversal stack levels beyond LDS
     size_t spill_elems = 0;
-    int32_t* h_live = nullptr;          // pinned: live-path counts of the last two iterations
+    int32_t* h_live = nullptr;          // pinned: live-path counts of the last two iterations, per stream group
     hipEvent_t ev_count[2] = {nullptr, nullptr};
+    // stream groups: disjoint bands of pixels, each a complete wavefront pipeline on
+    // its own stream, so one group's traversal tail overlaps another's work
+    int render_streams = NR_RENDER_STREAMS;
+    hipStream_t gstream[NR_MAX_GROUPS] = {};
+    hipEvent_t gev[NR_MAX_GROUPS][2] = {};
+    hipEvent_t gev_done[NR_MAX_GROUPS] = {};
+    hipEvent_t ev_packed = nullptr;
     int64_t last_iterations = 0;
     // timing
     bool timing = false;
@@ -1067,7 +1083,19 @@ nr_ctx* nr_create(int device) {
         nr_destroy(c);
         return nullptr;
     }
-    if (hipHostMalloc((void**)&c->h_live, sizeof(int32_t) * 4, hipHostMallocDefault) != hipSuccess) {
+    for (int g = 0; g < NR_MAX_GROUPS; g++) {
+        bool ok = hipStreamCreateWithFlags(&c->gstream[g], hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&c->gev[g][0], hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&c->gev[g][1], hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&c->gev_done[g], hipEventDisableTiming) == hipSuccess;
+        if (!ok) { g_err = "stream group setup failed"; nr_destroy(c); return nullptr; }
+    }
+    if (hipEventCreateWithFlags(&c->ev_packed, hipEventDisableTiming) != hipSuccess) {
+        g_err = "hipEventCreate failed";
+        nr_destroy(c);
+        return nullptr;
+    }
+    if (hipHostMalloc((void**)&c->h_live, sizeof(int32_t) * 2 * NR_MAX_GROUPS, hipHostMallocDefault) != hipSuccess) {
         g_err = "hipHostMalloc failed";
         nr_destroy(c);
         return nullptr;
@@ -1081,6 +1109,7 @@ void nr_destroy(nr_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    for (auto st : c->gstream) if (st) (void)hipStreamSynchronize(st);
     free_list(c->scene_mem);
     free_list(c->path_mem);
     if (c->rb_stream) (void)hipStreamSynchronize(c->rb_stream);
@@ -1101,6 +1130,12 @@ void nr_destroy(nr_ctx* c) {
     if (c->spill) (void)hipFree(c->spill);
     if (c->h_live) (void)hipHostFree(c->h_live);
     if (c->ev_layers) (void)hipEventDestroy(c->ev_layers);
+    for (int g = 0; g < NR_MAX_GROUPS; g++) {
+        if (c->gstream[g]) { (void)hipStreamSynchronize(c->gstream[g]); (void)hipStreamDestroy(c->gstream[g]); }
+        for (auto& ev : c->gev[g]) if (ev) (void)hipEventDestroy(ev);
+        if (c->gev_done[g]) (void)hipEventDestroy(c->gev_done[g]);
+    }
+    if (c->ev_packed) (void)hipEventDestroy(c->ev_packed);
     if (c->ev_rb_done) (void)hipEventDestroy(c->ev_rb_done);
     if (c->rb_stream) (void)hipStreamDestroy(c->rb_stream);
     (void)hipStreamDestroy(c->stream);
@@ -1400,12 +1435,12 @@ static int ensure_paths(nr_ctx* c, uint32_t n) {
     if (!e) e = al((void**)&P.hit, sizeof(GHit) * n);
     if (!e) e = al((void**)&P.qact, sizeof(int32_t) * 2 * (size_t)n);
     if (!e) e = al((void**)&P.qmat, sizeof(int32_t) * NR_MAT_TYPES * (size_t)n);
-    if (!e) e = al((void**)&P.counts, sizeof(int32_t) * CNT_N);
+    if (!e) e = al((void**)&P.counts, sizeof(int32_t) * CNT_N * NR_MAX_GROUPS);
     if (!e) e = al((void**)&P.finished, sizeof(uint8_t) * (size_t)n);
     if (!e) e = al((void**)&P.acc, sizeof(float4) * 3 * (size_t)n);
     if (e) { free_list(c->path_mem); return e; }
     HIPCK(hipMemsetAsync(P.finished, 0, n, c->stream));
-    HIPCK(hipMemsetAsync(P.counts, 0, sizeof(int32_t) * CNT_N, c->stream));
+    HIPCK(hipMemsetAsync(P.counts, 0, sizeof(int32_t) * CNT_N * NR_MAX_GROUPS, c->stream));
     P.cap = n;
     c->path_cap = n;
     return NR_OK;
@@ -1433,12 +1468,8 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     uint32_t N = v.H * v.W;
     int e = ensure_paths(c, N);
     if (e) return e;
-    StackCfg SC;
-    e = stack_spill(c, N, SC);
-    if (e) return e;
     const size_t lds = trav_lds_bytes();
-    const uint32_t gen_threads = ((v.W + 7) / 8) * ((v.H + 7) / 8) * 64;
-    dim3 gGen((gen_threads + 255) / 256), gN((N + 255) / 256), gS(c->shade_grid);
+    dim3 gN((N + 255) / 256), gS(c->shade_grid);
     size_t ei = 0;
     float t_ext = 0.0f, t_shade = 0.0f, t_other = 0.0f;
     std::vector<std::pair<size_t, int>> marks;   // (event index of start, category)
@@ -1460,51 +1491,110 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
         for (uint32_t s = d->sample_begin; s < d->sample_end; s++)
             hipLaunchKernelGGL(k_fold_zero, gN, dim3(256), 0, c->stream, lb, ld, ln, N, s);
     } else if (d->sample_end > d->sample_begin) {
-        TraceArgs A{d->seed, (uint32_t)((uint64_t)v.W * v.H), 0u, d->ray_depth, d->do_rr, d->rr_start_bounce,
-                    d->rr_survival, d->sample_end};
+        // Stream groups: G bands of whole tile rows, each with its own queues,
+        // counters, stack-spill columns and stream.  Timing mode runs one group on
+        // the context stream so per-kernel times are not shared with another group.
+        const uint32_t tiles_x = (v.W + 7) / 8, bands = (v.H + 7) / 8;
+        const int G = c->timing ? 1 : (int)std::max(1u, std::min<uint32_t>((uint32_t)c->render_streams, bands));
+        struct Group {
+            PathPtrs P;
+            TraceArgs A;
+            StackCfg SC;
+            hipStream_t st;
+            int64_t it = 0, bound = 0;
+            bool done = false;
+        } grp[NR_MAX_GROUPS];
+        const size_t lv = c->scene.stack_depth > LDS_STACK_LEVELS ? (size_t)(c->scene.stack_depth - LDS_STACK_LEVELS) : 0;
+        {
+            StackCfg all;
+            e = stack_spill(c, N + 128u * NR_MAX_GROUPS, all);
+            if (e) return e;
+            size_t spill_off = 0;
+            for (int g = 0; g < G; g++) {
+                const uint32_t b0 = bands * g / G, b1 = bands * (g + 1) / G;
+                const uint32_t q0 = b0 * 8 * v.W, q1 = std::min(b1 * 8, v.H) * v.W;
+                Group& gr = grp[g];
+                gr.P = c->P;
+                gr.P.qact = c->P.qact + q0;
+                gr.P.qmat = c->P.qmat + q0;
+                gr.P.counts = c->P.counts + CNT_N * g;
+                gr.A = TraceArgs{d->seed, (uint32_t)((uint64_t)v.W * v.H), 0u, d->ray_depth, d->do_rr,
+                                 d->rr_start_bounce, d->rr_survival, d->sample_end,
+                                 b0 * tiles_x * 64, b1 * tiles_x * 64, q1 - q0};
+                const uint32_t cap_g = (gr.A.npix + 127) / 128 * 128;
+                gr.SC.spill = all.spill + spill_off;
+                gr.SC.stride = cap_g;
+                spill_off += lv * cap_g;
+                gr.st = G == 1 ? c->stream : c->gstream[g];
+                gr.bound = gr.A.npix;
+            }
+        }
         mark(2);
         hipLaunchKernelGGL(k_pack_acc, gN, dim3(256), 0, c->stream, lb, ld, ln, c->P.acc, N, c->P.cap);
-        hipLaunchKernelGGL(k_start, gGen, dim3(256), 0, c->stream, v.cam, c->P, d->sample_begin, A);
+        if (G > 1) {
+            HIPCK(hipEventRecord(c->ev_packed, c->stream));
+            for (int g = 0; g < G; g++) HIPCK(hipStreamWaitEvent(grp[g].st, c->ev_packed, 0));
+        }
+        for (int g = 0; g < G; g++) {
+            const uint32_t nt = grp[g].A.t1 - grp[g].A.t0;
+            hipLaunchKernelGGL(k_start, dim3((nt + 255) / 256), dim3(256), 0, grp[g].st, v.cam, grp[g].P,
+                               d->sample_begin, grp[g].A);
+        }
         // The live count never grows from one iteration to the next (a path
         // either continues, or ends and restarts at most once), so a count read
         // back two iterations late is an upper bound for the traversal grid and
-        // the GPU always has the next iteration queued.
+        // each group always has its next iteration queued.
         const int64_t max_iters = (int64_t)(d->sample_end - d->sample_begin) * d->ray_depth + 2;
-        int64_t it = 0;
-        int64_t bound = N;
-        while (true) {
-            if (it >= 2) {
-                HIPCK(hipEventSynchronize(c->ev_count[it & 1]));
-                bound = c->h_live[it & 1];
-                if (bound == 0) break;
+        int active = G;
+        while (active > 0) {
+            for (int g = 0; g < G; g++) {
+                Group& gr = grp[g];
+                if (gr.done) continue;
+                const int64_t it = gr.it;
+                int32_t* live = c->h_live + 2 * g;
+                if (it >= 2) {
+                    HIPCK(hipEventSynchronize(c->gev[g][it & 1]));
+                    gr.bound = live[it & 1];
+                    if (gr.bound == 0) { gr.done = true; active--; continue; }
+                }
+                if (it > max_iters) return fail(NR_ERR_STATE, "streaming loop did not drain");
+                PathPtrs& P = gr.P;
+                TraceArgs& A = gr.A;
+                hipStream_t st = gr.st;
+                hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, st, P.counts, (int32_t)A.in);
+                mark(0);
+                const dim3 gx((uint32_t)((gr.bound + 127) / 128));
+                if (c->obj_types & NR_TY_OTHER)
+                    hipLaunchKernelGGL(k_extend<NR_TY_ALL>, gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                else
+                    hipLaunchKernelGGL(k_extend<NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH>, gx, dim3(128), lds, st,
+                                       c->d_scene, P, A, gr.SC);
+                mark(1);
+                if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL(k_shade<NR_MAT_LAMBERTIAN>, gS, dim3(256), 0, st, c->d_scene, P, A);
+                if (c->mat_present[NR_MAT_PBR]) hipLaunchKernelGGL(k_shade<NR_MAT_PBR>, gS, dim3(256), 0, st, c->d_scene, P, A);
+                if (c->mat_present[NR_MAT_DIELECTRIC]) hipLaunchKernelGGL(k_shade<NR_MAT_DIELECTRIC>, gS, dim3(256), 0, st, c->d_scene, P, A);
+                if (c->mat_present[NR_MAT_EMISSIVE]) hipLaunchKernelGGL(k_shade<NR_MAT_EMISSIVE>, gS, dim3(256), 0, st, c->d_scene, P, A);
+                if (c->mat_present[NR_MAT_ISOTROPIC]) hipLaunchKernelGGL(k_shade<NR_MAT_ISOTROPIC>, gS, dim3(256), 0, st, c->d_scene, P, A);
+                mark(2);
+                const uint32_t nt = A.t1 - A.t0;
+                hipLaunchKernelGGL(k_regen, dim3((nt + REGEN_BLOCK - 1) / REGEN_BLOCK), dim3(REGEN_BLOCK), 0, st,
+                                   v.cam, P, A);
+                HIPCK(hipGetLastError());
+                HIPCK(hipMemcpyAsync(live + (it & 1), P.counts + CNT_ACT + (1 - A.in), sizeof(int32_t),
+                                     hipMemcpyDeviceToHost, st));
+                HIPCK(hipEventRecord(c->gev[g][it & 1], st));
+                A.in = 1 - A.in;
+                gr.it++;
             }
-            if (it > max_iters) return fail(NR_ERR_STATE, "streaming loop did not drain");
-            hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, c->stream, c->P.counts, (int32_t)A.in);
-            mark(0);
-            if (c->obj_types & NR_TY_OTHER)
-                hipLaunchKernelGGL(k_extend<NR_TY_ALL>, dim3((uint32_t)((bound + 127) / 128)), dim3(128), lds,
-                                   c->stream, c->d_scene, c->P, A, SC);
-            else
-                hipLaunchKernelGGL(k_extend<NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH>,
-                                   dim3((uint32_t)((bound + 127) / 128)), dim3(128), lds, c->stream, c->d_scene, c->P,
-                                   A, SC);
-            mark(1);
-            if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL(k_shade<NR_MAT_LAMBERTIAN>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
-            if (c->mat_present[NR_MAT_PBR]) hipLaunchKernelGGL(k_shade<NR_MAT_PBR>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
-            if (c->mat_present[NR_MAT_DIELECTRIC]) hipLaunchKernelGGL(k_shade<NR_MAT_DIELECTRIC>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
-            if (c->mat_present[NR_MAT_EMISSIVE]) hipLaunchKernelGGL(k_shade<NR_MAT_EMISSIVE>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
-            if (c->mat_present[NR_MAT_ISOTROPIC]) hipLaunchKernelGGL(k_shade<NR_MAT_ISOTROPIC>, gS, dim3(256), 0, c->stream, c->d_scene, c->P, A);
-            mark(2);
-            hipLaunchKernelGGL(k_regen, dim3((gen_threads + REGEN_BLOCK - 1) / REGEN_BLOCK), dim3(REGEN_BLOCK), 0,
-                               c->stream, v.cam, c->P, A);
-            HIPCK(hipGetLastError());
-            HIPCK(hipMemcpyAsync(c->h_live + (it & 1), c->P.counts + CNT_ACT + (1 - A.in), sizeof(int32_t),
-                                 hipMemcpyDeviceToHost, c->stream));
-            HIPCK(hipEventRecord(c->ev_count[it & 1], c->stream));
-            A.in = 1 - A.in;
-            it++;
         }
-        c->last_iterations = it;
+        c->last_iterations = 0;
+        for (int g = 0; g < G; g++) c->last_iterations += grp[g].it;
+        if (G > 1) {
+            for (int g = 0; g < G; g++) {
+                HIPCK(hipEventRecord(c->gev_done[g], grp[g].st));
+                HIPCK(hipStreamWaitEvent(c->stream, c->gev_done[g], 0));
+            }
+        }
         mark(2);
         LayerWrite w(c, v);
         hipLaunchKernelGGL(k_unpack_acc, gN, dim3(256), 0, c->stream, c->P.acc, lb, ld, ln, N, c->P.cap);
@@ -1880,6 +1970,13 @@ int nr_readback(nr_ctx* c, int view, const nr_readback_desc* d, uint8_t* host) {
 // ---------------------------------------------------------------------------
 // scene trees and live edits
 // ---------------------------------------------------------------------------
+int nr_set_render_streams(nr_ctx* c, int n) {
+    if (!c) return fail(NR_ERR_INVALID, "null context");
+    if (n < 1 || n > NR_MAX_GROUPS) return fail(NR_ERR_INVALID, "render streams outside 1..4");
+    c->render_streams = n;
+    return NR_OK;
+}
+
 int nr_set_build_mode(nr_ctx* c, int mode) {
     if (!c) return fail(NR_ERR_INVALID, "null context");
     if (mode < NR_BUILD_HOST || mode > NR_BUILD_AUTO) return fail(NR_ERR_INVALID, "bad build mode");

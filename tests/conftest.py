@@ -21,6 +21,10 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def ctx():
+    # torch ships its own HIP runtime: it must initialise the device before the
+    # library does, or GPU tests that use torch tensors later find no device
+    import torch
+    torch.cuda.init()
     from nectarrender_amd import native
     c = native.Context(0)
     yield c

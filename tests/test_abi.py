@@ -61,6 +61,7 @@ def test_null_arguments_fail_cleanly_without_device():
     assert L.nr_readback(None, 0, None, None) == 1
     assert not L.nr_view_mask_buffer(None, 0)
     assert L.nr_set_build_mode(None, 1) == 1
+    assert L.nr_set_render_streams(None, 2) == 1
     assert L.nr_mesh_tree(None, 0, None, None, None, None, None) == 1
     assert L.nr_object_set_transform(None, 0, None, None, None) == 1
     from nectarrender_amd import denoise

@@ -144,3 +144,18 @@ def test_render_parity_c3_views(ctx, oracle):
         for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
             _assert_same(g[k], o[k], f"view {v} {name_}")
     assert desc.ray_depth == 16
+
+
+@pytest.mark.parametrize("groups", [1, 3, 4])
+def test_render_same_for_any_stream_group_count(ctx, oracle, groups):
+    """Stream groups only change the interleaving of pixels' work across streams: the AOVs equal the
+    oracle's for 1, 3 and 4 groups (the default of 2 runs in every other render test), including a view
+    whose band count (11 tile rows) does not divide evenly."""
+    s, cam, rp = scenes.mixed_scene(spp=4, res=88)
+    ctx.set_render_streams(groups)
+    try:
+        g, o = _render_both(ctx, oracle, s, cam, rp, 1, 4)
+    finally:
+        ctx.set_render_streams(2)
+    for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
+        _assert_same(g[k], o[k], name_)
