@@ -36,9 +36,9 @@ PATHS_PER_STEP = W * H
 # objects (128 B), 40 B ray/hit per segment.
 TRAVERSAL_BYTES_PER_PATH = 1942.58
 # HBM bytes k_extend moves per path on this workload, from rocprofv3 FETCH_SIZE (x 1.99992, the factor
-# a 1 GiB device-to-device copy gives) + WRITE_SIZE (x 1.0) over 10,368,000 paths:
+# a 1 GiB device-to-device copy gives) + WRITE_SIZE (x 1.0) over 18,662,400 paths:
 # profiles/k_extend_c2_1080p.md.
-TRAVERSAL_HBM_BYTES_PER_PATH = 1294.5
+TRAVERSAL_HBM_BYTES_PER_PATH = 1265.3
 HBM_PEAK_GBS = 8000.0
 
 
