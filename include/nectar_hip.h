@@ -76,9 +76,14 @@ enum nr_layer {
 };
 
 /* Estimator. PARITY reproduces trace_full_kernel draw for draw
- * (csrc/engine/src/trace.cu:47-79).  FAST adds next-event estimation to the
- * HDRI (importance sampled) and ObjectLights with power-heuristic MIS; it is
- * checked by convergence, not per pixel. */
+ * (csrc/engine/src/trace.cu:47-79).  FAST (no counterpart in the reference)
+ * adds next-event estimation to an HDRI sky: at every non-specular vertex a
+ * direction drawn from the map's luminance x sin(theta) CDFs, weighted against
+ * the scattering technique (the reference's mixture with lights[0] included)
+ * by the power heuristic, with a shadow ray; sample s of pixel i draws from
+ * the 64-bit Philox subsequence (s << 32) | i.  ObjectLights keep the
+ * reference's one-sample mixture.  Same expectation as PARITY; checked by
+ * convergence (white furnace, z-test against PARITY), not per pixel. */
 enum nr_mode { NR_MODE_PARITY = 0, NR_MODE_FAST = 1 };
 
 /* Where nr_scene_upload builds mesh trees (BVH<T>::build, csrc/hittable/include/bvh/bvh.h:36-55,
@@ -354,8 +359,9 @@ int nr_sync(nr_ctx* ctx);
 
 /* Per-kernel GPU time (ms) of the last nr_render / nr_trace_closest call,
  * measured with HIP events on the context stream: [0] whole call,
- * [1] traversal kernels, [2] shading kernels, [3] generate+accumulate,
- * [4] streaming iterations (= traversal launches) of nr_render.
+ * [1] traversal kernels (k_extend), [2] shading kernels, [3] generate+accumulate,
+ * [4] streaming iterations (= k_extend launches) of nr_render, [5] shadow-ray
+ * kernels (NR_MODE_FAST with an HDRI sky; 0 otherwise).
  * Returns the number of values written. */
 int nr_last_timings(nr_ctx* ctx, float* ms, int n);
 

@@ -764,6 +764,7 @@ struct RenderEngine {
     std::atomic<EngineType> engine_type{EngineType::PATHTRACER};
     std::atomic<RenderMode> render_mode{RenderMode::FULL};
     std::atomic<bool> show_axis_grid{true};
+    std::atomic<bool> mis{false};      // NR_MODE_FAST (no counterpart in the reference)
     std::chrono::steady_clock::time_point last_poll{};
 
     std::function<void()> on_render_started = [] {}, on_render_finished = [] {}, on_stopped = [] {},
@@ -870,7 +871,7 @@ struct RenderEngine {
             d.do_rr = 1;
             d.rr_start_bounce = 3;
             d.rr_survival = render_mode.load() == RenderMode::INTERACTIVE ? 0.15f : 0.95f;
-            d.mode = NR_MODE_PARITY;
+            d.mode = mis.load() ? NR_MODE_FAST : NR_MODE_PARITY;
             {
                 std::lock_guard<std::mutex> g(ctx->mu);
                 std::shared_lock<std::shared_mutex> lay(ctx->layout_mu);
@@ -1738,6 +1739,10 @@ PYBIND11_MODULE(_pathtracer, m) {
         .def("max_depth", [](const RenderEngine& e) { return e.ray_depth; })
         .def("set_max_depth", &RenderEngine::set_max_depth)
         .def("set_axis_grid_visible", [](RenderEngine& e, bool v) { e.show_axis_grid = v; })
+        .def("set_mis", [](RenderEngine& e, bool v) { e.mis = v; }, py::arg("enabled"),
+             "Next-event estimation to an HDRI sky with power-heuristic MIS (NR_MODE_FAST) for later "
+             "passes; off (the reference's estimator) by default.  Not in the reference's API.")
+        .def("mis", [](const RenderEngine& e) { return e.mis.load(); })
         .def("sample_index", [](const RenderEngine& e) { return e.sample_idx; },
              "Index of the next sample pass (1 after a reset).")
         .def("viewport", [](RenderEngine& e) { py::gil_scoped_release r; (void)e.viewport(); },

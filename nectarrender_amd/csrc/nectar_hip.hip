@@ -59,9 +59,11 @@ static int fail(int code, const std::string& msg) { g_err = msg; return code; }
             return fail(NR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
     } while (0)
 
-// counters layout
-// counters (one int each): live queues [0,2), material queues [2,8), error flag
-enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_N = 16 };
+// counters (one int each): live queues [0,2), material queues [2,8), error flag,
+// shadow-ray queue (fast mode)
+enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_SHD = 9, CNT_N = 16 };
+// rows of the HDRI marginal CDF that k_shade stages in LDS (larger maps read it from global memory)
+constexpr int SKY_MARG_LDS = 4097;
 #ifndef NR_LDS_LEVELS
 #define NR_LDS_LEVELS 8
 #endif
@@ -101,6 +103,12 @@ struct PathPtrs {
     int32_t* counts;    // CNT_N
     uint8_t* finished;  // 1 = the pixel's path ended, fold it and start its next sample
     float4* acc;        // running means during a render: beauty [0, cap), diffuse [cap, 2 cap), normal [2 cap, 3 cap)
+    // fast mode: one pending shadow ray per path (origin, direction, MIS-weighted
+    // contribution added to L if the ray reaches the sky), queued in qshd
+    float4* sho;
+    float4* shd;
+    float4* shc;
+    int32_t* qshd;
     uint32_t cap;
 };
 
@@ -115,6 +123,7 @@ struct TraceArgs {
     uint32_t s_end;     // passes [s_begin, s_end) are rendered
     uint32_t t0, t1;    // this stream group's range of the tiled pixel enumeration (tile_pixel)
     uint32_t npix;      // pixels in the group
+    int32_t fast;       // NR_MODE_FAST: 64-bit sample subsequence (pixel, sample), NEE + MIS
 };
 
 // ---------------------------------------------------------------------------
@@ -450,14 +459,26 @@ __device__ __forceinline__ bool tile_pixel(const GCamera& cam, uint32_t t, uint3
     return x < cam.W && y < cam.H;
 }
 
+// Generator of pixel i's sample s: the reference's 32-bit subsequence i + s*W*H
+// (trace.cu:58), or in fast mode the 64-bit subsequence (s << 32) | i, which
+// never repeats across samples.
+__device__ __forceinline__ Gen path_gen(const TraceArgs& A, uint32_t i, uint32_t s, uint32_t draw) {
+    Gen g{A.seed, i + s * A.WH, draw};
+    if (A.fast) { g.subseq = i; g.subhi = s; }
+    return g;
+}
+
+// atten.w of a path: the solid-angle pdf of the direction it was scattered
+// into (fast mode's MIS weight at a sky hit), or -1 after the camera or a
+// specular bounce
 __device__ __forceinline__ void start_path(const GCamera& cam, PathPtrs& P, uint32_t x, uint32_t y, uint32_t s,
-                                           uint32_t seed, uint32_t WH) {
+                                           const TraceArgs& A) {
     uint32_t i = y * cam.W + x;
-    Gen g{seed, i + s * WH, 0u};                                                // trace.cu:58
-    Ray r = camera_ray(cam, x, y, i, s, seed, g);
+    Gen g = path_gen(A, i, s, 0u);
+    Ray r = camera_ray(cam, x, y, i, s, A.seed, g);
     P.o[i] = make_float4(r.o.x, r.o.y, r.o.z, __int_as_float(0));
     P.d[i] = make_float4(r.d.x, r.d.y, r.d.z, __int_as_float((int)s));
-    P.atten[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    P.atten[i] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
     P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.diff[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -509,7 +530,7 @@ __global__ void __launch_bounds__(256) k_start(GCamera cam, PathPtrs P, uint32_t
     uint32_t tx = x >> 3, tw = min(8u, cam.W - tx * 8);
     uint32_t band0 = (A.t0 >> 6) / ((cam.W + 7) >> 3);
     uint32_t q = (band - band0) * 8 * cam.W + tx * 8 * bh + (y & 7) * tw + (x & 7);
-    start_path(cam, P, x, y, s, A.seed, A.WH);
+    start_path(cam, P, x, y, s, A);
     P.finished[y * cam.W + x] = 0;
     P.qact[q] = (int32_t)(y * cam.W + x);
 }
@@ -519,9 +540,10 @@ __global__ void k_counts(int32_t* counts, int32_t in) {
     int t = threadIdx.x;
     if (t < NR_MAT_TYPES) counts[CNT_MAT + t] = 0;
     if (t == 0) counts[CNT_ACT + (1 - in)] = 0;
+    if (t == 1) counts[CNT_SHD] = 0;
 }
 
-template <int TY>
+template <int TY, bool FAST>
 __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_extend(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A, StackCfg SC) {
     extern __shared__ int2 smem[];
     const GScene& S = *Sp;
@@ -542,7 +564,11 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_extend(const GScene* __res
         if (of) atomicOr(&P.counts[CNT_ERR], 1);
         if (oi < 0) {                                                           // trace.cu:19-22
             float4 a4 = P.atten[path], l4 = P.L[path];
-            f3 L = mk3(l4.x, l4.y, l4.z) + mk3(a4.x, a4.y, a4.z) * sky_sample(S.sky, r.d);
+            f3 Le = sky_sample(S.sky, r.d);
+            // fast mode: a scattered ray that finds the sky shares it with the
+            // shadow rays of next-event estimation (power heuristic)
+            if (FAST && S.sky.marg && a4.w > 0.0f) Le = Le * power_heuristic(a4.w, sky_cdf_pdf(S.sky, r.d));
+            f3 L = mk3(l4.x, l4.y, l4.z) + mk3(a4.x, a4.y, a4.z) * Le;
             P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
             P.finished[path] = 1;
         } else {
@@ -595,15 +621,28 @@ __device__ __forceinline__ f3 mat_eval(const GScene& S, const GMaterial& m, f3 n
     }
 }
 
-template <int M>
+// Fast mode adds next-event estimation to the HDRI sky at every non-specular
+// vertex whose scattered ray would still be traced (bounce + 1 < depth): a
+// direction from the sky's CDFs (marginal staged in LDS), weighted against the
+// scattering technique with the power heuristic, queued as a shadow ray whose
+// contribution k_shadow adds if it reaches the sky.
+template <int M, bool FAST>
 __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A) {
     const GScene& S = *Sp;
     const int n = P.counts[CNT_MAT + M];
     const int out = 1 - A.in;
     const int stride = gridDim.x * blockDim.x;
+    constexpr bool NEE_M = FAST && (M == NR_MAT_LAMBERTIAN || M == NR_MAT_PBR || M == NR_MAT_ISOTROPIC);
+    __shared__ float smarg[NEE_M ? SKY_MARG_LDS : 1];
+    const float* marg = S.sky.marg;
+    if (NEE_M && marg && S.sky.H < SKY_MARG_LDS && (int)(blockIdx.x * blockDim.x) < n) {
+        for (int k = threadIdx.x; k <= S.sky.H; k += blockDim.x) smarg[k] = marg[k];
+        __syncthreads();
+        marg = smarg;
+    }
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {     // workgroup-uniform trip count
     int i = base + threadIdx.x;
-    bool cont = false;
+    bool cont = false, shadow = false;
     int path = -1;
     if (i < n) {
         path = P.qmat[(size_t)M * P.cap + i];
@@ -617,7 +656,7 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
         f3 p = ld3(h.p), nn = ld3(h.n), tg = ld3(h.tg);
         f2 uv; uv.x = h.uv[0]; uv.y = h.uv[1];
         bool front = h.front_face != 0;
-        Gen g{A.seed, (uint32_t)path + sample * A.WH, P.draw[path]};
+        Gen g = path_gen(A, (uint32_t)path, sample, P.draw[path]);
         const GMaterial& m = S.materials[h.material];
         // emission (material.h:182-188, 306-312)
         f3 em = s3(0.0f);
@@ -626,6 +665,7 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
         L = L + atten * em;
         bool alive = false;
         f3 nd = din;
+        float sc_pdf = -1.0f;                                                   // pdf of the scattered direction (-1: specular)
         if (M == NR_MAT_DIELECTRIC) {                                           // material.h:220-244
             float ri = front ? (1.0f / m.ior) : m.ior;
             f3 unit = normalize(din);
@@ -679,11 +719,36 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
                 else if (M == NR_MAT_PBR) { dir = pbr_pdf_generate(uvw, view, alpha, sp, g); pdf = pbr_pdf_value(uvw, view, alpha, sp, dir); }
                 else { dir = unit_vector(g); pdf = 1.0f / NR_PI4; }
             }
+            if (NEE_M && marg && bounce + 1 < A.depth) {
+                // drawn after the scattering sample: the scattered direction does
+                // not depend on whether the sky has CDFs
+                float u1 = g.uniform(), u2 = g.uniform(), ps;
+                f3 ld = sky_cdf_sample(S.sky, marg, u1, u2, ps);
+                if (ps > 0.0f) {
+                    f3 f = mat_eval<M>(S, m, nn, sn, uv, normalize(-din), ld);
+                    if (f.x > 0.0f || f.y > 0.0f || f.z > 0.0f) {
+                        // pdf of the scattering technique in direction ld
+                        float mp;
+                        if (M == NR_MAT_LAMBERTIAN) mp = cosine_pdf_value(uvw, ld);
+                        else if (M == NR_MAT_PBR) mp = pbr_pdf_value(uvw, view, alpha, sp, ld);
+                        else mp = 1.0f / NR_PI4;
+                        if (S.first_light >= 0) mp = 0.5f * mp + 0.5f * light_pdf(S.objects[S.first_light], p, ld);
+                        f3 c = atten * f * sky_sample(S.sky, ld) * (power_heuristic(ps, mp) / ps);
+                        if (c.x > 0.0f || c.y > 0.0f || c.z > 0.0f) {
+                            P.sho[path] = make_float4(p.x, p.y, p.z, 0.0f);
+                            P.shd[path] = make_float4(ld.x, ld.y, ld.z, 0.0f);
+                            P.shc[path] = make_float4(c.x, c.y, c.z, 0.0f);
+                            shadow = true;
+                        }
+                    }
+                }
+            }
             if (!(pdf <= 0.0f)) {
                 f3 brdf = mat_eval<M>(S, m, nn, sn, uv, normalize(-din), dir);
                 atten = atten * divs(brdf, pdf);
                 nd = dir;
                 alive = true;
+                sc_pdf = pdf;
             }
         }
         if (alive) {
@@ -701,7 +766,7 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
         }
         P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
         if (alive && bounce + 1 < A.depth) {
-            P.atten[path] = make_float4(atten.x, atten.y, atten.z, 0.0f);
+            P.atten[path] = make_float4(atten.x, atten.y, atten.z, FAST ? sc_pdf : 0.0f);
             P.o[path] = make_float4(p.x, p.y, p.z, __int_as_float((int)(bounce + 1)));
             P.d[path] = make_float4(nd.x, nd.y, nd.z, d4.w);
             P.draw[path] = g.draw;
@@ -712,6 +777,34 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
     }
     int slot = block_append<256>(cont, &P.counts[CNT_ACT + out]);
     if (cont) P.qact[(size_t)out * P.cap + slot] = path;
+    if (NEE_M) {
+        int ss = block_append<256>(shadow, &P.counts[CNT_SHD]);
+        if (shadow) P.qshd[ss] = path;
+    }
+    }
+}
+
+// Shadow rays of next-event estimation (fast mode): a ray that reaches the sky
+// (the visibility of scene_closest) adds its contribution to the path's
+// radiance.  Runs after every k_shade of the iteration and before k_regen.
+template <int TY>
+__global__ void __launch_bounds__(128) NR_TRAV_ATTR k_shadow(const GScene* __restrict__ Sp, PathPtrs P, StackCfg SC) {
+    extern __shared__ int2 smem[];
+    const GScene& S = *Sp;
+    Stack st = make_stack(smem, S, SC);
+    const int n = P.counts[CNT_SHD];
+    if ((int)(blockIdx.x * blockDim.x) >= n) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        int path = P.qshd[i];
+        float4 o4 = P.sho[path], d4 = P.shd[path];
+        bool of = false;
+        bool blocked = scene_occluded<TY>(S, mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), st, of);
+        if (of) atomicOr(&P.counts[CNT_ERR], 1);
+        if (!blocked && !of) {
+            float4 c = P.shc[path], l = P.L[path];
+            P.L[path] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+        }
     }
 }
 
@@ -733,7 +826,7 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(GCamera cam, PathPtrs P, 
             fold(P.acc + P.cap + i, P.diff[i], a, b);
             fold(P.acc + 2 * (size_t)P.cap + i, P.nrm[i], a, b);
             if (s + 1 < A.s_end) {
-                start_path(cam, P, x, y, s + 1, A.seed, A.WH);
+                start_path(cam, P, x, y, s + 1, A);
                 restart = true;
             }
         }
@@ -966,6 +1059,61 @@ static int upload(nr_ctx* c, const T* host, size_t n, T** dev) {
     return NR_OK;
 }
 
+// Importance-sampling tables of an equirect HDRI for fast mode.  The sky
+// lookup filters bilinearly, so the radiance anywhere inside texel (i, j)
+// mixes its 3 x 3 neighbourhood (u wraps, v clamps): the texel weighs the
+// largest channel mean (negative and non-finite values count as 0) of that
+// neighbourhood times sin(theta) at the row centre, theta = pi (j + 0.5) / H.
+// The density then bounds the radiance from above everywhere, including the
+// half-texel rim a bright feature bleeds into, which would otherwise be found
+// only by rare scattered rays.  cond holds one CDF per row (W + 1 entries,
+// 0 ... 1), marg the CDF over rows (H + 1 entries); sums run in double, the
+// ends are exactly 0 and 1, and a texel whose float CDF step is zero has
+// probability zero for both the sampler and the pdf lookup.  Returns false for
+// a map with no positive texel (no tables, no sky sampling).
+static bool sky_cdfs(const float* data, int H, int W, std::vector<float>& marg, std::vector<float>& cond) {
+    const size_t hw = (size_t)H * W;
+    marg.assign((size_t)H + 1, 0.0f);
+    cond.assign((size_t)H * (W + 1), 0.0f);
+    std::vector<double> mean(hw), rowmax(hw);
+    for (size_t k = 0; k < hw; k++) {
+        double v = 0.0;
+        for (int ch = 0; ch < 3; ch++) {
+            const double x = (double)data[ch * hw + k];
+            if (std::isfinite(x) && x > 0.0) v += x;
+        }
+        mean[k] = v / 3.0;
+    }
+    for (int j = 0; j < H; j++)                                  // max over i - 1 .. i + 1 (wrapping)
+        for (int i = 0; i < W; i++) {
+            const double* r = mean.data() + (size_t)j * W;
+            rowmax[(size_t)j * W + i] = std::max(r[i], std::max(r[(i + W - 1) % W], r[(i + 1) % W]));
+        }
+    std::vector<double> row_sum(H), cw((size_t)W + 1);
+    double total = 0.0;
+    for (int j = 0; j < H; j++) {
+        const double st = std::sin(M_PI * ((double)j + 0.5) / (double)H);
+        const double* a = rowmax.data() + (size_t)std::max(j - 1, 0) * W;   // max over j - 1 .. j + 1 (clamped)
+        const double* b = rowmax.data() + (size_t)j * W;
+        const double* c = rowmax.data() + (size_t)std::min(j + 1, H - 1) * W;
+        cw[0] = 0.0;
+        for (int i = 0; i < W; i++) cw[i + 1] = cw[i] + std::max(a[i], std::max(b[i], c[i])) * st;
+        float* cr = cond.data() + (size_t)j * (W + 1);
+        for (int i = 0; i < W; i++) cr[i] = cw[W] > 0.0 ? (float)(cw[i] / cw[W]) : (float)i / (float)W;
+        cr[W] = 1.0f;
+        row_sum[j] = cw[W];
+        total += cw[W];
+    }
+    if (!(total > 0.0) || !std::isfinite(total)) return false;
+    double acc = 0.0;
+    for (int j = 0; j < H; j++) {
+        marg[j] = (float)(acc / total);
+        acc += row_sum[j];
+    }
+    marg[H] = 1.0f;
+    return true;
+}
+
 static size_t trav_lds_bytes() { return (size_t)LDS_STACK_LEVELS * 64 * sizeof(int2) * (128 / 64); }
 
 // Spill columns for the traversal stack levels beyond LDS, one per ray of a
@@ -1040,6 +1188,16 @@ static int build_mesh_on_device(nr_ctx* c, const nr_mesh_desc& md, GMesh& gm, GN
     gm.depth = r.depth;
     n_nodes = nt - 1;
     return NR_OK;
+}
+
+// one k_shade per material type present in the scene
+template <bool FAST>
+static void launch_shade(nr_ctx* c, dim3 gS, hipStream_t st, const PathPtrs& P, const TraceArgs& A) {
+    if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL((k_shade<NR_MAT_LAMBERTIAN, FAST>), gS, dim3(256), 0, st, c->d_scene, P, A);
+    if (c->mat_present[NR_MAT_PBR]) hipLaunchKernelGGL((k_shade<NR_MAT_PBR, FAST>), gS, dim3(256), 0, st, c->d_scene, P, A);
+    if (c->mat_present[NR_MAT_DIELECTRIC]) hipLaunchKernelGGL((k_shade<NR_MAT_DIELECTRIC, FAST>), gS, dim3(256), 0, st, c->d_scene, P, A);
+    if (c->mat_present[NR_MAT_EMISSIVE]) hipLaunchKernelGGL((k_shade<NR_MAT_EMISSIVE, FAST>), gS, dim3(256), 0, st, c->d_scene, P, A);
+    if (c->mat_present[NR_MAT_ISOTROPIC]) hipLaunchKernelGGL((k_shade<NR_MAT_ISOTROPIC, FAST>), gS, dim3(256), 0, st, c->d_scene, P, A);
 }
 
 extern "C" {
@@ -1325,6 +1483,9 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
     }
     S.sky.intensity = d->sky.intensity;
     S.sky.data = nullptr;
+    S.sky.marg = nullptr;
+    S.sky.cond = nullptr;
+    std::vector<float> sky_marg, sky_cond;      // alive until the stream sync below
     if (d->sky.type == NR_SKY_HDRI && d->sky.data) {
         if (d->sky.C < 3 || d->sky.H <= 0 || d->sky.W <= 0) return fail(NR_ERR_INVALID, "HDRI needs C >= 3");
         S.sky.H = d->sky.H; S.sky.W = d->sky.W;
@@ -1333,6 +1494,14 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
         int e = upload(c, d->sky.data, (size_t)3 * d->sky.H * d->sky.W, &p);
         if (e) return e;
         S.sky.data = p;
+        if (sky_cdfs(d->sky.data, d->sky.H, d->sky.W, sky_marg, sky_cond)) {
+            float *pm, *pc;
+            e = upload(c, sky_marg.data(), sky_marg.size(), &pm);
+            if (!e) e = upload(c, sky_cond.data(), sky_cond.size(), &pc);
+            if (e) return e;
+            S.sky.marg = pm;
+            S.sky.cond = pc;
+        }
     }
     // cube face transforms (shapes.h:192-223)
     const float fp[6][3] = {{0, 0, 0.5f}, {0, 0, -0.5f}, {0.5f, 0, 0}, {-0.5f, 0, 0}, {0, 0.5f, 0}, {0, -0.5f, 0}};
@@ -1438,6 +1607,10 @@ static int ensure_paths(nr_ctx* c, uint32_t n) {
     if (!e) e = al((void**)&P.counts, sizeof(int32_t) * CNT_N * NR_MAX_GROUPS);
     if (!e) e = al((void**)&P.finished, sizeof(uint8_t) * (size_t)n);
     if (!e) e = al((void**)&P.acc, sizeof(float4) * 3 * (size_t)n);
+    if (!e) e = al((void**)&P.sho, sizeof(float4) * n);
+    if (!e) e = al((void**)&P.shd, sizeof(float4) * n);
+    if (!e) e = al((void**)&P.shc, sizeof(float4) * n);
+    if (!e) e = al((void**)&P.qshd, sizeof(int32_t) * (size_t)n);
     if (e) { free_list(c->path_mem); return e; }
     HIPCK(hipMemsetAsync(P.finished, 0, n, c->stream));
     HIPCK(hipMemsetAsync(P.counts, 0, sizeof(int32_t) * CNT_N * NR_MAX_GROUPS, c->stream));
@@ -1463,7 +1636,8 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     if (d->sample_end > v.cam.n_pattern) return fail(NR_ERR_INVALID, "sample index beyond samples_per_pixel");
     if (d->sample_begin < 1 && d->sample_end > d->sample_begin)
         return fail(NR_ERR_INVALID, "sample passes start at 1 (running mean of passes 1..n-1)");
-    if (d->mode != NR_MODE_PARITY) return fail(NR_ERR_INVALID, "only NR_MODE_PARITY is implemented");
+    if (d->mode != NR_MODE_PARITY && d->mode != NR_MODE_FAST) return fail(NR_ERR_INVALID, "unknown render mode");
+    const bool fast = d->mode == NR_MODE_FAST;
     HIPCK(hipSetDevice(c->device));
     uint32_t N = v.H * v.W;
     int e = ensure_paths(c, N);
@@ -1471,7 +1645,7 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     const size_t lds = trav_lds_bytes();
     dim3 gN((N + 255) / 256), gS(c->shade_grid);
     size_t ei = 0;
-    float t_ext = 0.0f, t_shade = 0.0f, t_other = 0.0f;
+    float t_ext = 0.0f, t_shade = 0.0f, t_other = 0.0f, t_shadow = 0.0f;
     std::vector<std::pair<size_t, int>> marks;   // (event index of start, category)
     auto mark = [&](int cat) {
         if (!c->timing) return;
@@ -1517,10 +1691,11 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
                 gr.P = c->P;
                 gr.P.qact = c->P.qact + q0;
                 gr.P.qmat = c->P.qmat + q0;
+                gr.P.qshd = c->P.qshd + q0;
                 gr.P.counts = c->P.counts + CNT_N * g;
                 gr.A = TraceArgs{d->seed, (uint32_t)((uint64_t)v.W * v.H), 0u, d->ray_depth, d->do_rr,
                                  d->rr_start_bounce, d->rr_survival, d->sample_end,
-                                 b0 * tiles_x * 64, b1 * tiles_x * 64, q1 - q0};
+                                 b0 * tiles_x * 64, b1 * tiles_x * 64, q1 - q0, fast ? 1 : 0};
                 const uint32_t cap_g = (gr.A.npix + 127) / 128 * 128;
                 gr.SC.spill = all.spill + spill_off;
                 gr.SC.stride = cap_g;
@@ -1564,17 +1739,23 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
                 hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, st, P.counts, (int32_t)A.in);
                 mark(0);
                 const dim3 gx((uint32_t)((gr.bound + 127) / 128));
-                if (c->obj_types & NR_TY_OTHER)
-                    hipLaunchKernelGGL(k_extend<NR_TY_ALL>, gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
-                else
-                    hipLaunchKernelGGL(k_extend<NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH>, gx, dim3(128), lds, st,
-                                       c->d_scene, P, A, gr.SC);
+                const bool other = (c->obj_types & NR_TY_OTHER) != 0;
+                constexpr int TYI = NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH;
+                if (fast) {
+                    if (other) hipLaunchKernelGGL((k_extend<NR_TY_ALL, true>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                    else hipLaunchKernelGGL((k_extend<TYI, true>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                } else {
+                    if (other) hipLaunchKernelGGL((k_extend<NR_TY_ALL, false>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                    else hipLaunchKernelGGL((k_extend<TYI, false>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                }
                 mark(1);
-                if (c->mat_present[NR_MAT_LAMBERTIAN]) hipLaunchKernelGGL(k_shade<NR_MAT_LAMBERTIAN>, gS, dim3(256), 0, st, c->d_scene, P, A);
-                if (c->mat_present[NR_MAT_PBR]) hipLaunchKernelGGL(k_shade<NR_MAT_PBR>, gS, dim3(256), 0, st, c->d_scene, P, A);
-                if (c->mat_present[NR_MAT_DIELECTRIC]) hipLaunchKernelGGL(k_shade<NR_MAT_DIELECTRIC>, gS, dim3(256), 0, st, c->d_scene, P, A);
-                if (c->mat_present[NR_MAT_EMISSIVE]) hipLaunchKernelGGL(k_shade<NR_MAT_EMISSIVE>, gS, dim3(256), 0, st, c->d_scene, P, A);
-                if (c->mat_present[NR_MAT_ISOTROPIC]) hipLaunchKernelGGL(k_shade<NR_MAT_ISOTROPIC>, gS, dim3(256), 0, st, c->d_scene, P, A);
+                if (fast) launch_shade<true>(c, gS, st, P, A);
+                else launch_shade<false>(c, gS, st, P, A);
+                if (fast && c->scene.sky.marg) {
+                    mark(4);
+                    if (other) hipLaunchKernelGGL(k_shadow<NR_TY_ALL>, gx, dim3(128), lds, st, c->d_scene, P, gr.SC);
+                    else hipLaunchKernelGGL(k_shadow<TYI>, gx, dim3(128), lds, st, c->d_scene, P, gr.SC);
+                }
                 mark(2);
                 const uint32_t nt = A.t1 - A.t0;
                 hipLaunchKernelGGL(k_regen, dim3((nt + REGEN_BLOCK - 1) / REGEN_BLOCK), dim3(REGEN_BLOCK), 0, st,
@@ -1609,13 +1790,15 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
             (void)hipEventElapsedTime(&ms, c->ev[marks[k].first], c->ev[marks[k + 1].first]);
             if (marks[k].second == 0) t_ext += ms;
             else if (marks[k].second == 1) t_shade += ms;
+            else if (marks[k].second == 4) t_shadow += ms;
             else t_other += ms;
         }
         float tot = 0.0f;
         (void)hipEventElapsedTime(&tot, e0, e1);
         c->last_ms[0] = tot; c->last_ms[1] = t_ext; c->last_ms[2] = t_shade; c->last_ms[3] = t_other;
         c->last_ms[4] = (float)c->last_iterations;
-        c->n_last = 5;
+        c->last_ms[5] = t_shadow;
+        c->n_last = 6;
         int32_t err = 0;
         HIPCK(hipMemcpy(&err, c->P.counts + CNT_ERR, sizeof(int32_t), hipMemcpyDeviceToHost));
         if (err) return fail(NR_ERR_LIMIT, "traversal stack overflow");

@@ -94,8 +94,8 @@ NR_HD float box_t(const float* mn, const float* mx, const Ray& r, float tmin, fl
 // key = seed, counter = (draw/4, 0, subsequence, 0), word draw%4,
 // u = 2^-32 + word * 2^-32 in (0, 1].
 // ---------------------------------------------------------------------------
-NR_D uint32_t philox_word(uint32_t seed, uint32_t subseq, uint32_t draw) {
-    uint32_t c0 = draw >> 2, c1 = 0u, c2 = subseq, c3 = 0u;
+NR_D uint32_t philox_word(uint32_t seed, uint32_t subseq, uint32_t draw, uint32_t subhi = 0u) {
+    uint32_t c0 = draw >> 2, c1 = 0u, c2 = subseq, c3 = subhi;
     uint32_t k0 = seed, k1 = 0u;
 #pragma unroll
     for (int r = 0; r < 10; r++) {
@@ -109,10 +109,12 @@ NR_D uint32_t philox_word(uint32_t seed, uint32_t subseq, uint32_t draw) {
     return w == 0 ? c0 : (w == 1 ? c1 : (w == 2 ? c2 : c3));
 }
 
+// subhi: high word of a 64-bit subsequence (fast mode); the parity estimator
+// keeps the reference's 32-bit subsequence (subhi = 0)
 struct Gen {
-    uint32_t seed, subseq, draw;
+    uint32_t seed, subseq, draw, subhi = 0u;
     NR_D float uniform() {
-        uint32_t w = philox_word(seed, subseq, draw);
+        uint32_t w = philox_word(seed, subseq, draw, subhi);
         draw += 1u;
         const float inv = 2.3283064e-10f;
         return inv + ((float)w * inv);
@@ -372,11 +374,13 @@ NR_D void load_node(const GNode* nodes, int32_t i, float* lmn, float* lmx, float
 // lanes of the wave (or all lanes still working) hold one, then the wave runs
 // the triangle test once for all of them.  Each lane still does its own steps
 // in exactly the reference's order; only the interleaving across lanes changes.
-// Returns the BVH slot of the closest triangle (or -1), its t, u, v.
+// Returns the BVH slot of the closest triangle (or -1), its t, u, v; with ANY
+// a lane stops at its first hit (occlusion queries).
 // `overflow` is set if the stack would exceed its capacity.
 #ifndef NR_LEAF_BATCH
 #define NR_LEAF_BATCH 1
 #endif
+template <bool ANY = false>
 NR_D int32_t mesh_closest(const GMesh& M, const Ray& r, float tmin, float tmax, Stack& st, int sp0,
                           float& bt, float& bu, float& bv, bool& overflow) {
     int32_t best = -1;
@@ -419,11 +423,13 @@ NR_D int32_t mesh_closest(const GMesh& M, const Ray& r, float tmin, float tmax, 
                 const float4* tp = reinterpret_cast<const float4*>(M.tris + slot);
                 float4 a = tp[0], b = tp[1], c = tp[2];
                 float t, u, v;
-                if (tri_hit(r, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), mk3(b.z, b.w, c.x), tmin, tmax, t, u, v)) {
+                bool h = tri_hit(r, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), mk3(b.z, b.w, c.x), tmin, tmax, t, u, v);
+                if (h) {
                     best = slot; tmax = t; bt = t; bu = u; bv = v;
                 }
                 leaf = false;
-                if (sp > sp0) st.pop(sp, ref, te);
+                if (ANY && h) active = false;
+                else if (sp > sp0) st.pop(sp, ref, te);
                 else active = false;
             }
         }
@@ -527,7 +533,7 @@ __device__ NR_GENERIC_ATTR bool object_hit_generic(const GScene& S, const GObjec
 // set that covers it; the code of absent types, and its registers, drop out).
 enum : int { NR_TY_QUAD = 1, NR_TY_SPHERE = 2, NR_TY_MESH = 4, NR_TY_OTHER = 8, NR_TY_ALL = 15 };
 
-template <int TY>
+template <int TY, bool ANY = false>
 NR_D bool object_t(const GScene& S, const GObject& o, const Ray& r, Stack& st, int sp0, bool& overflow,
                    float& tw, int32_t& slot, float& u, float& v, float& tobj) {
     HRec rec;
@@ -540,7 +546,7 @@ NR_D bool object_t(const GScene& S, const GObject& o, const Ray& r, Stack& st, i
         hrec_clear(rec);
         if ((TY & NR_TY_MESH) && o.type == 4) {
             const GMesh& M = S.meshes[o.mesh];
-            slot = mesh_closest(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow);
+            slot = mesh_closest<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow);
             h = slot >= 0;
             if (h) mesh_interp(M, slot, orr, tobj, u, v, rec);
         } else if ((TY & NR_TY_QUAD) && o.type == 1) {
@@ -615,6 +621,42 @@ NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, 
         }
     }
     return best;
+}
+
+// Occlusion query with the visibility of scene_closest: true iff scene_closest
+// would return an object for r (some object's hit lies in (NR_EPS, inf)).  A
+// lane stops at the first such hit; mesh leaves stop at their first triangle.
+template <int TY>
+NR_D bool scene_occluded(const GScene& S, const Ray& r, Stack& st, bool& overflow) {
+    const float tmin = NR_EPS, tmax = NR_FMAX;
+    int sp = 0;
+    if (st.cap < 1 || S.n_objects == 0) return false;
+    st.push(sp, S.root_ref, box_t(S.root_min, S.root_max, r, tmin, tmax));
+    while (sp > 0) {
+        int32_t ref; float te;
+        st.pop(sp, ref, te);
+        if (!(te < tmax)) continue;
+        if (ref < 0) {
+            float tw, u, v, tobj;
+            int32_t slot;
+            if (object_t<TY, true>(S, S.objects[~ref], r, st, sp, overflow, tw, slot, u, v, tobj) && tmin < tw &&
+                tw < tmax)
+                return true;
+            if (overflow) return false;
+        } else {
+            float lmn[3], lmx[3], rmn[3], rmx[3]; int32_t l, rr;
+            load_node(S.nodes, ref, lmn, lmx, rmn, rmx, l, rr);
+            float tl = box_t(lmn, lmx, r, tmin, tmax);
+            float tr = box_t(rmn, rmx, r, tmin, tmax);
+            bool left_near = tl < tr;
+            int32_t near_i = left_near ? l : rr, far_i = left_near ? rr : l;
+            float near_t = left_near ? tl : tr, far_t = left_near ? tr : tl;
+            if (sp + 2 > st.cap) { overflow = true; return false; }
+            if (far_t < tmax) st.push(sp, far_i, far_t);
+            st.push(sp, near_i, near_t);
+        }
+    }
+    return false;
 }
 
 // ---------------------------------------------------------------------------
@@ -852,3 +894,53 @@ NR_HD Ray screen_space_ray(const GCamera& c, float su, float sv) {
     f3 d = normalize(mv(c.R, mk3(u, v, -c.focal_length)));
     return mkray(ld3(c.pos), d);
 }
+
+// ---------------------------------------------------------------------------
+// HDRI importance sampling (fast mode).  The sky lookup maps a direction d to
+// u = atan2(z, x) / 2pi + 0.5, v = acos(y) / pi after the sky rotation R; the
+// density is piecewise constant over texels in (u, v), so the solid-angle pdf is
+// p(u, v) / (2 pi^2 sin(pi v)).
+// ---------------------------------------------------------------------------
+NR_D int cdf_find(const float* cdf, int n, float x) {          // last k with cdf[k] <= x, k in [0, n)
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (cdf[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// direction (world), its solid-angle pdf and radiance; marg may be an LDS copy
+NR_D f3 sky_cdf_sample(const GSky& sk, const float* marg, float u1, float u2, float& pdf) {
+    const int H = sk.H, W = sk.W;
+    const int j = cdf_find(marg, H, u1);
+    const float m0 = marg[j], m1 = marg[j + 1];
+    const float dv = m1 > m0 ? (u1 - m0) / (m1 - m0) : 0.5f;
+    const float* row = sk.cond + (size_t)j * (W + 1);
+    const int i = cdf_find(row, W, u2);
+    const float c0 = row[i], c1 = row[i + 1];
+    const float du = c1 > c0 ? (u2 - c0) / (c1 - c0) : 0.5f;
+    const float u = ((float)i + fminf(fmaxf(du, 0.0f), 1.0f)) / (float)W;
+    const float v = ((float)j + fminf(fmaxf(dv, 0.0f), 1.0f)) / (float)H;
+    const float theta = v * NR_PI, phi = (u - 0.5f) * NR_PI2;
+    const float st = nrm::sinf_(theta), ct = nrm::cosf_(theta);
+    const f3 local = mk3(st * nrm::cosf_(phi), ct, st * nrm::sinf_(phi));
+    const float p_uv = (m1 - m0) * (c1 - c0) * (float)(W * H);
+    pdf = st > 1e-7f ? p_uv / (2.0f * NR_PI * NR_PI * st) : 0.0f;
+    return mtv(sk.rot, local);
+}
+
+NR_D float sky_cdf_pdf(const GSky& sk, f3 d) {
+    const int H = sk.H, W = sk.W;
+    f3 dir = mv(sk.rot, normalize(d));
+    float u = nrm::atan2f_(dir.z, dir.x) / NR_PI2 + 0.5f;
+    float v = nrm::acosf_(fminf(fmaxf(dir.y, -1.0f), 1.0f)) / NR_PI;
+    int i = min(max((int)(u * (float)W), 0), W - 1);
+    int j = min(max((int)(v * (float)H), 0), H - 1);
+    const float* row = sk.cond + (size_t)j * (W + 1);
+    const float p_uv = (sk.marg[j + 1] - sk.marg[j]) * (row[i + 1] - row[i]) * (float)(W * H);
+    const float st = nrm::sinf_(v * NR_PI);
+    return st > 1e-7f ? p_uv / (2.0f * NR_PI * NR_PI * st) : 0.0f;
+}
+
+NR_HD float power_heuristic(float a, float b) { return a * a / (a * a + b * b); }

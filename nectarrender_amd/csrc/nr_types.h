@@ -92,6 +92,11 @@ struct GSky {
     float intensity;
     int32_t H, W;
     const float* data;      // CHW
+    // importance sampling of the HDRI (fast mode): piecewise-constant density over
+    // texels proportional to luminance x sin(theta); marginal CDF over rows (H + 1)
+    // and one conditional CDF per row (H x (W + 1)); null when not built
+    const float* marg;
+    const float* cond;
 };
 
 // everything a traversal/shading kernel needs, passed by value

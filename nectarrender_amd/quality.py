@@ -8,8 +8,14 @@ csrc/data/src/stream.cu:146-159) against a reference of 64 independent
 float64: 4032 sample passes per pixel.
 
 Plain: render K passes for K = 1, 2, 4, ... and time the render; the time to
-reach the target is interpolated log-log between the bracketing K.
+reach the target is interpolated log-log between the bracketing K.  Every
+render is what the reference's engine produces for samples_per_pixel = K + 1
+(passes 1..K of a (K + 1)-sample N-rooks pattern, camera.h:45-61): pass s
+takes x-stratum s of the pattern, so passes 1..K of a larger pattern would
+cover only part of each pixel and the error would stall at the missing
+antialiasing instead of falling with K.
 Denoised: render K passes, run the U-Net (nr_denoise), time both.
+rp["mode"] selects the estimator of every render (abi.MODE_PARITY when absent).
 """
 from __future__ import annotations
 
@@ -35,15 +41,22 @@ def _display(ctx, abi, torch, img_chw, H, W):
     return out
 
 
+def with_spp(cam, n: int):
+    """Copy of a camera descriptor with an n-sample pattern."""
+    c = type(cam).from_buffer_copy(cam)
+    c.samples_per_pixel = int(n)
+    return c
+
+
 def reference(ctx, abi, torch, cam, rp, n_renders=64, spp=64, log=print):
     """float64 mean of n_renders independent renders of passes 1..spp-1 (display image on the device)."""
     H, W = int(cam.resolution[0]), int(cam.resolution[1])
     acc = np.zeros((3, H, W), np.float64)
     t0 = time.time()
     for i in range(n_renders):
-        ctx.set_camera(0, cam, REF_SEED0 + i)
+        ctx.set_camera(0, with_spp(cam, spp), REF_SEED0 + i)
         ctx.reset(0)
-        ctx.render(0, abi.render_desc(1, spp, rp["depth"], REF_SEED0 + i))
+        ctx.render(0, abi.render_desc(1, spp, rp["depth"], REF_SEED0 + i, mode=rp.get("mode", abi.MODE_PARITY)))
         acc += ctx.read(0)
         if (i + 1) % 16 == 0:
             log(f"reference: {i + 1}/{n_renders} renders of {spp - 1} passes, {time.time() - t0:.1f} s")
@@ -58,14 +71,14 @@ def rmse(a, b):
 def curves(ctx, abi, torch, cam, rp, ref_disp, unet=None, plain_ks=(1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024),
            denoise_ks=(1, 2, 4, 8, 16, 32)):
     H, W = int(cam.resolution[0]), int(cam.resolution[1])
-    ctx.set_camera(0, cam, TEST_SEED)
     plain, den = [], []
     out = torch.empty((3, H, W), dtype=torch.float32, device="cuda") if unet is not None else None
     for K in sorted(set(plain_ks) | set(denoise_ks)):
+        ctx.set_camera(0, with_spp(cam, K + 1), TEST_SEED)
         ctx.reset(0)
         ctx.sync()
         t0 = time.perf_counter()
-        ctx.render(0, abi.render_desc(1, K + 1, rp["depth"], TEST_SEED))
+        ctx.render(0, abi.render_desc(1, K + 1, rp["depth"], TEST_SEED, mode=rp.get("mode", abi.MODE_PARITY)))
         ctx.sync()
         t_render = time.perf_counter() - t0
         if K in plain_ks:
