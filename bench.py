@@ -3,16 +3,21 @@
 Workload: 1920x1080 view (resolution (1080, 1920) in the reference's (H, W)
 order) of a 1,003,520-triangle displaced icosphere (PBR metal, roughness 0.4)
 on a Lambertian quad under a procedural 2048x1024 fp32 HDRI with a 1e4-radiance
-0.5 degree sun; depth 12, Russian roulette from bounce 3, seed 0; synthetic,
+0.5 degree sun; depth 12, Russian roulette from bounce 3, seed 0, MIS on
+(fast mode: next-event estimation to the HDRI, power heuristic); synthetic,
 seeded scene data.  A step is one sample pass over all 2,073,600 pixels
-(one path per pixel); K steps render sample passes 1..K, the 64-spp image is
-K = 63 (the reference renders passes 1..n-1, engine.cpp:43).
+(one path per pixel); K steps render passes 0..K-1, the 64-spp image is K = 64.
+The same passes with the reference's estimator (passes 1..K, bit-exact with
+the oracle) are reported beside it.
 
 Output: one JSON line with the contract fields plus
-  roofline      traversal (k_extend) bytes / traversal time against HBM peak,
-                with the algorithmic byte model counted by the oracle
-  cpu_baseline  the CPU oracle (same estimator) on this host's cores, on
-                full 1080p passes for ~12 s (at most the 63 passes of the image)
+  roofline          traversal (k_extend) bytes / traversal time against HBM
+                    peak, with the algorithmic byte model counted by the oracle
+  parity_estimator  Mpaths/s and kernel times of the reference's estimator
+  time_to_rmse      seconds to RMSE 0.01 on this scene, plain and U-Net denoised;
+                    time_to_rmse_end_to_end the same on the 500-sphere scene
+  cpu_baseline      the CPU oracle (the reference's estimator) on this host's
+                    cores, on full 1080p passes for ~12 s
 """
 from __future__ import annotations
 
@@ -38,7 +43,7 @@ TRAVERSAL_BYTES_PER_PATH = 1942.58
 # HBM bytes k_extend moves per path on this workload, from rocprofv3 FETCH_SIZE (x 1.99992, the factor
 # a 1 GiB device-to-device copy gives) + WRITE_SIZE (x 1.0) over 18,662,400 paths:
 # profiles/k_extend_c2_1080p.md.
-TRAVERSAL_HBM_BYTES_PER_PATH = 1265.3
+TRAVERSAL_HBM_BYTES_PER_PATH = {0: 1265.3, 1: 1265.3}    # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -63,22 +68,33 @@ def cpu_baseline(budget_s: float = 12.0, max_passes: int = 63):
                       f"{dt:.1f} s, OpenMP over 16x16 tiles"}
 
 
-def quality_section(log):
-    """Seconds to RMSE <= 0.01 (display-referred) vs a 4032-pass float64 reference, plain and denoised,
-    on the 1080p three-sphere scene with 500 media/dielectric spheres (parity estimator)."""
+def quality_section(log, ctx=None, scene=None, end_to_end=False):
+    """Seconds to RMSE <= 0.01 (display-referred) vs a 4096-sample float64 reference, plain and denoised:
+    on the benchmark scene with MIS (ctx holds it), or (end_to_end) on the 1080p three-sphere scene with
+    500 media/dielectric spheres and the reference's estimator (4032 samples)."""
     import torch
 
     from nectarrender_amd import abi, native, quality, scenes
     from nectarrender_amd.denoise import WEIGHTS, UNet
-    s, cam, rp = scenes.config5(spp=2049)
-    ctx = native.Context(0)
-    ctx.upload(s)
+    own = ctx is None
+    if end_to_end:
+        s, cam, rp = scenes.config5(spp=64)
+        workload = ("1920x1080 (H=1080, W=1920) three-sphere scene + 500 constant-medium/dielectric spheres, "
+                    "depth 8, reference estimator, white balance 6500 K tint 0, ACES, sqrt gamma")
+        ref_desc = "64 renders x 63 passes (1..63 of 64), float64 mean, disjoint seeds"
+    else:
+        s, cam, rp = scene
+        rp = dict(rp, mode=abi.MODE_FAST)
+        workload = ("the benchmark scene (1080p, 1,003,520-triangle icosphere, 2048x1024 HDRI with 1e4 sun, "
+                    "depth 12, MIS on), white balance 6500 K tint 0, ACES, sqrt gamma")
+        ref_desc = "64 renders x 64 passes (0..63 of 64), float64 mean, disjoint seeds"
+    if own:
+        ctx = native.Context(0)
+        ctx.upload(s)
     ref_disp, _ = quality.reference(ctx, abi, torch, cam, rp, log=log)
     unet = UNet(0).load() if os.path.exists(WEIGHTS) else None
     plain, den = quality.curves(ctx, abi, torch, cam, rp, ref_disp, unet)
-    out = {"workload": "1920x1080 (H=1080, W=1920) three-sphere scene + 500 constant-medium/dielectric spheres, "
-                       "depth 8, white balance 6500 K tint 0, ACES, sqrt gamma",
-           "target_rmse": quality.TARGET, "reference": "64 renders x 63 passes, float64 mean, disjoint seeds",
+    out = {"workload": workload, "target_rmse": quality.TARGET, "reference": ref_desc,
            "plain_s": quality.time_to_target(plain),
            "plain": [[k, round(t, 5), round(e, 5)] for k, t, e in plain]}
     if unet is not None:
@@ -94,14 +110,17 @@ def quality_section(log):
     else:
         out["denoised_s"] = None
         out["denoised_note"] = "no trained weights at " + WEIGHTS
-    ctx.close()
+    if out["plain_s"] and out["denoised_s"]:
+        out["speedup"] = round(out["plain_s"] / out["denoised_s"], 2)
+    if own:
+        ctx.close()
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=63)
+    ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-quality", action="store_true", help="skip the seconds-to-RMSE measurement")
@@ -121,6 +140,7 @@ def main():
         import torch                      # its HIP runtime initialises before the library's
         torch.cuda.init()
     from nectarrender_amd import abi, native, scenes
+    from nectarrender_amd.quality import with_spp
 
     if args.calibrate_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -129,7 +149,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    scene, cam, rp = scenes.config2(spp=max(64, K + 1, Wm + 1))
+    scene, cam, rp = scenes.config2(spp=64)
     ctx = native.Context(0)
     # scene upload (outside the Mpaths/s region): mesh tree on the host, then on the GPU (same tree)
     build = {}
@@ -139,60 +159,73 @@ def main():
         ctx.upload(scene)
         ctx.sync()
         build[name] = round(time.perf_counter() - tb, 4)
-    ctx.set_camera(0, cam, rp["seed"])
     if args.streams:
         ctx.set_render_streams(args.streams)
-    # warmup passes, then a clean accumulation for the timed passes
-    if Wm > 0:
-        ctx.render(0, abi.render_desc(1, 1 + Wm, rp["depth"], rp["seed"]))
-    ctx.sync()
-    ctx.reset(0)
-    ctx.sync()
-    t0 = time.perf_counter()
-    ctx.render(0, abi.render_desc(1, 1 + K, rp["depth"], rp["seed"]))
-    ctx.sync()
-    t1 = time.perf_counter()
-    img = ctx.read(0)
-    # the same passes again with per-kernel HIP events (one stream group, so each
-    # kernel's time is its own) for the breakdown and the k_extend roofline
-    ctx.reset(0)
-    ctx.set_timing(True)
-    ctx.render(0, abi.render_desc(1, 1 + K, rp["depth"], rp["seed"]))
-    ctx.sync()
-    tm = ctx.timings()       # [total, traversal, shading, generate+accumulate] ms (HIP events), iterations
-    ctx.set_timing(False)
-    img_t = ctx.read(0)
-    ctx.close()
-    if not np.array_equal(img, img_t):
-        raise RuntimeError("the timed and the event-timed renders differ")
-    dt = t1 - t0
-    paths = K * PATHS_PER_STEP
-    value = paths / dt / 1e6
-    n_extend = int(tm[4])    # one k_extend launch per streaming iteration
-    trav_s = tm[1] / 1e3
-    achieved = TRAVERSAL_BYTES_PER_PATH * paths / trav_s / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": round(TRAVERSAL_HBM_BYTES_PER_PATH * paths / n_extend),
-            "kernel": "k_extend", "launches": n_extend, "avg_launch_ms": round(tm[1] / n_extend, 4),
-            "bytes_per_launch": round(TRAVERSAL_BYTES_PER_PATH * paths / n_extend)}
+
+    def measure(mode):
+        """K timed passes (fast mode: 0..K-1 of a K-sample pattern; the reference's estimator: 1..K of
+        K + 1), then the same passes again with per-kernel HIP events on one stream group."""
+        p0 = 0 if mode == abi.MODE_FAST else 1
+        ctx.set_camera(0, with_spp(cam, K + p0), rp["seed"])
+        if Wm > 0:
+            ctx.render(0, abi.render_desc(p0, p0 + min(Wm, K), rp["depth"], rp["seed"], mode=mode))
+        ctx.sync()
+        ctx.reset(0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.render(0, abi.render_desc(p0, p0 + K, rp["depth"], rp["seed"], mode=mode))
+        ctx.sync()
+        t1 = time.perf_counter()
+        img = ctx.read(0)
+        ctx.reset(0)
+        ctx.set_timing(True)
+        ctx.render(0, abi.render_desc(p0, p0 + K, rp["depth"], rp["seed"], mode=mode))
+        ctx.sync()
+        tm = ctx.timings()   # [total, k_extend, shading, generate+accumulate, iterations, k_shadow] ms
+        ctx.set_timing(False)
+        if not np.array_equal(img, ctx.read(0)):
+            raise RuntimeError("the timed and the event-timed renders differ")
+        paths = K * PATHS_PER_STEP
+        n_extend = int(tm[4])    # one k_extend launch per streaming iteration
+        achieved = TRAVERSAL_BYTES_PER_PATH * paths / (tm[1] / 1e3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(TRAVERSAL_HBM_BYTES_PER_PATH[mode] * paths / n_extend),
+                "kernel": "k_extend", "launches": n_extend, "avg_launch_ms": round(tm[1] / n_extend, 4),
+                "bytes_per_launch": round(TRAVERSAL_BYTES_PER_PATH * paths / n_extend)}
+        gpu = {"total": round(tm[0], 3), "traversal": round(tm[1], 3), "shading": round(tm[2], 3),
+               "generate_accumulate": round(tm[3], 3)}
+        if mode == abi.MODE_FAST:
+            gpu["shadow_rays"] = round(tm[5], 3)
+        gpu["note"] = ("event-timed rerun of the same passes on one stream group; the Mpaths/s figure is the "
+                       "default two stream groups")
+        return paths / (t1 - t0) / 1e6, (t1 - t0), gpu, roof, img
+
+    value, dt, gpu_ms, roof, img = measure(abi.MODE_FAST)
+    par_value, par_dt, par_gpu, par_roof, _ = measure(abi.MODE_PARITY)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mpaths/s", "n_gpus": 1, "steps": K,
         "warmup": Wm, "ms_per_step": round(dt / K * 1e3, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "1920x1080 (H=1080, W=1920), 1,003,520-triangle displaced icosphere PBR metal "
                                "rough 0.4 on Lambertian quad, 2048x1024 fp32 HDRI with 1e4 sun, depth 12, "
-                               "parity estimator, one path per pixel per step",
+                               "MIS on (next-event estimation to the HDRI, power heuristic), "
+                               "one path per pixel per step",
                    "spp_passes": K, "ray_depth": rp["depth"], "paths_per_step": PATHS_PER_STEP},
-        "gpu_ms": {"total": round(tm[0], 3), "traversal": round(tm[1], 3), "shading": round(tm[2], 3),
-                   "generate_accumulate": round(tm[3], 3), "note": "event-timed rerun of the same passes on "
-                   "one stream group; `value` is the default two stream groups"},
+        "gpu_ms": gpu_ms,
         "image_mean": float(img.mean()),
         "scene_upload": dict(build, triangles=1_003_520),
         "roofline": roof,
+        "parity_estimator": {"value": round(par_value, 3), "unit": "Mpaths/s",
+                             "ms_per_step": round(par_dt / K * 1e3, 4),
+                             "note": "the reference's estimator (passes 1..K, bit-exact with the oracle)",
+                             "gpu_ms": par_gpu, "roofline": par_roof},
     }
     if not args.no_quality:
-        out["time_to_rmse"] = quality_section(lambda m: print(m, file=sys.stderr, flush=True))
+        log = lambda m: print(m, file=sys.stderr, flush=True)   # noqa: E731
+        out["time_to_rmse"] = quality_section(log, ctx, (scene, cam, rp))
+        out["time_to_rmse_end_to_end"] = quality_section(log, end_to_end=True)
+    ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -81,9 +81,11 @@ enum nr_layer {
  * direction drawn from the map's luminance x sin(theta) CDFs, weighted against
  * the scattering technique (the reference's mixture with lights[0] included)
  * by the power heuristic, with a shadow ray; sample s of pixel i draws from
- * the 64-bit Philox subsequence (s << 32) | i.  ObjectLights keep the
- * reference's one-sample mixture.  Same expectation as PARITY; checked by
- * convergence (white furnace, z-test against PARITY), not per pixel. */
+ * the 64-bit Philox subsequence (s << 32) | i, and passes start at 0 (pass s
+ * is folded as the (s + 1)-th sample, so samples_per_pixel = n renders all n
+ * pattern entries).  ObjectLights keep the reference's one-sample mixture.
+ * Same expectation as PARITY; checked by convergence (white furnace, z-test
+ * against PARITY), not per pixel. */
 enum nr_mode { NR_MODE_PARITY = 0, NR_MODE_FAST = 1 };
 
 /* Where nr_scene_upload builds mesh trees (BVH<T>::build, csrc/hittable/include/bvh/bvh.h:36-55,

@@ -794,7 +794,7 @@ struct RenderEngine {
     uint32_t n_samples() const { return cam.p.samples_per_pixel; }
 
     void reset() {                                                           // engine.cpp:216-220
-        sample_idx = 1u;
+        sample_idx = mis.load() ? 0u : 1u;       // fast mode renders all n pattern entries
         call_with_gil(on_reset);
     }
 
@@ -1739,9 +1739,10 @@ PYBIND11_MODULE(_pathtracer, m) {
         .def("max_depth", [](const RenderEngine& e) { return e.ray_depth; })
         .def("set_max_depth", &RenderEngine::set_max_depth)
         .def("set_axis_grid_visible", [](RenderEngine& e, bool v) { e.show_axis_grid = v; })
-        .def("set_mis", [](RenderEngine& e, bool v) { e.mis = v; }, py::arg("enabled"),
-             "Next-event estimation to an HDRI sky with power-heuristic MIS (NR_MODE_FAST) for later "
-             "passes; off (the reference's estimator) by default.  Not in the reference's API.")
+        .def("set_mis", [](RenderEngine& e, bool v) { e.mis = v; e.reset(); }, py::arg("enabled"),
+             "Next-event estimation to an HDRI sky with power-heuristic MIS (NR_MODE_FAST) from the next "
+             "pass on; resets the accumulation (fast mode starts at pass 0).  Off (the reference's "
+             "estimator) by default.  Not in the reference's API.")
         .def("mis", [](const RenderEngine& e) { return e.mis.load(); })
         .def("sample_index", [](const RenderEngine& e) { return e.sample_idx; },
              "Index of the next sample pass (1 after a reset).")

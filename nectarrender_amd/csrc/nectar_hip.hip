@@ -14,6 +14,9 @@
 //                   BRDF sampling / mixture light sampling (brdf.cu:4-35),
 //                   bounce-0 AOVs, Russian roulette (trace.cu:63-78); survivors
 //                   are compacted into the next live queue
+//     k_shadow      (fast mode) shadow rays of next-event estimation to the
+//                   HDRI: an any-hit traversal, the MIS-weighted radiance is
+//                   added if the ray reaches the sky
 //     k_regen       folds every ended path's sample into the running-mean AOVs
 //                   (accumulate_samples_kernel, data.cu:28-40) and starts the
 //                   pixel's next sample in the same iteration
@@ -21,7 +24,9 @@
 // Path state lives in HBM as SoA float4 arrays indexed by pixel; queues hold
 // pixel indices.  The per-path Philox stream is (seed, pixel + s*W*H, draw id)
 // exactly as trace.cu:58 keys it, so every path makes the reference's draws,
-// and each pixel's samples are folded in order.
+// and each pixel's samples are folded in order.  Fast mode (no counterpart in
+// the reference) keys the stream by the 64-bit (s << 32) | pixel, renders
+// passes from 0 and adds next-event estimation; it has the same expectation.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -820,7 +825,7 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(GCamera cam, PathPtrs P, 
         if (P.finished[i]) {
             P.finished[i] = 0;
             uint32_t s = (uint32_t)__float_as_int(reinterpret_cast<const float*>(P.d + i)[3]);
-            float fn = (float)s;
+            float fn = (float)(A.fast ? s + 1 : s);             // fast mode counts pass 0 as the first
             float a = (fn - 1.0f) / fn, b = 1.0f / fn;
             fold(P.acc + i, P.L[i], a, b);
             fold(P.acc + P.cap + i, P.diff[i], a, b);
@@ -1634,10 +1639,10 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     if (view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_STATE, "view has no camera");
     View& v = c->views[view];
     if (d->sample_end > v.cam.n_pattern) return fail(NR_ERR_INVALID, "sample index beyond samples_per_pixel");
-    if (d->sample_begin < 1 && d->sample_end > d->sample_begin)
-        return fail(NR_ERR_INVALID, "sample passes start at 1 (running mean of passes 1..n-1)");
     if (d->mode != NR_MODE_PARITY && d->mode != NR_MODE_FAST) return fail(NR_ERR_INVALID, "unknown render mode");
     const bool fast = d->mode == NR_MODE_FAST;
+    if (!fast && d->sample_begin < 1 && d->sample_end > d->sample_begin)
+        return fail(NR_ERR_INVALID, "sample passes start at 1 (running mean of passes 1..n-1)");
     HIPCK(hipSetDevice(c->device));
     uint32_t N = v.H * v.W;
     int e = ensure_paths(c, N);
@@ -1663,7 +1668,7 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     if (d->sample_end > d->sample_begin && d->ray_depth == 0) {
         LayerWrite w(c, v);
         for (uint32_t s = d->sample_begin; s < d->sample_end; s++)
-            hipLaunchKernelGGL(k_fold_zero, gN, dim3(256), 0, c->stream, lb, ld, ln, N, s);
+            hipLaunchKernelGGL(k_fold_zero, gN, dim3(256), 0, c->stream, lb, ld, ln, N, fast ? s + 1 : s);
     } else if (d->sample_end > d->sample_begin) {
         // Stream groups: G bands of whole tile rows, each with its own queues,
         // counters, stack-spill columns and stream.  Timing mode runs one group on

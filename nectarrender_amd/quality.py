@@ -9,11 +9,12 @@ float64: 4032 sample passes per pixel.
 
 Plain: render K passes for K = 1, 2, 4, ... and time the render; the time to
 reach the target is interpolated log-log between the bracketing K.  Every
-render is what the reference's engine produces for samples_per_pixel = K + 1
-(passes 1..K of a (K + 1)-sample N-rooks pattern, camera.h:45-61): pass s
-takes x-stratum s of the pattern, so passes 1..K of a larger pattern would
-cover only part of each pixel and the error would stall at the missing
-antialiasing instead of falling with K.
+render is what the engine produces for a K-sample image: with the reference's
+estimator samples_per_pixel = K + 1 and passes 1..K (engine.cpp:43), in fast
+mode samples_per_pixel = K and passes 0..K-1.  Pass s takes x-stratum s of the
+N-rooks pattern (camera.h:45-61), so K passes of a larger pattern would cover
+only part of each pixel and the error would stall at the missing antialiasing
+instead of falling with K.
 Denoised: render K passes, run the U-Net (nr_denoise), time both.
 rp["mode"] selects the estimator of every render (abi.MODE_PARITY when absent).
 """
@@ -48,18 +49,25 @@ def with_spp(cam, n: int):
     return c
 
 
+def first_pass(abi, rp) -> int:
+    """0 in fast mode, 1 with the reference's estimator (engine.cpp:43)."""
+    return 0 if rp.get("mode", abi.MODE_PARITY) == abi.MODE_FAST else 1
+
+
 def reference(ctx, abi, torch, cam, rp, n_renders=64, spp=64, log=print):
-    """float64 mean of n_renders independent renders of passes 1..spp-1 (display image on the device)."""
+    """float64 mean of n_renders independent spp-sample renders (spp - 1 passes with the reference's
+    estimator); returns the display image on the device and the linear mean."""
     H, W = int(cam.resolution[0]), int(cam.resolution[1])
     acc = np.zeros((3, H, W), np.float64)
     t0 = time.time()
+    p0 = first_pass(abi, rp)
     for i in range(n_renders):
         ctx.set_camera(0, with_spp(cam, spp), REF_SEED0 + i)
         ctx.reset(0)
-        ctx.render(0, abi.render_desc(1, spp, rp["depth"], REF_SEED0 + i, mode=rp.get("mode", abi.MODE_PARITY)))
+        ctx.render(0, abi.render_desc(p0, spp, rp["depth"], REF_SEED0 + i, mode=rp.get("mode", abi.MODE_PARITY)))
         acc += ctx.read(0)
         if (i + 1) % 16 == 0:
-            log(f"reference: {i + 1}/{n_renders} renders of {spp - 1} passes, {time.time() - t0:.1f} s")
+            log(f"reference: {i + 1}/{n_renders} renders of {spp - p0} passes, {time.time() - t0:.1f} s")
     mean = (acc / n_renders).astype(np.float32)
     return _display(ctx, abi, torch, mean, H, W), mean
 
@@ -73,12 +81,13 @@ def curves(ctx, abi, torch, cam, rp, ref_disp, unet=None, plain_ks=(1, 2, 4, 8, 
     H, W = int(cam.resolution[0]), int(cam.resolution[1])
     plain, den = [], []
     out = torch.empty((3, H, W), dtype=torch.float32, device="cuda") if unet is not None else None
+    p0 = first_pass(abi, rp)
     for K in sorted(set(plain_ks) | set(denoise_ks)):
-        ctx.set_camera(0, with_spp(cam, K + 1), TEST_SEED)
+        ctx.set_camera(0, with_spp(cam, K + 1 - p0), TEST_SEED)
         ctx.reset(0)
         ctx.sync()
         t0 = time.perf_counter()
-        ctx.render(0, abi.render_desc(1, K + 1, rp["depth"], TEST_SEED, mode=rp.get("mode", abi.MODE_PARITY)))
+        ctx.render(0, abi.render_desc(p0, K + p0, rp["depth"], TEST_SEED, mode=rp.get("mode", abi.MODE_PARITY)))
         ctx.sync()
         t_render = time.perf_counter() - t0
         if K in plain_ks:

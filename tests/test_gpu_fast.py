@@ -42,7 +42,8 @@ def _render(ctx, s, cam, spp, mode, seed=3, depth=6, streams=None, upload=True):
     if streams is not None:
         ctx.set_render_streams(streams)
     ctx.reset(0)
-    ctx.render(0, abi.render_desc(1, spp, depth=depth, seed=seed, mode=mode))
+    # fast mode renders passes 0..spp-1, the reference's estimator passes 1..spp-1
+    ctx.render(0, abi.render_desc(0 if mode == abi.MODE_FAST else 1, spp, depth=depth, seed=seed, mode=mode))
     img = ctx.read(0, abi.LAYER_BEAUTY)
     if streams is not None:
         ctx.set_render_streams(2)
@@ -151,3 +152,26 @@ def test_fast_config2_scene_small(ctx):
     cam = abi.camera_desc((108, 192), (0.0, 0.5, 3.8), (-30.0, 20.15, 0.0), 4, 2.0, 3.8, 0.0, 2.0, 1.0)
     img = _render(ctx, s, cam, 4, abi.MODE_FAST, depth=12)
     assert np.isfinite(img).all() and (img >= 0).all() and img.mean() > 0
+
+
+def test_fast_pass_numbering(ctx):
+    """Fast mode folds pass s as sample s + 1 (passes 0..n-1 make an n-sample image); the reference's
+    estimator starts at pass 1 and refuses 0."""
+    s, cam = _furnace_scene(0.5, spp=8)
+    ctx.upload(s)
+    ctx.set_camera(0, cam, 3)
+    with pytest.raises(RuntimeError):
+        ctx.render(0, abi.render_desc(0, 4, depth=4, seed=3, mode=abi.MODE_PARITY))
+    ctx.reset(0)
+    ctx.render(0, abi.render_desc(0, 8, depth=4, seed=3, mode=abi.MODE_FAST))
+    whole = ctx.read(0)
+    ctx.reset(0)
+    for p in range(8):                                          # one call per pass: same running mean
+        ctx.render(0, abi.render_desc(p, p + 1, depth=4, seed=3, mode=abi.MODE_FAST))
+    np.testing.assert_array_equal(ctx.read(0), whole)
+    # pass 0 alone is the first sample itself (weight 1)
+    ctx.reset(0)
+    ctx.render(0, abi.render_desc(0, 1, depth=4, seed=3, mode=abi.MODE_FAST))
+    first = ctx.read(0)
+    sky = np.all(np.abs(whole - 1.0) < 1e-6, axis=0)
+    np.testing.assert_array_equal(first[:, sky], 1.0)

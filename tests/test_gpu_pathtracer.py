@@ -259,3 +259,18 @@ def test_default_readback_is_denoised():
         return np.abs(np.diff(a, axis=1)).mean() + np.abs(np.diff(a, axis=2)).mean()
 
     assert tv(got) < 0.5 * tv(plain)          # 4-pass noise removed
+
+
+def test_engine_mis_toggle_renders_all_pattern_entries():
+    """RenderEngine.set_mis (fast mode, no counterpart in the reference) restarts the accumulation at
+    pass 0 and renders all n samples; switching back restores the reference's passes 1..n-1."""
+    e = _engine()
+    assert not e.mis() and e.sample_index() == 1
+    e.set_mis(True)
+    assert e.mis() and e.sample_index() == 0
+    e.render()
+    assert e.sample_index() == SPP
+    img = e.layers().beauty.numpy_float()
+    assert np.isfinite(img).all() and img.mean() > 0
+    e.set_mis(False)
+    assert e.sample_index() == 1
