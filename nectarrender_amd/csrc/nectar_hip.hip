@@ -65,8 +65,14 @@ static int fail(int code, const std::string& msg) { g_err = msg; return code; }
     } while (0)
 
 // counters (one int each): live queues [0,2), material queues [2,8), error flag,
-// shadow-ray queue (fast mode)
-enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_SHD = 9, CNT_N = 16 };
+// shadow-ray queue (fast mode), occupied path slots after k_regen
+enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_SHD = 9, CNT_OCC = 10, CNT_N = 16 };
+// Path slots per pixel: sample s of pixel i runs in slot s % NR_SLOTS (path id
+// i + (s % NR_SLOTS) * cap), so up to NR_SLOTS consecutive samples of a pixel
+// are in flight and pixels with long paths do not hold back the whole view.
+#ifndef NR_SLOTS
+#define NR_SLOTS 2
+#endif
 // rows of the HDRI marginal CDF that k_shade stages in LDS (larger maps read it from global memory)
 constexpr int SKY_MARG_LDS = 4097;
 #ifndef NR_LDS_LEVELS
@@ -92,8 +98,9 @@ struct StackCfg {
     size_t stride;          // rays in the launch
 };
 
-// Path state, one slot per pixel.  o.w carries the path's bounce index and
-// d.w the sample index it is tracing (as integer bits).
+// Path state, NR_SLOTS slots per pixel (path id = pixel + slot * cap).  o.w
+// carries the path's bounce index and d.w the sample index it is tracing (as
+// integer bits).  Queues hold path ids; their stride is qcap = NR_SLOTS * cap.
 struct PathPtrs {
     float4* o;
     float4* d;
@@ -103,10 +110,12 @@ struct PathPtrs {
     float4* nrm;
     uint32_t* draw;
     GHit* hit;
-    int32_t* qact;      // 2 * cap
-    int32_t* qmat;      // NR_MAT_TYPES * cap
+    int32_t* qact;      // 2 * qcap
+    int32_t* qmat;      // NR_MAT_TYPES * qcap
     int32_t* counts;    // CNT_N
-    uint8_t* finished;  // 1 = the pixel's path ended, fold it and start its next sample
+    uint8_t* finished;  // per path slot: 0 running, 1 ended (fold when its turn comes), 2 idle
+    uint32_t* nf;       // per pixel: next sample to fold
+    uint32_t* ns;       // per pixel: next sample to start
     float4* acc;        // running means during a render: beauty [0, cap), diffuse [cap, 2 cap), normal [2 cap, 3 cap)
     // fast mode: one pending shadow ray per path (origin, direction, MIS-weighted
     // contribution added to L if the ray reaches the sky), queued in qshd
@@ -114,7 +123,8 @@ struct PathPtrs {
     float4* shd;
     float4* shc;
     int32_t* qshd;
-    uint32_t cap;
+    uint32_t cap;       // pixels
+    uint32_t qcap;      // path slots, NR_SLOTS * cap
 };
 
 struct TraceArgs {
@@ -244,7 +254,7 @@ __device__ __forceinline__ Ray camera_ray(const GCamera& c, uint32_t x, uint32_t
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-template <int TY>
+template <int TY, bool WIDE>
 __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_trace_closest(const GScene* __restrict__ Sp,
                                                                     const float* __restrict__ rays, int n,
                                                                     nr_hit* __restrict__ hits,
@@ -259,7 +269,7 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_trace_closest(const GScene
     HRec rec;
     hrec_clear(rec);
     bool of = false;
-    int oi = scene_closest<TY>(S, r, rec, st, of);
+    int oi = scene_closest<TY, WIDE>(S, r, rec, st, of);
     if (of) atomicOr(&counts[CNT_ERR], 1);
     nr_hit h;
     if (oi < 0) {
@@ -476,18 +486,24 @@ __device__ __forceinline__ Gen path_gen(const TraceArgs& A, uint32_t i, uint32_t
 // atten.w of a path: the solid-angle pdf of the direction it was scattered
 // into (fast mode's MIS weight at a sky hit), or -1 after the camera or a
 // specular bounce
-__device__ __forceinline__ void start_path(const GCamera& cam, PathPtrs& P, uint32_t x, uint32_t y, uint32_t s,
-                                           const TraceArgs& A) {
+__device__ __forceinline__ uint32_t path_pixel(const PathPtrs& P, uint32_t path) { return path % P.cap; }
+
+// Starts sample s of pixel (x, y) in its slot; returns the path id.
+__device__ __forceinline__ uint32_t start_path(const GCamera& cam, PathPtrs& P, uint32_t x, uint32_t y, uint32_t s,
+                                               const TraceArgs& A) {
     uint32_t i = y * cam.W + x;
+    uint32_t p = i + (s % NR_SLOTS) * P.cap;
     Gen g = path_gen(A, i, s, 0u);
     Ray r = camera_ray(cam, x, y, i, s, A.seed, g);
-    P.o[i] = make_float4(r.o.x, r.o.y, r.o.z, __int_as_float(0));
-    P.d[i] = make_float4(r.d.x, r.d.y, r.d.z, __int_as_float((int)s));
-    P.atten[i] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
-    P.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.diff[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.draw[i] = g.draw;
+    P.o[p] = make_float4(r.o.x, r.o.y, r.o.z, __int_as_float(0));
+    P.d[p] = make_float4(r.d.x, r.d.y, r.d.z, __int_as_float((int)s));
+    P.atten[p] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
+    P.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.diff[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.nrm[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.draw[p] = g.draw;
+    P.finished[p] = 0;
+    return p;
 }
 
 // running-mean fold of one finished sample (accumulate_samples_kernel): the
@@ -498,6 +514,37 @@ __device__ __forceinline__ void fold(float4* acc, float4 v, float a, float b) {
     m.y = m.y * a + v.y * b;
     m.z = m.z * a + v.z * b;
     *acc = m;
+}
+
+// 4-wide layout of a mesh tree (fast mode): wide node i = the grandchildren of
+// binary node i, in left-to-right order.
+__global__ void __launch_bounds__(256) k_wide_nodes(const GNode* __restrict__ nodes, int32_t n, GWNode* __restrict__ out) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const GNode b = nodes[i];
+    GWNode w;
+    int k = 0;
+    auto put = [&](const float* mn, const float* mx, int32_t ref) {
+        for (int a = 0; a < 3; a++) { w.lo[a][k] = mn[a]; w.hi[a][k] = mx[a]; }
+        w.ref[k] = ref;
+        k++;
+    };
+    for (int side = 0; side < 2; side++) {
+        const int32_t ch = side ? b.right : b.left;
+        if (ch >= 0) {
+            const GNode c = nodes[ch];
+            put(c.lmin, c.lmax, c.left);
+            put(c.rmin, c.rmax, c.right);
+        } else {
+            put(side ? b.rmin : b.lmin, side ? b.rmax : b.lmax, ch);
+        }
+    }
+    for (; k < 4; k++) {
+        for (int a = 0; a < 3; a++) { w.lo[a][k] = 0.0f; w.hi[a][k] = 0.0f; }
+        w.ref[k] = NR_WIDE_EMPTY;
+    }
+    w.pad[0] = w.pad[1] = w.pad[2] = w.pad[3] = 0;
+    out[i] = w;
 }
 
 // The planar C x H x W layers stay the state between calls; a render folds into
@@ -524,20 +571,26 @@ __global__ void __launch_bounds__(256) k_unpack_acc(const float4* __restrict__ a
     n[i] = z.x; n[HW + i] = z.y; n[2 * (size_t)HW + i] = z.z;
 }
 
-// First sample of every pixel, queued in tile order (closed-form dense slot).
-// P.qact of a stream group starts at the group's first queue position q0 =
-// t0 / 64 tiles' worth of pixels (groups are whole bands of tiles).
+// First NR_SLOTS samples of every pixel (fewer if the range is shorter),
+// queued in tile order (closed-form dense position q, sample k at q + k*npix).
+// P.qact of a stream group starts at the group's first queue position
+// (NR_SLOTS times the pixels of the bands before it).
 __global__ void __launch_bounds__(256) k_start(GCamera cam, PathPtrs P, uint32_t s, TraceArgs A) {
     uint32_t t = A.t0 + blockIdx.x * blockDim.x + threadIdx.x, x, y;
-    if (t == A.t0) P.counts[CNT_ACT + 0] = (int32_t)A.npix;
+    const uint32_t m = min((uint32_t)NR_SLOTS, A.s_end - s);       // samples started per pixel
+    if (t == A.t0) P.counts[CNT_ACT + 0] = (int32_t)(A.npix * m);
     if (t >= A.t1 || !tile_pixel(cam, t, x, y)) return;
     uint32_t band = y >> 3, bh = min(8u, cam.H - band * 8);
     uint32_t tx = x >> 3, tw = min(8u, cam.W - tx * 8);
     uint32_t band0 = (A.t0 >> 6) / ((cam.W + 7) >> 3);
     uint32_t q = (band - band0) * 8 * cam.W + tx * 8 * bh + (y & 7) * tw + (x & 7);
-    start_path(cam, P, x, y, s, A);
-    P.finished[y * cam.W + x] = 0;
-    P.qact[q] = (int32_t)(y * cam.W + x);
+    const uint32_t i = y * cam.W + x;
+    for (uint32_t k = 0; k < NR_SLOTS; k++) {
+        if (k < m) P.qact[q + k * A.npix] = (int32_t)start_path(cam, P, x, y, s + k, A);
+        else P.finished[i + ((s + k) % NR_SLOTS) * P.cap] = 2;
+    }
+    P.nf[i] = s;
+    P.ns[i] = s + m;
 }
 
 // Clears the counters one iteration appends to: material queues and the next live queue.
@@ -546,6 +599,7 @@ __global__ void k_counts(int32_t* counts, int32_t in) {
     if (t < NR_MAT_TYPES) counts[CNT_MAT + t] = 0;
     if (t == 0) counts[CNT_ACT + (1 - in)] = 0;
     if (t == 1) counts[CNT_SHD] = 0;
+    if (t == 2) counts[CNT_OCC] = 0;
 }
 
 template <int TY, bool FAST>
@@ -559,13 +613,13 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_extend(const GScene* __res
     int path = -1;
     int mtype = 0;
     if (i < n) {
-        path = P.qact[(size_t)A.in * P.cap + i];
+        path = P.qact[(size_t)A.in * P.qcap + i];
         float4 o4 = P.o[path], d4 = P.d[path];
         Ray r = mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
         HRec rec;
         hrec_clear(rec);
         bool of = false;
-        int oi = scene_closest<TY>(S, r, rec, st, of);
+        int oi = scene_closest<TY, FAST>(S, r, rec, st, of);
         if (of) atomicOr(&P.counts[CNT_ERR], 1);
         if (oi < 0) {                                                           // trace.cu:19-22
             float4 a4 = P.atten[path], l4 = P.L[path];
@@ -593,7 +647,7 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_extend(const GScene* __res
         }
     }
     const int slot = block_append_keyed<128, NR_MAT_TYPES>(mtype, &P.counts[CNT_MAT]);
-    if (mtype > 0) P.qmat[(size_t)mtype * P.cap + slot] = path;
+    if (mtype > 0) P.qmat[(size_t)mtype * P.qcap + slot] = path;
 }
 
 // material evaluate (material.h:59-67, 152-180, 351-357)
@@ -650,7 +704,7 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
     bool cont = false, shadow = false;
     int path = -1;
     if (i < n) {
-        path = P.qmat[(size_t)M * P.cap + i];
+        path = P.qmat[(size_t)M * P.qcap + i];
         GHit h = P.hit[path];
         float4 o4 = P.o[path], d4 = P.d[path], a4 = P.atten[path], l4 = P.L[path];
         const uint32_t bounce = (uint32_t)__float_as_int(o4.w);
@@ -661,7 +715,7 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
         f3 p = ld3(h.p), nn = ld3(h.n), tg = ld3(h.tg);
         f2 uv; uv.x = h.uv[0]; uv.y = h.uv[1];
         bool front = h.front_face != 0;
-        Gen g = path_gen(A, (uint32_t)path, sample, P.draw[path]);
+        Gen g = path_gen(A, path_pixel(P, (uint32_t)path), sample, P.draw[path]);
         const GMaterial& m = S.materials[h.material];
         // emission (material.h:182-188, 306-312)
         f3 em = s3(0.0f);
@@ -781,7 +835,7 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
         }
     }
     int slot = block_append<256>(cont, &P.counts[CNT_ACT + out]);
-    if (cont) P.qact[(size_t)out * P.cap + slot] = path;
+    if (cont) P.qact[(size_t)out * P.qcap + slot] = path;
     if (NEE_M) {
         int ss = block_append<256>(shadow, &P.counts[CNT_SHD]);
         if (shadow) P.qshd[ss] = path;
@@ -804,7 +858,7 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_shadow(const GScene* __res
         int path = P.qshd[i];
         float4 o4 = P.sho[path], d4 = P.shd[path];
         bool of = false;
-        bool blocked = scene_occluded<TY>(S, mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), st, of);
+        bool blocked = scene_occluded<TY, true>(S, mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), st, of);
         if (of) atomicOr(&P.counts[CNT_ERR], 1);
         if (!blocked && !of) {
             float4 c = P.shc[path], l = P.L[path];
@@ -813,31 +867,50 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_shadow(const GScene* __res
     }
 }
 
-// Fold the finished paths' samples and start their next samples, scanning all
-// pixels in tile order so the new primary rays of a wave are coherent.
+// Per pixel, scanning all pixels in tile order so the new primary rays of a
+// wave are coherent: fold the ended samples strictly in sample order (sample
+// nf lives in slot nf % NR_SLOTS; a sample that ends before its predecessor
+// waits), then start the next samples in the slots that became idle.  The
+// occupied slots (samples started and not yet folded, ns - nf) are counted:
+// that number never grows, so the host sizes later traversal grids from it.
 constexpr int REGEN_BLOCK = 1024;
 __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(GCamera cam, PathPtrs P, TraceArgs A) {
     uint32_t t = A.t0 + blockIdx.x * blockDim.x + threadIdx.x, x = 0, y = 0;
-    bool restart = false;
-    uint32_t i = 0;
+    uint32_t started[NR_SLOTS];
+    uint32_t n_start = 0, occ = 0;
     if (t < A.t1 && tile_pixel(cam, t, x, y)) {
-        i = y * cam.W + x;
-        if (P.finished[i]) {
-            P.finished[i] = 0;
-            uint32_t s = (uint32_t)__float_as_int(reinterpret_cast<const float*>(P.d + i)[3]);
-            float fn = (float)(A.fast ? s + 1 : s);             // fast mode counts pass 0 as the first
+        const uint32_t i = y * cam.W + x;
+        uint32_t nf = P.nf[i], ns = P.ns[i];
+#pragma unroll
+        for (int k = 0; k < NR_SLOTS; k++) {
+            const uint32_t p = i + (nf % NR_SLOTS) * P.cap;
+            if (nf >= ns || P.finished[p] != 1) break;
+            float fn = (float)(A.fast ? nf + 1 : nf);           // fast mode counts pass 0 as the first
             float a = (fn - 1.0f) / fn, b = 1.0f / fn;
-            fold(P.acc + i, P.L[i], a, b);
-            fold(P.acc + P.cap + i, P.diff[i], a, b);
-            fold(P.acc + 2 * (size_t)P.cap + i, P.nrm[i], a, b);
-            if (s + 1 < A.s_end) {
-                start_path(cam, P, x, y, s + 1, A);
-                restart = true;
-            }
+            fold(P.acc + i, P.L[p], a, b);
+            fold(P.acc + P.cap + i, P.diff[p], a, b);
+            fold(P.acc + 2 * (size_t)P.cap + i, P.nrm[p], a, b);
+            P.finished[p] = 2;
+            nf++;
         }
+#pragma unroll
+        for (int k = 0; k < NR_SLOTS; k++) {
+            if (ns >= A.s_end || ns - nf >= NR_SLOTS) break;    // slot ns % NR_SLOTS is idle when ns - nf < NR_SLOTS
+            started[k] = start_path(cam, P, x, y, ns, A);
+            n_start = k + 1;
+            ns++;
+        }
+        P.nf[i] = nf;
+        P.ns[i] = ns;
+        occ = ns - nf;
     }
-    int slot = block_append<REGEN_BLOCK>(restart, &P.counts[CNT_ACT + (1 - A.in)]);
-    if (restart) P.qact[(size_t)(1 - A.in) * P.cap + slot] = (int32_t)i;
+    int32_t* live = &P.counts[CNT_ACT + (1 - A.in)];
+#pragma unroll
+    for (int k = 0; k < NR_SLOTS; k++) {
+        int slot = block_append<REGEN_BLOCK>((uint32_t)k < n_start, live);
+        if ((uint32_t)k < n_start) P.qact[(size_t)(1 - A.in) * P.qcap + slot] = (int32_t)started[k];
+        (void)block_append<REGEN_BLOCK>((uint32_t)k < occ, &P.counts[CNT_OCC]);
+    }
 }
 
 // ray_depth 0: every pass adds a zero sample (trace.cu:63 runs no bounce)
@@ -1030,6 +1103,7 @@ struct nr_ctx {
     int64_t last_iterations = 0;
     // timing
     bool timing = false;
+    bool trace_wide = false;    // nr_trace_closest over the 4-wide mesh layout
     std::vector<hipEvent_t> ev;             // per-launch marks
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     float last_ms[8] = {0};
@@ -1118,6 +1192,11 @@ static bool sky_cdfs(const float* data, int H, int W, std::vector<float>& marg, 
     marg[H] = 1.0f;
     return true;
 }
+
+// Stack levels a mesh traversal of a tree of the given depth can hold: depth for
+// the binary near-first order (one far child per level), 3 per wide node and
+// ceil(depth / 2) + 1 wide levels for the 4-wide order.
+static int mesh_stack_levels(int depth) { return std::max(depth, 3 * ((depth + 1) / 2 + 1)); }
 
 static size_t trav_lds_bytes() { return (size_t)LDS_STACK_LEVELS * 64 * sizeof(int2) * (128 / 64); }
 
@@ -1422,6 +1501,15 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
         int e = upload(c, verts.data(), verts.size(), &dv);
         if (e) return e;
         gm.nodes = dn; gm.tris = dt; gm.idx = di; gm.verts = dv;
+        gm.wnodes = nullptr;
+        if (n_nodes > 0) {
+            GWNode* dw = nullptr;
+            HIPCK(hipMalloc((void**)&dw, sizeof(GWNode) * (size_t)n_nodes));
+            c->scene_mem.push_back(dw);
+            hipLaunchKernelGGL(k_wide_nodes, dim3((n_nodes + 255) / 256), dim3(256), 0, c->stream, dn, n_nodes, dw);
+            HIPCK(hipGetLastError());
+            gm.wnodes = dw;
+        }
         meshes[mi] = gm;
         mesh_nodes[mi] = n_nodes;
     }
@@ -1468,7 +1556,7 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
         if (S.first_light < 0 && objs[s].type == NR_HIT_OBJECT_LIGHT) S.first_light = s;   // registry.h:240-244
     }
     S.n_objects = n;
-    S.stack_depth = T.depth + max_mesh_depth + 4;
+    S.stack_depth = T.depth + mesh_stack_levels(max_mesh_depth) + 4;
     {
         GNode* dn; GObject* dob; GMesh* dm; GMaterial* dmat; GImage* dimg;
         int e = upload(c, tnodes.data(), tnodes.size(), &dn);
@@ -1599,27 +1687,31 @@ static int ensure_paths(nr_ctx* c, uint32_t n) {
         c->path_mem.push_back(*p);
         return NR_OK;
     };
-    int e = al((void**)&P.o, sizeof(float4) * n);
-    if (!e) e = al((void**)&P.d, sizeof(float4) * n);
-    if (!e) e = al((void**)&P.atten, sizeof(float4) * n);
-    if (!e) e = al((void**)&P.L, sizeof(float4) * n);
-    if (!e) e = al((void**)&P.diff, sizeof(float4) * n);
-    if (!e) e = al((void**)&P.nrm, sizeof(float4) * n);
-    if (!e) e = al((void**)&P.draw, sizeof(uint32_t) * n);
-    if (!e) e = al((void**)&P.hit, sizeof(GHit) * n);
-    if (!e) e = al((void**)&P.qact, sizeof(int32_t) * 2 * (size_t)n);
-    if (!e) e = al((void**)&P.qmat, sizeof(int32_t) * NR_MAT_TYPES * (size_t)n);
+    const size_t np = (size_t)NR_SLOTS * n;      // path slots
+    int e = al((void**)&P.o, sizeof(float4) * np);
+    if (!e) e = al((void**)&P.d, sizeof(float4) * np);
+    if (!e) e = al((void**)&P.atten, sizeof(float4) * np);
+    if (!e) e = al((void**)&P.L, sizeof(float4) * np);
+    if (!e) e = al((void**)&P.diff, sizeof(float4) * np);
+    if (!e) e = al((void**)&P.nrm, sizeof(float4) * np);
+    if (!e) e = al((void**)&P.draw, sizeof(uint32_t) * np);
+    if (!e) e = al((void**)&P.hit, sizeof(GHit) * np);
+    if (!e) e = al((void**)&P.qact, sizeof(int32_t) * 2 * np);
+    if (!e) e = al((void**)&P.qmat, sizeof(int32_t) * NR_MAT_TYPES * np);
     if (!e) e = al((void**)&P.counts, sizeof(int32_t) * CNT_N * NR_MAX_GROUPS);
-    if (!e) e = al((void**)&P.finished, sizeof(uint8_t) * (size_t)n);
+    if (!e) e = al((void**)&P.finished, sizeof(uint8_t) * np);
+    if (!e) e = al((void**)&P.nf, sizeof(uint32_t) * (size_t)n);
+    if (!e) e = al((void**)&P.ns, sizeof(uint32_t) * (size_t)n);
     if (!e) e = al((void**)&P.acc, sizeof(float4) * 3 * (size_t)n);
-    if (!e) e = al((void**)&P.sho, sizeof(float4) * n);
-    if (!e) e = al((void**)&P.shd, sizeof(float4) * n);
-    if (!e) e = al((void**)&P.shc, sizeof(float4) * n);
-    if (!e) e = al((void**)&P.qshd, sizeof(int32_t) * (size_t)n);
+    if (!e) e = al((void**)&P.sho, sizeof(float4) * np);
+    if (!e) e = al((void**)&P.shd, sizeof(float4) * np);
+    if (!e) e = al((void**)&P.shc, sizeof(float4) * np);
+    if (!e) e = al((void**)&P.qshd, sizeof(int32_t) * np);
     if (e) { free_list(c->path_mem); return e; }
-    HIPCK(hipMemsetAsync(P.finished, 0, n, c->stream));
+    HIPCK(hipMemsetAsync(P.finished, 2, np, c->stream));
     HIPCK(hipMemsetAsync(P.counts, 0, sizeof(int32_t) * CNT_N * NR_MAX_GROUPS, c->stream));
     P.cap = n;
+    P.qcap = (uint32_t)np;
     c->path_cap = n;
     return NR_OK;
 }
@@ -1645,6 +1737,7 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
         return fail(NR_ERR_INVALID, "sample passes start at 1 (running mean of passes 1..n-1)");
     HIPCK(hipSetDevice(c->device));
     uint32_t N = v.H * v.W;
+    if ((uint64_t)N * NR_SLOTS > (uint64_t)INT32_MAX) return fail(NR_ERR_LIMIT, "view too large for 32-bit path ids");
     int e = ensure_paths(c, N);
     if (e) return e;
     const size_t lds = trav_lds_bytes();
@@ -1686,7 +1779,7 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
         const size_t lv = c->scene.stack_depth > LDS_STACK_LEVELS ? (size_t)(c->scene.stack_depth - LDS_STACK_LEVELS) : 0;
         {
             StackCfg all;
-            e = stack_spill(c, N + 128u * NR_MAX_GROUPS, all);
+            e = stack_spill(c, NR_SLOTS * N + 128u * NR_MAX_GROUPS, all);
             if (e) return e;
             size_t spill_off = 0;
             for (int g = 0; g < G; g++) {
@@ -1694,19 +1787,19 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
                 const uint32_t q0 = b0 * 8 * v.W, q1 = std::min(b1 * 8, v.H) * v.W;
                 Group& gr = grp[g];
                 gr.P = c->P;
-                gr.P.qact = c->P.qact + q0;
-                gr.P.qmat = c->P.qmat + q0;
-                gr.P.qshd = c->P.qshd + q0;
+                gr.P.qact = c->P.qact + (size_t)NR_SLOTS * q0;
+                gr.P.qmat = c->P.qmat + (size_t)NR_SLOTS * q0;
+                gr.P.qshd = c->P.qshd + (size_t)NR_SLOTS * q0;
                 gr.P.counts = c->P.counts + CNT_N * g;
                 gr.A = TraceArgs{d->seed, (uint32_t)((uint64_t)v.W * v.H), 0u, d->ray_depth, d->do_rr,
                                  d->rr_start_bounce, d->rr_survival, d->sample_end,
                                  b0 * tiles_x * 64, b1 * tiles_x * 64, q1 - q0, fast ? 1 : 0};
-                const uint32_t cap_g = (gr.A.npix + 127) / 128 * 128;
+                const uint32_t cap_g = (NR_SLOTS * gr.A.npix + 127) / 128 * 128;
                 gr.SC.spill = all.spill + spill_off;
                 gr.SC.stride = cap_g;
                 spill_off += lv * cap_g;
                 gr.st = G == 1 ? c->stream : c->gstream[g];
-                gr.bound = gr.A.npix;
+                gr.bound = (int64_t)gr.A.npix * std::min<uint32_t>(NR_SLOTS, d->sample_end - d->sample_begin);
             }
         }
         mark(2);
@@ -1720,10 +1813,10 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
             hipLaunchKernelGGL(k_start, dim3((nt + 255) / 256), dim3(256), 0, grp[g].st, v.cam, grp[g].P,
                                d->sample_begin, grp[g].A);
         }
-        // The live count never grows from one iteration to the next (a path
-        // either continues, or ends and restarts at most once), so a count read
-        // back two iterations late is an upper bound for the traversal grid and
-        // each group always has its next iteration queued.
+        // The occupied path slots (samples started and not folded) never grow
+        // from one iteration to the next and bound the live paths, so a count
+        // read back two iterations late is an upper bound for the traversal
+        // grid and each group always has its next iteration queued.
         const int64_t max_iters = (int64_t)(d->sample_end - d->sample_begin) * d->ray_depth + 2;
         int active = G;
         while (active > 0) {
@@ -1766,8 +1859,7 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
                 hipLaunchKernelGGL(k_regen, dim3((nt + REGEN_BLOCK - 1) / REGEN_BLOCK), dim3(REGEN_BLOCK), 0, st,
                                    v.cam, P, A);
                 HIPCK(hipGetLastError());
-                HIPCK(hipMemcpyAsync(live + (it & 1), P.counts + CNT_ACT + (1 - A.in), sizeof(int32_t),
-                                     hipMemcpyDeviceToHost, st));
+                HIPCK(hipMemcpyAsync(live + (it & 1), P.counts + CNT_OCC, sizeof(int32_t), hipMemcpyDeviceToHost, st));
                 HIPCK(hipEventRecord(c->gev[g][it & 1], st));
                 A.in = 1 - A.in;
                 gr.it++;
@@ -1840,6 +1932,13 @@ float* nr_aov_device_ptr(nr_ctx* c, int view, int layer) {
     return c->views[view].layer[layer];
 }
 
+int nr_set_trace_layout(nr_ctx* c, int wide) {
+    if (!c) return fail(NR_ERR_INVALID, "null context");
+    if (wide != 0 && wide != 1) return fail(NR_ERR_INVALID, "layout must be 0 (binary) or 1 (4-wide)");
+    c->trace_wide = wide == 1;
+    return NR_OK;
+}
+
 int nr_trace_closest_device(nr_ctx* c, const float* d_rays, int32_t n, nr_hit* d_hits) {
     if (!c) return fail(NR_ERR_INVALID, "null context");
     if (!c->has_scene) return fail(NR_ERR_STATE, "trace before scene upload");
@@ -1852,12 +1951,16 @@ int nr_trace_closest_device(nr_ctx* c, const float* d_rays, int32_t n, nr_hit* d
     HIPCK(hipMemsetAsync(c->d_err + CNT_ERR, 0, sizeof(int32_t), c->stream));
     hipEvent_t e0 = c->ev_begin, e1 = c->ev_end;
     if (c->timing) (void)hipEventRecord(e0, c->stream);
-    if (c->obj_types & NR_TY_OTHER)
-        hipLaunchKernelGGL(k_trace_closest<NR_TY_ALL>, dim3((n + 127) / 128), dim3(128), trav_lds_bytes(), c->stream,
-                           c->d_scene, d_rays, n, d_hits, c->d_err, SC);
-    else
-        hipLaunchKernelGGL(k_trace_closest<NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH>, dim3((n + 127) / 128), dim3(128),
-                           trav_lds_bytes(), c->stream, c->d_scene, d_rays, n, d_hits, c->d_err, SC);
+    constexpr int TYI = NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH;
+    const dim3 g((n + 127) / 128), b(128);
+    const size_t lds = trav_lds_bytes();
+    if (c->obj_types & NR_TY_OTHER) {
+        if (c->trace_wide) hipLaunchKernelGGL((k_trace_closest<NR_TY_ALL, true>), g, b, lds, c->stream, c->d_scene, d_rays, n, d_hits, c->d_err, SC);
+        else hipLaunchKernelGGL((k_trace_closest<NR_TY_ALL, false>), g, b, lds, c->stream, c->d_scene, d_rays, n, d_hits, c->d_err, SC);
+    } else {
+        if (c->trace_wide) hipLaunchKernelGGL((k_trace_closest<TYI, true>), g, b, lds, c->stream, c->d_scene, d_rays, n, d_hits, c->d_err, SC);
+        else hipLaunchKernelGGL((k_trace_closest<TYI, false>), g, b, lds, c->stream, c->d_scene, d_rays, n, d_hits, c->d_err, SC);
+    }
     HIPCK(hipGetLastError());
     if (c->timing) {
         (void)hipEventRecord(e1, c->stream);
@@ -2217,7 +2320,7 @@ int nr_object_set_transform(nr_ctx* c, int32_t index, const float position[3], c
         objs[k] = c->reg_objects[T.order[k]];
         if (S.first_light < 0 && objs[k].type == NR_HIT_OBJECT_LIGHT) S.first_light = k;   // registry.h:240-244
     }
-    S.stack_depth = T.depth + c->max_mesh_depth + 4;
+    S.stack_depth = T.depth + mesh_stack_levels(c->max_mesh_depth) + 4;
     // same sizes as the full upload: rewrite the tables in place once the queued work is done
     HIPCK(hipStreamSynchronize(c->stream));
     if (!tnodes.empty()) HIPCK(hipMemcpy((void*)S.nodes, tnodes.data(), sizeof(GNode) * tnodes.size(), hipMemcpyHostToDevice));

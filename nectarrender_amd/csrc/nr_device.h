@@ -438,6 +438,89 @@ NR_D int32_t mesh_closest(const GMesh& M, const Ray& r, float tmin, float tmax, 
     return best;
 }
 
+// The same closest-hit query over the 4-wide layout (fast mode): one 128 B
+// node per two binary levels, children entered nearest first (sorted by entry
+// distance), the nearer ones pushed behind the farther.  The closest hit is the
+// binary traversal's; only which of two triangles hit at exactly the same t
+// is reported may differ.  Leaf batching as in mesh_closest.
+template <bool ANY = false>
+NR_D int32_t mesh_closest_w(const GMesh& M, const Ray& r, float tmin, float tmax, Stack& st, int sp0,
+                            float& bt, float& bu, float& bv, bool& overflow) {
+    int32_t best = -1;
+    int sp = sp0;
+    if (sp0 + 1 > st.cap) { overflow = true; return -1; }
+    int32_t ref = M.root_ref;
+    float te = box_t(M.root_min, M.root_max, r, tmin, tmax);
+    bool active = true;
+    bool leaf = false;
+    while (true) {
+        if (active && !leaf) {
+            if (!(te < tmax)) {
+                if (sp > sp0) st.pop(sp, ref, te);
+                else active = false;
+            } else if (ref < 0) {
+                leaf = true;
+            } else {
+                const float4* p = reinterpret_cast<const float4*>(M.wnodes + ref);
+                float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3], q4 = p[4], q5 = p[5], q6 = p[6];
+                const float lx[4] = {q0.x, q0.y, q0.z, q0.w}, ly[4] = {q1.x, q1.y, q1.z, q1.w};
+                const float lz[4] = {q2.x, q2.y, q2.z, q2.w}, hx[4] = {q3.x, q3.y, q3.z, q3.w};
+                const float hy[4] = {q4.x, q4.y, q4.z, q4.w}, hz[4] = {q5.x, q5.y, q5.z, q5.w};
+                int32_t c[4] = {__float_as_int(q6.x), __float_as_int(q6.y), __float_as_int(q6.z),
+                                __float_as_int(q6.w)};
+                float t[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float mn[3] = {lx[k], ly[k], lz[k]}, mx[3] = {hx[k], hy[k], hz[k]};
+                    t[k] = c[k] != NR_WIDE_EMPTY ? box_t(mn, mx, r, tmin, tmax) : NR_FMAX;
+                }
+                // sorting network over (t, ref), ascending t
+#define NR_CSWAP(a, b)                                                         \
+    if (t[b] < t[a]) {                                                         \
+        float tt = t[a]; t[a] = t[b]; t[b] = tt;                               \
+        int32_t cc = c[a]; c[a] = c[b]; c[b] = cc;                             \
+    }
+                NR_CSWAP(0, 1) NR_CSWAP(2, 3) NR_CSWAP(0, 2) NR_CSWAP(1, 3) NR_CSWAP(1, 2)
+#undef NR_CSWAP
+                const int nh = (t[0] < tmax) + (t[1] < tmax) + (t[2] < tmax) + (t[3] < tmax);
+                if (nh == 0) {
+                    if (sp > sp0) st.pop(sp, ref, te);
+                    else active = false;
+                } else if (sp + nh - 1 > st.cap) {
+                    overflow = true;
+                    active = false;
+                } else {
+                    if (nh > 3) st.push(sp, c[3], t[3]);
+                    if (nh > 2) st.push(sp, c[2], t[2]);
+                    if (nh > 1) st.push(sp, c[1], t[1]);
+                    ref = c[0];
+                    te = t[0];
+                }
+            }
+        }
+        unsigned long long held = __ballot(leaf);
+        unsigned long long walking = __ballot(active && !leaf);
+        if (held != 0ull && (__popcll(held) >= NR_LEAF_BATCH || walking == 0ull)) {
+            if (leaf) {
+                int32_t slot = ~ref;
+                const float4* tp = reinterpret_cast<const float4*>(M.tris + slot);
+                float4 a = tp[0], b = tp[1], cc = tp[2];
+                float tt, u, v;
+                bool h = tri_hit(r, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), mk3(b.z, b.w, cc.x), tmin, tmax, tt, u, v);
+                if (h) {
+                    best = slot; tmax = tt; bt = tt; bu = u; bv = v;
+                }
+                leaf = false;
+                if (ANY && h) active = false;
+                else if (sp > sp0) st.pop(sp, ref, te);
+                else active = false;
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+    }
+    return best;
+}
+
 // interpolate_hit (hittable/mesh.h:367-383)
 NR_D void mesh_interp(const GMesh& M, int32_t slot, const Ray& r, float t, float u, float v, HRec& rec) {
     const uint32_t* ix = M.idx + 3 * (size_t)slot;
@@ -533,7 +616,7 @@ __device__ NR_GENERIC_ATTR bool object_hit_generic(const GScene& S, const GObjec
 // set that covers it; the code of absent types, and its registers, drop out).
 enum : int { NR_TY_QUAD = 1, NR_TY_SPHERE = 2, NR_TY_MESH = 4, NR_TY_OTHER = 8, NR_TY_ALL = 15 };
 
-template <int TY, bool ANY = false>
+template <int TY, bool ANY = false, bool WIDE = false>
 NR_D bool object_t(const GScene& S, const GObject& o, const Ray& r, Stack& st, int sp0, bool& overflow,
                    float& tw, int32_t& slot, float& u, float& v, float& tobj) {
     HRec rec;
@@ -546,7 +629,8 @@ NR_D bool object_t(const GScene& S, const GObject& o, const Ray& r, Stack& st, i
         hrec_clear(rec);
         if ((TY & NR_TY_MESH) && o.type == 4) {
             const GMesh& M = S.meshes[o.mesh];
-            slot = mesh_closest<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow);
+            slot = (WIDE && M.wnodes) ? mesh_closest_w<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow)
+                                      : mesh_closest<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow);
             h = slot >= 0;
             if (h) mesh_interp(M, slot, orr, tobj, u, v, rec);
         } else if ((TY & NR_TY_QUAD) && o.type == 1) {
@@ -567,7 +651,8 @@ NR_D bool object_t(const GScene& S, const GObject& o, const Ray& r, Stack& st, i
 // index of the closest hit or -1; rec receives the world-space record.  Only
 // (object, t, triangle slot, u, v) stay live during the traversal; the record
 // of the winner is rebuilt at the end by the same deterministic arithmetic.
-template <int TY>
+// WIDE: meshes are traversed over their 4-wide layout (fast mode)
+template <int TY, bool WIDE = false>
 NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, bool& overflow) {
     float tmin = NR_EPS, tmax = NR_FMAX;
     int32_t best = -1, bslot = -1;
@@ -583,7 +668,7 @@ NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, 
             int32_t oi = ~ref;
             float tw, u, v, tobj;
             int32_t slot;
-            if (object_t<TY>(S, S.objects[oi], r, st, sp, overflow, tw, slot, u, v, tobj)) {
+            if (object_t<TY, false, WIDE>(S, S.objects[oi], r, st, sp, overflow, tw, slot, u, v, tobj)) {
                 if (tmin < tw && tw < tmax) {
                     best = oi; bslot = slot; bu = u; bv = v; btobj = tobj;
                     tmax = tw;
@@ -626,7 +711,7 @@ NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, 
 // Occlusion query with the visibility of scene_closest: true iff scene_closest
 // would return an object for r (some object's hit lies in (NR_EPS, inf)).  A
 // lane stops at the first such hit; mesh leaves stop at their first triangle.
-template <int TY>
+template <int TY, bool WIDE = true>
 NR_D bool scene_occluded(const GScene& S, const Ray& r, Stack& st, bool& overflow) {
     const float tmin = NR_EPS, tmax = NR_FMAX;
     int sp = 0;
@@ -639,7 +724,7 @@ NR_D bool scene_occluded(const GScene& S, const Ray& r, Stack& st, bool& overflo
         if (ref < 0) {
             float tw, u, v, tobj;
             int32_t slot;
-            if (object_t<TY, true>(S, S.objects[~ref], r, st, sp, overflow, tw, slot, u, v, tobj) && tmin < tw &&
+            if (object_t<TY, true, WIDE>(S, S.objects[~ref], r, st, sp, overflow, tw, slot, u, v, tobj) && tmin < tw &&
                 tw < tmax)
                 return true;
             if (overflow) return false;

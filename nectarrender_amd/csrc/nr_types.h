@@ -3,6 +3,8 @@
 // Layout in HBM (all tables are flat arrays, one allocation each):
 //   GNode    64 B   binary BVH node holding BOTH child boxes + child refs, so one
 //                   interior visit is one 64 B line (4 x dwordx4 loads)
+//   GWNode  128 B   4-wide mesh node for fast mode: the grandchildren of binary
+//                   node i at index i, boxes SoA, one 128 B line (8 x dwordx4)
 //   GTri     48 B   triangle positions v0,v1,v2 in BVH leaf order + original index
 //   GVertex  48 B   shading attributes (normal, tangent, uv), read once per closest hit
 //   GObject 128 B   object record (transform, type, material, mesh)
@@ -29,6 +31,19 @@ struct alignas(64) GNode {
 };
 static_assert(sizeof(GNode) == 64, "GNode must be one 64 B line");
 
+// Wide node i holds the up to four grandchildren of binary node i (a child that
+// is a leaf stands for itself), so every binary interior ref is also a wide
+// node index and the binary root is the wide root.  Unused slots have ref
+// NR_WIDE_EMPTY and are never box-tested.
+#define NR_WIDE_EMPTY ((int32_t)0x80000000)
+struct alignas(128) GWNode {
+    float lo[3][4];         // [axis][child]
+    float hi[3][4];
+    int32_t ref[4];         // binary refs (>= 0 node, < 0 leaf ~slot)
+    int32_t pad[4];
+};
+static_assert(sizeof(GWNode) == 128, "GWNode is one 128 B line");
+
 struct alignas(16) GTri {
     float v0[3], v1[3], v2[3];
     int32_t orig;
@@ -43,6 +58,7 @@ struct alignas(16) GVertex {
 
 struct alignas(16) GMesh {
     const GNode* nodes;
+    const GWNode* wnodes;   // same tree, 4 wide (null for a one-triangle mesh)
     const GTri* tris;       // BVH slot order
     const uint32_t* idx;    // 3 vertex indices per BVH slot
     const GVertex* verts;

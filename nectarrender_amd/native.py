@@ -57,6 +57,7 @@ def lib() -> C.CDLL:
         "nr_readback": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrReadbackDesc), C.POINTER(C.c_uint8)]),
         "nr_view_mask_buffer": (vp, [vp, C.c_int]),
         "nr_set_build_mode": (C.c_int, [vp, C.c_int]),
+        "nr_set_trace_layout": (C.c_int, [vp, C.c_int]),
         "nr_set_render_streams": (C.c_int, [vp, C.c_int]),
         "nr_mesh_tree": (C.c_int, [vp, C.c_int, vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                    C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
@@ -191,6 +192,10 @@ class Context:
     def set_render_streams(self, n: int):
         """Stream groups of later nr_render calls (1..4)."""
         _check(lib().nr_set_render_streams(self._h, n))
+
+    def set_trace_layout(self, wide: bool):
+        """Mesh layout of later trace_closest calls: the binary tree (False) or fast mode's 4-wide one."""
+        _check(lib().nr_set_trace_layout(self._h, int(bool(wide))))
 
     def set_build_mode(self, mode: int):
         """Mesh tree builder of later uploads: abi.BUILD_HOST, BUILD_DEVICE or BUILD_AUTO."""

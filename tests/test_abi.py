@@ -23,7 +23,7 @@ def test_header_declares_entry_points():
     for n in ("nr_create", "nr_destroy", "nr_scene_upload", "nr_camera_set", "nr_render", "nr_trace_closest",
               "nr_aov_read", "nr_post", "nr_last_error", "nr_view_size", "nr_unet_create", "nr_unet_forward",
               "nr_unet_train_step", "nr_unet_gradients", "nr_denoise", "nr_viewport", "nr_pick",
-              "nr_selection_mask", "nr_overlay_device", "nr_readback", "nr_set_build_mode", "nr_mesh_tree",
+              "nr_selection_mask", "nr_overlay_device", "nr_readback", "nr_set_build_mode", "nr_set_trace_layout", "nr_mesh_tree",
               "nr_object_set_transform", "nr_view_mask_buffer"):
         assert n in names
 
@@ -61,6 +61,7 @@ def test_null_arguments_fail_cleanly_without_device():
     assert L.nr_readback(None, 0, None, None) == 1
     assert not L.nr_view_mask_buffer(None, 0)
     assert L.nr_set_build_mode(None, 1) == 1
+    assert L.nr_set_trace_layout(None, 1) == 1
     assert L.nr_set_render_streams(None, 2) == 1
     assert L.nr_mesh_tree(None, 0, None, None, None, None, None) == 1
     assert L.nr_object_set_transform(None, 0, None, None, None) == 1

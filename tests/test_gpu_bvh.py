@@ -135,3 +135,37 @@ def test_set_transform_equals_fresh_upload(ctx, oracle):
         hit = got["object_id"] >= 0
         for f in ("t", "p", "n"):
             np.testing.assert_array_equal(got[f][hit], ref[f][hit])
+
+
+@pytest.mark.parametrize("scene", ["icosphere", "mixed", "soup"])
+def test_wide_layout_finds_the_binary_closest_hit(ctx, scene):
+    """Fast mode's 4-wide mesh traversal returns the binary traversal's closest hit: the same object and
+    distance for every ray, and the same triangle except where two triangles are hit at exactly the same
+    distance (an edge or vertex hit, where the visit order decides)."""
+    if scene == "icosphere":
+        v, t = scenes.displaced_icosphere(freq=80)
+        s = _mesh_scene(v, t)
+        rays = scenes.random_rays(200_000, 3, center=(0, 0, 0), spread=3.0)
+    elif scene == "mixed":
+        s, _, _ = scenes.mixed_scene()
+        rays = scenes.random_rays(200_000, 4, center=(0, 0.3, 0), spread=2.5)
+    else:
+        v, t = _soup(20_000, seed=11)
+        s = _mesh_scene(v, t)
+        rays = scenes.random_rays(200_000, 5, center=(0, 0, 0), spread=2.0)
+    ctx.upload(s)
+    ref = ctx.trace_closest(rays)
+    ctx.set_trace_layout(True)
+    try:
+        got = ctx.trace_closest(rays)
+    finally:
+        ctx.set_trace_layout(False)
+    np.testing.assert_array_equal(got["object_id"], ref["object_id"])
+    hit = ref["object_id"] >= 0
+    assert hit.sum() > 10_000
+    np.testing.assert_array_equal(got["t"][hit], ref["t"][hit])
+    same = got["prim"] == ref["prim"]
+    print(f"{scene}: {hit.sum()} hits, {(~same).sum()} equal-distance ties reported differently")
+    assert (~same).sum() <= max(5, hit.sum() // 10_000)
+    for f in ("p", "n", "uv"):
+        np.testing.assert_array_equal(got[f][hit & same], ref[f][hit & same])
