@@ -112,6 +112,14 @@ def quality_section(log, ctx=None, scene=None, end_to_end=False):
         out["denoised_note"] = "no trained weights at " + WEIGHTS
     if out["plain_s"] and out["denoised_s"]:
         out["speedup"] = round(out["plain_s"] / out["denoised_s"], 2)
+    four = [(k, tr + td, e) for k, tr, td, e in den if k == 4] if unet is not None else []
+    if four and out["plain_s"]:
+        # the 4-sample + U-Net frame on its own: its time, its error, and the speedup over the
+        # plain path tracer's time to the same error
+        k, t4, e4 = four[0]
+        same = quality.time_to_target(plain, target=e4)
+        out["denoised_4spp"] = {"seconds": round(t4, 5), "rmse": round(e4, 5),
+                                "plain_s_to_same_rmse": same, "speedup": round(same / t4, 2) if same else None}
     if own:
         ctx.close()
     return out

@@ -83,7 +83,7 @@ def curves(ctx, abi, torch, cam, rp, ref_disp, unet=None, plain_ks=(1, 2, 4, 8, 
     out = torch.empty((3, H, W), dtype=torch.float32, device="cuda") if unet is not None else None
     p0 = first_pass(abi, rp)
     for K in sorted(set(plain_ks) | set(denoise_ks)):
-        ctx.set_camera(0, with_spp(cam, K + 1 - p0), TEST_SEED)
+        ctx.set_camera(0, with_spp(cam, K + p0), TEST_SEED)
         ctx.reset(0)
         ctx.sync()
         t0 = time.perf_counter()
