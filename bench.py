@@ -40,10 +40,13 @@ PATHS_PER_STEP = W * H
 # 23.873 interior nodes (64 B), 1.8217 triangle tests (48 B), 1.9960 top-level
 # objects (128 B), 40 B ray/hit per segment.
 TRAVERSAL_BYTES_PER_PATH = 1942.58
+# The scattered rays of the MIS estimator follow the same distribution (next-event
+# estimation adds shadow rays, traced by k_shadow, and changes no scattering
+# decision's probability), so the same per-path model applies to its k_extend.
 # HBM bytes k_extend moves per path on this workload, from rocprofv3 FETCH_SIZE (x 1.99992, the factor
-# a 1 GiB device-to-device copy gives) + WRITE_SIZE (x 1.0) over 18,662,400 paths:
+# a 1 GiB device-to-device copy gives) + WRITE_SIZE (x 1.0) over 18,662,400 paths per estimator:
 # profiles/k_extend_c2_1080p.md.
-TRAVERSAL_HBM_BYTES_PER_PATH = {0: 1265.3, 1: 1265.3}    # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
+TRAVERSAL_HBM_BYTES_PER_PATH = {0: 1105.1, 1: 1071.0}    # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
 HBM_PEAK_GBS = 8000.0
 
 
