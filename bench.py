@@ -128,6 +128,41 @@ def quality_section(log, ctx=None, scene=None, end_to_end=False):
     return out
 
 
+def views_section(log):
+    """The 16-view configuration (10,000 spheres of every material, views of 256x256 .. 1920x1080,
+    10,142,016 pixels, 32 samples: passes 1..31, depth 16, reference estimator): all views in one
+    nr_render_views call against one nr_render call per view, on the same passes."""
+    from nectarrender_amd import abi, native, scenes
+    s, cams, rp = scenes.config3()
+    ctx = native.Context(0)
+    ctx.upload(s)
+    views = list(range(len(cams)))
+    for v, cam in enumerate(cams):
+        ctx.set_camera(v, cam, rp["seed"] + v)
+    px = sum(int(c.resolution[0]) * int(c.resolution[1]) for c in cams)
+    d = abi.render_desc(1, 32, rp["depth"], rp["seed"])
+    ctx.render_views(views, abi.render_desc(1, 2, rp["depth"], rp["seed"]))   # warmup pass
+    out = {"workload": f"16 views, {px} pixels, 10,000 spheres (PBR, dielectric, Lambertian, emissive, "
+                       "constant media), passes 1..31, depth 16, reference estimator"}
+    for name in ("one_call", "per_view"):
+        for v in views:
+            ctx.reset(v)
+        ctx.sync()
+        t0 = time.perf_counter()
+        if name == "one_call":
+            ctx.render_views(views, d)
+        else:
+            for v in views:
+                ctx.render(v, d)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        out[name] = {"seconds": round(dt, 4), "Mpaths_s": round(px * 31 / dt / 1e6, 1)}
+        log(f"views: {name} {dt:.3f} s")
+    out["speedup"] = round(out["per_view"]["seconds"] / out["one_call"]["seconds"], 2)
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +271,7 @@ def main():
         log = lambda m: print(m, file=sys.stderr, flush=True)   # noqa: E731
         out["time_to_rmse"] = quality_section(log, ctx, (scene, cam, rp))
         out["time_to_rmse_end_to_end"] = quality_section(log, end_to_end=True)
+        out["batched_views"] = views_section(log)
     ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()

@@ -253,6 +253,15 @@ int nr_aov_reset(nr_ctx* ctx, int view);
  * context's stream; nr_sync waits. */
 int nr_render(nr_ctx* ctx, int view, const nr_render_desc* desc);
 
+/* The same sample passes for several views (1 to NR_MAX_VIEWS, distinct) in one
+ * streaming pipeline: every launch serves the paths of all of them, so small
+ * and uneven views share the GPU instead of running one after another.  Each
+ * view's AOVs equal what nr_render gives it with the same desc.  The batch form
+ * of the reference's one-engine-per-camera loop (engine.cpp:42-74); the
+ * reference has no multi-view call.  Waits for the context's previous work
+ * before it starts. */
+int nr_render_views(nr_ctx* ctx, const int32_t* views, int n_views, const nr_render_desc* desc);
+
 /* Image size of a view (H = resolution[0], W = resolution[1]). */
 int nr_view_size(nr_ctx* ctx, int view, int32_t* H, int32_t* W);
 

@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <random>
@@ -127,13 +128,24 @@ struct PathPtrs {
     float4* shd;
     float4* shc;
     int32_t* qshd;
+    int32_t* err;       // traversal-overflow flag (group 0's CNT_ERR; every group sets it)
     uint32_t cap;       // pixels
-    uint32_t qcap;      // path slots, NR_SLOTS * cap
+    uint32_t qcap;      // queue entries per queue (NR_SLOTS * padded pixels)
+};
+
+// A view of a render call: its camera, where its 8x8 tiles start in the call's
+// tile enumeration and where its pixels start in the path arrays.  All views of
+// one call share every launch of the streaming pipeline.
+struct GView {
+    GCamera cam;
+    uint32_t tile0;     // first tile of the view (tiles left to right within bands of 8 rows)
+    uint32_t pix0;      // first path-array pixel of the view (its pixel y*W + x is pix0 + y*W + x)
+    uint32_t tiles_x;   // tiles per band
+    uint32_t npix;      // W * H: the subsequence of sample s is pixel + s*W*H (mod 2^32)
 };
 
 struct TraceArgs {
     uint32_t seed;
-    uint32_t WH;        // pixels of the view: the subsequence of sample s is pixel + s*W*H (mod 2^32)
     uint32_t in;        // live queue read by this iteration (the other one is filled)
     uint32_t depth;
     int32_t do_rr;
@@ -141,8 +153,9 @@ struct TraceArgs {
     float rr_survival;
     uint32_t s_end;     // passes [s_begin, s_end) are rendered
     uint32_t t0, t1;    // this stream group's range of the tiled pixel enumeration (tile_pixel)
-    uint32_t npix;      // pixels in the group
     int32_t fast;       // NR_MODE_FAST: 64-bit sample subsequence (pixel, sample), NEE + MIS
+    uint32_t n_views;
+    const GView* views; // device table, n_views entries
 };
 
 // ---------------------------------------------------------------------------
@@ -468,21 +481,32 @@ __global__ void __launch_bounds__(256) k_overlay(uint8_t* __restrict__ img, cons
 // folded strictly in order, so the AOVs equal the reference's pass-by-pass
 // result bit for bit.
 
-// 8x8 pixel tiles, tiles left to right within bands of 8 rows: thread t of the
-// tiled grid maps to pixel (x, y); partial tiles at the edges are skipped.
-__device__ __forceinline__ bool tile_pixel(const GCamera& cam, uint32_t t, uint32_t& x, uint32_t& y) {
-    uint32_t tiles_x = (cam.W + 7) >> 3;
-    uint32_t tile = t >> 6, within = t & 63;
-    x = (tile % tiles_x) * 8 + (within & 7);
-    y = (tile / tiles_x) * 8 + (within >> 3);
-    return x < cam.W && y < cam.H;
+// 8x8 pixel tiles, tiles left to right within bands of 8 rows, the views one
+// after another: thread t of the tiled grid maps to view v, pixel (x, y);
+// partial tiles at the edges are skipped.
+__device__ __forceinline__ bool tile_pixel(const TraceArgs& A, uint32_t t, uint32_t& v, uint32_t& x, uint32_t& y) {
+    const uint32_t tile = t >> 6, within = t & 63;
+    v = 0;
+    while (v + 1 < A.n_views && tile >= A.views[v + 1].tile0) v++;
+    const GView& V = A.views[v];
+    const uint32_t lt = tile - V.tile0;
+    x = (lt % V.tiles_x) * 8 + (within & 7);
+    y = (lt / V.tiles_x) * 8 + (within >> 3);
+    return x < V.cam.W && y < V.cam.H;
+}
+
+// view of a path-array pixel
+__device__ __forceinline__ uint32_t pixel_view(const TraceArgs& A, uint32_t g) {
+    uint32_t v = 0;
+    while (v + 1 < A.n_views && g >= A.views[v + 1].pix0) v++;
+    return v;
 }
 
 // Generator of pixel i's sample s: the reference's 32-bit subsequence i + s*W*H
 // (trace.cu:58), or in fast mode the 64-bit subsequence (s << 32) | i, which
 // never repeats across samples.
-__device__ __forceinline__ Gen path_gen(const TraceArgs& A, uint32_t i, uint32_t s, uint32_t draw) {
-    Gen g{A.seed, i + s * A.WH, draw};
+__device__ __forceinline__ Gen path_gen(const TraceArgs& A, uint32_t i, uint32_t s, uint32_t draw, uint32_t WH) {
+    Gen g{A.seed, i + s * WH, draw};
     if (A.fast) { g.subseq = i; g.subhi = s; }
     return g;
 }
@@ -492,12 +516,13 @@ __device__ __forceinline__ Gen path_gen(const TraceArgs& A, uint32_t i, uint32_t
 // specular bounce
 __device__ __forceinline__ uint32_t path_pixel(const PathPtrs& P, uint32_t path) { return path % P.cap; }
 
-// Starts sample s of pixel (x, y) in its slot; returns the path id.
-__device__ __forceinline__ uint32_t start_path(const GCamera& cam, PathPtrs& P, uint32_t x, uint32_t y, uint32_t s,
+// Starts sample s of pixel (x, y) of view V in its slot; returns the path id.
+__device__ __forceinline__ uint32_t start_path(const GView& V, PathPtrs& P, uint32_t x, uint32_t y, uint32_t s,
                                                const TraceArgs& A) {
+    const GCamera& cam = V.cam;
     uint32_t i = y * cam.W + x;
-    uint32_t p = i + (s % NR_SLOTS) * P.cap;
-    Gen g = path_gen(A, i, s, 0u);
+    uint32_t p = V.pix0 + i + (s % NR_SLOTS) * P.cap;
+    Gen g = path_gen(A, i, s, 0u, V.npix);
     Ray r = camera_ray(cam, x, y, i, s, A.seed, g);
     P.o[p] = make_float4(r.o.x, r.o.y, r.o.z, __int_as_float(0));
     P.d[p] = make_float4(r.d.x, r.d.y, r.d.z, __int_as_float((int)s));
@@ -576,25 +601,32 @@ __global__ void __launch_bounds__(256) k_unpack_acc(const float4* __restrict__ a
 }
 
 // First NR_SLOTS samples of every pixel (fewer if the range is shorter),
-// queued in tile order (closed-form dense position q, sample k at q + k*npix).
-// P.qact of a stream group starts at the group's first queue position
-// (NR_SLOTS times the pixels of the bands before it).
-__global__ void __launch_bounds__(256) k_start(GCamera cam, PathPtrs P, uint32_t s, TraceArgs A) {
-    uint32_t t = A.t0 + blockIdx.x * blockDim.x + threadIdx.x, x, y;
+// appended to the live queue in tile order, one workgroup-aggregated append per
+// sample rank.  The group's counters were cleared by k_clear_counts.
+__global__ void __launch_bounds__(256) k_start(PathPtrs P, uint32_t s, TraceArgs A) {
+    uint32_t t = A.t0 + blockIdx.x * blockDim.x + threadIdx.x, v = 0, x = 0, y = 0;
     const uint32_t m = min((uint32_t)NR_SLOTS, A.s_end - s);       // samples started per pixel
-    if (t == A.t0) P.counts[CNT_ACT + 0] = (int32_t)(A.npix * m);
-    if (t >= A.t1 || !tile_pixel(cam, t, x, y)) return;
-    uint32_t band = y >> 3, bh = min(8u, cam.H - band * 8);
-    uint32_t tx = x >> 3, tw = min(8u, cam.W - tx * 8);
-    uint32_t band0 = (A.t0 >> 6) / ((cam.W + 7) >> 3);
-    uint32_t q = (band - band0) * 8 * cam.W + tx * 8 * bh + (y & 7) * tw + (x & 7);
-    const uint32_t i = y * cam.W + x;
-    for (uint32_t k = 0; k < NR_SLOTS; k++) {
-        if (k < m) P.qact[q + k * A.npix] = (int32_t)start_path(cam, P, x, y, s + k, A);
-        else P.finished[i + ((s + k) % NR_SLOTS) * P.cap] = 2;
+    const bool valid = t < A.t1 && tile_pixel(A, t, v, x, y);
+    uint32_t g = 0;
+    if (valid) {
+        const GView& V = A.views[v];
+        g = V.pix0 + y * V.cam.W + x;
+        P.nf[g] = s;
+        P.ns[g] = s + m;
     }
-    P.nf[i] = s;
-    P.ns[i] = s + m;
+    for (uint32_t k = 0; k < NR_SLOTS; k++) {
+        const bool go = valid && k < m;
+        int32_t path = 0;
+        if (go) path = (int32_t)start_path(A.views[v], P, x, y, s + k, A);
+        else if (valid) P.finished[g + ((s + k) % NR_SLOTS) * P.cap] = 2;
+        const int slot = block_append<256>(go, &P.counts[CNT_ACT + 0]);
+        if (go) P.qact[slot] = path;
+    }
+}
+
+// every counter of a group but the sticky overflow flag
+__global__ void k_clear_counts(int32_t* counts) {
+    if (threadIdx.x < CNT_N && threadIdx.x != CNT_ERR) counts[threadIdx.x] = 0;
 }
 
 // Clears the counters one iteration appends to: material queues and the next live queue.
@@ -624,7 +656,7 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_extend(const GScene* __res
         hrec_clear(rec);
         bool of = false;
         int oi = scene_closest<TY, FAST>(S, r, rec, st, of);
-        if (of) atomicOr(&P.counts[CNT_ERR], 1);
+        if (of) atomicOr(P.err, 1);
         if (oi < 0) {                                                           // trace.cu:19-22
             float4 a4 = P.atten[path], l4 = P.L[path];
             f3 Le = sky_sample(S.sky, r.d);
@@ -719,7 +751,9 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
         f3 p = ld3(h.p), nn = ld3(h.n), tg = ld3(h.tg);
         f2 uv; uv.x = h.uv[0]; uv.y = h.uv[1];
         bool front = h.front_face != 0;
-        Gen g = path_gen(A, path_pixel(P, (uint32_t)path), sample, P.draw[path]);
+        const uint32_t gp = path_pixel(P, (uint32_t)path);
+        const GView& V = A.views[pixel_view(A, gp)];
+        Gen g = path_gen(A, gp - V.pix0, sample, P.draw[path], V.npix);
         const GMaterial& m = S.materials[h.material];
         // emission (material.h:182-188, 306-312)
         f3 em = s3(0.0f);
@@ -863,7 +897,7 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_shadow(const GScene* __res
         float4 o4 = P.sho[path], d4 = P.shd[path];
         bool of = false;
         bool blocked = scene_occluded<TY, true>(S, mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), st, of);
-        if (of) atomicOr(&P.counts[CNT_ERR], 1);
+        if (of) atomicOr(P.err, 1);
         if (!blocked && !of) {
             float4 c = P.shc[path], l = P.L[path];
             P.L[path] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
@@ -878,12 +912,13 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_shadow(const GScene* __res
 // occupied slots (samples started and not yet folded, ns - nf) are counted:
 // that number never grows, so the host sizes later traversal grids from it.
 constexpr int REGEN_BLOCK = 1024;
-__global__ void __launch_bounds__(REGEN_BLOCK) k_regen(GCamera cam, PathPtrs P, TraceArgs A) {
-    uint32_t t = A.t0 + blockIdx.x * blockDim.x + threadIdx.x, x = 0, y = 0;
+__global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) {
+    uint32_t t = A.t0 + blockIdx.x * blockDim.x + threadIdx.x, v = 0, x = 0, y = 0;
     uint32_t started[NR_SLOTS];
     uint32_t n_start = 0, occ = 0;
-    if (t < A.t1 && tile_pixel(cam, t, x, y)) {
-        const uint32_t i = y * cam.W + x;
+    if (t < A.t1 && tile_pixel(A, t, v, x, y)) {
+        const GView& V = A.views[v];
+        const uint32_t i = V.pix0 + y * V.cam.W + x;
         uint32_t nf = P.nf[i], ns = P.ns[i];
 #pragma unroll
         for (int k = 0; k < NR_SLOTS; k++) {
@@ -900,7 +935,7 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(GCamera cam, PathPtrs P, 
 #pragma unroll
         for (int k = 0; k < NR_SLOTS; k++) {
             if (ns >= A.s_end || ns - nf >= NR_SLOTS) break;    // slot ns % NR_SLOTS is idle when ns - nf < NR_SLOTS
-            started[k] = start_path(cam, P, x, y, ns, A);
+            started[k] = start_path(V, P, x, y, ns, A);
             n_start = k + 1;
             ns++;
         }
@@ -1090,6 +1125,8 @@ struct nr_ctx {
     std::vector<void*> path_mem;
     uint32_t path_cap = 0;
     int32_t* d_err = nullptr;           // counters of nr_trace_closest (CNT_N ints)
+    GView* d_views = nullptr;           // view table of the current render call (NR_MAX_VIEWS)
+    uint32_t q_cap = 0;                 // padded pixels the path queues hold
     // persistent launch shapes
     int n_cu = 0;
     int shade_grid = 0;                 // workgroups of 256 for k_shade (grid-stride)
@@ -1358,6 +1395,7 @@ void nr_destroy(nr_ctx* c) {
     for (auto st : c->gstream) if (st) (void)hipStreamSynchronize(st);
     free_list(c->scene_mem);
     free_list(c->path_mem);
+    if (c->d_views) (void)hipFree(c->d_views);
     if (c->rb_stream) (void)hipStreamSynchronize(c->rb_stream);
     for (auto& v : c->views) {
         if (v.d_pattern) (void)hipFree(v.d_pattern);
@@ -1680,11 +1718,15 @@ int nr_aov_reset(nr_ctx* c, int view) {
     return NR_OK;
 }
 
-static int ensure_paths(nr_ctx* c, uint32_t n) {
-    if (n <= c->path_cap) return NR_OK;
+// Path state for n pixels and queues for npad padded pixels (whole 8x8 tiles).
+static int ensure_paths(nr_ctx* c, uint32_t n, uint32_t npad) {
+    if (n <= c->path_cap && npad <= c->q_cap) return NR_OK;
+    n = std::max(n, c->path_cap);
+    npad = std::max(npad, c->q_cap);
     HIPCK(hipStreamSynchronize(c->stream));
     free_list(c->path_mem);
     c->path_cap = 0;
+    c->q_cap = 0;
     PathPtrs& P = c->P;
     auto al = [&](void** p, size_t bytes) -> int {
         HIPCK(hipMalloc(p, bytes));
@@ -1700,8 +1742,9 @@ static int ensure_paths(nr_ctx* c, uint32_t n) {
     if (!e) e = al((void**)&P.nrm, sizeof(float4) * np);
     if (!e) e = al((void**)&P.draw, sizeof(uint32_t) * np);
     if (!e) e = al((void**)&P.hit, sizeof(GHit) * np);
-    if (!e) e = al((void**)&P.qact, sizeof(int32_t) * 2 * np);
-    if (!e) e = al((void**)&P.qmat, sizeof(int32_t) * NR_MAT_TYPES * np);
+    const size_t nq = (size_t)NR_SLOTS * npad;   // entries per queue
+    if (!e) e = al((void**)&P.qact, sizeof(int32_t) * 2 * nq);
+    if (!e) e = al((void**)&P.qmat, sizeof(int32_t) * NR_MAT_TYPES * nq);
     if (!e) e = al((void**)&P.counts, sizeof(int32_t) * CNT_N * NR_MAX_GROUPS);
     if (!e) e = al((void**)&P.finished, sizeof(uint8_t) * np);
     if (!e) e = al((void**)&P.nf, sizeof(uint32_t) * (size_t)n);
@@ -1710,13 +1753,15 @@ static int ensure_paths(nr_ctx* c, uint32_t n) {
     if (!e) e = al((void**)&P.sho, sizeof(float4) * np);
     if (!e) e = al((void**)&P.shd, sizeof(float4) * np);
     if (!e) e = al((void**)&P.shc, sizeof(float4) * np);
-    if (!e) e = al((void**)&P.qshd, sizeof(int32_t) * np);
+    if (!e) e = al((void**)&P.qshd, sizeof(int32_t) * nq);
     if (e) { free_list(c->path_mem); return e; }
     HIPCK(hipMemsetAsync(P.finished, 2, np, c->stream));
     HIPCK(hipMemsetAsync(P.counts, 0, sizeof(int32_t) * CNT_N * NR_MAX_GROUPS, c->stream));
     P.cap = n;
-    P.qcap = (uint32_t)np;
+    P.qcap = (uint32_t)nq;
+    P.err = P.counts + CNT_ERR;
     c->path_cap = n;
+    c->q_cap = npad;
     return NR_OK;
 }
 
@@ -1730,22 +1775,51 @@ static hipEvent_t ev_at(nr_ctx* c, size_t i) {
 }
 
 int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
-    if (!c || !d) return fail(NR_ERR_INVALID, "null argument");
+    const int32_t v = view;
+    return nr_render_views(c, &v, 1, d);
+}
+
+int nr_render_views(nr_ctx* c, const int32_t* view_ids, int n_views, const nr_render_desc* d) {
+    if (!c || !d || !view_ids) return fail(NR_ERR_INVALID, "null argument");
     if (!c->has_scene) return fail(NR_ERR_STATE, "nr_render before nr_scene_upload");
-    if (view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_STATE, "view has no camera");
-    View& v = c->views[view];
-    if (d->sample_end > v.cam.n_pattern) return fail(NR_ERR_INVALID, "sample index beyond samples_per_pixel");
+    if (n_views < 1 || n_views > NR_MAX_VIEWS) return fail(NR_ERR_INVALID, "1 to NR_MAX_VIEWS views per call");
+    std::vector<int32_t> ids(view_ids, view_ids + n_views);
+    for (int32_t id : ids)
+        if (id < 0 || id >= NR_MAX_VIEWS || !c->views[id].valid) return fail(NR_ERR_STATE, "view has no camera");
+    {
+        std::vector<int32_t> u(ids);
+        std::sort(u.begin(), u.end());
+        if (std::adjacent_find(u.begin(), u.end()) != u.end()) return fail(NR_ERR_INVALID, "a view appears twice");
+    }
+    for (int32_t id : ids)
+        if (d->sample_end > c->views[id].cam.n_pattern) return fail(NR_ERR_INVALID, "sample index beyond samples_per_pixel");
     if (d->mode != NR_MODE_PARITY && d->mode != NR_MODE_FAST) return fail(NR_ERR_INVALID, "unknown render mode");
     const bool fast = d->mode == NR_MODE_FAST;
     if (!fast && d->sample_begin < 1 && d->sample_end > d->sample_begin)
         return fail(NR_ERR_INVALID, "sample passes start at 1 (running mean of passes 1..n-1)");
     HIPCK(hipSetDevice(c->device));
-    uint32_t N = v.H * v.W;
-    if ((uint64_t)N * NR_SLOTS > (uint64_t)INT32_MAX) return fail(NR_ERR_LIMIT, "view too large for 32-bit path ids");
-    int e = ensure_paths(c, N);
+    // the views one after another in the tile enumeration and in the path arrays
+    GView hv[NR_MAX_VIEWS];
+    uint64_t tiles = 0, npix = 0;
+    for (int k = 0; k < n_views; k++) {
+        const View& v = c->views[ids[k]];
+        hv[k].cam = v.cam;
+        hv[k].tile0 = (uint32_t)tiles;
+        hv[k].pix0 = (uint32_t)npix;
+        hv[k].tiles_x = (v.W + 7) / 8;
+        hv[k].npix = v.W * v.H;
+        tiles += (uint64_t)hv[k].tiles_x * ((v.H + 7) / 8);
+        npix += (uint64_t)v.W * v.H;
+    }
+    if (tiles * 64 * NR_SLOTS > (uint64_t)INT32_MAX) return fail(NR_ERR_LIMIT, "views too large for 32-bit path ids");
+    const uint32_t N = (uint32_t)npix, T = (uint32_t)tiles;
+    int e = ensure_paths(c, N, T * 64);
     if (e) return e;
+    if (!c->d_views) HIPCK(hipMalloc((void**)&c->d_views, sizeof(GView) * NR_MAX_VIEWS));
+    HIPCK(hipStreamSynchronize(c->stream));              // kernels of the previous call read the table
+    HIPCK(hipMemcpy(c->d_views, hv, sizeof(GView) * n_views, hipMemcpyHostToDevice));
     const size_t lds = trav_lds_bytes();
-    dim3 gN((N + 255) / 256), gS(c->shade_grid);
+    dim3 gS(c->shade_grid);
     size_t ei = 0;
     float t_ext = 0.0f, t_shade = 0.0f, t_other = 0.0f, t_shadow = 0.0f;
     std::vector<std::pair<size_t, int>> marks;   // (event index of start, category)
@@ -1756,22 +1830,33 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
         marks.push_back({ei, cat});
         ei++;
     };
+    auto layer_of = [&](int k, int l) { return c->views[ids[k]].layer[l]; };
+    // views whose layers this call writes, locked in ascending view order
+    auto write_guards = [&]() {
+        std::vector<int32_t> order(ids);
+        std::sort(order.begin(), order.end());
+        std::vector<std::unique_ptr<LayerWrite>> w;
+        for (int32_t id : order) w.emplace_back(new LayerWrite(c, c->views[id]));
+        return w;
+    };
     hipEvent_t e0 = c->ev_begin, e1 = c->ev_end;
     if (c->timing) (void)hipEventRecord(e0, c->stream);
-    float* lb = v.layer[NR_LAYER_BEAUTY];
-    float* ld = v.layer[NR_LAYER_DIFFUSE];
-    float* ln = v.layer[NR_LAYER_WORLD_NORMAL];
     c->last_iterations = 0;
     if (d->sample_end > d->sample_begin && d->ray_depth == 0) {
-        LayerWrite w(c, v);
-        for (uint32_t s = d->sample_begin; s < d->sample_end; s++)
-            hipLaunchKernelGGL(k_fold_zero, gN, dim3(256), 0, c->stream, lb, ld, ln, N, fast ? s + 1 : s);
+        auto w = write_guards();
+        for (int k = 0; k < n_views; k++) {
+            const uint32_t n = hv[k].npix;
+            for (uint32_t s = d->sample_begin; s < d->sample_end; s++)
+                hipLaunchKernelGGL(k_fold_zero, dim3((n + 255) / 256), dim3(256), 0, c->stream, layer_of(k, NR_LAYER_BEAUTY),
+                                   layer_of(k, NR_LAYER_DIFFUSE), layer_of(k, NR_LAYER_WORLD_NORMAL), n, fast ? s + 1 : s);
+        }
     } else if (d->sample_end > d->sample_begin) {
-        // Stream groups: G bands of whole tile rows, each with its own queues,
-        // counters, stack-spill columns and stream.  Timing mode runs one group on
-        // the context stream so per-kernel times are not shared with another group.
-        const uint32_t tiles_x = (v.W + 7) / 8, bands = (v.H + 7) / 8;
-        const int G = c->timing ? 1 : (int)std::max(1u, std::min<uint32_t>((uint32_t)c->render_streams, bands));
+        // Stream groups: G contiguous ranges of whole tiles, each with its own
+        // queues, counters, stack-spill columns and stream.  Timing mode runs one
+        // group on the context stream so per-kernel times are not shared with
+        // another group.
+        const int G = c->timing ? 1 : (int)std::max(1u, std::min<uint32_t>((uint32_t)c->render_streams, T));
+        const uint32_t m = std::min<uint32_t>(NR_SLOTS, d->sample_end - d->sample_begin);
         struct Group {
             PathPtrs P;
             TraceArgs A;
@@ -1783,39 +1868,42 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
         const size_t lv = c->scene.stack_depth > LDS_STACK_LEVELS ? (size_t)(c->scene.stack_depth - LDS_STACK_LEVELS) : 0;
         {
             StackCfg all;
-            e = stack_spill(c, NR_SLOTS * N + 128u * NR_MAX_GROUPS, all);
+            e = stack_spill(c, NR_SLOTS * T * 64 + 128u * NR_MAX_GROUPS, all);
             if (e) return e;
             size_t spill_off = 0;
             for (int g = 0; g < G; g++) {
-                const uint32_t b0 = bands * g / G, b1 = bands * (g + 1) / G;
-                const uint32_t q0 = b0 * 8 * v.W, q1 = std::min(b1 * 8, v.H) * v.W;
+                const uint32_t T0 = (uint32_t)((uint64_t)T * g / G), T1 = (uint32_t)((uint64_t)T * (g + 1) / G);
+                const size_t q0 = (size_t)NR_SLOTS * T0 * 64;       // queue entries of the groups before
                 Group& gr = grp[g];
                 gr.P = c->P;
-                gr.P.qact = c->P.qact + (size_t)NR_SLOTS * q0;
-                gr.P.qmat = c->P.qmat + (size_t)NR_SLOTS * q0;
-                gr.P.qshd = c->P.qshd + (size_t)NR_SLOTS * q0;
+                gr.P.qact = c->P.qact + q0;
+                gr.P.qmat = c->P.qmat + q0;
+                gr.P.qshd = c->P.qshd + q0;
                 gr.P.counts = c->P.counts + CNT_N * g;
-                gr.A = TraceArgs{d->seed, (uint32_t)((uint64_t)v.W * v.H), 0u, d->ray_depth, d->do_rr,
-                                 d->rr_start_bounce, d->rr_survival, d->sample_end,
-                                 b0 * tiles_x * 64, b1 * tiles_x * 64, q1 - q0, fast ? 1 : 0};
-                const uint32_t cap_g = (NR_SLOTS * gr.A.npix + 127) / 128 * 128;
+                gr.A = TraceArgs{d->seed, 0u, d->ray_depth, d->do_rr, d->rr_start_bounce, d->rr_survival,
+                                 d->sample_end, T0 * 64, T1 * 64, fast ? 1 : 0, (uint32_t)n_views, c->d_views};
+                const uint32_t cap_g = (NR_SLOTS * (T1 - T0) * 64 + 127) / 128 * 128;
                 gr.SC.spill = all.spill + spill_off;
                 gr.SC.stride = cap_g;
                 spill_off += lv * cap_g;
                 gr.st = G == 1 ? c->stream : c->gstream[g];
-                gr.bound = (int64_t)gr.A.npix * std::min<uint32_t>(NR_SLOTS, d->sample_end - d->sample_begin);
+                gr.bound = (int64_t)(T1 - T0) * 64 * m;
             }
         }
         mark(2);
-        hipLaunchKernelGGL(k_pack_acc, gN, dim3(256), 0, c->stream, lb, ld, ln, c->P.acc, N, c->P.cap);
+        for (int k = 0; k < n_views; k++)
+            hipLaunchKernelGGL(k_pack_acc, dim3((hv[k].npix + 255) / 256), dim3(256), 0, c->stream,
+                               layer_of(k, NR_LAYER_BEAUTY), layer_of(k, NR_LAYER_DIFFUSE),
+                               layer_of(k, NR_LAYER_WORLD_NORMAL), c->P.acc + hv[k].pix0, hv[k].npix, c->P.cap);
         if (G > 1) {
             HIPCK(hipEventRecord(c->ev_packed, c->stream));
             for (int g = 0; g < G; g++) HIPCK(hipStreamWaitEvent(grp[g].st, c->ev_packed, 0));
         }
         for (int g = 0; g < G; g++) {
             const uint32_t nt = grp[g].A.t1 - grp[g].A.t0;
-            hipLaunchKernelGGL(k_start, dim3((nt + 255) / 256), dim3(256), 0, grp[g].st, v.cam, grp[g].P,
-                               d->sample_begin, grp[g].A);
+            hipLaunchKernelGGL(k_clear_counts, dim3(1), dim3(64), 0, grp[g].st, grp[g].P.counts);
+            hipLaunchKernelGGL(k_start, dim3((nt + 255) / 256), dim3(256), 0, grp[g].st, grp[g].P, d->sample_begin,
+                               grp[g].A);
         }
         // The occupied path slots (samples started and not folded) never grow
         // from one iteration to the next and bound the live paths, so a count
@@ -1860,8 +1948,7 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
                 }
                 mark(2);
                 const uint32_t nt = A.t1 - A.t0;
-                hipLaunchKernelGGL(k_regen, dim3((nt + REGEN_BLOCK - 1) / REGEN_BLOCK), dim3(REGEN_BLOCK), 0, st,
-                                   v.cam, P, A);
+                hipLaunchKernelGGL(k_regen, dim3((nt + REGEN_BLOCK - 1) / REGEN_BLOCK), dim3(REGEN_BLOCK), 0, st, P, A);
                 HIPCK(hipGetLastError());
                 HIPCK(hipMemcpyAsync(live + (it & 1), P.counts + CNT_OCC, sizeof(int32_t), hipMemcpyDeviceToHost, st));
                 HIPCK(hipEventRecord(c->gev[g][it & 1], st));
@@ -1878,8 +1965,11 @@ int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
             }
         }
         mark(2);
-        LayerWrite w(c, v);
-        hipLaunchKernelGGL(k_unpack_acc, gN, dim3(256), 0, c->stream, c->P.acc, lb, ld, ln, N, c->P.cap);
+        auto w = write_guards();
+        for (int k = 0; k < n_views; k++)
+            hipLaunchKernelGGL(k_unpack_acc, dim3((hv[k].npix + 255) / 256), dim3(256), 0, c->stream,
+                               c->P.acc + hv[k].pix0, layer_of(k, NR_LAYER_BEAUTY), layer_of(k, NR_LAYER_DIFFUSE),
+                               layer_of(k, NR_LAYER_WORLD_NORMAL), hv[k].npix, c->P.cap);
     }
     HIPCK(hipGetLastError());
     if (c->timing) {

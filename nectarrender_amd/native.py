@@ -40,6 +40,7 @@ def lib() -> C.CDLL:
         "nr_camera_set": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrCameraDesc), C.c_uint32]),
         "nr_aov_reset": (C.c_int, [vp, C.c_int]),
         "nr_render": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrRenderDesc)]),
+        "nr_render_views": (C.c_int, [vp, C.POINTER(C.c_int32), C.c_int, C.POINTER(abi.NrRenderDesc)]),
         "nr_aov_read": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_size_t]),
         "nr_aov_device_ptr": (vp, [vp, C.c_int, C.c_int]),
         "nr_trace_closest": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int32, C.POINTER(abi.NrHit)]),
@@ -120,6 +121,11 @@ class Context:
 
     def render(self, view: int, desc: abi.NrRenderDesc):
         _check(lib().nr_render(self._h, view, C.byref(desc)))
+
+    def render_views(self, views, desc: abi.NrRenderDesc):
+        """The same passes for several views in one streaming pipeline (nr_render_views)."""
+        ids = (C.c_int32 * len(views))(*map(int, views))
+        _check(lib().nr_render_views(self._h, ids, len(views), C.byref(desc)))
 
     def sync(self):
         _check(lib().nr_sync(self._h))

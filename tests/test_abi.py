@@ -20,7 +20,8 @@ def declared():
 
 def test_header_declares_entry_points():
     names = declared()
-    for n in ("nr_create", "nr_destroy", "nr_scene_upload", "nr_camera_set", "nr_render", "nr_trace_closest",
+    for n in ("nr_create", "nr_destroy", "nr_scene_upload", "nr_camera_set", "nr_render", "nr_render_views",
+              "nr_trace_closest",
               "nr_aov_read", "nr_post", "nr_last_error", "nr_view_size", "nr_unet_create", "nr_unet_forward",
               "nr_unet_train_step", "nr_unet_gradients", "nr_denoise", "nr_viewport", "nr_pick",
               "nr_selection_mask", "nr_overlay_device", "nr_readback", "nr_set_build_mode", "nr_set_trace_layout", "nr_mesh_tree",
@@ -50,6 +51,7 @@ def test_null_arguments_fail_cleanly_without_device():
     assert L.nr_scene_upload(None, None) == 1
     assert b"null" in L.nr_last_error()
     assert L.nr_render(None, 0, None) == 1
+    assert L.nr_render_views(None, None, 0, None) == 1
     assert L.nr_trace_closest(None, None, 0, None) == 1
     assert L.nr_post(None, 0, None, None, None) == 1
     assert L.nr_set_timing(None, 1) == 1

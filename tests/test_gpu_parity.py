@@ -159,3 +159,36 @@ def test_render_same_for_any_stream_group_count(ctx, oracle, groups):
         ctx.set_render_streams(2)
     for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
         _assert_same(g[k], o[k], name_)
+
+
+@pytest.mark.parametrize("mode", [abi.MODE_PARITY, abi.MODE_FAST])
+def test_render_views_in_one_pipeline_equals_one_at_a_time(ctx, mode):
+    """16 views of different sizes rendered by one nr_render_views call (every launch serves all of them)
+    equal the same views rendered one nr_render call at a time, bit for bit; so does a subset given out of
+    order."""
+    s, cams, rp = scenes.config3(scale=0.06)
+    ctx.upload(s)
+    p0 = 0 if mode == abi.MODE_FAST else 1
+    for v, cam in enumerate(cams):
+        cam.samples_per_pixel = 5
+        ctx.set_camera(v, cam, rp["seed"] + v)
+    d = abi.render_desc(p0, 5, rp["depth"], rp["seed"], mode=mode)
+    layers = (abi.LAYER_BEAUTY, abi.LAYER_DIFFUSE, abi.LAYER_WORLD_NORMAL)
+    single = []
+    for v in range(len(cams)):
+        ctx.reset(v)
+        ctx.render(v, d)
+        single.append([ctx.read(v, L) for L in layers])
+    for v in range(len(cams)):
+        ctx.reset(v)
+    ctx.render_views(list(range(len(cams))), d)
+    for v in range(len(cams)):
+        for k, L in enumerate(layers):
+            _assert_same(ctx.read(v, L), single[v][k], f"view {v} layer {L}")
+    ctx.reset(3)
+    ctx.reset(0)
+    ctx.render_views([3, 0], d)
+    for v in (3, 0):
+        _assert_same(ctx.read(v, abi.LAYER_BEAUTY), single[v][0], f"subset view {v}")
+    with pytest.raises(RuntimeError):
+        ctx.render_views([2, 2], d)
