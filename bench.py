@@ -105,6 +105,26 @@ def quality_section(log, ctx=None, scene=None, end_to_end=False):
         out["denoised_s"] = quality.time_to_target(pts)
         out["denoised"] = [[k, round(tr, 5), round(td, 5), round(e, 5)] for k, tr, td, e in den]
         # denoiser throughput at this size (HIP events around the U-Net launch sequence)
+        if end_to_end:
+            # 4 samples + U-Net with the per-tile fallback (64^2 tiles whose error estimate exceeds 0.02
+            # restart with 64 samples and are denoised again), timed end to end
+            from nectarrender_amd.adaptive import render_denoised_adaptive
+            H, W = int(cam.resolution[0]), int(cam.resolution[1])
+            frame = torch.empty((3, H, W), dtype=torch.float32, device="cuda")
+            render_denoised_adaptive(ctx, unet, 0, cam, rp, quality.display_desc(abi), frame, seed=quality.TEST_SEED)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            info = render_denoised_adaptive(ctx, unet, 0, cam, rp, quality.display_desc(abi), frame,
+                                            seed=quality.TEST_SEED)
+            torch.cuda.synchronize()
+            ta = time.perf_counter() - t0
+            d = quality._display(ctx, abi, torch, frame, H, W)
+            out["denoised_fallback"] = {
+                "seconds": round(ta, 5), "rmse": round(quality.rmse(d, ref_disp), 5),
+                "fallback_tiles": info["fallback_tiles"], "tiles": info["tiles"],
+                "max_tile_rmse_estimate": round(float(info["tile_rmse"].max()), 5),
+                "note": "4 samples + U-Net; two 4-sample plain renders judge each 64x64 tile; tiles above 0.02 "
+                        "re-render with 64 samples and the U-Net runs again"}
         ms, flops = unet.last_stats()
         out["denoiser"] = {"ms": round(ms, 3), "tflops": round(flops / ms / 1e9, 2), "peak_tflops": 2500.0,
                            "frac": round(flops / ms / 1e9 / 2500.0, 4), "flops_per_frame": flops,

@@ -262,6 +262,15 @@ int nr_render(nr_ctx* ctx, int view, const nr_render_desc* desc);
  * before it starts. */
 int nr_render_views(nr_ctx* ctx, const int32_t* views, int n_views, const nr_render_desc* desc);
 
+/* nr_render restricted per tile: pixel (x, y) renders only the passes of
+ * [sample_begin, sample_end) below tile_end[(y / tile_size) * cols + x / tile_size]
+ * (host array, cols = ceil(W / tile_size), rows = ceil(H / tile_size); tile_size
+ * a power of two in [8, 4096]).  A tile whose limit is at most sample_begin keeps
+ * its AOVs; rendering from the first pass (0 in NR_MODE_FAST, 1 otherwise)
+ * restarts a tile's running mean.  The per-tile fallback of a denoised frame to
+ * more samples; no counterpart in the reference. */
+int nr_render_tiles(nr_ctx* ctx, int view, const nr_render_desc* desc, const uint32_t* tile_end, int tile_size);
+
 /* Image size of a view (H = resolution[0], W = resolution[1]). */
 int nr_view_size(nr_ctx* ctx, int view, int32_t* H, int32_t* W);
 
@@ -283,6 +292,19 @@ int nr_trace_closest_device(nr_ctx* ctx, const float* d_rays, int32_t n, nr_hit*
  * csrc/data/src/stream.cu:146-159).  If out_f32 is
  * non-NULL the display-referred float image (before x255) is written too. */
 int nr_post(nr_ctx* ctx, int view, const nr_post_desc* desc, uint8_t* out_u8, float* out_f32);
+
+/* Per-tile error estimate of a denoised frame (all device pointers, planar
+ * 3 x H x W display-referred fp32, e.g. from nr_post_device): for each
+ * tile_size^2 tile, the mean over its pixels and channels of
+ * (D - J1)^2 - (J1 - J2)^2 / 2, where J1 and J2 are two plain renders of the
+ * same view independent of D and of each other.  Its expectation is D's mean
+ * squared error plus the judges' own display-space bias (tonemapping a
+ * few-sample render is biased low), so it errs towards more fallback; it can
+ * come out negative.  d_out receives rows x cols floats, row major.
+ * Synchronous.  No counterpart in the reference: the plain path tracer judging
+ * the denoiser. */
+int nr_tile_error(nr_ctx* ctx, const float* d_denoised, const float* d_judge1, const float* d_judge2, int32_t H,
+                  int32_t W, int32_t tile_size, float* d_out);
 
 /* The same post-processing of any planar 3 x H x W fp32 device image (a
  * denoised frame, an accumulated reference); outputs are device pointers,

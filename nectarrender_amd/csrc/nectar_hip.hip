@@ -156,7 +156,16 @@ struct TraceArgs {
     int32_t fast;       // NR_MODE_FAST: 64-bit sample subsequence (pixel, sample), NEE + MIS
     uint32_t n_views;
     const GView* views; // device table, n_views entries
+    // optional per-tile sample limit (one view): pixel (x, y) renders passes below
+    // min(s_end, tile_end[(y >> tile_shift) * tile_cols + (x >> tile_shift)])
+    const uint32_t* tile_end;
+    uint32_t tile_shift, tile_cols;
 };
+
+__device__ __forceinline__ uint32_t pixel_end(const TraceArgs& A, uint32_t x, uint32_t y) {
+    if (!A.tile_end) return A.s_end;
+    return min(A.s_end, A.tile_end[(y >> A.tile_shift) * A.tile_cols + (x >> A.tile_shift)]);
+}
 
 // ---------------------------------------------------------------------------
 // wave-level queue append: one atomic per wave, lanes get consecutive slots
@@ -545,6 +554,40 @@ __device__ __forceinline__ void fold(float4* acc, float4 v, float a, float b) {
     *acc = m;
 }
 
+// Per-tile error estimate of a denoised display image D against two independent
+// plain renders J1, J2 of the same pixels (display images, planar 3 x H x W):
+// the mean over the tile's pixels and channels of (D - J1)^2 - (J1 - J2)^2 / 2.
+// With J1 independent of D, E[(D - J1)^2] = MSE(D) + Var(J1) + bias(J1)^2 +
+// 2 E[D - T] bias(J1) (T the converged display image) and E[(J1 - J2)^2] / 2 =
+// Var(J1): an unbiased MSE estimate where the judges are unbiased, and above
+// it by the judges' display-space bias (the display transform is concave, so a
+// few-sample render displays darker than its expectation).  One workgroup per
+// tile; fixed-order tree reduction (deterministic).
+__global__ void __launch_bounds__(256) k_tile_error(const float* __restrict__ D, const float* __restrict__ J1,
+                                                    const float* __restrict__ J2, uint32_t H, uint32_t W,
+                                                    uint32_t ts, uint32_t cols, float* __restrict__ out) {
+    __shared__ float red[256];
+    const uint32_t tile = blockIdx.x, tx = tile % cols, ty = tile / cols;
+    const uint32_t x0 = tx * ts, y0 = ty * ts;
+    const uint32_t w = min(ts, W - x0), h = min(ts, H - y0);
+    const size_t HW = (size_t)H * W;
+    float acc = 0.0f;
+    for (uint32_t k = threadIdx.x; k < w * h; k += blockDim.x) {
+        const size_t i = (size_t)(y0 + k / w) * W + x0 + k % w;
+        for (int c = 0; c < 3; c++) {
+            const float d = D[c * HW + i], a = J1[c * HW + i], b = J2[c * HW + i];
+            acc += (d - a) * (d - a) - 0.5f * (a - b) * (a - b);
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[tile] = red[0] / (float)(3 * w * h);
+}
+
 // 4-wide layout of a mesh tree (fast mode): wide node i = the grandchildren of
 // binary node i, in left-to-right order.
 __global__ void __launch_bounds__(256) k_wide_nodes(const GNode* __restrict__ nodes, int32_t n, GWNode* __restrict__ out) {
@@ -605,10 +648,11 @@ __global__ void __launch_bounds__(256) k_unpack_acc(const float4* __restrict__ a
 // sample rank.  The group's counters were cleared by k_clear_counts.
 __global__ void __launch_bounds__(256) k_start(PathPtrs P, uint32_t s, TraceArgs A) {
     uint32_t t = A.t0 + blockIdx.x * blockDim.x + threadIdx.x, v = 0, x = 0, y = 0;
-    const uint32_t m = min((uint32_t)NR_SLOTS, A.s_end - s);       // samples started per pixel
     const bool valid = t < A.t1 && tile_pixel(A, t, v, x, y);
-    uint32_t g = 0;
+    uint32_t g = 0, m = 0;                                          // samples started for the pixel
     if (valid) {
+        const uint32_t end = pixel_end(A, x, y);
+        m = end > s ? min((uint32_t)NR_SLOTS, end - s) : 0u;
         const GView& V = A.views[v];
         g = V.pix0 + y * V.cam.W + x;
         P.nf[g] = s;
@@ -919,6 +963,7 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) 
     if (t < A.t1 && tile_pixel(A, t, v, x, y)) {
         const GView& V = A.views[v];
         const uint32_t i = V.pix0 + y * V.cam.W + x;
+        const uint32_t end = pixel_end(A, x, y);
         uint32_t nf = P.nf[i], ns = P.ns[i];
 #pragma unroll
         for (int k = 0; k < NR_SLOTS; k++) {
@@ -934,7 +979,7 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) 
         }
 #pragma unroll
         for (int k = 0; k < NR_SLOTS; k++) {
-            if (ns >= A.s_end || ns - nf >= NR_SLOTS) break;    // slot ns % NR_SLOTS is idle when ns - nf < NR_SLOTS
+            if (ns >= end || ns - nf >= NR_SLOTS) break;        // slot ns % NR_SLOTS is idle when ns - nf < NR_SLOTS
             started[k] = start_path(V, P, x, y, ns, A);
             n_start = k + 1;
             ns++;
@@ -1126,6 +1171,8 @@ struct nr_ctx {
     uint32_t path_cap = 0;
     int32_t* d_err = nullptr;           // counters of nr_trace_closest (CNT_N ints)
     GView* d_views = nullptr;           // view table of the current render call (NR_MAX_VIEWS)
+    uint32_t* d_tile_end = nullptr;     // per-tile sample limits of nr_render_tiles
+    size_t tile_end_cap = 0;
     uint32_t q_cap = 0;                 // padded pixels the path queues hold
     // persistent launch shapes
     int n_cu = 0;
@@ -1396,6 +1443,7 @@ void nr_destroy(nr_ctx* c) {
     free_list(c->scene_mem);
     free_list(c->path_mem);
     if (c->d_views) (void)hipFree(c->d_views);
+    if (c->d_tile_end) (void)hipFree(c->d_tile_end);
     if (c->rb_stream) (void)hipStreamSynchronize(c->rb_stream);
     for (auto& v : c->views) {
         if (v.d_pattern) (void)hipFree(v.d_pattern);
@@ -1774,12 +1822,45 @@ static hipEvent_t ev_at(nr_ctx* c, size_t i) {
     return c->ev[i];
 }
 
+// per-tile sample limits of nr_render_tiles (host table, copied to c->d_tile_end)
+struct TileLimitHost {
+    const uint32_t* end;
+    size_t n;
+    uint32_t shift, cols;
+};
+struct TileLimit {
+    const uint32_t* d_end;
+    uint32_t shift, cols;
+};
+
+static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr_render_desc* d,
+                       const TileLimitHost* tlh);
+
 int nr_render(nr_ctx* c, int view, const nr_render_desc* d) {
     const int32_t v = view;
-    return nr_render_views(c, &v, 1, d);
+    return render_impl(c, &v, 1, d, nullptr);
 }
 
 int nr_render_views(nr_ctx* c, const int32_t* view_ids, int n_views, const nr_render_desc* d) {
+    return render_impl(c, view_ids, n_views, d, nullptr);
+}
+
+int nr_render_tiles(nr_ctx* c, int view, const nr_render_desc* d, const uint32_t* tile_end, int tile_size) {
+    if (!c || !d || !tile_end) return fail(NR_ERR_INVALID, "null argument");
+    if (view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_STATE, "view has no camera");
+    if (tile_size < 8 || tile_size > 4096 || (tile_size & (tile_size - 1)))
+        return fail(NR_ERR_INVALID, "tile_size must be a power of two in [8, 4096]");
+    uint32_t shift = 0;
+    while ((1 << shift) < tile_size) shift++;
+    const View& v = c->views[view];
+    TileLimitHost t{tile_end, (size_t)((v.H + tile_size - 1) >> shift) * ((v.W + tile_size - 1) >> shift), shift,
+                    (v.W + tile_size - 1) >> shift};
+    const int32_t id = view;
+    return render_impl(c, &id, 1, d, &t);
+}
+
+static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr_render_desc* d,
+                       const TileLimitHost* tlh) {
     if (!c || !d || !view_ids) return fail(NR_ERR_INVALID, "null argument");
     if (!c->has_scene) return fail(NR_ERR_STATE, "nr_render before nr_scene_upload");
     if (n_views < 1 || n_views > NR_MAX_VIEWS) return fail(NR_ERR_INVALID, "1 to NR_MAX_VIEWS views per call");
@@ -1816,8 +1897,22 @@ int nr_render_views(nr_ctx* c, const int32_t* view_ids, int n_views, const nr_re
     int e = ensure_paths(c, N, T * 64);
     if (e) return e;
     if (!c->d_views) HIPCK(hipMalloc((void**)&c->d_views, sizeof(GView) * NR_MAX_VIEWS));
-    HIPCK(hipStreamSynchronize(c->stream));              // kernels of the previous call read the table
+    HIPCK(hipStreamSynchronize(c->stream));              // kernels of the previous call read the tables
     HIPCK(hipMemcpy(c->d_views, hv, sizeof(GView) * n_views, hipMemcpyHostToDevice));
+    TileLimit tl_dev{nullptr, 0u, 0u};
+    const TileLimit* tl = nullptr;
+    if (tlh) {
+        if (tlh->n > c->tile_end_cap) {
+            if (c->d_tile_end) HIPCK(hipFree(c->d_tile_end));
+            c->d_tile_end = nullptr;
+            c->tile_end_cap = 0;
+            HIPCK(hipMalloc((void**)&c->d_tile_end, sizeof(uint32_t) * tlh->n));
+            c->tile_end_cap = tlh->n;
+        }
+        HIPCK(hipMemcpy(c->d_tile_end, tlh->end, sizeof(uint32_t) * tlh->n, hipMemcpyHostToDevice));
+        tl_dev = TileLimit{c->d_tile_end, tlh->shift, tlh->cols};
+        tl = &tl_dev;
+    }
     const size_t lds = trav_lds_bytes();
     dim3 gS(c->shade_grid);
     size_t ei = 0;
@@ -1881,7 +1976,8 @@ int nr_render_views(nr_ctx* c, const int32_t* view_ids, int n_views, const nr_re
                 gr.P.qshd = c->P.qshd + q0;
                 gr.P.counts = c->P.counts + CNT_N * g;
                 gr.A = TraceArgs{d->seed, 0u, d->ray_depth, d->do_rr, d->rr_start_bounce, d->rr_survival,
-                                 d->sample_end, T0 * 64, T1 * 64, fast ? 1 : 0, (uint32_t)n_views, c->d_views};
+                                 d->sample_end, T0 * 64, T1 * 64, fast ? 1 : 0, (uint32_t)n_views, c->d_views,
+                                 tl ? tl->d_end : nullptr, tl ? tl->shift : 0u, tl ? tl->cols : 0u};
                 const uint32_t cap_g = (NR_SLOTS * (T1 - T0) * 64 + 127) / 128 * 128;
                 gr.SC.spill = all.spill + spill_off;
                 gr.SC.stride = cap_g;
@@ -2147,6 +2243,20 @@ int nr_post_device(nr_ctx* c, const float* img, int32_t H, int32_t W, const nr_p
     HIPCK(hipSetDevice(c->device));
     const uint32_t HW = (uint32_t)H * (uint32_t)W;
     hipLaunchKernelGGL(k_post, dim3((HW + 255) / 256), dim3(256), 0, c->stream, img, HW, post_args(d), d8, df);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(c->stream));
+    return NR_OK;
+}
+
+int nr_tile_error(nr_ctx* c, const float* d_denoised, const float* d_judge1, const float* d_judge2, int32_t H,
+                  int32_t W, int32_t tile_size, float* d_out) {
+    if (!c || !d_denoised || !d_judge1 || !d_judge2 || !d_out) return fail(NR_ERR_INVALID, "null argument");
+    if (H <= 0 || W <= 0 || (uint64_t)H * W > (1ull << 31)) return fail(NR_ERR_INVALID, "bad image size");
+    if (tile_size < 1 || tile_size > 4096) return fail(NR_ERR_INVALID, "tile_size must be in [1, 4096]");
+    HIPCK(hipSetDevice(c->device));
+    const uint32_t cols = (W + tile_size - 1) / tile_size, rows = (H + tile_size - 1) / tile_size;
+    hipLaunchKernelGGL(k_tile_error, dim3(cols * rows), dim3(256), 0, c->stream, d_denoised, d_judge1, d_judge2,
+                       (uint32_t)H, (uint32_t)W, (uint32_t)tile_size, cols, d_out);
     HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(c->stream));
     return NR_OK;

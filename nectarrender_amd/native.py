@@ -41,6 +41,8 @@ def lib() -> C.CDLL:
         "nr_aov_reset": (C.c_int, [vp, C.c_int]),
         "nr_render": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrRenderDesc)]),
         "nr_render_views": (C.c_int, [vp, C.POINTER(C.c_int32), C.c_int, C.POINTER(abi.NrRenderDesc)]),
+        "nr_render_tiles": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrRenderDesc), C.POINTER(C.c_uint32), C.c_int]),
+        "nr_tile_error": (C.c_int, [vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, vp]),
         "nr_aov_read": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_size_t]),
         "nr_aov_device_ptr": (vp, [vp, C.c_int, C.c_int]),
         "nr_trace_closest": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int32, C.POINTER(abi.NrHit)]),
@@ -121,6 +123,17 @@ class Context:
 
     def render(self, view: int, desc: abi.NrRenderDesc):
         _check(lib().nr_render(self._h, view, C.byref(desc)))
+
+    def render_tiles(self, view: int, desc: abi.NrRenderDesc, tile_end: np.ndarray, tile_size: int):
+        """nr_render with a per-tile sample limit (rows x cols uint32, row major)."""
+        t = np.ascontiguousarray(tile_end, dtype=np.uint32)
+        _check(lib().nr_render_tiles(self._h, view, C.byref(desc), t.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                     int(tile_size)))
+
+    def tile_error(self, d_denoised: int, d_judge1: int, d_judge2: int, H: int, W: int, tile_size: int, d_out: int):
+        """Per-tile MSE estimate of a denoised display image against two independent plain ones (device ptrs)."""
+        _check(lib().nr_tile_error(self._h, C.c_void_p(d_denoised), C.c_void_p(d_judge1), C.c_void_p(d_judge2),
+                                   H, W, tile_size, C.c_void_p(d_out)))
 
     def render_views(self, views, desc: abi.NrRenderDesc):
         """The same passes for several views in one streaming pipeline (nr_render_views)."""

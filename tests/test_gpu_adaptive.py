@@ -1,0 +1,125 @@
+"""Per-tile fallback of a denoised frame to more samples: nr_render_tiles (per-tile sample limits),
+nr_tile_error (the plain path tracer judging the denoiser) and the pipeline that combines them
+(nectarrender_amd.adaptive)."""
+import numpy as np
+import pytest
+
+from nectarrender_amd import abi, scenes
+from nectarrender_amd.quality import display_desc, with_spp
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def test_tile_error_formula(ctx):
+    """Mean over each tile (ragged at the edges) of (D - J1)^2 - (J1 - J2)^2 / 2, against numpy."""
+    rng = np.random.default_rng(3)
+    H, W, ts = 100, 150, 32
+    D, J1, J2 = (rng.random((3, H, W), dtype=np.float32) for _ in range(3))
+    dev = [torch.from_numpy(a).cuda() for a in (D, J1, J2)]
+    rows, cols = (H + ts - 1) // ts, (W + ts - 1) // ts
+    out = torch.empty(rows * cols, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    ctx.tile_error(dev[0].data_ptr(), dev[1].data_ptr(), dev[2].data_ptr(), H, W, ts, out.data_ptr())
+    got = out.cpu().numpy().reshape(rows, cols)
+    e = (D.astype(np.float64) - J1) ** 2 - 0.5 * (J1.astype(np.float64) - J2) ** 2
+    for r in range(rows):
+        for c in range(cols):
+            ref = e[:, r * ts:(r + 1) * ts, c * ts:(c + 1) * ts].mean()
+            assert abs(got[r, c] - ref) <= 1e-5 * max(1.0, abs(ref)), (r, c, got[r, c], ref)
+
+
+@pytest.mark.parametrize("mode", [abi.MODE_PARITY, abi.MODE_FAST])
+def test_render_tiles_restarts_or_keeps_each_tile(ctx, mode):
+    """Tiles limited to 0 keep their k-sample AOVs; tiles rendered again from the first pass hold exactly
+    what a full K-sample render of the frame holds there."""
+    s, cam, rp = scenes.mixed_scene(spp=16, res=96)
+    ctx.upload(s)
+    p0 = 0 if mode == abi.MODE_FAST else 1
+    K, k, ts = 12, 3, 32
+    ctx.set_camera(0, with_spp(cam, K + p0), 5)
+    ctx.reset(0)
+    ctx.render(0, abi.render_desc(p0, K + p0, rp["depth"], 5, mode=mode))
+    full = ctx.read(0)
+    ctx.reset(0)
+    ctx.render(0, abi.render_desc(p0, k + p0, rp["depth"], 5, mode=mode))
+    part = ctx.read(0)
+    ends = np.zeros((3, 3), np.uint32)
+    ends[0, 1] = ends[2, 2] = ends[1, 0] = K + p0
+    ctx.render_tiles(0, abi.render_desc(p0, K + p0, rp["depth"], 5, mode=mode), ends, ts)
+    got = ctx.read(0)
+    for r in range(3):
+        for c in range(3):
+            sl = (slice(None), slice(r * ts, (r + 1) * ts), slice(c * ts, (c + 1) * ts))
+            want = full[sl] if ends[r, c] else part[sl]
+            assert np.array_equal(got[sl], want, equal_nan=True), (r, c)
+
+
+def _setup(ctx, res=128):
+    s, cam, rp = scenes.config5(n_spheres=60, res=(res, res), spp=1024)
+    ctx.upload(s)
+    return s, cam, rp
+
+
+@pytest.fixture(scope="module")
+def unet():
+    from nectarrender_amd.denoise import WEIGHTS, UNet
+    import os
+    if not os.path.exists(WEIGHTS):
+        pytest.skip("no trained weights")
+    u = UNet(0).load()
+    yield u
+    u.close()
+
+
+def test_tile_error_tracks_the_true_error(ctx, unet):
+    """On a 128^2 frame the frame-mean estimated MSE of the 4-sample denoised image is at least its MSE
+    against a 1023-sample render and at most 4 times it: display-referred judges add their own
+    tonemapping bias (E[f(J)] != f(E[J]) for the concave display transform f), so the estimate errs on
+    the side of more fallback."""
+    from nectarrender_amd.adaptive import render_denoised_adaptive
+    s, cam, rp = _setup(ctx)
+    H, W = 128, 128
+    post = display_desc(abi)
+    out = torch.empty((3, H, W), dtype=torch.float32, device="cuda")
+    info = render_denoised_adaptive(ctx, unet, 0, cam, rp, post, out, k=4, threshold=1e9, tile=32)
+    assert info["fallback_tiles"] == 0
+    ref_img = torch.empty((3, H, W), dtype=torch.float32, device="cuda")
+    ctx.set_camera(1, with_spp(cam, 1024), 777)
+    ctx.reset(1)
+    ctx.render(1, abi.render_desc(1, 1024, rp["depth"], 777))
+    ctx.post_device(ctx.aov_device_ptr(1, abi.LAYER_BEAUTY), H, W, post, 0, ref_img.data_ptr())
+    den = torch.empty_like(ref_img)
+    ctx.post_device(out.data_ptr(), H, W, post, 0, den.data_ptr())
+    true_mse = float(((den.double() - ref_img.double()) ** 2).mean())
+    est_mse = float(np.mean(info["tile_rmse"] ** 2))
+    print(f"estimated RMSE {np.sqrt(est_mse):.4f}, measured against 1023 samples {np.sqrt(true_mse):.4f}")
+    assert 0.8 * true_mse < est_mse < 4.0 * true_mse
+
+
+def test_fallback_threshold_extremes(ctx, unet):
+    """Threshold 0: every tile falls back, and the frame is the U-Net on a full fallback-sample render.
+    A huge threshold: nothing falls back, and the frame is the U-Net on the k-sample render."""
+    from nectarrender_amd.adaptive import render_denoised_adaptive
+    s, cam, rp = _setup(ctx, res=96)
+    H, W = 96, 96
+    post = display_desc(abi)
+    out = torch.empty((3, H, W), dtype=torch.float32, device="cuda")
+    ref = torch.empty_like(out)
+    info = render_denoised_adaptive(ctx, unet, 0, cam, rp, post, out, k=2, threshold=-1.0, tile=32, fallback=8,
+                                    seed=9)
+    assert info["fallback_tiles"] == info["tiles"] == 9
+    ctx.set_camera(2, with_spp(cam, 9), 9)
+    ctx.reset(2)
+    ctx.render(2, abi.render_desc(1, 9, rp["depth"], 9))
+    unet.denoise_view(ctx, 2, ref)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    info = render_denoised_adaptive(ctx, unet, 0, cam, rp, post, out, k=2, threshold=1e9, tile=32, seed=9)
+    assert info["fallback_tiles"] == 0
+    ctx.set_camera(2, with_spp(cam, 3), 9)
+    ctx.reset(2)
+    ctx.render(2, abi.render_desc(1, 3, rp["depth"], 9))
+    unet.denoise_view(ctx, 2, ref)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
