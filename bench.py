@@ -192,6 +192,7 @@ def main():
     ap.add_argument("--no-quality", action="store_true", help="skip the seconds-to-RMSE measurement")
     ap.add_argument("--streams", type=int, default=0,
                     help="stream groups per render (1..4; default: the library's default)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default tuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (a kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -225,8 +226,14 @@ def main():
         ctx.upload(scene)
         ctx.sync()
         build[name] = round(time.perf_counter() - tb, 4)
+    tuning = None
     if args.streams:
         ctx.set_render_streams(args.streams)
+    elif not args.no_autotune:
+        # the library's autotuner on 8 MIS passes of this workload: path slots per pixel, stream groups,
+        # shading grid (the result does not depend on them)
+        ctx.set_camera(0, with_spp(cam, 8), rp["seed"])
+        tuning = ctx.autotune(0, abi.render_desc(0, 8, rp["depth"], rp["seed"], mode=abi.MODE_FAST))
 
     def measure(mode):
         """K timed passes (fast mode: 0..K-1 of a K-sample pattern; the reference's estimator: 1..K of
@@ -282,6 +289,7 @@ def main():
         "image_mean": float(img.mean()),
         "scene_upload": dict(build, triangles=1_003_520),
         "roofline": roof,
+        "tuning": tuning or ctx.get_tuning(),
         "parity_estimator": {"value": round(par_value, 3), "unit": "Mpaths/s",
                              "ms_per_step": round(par_dt / K * 1e3, 4),
                              "note": "the reference's estimator (passes 1..K, bit-exact with the oracle)",

@@ -369,6 +369,25 @@ int nr_readback(nr_ctx* ctx, int view, const nr_readback_desc* desc, uint8_t* ho
  * (nr_set_timing) always renders with one group. */
 int nr_set_render_streams(nr_ctx* ctx, int n);
 
+/* Launch tunings of nr_render (none changes the result: every pixel's samples
+ * are folded in order with the same random numbers).  No counterpart in the
+ * reference (one CUDA thread per pixel, launch shape fixed). */
+typedef struct nr_tuning {
+    int32_t slots;            /* path slots per pixel, 1..8 (default 4): samples of a pixel in flight;
+                                 the path state and queues hold slots x pixels entries */
+    int32_t stream_groups;    /* 1..4 (default 2), as nr_set_render_streams */
+    int32_t shade_wg_per_cu;  /* k_shade grid: 256-thread workgroups per CU, 1..32 (default 8) */
+} nr_tuning;
+int nr_set_tuning(nr_ctx* ctx, const nr_tuning* tuning);
+int nr_get_tuning(nr_ctx* ctx, nr_tuning* tuning);
+
+/* The project's autotuner: renders `desc` on `view` under candidate tunings
+ * (coordinate descent over slots {1,2,4,6,8}, stream groups {1..4}, shade
+ * workgroups per CU {4,8,16}), keeps the fastest, writes it to *tuning (may be
+ * NULL) and clears the view's AOVs.  A desc of a few passes of the real
+ * workload is enough. */
+int nr_autotune(nr_ctx* ctx, int view, const nr_render_desc* desc, nr_tuning* tuning);
+
 /* Mesh tree builder for later nr_scene_upload calls (enum nr_build_mode). */
 int nr_set_build_mode(nr_ctx* ctx, int mode);
 

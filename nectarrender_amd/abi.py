@@ -67,6 +67,11 @@ class NrCameraDesc(C.Structure):
                 ("aperture", C.c_float), ("sensor_width", C.c_float), ("shutter_speed", C.c_float)]
 
 
+class NrTuning(C.Structure):
+    """nr_tuning: launch tunings of nr_render (results do not depend on them)."""
+    _fields_ = [("slots", C.c_int32), ("stream_groups", C.c_int32), ("shade_wg_per_cu", C.c_int32)]
+
+
 class NrRenderDesc(C.Structure):
     _fields_ = [("sample_begin", C.c_uint32), ("sample_end", C.c_uint32), ("ray_depth", C.c_uint32),
                 ("seed", C.c_uint32), ("do_rr", C.c_int32), ("rr_start_bounce", C.c_uint32),

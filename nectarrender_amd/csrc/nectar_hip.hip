@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -68,14 +69,14 @@ static int fail(int code, const std::string& msg) { g_err = msg; return code; }
 // counters (one int each): live queues [0,2), material queues [2,8), error flag,
 // shadow-ray queue (fast mode), occupied path slots after k_regen
 enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_SHD = 9, CNT_OCC = 10, CNT_N = 16 };
-// Path slots per pixel: sample s of pixel i runs in slot s % NR_SLOTS (path id
-// i + (s % NR_SLOTS) * cap), so up to NR_SLOTS consecutive samples of a pixel
-// are in flight and pixels with long paths do not hold back the whole view
-// (1080p workload, 64 samples: 266 iterations with one slot, 146 with 2, 86
-// with 4; 4 is the fastest for both estimators).
-#ifndef NR_SLOTS
-#define NR_SLOTS 4
-#endif
+// Path slots per pixel (a tuning of the context, 1..NR_SLOTS_MAX, default 4):
+// sample s of pixel i runs in slot s % slots (path id i + (s % slots) * cap),
+// so up to `slots` consecutive samples of a pixel are in flight and pixels with
+// long paths do not hold back the whole view (1080p workload, 64 samples: 266
+// iterations with one slot, 146 with 2, 86 with 4; 4 is the fastest for both
+// estimators there).
+#define NR_SLOTS_MAX 8
+#define NR_SLOTS_DEFAULT 4
 // rows of the HDRI marginal CDF that k_shade stages in LDS (larger maps read it from global memory)
 constexpr int SKY_MARG_LDS = 4097;
 #ifndef NR_LDS_LEVELS
@@ -103,9 +104,9 @@ struct StackCfg {
     size_t stride;          // rays in the launch
 };
 
-// Path state, NR_SLOTS slots per pixel (path id = pixel + slot * cap).  o.w
+// Path state, `slots` slots per pixel (path id = pixel + slot * cap).  o.w
 // carries the path's bounce index and d.w the sample index it is tracing (as
-// integer bits).  Queues hold path ids; their stride is qcap = NR_SLOTS * cap.
+// integer bits).  Queues hold path ids; their stride is qcap = slots * padded pixels.
 struct PathPtrs {
     float4* o;
     float4* d;
@@ -130,7 +131,8 @@ struct PathPtrs {
     int32_t* qshd;
     int32_t* err;       // traversal-overflow flag (group 0's CNT_ERR; every group sets it)
     uint32_t cap;       // pixels
-    uint32_t qcap;      // queue entries per queue (NR_SLOTS * padded pixels)
+    uint32_t qcap;      // queue entries per queue (slots * padded pixels)
+    uint32_t slots;     // path slots per pixel, 1..NR_SLOTS_MAX
 };
 
 // A view of a render call: its camera, where its 8x8 tiles start in the call's
@@ -530,7 +532,7 @@ __device__ __forceinline__ uint32_t start_path(const GView& V, PathPtrs& P, uint
                                                const TraceArgs& A) {
     const GCamera& cam = V.cam;
     uint32_t i = y * cam.W + x;
-    uint32_t p = V.pix0 + i + (s % NR_SLOTS) * P.cap;
+    uint32_t p = V.pix0 + i + (s % P.slots) * P.cap;
     Gen g = path_gen(A, i, s, 0u, V.npix);
     Ray r = camera_ray(cam, x, y, i, s, A.seed, g);
     P.o[p] = make_float4(r.o.x, r.o.y, r.o.z, __int_as_float(0));
@@ -643,7 +645,7 @@ __global__ void __launch_bounds__(256) k_unpack_acc(const float4* __restrict__ a
     n[i] = z.x; n[HW + i] = z.y; n[2 * (size_t)HW + i] = z.z;
 }
 
-// First NR_SLOTS samples of every pixel (fewer if the range is shorter),
+// First `slots` samples of every pixel (fewer if the range is shorter),
 // appended to the live queue in tile order, one workgroup-aggregated append per
 // sample rank.  The group's counters were cleared by k_clear_counts.
 __global__ void __launch_bounds__(256) k_start(PathPtrs P, uint32_t s, TraceArgs A) {
@@ -652,17 +654,17 @@ __global__ void __launch_bounds__(256) k_start(PathPtrs P, uint32_t s, TraceArgs
     uint32_t g = 0, m = 0;                                          // samples started for the pixel
     if (valid) {
         const uint32_t end = pixel_end(A, x, y);
-        m = end > s ? min((uint32_t)NR_SLOTS, end - s) : 0u;
+        m = end > s ? min(P.slots, end - s) : 0u;
         const GView& V = A.views[v];
         g = V.pix0 + y * V.cam.W + x;
         P.nf[g] = s;
         P.ns[g] = s + m;
     }
-    for (uint32_t k = 0; k < NR_SLOTS; k++) {
+    for (uint32_t k = 0; k < P.slots; k++) {                       // uniform trip count
         const bool go = valid && k < m;
         int32_t path = 0;
         if (go) path = (int32_t)start_path(A.views[v], P, x, y, s + k, A);
-        else if (valid) P.finished[g + ((s + k) % NR_SLOTS) * P.cap] = 2;
+        else if (valid) P.finished[g + ((s + k) % P.slots) * P.cap] = 2;
         const int slot = block_append<256>(go, &P.counts[CNT_ACT + 0]);
         if (go) P.qact[slot] = path;
     }
@@ -951,14 +953,14 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_shadow(const GScene* __res
 
 // Per pixel, scanning all pixels in tile order so the new primary rays of a
 // wave are coherent: fold the ended samples strictly in sample order (sample
-// nf lives in slot nf % NR_SLOTS; a sample that ends before its predecessor
+// nf lives in slot nf % slots; a sample that ends before its predecessor
 // waits), then start the next samples in the slots that became idle.  The
 // occupied slots (samples started and not yet folded, ns - nf) are counted:
 // that number never grows, so the host sizes later traversal grids from it.
 constexpr int REGEN_BLOCK = 1024;
 __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) {
     uint32_t t = A.t0 + blockIdx.x * blockDim.x + threadIdx.x, v = 0, x = 0, y = 0;
-    uint32_t started[NR_SLOTS];
+    uint32_t started[NR_SLOTS_MAX];
     uint32_t n_start = 0, occ = 0;
     if (t < A.t1 && tile_pixel(A, t, v, x, y)) {
         const GView& V = A.views[v];
@@ -966,8 +968,8 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) 
         const uint32_t end = pixel_end(A, x, y);
         uint32_t nf = P.nf[i], ns = P.ns[i];
 #pragma unroll
-        for (int k = 0; k < NR_SLOTS; k++) {
-            const uint32_t p = i + (nf % NR_SLOTS) * P.cap;
+        for (int k = 0; k < NR_SLOTS_MAX; k++) {
+            const uint32_t p = i + (nf % P.slots) * P.cap;
             if (nf >= ns || P.finished[p] != 1) break;
             float fn = (float)(A.fast ? nf + 1 : nf);           // fast mode counts pass 0 as the first
             float a = (fn - 1.0f) / fn, b = 1.0f / fn;
@@ -978,8 +980,8 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) 
             nf++;
         }
 #pragma unroll
-        for (int k = 0; k < NR_SLOTS; k++) {
-            if (ns >= end || ns - nf >= NR_SLOTS) break;        // slot ns % NR_SLOTS is idle when ns - nf < NR_SLOTS
+        for (int k = 0; k < NR_SLOTS_MAX; k++) {
+            if (ns >= end || ns - nf >= P.slots) break;         // slot ns % slots is idle when ns - nf < slots
             started[k] = start_path(V, P, x, y, ns, A);
             n_start = k + 1;
             ns++;
@@ -990,7 +992,8 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) 
     }
     int32_t* live = &P.counts[CNT_ACT + (1 - A.in)];
 #pragma unroll
-    for (int k = 0; k < NR_SLOTS; k++) {
+    for (int k = 0; k < NR_SLOTS_MAX; k++) {
+        if ((uint32_t)k >= P.slots) break;                       // uniform
         int slot = block_append<REGEN_BLOCK>((uint32_t)k < n_start, live);
         if ((uint32_t)k < n_start) P.qact[(size_t)(1 - A.in) * P.qcap + slot] = (int32_t)started[k];
         (void)block_append<REGEN_BLOCK>((uint32_t)k < occ, &P.counts[CNT_OCC]);
@@ -1184,6 +1187,7 @@ struct nr_ctx {
     // stream groups: disjoint bands of pixels, each a complete wavefront pipeline on
     // its own stream, so one group's traversal tail overlaps another's work
     int render_streams = NR_RENDER_STREAMS;
+    int slots = NR_SLOTS_DEFAULT;       // path slots per pixel (nr_set_tuning)
     hipStream_t gstream[NR_MAX_GROUPS] = {};
     hipEvent_t gev[NR_MAX_GROUPS][2] = {};
     hipEvent_t gev_done[NR_MAX_GROUPS] = {};
@@ -1768,7 +1772,7 @@ int nr_aov_reset(nr_ctx* c, int view) {
 
 // Path state for n pixels and queues for npad padded pixels (whole 8x8 tiles).
 static int ensure_paths(nr_ctx* c, uint32_t n, uint32_t npad) {
-    if (n <= c->path_cap && npad <= c->q_cap) return NR_OK;
+    if (n <= c->path_cap && npad <= c->q_cap && c->P.slots == (uint32_t)c->slots) return NR_OK;
     n = std::max(n, c->path_cap);
     npad = std::max(npad, c->q_cap);
     HIPCK(hipStreamSynchronize(c->stream));
@@ -1781,7 +1785,8 @@ static int ensure_paths(nr_ctx* c, uint32_t n, uint32_t npad) {
         c->path_mem.push_back(*p);
         return NR_OK;
     };
-    const size_t np = (size_t)NR_SLOTS * n;      // path slots
+    const uint32_t slots = (uint32_t)c->slots;
+    const size_t np = (size_t)slots * n;         // path slots
     int e = al((void**)&P.o, sizeof(float4) * np);
     if (!e) e = al((void**)&P.d, sizeof(float4) * np);
     if (!e) e = al((void**)&P.atten, sizeof(float4) * np);
@@ -1790,7 +1795,7 @@ static int ensure_paths(nr_ctx* c, uint32_t n, uint32_t npad) {
     if (!e) e = al((void**)&P.nrm, sizeof(float4) * np);
     if (!e) e = al((void**)&P.draw, sizeof(uint32_t) * np);
     if (!e) e = al((void**)&P.hit, sizeof(GHit) * np);
-    const size_t nq = (size_t)NR_SLOTS * npad;   // entries per queue
+    const size_t nq = (size_t)slots * npad;      // entries per queue
     if (!e) e = al((void**)&P.qact, sizeof(int32_t) * 2 * nq);
     if (!e) e = al((void**)&P.qmat, sizeof(int32_t) * NR_MAT_TYPES * nq);
     if (!e) e = al((void**)&P.counts, sizeof(int32_t) * CNT_N * NR_MAX_GROUPS);
@@ -1808,6 +1813,7 @@ static int ensure_paths(nr_ctx* c, uint32_t n, uint32_t npad) {
     P.cap = n;
     P.qcap = (uint32_t)nq;
     P.err = P.counts + CNT_ERR;
+    P.slots = slots;
     c->path_cap = n;
     c->q_cap = npad;
     return NR_OK;
@@ -1892,7 +1898,7 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
         tiles += (uint64_t)hv[k].tiles_x * ((v.H + 7) / 8);
         npix += (uint64_t)v.W * v.H;
     }
-    if (tiles * 64 * NR_SLOTS > (uint64_t)INT32_MAX) return fail(NR_ERR_LIMIT, "views too large for 32-bit path ids");
+    if (tiles * 64 * (uint64_t)c->slots > (uint64_t)INT32_MAX) return fail(NR_ERR_LIMIT, "views too large for 32-bit path ids");
     const uint32_t N = (uint32_t)npix, T = (uint32_t)tiles;
     int e = ensure_paths(c, N, T * 64);
     if (e) return e;
@@ -1951,7 +1957,8 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
         // group on the context stream so per-kernel times are not shared with
         // another group.
         const int G = c->timing ? 1 : (int)std::max(1u, std::min<uint32_t>((uint32_t)c->render_streams, T));
-        const uint32_t m = std::min<uint32_t>(NR_SLOTS, d->sample_end - d->sample_begin);
+        const uint32_t S = c->P.slots;
+        const uint32_t m = std::min<uint32_t>(S, d->sample_end - d->sample_begin);
         struct Group {
             PathPtrs P;
             TraceArgs A;
@@ -1963,12 +1970,12 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
         const size_t lv = c->scene.stack_depth > LDS_STACK_LEVELS ? (size_t)(c->scene.stack_depth - LDS_STACK_LEVELS) : 0;
         {
             StackCfg all;
-            e = stack_spill(c, NR_SLOTS * T * 64 + 128u * NR_MAX_GROUPS, all);
+            e = stack_spill(c, S * T * 64 + 128u * NR_MAX_GROUPS, all);
             if (e) return e;
             size_t spill_off = 0;
             for (int g = 0; g < G; g++) {
                 const uint32_t T0 = (uint32_t)((uint64_t)T * g / G), T1 = (uint32_t)((uint64_t)T * (g + 1) / G);
-                const size_t q0 = (size_t)NR_SLOTS * T0 * 64;       // queue entries of the groups before
+                const size_t q0 = (size_t)S * T0 * 64;              // queue entries of the groups before
                 Group& gr = grp[g];
                 gr.P = c->P;
                 gr.P.qact = c->P.qact + q0;
@@ -1978,7 +1985,7 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
                 gr.A = TraceArgs{d->seed, 0u, d->ray_depth, d->do_rr, d->rr_start_bounce, d->rr_survival,
                                  d->sample_end, T0 * 64, T1 * 64, fast ? 1 : 0, (uint32_t)n_views, c->d_views,
                                  tl ? tl->d_end : nullptr, tl ? tl->shift : 0u, tl ? tl->cols : 0u};
-                const uint32_t cap_g = (NR_SLOTS * (T1 - T0) * 64 + 127) / 128 * 128;
+                const uint32_t cap_g = (S * (T1 - T0) * 64 + 127) / 128 * 128;
                 gr.SC.spill = all.spill + spill_off;
                 gr.SC.stride = cap_g;
                 spill_off += lv * cap_g;
@@ -2469,6 +2476,74 @@ int nr_set_render_streams(nr_ctx* c, int n) {
     if (!c) return fail(NR_ERR_INVALID, "null context");
     if (n < 1 || n > NR_MAX_GROUPS) return fail(NR_ERR_INVALID, "render streams outside 1..4");
     c->render_streams = n;
+    return NR_OK;
+}
+
+int nr_set_tuning(nr_ctx* c, const nr_tuning* t) {
+    if (!c || !t) return fail(NR_ERR_INVALID, "null argument");
+    if (t->slots < 1 || t->slots > NR_SLOTS_MAX) return fail(NR_ERR_INVALID, "slots outside 1..8");
+    if (t->stream_groups < 1 || t->stream_groups > NR_MAX_GROUPS) return fail(NR_ERR_INVALID, "stream groups outside 1..4");
+    if (t->shade_wg_per_cu < 1 || t->shade_wg_per_cu > 32) return fail(NR_ERR_INVALID, "shade workgroups per CU outside 1..32");
+    HIPCK(hipSetDevice(c->device));
+    HIPCK(hipStreamSynchronize(c->stream));                 // path state may be reallocated by the next render
+    c->slots = t->slots;
+    c->render_streams = t->stream_groups;
+    c->shade_grid = c->n_cu * t->shade_wg_per_cu;
+    return NR_OK;
+}
+
+int nr_get_tuning(nr_ctx* c, nr_tuning* t) {
+    if (!c || !t) return fail(NR_ERR_INVALID, "null argument");
+    t->slots = c->slots;
+    t->stream_groups = c->render_streams;
+    t->shade_wg_per_cu = c->shade_grid / std::max(1, c->n_cu);
+    return NR_OK;
+}
+
+// Coordinate descent over the tunings, timing `desc` on `view` (wall clock
+// around the render and its completion; the first pass is rendered once
+// untimed after each change so allocations stay outside the timing).
+int nr_autotune(nr_ctx* c, int view, const nr_render_desc* d, nr_tuning* out) {
+    if (!c || !d) return fail(NR_ERR_INVALID, "null argument");
+    if (d->sample_end <= d->sample_begin) return fail(NR_ERR_INVALID, "autotune needs at least one pass");
+    nr_tuning best;
+    if (int e = nr_get_tuning(c, &best)) return e;
+    auto measure = [&](const nr_tuning& t, double& ms) -> int {
+        if (int e = nr_set_tuning(c, &t)) return e;
+        nr_render_desc w = *d;
+        w.sample_end = w.sample_begin + 1;
+        if (int e = nr_aov_reset(c, view)) return e;
+        if (int e = nr_render(c, view, &w)) return e;
+        HIPCK(hipStreamSynchronize(c->stream));
+        if (int e = nr_aov_reset(c, view)) return e;
+        HIPCK(hipStreamSynchronize(c->stream));
+        auto t0 = std::chrono::steady_clock::now();
+        if (int e = nr_render(c, view, d)) return e;
+        HIPCK(hipStreamSynchronize(c->stream));
+        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return NR_OK;
+    };
+    double best_ms = 0.0;
+    if (int e = measure(best, best_ms)) return e;
+    const int slot_c[] = {1, 2, 4, 6, 8}, group_c[] = {1, 2, 3, 4}, shade_c[] = {4, 8, 16};
+    for (int axis = 0; axis < 3; axis++) {
+        const int* cand = axis == 0 ? slot_c : (axis == 1 ? group_c : shade_c);
+        const int n = axis == 0 ? 5 : (axis == 1 ? 4 : 3);
+        nr_tuning base = best;
+        for (int k = 0; k < n; k++) {
+            nr_tuning t = base;
+            int& field = axis == 0 ? t.slots : (axis == 1 ? t.stream_groups : t.shade_wg_per_cu);
+            if (field == cand[k]) continue;
+            field = cand[k];
+            double ms = 0.0;
+            if (int e = measure(t, ms)) return e;
+            if (ms < best_ms) { best_ms = ms; best = t; }
+        }
+    }
+    if (int e = nr_set_tuning(c, &best)) return e;
+    if (int e = nr_aov_reset(c, view)) return e;
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (out) *out = best;
     return NR_OK;
 }
 

@@ -42,6 +42,9 @@ def lib() -> C.CDLL:
         "nr_render": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrRenderDesc)]),
         "nr_render_views": (C.c_int, [vp, C.POINTER(C.c_int32), C.c_int, C.POINTER(abi.NrRenderDesc)]),
         "nr_render_tiles": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrRenderDesc), C.POINTER(C.c_uint32), C.c_int]),
+        "nr_set_tuning": (C.c_int, [vp, C.POINTER(abi.NrTuning)]),
+        "nr_get_tuning": (C.c_int, [vp, C.POINTER(abi.NrTuning)]),
+        "nr_autotune": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrRenderDesc), C.POINTER(abi.NrTuning)]),
         "nr_tile_error": (C.c_int, [vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, vp]),
         "nr_aov_read": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_size_t]),
         "nr_aov_device_ptr": (vp, [vp, C.c_int, C.c_int]),
@@ -123,6 +126,20 @@ class Context:
 
     def render(self, view: int, desc: abi.NrRenderDesc):
         _check(lib().nr_render(self._h, view, C.byref(desc)))
+
+    def set_tuning(self, slots: int = 4, stream_groups: int = 2, shade_wg_per_cu: int = 8):
+        _check(lib().nr_set_tuning(self._h, C.byref(abi.NrTuning(slots, stream_groups, shade_wg_per_cu))))
+
+    def get_tuning(self) -> dict:
+        t = abi.NrTuning()
+        _check(lib().nr_get_tuning(self._h, C.byref(t)))
+        return {"slots": t.slots, "stream_groups": t.stream_groups, "shade_wg_per_cu": t.shade_wg_per_cu}
+
+    def autotune(self, view: int, desc: abi.NrRenderDesc) -> dict:
+        """Times candidate tunings on `desc`, keeps the fastest (the view's AOVs are cleared)."""
+        t = abi.NrTuning()
+        _check(lib().nr_autotune(self._h, view, C.byref(desc), C.byref(t)))
+        return {"slots": t.slots, "stream_groups": t.stream_groups, "shade_wg_per_cu": t.shade_wg_per_cu}
 
     def render_tiles(self, view: int, desc: abi.NrRenderDesc, tile_end: np.ndarray, tile_size: int):
         """nr_render with a per-tile sample limit (rows x cols uint32, row major)."""

@@ -20,7 +20,8 @@ def declared():
 
 def test_header_declares_entry_points():
     names = declared()
-    for n in ("nr_create", "nr_destroy", "nr_scene_upload", "nr_camera_set", "nr_render", "nr_render_views", "nr_render_tiles", "nr_tile_error",
+    for n in ("nr_create", "nr_destroy", "nr_scene_upload", "nr_camera_set", "nr_render", "nr_render_views", "nr_render_tiles", "nr_tile_error", "nr_set_tuning",
+              "nr_get_tuning", "nr_autotune",
               "nr_trace_closest",
               "nr_aov_read", "nr_post", "nr_last_error", "nr_view_size", "nr_unet_create", "nr_unet_forward",
               "nr_unet_train_step", "nr_unet_gradients", "nr_denoise", "nr_viewport", "nr_pick",
@@ -53,6 +54,9 @@ def test_null_arguments_fail_cleanly_without_device():
     assert L.nr_render(None, 0, None) == 1
     assert L.nr_render_views(None, None, 0, None) == 1
     assert L.nr_render_tiles(None, 0, None, None, 64) == 1
+    assert L.nr_set_tuning(None, None) == 1
+    assert L.nr_get_tuning(None, None) == 1
+    assert L.nr_autotune(None, 0, None, None) == 1
     assert L.nr_tile_error(None, None, None, None, 8, 8, 8, None) == 1
     assert L.nr_trace_closest(None, None, 0, None) == 1
     assert L.nr_post(None, 0, None, None, None) == 1

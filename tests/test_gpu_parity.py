@@ -192,3 +192,32 @@ def test_render_views_in_one_pipeline_equals_one_at_a_time(ctx, mode):
         _assert_same(ctx.read(v, abi.LAYER_BEAUTY), single[v][0], f"subset view {v}")
     with pytest.raises(RuntimeError):
         ctx.render_views([2, 2], d)
+
+
+@pytest.mark.parametrize("tuning", [(1, 1, 8), (2, 2, 4), (8, 3, 16), (6, 4, 2)])
+def test_render_same_for_any_tuning(ctx, oracle, tuning):
+    """Path slots per pixel, stream groups and the shading grid change only how work is scheduled: the AOVs
+    equal the oracle's for every tuning."""
+    s, cam, rp = scenes.mixed_scene(spp=5, res=72)
+    ctx.set_tuning(*tuning)
+    try:
+        g, o = _render_both(ctx, oracle, s, cam, rp, 1, 5)
+    finally:
+        ctx.set_tuning(4, 2, 8)
+    for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
+        _assert_same(g[k], o[k], f"{tuning} {name_}")
+
+
+def test_autotune_picks_a_valid_tuning_and_clears_the_view(ctx, oracle):
+    s, cam, rp = scenes.mixed_scene(spp=8, res=96)
+    ctx.upload(s)
+    ctx.set_camera(0, cam, rp["seed"])
+    t = ctx.autotune(0, abi.render_desc(1, 8, rp["depth"], rp["seed"]))
+    try:
+        assert 1 <= t["slots"] <= 8 and 1 <= t["stream_groups"] <= 4 and 1 <= t["shade_wg_per_cu"] <= 32
+        assert ctx.get_tuning() == t
+        assert not ctx.read(0).any()
+        g, o = _render_both(ctx, oracle, s, cam, rp, 1, 8)
+        _assert_same(g[0], o[0], "after autotune")
+    finally:
+        ctx.set_tuning(4, 2, 8)
