@@ -1237,15 +1237,17 @@ static int upload(nr_ctx* c, const T* host, size_t n, T** dev) {
 // neighbourhood times sin(theta) at the row centre, theta = pi (j + 0.5) / H.
 // The density then bounds the radiance from above everywhere, including the
 // half-texel rim a bright feature bleeds into, which would otherwise be found
-// only by rare scattered rays.  cond holds one CDF per row (W + 1 entries,
-// 0 ... 1), marg the CDF over rows (H + 1 entries); sums run in double, the
-// ends are exactly 0 and 1, and a texel whose float CDF step is zero has
-// probability zero for both the sampler and the pdf lookup.  Returns false for
-// a map with no positive texel (no tables, no sky sampling).
-static bool sky_cdfs(const float* data, int H, int W, std::vector<float>& marg, std::vector<float>& cond) {
+// only by rare scattered rays.  marg is the CDF over rows (H + 1 entries, sums
+// in double, ends exactly 0 and 1); alias holds per texel a float4 (keep
+// probability, alias texel as int bits, the texel's probability within its
+// row, 0): a Walker alias table of each row built with Vose's method (columns
+// whose scaled weight is below 1 take the excess of columns above 1, in the
+// order of two work stacks).  Returns false for a map with no positive texel
+// (no tables, no sky sampling).
+static bool sky_cdfs(const float* data, int H, int W, std::vector<float>& marg, std::vector<float>& alias) {
     const size_t hw = (size_t)H * W;
     marg.assign((size_t)H + 1, 0.0f);
-    cond.assign((size_t)H * (W + 1), 0.0f);
+    alias.assign(4 * hw, 0.0f);
     std::vector<double> mean(hw), rowmax(hw);
     for (size_t k = 0; k < hw; k++) {
         double v = 0.0;
@@ -1260,20 +1262,41 @@ static bool sky_cdfs(const float* data, int H, int W, std::vector<float>& marg, 
             const double* r = mean.data() + (size_t)j * W;
             rowmax[(size_t)j * W + i] = std::max(r[i], std::max(r[(i + W - 1) % W], r[(i + 1) % W]));
         }
-    std::vector<double> row_sum(H), cw((size_t)W + 1);
+    std::vector<double> row_sum(H), w((size_t)W), q((size_t)W);
+    std::vector<int> small, large;
     double total = 0.0;
     for (int j = 0; j < H; j++) {
         const double st = std::sin(M_PI * ((double)j + 0.5) / (double)H);
         const double* a = rowmax.data() + (size_t)std::max(j - 1, 0) * W;   // max over j - 1 .. j + 1 (clamped)
         const double* b = rowmax.data() + (size_t)j * W;
         const double* c = rowmax.data() + (size_t)std::min(j + 1, H - 1) * W;
-        cw[0] = 0.0;
-        for (int i = 0; i < W; i++) cw[i + 1] = cw[i] + std::max(a[i], std::max(b[i], c[i])) * st;
-        float* cr = cond.data() + (size_t)j * (W + 1);
-        for (int i = 0; i < W; i++) cr[i] = cw[W] > 0.0 ? (float)(cw[i] / cw[W]) : (float)i / (float)W;
-        cr[W] = 1.0f;
-        row_sum[j] = cw[W];
-        total += cw[W];
+        double sum = 0.0;
+        for (int i = 0; i < W; i++) {
+            w[i] = std::max(a[i], std::max(b[i], c[i]));
+            sum += w[i];
+        }
+        float* ar = alias.data() + 4 * (size_t)j * W;
+        small.clear();
+        large.clear();
+        for (int i = 0; i < W; i++) {
+            q[i] = sum > 0.0 ? w[i] * (double)W / sum : 1.0;
+            ar[4 * i + 2] = sum > 0.0 ? (float)(w[i] / sum) : 1.0f / (float)W;
+            (q[i] < 1.0 ? small : large).push_back(i);
+        }
+        while (!small.empty() && !large.empty()) {
+            const int sm = small.back(), lg = large.back();
+            small.pop_back();
+            large.pop_back();
+            ar[4 * sm] = (float)q[sm];
+            int32_t al = lg;
+            std::memcpy(&ar[4 * sm + 1], &al, sizeof(al));
+            q[lg] = (q[lg] + q[sm]) - 1.0;
+            (q[lg] < 1.0 ? small : large).push_back(lg);
+        }
+        for (int rest : large) { ar[4 * rest] = 1.0f; std::memcpy(&ar[4 * rest + 1], &rest, sizeof(rest)); }
+        for (int rest : small) { ar[4 * rest] = 1.0f; std::memcpy(&ar[4 * rest + 1], &rest, sizeof(rest)); }
+        row_sum[j] = sum * st;
+        total += row_sum[j];
     }
     if (!(total > 0.0) || !std::isfinite(total)) return false;
     double acc = 0.0;
@@ -1671,8 +1694,9 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
     S.sky.intensity = d->sky.intensity;
     S.sky.data = nullptr;
     S.sky.marg = nullptr;
-    S.sky.cond = nullptr;
-    std::vector<float> sky_marg, sky_cond;      // alive until the stream sync below
+    S.sky.alias = nullptr;
+    S.sky.rgbx = nullptr;
+    std::vector<float> sky_marg, sky_alias, sky_rgbx;     // alive until the stream sync below
     if (d->sky.type == NR_SKY_HDRI && d->sky.data) {
         if (d->sky.C < 3 || d->sky.H <= 0 || d->sky.W <= 0) return fail(NR_ERR_INVALID, "HDRI needs C >= 3");
         S.sky.H = d->sky.H; S.sky.W = d->sky.W;
@@ -1681,13 +1705,21 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
         int e = upload(c, d->sky.data, (size_t)3 * d->sky.H * d->sky.W, &p);
         if (e) return e;
         S.sky.data = p;
-        if (sky_cdfs(d->sky.data, d->sky.H, d->sky.W, sky_marg, sky_cond)) {
-            float *pm, *pc;
+        const size_t hw = (size_t)d->sky.H * d->sky.W;
+        sky_rgbx.assign(4 * hw, 0.0f);
+        for (size_t k = 0; k < hw; k++)
+            for (int ch = 0; ch < 3; ch++) sky_rgbx[4 * k + ch] = d->sky.data[ch * hw + k];
+        float* px;
+        e = upload(c, sky_rgbx.data(), sky_rgbx.size(), &px);
+        if (e) return e;
+        S.sky.rgbx = px;
+        if (sky_cdfs(d->sky.data, d->sky.H, d->sky.W, sky_marg, sky_alias)) {
+            float *pm, *pa;
             e = upload(c, sky_marg.data(), sky_marg.size(), &pm);
-            if (!e) e = upload(c, sky_cond.data(), sky_cond.size(), &pc);
+            if (!e) e = upload(c, sky_alias.data(), sky_alias.size(), &pa);
             if (e) return e;
             S.sky.marg = pm;
-            S.sky.cond = pc;
+            S.sky.alias = pa;
         }
     }
     // cube face transforms (shapes.h:192-223)

@@ -787,10 +787,17 @@ NR_D f3 sky_sample(const GSky& sk, f3 d) {
         y1 = (int)fmaxf(0.0f, fminf((float)(H - 1), (float)y1));
         size_t hw = (size_t)H * W;
         size_t i00 = (size_t)y0 * W + x0, i10 = (size_t)y0 * W + x1, i01 = (size_t)y1 * W + x0, i11 = (size_t)y1 * W + x1;
-        f3 c00 = mk3(sk.data[i00], sk.data[hw + i00], sk.data[2 * hw + i00]);
-        f3 c10 = mk3(sk.data[i10], sk.data[hw + i10], sk.data[2 * hw + i10]);
-        f3 c01 = mk3(sk.data[i01], sk.data[hw + i01], sk.data[2 * hw + i01]);
-        f3 c11 = mk3(sk.data[i11], sk.data[hw + i11], sk.data[2 * hw + i11]);
+        f3 c00, c10, c01, c11;
+        if (sk.rgbx) {                                  // the same texel values, interleaved
+            const float4* q = reinterpret_cast<const float4*>(sk.rgbx);
+            float4 a = q[i00], b = q[i10], c = q[i01], d = q[i11];
+            c00 = mk3(a.x, a.y, a.z); c10 = mk3(b.x, b.y, b.z); c01 = mk3(c.x, c.y, c.z); c11 = mk3(d.x, d.y, d.z);
+        } else {
+            c00 = mk3(sk.data[i00], sk.data[hw + i00], sk.data[2 * hw + i00]);
+            c10 = mk3(sk.data[i10], sk.data[hw + i10], sk.data[2 * hw + i10]);
+            c01 = mk3(sk.data[i01], sk.data[hw + i01], sk.data[2 * hw + i01]);
+            c11 = mk3(sk.data[i11], sk.data[hw + i11], sk.data[2 * hw + i11]);
+        }
         f3 top = c00 * (1.0f - tx) + c10 * tx;
         f3 bot = c01 * (1.0f - tx) + c11 * tx;
         return (top * (1.0f - ty) + bot * ty) * sk.intensity;
@@ -1001,16 +1008,29 @@ NR_D f3 sky_cdf_sample(const GSky& sk, const float* marg, float u1, float u2, fl
     const int j = cdf_find(marg, H, u1);
     const float m0 = marg[j], m1 = marg[j + 1];
     const float dv = m1 > m0 ? (u1 - m0) / (m1 - m0) : 0.5f;
-    const float* row = sk.cond + (size_t)j * (W + 1);
-    const int i = cdf_find(row, W, u2);
-    const float c0 = row[i], c1 = row[i + 1];
-    const float du = c1 > c0 ? (u2 - c0) / (c1 - c0) : 0.5f;
+    // texel of the row by its alias table: u2 * W picks a column and a fraction,
+    // the fraction decides between the column and its alias and, rescaled,
+    // places the point inside the chosen texel
+    const float4* row = reinterpret_cast<const float4*>(sk.alias) + (size_t)j * W;
+    const float x = u2 * (float)W;
+    int i = min((int)x, W - 1);
+    const float f = fminf(x - (float)i, 1.0f);
+    const float4 e = row[i];
+    float du, pt;
+    if (f < e.x || e.x >= 1.0f) {
+        du = e.x > 0.0f ? f / e.x : 0.5f;
+        pt = e.z;
+    } else {
+        i = min(max(__float_as_int(e.y), 0), W - 1);
+        du = (f - e.x) / (1.0f - e.x);
+        pt = row[i].z;
+    }
     const float u = ((float)i + fminf(fmaxf(du, 0.0f), 1.0f)) / (float)W;
     const float v = ((float)j + fminf(fmaxf(dv, 0.0f), 1.0f)) / (float)H;
     const float theta = v * NR_PI, phi = (u - 0.5f) * NR_PI2;
     const float st = nrm::sinf_(theta), ct = nrm::cosf_(theta);
     const f3 local = mk3(st * nrm::cosf_(phi), ct, st * nrm::sinf_(phi));
-    const float p_uv = (m1 - m0) * (c1 - c0) * (float)(W * H);
+    const float p_uv = (m1 - m0) * pt * (float)(W * H);
     pdf = st > 1e-7f ? p_uv / (2.0f * NR_PI * NR_PI * st) : 0.0f;
     return mtv(sk.rot, local);
 }
@@ -1022,8 +1042,8 @@ NR_D float sky_cdf_pdf(const GSky& sk, f3 d) {
     float v = nrm::acosf_(fminf(fmaxf(dir.y, -1.0f), 1.0f)) / NR_PI;
     int i = min(max((int)(u * (float)W), 0), W - 1);
     int j = min(max((int)(v * (float)H), 0), H - 1);
-    const float* row = sk.cond + (size_t)j * (W + 1);
-    const float p_uv = (sk.marg[j + 1] - sk.marg[j]) * (row[i + 1] - row[i]) * (float)(W * H);
+    const float pt = reinterpret_cast<const float4*>(sk.alias)[(size_t)j * W + i].z;
+    const float p_uv = (sk.marg[j + 1] - sk.marg[j]) * pt * (float)(W * H);
     const float st = nrm::sinf_(v * NR_PI);
     return st > 1e-7f ? p_uv / (2.0f * NR_PI * NR_PI * st) : 0.0f;
 }

@@ -109,10 +109,16 @@ struct GSky {
     int32_t H, W;
     const float* data;      // CHW
     // importance sampling of the HDRI (fast mode): piecewise-constant density over
-    // texels proportional to luminance x sin(theta); marginal CDF over rows (H + 1)
-    // and one conditional CDF per row (H x (W + 1)); null when not built
+    // texels (weights of sky_cdfs); marginal CDF over rows (H + 1 floats, searched
+    // in LDS) and a per-row alias table; null when not built
     const float* marg;
-    const float* cond;
+    // per texel, float4: Walker/Vose alias table of its row (probability of
+    // keeping the texel, alias texel as int bits) and the texel's probability
+    // within its row; H x W entries
+    const float* alias;
+    // the map as one float4 (r, g, b, 0) per texel: the bilinear lookup's four
+    // texels in four 16-byte loads instead of twelve scattered planar ones
+    const float* rgbx;
 };
 
 // everything a traversal/shading kernel needs, passed by value
