@@ -78,7 +78,8 @@ enum nr_layer {
 /* Estimator. PARITY reproduces trace_full_kernel draw for draw
  * (csrc/engine/src/trace.cu:47-79).  FAST (no counterpart in the reference)
  * adds next-event estimation to an HDRI sky: at every non-specular vertex a
- * direction drawn from the map's luminance x sin(theta) CDFs, weighted against
+ * direction drawn from the map's radiance x sin(theta) (marginal CDF over rows,
+ * per-row alias tables), weighted against
  * the scattering technique (the reference's mixture with lights[0] included)
  * by the power heuristic, with a shadow ray; sample s of pixel i draws from
  * the 64-bit Philox subsequence (s << 32) | i, and passes start at 0 (pass s

@@ -764,7 +764,8 @@ __device__ __forceinline__ f3 mat_eval(const GScene& S, const GMaterial& m, f3 n
 
 // Fast mode adds next-event estimation to the HDRI sky at every non-specular
 // vertex whose scattered ray would still be traced (bounce + 1 < depth): a
-// direction from the sky's CDFs (marginal staged in LDS), weighted against the
+// direction from the sky's tables (marginal CDF staged in LDS, per-row alias
+// tables), weighted against the
 // scattering technique with the power heuristic, queued as a shadow ray whose
 // contribution k_shadow adds if it reaches the sky.
 template <int M, bool FAST>
@@ -864,7 +865,7 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
             }
             if (NEE_M && marg && bounce + 1 < A.depth) {
                 // drawn after the scattering sample: the scattered direction does
-                // not depend on whether the sky has CDFs
+                // not depend on whether the sky has sampling tables
                 float u1 = g.uniform(), u2 = g.uniform(), ps;
                 f3 ld = sky_cdf_sample(S.sky, marg, u1, u2, ps);
                 if (ps > 0.0f) {
