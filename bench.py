@@ -44,9 +44,10 @@ TRAVERSAL_BYTES_PER_PATH = 1942.58
 # estimation adds shadow rays, traced by k_shadow, and changes no scattering
 # decision's probability), so the same per-path model applies to its k_extend.
 # HBM bytes k_extend moves per path on this workload, from rocprofv3 FETCH_SIZE (x 1.99992, the factor
-# a 1 GiB device-to-device copy gives) + WRITE_SIZE (x 1.0) over 18,662,400 paths per estimator:
+# a 1 GiB device-to-device copy gives) + WRITE_SIZE (x 1.0) over 18,662,400 paths per estimator,
+# default tuning (4 path slots per pixel):
 # profiles/k_extend_c2_1080p.md.
-TRAVERSAL_HBM_BYTES_PER_PATH = {0: 1105.1, 1: 1071.0}    # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
+TRAVERSAL_HBM_BYTES_PER_PATH = {0: 864.7, 1: 840.4}    # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
 HBM_PEAK_GBS = 8000.0
 
 
