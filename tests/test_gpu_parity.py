@@ -195,10 +195,23 @@ def test_render_views_in_one_pipeline_equals_one_at_a_time(ctx, mode):
 
 
 @pytest.mark.parametrize("tuning", [(1, 1, 8), (2, 2, 4), (8, 3, 16), (6, 4, 2)])
-def test_render_same_for_any_tuning(ctx, oracle, tuning):
+@pytest.mark.parametrize("scene", ["mixed", "config1", "icosphere"])
+def test_render_same_for_any_tuning(ctx, oracle, tuning, scene):
     """Path slots per pixel, stream groups and the shading grid change only how work is scheduled: the AOVs
-    equal the oracle's for every tuning."""
-    s, cam, rp = scenes.mixed_scene(spp=5, res=72)
+    equal the oracle's for every tuning, on scenes of every object type, of quads and spheres, and of a mesh."""
+    if scene == "mixed":
+        s, cam, rp = scenes.mixed_scene(spp=5, res=72)
+    elif scene == "config1":
+        s, cam, rp = scenes.config1(spp=5)
+        cam = abi.camera_desc((64, 64), (0, 1, 4), (-10, 0, 0), 5, 2.0, 2.0, 0.0, 2.0, 1.0)
+    else:
+        v, t = scenes.displaced_icosphere(freq=24)
+        s = abi.SceneDesc()
+        s.mesh(s.add_mesh(v, t), (0, 0, 0), (0, 0, 0), (1, 1, 1), abi.Material.pbr(roughness=0.4, metallic=1.0))
+        s.quad((0, -1.15, 0), (0, 0, 0), (12, 1, 12), abi.Material.lambertian((0.5, 0.5, 0.5)))
+        s.sky_simple((1, 1, 1), (0.5, 0.7, 1.0))
+        cam = abi.camera_desc((48, 64), (0.0, 0.5, 3.8), (-8.0, 0.0, 0.0), 5, 2.0, 3.8, 0.0, 2.0, 1.0)
+        rp = dict(depth=6, seed=3)
     ctx.set_tuning(*tuning)
     try:
         g, o = _render_both(ctx, oracle, s, cam, rp, 1, 5)
