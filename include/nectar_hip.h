@@ -411,10 +411,10 @@ int nr_object_set_transform(nr_ctx* ctx, int32_t index, const float position[3],
 int nr_sync(nr_ctx* ctx);
 
 /* Mesh-tree layout of later nr_trace_closest(_device) calls: 0 = the
- * reference's binary tree in its near-first order (default, what
- * NR_MODE_PARITY renders traverse), 1 = the 4-wide layout NR_MODE_FAST renders
- * traverse (same closest hit; of two triangles hit at exactly the same t
- * either may be reported).  No counterpart in the reference: the test hook of
+ * reference's binary tree in its near-first order (default, what extension
+ * rays traverse), 1 = the 4-wide layout NR_MODE_FAST's shadow rays traverse
+ * (same closest hit; of two triangles hit at exactly the same t either may be
+ * reported).  No counterpart in the reference: the test hook of
  * the wide traversal. */
 int nr_set_trace_layout(nr_ctx* ctx, int wide);
 

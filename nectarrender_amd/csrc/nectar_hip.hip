@@ -88,8 +88,9 @@ constexpr int LDS_STACK_LEVELS = NR_LDS_LEVELS;   // traversal stack levels kept
 #define NR_RENDER_STREAMS 2
 #endif
 // register budget of the traversal kernels: minimum waves per SIMD (0 = compiler's
-// choice).  6 caps them at 80 VGPRs: the 4-wide traversal would otherwise take 81
-// and run 5 waves per SIMD (fast mode 800 -> 824 Mpaths/s on the 1080p workload).
+// choice).  6 caps them at 80 VGPRs: the kernels for scenes with every object
+// type would otherwise take 81 and run 5 waves per SIMD; the quad / sphere /
+// mesh kernels take 71-74 either way.
 #ifndef NR_TRAV_OCC
 #define NR_TRAV_OCC 6
 #endif
@@ -701,7 +702,7 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_extend(const GScene* __res
         HRec rec;
         hrec_clear(rec);
         bool of = false;
-        int oi = scene_closest<TY, FAST>(S, r, rec, st, of);
+        int oi = scene_closest<TY>(S, r, rec, st, of);
         if (of) atomicOr(P.err, 1);
         if (oi < 0) {                                                           // trace.cu:19-22
             float4 a4 = P.atten[path], l4 = P.L[path];
