@@ -200,7 +200,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
-              f"tiles/samples with no exchange beyond a final sum, see DESIGN.md)", file=sys.stderr)
+              f"tiles or samples, with one final sum of the per-GPU AOVs as the only exchange)", file=sys.stderr)
         sys.exit(2)
     K, Wm = args.steps, args.warmup
 
