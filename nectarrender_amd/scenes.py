@@ -333,6 +333,29 @@ def config5(n_spheres=500, seed=0, res=(1080, 1920), spp=4096):
     return s, cam, dict(depth=8, seed=0)
 
 
+def training_scene_hdri_mesh(seed: int, res=256, spp=1024):
+    """Randomised scenes in the style of config2 for the denoiser: a displaced icosphere (frequency
+    12-40, fBm amplitude 0.05-0.15) of PBR metal or plastic on a Lambertian ground under a procedural
+    HDRI with a small, very bright sun (radiance 1e3-1e4, 0.5-2 degrees), a camera on a ring."""
+    rng = np.random.default_rng(20_000 + seed)
+    s = SceneDesc()
+    v, t = displaced_icosphere(freq=int(rng.integers(12, 41)), amp=float(rng.uniform(0.05, 0.15)), seed=seed)
+    m = s.add_mesh(v, t)
+    s.mesh(m, (0, 0, 0), tuple(rng.uniform(0, 90, 3)), (1, 1, 1),
+           Material.pbr(albedo=tuple(rng.uniform(0.3, 1.0, 3)), roughness=float(rng.uniform(0.1, 0.7)),
+                        metallic=float(rng.random() < 0.7)))
+    s.quad((0, -1.15, 0), (0, 0, 0), (12, 1, 12), Material.lambertian(tuple(rng.uniform(0.2, 0.7, 3))))
+    s.sky_hdri(procedural_hdri(256, 128, seed=seed, sigma=float(rng.uniform(0.2, 0.6)),
+                               sun_radiance=float(rng.uniform(1e3, 1e4)), sun_diameter_deg=float(rng.uniform(0.5, 2.0)),
+                               sun_elevation_deg=float(rng.uniform(15, 70)), sun_azimuth_deg=float(rng.uniform(0, 360))))
+    ang = rng.uniform(0, 2 * np.pi)
+    dist = rng.uniform(2.8, 5.0)
+    pos = (dist * np.cos(ang), float(rng.uniform(-0.3, 1.8)), dist * np.sin(ang))
+    cam = camera_desc((res, res), pos, look_rotation(pos, (0.0, 0.0, 0.0)), spp, float(rng.uniform(1.5, 3.0)), dist,
+                      0.0, 2.0, 1.0)
+    return s, cam, dict(depth=12, seed=int(rng.integers(1, 1 << 30)))
+
+
 def training_scene(seed: int, res=256, spp=1024):
     """Randomised scenes in the style of config1/config5 for the denoiser: a ground quad, 3-40 spheres of
     every material (some inside constant media), sometimes a small mesh, a simple or HDRI sky, a camera
