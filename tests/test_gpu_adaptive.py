@@ -123,3 +123,45 @@ def test_fallback_threshold_extremes(ctx, unet):
     unet.denoise_view(ctx, 2, ref)
     torch.cuda.synchronize()
     assert torch.equal(out, ref)
+
+
+def test_render_tiles_ragged_edges_and_small_tiles(ctx):
+    """8-pixel tiles on a 61 x 45 view (partial tiles on both edges): a checkerboard of tiles continues
+    to 6 passes, the rest keeps 2, and each pixel equals the full render with its own pass count."""
+    s, cam, rp = scenes.mixed_scene(spp=8, res=64)
+    cam = abi.camera_desc((45, 61), (0, 0.4, 3.0), (-5, 0, 0), 8, 2.0, 3.0, 0.0, 2.0, 1.0)
+    ctx.upload(s)
+    ctx.set_camera(0, cam, 11)
+    full = {}
+    for k in (2, 6):
+        ctx.reset(0)
+        ctx.render(0, abi.render_desc(1, k + 1, rp["depth"], 11))
+        full[k] = ctx.read(0)
+    ctx.reset(0)
+    ctx.render(0, abi.render_desc(1, 3, rp["depth"], 11))
+    rows, cols = (45 + 7) // 8, (61 + 7) // 8
+    ends = np.zeros((rows, cols), np.uint32)
+    ends[(np.add.outer(np.arange(rows), np.arange(cols)) % 2) == 0] = 7
+    ctx.render_tiles(0, abi.render_desc(3, 7, rp["depth"], 11), ends, 8)
+    got = ctx.read(0)
+    for r in range(rows):
+        for c in range(cols):
+            sl = (slice(None), slice(r * 8, (r + 1) * 8), slice(c * 8, (c + 1) * 8))
+            want = full[6][sl] if ends[r, c] else full[2][sl]
+            assert np.array_equal(got[sl], want, equal_nan=True), (r, c)
+
+
+def test_render_views_depth_zero_and_mixed_sizes(ctx):
+    """ray_depth 0 folds a zero sample into every view of a multi-view call."""
+    s, cams, rp = scenes.config3(scale=0.02, n_views=3)
+    ctx.upload(s)
+    for v, cam in enumerate(cams):
+        cam.samples_per_pixel = 4
+        ctx.set_camera(v, cam, v)
+        ctx.reset(v)
+    ctx.render_views([0, 1, 2], abi.render_desc(1, 3, rp["depth"], 1))
+    before = [ctx.read(v) for v in range(3)]
+    ctx.render_views([2, 0, 1], abi.render_desc(3, 4, 0, 1))
+    for v in range(3):
+        after = ctx.read(v)
+        np.testing.assert_allclose(after, before[v] * np.float32(2.0 / 3.0), rtol=1e-6, atol=1e-7)
