@@ -539,9 +539,8 @@ __device__ __forceinline__ uint32_t start_path(const GView& V, PathPtrs& P, uint
     P.o[p] = make_float4(r.o.x, r.o.y, r.o.z, __int_as_float(0));
     P.d[p] = make_float4(r.d.x, r.d.y, r.d.z, __int_as_float((int)s));
     P.atten[p] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
-    P.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.diff[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.nrm[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // L, diff and nrm are not cleared: the first segment of the path writes
+    // all three (radiance counted from zero at bounce 0)
     P.draw[p] = g.draw;
     P.finished[p] = 0;
     return p;
@@ -709,7 +708,12 @@ k_extend(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A, StackCfg SC) {
         int oi = scene_closest<TY>(S, r, rec, st, of);
         if (of) atomicOr(P.err, 1);
         if (oi < 0) {                                                           // trace.cu:19-22
-            float4 a4 = P.atten[path], l4 = P.L[path];
+            const bool first = __float_as_int(o4.w) == 0;
+            float4 a4 = P.atten[path], l4 = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.L[path];
+            if (first) {                                                        // no hit: zero AOVs
+                P.diff[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                P.nrm[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
             f3 Le = sky_sample(S.sky, r.d);
             // fast mode: a scattered ray that finds the sky shares it with the
             // shadow rays of next-event estimation (power heuristic)
@@ -794,8 +798,9 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
     if (i < n) {
         path = P.qmat[(size_t)M * P.qcap + i];
         GHit h = P.hit[path];
-        float4 o4 = P.o[path], d4 = P.d[path], a4 = P.atten[path], l4 = P.L[path];
+        float4 o4 = P.o[path], d4 = P.d[path], a4 = P.atten[path];
         const uint32_t bounce = (uint32_t)__float_as_int(o4.w);
+        const float4 l4 = bounce == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.L[path];
         const uint32_t sample = (uint32_t)__float_as_int(d4.w);
         f3 din = mk3(d4.x, d4.y, d4.z);
         f3 atten = mk3(a4.x, a4.y, a4.z);
@@ -900,12 +905,13 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
                 sc_pdf = pdf;
             }
         }
+        if (bounce == 0) {                                                      // trace.cu:67-70 (zero when
+            f3 dv = s3(0.0f), nv = s3(0.0f);                                    // the path ends here)
+            if (alive) { dv = dv + atten; nv = nv + nn; }
+            P.diff[path] = make_float4(dv.x, dv.y, dv.z, 0.0f);
+            P.nrm[path] = make_float4(nv.x, nv.y, nv.z, 0.0f);
+        }
         if (alive) {
-            if (bounce == 0) {                                                  // trace.cu:67-70
-                f3 dv = s3(0.0f) + atten, nv = s3(0.0f) + nn;
-                P.diff[path] = make_float4(dv.x, dv.y, dv.z, 0.0f);
-                P.nrm[path] = make_float4(nv.x, nv.y, nv.z, 0.0f);
-            }
             if (A.do_rr && bounce >= A.rr_start && bounce + 1 < A.depth) {      // trace.cu:72-77
                 float survival = fmaxf(atten.x, fmaxf(atten.y, atten.z));
                 survival = fminf(survival, A.rr_survival);
