@@ -207,6 +207,69 @@ __device__ __forceinline__ int block_append(bool pred, int32_t* counter) {
     return pred ? wbase[wave] + __popcll(below) : -1;
 }
 
+// workgroup-level append of `cnt` (< NR_SLOTS_MAX + 1) items per thread with
+// one atomic per workgroup: item k of a thread lands at pos[k]; the
+// workgroup's items are contiguous, ordered by rank k, then by thread.  When
+// `tally` is given, the workgroup's sum of `t` is added to it (one more
+// atomic).  Same-address device atomics serialise, so one per workgroup
+// instead of one per rank matters for launches of thousands of workgroups.
+template <int BLOCK>
+__device__ __forceinline__ void block_append_ranks(uint32_t cnt, int32_t* counter, int* pos, uint32_t t,
+                                                   int32_t* tally) {
+    constexpr int NW = BLOCK / 64;
+    constexpr int NE = NR_SLOTS_MAX * NW;              // (rank, wave) entries, rank-major
+    constexpr int PER = (NE + 63) / 64;
+    static_assert(NW <= 64, "one wave scans the workgroup");
+    __shared__ int rbase[NE];
+    __shared__ int tsum[NW];
+    const int wave = threadIdx.x >> 6, lane = __lane_id();
+    const unsigned long long below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    unsigned long long m[NR_SLOTS_MAX];
+    int tw = 0;
+#pragma unroll
+    for (int k = 0; k < NR_SLOTS_MAX; k++) {
+        m[k] = __ballot(cnt > (uint32_t)k);
+        tw += __popcll(__ballot(t > (uint32_t)k));
+        if (lane == 0) rbase[k * NW + wave] = __popcll(m[k]);
+    }
+    if (lane == 0) tsum[wave] = tw;
+    __syncthreads();
+    if (wave == 0) {
+        int v[PER], sum = 0;
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int e = lane * PER + j;
+            v[j] = e < NE ? rbase[e] : 0;
+            sum += v[j];
+        }
+        int inc = sum;                                  // inclusive scan over the lanes
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(inc, d);
+            if (lane >= d) inc += y;
+        }
+        const int total = __shfl(inc, 63);
+        int ts = lane < NW ? tsum[lane] : 0;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) ts += __shfl_xor(ts, d);
+        int base = 0;
+        if (lane == 0) {
+            if (total) base = atomicAdd(counter, total);
+            if (tally && ts) atomicAdd(tally, ts);
+        }
+        int run = __shfl(base, 0) + inc - sum;
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int e = lane * PER + j;
+            if (e < NE) rbase[e] = run;
+            run += v[j];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NR_SLOTS_MAX; k++) pos[k] = rbase[k * NW + wave] + __popcll(m[k] & below);
+}
+
 // workgroup-level append of each thread's item to queue `key` (1 <= key < NQ,
 // 0 = none): one atomic per workgroup and non-empty queue.
 template <int BLOCK, int NQ>
@@ -660,13 +723,13 @@ __global__ void __launch_bounds__(256) k_start(PathPtrs P, uint32_t s, TraceArgs
         P.nf[g] = s;
         P.ns[g] = s + m;
     }
-    for (uint32_t k = 0; k < P.slots; k++) {                       // uniform trip count
-        const bool go = valid && k < m;
-        int32_t path = 0;
-        if (go) path = (int32_t)start_path(A.views[v], P, x, y, s + k, A);
-        else if (valid) P.finished[g + ((s + k) % P.slots) * P.cap] = 2;
-        const int slot = block_append<256>(go, &P.counts[CNT_ACT + 0]);
-        if (go) P.qact[slot] = path;
+    int pos[NR_SLOTS_MAX];
+    block_append_ranks<256>(m, &P.counts[CNT_ACT + 0], pos, 0u, nullptr);
+#pragma unroll
+    for (uint32_t k = 0; k < NR_SLOTS_MAX; k++) {
+        if (!valid || k >= P.slots) break;
+        if (k < m) P.qact[pos[k]] = (int32_t)start_path(A.views[v], P, x, y, s + k, A);
+        else P.finished[g + ((s + k) % P.slots) * P.cap] = 2;
     }
 }
 
@@ -1002,14 +1065,11 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) 
         P.ns[i] = ns;
         occ = ns - nf;
     }
-    int32_t* live = &P.counts[CNT_ACT + (1 - A.in)];
+    int pos[NR_SLOTS_MAX];
+    block_append_ranks<REGEN_BLOCK>(n_start, &P.counts[CNT_ACT + (1 - A.in)], pos, occ, &P.counts[CNT_OCC]);
 #pragma unroll
-    for (int k = 0; k < NR_SLOTS_MAX; k++) {
-        if ((uint32_t)k >= P.slots) break;                       // uniform
-        int slot = block_append<REGEN_BLOCK>((uint32_t)k < n_start, live);
-        if ((uint32_t)k < n_start) P.qact[(size_t)(1 - A.in) * P.qcap + slot] = (int32_t)started[k];
-        (void)block_append<REGEN_BLOCK>((uint32_t)k < occ, &P.counts[CNT_OCC]);
-    }
+    for (int k = 0; k < NR_SLOTS_MAX; k++)
+        if ((uint32_t)k < n_start) P.qact[(size_t)(1 - A.in) * P.qcap + pos[k]] = (int32_t)started[k];
 }
 
 // ray_depth 0: every pass adds a zero sample (trace.cu:63 runs no bounce)
