@@ -47,7 +47,7 @@ TRAVERSAL_BYTES_PER_PATH = 1942.58
 # a 1 GiB device-to-device copy gives) + WRITE_SIZE (x 1.0) over 18,662,400 paths per estimator,
 # default tuning (4 path slots per pixel):
 # profiles/k_extend_c2_1080p.md.
-TRAVERSAL_HBM_BYTES_PER_PATH = {0: 864.6, 1: 944.1}    # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
+TRAVERSAL_HBM_BYTES_PER_PATH = {0: 960.5, 1: 1041.3}    # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
 HBM_PEAK_GBS = 8000.0
 
 
