@@ -38,6 +38,7 @@
 #include <numeric>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/nectar_denoise.h"
@@ -2116,19 +2117,31 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
         // from one iteration to the next and bound the live paths, so a count
         // read back two iterations late is an upper bound for the traversal
         // grid and each group always has its next iteration queued.
+        // With several groups the host takes whichever group's count is back
+        // (event query) and only spins when none is: waiting on one group
+        // while another one's queue runs dry leaves that group's stream idle.
         const int64_t max_iters = (int64_t)(d->sample_end - d->sample_begin) * d->ray_depth + 2;
         int active = G;
         while (active > 0) {
+            bool progressed = false;
             for (int g = 0; g < G; g++) {
                 Group& gr = grp[g];
                 if (gr.done) continue;
                 const int64_t it = gr.it;
                 int32_t* live = c->h_live + 2 * g;
                 if (it >= 2) {
-                    HIPCK(hipEventSynchronize(c->gev[g][it & 1]));
+                    if (G > 1) {
+                        const hipError_t q = hipEventQuery(c->gev[g][it & 1]);
+                        if (q == hipErrorNotReady) continue;
+                        HIPCK(q);
+                    } else {
+                        HIPCK(hipEventSynchronize(c->gev[g][it & 1]));
+                    }
+                    progressed = true;
                     gr.bound = live[it & 1];
                     if (gr.bound == 0) { gr.done = true; active--; continue; }
                 }
+                progressed = true;
                 if (it > max_iters) return fail(NR_ERR_STATE, "streaming loop did not drain");
                 PathPtrs& P = gr.P;
                 TraceArgs& A = gr.A;
@@ -2146,8 +2159,10 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
                     else hipLaunchKernelGGL((k_extend<TYI, false>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
                 }
                 mark(1);
-                if (fast) launch_shade<true>(c, gS, st, P, A);
-                else launch_shade<false>(c, gS, st, P, A);
+                // the shading grid-stride loops need no more workgroups than live paths
+                const dim3 gSi((uint32_t)std::max<int64_t>(1, std::min<int64_t>(gS.x, (gr.bound + 255) / 256)));
+                if (fast) launch_shade<true>(c, gSi, st, P, A);
+                else launch_shade<false>(c, gSi, st, P, A);
                 if (fast && c->scene.sky.marg) {
                     mark(4);
                     if (other) hipLaunchKernelGGL(k_shadow<NR_TY_ALL>, gx, dim3(128), lds, st, c->d_scene, P, gr.SC);
@@ -2162,6 +2177,7 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
                 A.in = 1 - A.in;
                 gr.it++;
             }
+            if (!progressed) std::this_thread::yield();
         }
         c->last_iterations = 0;
         for (int g = 0; g < G; g++) c->last_iterations += grp[g].it;
