@@ -90,13 +90,16 @@ class OracleScene:
     def __init__(self, scene: abi.SceneDesc):
         self._keep = scene
         d = scene.to_c()
+        self._free = lib().nro_scene_free
         self.h = lib().nro_scene_build(C.byref(d))
         if not self.h:
             raise RuntimeError("oracle scene build failed")
 
     def __del__(self):
+        # the free function is bound at construction: at interpreter exit the
+        # module's globals may already be gone
         if getattr(self, "h", None):
-            lib().nro_scene_free(self.h)
+            self._free(self.h)
             self.h = None
 
     def trace_closest(self, rays: np.ndarray, threads: int = 0) -> np.ndarray:
@@ -126,11 +129,12 @@ class OracleCamera:
     def __init__(self, cam: abi.NrCameraDesc, seed: int):
         self.desc = cam
         self.H, self.W = int(cam.resolution[0]), int(cam.resolution[1])
+        self._free = lib().nro_camera_free
         self.h = lib().nro_camera_build(C.byref(cam), seed)
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().nro_camera_free(self.h)
+            self._free(self.h)
             self.h = None
 
     def ray(self, x, y, s, seed):
