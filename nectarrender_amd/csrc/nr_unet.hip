@@ -657,6 +657,14 @@ void launch_conv(nr_unet* u, const Layer& l, bool backward, View in, int cin_gro
     a.out_f32 = out_f32; a.resid = resid;
     const int cpad = up16(cout);
     int TM = (cpad % 64 == 0) ? 4 : (cpad % 48 == 0) ? 3 : (cpad % 32 == 0) ? 2 : 1;
+    const int64_t tiles = (int64_t)((W + CL_TILE_W - 1) / CL_TILE_W) * (N * ((H + CL_TILE_H - 1) / CL_TILE_H));
+    // coarse levels have few pixel tiles: fewer output channels per workgroup
+    // (same arithmetic per channel) until the launch has about 4 workgroups per CU
+    while (TM > 1 && tiles * (cpad / (16 * TM)) < 1024) {
+        int t = TM - 1;
+        while (t > 1 && (cpad / 16) % t != 0) t--;
+        TM = t;
+    }
     dim3 grid((W + CL_TILE_W - 1) / CL_TILE_W, N * ((H + CL_TILE_H - 1) / CL_TILE_H), cpad / (16 * TM));
     dim3 block(256);
     const int mode = a.pool ? 2 : a.up ? 1 : 0;
