@@ -234,3 +234,23 @@ def test_autotune_picks_a_valid_tuning_and_clears_the_view(ctx, oracle):
         _assert_same(g[0], o[0], "after autotune")
     finally:
         ctx.set_tuning(4, 2, 8)
+
+
+def test_scene_without_objects(ctx, oracle):
+    """No objects, only an HDRI sky: every camera ray misses (scene_closest and scene_occluded return
+    before touching a tree).  Bit-exact against the oracle with the reference's estimator; with MIS the
+    AOVs are zero and the beauty is the same sky (other jitter, so compared by its mean)."""
+    s = abi.SceneDesc()
+    s.sky_hdri(scenes.procedural_hdri(64, 32, seed=3, sigma=0.4, sun_radiance=1e3, sun_diameter_deg=2.0,
+                                      sun_elevation_deg=40, sun_azimuth_deg=30))
+    cam = abi.camera_desc((20, 30), (0, 0.5, 3.0), (-5, 0, 0), 5, 2.0, 3.0, 0.0, 2.0, 1.0)
+    rp = {"depth": 8, "seed": 3}
+    g, o = _render_both(ctx, oracle, s, cam, rp, 1, 4)
+    for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
+        _assert_same(g[k], o[k], name_)
+    ctx.reset(0)
+    ctx.render(0, abi.render_desc(0, 4, rp["depth"], rp["seed"], mode=abi.MODE_FAST))
+    fast = [ctx.read(0, L) for L in (abi.LAYER_BEAUTY, abi.LAYER_DIFFUSE, abi.LAYER_WORLD_NORMAL)]
+    assert np.isfinite(fast[0]).all()
+    assert not fast[1].any() and not fast[2].any()
+    assert abs(float(fast[0].mean()) / float(o[0].mean()) - 1.0) < 0.05
