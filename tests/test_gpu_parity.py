@@ -254,3 +254,14 @@ def test_scene_without_objects(ctx, oracle):
     assert np.isfinite(fast[0]).all()
     assert not fast[1].any() and not fast[2].any()
     assert abs(float(fast[0].mean()) / float(o[0].mean()) - 1.0) < 0.05
+
+
+@pytest.mark.parametrize("hw", [(1, 1), (1, 37), (37, 1)])
+def test_degenerate_view_sizes(ctx, oracle, hw):
+    """Views of one pixel, one row and one column (8x8 tiles almost entirely outside the view)."""
+    s, cam, rp = scenes.mixed_scene(spp=4, res=16)
+    cam.resolution[0], cam.resolution[1] = hw
+    g, o = _render_both(ctx, oracle, s, cam, rp, 1, 3)
+    for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
+        assert g[k].shape == (3, hw[0], hw[1])
+        _assert_same(g[k], o[k], name_)
