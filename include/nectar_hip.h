@@ -265,8 +265,9 @@ int nr_render_views(nr_ctx* ctx, const int32_t* views, int n_views, const nr_ren
 
 /* nr_render restricted per tile: pixel (x, y) renders only the passes of
  * [sample_begin, sample_end) below tile_end[(y / tile_size) * cols + x / tile_size]
- * (host array, cols = ceil(W / tile_size), rows = ceil(H / tile_size); tile_size
- * a power of two in [8, 4096]).  A tile whose limit is at most sample_begin keeps
+ * (host array of exactly rows x cols values, cols = ceil(W / tile_size), rows =
+ * ceil(H / tile_size), which the call reads in full; tile_size a power of two in
+ * [8, 4096]).  A tile whose limit is at most sample_begin keeps
  * its AOVs; rendering from the first pass (0 in NR_MODE_FAST, 1 otherwise)
  * restarts a tile's running mean.  The per-tile fallback of a denoised frame to
  * more samples; no counterpart in the reference. */
@@ -407,7 +408,9 @@ int nr_mesh_tree(nr_ctx* ctx, int mesh, void* nodes64, int32_t* slot_orig, int32
 int nr_object_set_transform(nr_ctx* ctx, int32_t index, const float position[3], const float rotation[3],
                             const float scale[3]);
 
-/* Wait for all work of the context. */
+/* Wait for all work of the context.  NR_ERR_LIMIT if the last nr_render* call's
+ * traversal overflowed its stack (the flag covers that call only; nr_aov_read
+ * reports it too). */
 int nr_sync(nr_ctx* ctx);
 
 /* Mesh-tree layout of later nr_trace_closest(_device) calls: 0 = the

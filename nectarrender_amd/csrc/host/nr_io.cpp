@@ -266,6 +266,14 @@ int paeth(int a, int b, int c) {
     return c;
 }
 
+// Largest width or height any image reader accepts (stb_image's STBI_MAX_DIMENSIONS).
+constexpr uint32_t MAX_DIM = 1u << 24;
+
+// PNG (ISO/IEC 15948): every colour type / bit depth pair the specification
+// allows, Adam7 interlacing, tRNS transparency.  Pairs the specification
+// forbids, a tRNS chunk on a type that already has alpha, a tRNS chunk of the
+// wrong size and images larger than MAX_DIM on a side are rejected, as
+// stb_image rejects them.
 bool decode_png(const std::string& buf, Raw8& out, std::string& err) {
     const uint8_t* d = (const uint8_t*)buf.data();
     const size_t n = buf.size();
@@ -284,9 +292,29 @@ bool decode_png(const std::string& buf, Raw8& out, std::string& err) {
         if (type == "IHDR") {
             if (len < 13) { err = "bad IHDR"; return false; }
             W = be32(c); H = be32(c + 4); depth = c[8]; ctype = c[9]; interlace = c[12];
+            if (W == 0 || H == 0 || W > MAX_DIM || H > MAX_DIM) { err = "PNG size out of range"; return false; }
+            if (interlace > 1) { err = "bad PNG interlace method"; return false; }
+            bool ok;
+            switch (ctype) {
+                case 0: ok = depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16; break;
+                case 3: ok = depth == 1 || depth == 2 || depth == 4 || depth == 8; break;
+                case 2: case 4: case 6: ok = depth == 8 || depth == 16; break;
+                default: err = "bad PNG colour type"; return false;
+            }
+            if (!ok) { err = "bad PNG bit depth for its colour type"; return false; }
         } else if (type == "PLTE") {
+            if (len == 0 || len % 3 != 0 || len > 768) { err = "bad PLTE"; return false; }
             pal.assign(c, c + len);
         } else if (type == "tRNS") {
+            if (ctype < 0) { err = "tRNS before IHDR"; return false; }
+            if (ctype == 4 || ctype == 6) { err = "tRNS with alpha"; return false; }
+            if (ctype == 3) {
+                if (pal.empty()) { err = "tRNS before PLTE"; return false; }
+                if (len > pal.size() / 3) { err = "bad tRNS length"; return false; }
+            } else if (len != (ctype == 0 ? 2u : 6u)) {
+                err = "bad tRNS length";
+                return false;
+            }
             trns.assign(c, c + len);
             has_trns = true;
         } else if (type == "IDAT") {
@@ -297,56 +325,55 @@ bool decode_png(const std::string& buf, Raw8& out, std::string& err) {
         p += 12 + (size_t)len;
     }
     if (W == 0 || H == 0 || ctype < 0) { err = "PNG without IHDR"; return false; }
-    if (interlace) { err = "interlaced PNG is not supported"; return false; }
-    int spp;   // samples per pixel in the file
-    switch (ctype) {
-        case 0: spp = 1; break;
-        case 2: spp = 3; break;
-        case 3: spp = 1; break;
-        case 4: spp = 2; break;
-        case 6: spp = 4; break;
-        default: err = "bad PNG colour type"; return false;
-    }
-    if (!(depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16)) { err = "bad PNG bit depth"; return false; }
+    if (ctype == 3 && pal.empty()) { err = "PNG palette image without PLTE"; return false; }
+    const int spp = ctype == 2 ? 3 : (ctype == 4 ? 2 : (ctype == 6 ? 4 : 1));   // samples per pixel in the file
     const size_t bits_pp = (size_t)spp * depth;
-    const size_t stride = (W * bits_pp + 7) / 8;
     const size_t bpp = std::max<size_t>(1, bits_pp / 8);
-    std::vector<uint8_t> raw((stride + 1) * H);
+    // sub-images: the whole image, or the seven Adam7 passes (x0, y0, dx, dy)
+    static const uint32_t adam7[7][4] = {{0, 0, 8, 8}, {4, 0, 8, 8}, {0, 4, 4, 8}, {2, 0, 4, 4},
+                                         {0, 2, 2, 4}, {1, 0, 2, 2}, {0, 1, 1, 2}};
+    struct Pass { uint32_t x0, y0, dx, dy, w, h; size_t stride, off; };
+    std::vector<Pass> passes;
+    size_t total = 0;
+    for (int k = 0; k < (interlace ? 7 : 1); k++) {
+        Pass ps{0, 0, 1, 1, W, H, 0, 0};
+        if (interlace) {
+            ps.x0 = adam7[k][0]; ps.y0 = adam7[k][1]; ps.dx = adam7[k][2]; ps.dy = adam7[k][3];
+            ps.w = W > ps.x0 ? (W - ps.x0 + ps.dx - 1) / ps.dx : 0;
+            ps.h = H > ps.y0 ? (H - ps.y0 + ps.dy - 1) / ps.dy : 0;
+        }
+        if (ps.w == 0 || ps.h == 0) continue;
+        ps.stride = ((size_t)ps.w * bits_pp + 7) / 8;
+        ps.off = total;
+        total += (ps.stride + 1) * ps.h;
+        passes.push_back(ps);
+    }
+    const size_t outC0 = ctype == 3 ? (has_trns ? 4 : 3) : spp + (has_trns ? 1 : 0);
+    if (total > ((size_t)1 << 32) || (size_t)W * H * outC0 > ((size_t)1 << 32)) { err = "PNG too large"; return false; }
+    std::vector<uint8_t> raw(total);
     z_stream zs{};
     if (inflateInit(&zs) != Z_OK) { err = "zlib init failed"; return false; }
     zs.next_in = idat.data();
     zs.avail_in = (uInt)idat.size();
-    zs.next_out = raw.data();
-    zs.avail_out = (uInt)raw.size();
-    int zr = inflate(&zs, Z_FINISH);
-    inflateEnd(&zs);
-    if (zr != Z_STREAM_END && !(zr == Z_BUF_ERROR && zs.avail_out == 0)) { err = "corrupt PNG data"; return false; }
-    if (zs.total_out < raw.size()) { err = "PNG data too short"; return false; }
-    std::vector<uint8_t> img(stride * H);
-    std::vector<uint8_t> prev(stride, 0);
-    for (uint32_t y = 0; y < H; y++) {
-        const uint8_t* src = &raw[y * (stride + 1)];
-        int f = src[0];
-        src++;
-        uint8_t* cur = &img[y * stride];
-        for (size_t i = 0; i < stride; i++) {
-            int a = i >= bpp ? cur[i - bpp] : 0, b = prev[i], cc = i >= bpp ? prev[i - bpp] : 0;
-            int v = src[i];
-            switch (f) {
-                case 0: break;
-                case 1: v += a; break;
-                case 2: v += b; break;
-                case 3: v += (a + b) >> 1; break;
-                case 4: v += paeth(a, b, cc); break;
-                default: err = "bad PNG filter"; return false;
-            }
-            cur[i] = (uint8_t)v;
-        }
-        std::memcpy(prev.data(), cur, stride);
+    size_t done = 0;
+    int zr = Z_OK;
+    while (done < total) {   // inflate in pieces: avail_out is 32 bits
+        const size_t chunk = std::min<size_t>(total - done, (size_t)1 << 30);
+        zs.next_out = raw.data() + done;
+        zs.avail_out = (uInt)chunk;
+        zr = inflate(&zs, Z_FINISH);
+        done += chunk - zs.avail_out;
+        if (zr == Z_STREAM_END || (zr != Z_OK && zr != Z_BUF_ERROR) || zs.avail_out != 0) break;
     }
+    inflateEnd(&zs);
+    if (zr != Z_STREAM_END && !(zr == Z_BUF_ERROR && done == total) && !(zr == Z_OK && done == total)) {
+        err = "corrupt PNG data";
+        return false;
+    }
+    if (done < total) { err = "PNG data too short"; return false; }
     // palette images expand to RGB(A); grey/RGB with tRNS gain an alpha channel
     // (0 where the pixel equals the key colour), which counts as a channel
-    const int outC = ctype == 3 ? (has_trns ? 4 : 3) : spp + (has_trns ? 1 : 0);
+    const int outC = (int)outC0;
     out.C = outC;
     out.Cout = outC;
     out.H = (int)H; out.W = (int)W;
@@ -354,46 +381,67 @@ bool decode_png(const std::string& buf, Raw8& out, std::string& err) {
     static const int scale[9] = {0, 0xff, 0x55, 0, 0x11, 0, 0, 0, 0x01};
     uint16_t key[3] = {0, 0, 0};
     if (has_trns && ctype != 3)
-        for (int k = 0; k < spp && (size_t)2 * k + 1 < trns.size(); k++) {
+        for (int k = 0; k < spp; k++) {
             uint16_t v = (uint16_t)(trns[2 * k] << 8 | trns[2 * k + 1]);
             key[k] = depth == 16 ? v : (uint16_t)((v & 255) * scale[depth]);
         }
-    for (uint32_t y = 0; y < H; y++) {
-        const uint8_t* row = &img[y * stride];
-        for (uint32_t x = 0; x < W; x++) {
-            uint8_t* o = &out.hwc[((size_t)y * W + x) * outC];
-            if (ctype == 3) {
-                int idx;
-                if (depth == 8) {
-                    idx = row[x];
-                } else {
-                    size_t bit = (size_t)x * depth;
-                    idx = (row[bit / 8] >> (8 - depth - (int)(bit % 8))) & ((1 << depth) - 1);
+    auto pixel = [&](const uint8_t* row, uint32_t x, uint8_t* o) {
+        if (ctype == 3) {
+            int idx;
+            if (depth == 8) {
+                idx = row[x];
+            } else {
+                size_t bit = (size_t)x * depth;
+                idx = (row[bit / 8] >> (8 - depth - (int)(bit % 8))) & ((1 << depth) - 1);
+            }
+            for (int k = 0; k < 3; k++) o[k] = (size_t)idx * 3 + k < pal.size() ? pal[idx * 3 + k] : 0;
+            if (outC == 4) o[3] = (size_t)idx < trns.size() ? trns[idx] : 255;
+            return;
+        }
+        uint16_t v16[4];
+        for (int k = 0; k < spp; k++) {
+            if (depth == 16) {
+                v16[k] = (uint16_t)(row[((size_t)x * spp + k) * 2] << 8 | row[((size_t)x * spp + k) * 2 + 1]);
+                o[k] = row[((size_t)x * spp + k) * 2];                                  // high byte
+            } else if (depth == 8) {
+                v16[k] = row[(size_t)x * spp + k];
+                o[k] = row[(size_t)x * spp + k];
+            } else {   // 1, 2, 4-bit grey
+                size_t bit = (size_t)x * depth;
+                int v = (row[bit / 8] >> (8 - depth - (int)(bit % 8))) & ((1 << depth) - 1);
+                v16[k] = (uint16_t)(v * scale[depth]);
+                o[k] = (uint8_t)(v * scale[depth]);
+            }
+        }
+        if (has_trns) {
+            bool eq = true;
+            for (int k = 0; k < spp; k++) eq = eq && v16[k] == key[k];
+            o[spp] = eq ? 0 : 255;
+        }
+    };
+    for (const Pass& ps : passes) {
+        std::vector<uint8_t> cur(ps.stride), prev(ps.stride, 0);   // the filters' previous row restarts per pass
+        for (uint32_t y = 0; y < ps.h; y++) {
+            const uint8_t* src = &raw[ps.off + (size_t)y * (ps.stride + 1)];
+            int f = src[0];
+            src++;
+            for (size_t i = 0; i < ps.stride; i++) {
+                int a = i >= bpp ? cur[i - bpp] : 0, b = prev[i], cc = i >= bpp ? prev[i - bpp] : 0;
+                int v = src[i];
+                switch (f) {
+                    case 0: break;
+                    case 1: v += a; break;
+                    case 2: v += b; break;
+                    case 3: v += (a + b) >> 1; break;
+                    case 4: v += paeth(a, b, cc); break;
+                    default: err = "bad PNG filter"; return false;
                 }
-                for (int k = 0; k < 3; k++) o[k] = (size_t)idx * 3 + k < pal.size() ? pal[idx * 3 + k] : 0;
-                if (outC == 4) o[3] = (size_t)idx < trns.size() ? trns[idx] : 255;
-                continue;
+                cur[i] = (uint8_t)v;
             }
-            uint16_t v16[4];
-            for (int k = 0; k < spp; k++) {
-                if (depth == 16) {
-                    v16[k] = (uint16_t)(row[(x * spp + k) * 2] << 8 | row[(x * spp + k) * 2 + 1]);
-                    o[k] = row[(x * spp + k) * 2];                                  // high byte
-                } else if (depth == 8) {
-                    v16[k] = row[x * spp + k];
-                    o[k] = row[x * spp + k];
-                } else {   // 1, 2, 4-bit grey
-                    size_t bit = (size_t)x * depth;
-                    int v = (row[bit / 8] >> (8 - depth - (int)(bit % 8))) & ((1 << depth) - 1);
-                    v16[k] = (uint16_t)(v * scale[depth]);
-                    o[k] = (uint8_t)(v * scale[depth]);
-                }
-            }
-            if (has_trns) {
-                bool eq = true;
-                for (int k = 0; k < spp; k++) eq = eq && v16[k] == key[k];
-                o[spp] = eq ? 0 : 255;
-            }
+            const size_t oy = ps.y0 + (size_t)y * ps.dy;
+            for (uint32_t x = 0; x < ps.w; x++)
+                pixel(cur.data(), x, &out.hwc[(oy * W + ps.x0 + (size_t)x * ps.dx) * outC]);
+            std::swap(prev, cur);
         }
     }
     return true;
@@ -424,7 +472,8 @@ bool decode_pnm(const std::string& b, Raw8& out, std::string& err) {
     int C = b[1] == '5' ? 1 : 3;
     if (v[2] <= 0 || v[2] > 255) { err = "only 8-bit PNM is supported"; return false; }
     size_t need = (size_t)v[0] * v[1] * C;
-    if (v[0] <= 0 || v[1] <= 0 || p + need > b.size()) { err = "truncated PNM"; return false; }
+    if (v[0] <= 0 || v[1] <= 0 || (uint32_t)v[0] > MAX_DIM || (uint32_t)v[1] > MAX_DIM) { err = "PNM size out of range"; return false; }
+    if (p + need > b.size()) { err = "truncated PNM"; return false; }
     out.C = C; out.Cout = C; out.W = v[0]; out.H = v[1];
     out.hwc.assign(b.begin() + p, b.begin() + p + need);
     return true;
@@ -452,7 +501,8 @@ bool decode_hdr(const std::string& b, RawF& out, std::string& err) {
     if (!fmt) { err = "unsupported HDR format"; return false; }
     line(s);
     int H = 0, W = 0;
-    if (std::sscanf(s.c_str(), "-Y %d +X %d", &H, &W) != 2 || H <= 0 || W <= 0) {
+    if (std::sscanf(s.c_str(), "-Y %d +X %d", &H, &W) != 2 || H <= 0 || W <= 0 || (uint32_t)H > MAX_DIM ||
+        (uint32_t)W > MAX_DIM) {
         err = "unsupported HDR orientation";
         return false;
     }

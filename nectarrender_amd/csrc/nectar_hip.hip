@@ -734,9 +734,14 @@ __global__ void __launch_bounds__(256) k_start(PathPtrs P, uint32_t s, TraceArgs
     }
 }
 
-// every counter of a group but the sticky overflow flag
+// every counter of a group but the overflow flag (shared by the groups; cleared
+// once per render call by k_clear_err)
 __global__ void k_clear_counts(int32_t* counts) {
     if (threadIdx.x < CNT_N && threadIdx.x != CNT_ERR) counts[threadIdx.x] = 0;
+}
+
+__global__ void k_clear_err(int32_t* err) {
+    if (threadIdx.x == 0) *err = 0;
 }
 
 // Clears the counters one iteration appends to: material queues and the next live queue.
@@ -1073,11 +1078,16 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) 
         if ((uint32_t)k < n_start) P.qact[(size_t)(1 - A.in) * P.qcap + pos[k]] = (int32_t)started[k];
 }
 
-// ray_depth 0: every pass adds a zero sample (trace.cu:63 runs no bounce)
-__global__ void __launch_bounds__(256) k_fold_zero(float* beauty, float* diffuse, float* normal, uint32_t n, uint32_t s) {
+// ray_depth 0: every pass adds a zero sample (trace.cu:63 runs no bounce).  Pass s
+// is folded as sample number fs; with per-tile limits (tile_end non-null, a view
+// W pixels wide) a pixel whose tile ends at or before pass s keeps its AOVs.
+__global__ void __launch_bounds__(256) k_fold_zero(float* beauty, float* diffuse, float* normal, uint32_t n, uint32_t s,
+                                                   uint32_t fs, uint32_t W, const uint32_t* __restrict__ tile_end,
+                                                   uint32_t tile_shift, uint32_t tile_cols) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float fn = (float)s;
+    if (tile_end && tile_end[((i / W) >> tile_shift) * tile_cols + ((i % W) >> tile_shift)] <= s) return;
+    float fn = (float)fs;
     float a = (fn - 1.0f) / fn, b = 1.0f / fn;
     float* layers[3] = {beauty, diffuse, normal};
     for (float* l : layers)
@@ -2007,6 +2017,9 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
     const uint32_t N = (uint32_t)npix, T = (uint32_t)tiles;
     int e = ensure_paths(c, N, T * 64);
     if (e) return e;
+    // the traversal-overflow flag reports this call only (read by nr_sync, nr_aov_read
+    // and timing mode)
+    hipLaunchKernelGGL(k_clear_err, dim3(1), dim3(64), 0, c->stream, c->P.err);
     if (!c->d_views) HIPCK(hipMalloc((void**)&c->d_views, sizeof(GView) * NR_MAX_VIEWS));
     HIPCK(hipStreamSynchronize(c->stream));              // kernels of the previous call read the tables
     HIPCK(hipMemcpy(c->d_views, hv, sizeof(GView) * n_views, hipMemcpyHostToDevice));
@@ -2054,7 +2067,9 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
             const uint32_t n = hv[k].npix;
             for (uint32_t s = d->sample_begin; s < d->sample_end; s++)
                 hipLaunchKernelGGL(k_fold_zero, dim3((n + 255) / 256), dim3(256), 0, c->stream, layer_of(k, NR_LAYER_BEAUTY),
-                                   layer_of(k, NR_LAYER_DIFFUSE), layer_of(k, NR_LAYER_WORLD_NORMAL), n, fast ? s + 1 : s);
+                                   layer_of(k, NR_LAYER_DIFFUSE), layer_of(k, NR_LAYER_WORLD_NORMAL), n, s,
+                                   fast ? s + 1 : s, c->views[ids[k]].W, tl ? tl->d_end : nullptr, tl ? tl->shift : 0u,
+                                   tl ? tl->cols : 0u);
         }
     } else if (d->sample_end > d->sample_begin) {
         // Stream groups: G contiguous ranges of whole tiles, each with its own
@@ -2120,8 +2135,25 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
         // With several groups the host takes whichever group's count is back
         // (event query) and only spins when none is: waiting on one group
         // while another one's queue runs dry leaves that group's stream idle.
+        // Every exit from here on, error returns included, makes the context stream
+        // wait for the group streams, so a later synchronisation of the context
+        // stream (before path or scene memory is freed) covers their kernels.
+        struct GroupJoin {
+            nr_ctx* c;
+            int G;
+            const Group* grp;
+            ~GroupJoin() {
+                if (G <= 1) return;
+                for (int g = 0; g < G; g++) {
+                    (void)hipEventRecord(c->gev_done[g], grp[g].st);
+                    (void)hipStreamWaitEvent(c->stream, c->gev_done[g], 0);
+                }
+            }
+        };
         const int64_t max_iters = (int64_t)(d->sample_end - d->sample_begin) * d->ray_depth + 2;
         int active = G;
+        {
+        GroupJoin join{c, G, grp};
         while (active > 0) {
             bool progressed = false;
             for (int g = 0; g < G; g++) {
@@ -2181,12 +2213,7 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
         }
         c->last_iterations = 0;
         for (int g = 0; g < G; g++) c->last_iterations += grp[g].it;
-        if (G > 1) {
-            for (int g = 0; g < G; g++) {
-                HIPCK(hipEventRecord(c->gev_done[g], grp[g].st));
-                HIPCK(hipStreamWaitEvent(c->stream, c->gev_done[g], 0));
-            }
-        }
+        }   // GroupJoin: the context stream waits for every group
         mark(2);
         auto w = write_guards();
         for (int k = 0; k < n_views; k++)
@@ -2237,8 +2264,8 @@ int nr_aov_read(nr_ctx* c, int view, int layer, float* host, size_t n_floats) {
     HIPCK(hipMemcpyAsync(host, v.layer[layer], need * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     int32_t err = 0;
-    if (c->P.counts) {
-        HIPCK(hipMemcpy(&err, c->P.counts + CNT_ERR, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (c->P.err) {                                     // the last render's traversal-overflow flag
+        HIPCK(hipMemcpy(&err, c->P.err, sizeof(int32_t), hipMemcpyDeviceToHost));
         if (err) return fail(NR_ERR_LIMIT, "traversal stack overflow during render");
     }
     return NR_OK;
@@ -2733,6 +2760,11 @@ int nr_sync(nr_ctx* c) {
     if (!c) return fail(NR_ERR_INVALID, "null context");
     HIPCK(hipSetDevice(c->device));
     HIPCK(hipStreamSynchronize(c->stream));
+    if (c->P.err) {                                     // the last render's traversal-overflow flag
+        int32_t err = 0;
+        HIPCK(hipMemcpy(&err, c->P.err, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (err) return fail(NR_ERR_LIMIT, "traversal stack overflow during render");
+    }
     return NR_OK;
 }
 

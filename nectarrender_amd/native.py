@@ -142,8 +142,19 @@ class Context:
         return {"slots": t.slots, "stream_groups": t.stream_groups, "shade_wg_per_cu": t.shade_wg_per_cu}
 
     def render_tiles(self, view: int, desc: abi.NrRenderDesc, tile_end: np.ndarray, tile_size: int):
-        """nr_render with a per-tile sample limit (rows x cols uint32, row major)."""
+        """nr_render with a per-tile sample limit (rows x cols uint32, row major, rows =
+        ceil(H / tile_size), cols = ceil(W / tile_size)).  The C entry point reads exactly
+        rows x cols values, so a table of another size is refused here."""
         t = np.ascontiguousarray(tile_end, dtype=np.uint32)
+        H32, W32 = C.c_int32(), C.c_int32()
+        _check(lib().nr_view_size(self._h, view, C.byref(H32), C.byref(W32)))
+        H, W = H32.value, W32.value
+        ts = int(tile_size)
+        if ts <= 0:
+            raise ValueError("tile_size must be positive")
+        need = -(-H // ts) * -(-W // ts)
+        if t.size != need:
+            raise ValueError(f"tile_end holds {t.size} limits; a {H}x{W} view in {ts}-pixel tiles needs {need}")
         _check(lib().nr_render_tiles(self._h, view, C.byref(desc), t.ctypes.data_as(C.POINTER(C.c_uint32)),
                                      int(tile_size)))
 

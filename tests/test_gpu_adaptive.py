@@ -55,6 +55,37 @@ def test_render_tiles_restarts_or_keeps_each_tile(ctx, mode):
             assert np.array_equal(got[sl], want, equal_nan=True), (r, c)
 
 
+@pytest.mark.parametrize("mode", [abi.MODE_PARITY, abi.MODE_FAST])
+def test_render_tiles_depth_zero_keeps_tiles(ctx, mode):
+    """ray_depth 0 folds zero samples, but only into the tiles whose limit admits the pass:
+    a tile limited to 0 keeps the AOVs of an earlier render, bit for bit, and a tile limited
+    within the range folds only its passes (the running mean scales by (n-1)/n per pass)."""
+    s, cam, rp = scenes.mixed_scene(spp=16, res=40)
+    ctx.upload(s)
+    p0 = 0 if mode == abi.MODE_FAST else 1
+    ctx.set_camera(0, with_spp(cam, 12 + p0), 5)
+    ctx.reset(0)
+    ctx.render(0, abi.render_desc(p0, 4 + p0, rp["depth"], 5, mode=mode))
+    before = ctx.read(0)
+    ts = 16
+    ends = np.zeros((3, 3), np.uint32)                       # 40 x 40 in 16-pixel tiles: 3 x 3, ragged
+    ends[0, 0] = 12 + p0                                     # all of passes 4..11 + p0: 8 zero folds
+    ends[1, 2] = 6 + p0                                      # passes 4, 5 (+ p0): 2 zero folds
+    ends[2, 1] = 4 + p0                                      # at the range start: kept
+    ctx.render_tiles(0, abi.render_desc(4 + p0, 12 + p0, 0, 5, mode=mode), ends, ts)
+    got = ctx.read(0)
+    for r in range(3):
+        for c in range(3):
+            sl = (slice(None), slice(r * ts, (r + 1) * ts), slice(c * ts, (c + 1) * ts))
+            want = before[sl].copy()
+            for sp in range(4 + p0, min(12 + p0, int(ends[r, c]))):
+                fn = np.float32(sp + 1 if mode == abi.MODE_FAST else sp)
+                want = want * ((fn - np.float32(1)) / fn) + np.float32(0) * (np.float32(1) / fn)
+            assert np.array_equal(got[sl], want, equal_nan=True), (r, c)
+    with pytest.raises(ValueError):
+        ctx.render_tiles(0, abi.render_desc(4 + p0, 12 + p0, 0, 5, mode=mode), np.zeros(5, np.uint32), ts)
+
+
 def _setup(ctx, res=128):
     s, cam, rp = scenes.config5(n_spheres=60, res=(res, res), spp=1024)
     ctx.upload(s)

@@ -25,12 +25,42 @@ needs_ref = pytest.mark.skipif(not ioref.available(), reason="reference readers 
 # ---------------------------------------------------------------------------
 # writers
 # ---------------------------------------------------------------------------
-def write_png(path, arr, ctype, depth=8, palette=None, trns=None, filters=(0, 1, 2, 3, 4)):
-    """arr: H x W x spp integer samples (or H x W palette indices)."""
+def write_png(path, arr, ctype, depth=8, palette=None, trns=None, filters=(0, 1, 2, 3, 4), interlace=False,
+              ihdr=None):
+    """arr: H x W x spp integer samples (or H x W palette indices).  interlace: Adam7
+    (each pass filtered as an image of its own); ihdr: raw IHDR fields to write instead
+    (W, H, depth, ctype), for malformed files."""
     arr = np.asarray(arr)
     H, W = arr.shape[:2]
     spp = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
     a = arr.reshape(H, W, spp).astype(np.int64)
+    if interlace:
+        raw = bytearray()
+        for x0, y0, dx, dy in ((0, 0, 8, 8), (4, 0, 8, 8), (0, 4, 4, 8), (2, 0, 4, 4), (0, 2, 2, 4), (1, 0, 2, 2),
+                               (0, 1, 1, 2)):
+            sub = a[y0::dy, x0::dx]
+            if sub.shape[0] and sub.shape[1]:
+                raw += _filtered_rows(sub, depth, spp, filters)
+    else:
+        raw = _filtered_rows(a, depth, spp, filters)
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+
+    fw, fh_, fd, fc = ihdr if ihdr is not None else (W, H, depth, ctype)
+    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", fw, fh_, fd, fc, 0, 0, 1 if interlace else 0))
+    if palette is not None:
+        png += chunk(b"PLTE", bytes(np.asarray(palette, np.uint8).reshape(-1)))
+    if trns is not None:
+        png += chunk(b"tRNS", bytes(trns))
+    png += chunk(b"IDAT", zlib.compress(bytes(raw), 6)) + chunk(b"IEND", b"")
+    with open(path, "wb") as fh:
+        fh.write(png)
+
+
+def _filtered_rows(a, depth, spp, filters):
+    """PNG scanlines of one (sub-)image a (h x w x spp), each row with its filter byte."""
+    H = a.shape[0]
     rows = []
     for y in range(H):
         if depth == 16:
@@ -70,18 +100,7 @@ def write_png(path, arr, ctype, depth=8, palette=None, trns=None, filters=(0, 1,
         raw.append(f)
         raw += out
         prev = line
-
-    def chunk(t, d):
-        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
-
-    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", W, H, depth, ctype, 0, 0, 0))
-    if palette is not None:
-        png += chunk(b"PLTE", bytes(np.asarray(palette, np.uint8).reshape(-1)))
-    if trns is not None:
-        png += chunk(b"tRNS", bytes(trns))
-    png += chunk(b"IDAT", zlib.compress(bytes(raw), 6)) + chunk(b"IEND", b"")
-    with open(path, "wb") as fh:
-        fh.write(png)
+    return raw
 
 
 def write_rgbe(path, rgbe, rle):
@@ -238,6 +257,13 @@ def _png_cases(rng):
     rgb = rng.integers(0, 4, (H, W, 3)) * 60
     key = rgb[0, 0]
     yield "rgb8_trns", dict(arr=rgb, ctype=2, trns=b"".join(struct.pack(">H", int(k)) for k in key))
+    # Adam7: sizes where some passes are empty (1x1, 3x2) and ragged ones
+    for h, w in ((1, 1), (3, 2), (9, 13), (17, 10)):
+        yield f"rgb8_adam7_{h}x{w}", dict(arr=rng.integers(0, 256, (h, w, 3)), ctype=2, interlace=True)
+    yield "rgba16_adam7", dict(arr=rng.integers(0, 65536, (H, W, 4)), ctype=6, depth=16, interlace=True)
+    yield "pal2_adam7", dict(arr=rng.integers(0, 4, (H, W)), ctype=3, depth=2, palette=pal[:4], interlace=True,
+                             trns=[0, 128])
+    yield "pal1_adam7", dict(arr=rng.integers(0, 2, (10, 19)), ctype=3, depth=1, palette=pal[:2], interlace=True)
 
 
 @needs_ref
@@ -250,6 +276,35 @@ def test_png_matches_reference(tmp_path):
         got, c = pt.io.load_image_uint8(p)
         assert c == c_ref, name
         np.testing.assert_array_equal(got, ref, err_msg=name)
+
+
+def _bad_png_cases(rng):
+    """Files the PNG specification forbids.  stb_image rejects each of them except the
+    colour-type / bit-depth pairs below 8 bits for RGB(A), which it decodes anyway; the
+    product rejects those too (the specification's table of allowed pairs)."""
+    rgba = rng.integers(0, 256, (4, 5, 4))
+    yield "tRNS on RGBA", dict(arr=rgba, ctype=6, trns=bytes(8))
+    yield "tRNS on grey+alpha", dict(arr=rng.integers(0, 256, (4, 5, 2)), ctype=4, trns=bytes(2))
+    yield "tRNS of the wrong size", dict(arr=rng.integers(0, 256, (4, 5, 3)), ctype=2, trns=bytes(4))
+    pal = rng.integers(0, 256, (4, 3))
+    yield "palette at 16 bits", dict(arr=rng.integers(0, 4, (4, 5)), ctype=3, palette=pal, ihdr=(5, 4, 16, 3))
+    yield "RGB at 4 bits (stb decodes it)", dict(arr=rng.integers(0, 16, (4, 5, 3)), ctype=2, depth=4)
+    yield "RGBA at 2 bits (stb decodes it)", dict(arr=rng.integers(0, 4, (4, 5, 4)), ctype=6, depth=2)
+    yield "palette without PLTE", dict(arr=rng.integers(0, 4, (4, 5)), ctype=3)
+    yield "width beyond 2^24", dict(arr=rng.integers(0, 256, (1, 2, 3)), ctype=2, ihdr=(1 << 25, 1, 8, 2))
+    yield "height 2^31", dict(arr=rng.integers(0, 256, (1, 2, 3)), ctype=2, ihdr=(2, 1 << 31, 8, 2))
+
+
+def test_png_malformed_rejected(tmp_path):
+    rng = np.random.default_rng(9)
+    for name, kw in _bad_png_cases(rng):
+        p = str(tmp_path / "bad.png")
+        write_png(p, **kw)
+        with pytest.raises(RuntimeError):
+            pt.io.load_image_uint8(p)
+        if ioref.available() and "stb decodes it" not in name:
+            with pytest.raises(RuntimeError):
+                ioref.image_u8(p)
 
 
 def test_grey_png_replicated(tmp_path):
