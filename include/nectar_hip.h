@@ -289,6 +289,19 @@ int nr_trace_closest(nr_ctx* ctx, const float* rays, int32_t n, nr_hit* hits);
 /* Same with device-resident rays/hits, enqueued on the context stream. */
 int nr_trace_closest_device(nr_ctx* ctx, const float* d_rays, int32_t n, nr_hit* d_hits);
 
+/* nr_trace_closest_device without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The traversal-stack spill and
+ * the counters live in a file-scope device array of 16 MiB per GPU (held until the
+ * process ends; its address is looked up once per device by nr_create), and rays
+ * are traced in chunks of as many rays as it holds spill columns for, so any n is
+ * accepted; a scene whose stack would not fit one 128-ray workgroup is refused
+ * with NR_ERR_LIMIT.  The array keeps the last call's contents; the call clears
+ * what must start at zero with a kernel (no memset node).  Calls that overlap on
+ * one GPU (other streams, other contexts on the same device) race on the array.
+ * Same hits as nr_trace_closest_device; asynchronous on the context stream. */
+int nr_trace_closest_device_noalloc(nr_ctx* ctx, const float* d_rays, int32_t n, nr_hit* d_hits);
+
 /* White balance -> tonemap -> gamma -> uint8 CHW of a view's AOV layer
  * desc->layer (BEAUTY unless relinked) (process_stream_kernel,
  * csrc/data/src/stream.cu:146-159).  If out_f32 is

@@ -1212,6 +1212,37 @@ static std::vector<float> n_rooks(uint32_t n, uint32_t seed) {
 }  // namespace host
 
 // ---------------------------------------------------------------------------
+// scratch of nr_trace_closest_device_noalloc
+// ---------------------------------------------------------------------------
+// One file-scope array per GPU the library runs on, held until the process
+// ends: bytes [0, 256) hold the counters k_trace_closest writes (the overflow
+// flag at CNT_ERR), the rest traversal-stack spill columns for one chunk of rays.
+// Host code never uses the array's name as an address (on the host it is only a
+// stand-in for the symbol): nr_create looks up each device's copy with
+// hipGetSymbolAddress once, with that device current, and every launch takes the
+// pointer from that table.
+constexpr size_t NR_NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+constexpr size_t NR_NOALLOC_COUNTER_BYTES = 256;
+static __device__ __align__(256) int2 scratch[NR_NOALLOC_SCRATCH_BYTES / sizeof(int2)];
+constexpr int NR_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NR_MAX_DEVICES] = {};
+
+// The device copy of `scratch` on the current device `device` (looked up once).
+static int scratch_of_device(int device, void** out) {
+    if (device < 0 || device >= NR_MAX_DEVICES) return fail(NR_ERR_LIMIT, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return fail(NR_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return NR_OK;
+}
+
+// ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
 struct View {
@@ -1237,6 +1268,7 @@ struct View {
 struct nr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    void* scratch = nullptr;            // this device's copy of `scratch` (nr_trace_closest_device_noalloc)
     bool has_scene = false;
     GScene scene{};
     GScene* d_scene = nullptr;          // device copy of `scene`
@@ -1542,6 +1574,12 @@ nr_ctx* nr_create(int device) {
     }
     c->n_cu = prop.multiProcessorCount;
     c->shade_grid = c->n_cu * 8;
+    if (scratch_of_device(device, &c->scratch) != NR_OK) {
+        const std::string msg = g_err;
+        nr_destroy(c);
+        g_err = msg;
+        return nullptr;
+    }
     return c;
 }
 
@@ -2313,6 +2351,47 @@ int nr_trace_closest_device(nr_ctx* c, const float* d_rays, int32_t n, nr_hit* d
         (void)hipEventElapsedTime(&ms, e0, e1);
         c->last_ms[0] = ms; c->last_ms[1] = ms; c->last_ms[2] = 0.0f; c->last_ms[3] = 0.0f;
         c->n_last = 4;
+    }
+    return NR_OK;
+}
+
+int nr_trace_closest_device_noalloc(nr_ctx* c, const float* d_rays, int32_t n, nr_hit* d_hits) {
+    if (!c) return fail(NR_ERR_INVALID, "null context");
+    if (!c->has_scene) return fail(NR_ERR_STATE, "trace before scene upload");
+    if (n < 0) return fail(NR_ERR_INVALID, "negative ray count");
+    if (n == 0) return NR_OK;
+    if (!d_rays || !d_hits) return fail(NR_ERR_INVALID, "null ray or hit buffer");
+    if (!c->scratch) return fail(NR_ERR_STATE, "no scratch on this device");
+    HIPCK(hipSetDevice(c->device));
+    // rays per launch: as many spill columns as the scratch holds, whole workgroups
+    const size_t lv = c->scene.stack_depth > LDS_STACK_LEVELS ? (size_t)(c->scene.stack_depth - LDS_STACK_LEVELS) : 0;
+    const size_t col_bytes = lv * sizeof(int2);
+    const size_t room = NR_NOALLOC_SCRATCH_BYTES - NR_NOALLOC_COUNTER_BYTES;
+    size_t chunk = col_bytes ? room / col_bytes / 128 * 128 : (size_t)n;
+    if (chunk == 0)
+        return fail(NR_ERR_LIMIT, "the scene's traversal stack (" + std::to_string(c->scene.stack_depth) +
+                                      " levels) needs more than the " + std::to_string(NR_NOALLOC_SCRATCH_BYTES >> 20) +
+                                      " MiB scratch holds for one workgroup of 128 rays");
+    chunk = std::min(chunk, (size_t)n);
+    int32_t* counters = (int32_t*)c->scratch;
+    int2* spill = (int2*)((char*)c->scratch + NR_NOALLOC_COUNTER_BYTES);
+    hipLaunchKernelGGL(k_clear_err, dim3(1), dim3(64), 0, c->stream, counters + CNT_ERR);
+    constexpr int TYI = NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH;
+    const size_t lds = trav_lds_bytes();
+    for (size_t off = 0; off < (size_t)n; off += chunk) {
+        const int m = (int)std::min(chunk, (size_t)n - off);
+        const StackCfg SC{spill, (size_t)m};
+        const dim3 g((m + 127) / 128), b(128);
+        const float* r = d_rays + 6 * off;
+        nr_hit* h = d_hits + off;
+        if (c->obj_types & NR_TY_OTHER) {
+            if (c->trace_wide) hipLaunchKernelGGL((k_trace_closest<NR_TY_ALL, true>), g, b, lds, c->stream, c->d_scene, r, m, h, counters, SC);
+            else hipLaunchKernelGGL((k_trace_closest<NR_TY_ALL, false>), g, b, lds, c->stream, c->d_scene, r, m, h, counters, SC);
+        } else {
+            if (c->trace_wide) hipLaunchKernelGGL((k_trace_closest<TYI, true>), g, b, lds, c->stream, c->d_scene, r, m, h, counters, SC);
+            else hipLaunchKernelGGL((k_trace_closest<TYI, false>), g, b, lds, c->stream, c->d_scene, r, m, h, counters, SC);
+        }
+        HIPCK(hipGetLastError());
     }
     return NR_OK;
 }

@@ -50,6 +50,7 @@ def lib() -> C.CDLL:
         "nr_aov_device_ptr": (vp, [vp, C.c_int, C.c_int]),
         "nr_trace_closest": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int32, C.POINTER(abi.NrHit)]),
         "nr_trace_closest_device": (C.c_int, [vp, vp, C.c_int32, vp]),
+        "nr_trace_closest_device_noalloc": (C.c_int, [vp, vp, C.c_int32, vp]),
         "nr_post": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrPostDesc), C.POINTER(C.c_uint8), C.POINTER(C.c_float)]),
         "nr_sync": (C.c_int, [vp]),
         "nr_post_device": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.POINTER(abi.NrPostDesc), vp, vp]),
@@ -186,6 +187,10 @@ class Context:
 
     def trace_closest_device(self, d_rays: int, n: int, d_hits: int):
         _check(lib().nr_trace_closest_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits)))
+
+    def trace_closest_device_noalloc(self, d_rays: int, n: int, d_hits: int):
+        """nr_trace_closest_device with its scratch in the library's per-GPU static array."""
+        _check(lib().nr_trace_closest_device_noalloc(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits)))
 
     def post(self, view: int, desc: abi.NrPostDesc):
         H, W = self.views[view]

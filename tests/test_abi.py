@@ -25,7 +25,7 @@ def test_header_declares_entry_points():
               "nr_trace_closest",
               "nr_aov_read", "nr_post", "nr_last_error", "nr_view_size", "nr_unet_create", "nr_unet_forward",
               "nr_unet_train_step", "nr_unet_gradients", "nr_denoise", "nr_viewport", "nr_pick",
-              "nr_selection_mask", "nr_overlay_device", "nr_readback", "nr_set_build_mode", "nr_set_trace_layout", "nr_mesh_tree",
+              "nr_selection_mask", "nr_overlay_device", "nr_readback", "nr_set_build_mode", "nr_set_trace_layout", "nr_mesh_tree", "nr_trace_closest_device_noalloc",
               "nr_object_set_transform", "nr_view_mask_buffer"):
         assert n in names
 
@@ -59,6 +59,7 @@ def test_null_arguments_fail_cleanly_without_device():
     assert L.nr_autotune(None, 0, None, None) == 1
     assert L.nr_tile_error(None, None, None, None, 8, 8, 8, None) == 1
     assert L.nr_trace_closest(None, None, 0, None) == 1
+    assert L.nr_trace_closest_device_noalloc(None, None, 0, None) == 1
     assert L.nr_post(None, 0, None, None, None) == 1
     assert L.nr_set_timing(None, 1) == 1
     assert L.nr_sync(None) == 1
