@@ -14,7 +14,7 @@
 //   acos     atan2(sqrt(1 - x^2), x) (x^2 and 1 - x^2 are exact in double for float x)
 //   pow      exp(y log x) in double, with the IEEE special cases the path tracer meets
 //   log10    log / ln 10 in double; 10^k for whole k by repeated multiplication
-// Accuracy: < 1 ulp of float against float64 references (tests/test_mathfn.py).
+// Accuracy: < 1 ulp of float against float64 references (tests/test_oracle.py, test_mathfn_*).
 // Compile with -ffp-contract=off so no step is fused.
 #pragma once
 #include <stdint.h>

@@ -1137,6 +1137,20 @@ static Ray camera_ray(const Camera& c, uint32_t x, uint32_t y, uint32_t pixel, u
 // ---------------------------------------------------------------------------
 struct PathOut { V3 beauty{0, 0, 0}, diffuse{0, 0, 0}, normal{0, 0, 0}; int64_t segments = 0; };
 
+// sample_cosine_brdf (brdf.cu:4-35): with an ObjectLight in the scene, the 50/50
+// MixturePDF of the material's pdf and lights[0] (pdf.h:262-283), else the material's
+static void sample_brdf(const Scene& S, const Scatter& sr, V3 p, Gen& g, V3& dir, float& pdf_val) {
+    if (S.first_light >= 0) {
+        const Object& L = S.objects[S.first_light];
+        if (g.random_float() < 0.5f) dir = sr.pdf.generate(g);                          // pdf.h:274-277
+        else dir = light_random(L, p, g);
+        pdf_val = 0.5f * sr.pdf.value(dir) + 0.5f * light_pdf_value(L, p, dir);
+    } else {
+        dir = sr.pdf.generate(g);
+        pdf_val = sr.pdf.value(dir);
+    }
+}
+
 static void trace_path(const Scene& S, const Camera& c, const nr_render_desc& d, uint32_t s,
                        uint32_t x, uint32_t y, PathOut& out, Counters* cnt) {
     uint32_t pixel = y * (uint32_t)c.W + x;
@@ -1164,17 +1178,8 @@ static void trace_path(const Scene& S, const Camera& c, const nr_render_desc& d,
                 ray = sr.skip_ray;
                 alive = true;
             } else {
-                // sample_cosine_brdf (brdf.cu:4-35)
                 V3 dir; float pdf_val;
-                if (S.first_light >= 0) {
-                    const Object& L = S.objects[S.first_light];
-                    if (g.random_float() < 0.5f) dir = sr.pdf.generate(g);              // pdf.h:274-277
-                    else dir = light_random(L, rec.p, g);
-                    pdf_val = 0.5f * sr.pdf.value(dir) + 0.5f * light_pdf_value(L, rec.p, dir);
-                } else {
-                    dir = sr.pdf.generate(g);
-                    pdf_val = sr.pdf.value(dir);
-                }
+                sample_brdf(S, sr, rec.p, g, dir, pdf_val);
                 if (pdf_val <= 0.0f) {
                     alive = false;
                 } else {
@@ -1721,6 +1726,83 @@ int nro_post(const float* chw, int H, int W, const nr_post_desc* d, float* out_f
     }
     return 0;
 }
+
+// ---- single-vertex probes (closed-form tests of the pieces a path is made of) ----
+static HitRec rec_from(const nr_hit* h) {
+    HitRec r;
+    r.object_id = h->object_id; r.material_id = h->material_id; r.prim = h->prim;
+    r.p = {h->p[0], h->p[1], h->p[2]}; r.n = {h->n[0], h->n[1], h->n[2]};
+    r.tangent = {h->tangent[0], h->tangent[1], h->tangent[2]};
+    r.uv = {h->uv[0], h->uv[1]}; r.t = h->t; r.front_face = h->front_face != 0;
+    return r;
+}
+static const Object* object_by_id(const Scene& S, int object_id) {
+    for (const Object& o : S.objects)
+        if (o.object_id == object_id) return &o;
+    return nullptr;
+}
+
+int nro_material_evaluate(const nro_scene* s, int material, const nr_hit* rec, const float* view3, const float* light3,
+                          float* out3) {
+    const Scene& S = s->s;
+    if (material < 0 || material >= (int)S.materials.size()) return -1;
+    V3 f = mat_evaluate(S, S.materials[material], rec_from(rec), {view3[0], view3[1], view3[2]},
+                        {light3[0], light3[1], light3[2]});
+    out3[0] = f.x; out3[1] = f.y; out3[2] = f.z;
+    return 0;
+}
+
+int nro_material_sample(const nro_scene* s, int material, const nr_hit* rec, const float* ray6, uint32_t seed,
+                        uint32_t subseq, uint32_t offset, float* dir3, float* pdf, float* atten3, int* kind) {
+    const Scene& S = s->s;
+    if (material < 0 || material >= (int)S.materials.size()) return -1;
+    const HitRec r = rec_from(rec);
+    const Ray ray = make_ray({ray6[0], ray6[1], ray6[2]}, {ray6[3], ray6[4], ray6[5]});
+    Gen g{seed, subseq, offset};
+    Scatter sr;
+    V3 dir{0, 0, 0};
+    float pv = 0.0f;
+    if (!mat_scatter(S, S.materials[material], r, ray, sr, g)) {
+        *kind = 0;
+    } else if (sr.skip_pdf) {
+        *kind = 1;
+        dir = sr.skip_ray.d;
+        pv = -1.0f;
+    } else {
+        *kind = 2;
+        sample_brdf(S, sr, r.p, g, dir, pv);
+    }
+    dir3[0] = dir.x; dir3[1] = dir.y; dir3[2] = dir.z;
+    *pdf = pv;
+    atten3[0] = sr.atten.x; atten3[1] = sr.atten.y; atten3[2] = sr.atten.z;
+    return (int)g.draw;
+}
+
+float nro_light_pdf(const nro_scene* s, int object_id, const float* origin3, const float* dir3) {
+    const Object* o = object_by_id(s->s, object_id);
+    if (!o) return -1.0f;
+    return light_pdf_value(*o, {origin3[0], origin3[1], origin3[2]}, {dir3[0], dir3[1], dir3[2]});
+}
+
+int nro_light_random(const nro_scene* s, int object_id, const float* origin3, uint32_t seed, uint32_t subseq,
+                     uint32_t offset, float* out3) {
+    const Object* o = object_by_id(s->s, object_id);
+    if (!o) return -1;
+    Gen g{seed, subseq, offset};
+    V3 d = light_random(*o, {origin3[0], origin3[1], origin3[2]}, g);
+    out3[0] = d.x; out3[1] = d.y; out3[2] = d.z;
+    return 0;
+}
+
+int nro_texture_sample(const nro_scene* s, int material, int slot, float u, float v, float* out3) {
+    const Scene& S = s->s;
+    if (material < 0 || material >= (int)S.materials.size() || slot < 0 || slot >= 5) return -1;
+    V3 c = tex_sample(S, S.materials[material].tex[slot], {u, v});
+    out3[0] = c.x; out3[1] = c.y; out3[2] = c.z;
+    return 0;
+}
+
+void nro_hash_constants(uint32_t* out64) { std::memcpy(out64, HASH_CONSTANTS, sizeof(HASH_CONSTANTS)); }
 
 void nro_philox_block(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox10(ctr, key, out); }
 float nro_philox_uniform(uint32_t seed, uint32_t subseq, uint32_t offset) { return philox_uniform(seed, subseq, offset); }

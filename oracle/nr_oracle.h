@@ -71,6 +71,26 @@ int nro_pick(const nro_scene* s, const nro_camera* c, float su, float sv, nr_hit
 int nro_selection_mask(const nro_scene* s, const nro_camera* c, int32_t object_id, int radius, uint8_t* out);
 int nro_overlay(uint8_t* chw_u8, const uint8_t* mask, const float* color3, int H, int W);
 
+/* Single-vertex probes.  rec: a hit record as nr_trace_closest returns it.
+ * material_evaluate: Material::evaluate(rec, view, light) (material.h:59-67, 152-180, 351-357).
+ * material_sample: Material::scatter, then for non-specular materials
+ * sample_cosine_brdf (brdf.cu:4-35, the 50/50 mixture with lights[0] when the scene
+ * has an ObjectLight) with Philox stream (seed, subseq) from draw `offset`; kind =
+ * 0 absorbed, 1 specular (dir = the scattered ray, pdf = -1), 2 sampled (dir, pdf);
+ * returns the next draw index.  light_pdf / light_random: pdf_value / random of the
+ * object registered as object_id (primitives.h:34-71, shapes.h:60-108).
+ * texture_sample: TextureView::sample (texture.h:49-91) of a material's texture slot. */
+int nro_material_evaluate(const nro_scene* s, int material, const nr_hit* rec, const float* view3, const float* light3,
+                          float* out3);
+int nro_material_sample(const nro_scene* s, int material, const nr_hit* rec, const float* ray6, uint32_t seed,
+                        uint32_t subseq, uint32_t offset, float* dir3, float* pdf, float* atten3, int* kind);
+float nro_light_pdf(const nro_scene* s, int object_id, const float* origin3, const float* dir3);
+int nro_light_random(const nro_scene* s, int object_id, const float* origin3, uint32_t seed, uint32_t subseq,
+                     uint32_t offset, float* out3);
+int nro_texture_sample(const nro_scene* s, int material, int slot, float u, float v, float* out3);
+/* the 64 mixing constants of hash.h:17-34 as the oracle holds them */
+void nro_hash_constants(uint32_t* out64);
+
 /* RNG and hashing */
 void nro_philox_block(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float nro_philox_uniform(uint32_t seed, uint32_t subsequence, uint32_t offset);

@@ -62,6 +62,13 @@ def lib():
         "nro_pdf_cosine_value": (C.c_float, [fp, fp]),
         "nro_pdf_cosine_generate": (C.c_int, [fp, C.c_uint32, C.c_uint32, C.c_uint32, fp]),
         "nro_mathfn": (None, [C.c_int, fp, fp, fp, C.c_int]),
+        "nro_material_evaluate": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrHit), fp, fp, fp]),
+        "nro_material_sample": (C.c_int, [vp, C.c_int, C.POINTER(abi.NrHit), fp, C.c_uint32, C.c_uint32, C.c_uint32,
+                                          fp, fp, fp, C.POINTER(C.c_int)]),
+        "nro_light_pdf": (C.c_float, [vp, C.c_int, fp, fp]),
+        "nro_light_random": (C.c_int, [vp, C.c_int, fp, C.c_uint32, C.c_uint32, C.c_uint32, fp]),
+        "nro_texture_sample": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_float, fp]),
+        "nro_hash_constants": (None, [C.POINTER(C.c_uint32)]),
         "nro_viewport": (C.c_int, [vp, vp, C.c_int, fp, fp, fp, C.c_int]),
         "nro_triplanar_grid": (C.c_int, [fp, fp]),
         "nro_center_ray": (C.c_int, [vp, C.c_uint32, C.c_uint32, fp]),
@@ -123,6 +130,54 @@ class OracleScene:
         out = np.zeros(3, np.float32)
         lib().nro_sky_sample(self.h, _fp(f3(d)), _fp(out))
         return out
+
+    # single-vertex probes
+    def material_evaluate(self, material: int, rec, view, light):
+        out = np.zeros(3, np.float32)
+        if lib().nro_material_evaluate(self.h, material, C.byref(rec), _fp(f3(view)), _fp(f3(light)), _fp(out)):
+            raise RuntimeError("bad material")
+        return out
+
+    def material_sample(self, material: int, rec, ray6, seed: int, subseq: int, offset: int = 0):
+        """(kind, direction, pdf, attenuation, next draw): Material::scatter + sample_cosine_brdf."""
+        d, a = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        pdf, kind = C.c_float(), C.c_int()
+        nxt = lib().nro_material_sample(self.h, material, C.byref(rec), _fp(f3(ray6)), seed, subseq, offset, _fp(d),
+                                        C.byref(pdf), _fp(a), C.byref(kind))
+        if nxt < 0:
+            raise RuntimeError("bad material")
+        return kind.value, d, pdf.value, a, nxt
+
+    def light_pdf(self, object_id: int, origin, direction) -> float:
+        return lib().nro_light_pdf(self.h, object_id, _fp(f3(origin)), _fp(f3(direction)))
+
+    def light_random(self, object_id: int, origin, seed: int, subseq: int, offset: int = 0):
+        out = np.zeros(3, np.float32)
+        if lib().nro_light_random(self.h, object_id, _fp(f3(origin)), seed, subseq, offset, _fp(out)):
+            raise RuntimeError("no such object")
+        return out
+
+    def texture_sample(self, material: int, slot: int, u: float, v: float):
+        out = np.zeros(3, np.float32)
+        if lib().nro_texture_sample(self.h, material, slot, u, v, _fp(out)):
+            raise RuntimeError("bad material or slot")
+        return out
+
+
+def hit_record(p, n, tangent=(1, 0, 0), uv=(0.5, 0.5), front=True, material=0):
+    """An nr_hit for the single-vertex probes."""
+    h = abi.NrHit()
+    h.object_id, h.material_id, h.prim, h.front_face, h.t = 0, material, -1, 1 if front else 0, 1.0
+    for k in range(3):
+        h.p[k], h.n[k], h.tangent[k] = float(p[k]), float(n[k]), float(tangent[k])
+    h.uv[0], h.uv[1] = float(uv[0]), float(uv[1])
+    return h
+
+
+def hash_constants() -> np.ndarray:
+    out = (C.c_uint32 * 64)()
+    lib().nro_hash_constants(out)
+    return np.array(list(out), np.uint32)
 
 
 class OracleCamera:
@@ -236,6 +291,12 @@ def post(chw: np.ndarray, desc: abi.NrPostDesc):
     u8 = np.empty(img.shape, np.uint8)
     lib().nro_post(_fp(img), H, W, C.byref(desc), _fp(f), u8.ctypes.data_as(C.POINTER(C.c_uint8)))
     return u8, f
+
+
+def pcg_vector2(f: float, seed: int):
+    out = np.zeros(2, np.float32)
+    lib().nro_pcg_vector2(f, seed, _fp(out))
+    return out
 
 
 def philox_uniform(seed, subseq, offset):

@@ -49,6 +49,38 @@ def _same_tree(a, b):
     np.testing.assert_array_equal(oa, ob)
 
 
+def _oracle_tree(oracle, v, t):
+    import test_oracle_pins as pins
+    return pins.oracle_pairs(oracle, pins.triangle_boxes(v, t))
+
+
+@pytest.mark.parametrize("freq", [12, 80])
+def test_product_trees_equal_the_oracle_tree(ctx, oracle, freq):
+    """Both product builders (host and device) against the oracle's binned-SAH builder, node for node:
+    boxes, child references and the leaf order (slot -> original triangle)."""
+    v, t = scenes.displaced_icosphere(freq=freq)
+    want, order = _oracle_tree(oracle, v, t)
+    for nodes, orig, _ in _trees(ctx, _mesh_scene(v, t)):
+        assert nodes.shape == want.shape
+        for f in ("left", "right", "lmin", "lmax", "rmin", "rmax"):
+            np.testing.assert_array_equal(nodes[f], want[f], err_msg=f)
+        np.testing.assert_array_equal(orig, order)
+
+
+def test_product_trees_of_the_1m_triangle_mesh_match_the_committed_digest(ctx):
+    """The 1,003,520-triangle mesh: the product's host and device trees hash to the digest the
+    oracle's tree has (tests/golden/c2_mesh_tree.txt, checked against the oracle on the CPU)."""
+    import os
+
+    import test_oracle_pins as pins
+    want = open(os.path.join(os.path.dirname(__file__), "golden", "c2_mesh_tree.txt")).read().split()
+    v, t = scenes.displaced_icosphere()
+    for nodes, orig, _ in _trees(ctx, _mesh_scene(v, t)):
+        nodes = nodes.copy()
+        nodes["pad"] = 0
+        assert pins.tree_digest(nodes, orig) == want[0]
+
+
 @pytest.mark.parametrize("n", [2, 3, 17, 511, 512, 513, 4096, 60_000])
 def test_device_tree_equals_host_tree_random_soup(ctx, n):
     v, t = _soup(n, seed=n)

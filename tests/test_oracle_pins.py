@@ -1,0 +1,654 @@
+"""Independent pins of the oracle's pieces that no other test fixes.
+
+Each check states what the reference's code computes in a form of its own (a
+closed form, a textbook identity, a distribution test or a table extracted from
+the reference's source), chosen so that a plausible mistake in the oracle's
+restatement (a dropped term, a swapped axis, a wrong sign or constant) fails:
+
+  post          Reinhard-extended and the tm_alpha blend (stream.cu:13-20, 49-58);
+                white balance through the published Tanner Helland Kelvin fit,
+                evaluated in float64, with the tint (stream.cu:65-109)
+  PBR evaluate  the value at v = l = n; Cook-Torrance with GGX D and the
+                tan-form Smith G1 (Walter et al. 2007); reciprocity
+                (material.h:152-180)
+  light PDFs    quad: pdf x cos / r^2 x area = 1 at every point of the light;
+                sphere: 1 / (2 pi (1 - cos theta_max)) inside the cone, 0 outside,
+                integral 1; random(): uniform on the quad / uniform in the cone;
+                the 50/50 mixture with lights[0] (primitives.h:34-71,
+                shapes.h:60-108, pdf.h:262-283, brdf.cu:4-35)
+  media         P(scatter) = 1 - exp(-sigma chord), exponential free flight
+                (volumes.h:14-45); isotropic f / pdf = albedo, uniform
+                directions (material.h:325-368)
+  camera        pixel pitch and image centre of a 1080 x 1920 view
+                (camera.h:124-136, 261-278)
+  sky           sky(d; r, k) = k sky(R_y(r) d; 0, 1) (skylight.h:71-122)
+  cube          face hits along the axes (shapes.h:142-173)
+  texture       texel lookup with a uv scale (texture.h:49-91)
+  PCG           the RXS-M-XS output of PCG (O'Neill 2014; Jarzynski and Olano
+                2020) where the mixes are the identity; the mixing table
+                extracted from hash.h:17-34 (tests/golden/make_hash_constants.py)
+  BVH           the 1,003,520-triangle tree's hash (tests/golden/c2_mesh_tree.txt),
+                which the GPU test of the product's builders checks as well
+"""
+import hashlib
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from nectarrender_amd import abi, scenes
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FMIN = 1e-8
+EPS = 1e-4
+PI = math.pi
+
+
+# ---------------------------------------------------------------------------
+# post
+# ---------------------------------------------------------------------------
+def _post(oracle, img, **kw):
+    """(display float image, uint8 image) of nro_post."""
+    u8, f = oracle.post(img.astype(np.float32), abi.post_desc(**kw))
+    return f, u8
+
+
+def _ramp(n=4096, top=20.0):
+    x = np.linspace(0.0, top, 3 * n, dtype=np.float32).reshape(3, 1, n)
+    return x
+
+
+@pytest.mark.parametrize("white", [1.0, 4.0])
+@pytest.mark.parametrize("alpha", [0.0, 0.5, 1.0])
+def test_reinhard_extended_and_alpha_blend(oracle, white, alpha):
+    x = _ramp()
+    f, _ = _post(oracle, x, linear_to_gamma=False, apply_white_balance=False, apply_tonemapping=True, tm_method=1,
+                 tm_white_point=white, tm_alpha=alpha)
+    X = x.astype(np.float64)
+    tm = X * (1.0 + X / (white * white)) / (1.0 + X + FMIN)
+    want = np.clip((1.0 - alpha) * X + alpha * tm, 0.0, 1.0)
+    assert np.abs(f - want).max() < 4e-6
+
+
+@pytest.mark.parametrize("alpha", [0.0, 0.5, 1.0])
+def test_fixed_reinhard_and_aces_blend(oracle, alpha):
+    x = _ramp(top=8.0)
+    X = x.astype(np.float64)
+    f, _ = _post(oracle, x, linear_to_gamma=False, apply_white_balance=False, apply_tonemapping=True, tm_method=0,
+                 tm_alpha=alpha, fix_reinhard=True)
+    want = np.clip((1 - alpha) * X + alpha * X / (X + 1.0 + FMIN), 0, 1)
+    assert np.abs(f - want).max() < 4e-6
+    f, _ = _post(oracle, x, linear_to_gamma=False, apply_white_balance=False, apply_tonemapping=True, tm_method=2,
+                 tm_alpha=alpha)
+    aces = np.clip(X * (2.51 * X + 0.03) / (X * (2.43 * X + 0.59) + 0.14), 0, 1)   # Narkowicz 2015
+    want = np.clip((1 - alpha) * X + alpha * aces, 0, 1)
+    assert np.abs(f - want).max() < 4e-6
+
+
+def _kelvin_rgb(kelvin):
+    """Tanner Helland's published fit of blackbody colour (2012), in float64, clamped to [0, 1]."""
+    t = kelvin / 100.0
+    if t <= 66.0:
+        r = 255.0
+        g = 99.4708025861 * math.log(t) - 161.1195681661
+    else:
+        r = 329.698727446 * (t - 60.0) ** -0.1332047592
+        g = 288.1221695283 * (t - 60.0) ** -0.0755148492
+    if t >= 66.0:
+        b = 255.0
+    elif t <= 19.0:
+        b = 0.0
+    else:
+        b = 138.5177312231 * math.log(t - 10.0) - 305.0447927307
+    return np.clip(np.array([r, g, b]) / 255.0, 0.0, 1.0)
+
+
+@pytest.mark.parametrize("kelvin", [500.0, 1000.0, 3000.0, 5200.0, 6500.0, 6600.0, 12000.0, 40000.0, 90000.0])
+@pytest.mark.parametrize("tint", [0.0, 1.0, -40.0])
+def test_white_balance_kelvin_fit(oracle, kelvin, tint):
+    rng = np.random.default_rng(int(kelvin) + int(tint * 10))
+    x = rng.random((3, 8, 64), dtype=np.float32) * 0.9
+    f, _ = _post(oracle, x, linear_to_gamma=False, apply_white_balance=True, wb_temperature=kelvin, wb_tint=tint,
+                 apply_tonemapping=False)
+    ref = _kelvin_rgb(6500.0)
+    tgt = _kelvin_rgb(min(max(kelvin, 1000.0), 40000.0))
+    ratio = tgt / np.maximum(ref, 1e-4)
+    t = 0.0025 * tint
+    gain = ratio * np.array([1 + t / 2, 1 - t, 1 + t / 2])
+    want = np.clip(x.astype(np.float64) * gain[:, None, None], 0, 1)
+    assert np.abs(f - want).max() < 2e-5, (kelvin, tint)
+    if kelvin == 6500.0 and tint == 0.0:
+        assert np.array_equal(f, np.clip(x, 0, 1))
+
+
+# ---------------------------------------------------------------------------
+# PBR evaluate
+# ---------------------------------------------------------------------------
+def _pbr_scene(albedo, rough, metal):
+    s = abi.SceneDesc()
+    s.sphere((0, 0, 0), 1.0, abi.Material.pbr(albedo=albedo, roughness=rough, metallic=metal))
+    return s, len(s.materials) - 1
+
+
+def _cook_torrance(albedo, rough, metal, n, v, l):
+    """Textbook GGX microfacet BRDF x cos(theta_l): D (Trowbridge-Reitz), Smith G1 in its tan form,
+    Schlick Fresnel, Lambert diffuse weighted by (1 - metallic)."""
+    a = max(rough * rough, 1e-3)
+    albedo = np.asarray(albedo, np.float64)
+    nl, nv = float(n @ l), float(n @ v)
+    if nl <= 0 or nv <= 0:
+        return np.zeros(3)
+    h = (v + l) / np.linalg.norm(v + l)
+    nh, vh = max(float(n @ h), 0.0), max(float(v @ h), 0.0)
+    D = a * a / max(PI * (nh * nh * (a * a - 1) + 1) ** 2, 1e-8)      # pdf.h:100-104 floors the denominator
+
+    def g1(c):
+        tan2 = (1 - c * c) / (c * c)
+        return 2.0 / (1.0 + math.sqrt(1.0 + a * a * tan2))
+    F0 = 0.04 * (1 - metal) + albedo * metal
+    F = F0 + (1 - F0) * (1 - vh) ** 5
+    spec = F * D * g1(nl) * g1(nv) / max(4 * nv * nl, 1e-4)
+    return (albedo * (1 - metal) / PI + spec) * nl
+
+
+@pytest.mark.parametrize("albedo,rough,metal", [((0.8, 0.5, 0.2), 0.5, 0.0), ((0.9, 0.6, 0.3), 0.3, 1.0),
+                                                ((0.2, 0.7, 0.9), 0.8, 0.4), ((1.0, 1.0, 1.0), 0.02, 0.5)])
+def test_pbr_evaluate_closed_forms(oracle, albedo, rough, metal):
+    s, mid = _pbr_scene(albedo, rough, metal)
+    os_ = oracle.OracleScene(s)
+    n = np.array([0.0, 0.0, 1.0])
+    rec = oracle.hit_record((0, 0, 0), n, tangent=(1, 0, 0), material=mid)
+    # v = l = n: albedo (1 - m) / pi + F0 D / 4 with D = 1 / (pi alpha^2), i.e. alpha^2 / (pi alpha^4),
+    # whose denominator the reference floors at 1e-8 (pdf.h:100-104; reached for alpha < 0.0075)
+    a = max(rough * rough, 1e-3)
+    F0 = 0.04 * (1 - metal) + np.asarray(albedo) * metal
+    want = np.asarray(albedo) * (1 - metal) / PI + F0 * (a * a / max(PI * a ** 4, 1e-8)) / 4
+    got = os_.material_evaluate(mid, rec, n, n)
+    np.testing.assert_allclose(got, want, rtol=2e-5)
+    rng = np.random.default_rng(17)
+    for _ in range(200):
+        v = rng.normal(size=3)
+        l_ = rng.normal(size=3)
+        v[2], l_[2] = abs(v[2]) + 0.05, abs(l_[2]) + 0.05
+        v, l_ = v / np.linalg.norm(v), l_ / np.linalg.norm(l_)
+        v32, l32 = v.astype(np.float32), l_.astype(np.float32)
+        f_vl = os_.material_evaluate(mid, rec, v32, l32)
+        f_lv = os_.material_evaluate(mid, rec, l32, v32)
+        want = _cook_torrance(albedo, rough, metal, n, v32.astype(np.float64), l32.astype(np.float64))
+        np.testing.assert_allclose(f_vl, want, rtol=2e-4, atol=1e-6)
+        # reciprocity of the BRDF itself (the returned value carries cos(theta_l))
+        np.testing.assert_allclose(f_vl / float(l32[2]), f_lv / float(v32[2]), rtol=2e-4, atol=1e-6)
+    assert not os_.material_evaluate(mid, rec, n, np.array([0, 0.6, -0.8], np.float32)).any()
+
+
+# ---------------------------------------------------------------------------
+# light PDFs, light sampling, the mixture
+# ---------------------------------------------------------------------------
+def _euler(deg):
+    """R = Rz Ry Rx for Euler angles in degrees (right-handed)."""
+    x, y, z = np.radians(deg)
+    Rx = np.array([[1, 0, 0], [0, math.cos(x), -math.sin(x)], [0, math.sin(x), math.cos(x)]])
+    Ry = np.array([[math.cos(y), 0, math.sin(y)], [0, 1, 0], [-math.sin(y), 0, math.cos(y)]])
+    Rz = np.array([[math.cos(z), -math.sin(z), 0], [math.sin(z), math.cos(z), 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
+QUAD_POS, QUAD_ROT, QUAD_SCALE = np.array([0.2, 2.0, -0.1]), (15.0, 0.0, -10.0), (1.5, 1.0, 0.8)
+
+
+def _quad_light_scene():
+    s = abi.SceneDesc()
+    s.quad((0, 0, 0), (0, 0, 0), (10, 1, 10), abi.Material.lambertian((0.5, 0.5, 0.5)))
+    lid = s.object_light(abi.SceneDesc.bound(abi.HIT_QUAD, QUAD_POS, QUAD_ROT, QUAD_SCALE), 4.0)
+    return s, lid
+
+
+def _quad_point(a, b):
+    """World point of the light at quad coordinates (a, b) in (-0.5, 0.5)^2: the unit quad spans x and z
+    of its object space with normal +y (primitives.h:8-16)."""
+    R = _euler(QUAD_ROT)
+    return R @ np.array([a * QUAD_SCALE[0], 0.0, b * QUAD_SCALE[2]]) + QUAD_POS, R @ np.array([0.0, 1.0, 0.0])
+
+
+def test_quad_light_pdf_is_solid_angle_density(oracle):
+    s, lid = _quad_light_scene()
+    os_ = oracle.OracleScene(s)
+    rng = np.random.default_rng(5)
+    area = QUAD_SCALE[0] * QUAD_SCALE[2]
+    for _ in range(300):
+        o = rng.uniform(-1, 1, 3) * np.array([2.0, 0.5, 2.0]) + np.array([0, 0.6, 0])
+        a, b = rng.uniform(-0.45, 0.45, 2)
+        x, nw = _quad_point(a, b)
+        d = (x - o).astype(np.float32)
+        D = d.astype(np.float64)
+        cos = abs(D @ nw) / np.linalg.norm(D)
+        want = (D @ D) / (cos * area)                 # pdf x cos / r^2 x area = 1 at every point of the light
+        got = os_.light_pdf(lid, o.astype(np.float32), d)
+        assert abs(got - want) <= 5e-4 * want, (got, want)
+    # directions that pass beside the light
+    o = np.array([0, 0.5, 0], np.float32)
+    for a, b in ((0.9, 0.0), (0.0, -0.8), (1.5, 1.5)):
+        x, _ = _quad_point(a, b)
+        assert os_.light_pdf(lid, o, (x - o).astype(np.float32)) == 0.0
+
+
+def test_quad_light_random_is_uniform_on_the_quad(oracle):
+    s, lid = _quad_light_scene()
+    os_ = oracle.OracleScene(s)
+    o = np.array([0.3, 0.4, -0.2], np.float32)
+    R = _euler(QUAD_ROT)
+    n = 20000
+    ab = np.empty((n, 2))
+    for i in range(n):
+        d = os_.light_random(lid, o, 11, i, 0).astype(np.float64)
+        local = R.T @ (o + d - QUAD_POS)
+        assert abs(local[1]) < 1e-5
+        ab[i] = local[0] / QUAD_SCALE[0], local[2] / QUAD_SCALE[2]
+    assert (np.abs(ab) <= 0.5 + 1e-6).all()
+    h, _, _ = np.histogram2d(ab[:, 0], ab[:, 1], bins=8, range=[[-0.5, 0.5], [-0.5, 0.5]])
+    chi2 = ((h - n / 64) ** 2 / (n / 64)).sum()
+    assert stats.chi2.sf(chi2, 63) > 1e-4, chi2
+
+
+SPHERE_R = 0.6
+
+
+def _sphere_light_scene(centre):
+    s = abi.SceneDesc()
+    s.quad((0, -3, 0), (0, 0, 0), (10, 1, 10), abi.Material.lambertian((0.5, 0.5, 0.5)))
+    lid = s.object_light(abi.SceneDesc.bound(abi.HIT_SPHERE, centre, radius=SPHERE_R), 4.0)
+    return s, lid
+
+
+def _uniform_dirs(n, seed):
+    d = np.random.default_rng(seed).normal(size=(n, 3))
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def test_sphere_light_pdf_cone_and_integral(oracle):
+    """A sphere light seen from outside: 1 / (2 pi (1 - cos theta_max)) on the directions that reach it,
+    0 elsewhere, and the pdf integrates to 1 over the sphere of directions."""
+    s, lid = _sphere_light_scene((0.0, 0.0, 0.0))
+    os_ = oracle.OracleScene(s)
+    o = np.array([0.4, 1.9, -0.3])
+    axis = -o
+    dist = np.linalg.norm(axis)
+    cmax = math.sqrt(1 - SPHERE_R ** 2 / dist ** 2)
+    inside = 1.0 / (2 * PI * (1 - cmax))
+    d = _uniform_dirs(200_000, 8)
+    cosang = d @ axis / dist
+    pdf = np.array([os_.light_pdf(lid, o.astype(np.float32), v.astype(np.float32)) for v in d])
+    clear_in, clear_out = cosang > cmax + 2e-4, cosang < cmax - 2e-4
+    np.testing.assert_allclose(pdf[clear_in], inside, rtol=2e-5)
+    assert (pdf[clear_out] == 0).all()
+    integral = pdf.mean() * 4 * PI                 # E over uniform directions of pdf / (1 / 4 pi)
+    se = pdf.std() * 4 * PI / math.sqrt(len(pdf))
+    assert abs(integral - 1.0) < 4 * se + 1e-3, (integral, se)
+
+
+def test_sphere_light_pdf_support_is_tested_at_the_world_origin(oracle):
+    """Sphere::pdf_value (shapes.h:60-76) intersects the world-space ray with the sphere in its object
+    space, i.e. with a sphere of the light's radius around the world origin, while cos theta_max uses the
+    light's true centre.  For a light away from the origin the pdf is therefore non-zero exactly on the
+    directions toward the origin's sphere (kept as the reference computes it)."""
+    c = np.array([0.4, 2.2, -0.3])
+    s, lid = _sphere_light_scene(c)
+    os_ = oracle.OracleScene(s)
+    o = np.array([1.5, -1.0, 0.8])
+    dist_c = np.linalg.norm(c - o)
+    value = 1.0 / (2 * PI * (1 - math.sqrt(1 - SPHERE_R ** 2 / dist_c ** 2)))
+    to0 = -o
+    d0 = np.linalg.norm(to0)
+    c0 = math.sqrt(1 - SPHERE_R ** 2 / d0 ** 2)       # the cone toward the origin's sphere
+    d = _uniform_dirs(50_000, 9)
+    cosang = d @ to0 / d0
+    pdf = np.array([os_.light_pdf(lid, o.astype(np.float32), v.astype(np.float32)) for v in d])
+    np.testing.assert_allclose(pdf[cosang > c0 + 2e-4], value, rtol=2e-5)
+    assert (pdf[cosang < c0 - 2e-4] == 0).all()
+    assert (cosang > c0 + 2e-4).sum() > 500
+
+
+def test_sphere_light_random_fills_the_cone_uniformly(oracle):
+    c = np.array([0.4, 2.2, -0.3])
+    s, lid = _sphere_light_scene(c)
+    os_ = oracle.OracleScene(s)
+    o = np.array([0.0, 0.3, 0.1])
+    axis = c - o
+    dist = np.linalg.norm(axis)
+    cmax = math.sqrt(1 - SPHERE_R ** 2 / dist ** 2)
+    cs = []
+    for i in range(20000):
+        d = os_.light_random(lid, o.astype(np.float32), 3, i, 0).astype(np.float64)
+        cs.append(d @ axis / (np.linalg.norm(d) * dist))
+    cs = np.array(cs)
+    assert cs.min() >= cmax - 1e-5 and cs.max() <= 1 + 1e-6
+    # uniform over the cone's solid angle: cos(angle) uniform on [cos theta_max, 1]
+    assert stats.kstest((cs - cmax) / (1 - cmax), "uniform").pvalue > 1e-3
+
+
+def _cosine_hit_fraction(o, n, hit, m=400_000, seed=1):
+    """Share of cosine-distributed directions around n from o that hit the light (numpy, float64)."""
+    rng = np.random.default_rng(seed)
+    r1, r2 = rng.random(m), rng.random(m)
+    local = np.stack([np.cos(2 * PI * r1) * np.sqrt(r2), np.sin(2 * PI * r1) * np.sqrt(r2), np.sqrt(1 - r2)], 1)
+    t = np.array([1.0, 0, 0]) if abs(n[0]) < 0.9 else np.array([0, 1.0, 0])
+    u = np.cross(n, t)
+    u /= np.linalg.norm(u)
+    v = np.cross(n, u)
+    d = local[:, :1] * u + local[:, 1:2] * v + local[:, 2:] * n
+    return hit(o, d).mean()
+
+
+def _hits_quad(o, d):
+    R = _euler(QUAD_ROT)
+    nw = R @ np.array([0, 1.0, 0])
+    denom = d @ nw
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = ((QUAD_POS - o) @ nw) / denom
+    p = o + t[:, None] * d
+    local = (p - QUAD_POS) @ R
+    return (t > 0) & (np.abs(local[:, 0]) < 0.5 * QUAD_SCALE[0]) & (np.abs(local[:, 2]) < 0.5 * QUAD_SCALE[2])
+
+
+def test_mixture_with_first_light_is_half_and_half(oracle):
+    """A Lambertian vertex with an ObjectLight in the scene draws from the 50/50 mixture of the cosine
+    pdf and the light (brdf.cu:16-23): its pdf is 0.5 p_cos + 0.5 p_light, and the share of directions
+    that reach the light is 0.5 + 0.5 q, q the cosine lobe's own share."""
+    s, lid = _quad_light_scene()
+    os_ = oracle.OracleScene(s)
+    o = np.array([0.1, 0.0, 0.2])
+    n = np.array([0.0, 1.0, 0.0])
+    rec = oracle.hit_record(o, n, tangent=(1, 0, 0), material=1)
+    ray = np.array([0.5, 2.0, 0.3, -0.4, -1.0, -0.1], np.float32)
+    N = 20000
+    hits = 0
+    for i in range(N):
+        kind, d, pdf, atten, _ = os_.material_sample(1, rec, ray, 9, i)
+        assert kind == 2
+        lp = os_.light_pdf(lid, o.astype(np.float32), d)
+        cp = max(0.0, float(np.float32(d @ n / np.linalg.norm(d))) / PI)
+        assert abs(pdf - (0.5 * cp + 0.5 * lp)) <= 1e-5 * max(pdf, 1e-3), (pdf, cp, lp)
+        hits += lp > 0
+    q = _cosine_hit_fraction(o, n, _hits_quad)
+    p = 0.5 + 0.5 * q
+    sd = math.sqrt(p * (1 - p) / N)
+    assert abs(hits / N - p) < 4.5 * sd, (hits / N, p, sd)
+
+
+# ---------------------------------------------------------------------------
+# media and the isotropic phase function
+# ---------------------------------------------------------------------------
+def _tri_cdf(x):
+    """CDF of (U1 + U2) / 2 for independent uniforms U1, U2 (the triangular law on [0, 1])."""
+    x = np.clip(x, 0.0, 1.0)
+    return np.where(x <= 0.5, 2 * x * x, 1 - 2 * (1 - x) ** 2)
+
+
+def test_medium_free_flight(oracle):
+    """ConstantMedium::hit (volumes.h:14-45) scatters where -ln(X) / sigma falls inside the boundary,
+    X the hash of the ray: the mean of independent uniform hashes of its origin and of its direction
+    (hash.h:107-124), so X is triangular on [0, 1] and P(scatter | chord c) = 1 - F(exp(-sigma c)) with
+    F(x) = 2x^2 below 1/2 and 1 - 2(1 - x)^2 above.  Hashing only one of the two vectors (X uniform:
+    1 - exp(-sigma c)) fails this test."""
+    R, sigma = 1.0, 0.7
+    s = abi.SceneDesc()
+    s.constant_medium(abi.SceneDesc.bound(abi.HIT_SPHERE, (0, 0, 0), radius=R), sigma, (0.8, 0.8, 0.8))
+    os_ = oracle.OracleScene(s)
+    rng = np.random.default_rng(4)
+    n = 80000
+    b = np.sqrt(rng.random(n)) * 0.97 * R                       # offsets from the centre, uniform over the disc
+    phi = rng.random(n) * 2 * PI
+    o = np.stack([b * np.cos(phi), b * np.sin(phi), np.full(n, -4.0)], 1).astype(np.float32)
+    d = np.concatenate([rng.normal(size=(n, 2)) * 1e-4, np.ones((n, 1))], 1).astype(np.float32)
+    O, D = o.astype(np.float64), d.astype(np.float64)
+    hh = np.einsum("ij,ij->i", D, -O)                          # chord of each ray through the sphere
+    aa = np.einsum("ij,ij->i", D, D)
+    cc = np.einsum("ij,ij->i", O, O) - (R + FMIN) ** 2
+    disc = hh * hh - aa * cc
+    chord = 2 * np.sqrt(np.maximum(disc, 0)) / aa * np.sqrt(aa)
+    h = os_.trace_closest(np.concatenate([o, d], 1))
+    hit = h["object_id"] >= 0
+    p = 1 - _tri_cdf(np.exp(-sigma * chord))
+    for lo, hi in ((0, 0.4), (0.4, 0.7), (0.7, 0.97)):
+        m = (b >= lo * R) & (b < hi * R)
+        z = (hit[m].sum() - p[m].sum()) / math.sqrt((p[m] * (1 - p[m])).sum())
+        assert abs(z) < 4.5, (lo, hi, z)
+    # the free-flight distance past the entry point, given a scatter: -ln(X) / sigma with X >= exp(-sigma c)
+    entry = (hh - np.sqrt(np.maximum(disc, 0))) / aa * np.sqrt(aa)
+    dist = h["t"][hit].astype(np.float64) * np.sqrt(aa[hit]) - entry[hit]
+    tail = 1 - _tri_cdf(np.exp(-sigma * chord[hit]))
+    u = (1 - _tri_cdf(np.exp(-sigma * np.clip(dist, 0, None)))) / tail
+    assert stats.kstest(np.clip(u, 0, 1), "uniform").pvalue > 1e-3
+
+
+def test_isotropic_phase_function(oracle):
+    s = abi.SceneDesc()
+    s.constant_medium(abi.SceneDesc.bound(abi.HIT_SPHERE, (0, 0, 0), radius=1.0), 0.5, (0.8, 0.3, 0.6))
+    os_ = oracle.OracleScene(s)
+    mid = 1
+    rec = oracle.hit_record((0.1, 0.2, 0.0), (1, 0, 0), material=mid)
+    ray = np.array([0, 0, -3, 0, 0.1, 1], np.float32)
+    dirs = []
+    for i in range(20000):
+        kind, d, pdf, atten, _ = os_.material_sample(mid, rec, ray, 5, i)
+        assert kind == 2
+        f = os_.material_evaluate(mid, rec, -ray[3:], d)
+        np.testing.assert_allclose(f / pdf, [0.8, 0.3, 0.6], rtol=1e-6)
+        assert np.array_equal(atten, np.float32([0.8, 0.3, 0.6]))
+        dirs.append(d)
+    d = np.array(dirs, np.float64)
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    for k in range(3):                              # Archimedes: each coordinate is uniform on [-1, 1]
+        assert stats.kstest((d[:, k] + 1) / 2, "uniform").pvalue > 1e-3
+
+
+# ---------------------------------------------------------------------------
+# camera, sky, cube, textures
+# ---------------------------------------------------------------------------
+def test_non_square_camera_geometry(oracle):
+    """A 1080 x 1920 view (resolution (H, W) = (1080, 1920)): both axes step sensor / H per pixel, and the
+    ray at zero offset through pixel (x, y) = ((H - 1) / 2, (W - 1) / 2) is the optical axis (the
+    reference's layout puts res[0] = H under the x pixel index, camera.h:176-177)."""
+    Hh, Ww, sensor, focal = 1080, 1920, 2.0, 1.8
+    pos, rot = np.array([0.5, 1.0, 4.0]), (-10.0, 25.0, 5.0)
+    cam = abi.camera_desc((Hh, Ww), pos, rot, 4, focal, 3.0, 0.0, sensor, 1.0)
+    seed = 7
+    oc = oracle.OracleCamera(cam, seed)
+    pattern = oracle.nrooks(4, seed)
+    R = _euler(rot)
+    pitch = sensor / Hh
+    cx, cy = (Hh - 1) / 2.0, (Ww - 1) / 2.0
+    for x, y in ((0, 0), (Ww - 1, Hh - 1), (Ww - 1, 0), (0, Hh - 1), (960, 540), (539, 959), (1234, 77)):
+        for s in range(4):
+            o, d = oc.ray(x, y, s, seed)[:3], oc.ray(x, y, s, seed)[3:6]
+            pixel = y * Ww + x
+            shift = oracle.pcg_vector2(float(np.float32(pixel)), seed)
+            ox = np.float32(np.fmod(np.float32(pattern[s, 0] + shift[0]), np.float32(1))) - np.float32(0.5)
+            oy = np.float32(np.fmod(np.float32(pattern[s, 1] + shift[1]), np.float32(1))) - np.float32(0.5)
+            local = np.array([pitch * (x + ox - cx), -pitch * (y + oy - cy), -focal])
+            want = R @ local
+            want /= np.linalg.norm(want)
+            np.testing.assert_allclose(o, pos, atol=1e-6)
+            np.testing.assert_allclose(d, want, atol=2e-6)
+
+
+def _smooth_hdri(H=32, W=64):
+    v, u = np.meshgrid((np.arange(H) + 0.5) / H, (np.arange(W) + 0.5) / W, indexing="ij")
+    img = np.stack([1.5 + np.sin(2 * PI * u) * np.sin(PI * v), 1.2 + 0.8 * np.cos(2 * PI * u + 0.7) * np.sin(PI * v),
+                    1.0 + 0.5 * u * (1 - u) + 0.3 * v])
+    return img.astype(np.float32)
+
+
+@pytest.mark.parametrize("rot,k", [(90.0, 1.0), (-37.5, 0.25), (200.0, 3.0)])
+def test_hdri_rotation_and_intensity(oracle, rot, k):
+    img = _smooth_hdri()
+    a, b = abi.SceneDesc(), abi.SceneDesc()
+    a.sky_hdri(img, rotation=rot, intensity=k)
+    b.sky_hdri(img)
+    A, B = oracle.OracleScene(a), oracle.OracleScene(b)
+    r = math.radians(rot)
+    Ry = np.array([[math.cos(r), 0, math.sin(r)], [0, 1, 0], [-math.sin(r), 0, math.cos(r)]])
+    rng = np.random.default_rng(2)
+    for _ in range(300):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        if abs(d[1]) > 0.97:
+            continue
+        got = A.sky(d.astype(np.float32))
+        want = k * B.sky((Ry @ d).astype(np.float32))
+        np.testing.assert_allclose(got, want, rtol=3e-4)
+
+
+def test_cube_face_hits(oracle):
+    c, sc = np.array([0.3, -0.2, 0.1]), np.array([1.0, 2.0, 0.5])
+    s = abi.SceneDesc()
+    s.cube(c, (0, 0, 0), sc, abi.Material.lambertian())
+    os_ = oracle.OracleScene(s)
+    rays, want_t, want_n = [], [], []
+    for axis in range(3):
+        for sign in (1.0, -1.0):
+            d = np.zeros(3)
+            d[axis] = -sign
+            o = c + 0.01 * np.array([1.0, -0.7, 0.4])
+            o[axis] = c[axis] + sign * 5.0
+            rays.append(np.concatenate([o, d]))
+            want_t.append(5.0 - sc[axis] / 2 - EPS)              # the hit point is pushed EPS off the face
+            n = np.zeros(3)
+            n[axis] = sign
+            want_n.append(n)
+    h = os_.trace_closest(np.array(rays, np.float32))
+    assert (h["object_id"] == 0).all() and (h["front_face"] == 1).all()
+    np.testing.assert_allclose(h["t"], want_t, atol=2e-6)
+    np.testing.assert_allclose(h["n"], want_n, atol=5e-6)   # face rotations in float32 (cos 90 deg != 0)
+    # a cube turned 90 degrees about y swaps its x and z extents
+    s2 = abi.SceneDesc()
+    s2.cube(c, (0, 90, 0), sc, abi.Material.lambertian())
+    h2 = oracle.OracleScene(s2).trace_closest(np.array(rays, np.float32))
+    np.testing.assert_allclose(h2["t"][0], 5.0 - sc[2] / 2 - EPS, atol=2e-6)
+    np.testing.assert_allclose(h2["t"][4], 5.0 - sc[0] / 2 - EPS, atol=2e-6)
+
+
+@pytest.mark.parametrize("scale", [1.0, 0.5, 0.25, 0.3])
+def test_texture_lookup_with_scale(oracle, scale):
+    rng = np.random.default_rng(12)
+    Ht, Wt = 5, 7
+    img = rng.integers(0, 256, (3, Ht, Wt)).astype(np.uint8)
+    s = abi.SceneDesc()
+    i = s.add_image(img)
+    s.sphere((0, 0, 0), 1.0, abi.Material.lambertian(abi.Texture(image=i, scale=scale)))
+    os_ = oracle.OracleScene(s)
+    mid = len(s.materials) - 1
+    checked = 0
+    for u, v in rng.random((400, 2)):
+        su, sv = (u / scale) % 1.0, (v / scale) % 1.0
+        fx, fy = su * Wt, (1.0 - sv) * Ht
+        if min(abs(fx - round(fx)), abs(fy - round(fy)), su, 1 - su, sv, 1 - sv) < 1e-3:
+            continue                                               # too close to a texel edge for float32
+        col, row = min(int(fx), Wt - 1), min(int(fy), Ht - 1)
+        got = os_.texture_sample(mid, 0, float(u), float(v))
+        np.testing.assert_allclose(got, img[:, row, col] / 255.0, rtol=1e-6)
+        checked += 1
+    assert checked > 300
+
+
+# ---------------------------------------------------------------------------
+# PCG hash
+# ---------------------------------------------------------------------------
+def _pcg_output(state):
+    """PCG's RXS-M-XS output permutation of a 32-bit state (O'Neill 2014), as the PCG hash of
+    Jarzynski and Olano (JCGT 2020) applies it to input * 747796405 + 2891336453."""
+    state = state.astype(np.uint64) & 0xFFFFFFFF
+    word = (((state >> ((state >> 28) + 4)) ^ state) * 277803737) & 0xFFFFFFFF
+    return ((word >> 22) ^ word).astype(np.uint32)
+
+
+def _golden_table():
+    with open(os.path.join(GOLDEN, "hash_constants.txt")) as f:
+        return np.array([int(x, 16) for x in f.read().split()], np.uint32)
+
+
+def test_pcg_hash_where_the_mixes_are_the_identity(oracle):
+    """For i < 8192 with seed = stream = 0 the three mixes v ^= (v >> 16) * c; v ^ (v >> 13) leave i
+    unchanged, so the hash is the published PCG hash of i."""
+    i = np.arange(8192, dtype=np.uint64)
+    want = _pcg_output(i * 747796405 + 2891336453)
+    got = np.array([oracle.lib().nro_pcg_hash_int(int(k), 0, 0) for k in range(8192)], np.uint32)
+    assert np.array_equal(got, want)
+
+
+def test_hash_table_is_the_reference_table(oracle):
+    table = _golden_table()
+    assert table.shape == (64,)
+    assert np.array_equal(oracle.hash_constants(), table)
+    # the product's device copy of the table
+    src = open(os.path.join(ROOT, "nectarrender_amd", "csrc", "nr_device.h")).read()
+    body = re.search(r"NR_HASH_TABLE\[64\]\s*=\s*\{(.*?)\};", src, re.S).group(1)
+    prod = np.array([int(w, 16) for w in re.findall(r"0x([0-9A-Fa-f]+)u", body)], np.uint32)
+    assert np.array_equal(prod, table)
+
+
+def test_pcg_hash_with_mixing(oracle):
+    """Seeds, streams and inputs that make every mix act, against the hash built from the extracted table."""
+    T = _golden_table().astype(np.uint64)
+    M = 0xFFFFFFFF
+    rng = np.random.default_rng(6)
+    for _ in range(2000):
+        i, seed, stream = (int(x) for x in rng.integers(0, 2 ** 32, 3, dtype=np.uint64))
+        s = ((seed * 1759274928) ^ (stream * 2246822519)) & M
+        s ^= i
+        for k in range(3):
+            s ^= ((s >> 16) * int(T[(stream * 3 + k) & 63])) & M
+            s ^= s >> 13
+        want = int(_pcg_output(np.array([(s * 747796405 + 2891336453) & M]))[0])
+        assert oracle.lib().nro_pcg_hash_int(i, seed, stream) == want
+
+
+# ---------------------------------------------------------------------------
+# BVH of the 1,003,520-triangle mesh
+# ---------------------------------------------------------------------------
+def oracle_pairs(oracle, boxes):
+    """The oracle's tree of `boxes` in the child-pair form the product uploads (GNode, 64 bytes): interior
+    node k becomes pair node (number of interior nodes before k in preorder), a leaf reference is ~slot."""
+    nodes, nb, order = oracle.bvh_build(boxes)
+    inner = nodes[:, 2] == -1
+    pidx = np.cumsum(inner) - 1
+    ref = np.where(inner, pidx, ~nodes[:, 2])
+    out = np.zeros(int(inner.sum()), abi.NODE_DTYPE)
+    k = np.nonzero(inner)[0]
+    L, Rr = nodes[k, 0], nodes[k, 1]
+    out["lmin"], out["lmax"] = nb[L, :3], nb[L, 3:]
+    out["rmin"], out["rmax"] = nb[Rr, :3], nb[Rr, 3:]
+    out["left"], out["right"] = ref[L], ref[Rr]
+    return out, order
+
+
+def triangle_boxes(v, t):
+    p = v[:, :3][t]                                   # triangles x 3 corners x xyz
+    return np.concatenate([p.min(1), p.max(1)], 1).astype(np.float32)
+
+
+def tree_digest(pairs, order) -> str:
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(pairs).tobytes())
+    h.update(np.ascontiguousarray(order, np.int32).tobytes())
+    return h.hexdigest()
+
+
+def c2_tree_digest(oracle):
+    v, t = scenes.displaced_icosphere()
+    pairs, order = oracle_pairs(oracle, triangle_boxes(v, t))
+    return tree_digest(pairs, order), len(t)
+
+
+def test_c2_mesh_tree_digest(oracle):
+    """The oracle's binned-SAH tree of the 1,003,520-triangle mesh, as committed by
+    tests/golden/make_c2_tree_digest.py (the GPU test compares the product's host and device builds
+    with the same digest)."""
+    want = open(os.path.join(GOLDEN, "c2_mesh_tree.txt")).read().split()
+    got, n = c2_tree_digest(oracle)
+    assert n == 1_003_520
+    assert got == want[0]
