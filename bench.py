@@ -40,14 +40,18 @@ PATHS_PER_STEP = W * H
 # 23.873 interior nodes (64 B), 1.8217 triangle tests (48 B), 1.9960 top-level
 # objects (128 B), 40 B ray/hit per segment.
 TRAVERSAL_BYTES_PER_PATH = 1942.58
+# The same model per traced segment (ray): the path's 24 accumulate bytes aside,
+# (1942.58 - 24) / 1.7945 = 1069.14 B.  k_extend traces the segments of the
+# streaming iterations; once few paths are live, tail mode (k_tail) traces the
+# rest, so k_extend's algorithmic bytes are its traced rays (nr_last_timings[7])
+# times this.
+TRAVERSAL_BYTES_PER_SEGMENT = (TRAVERSAL_BYTES_PER_PATH - 24.0) / 1.7945
 # The scattered rays of the MIS estimator follow the same distribution (next-event
 # estimation adds shadow rays, traced by k_shadow, and changes no scattering
-# decision's probability), so the same per-path model applies to its k_extend.
-# HBM bytes k_extend moves per path on this workload, from rocprofv3 FETCH_SIZE (x 1.99992, the factor
-# a 1 GiB device-to-device copy gives) + WRITE_SIZE (x 1.0) over 18,662,400 paths per estimator,
-# default tuning (4 path slots per pixel):
-# profiles/k_extend_c2_1080p.md.
-TRAVERSAL_HBM_BYTES_PER_PATH = {0: 960.5, 1: 1041.3}    # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
+# decision's probability), so the same model applies to its k_extend.
+# HBM bytes k_extend moves per traced ray on this workload, from rocprofv3 FETCH_SIZE (scaled by the
+# factor a 1 GiB device-to-device copy gives) + WRITE_SIZE, default tuning: profiles/k_extend_c2_1080p.md.
+TRAVERSAL_HBM_BYTES_PER_SEGMENT = {0: 535.2, 1: 580.3}  # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -194,6 +198,7 @@ def main():
     ap.add_argument("--streams", type=int, default=0,
                     help="stream groups per render (1..4; default: the library's default)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default tuning")
+    ap.add_argument("--tuning", default="", help="slots,groups,shade,tail: set this tuning instead of autotuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (a kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -228,7 +233,9 @@ def main():
         ctx.sync()
         build[name] = round(time.perf_counter() - tb, 4)
     tuning = None
-    if args.streams:
+    if args.tuning:
+        ctx.set_tuning(*[int(v) for v in args.tuning.split(",")])
+    elif args.streams:
         ctx.set_render_streams(args.streams)
     elif not args.no_autotune:
         # the library's autotuner on 8 MIS passes of this workload: path slots per pixel, stream groups,
@@ -255,22 +262,28 @@ def main():
         ctx.set_timing(True)
         ctx.render(0, abi.render_desc(p0, p0 + K, rp["depth"], rp["seed"], mode=mode))
         ctx.sync()
-        tm = ctx.timings()   # [total, k_extend, shading, generate+accumulate, iterations, k_shadow] ms
+        # [total, k_extend, shading, generate+accumulate, iterations, k_shadow, k_tail, rays k_extend traced]
+        tm = ctx.timings()
         ctx.set_timing(False)
         if not np.array_equal(img, ctx.read(0)):
             raise RuntimeError("the timed and the event-timed renders differ")
         paths = K * PATHS_PER_STEP
         n_extend = int(tm[4])    # one k_extend launch per streaming iteration
-        achieved = TRAVERSAL_BYTES_PER_PATH * paths / (tm[1] / 1e3) / 1e9
+        traced = float(tm[7])
+        alg_bytes = TRAVERSAL_BYTES_PER_SEGMENT * traced
+        achieved = alg_bytes / (tm[1] / 1e3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": round(TRAVERSAL_HBM_BYTES_PER_PATH[mode] * paths / n_extend),
+                "traffic": round(TRAVERSAL_HBM_BYTES_PER_SEGMENT[mode] * traced / n_extend),
                 "kernel": "k_extend", "launches": n_extend, "avg_launch_ms": round(tm[1] / n_extend, 4),
-                "bytes_per_launch": round(TRAVERSAL_BYTES_PER_PATH * paths / n_extend)}
+                "bytes_per_launch": round(alg_bytes / n_extend), "rays_traced": int(traced),
+                "segments_per_path_in_k_extend": round(traced / paths, 4)}
         gpu = {"total": round(tm[0], 3), "traversal": round(tm[1], 3), "shading": round(tm[2], 3),
                "generate_accumulate": round(tm[3], 3)}
         if mode == abi.MODE_FAST:
             gpu["shadow_rays"] = round(tm[5], 3)
+        if len(tm) > 6:
+            gpu["tail"] = round(tm[6], 3)
         gpu["note"] = ("event-timed rerun of the same passes on one stream group; the Mpaths/s figure is the "
                        "default two stream groups")
         return paths / (t1 - t0) / 1e6, (t1 - t0), gpu, roof, img

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define NR_ABI_VERSION 4
+#define NR_ABI_VERSION 5
 
 enum nr_status {
     NR_OK = 0,
@@ -392,14 +392,18 @@ typedef struct nr_tuning {
                                  the path state and queues hold slots x pixels entries */
     int32_t stream_groups;    /* 1..4 (default 2), as nr_set_render_streams */
     int32_t shade_wg_per_cu;  /* k_shade grid: 256-thread workgroups per CU, 1..32 (default 8) */
+    int32_t tail_paths;       /* tail mode: once a stream group's live paths are at most this many
+                                 (default 262144; 0 = never), one kernel finishes each remaining
+                                 pixel with one thread, instead of wavefront iterations that each
+                                 last as long as their slowest ray */
 } nr_tuning;
 int nr_set_tuning(nr_ctx* ctx, const nr_tuning* tuning);
 int nr_get_tuning(nr_ctx* ctx, nr_tuning* tuning);
 
 /* The project's autotuner: renders `desc` on `view` under candidate tunings
  * (coordinate descent over slots {1,2,4,6,8}, stream groups {1..4}, shade
- * workgroups per CU {4,8,16}), keeps the fastest, writes it to *tuning (may be
- * NULL) and clears the view's AOVs.  A desc of a few passes of the real
+ * workgroups per CU {4,8,16}, tail paths {0, 64k, 256k, 1M}), keeps the fastest,
+ * writes it to *tuning (may be NULL) and clears the view's AOVs.  A desc of a few passes of the real
  * workload is enough. */
 int nr_autotune(nr_ctx* ctx, int view, const nr_render_desc* desc, nr_tuning* tuning);
 
@@ -427,18 +431,22 @@ int nr_object_set_transform(nr_ctx* ctx, int32_t index, const float position[3],
 int nr_sync(nr_ctx* ctx);
 
 /* Mesh-tree layout of later nr_trace_closest(_device) calls: 0 = the
- * reference's binary tree in its near-first order (default, what extension
- * rays traverse), 1 = the 4-wide layout NR_MODE_FAST's shadow rays traverse
- * (same closest hit; of two triangles hit at exactly the same t either may be
- * reported).  No counterpart in the reference: the test hook of
- * the wide traversal. */
-int nr_set_trace_layout(nr_ctx* ctx, int wide);
+ * reference's binary tree in its near-first order (default, what the
+ * reference's estimator traverses), 1 = the 4-wide layout, 2 = the 4-wide
+ * layout with child boxes quantised to 8 bits per plane (rounded outward),
+ * which NR_MODE_FAST's extension and shadow rays traverse.  All three find the
+ * same closest hit; of two triangles hit at exactly the same t either may be
+ * reported.  No counterpart in the reference: the test hook of the wide
+ * traversals. */
+int nr_set_trace_layout(nr_ctx* ctx, int layout);
 
 /* Per-kernel GPU time (ms) of the last nr_render / nr_trace_closest call,
  * measured with HIP events on the context stream: [0] whole call,
  * [1] traversal kernels (k_extend), [2] shading kernels, [3] generate+accumulate,
  * [4] streaming iterations (= k_extend launches) of nr_render, [5] shadow-ray
- * kernels (NR_MODE_FAST with an HDRI sky; 0 otherwise).
+ * kernels (NR_MODE_FAST with an HDRI sky; 0 otherwise), [6] tail mode (k_tail:
+ * traversal, shading and shadow rays of the last paths), [7] rays k_extend
+ * traced (path segments; the rest of the render's segments ran in k_tail).
  * Returns the number of values written. */
 int nr_last_timings(nr_ctx* ctx, float* ms, int n);
 

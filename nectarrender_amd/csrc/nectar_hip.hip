@@ -68,8 +68,10 @@ static int fail(int code, const std::string& msg) { g_err = msg; return code; }
     } while (0)
 
 // counters (one int each): live queues [0,2), material queues [2,8), error flag,
-// shadow-ray queue (fast mode), occupied path slots after k_regen
-enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_SHD = 9, CNT_OCC = 10, CNT_N = 16 };
+// shadow-ray queue (fast mode), occupied path slots and pixels with samples not
+// yet started after k_regen, rays k_extend traced during the render (64 bits
+// at [12, 14))
+enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_SHD = 9, CNT_OCC = 10, CNT_PEND = 11, CNT_TRACED = 12, CNT_N = 16 };
 // Path slots per pixel (a tuning of the context, 1..NR_SLOTS_MAX, default 4):
 // sample s of pixel i runs in slot s % slots (path id i + (s % slots) * cap),
 // so up to `slots` consecutive samples of a pixel are in flight and pixels with
@@ -78,6 +80,11 @@ enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_SHD = 9, CNT_OCC = 10, CNT_N =
 // estimators there).
 #define NR_SLOTS_MAX 8
 #define NR_SLOTS_DEFAULT 4
+// Live paths (upper bound) of a stream group below which the group switches
+// to tail mode (k_tail)
+#ifndef NR_TAIL_PATHS
+#define NR_TAIL_PATHS 262144
+#endif
 // rows of the HDRI marginal CDF that k_shade stages in LDS (larger maps read it from global memory)
 constexpr int SKY_MARG_LDS = 4097;
 #ifndef NR_LDS_LEVELS
@@ -95,10 +102,23 @@ constexpr int LDS_STACK_LEVELS = NR_LDS_LEVELS;   // traversal stack levels kept
 #ifndef NR_TRAV_OCC
 #define NR_TRAV_OCC 6
 #endif
+// waves per SIMD of k_extend for quad / sphere / mesh scenes
+#ifndef NR_EXT_WAVES
+#define NR_EXT_WAVES 7
+#endif
 #if NR_TRAV_OCC > 0
 #define NR_TRAV_ATTR __attribute__((amdgpu_waves_per_eu(NR_TRAV_OCC, 8)))
 #else
 #define NR_TRAV_ATTR
+#endif
+
+// Mesh-tree layouts of the fast-mode traversals (NR_LAY_*): the reference's
+// estimator always walks the binary tree in the reference's order.
+#ifndef NR_EXT_FAST_LAY
+#define NR_EXT_FAST_LAY NR_LAY_QUANT
+#endif
+#ifndef NR_SHADOW_LAY
+#define NR_SHADOW_LAY NR_LAY_WIDE
 #endif
 
 struct StackCfg {
@@ -302,8 +322,8 @@ __device__ __forceinline__ int block_append_keyed(int key, int32_t* counters) {
 __device__ __forceinline__ Stack make_stack(int2* smem, const GScene& S, const StackCfg& SC) {
     int tid = threadIdx.x;
     Stack st;
-    st.lds = smem + (tid >> 6) * LDS_STACK_LEVELS * 64;
-    st.spill = SC.spill;
+    st.lds = (NR_LDS nr_i2*)(smem + (tid >> 6) * LDS_STACK_LEVELS * 64);
+    st.spill = (NR_GLB nr_i2*)SC.spill;
     st.lane = tid & 63;
     st.lds_cap = LDS_STACK_LEVELS;
     st.cap = S.stack_depth;
@@ -347,7 +367,7 @@ __device__ __forceinline__ Ray camera_ray(const GCamera& c, uint32_t x, uint32_t
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-template <int TY, bool WIDE>
+template <int TY, int LAY>
 __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_trace_closest(const GScene* __restrict__ Sp,
                                                                     const float* __restrict__ rays, int n,
                                                                     nr_hit* __restrict__ hits,
@@ -362,7 +382,7 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_trace_closest(const GScene
     HRec rec;
     hrec_clear(rec);
     bool of = false;
-    int oi = scene_closest<TY, WIDE>(S, r, rec, st, of);
+    int oi = scene_closest<TY, LAY>(S, r, rec, st, of);
     if (of) atomicOr(&counts[CNT_ERR], 1);
     nr_hit h;
     if (oi < 0) {
@@ -685,6 +705,59 @@ __global__ void __launch_bounds__(256) k_wide_nodes(const GNode* __restrict__ no
     out[i] = w;
 }
 
+// Quantised 4-wide layout (fast mode) from the 4-wide one: per axis the grid
+// starts at the union box's lower bound and steps by the smallest power of two
+// 2^e (e >= -126) with origin + 255 * 2^e >= the union's upper bound; each child
+// plane is rounded outward to the grid and checked with the float additions
+// the traversal decodes with (origin + q * 2^e, q * 2^e exact), stepping q
+// outward until the decoded box contains the exact one.
+__device__ __forceinline__ float qdecode(float origin, uint32_t q, float s) { return origin + (float)q * s; }
+__global__ void __launch_bounds__(256) k_quant_nodes(const GWNode* __restrict__ wn, int32_t n, GQNode* __restrict__ out) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const GWNode w = wn[i];
+    GQNode q;
+    uint32_t exps = 0;
+    for (int a = 0; a < 3; a++) {
+        float lo = NR_FMAX, hi = -NR_FMAX;
+        for (int k = 0; k < 4; k++)
+            if (w.ref[k] != NR_WIDE_EMPTY) { lo = fminf(lo, w.lo[a][k]); hi = fmaxf(hi, w.hi[a][k]); }
+        if (!(lo <= hi)) { lo = 0.0f; hi = 0.0f; }
+        int e = -126;
+        const float ext = hi - lo;
+        if (ext > 0.0f) {
+            int k2 = 0;
+            (void)frexpf(ext / 255.0f, &k2);                 // ext / 255 < 2^k2
+            e = max(k2 - 1, -126);
+        }
+        auto scale = [](int ee) { return __uint_as_float((uint32_t)(ee + 127) << 23); };
+        while (e < 127 && qdecode(lo, 255u, scale(e)) < hi) e++;
+        const float sc = scale(e);
+        uint32_t ql = 0, qh = 0;
+        for (int k = 0; k < 4; k++) {
+            uint32_t a0 = 0, b0 = 0;
+            if (w.ref[k] != NR_WIDE_EMPTY) {
+                float fl = floorf((w.lo[a][k] - lo) / sc), fh = ceilf((w.hi[a][k] - lo) / sc);
+                int il = (int)fminf(fmaxf(fl, 0.0f), 255.0f), ih = (int)fminf(fmaxf(fh, 0.0f), 255.0f);
+                while (il > 0 && qdecode(lo, (uint32_t)il, sc) > w.lo[a][k]) il--;
+                while (ih < 255 && qdecode(lo, (uint32_t)ih, sc) < w.hi[a][k]) ih++;
+                a0 = (uint32_t)il;
+                b0 = (uint32_t)ih;
+            }
+            ql |= a0 << (8 * k);
+            qh |= b0 << (8 * k);
+        }
+        q.origin[a] = lo;
+        q.qlo[a] = ql;
+        q.qhi[a] = qh;
+        exps |= (uint32_t)(e + 127) << (8 * a);
+    }
+    q.exps = exps;
+    for (int k = 0; k < 4; k++) q.ref[k] = w.ref[k];
+    q.pad[0] = q.pad[1] = 0;
+    out[i] = q;
+}
+
 // The planar C x H x W layers stay the state between calls; a render folds into
 // pixel-interleaved copies (one 16-byte read-modify-write per layer and pixel)
 // and writes them back at the end.
@@ -751,13 +824,57 @@ __global__ void k_counts(int32_t* counts, int32_t in) {
     if (t == 0) counts[CNT_ACT + (1 - in)] = 0;
     if (t == 1) counts[CNT_SHD] = 0;
     if (t == 2) counts[CNT_OCC] = 0;
+    if (t == 3) counts[CNT_PEND] = 0;
 }
 
 // 8 waves per SIMD for the quad / sphere / mesh kernels (64 VGPRs, 20 bytes of
 // scratch per lane: 2% faster than 71 VGPRs at 7 waves on the 1080p workload);
 // the every-object-type kernels keep the 80-VGPR budget of NR_TRAV_OCC
+// One extension segment of a path (trace.cu:19-22 and the closest hit of
+// trace_ray): a miss adds the sky and ends the path, a hit stores its record.
+// Returns the material type of the hit, 0 on a miss.
 template <int TY, bool FAST>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(TY == NR_TY_ALL ? (NR_TRAV_OCC > 0 ? NR_TRAV_OCC : 1) : 8, 8)))
+__device__ __forceinline__ int extend_one(const GScene& S, const PathPtrs& P, int path, Stack& st) {
+    float4 o4 = P.o[path], d4 = P.d[path];
+    Ray r = mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
+    HRec rec;
+    hrec_clear(rec);
+    bool of = false;
+    int oi = scene_closest<TY, FAST ? NR_EXT_FAST_LAY : NR_LAY_BINARY>(S, r, rec, st, of);
+    if (of) atomicOr(P.err, 1);
+    if (oi < 0) {                                                               // trace.cu:19-22
+        const bool first = __float_as_int(o4.w) == 0;
+        float4 a4 = P.atten[path], l4 = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.L[path];
+        if (first) {                                                            // no hit: zero AOVs
+            P.diff[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            P.nrm[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        f3 Le = sky_sample(S.sky, r.d);
+        // fast mode: a scattered ray that finds the sky shares it with the
+        // shadow rays of next-event estimation (power heuristic)
+        if (FAST && S.sky.marg && a4.w > 0.0f) Le = Le * power_heuristic(a4.w, sky_cdf_pdf(S.sky, r.d));
+        f3 L = mk3(l4.x, l4.y, l4.z) + mk3(a4.x, a4.y, a4.z) * Le;
+        P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
+        P.finished[path] = 1;
+        return 0;
+    }
+    const GObject& ob = S.objects[oi];
+    GHit h;
+    h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
+    h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
+    h.tg[0] = rec.tg.x; h.tg[1] = rec.tg.y; h.tg[2] = rec.tg.z;
+    h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
+    h.material = ob.material;
+    h.front_face = rec.front ? 1 : 0;
+    h.object_id = ob.object_id;
+    h.prim = rec.prim;
+    h.t = rec.t;
+    P.hit[path] = h;
+    return ob.pad[0];
+}
+
+template <int TY, bool FAST>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(TY == NR_TY_ALL ? (NR_TRAV_OCC > 0 ? NR_TRAV_OCC : 1) : NR_EXT_WAVES, 8)))
 k_extend(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A, StackCfg SC) {
     extern __shared__ int2 smem[];
     const GScene& S = *Sp;
@@ -765,46 +882,12 @@ k_extend(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A, StackCfg SC) {
     const int n = P.counts[CNT_ACT + A.in];
     if ((int)(blockIdx.x * blockDim.x) >= n) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) atomicAdd(reinterpret_cast<unsigned long long*>(P.counts + CNT_TRACED), (unsigned long long)n);
     int path = -1;
     int mtype = 0;
     if (i < n) {
         path = P.qact[(size_t)A.in * P.qcap + i];
-        float4 o4 = P.o[path], d4 = P.d[path];
-        Ray r = mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
-        HRec rec;
-        hrec_clear(rec);
-        bool of = false;
-        int oi = scene_closest<TY>(S, r, rec, st, of);
-        if (of) atomicOr(P.err, 1);
-        if (oi < 0) {                                                           // trace.cu:19-22
-            const bool first = __float_as_int(o4.w) == 0;
-            float4 a4 = P.atten[path], l4 = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.L[path];
-            if (first) {                                                        // no hit: zero AOVs
-                P.diff[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                P.nrm[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            }
-            f3 Le = sky_sample(S.sky, r.d);
-            // fast mode: a scattered ray that finds the sky shares it with the
-            // shadow rays of next-event estimation (power heuristic)
-            if (FAST && S.sky.marg && a4.w > 0.0f) Le = Le * power_heuristic(a4.w, sky_cdf_pdf(S.sky, r.d));
-            f3 L = mk3(l4.x, l4.y, l4.z) + mk3(a4.x, a4.y, a4.z) * Le;
-            P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
-            P.finished[path] = 1;
-        } else {
-            const GObject& ob = S.objects[oi];
-            GHit h;
-            h.p[0] = rec.p.x; h.p[1] = rec.p.y; h.p[2] = rec.p.z;
-            h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
-            h.tg[0] = rec.tg.x; h.tg[1] = rec.tg.y; h.tg[2] = rec.tg.z;
-            h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
-            h.material = ob.material;
-            h.front_face = rec.front ? 1 : 0;
-            h.object_id = ob.object_id;
-            h.prim = rec.prim;
-            h.t = rec.t;
-            P.hit[path] = h;
-            mtype = ob.pad[0];
-        }
+        mtype = extend_one<TY, FAST>(S, P, path, st);
     }
     const int slot = block_append_keyed<128, NR_MAT_TYPES>(mtype, &P.counts[CNT_MAT]);
     if (mtype > 0) P.qmat[(size_t)mtype * P.qcap + slot] = path;
@@ -846,6 +929,148 @@ __device__ __forceinline__ f3 mat_eval(const GScene& S, const GMaterial& m, f3 n
 // tables), weighted against the
 // scattering technique with the power heuristic, queued as a shadow ray whose
 // contribution k_shadow adds if it reaches the sky.
+// One shading step of a path at its stored hit (trace.cu:23-40, 63-78):
+// emission, scatter, mixture PDF, fast mode's next-event sample, bounce-0
+// AOVs, Russian roulette.  cont: the path continues; shadow: a shadow ray is
+// pending (fast mode).  marg: the HDRI's marginal CDF (LDS or global).
+template <int M, bool FAST>
+__device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, const TraceArgs& A, int path,
+                                          const float* marg, bool& cont, bool& shadow) {
+    constexpr bool NEE_M = FAST && (M == NR_MAT_LAMBERTIAN || M == NR_MAT_PBR || M == NR_MAT_ISOTROPIC);
+    GHit h = P.hit[path];
+    float4 o4 = P.o[path], d4 = P.d[path], a4 = P.atten[path];
+    const uint32_t bounce = (uint32_t)__float_as_int(o4.w);
+    const float4 l4 = bounce == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.L[path];
+    const uint32_t sample = (uint32_t)__float_as_int(d4.w);
+    f3 din = mk3(d4.x, d4.y, d4.z);
+    f3 atten = mk3(a4.x, a4.y, a4.z);
+    f3 L = mk3(l4.x, l4.y, l4.z);
+    f3 p = ld3(h.p), nn = ld3(h.n), tg = ld3(h.tg);
+    f2 uv; uv.x = h.uv[0]; uv.y = h.uv[1];
+    bool front = h.front_face != 0;
+    const uint32_t gp = path_pixel(P, (uint32_t)path);
+    const GView& V = A.views[pixel_view(A, gp)];
+    Gen g = path_gen(A, gp - V.pix0, sample, P.draw[path], V.npix);
+    const GMaterial& m = S.materials[h.material];
+    // emission (material.h:182-188, 306-312)
+    f3 em = s3(0.0f);
+    if (M == NR_MAT_PBR && front) em = tex_sample(S, m.tex[3], uv);
+    if (M == NR_MAT_EMISSIVE && front) em = tex_sample(S, m.tex[0], uv) * m.brightness;
+    L = L + atten * em;
+    bool alive = false;
+    f3 nd = din;
+    float sc_pdf = -1.0f;                                                   // pdf of the scattered direction (-1: specular)
+    if (M == NR_MAT_DIELECTRIC) {                                           // material.h:220-244
+        float ri = front ? (1.0f / m.ior) : m.ior;
+        f3 unit = normalize(din);
+        float cos_theta = fminf(dot(-unit, nn), 1.0f);
+        float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+        bool refl = (double)(ri * sin_theta) > 1.0;
+        if (!refl) {
+            float r0 = (1.0f - ri) / (1.0f + ri);
+            r0 = r0 * r0;
+            float refl_p = r0 + (1.0f - r0) * nrm::powf_((1.0f - cos_theta), 5.0f);
+            refl = refl_p > g.uniform();
+        }
+        nd = refl ? reflect(unit, nn) : refract(unit, nn, ri);
+        atten = atten * ld3(m.tint);
+        alive = true;
+    } else if (M != NR_MAT_EMISSIVE) {
+        ONB uvw;
+        f3 view = s3(0.0f), sn = nn;
+        float alpha = 0.0f, sp = 0.0f;
+        if (M == NR_MAT_LAMBERTIAN) {
+            uvw.build(nn);
+        } else if (M == NR_MAT_PBR) {                                       // material.h:127-150
+            sn = pbr_shading_normal(S, m, nn, tg, uv);
+            f3 albedo = tex_sample(S, m.tex[0], uv);
+            float metallic = tex_sample(S, m.tex[2], uv).x;
+            f3 F0 = s3(0.04f) * (1.0f - metallic) + albedo * metallic;
+            sp = fminf(0.9f, fmaxf(0.1f, fmaxf(F0.x, fmaxf(F0.y, F0.z))));
+            view = normalize(-din);
+            float rough = tex_sample(S, m.tex[1], uv).x;
+            alpha = fmaxf(rough * rough, 1e-3f);
+            uvw.build(sn);
+        }
+        f3 dir;
+        float pdf;
+        if (S.first_light >= 0) {                                          // brdf.cu:16-23
+            const GObject& Lo = S.objects[S.first_light];
+            if (g.rf() < 0.5f) {
+                if (M == NR_MAT_LAMBERTIAN) dir = uvw.xf(cosine_dir(g));
+                else if (M == NR_MAT_PBR) dir = pbr_pdf_generate(uvw, view, alpha, sp, g);
+                else dir = unit_vector(g);
+            } else {
+                dir = light_random(Lo, p, g);
+            }
+            float mp;
+            if (M == NR_MAT_LAMBERTIAN) mp = cosine_pdf_value(uvw, dir);
+            else if (M == NR_MAT_PBR) mp = pbr_pdf_value(uvw, view, alpha, sp, dir);
+            else mp = 1.0f / NR_PI4;
+            pdf = 0.5f * mp + 0.5f * light_pdf(Lo, p, dir);
+        } else {
+            if (M == NR_MAT_LAMBERTIAN) { dir = uvw.xf(cosine_dir(g)); pdf = cosine_pdf_value(uvw, dir); }
+            else if (M == NR_MAT_PBR) { dir = pbr_pdf_generate(uvw, view, alpha, sp, g); pdf = pbr_pdf_value(uvw, view, alpha, sp, dir); }
+            else { dir = unit_vector(g); pdf = 1.0f / NR_PI4; }
+        }
+        if (NEE_M && marg && bounce + 1 < A.depth) {
+            // drawn after the scattering sample: the scattered direction does
+            // not depend on whether the sky has sampling tables
+            float u1 = g.uniform(), u2 = g.uniform(), ps;
+            f3 ld = sky_cdf_sample(S.sky, marg, u1, u2, ps);
+            if (ps > 0.0f) {
+                f3 f = mat_eval<M>(S, m, nn, sn, uv, normalize(-din), ld);
+                if (f.x > 0.0f || f.y > 0.0f || f.z > 0.0f) {
+                    // pdf of the scattering technique in direction ld
+                    float mp;
+                    if (M == NR_MAT_LAMBERTIAN) mp = cosine_pdf_value(uvw, ld);
+                    else if (M == NR_MAT_PBR) mp = pbr_pdf_value(uvw, view, alpha, sp, ld);
+                    else mp = 1.0f / NR_PI4;
+                    if (S.first_light >= 0) mp = 0.5f * mp + 0.5f * light_pdf(S.objects[S.first_light], p, ld);
+                    f3 c = atten * f * sky_sample(S.sky, ld) * (power_heuristic(ps, mp) / ps);
+                    if (c.x > 0.0f || c.y > 0.0f || c.z > 0.0f) {
+                        P.sho[path] = make_float4(p.x, p.y, p.z, 0.0f);
+                        P.shd[path] = make_float4(ld.x, ld.y, ld.z, 0.0f);
+                        P.shc[path] = make_float4(c.x, c.y, c.z, 0.0f);
+                        shadow = true;
+                    }
+                }
+            }
+        }
+        if (!(pdf <= 0.0f)) {
+            f3 brdf = mat_eval<M>(S, m, nn, sn, uv, normalize(-din), dir);
+            atten = atten * divs(brdf, pdf);
+            nd = dir;
+            alive = true;
+            sc_pdf = pdf;
+        }
+    }
+    if (bounce == 0) {                                                      // trace.cu:67-70 (zero when
+        f3 dv = s3(0.0f), nv = s3(0.0f);                                    // the path ends here)
+        if (alive) { dv = dv + atten; nv = nv + nn; }
+        P.diff[path] = make_float4(dv.x, dv.y, dv.z, 0.0f);
+        P.nrm[path] = make_float4(nv.x, nv.y, nv.z, 0.0f);
+    }
+    if (alive) {
+        if (A.do_rr && bounce >= A.rr_start && bounce + 1 < A.depth) {      // trace.cu:72-77
+            float survival = fmaxf(atten.x, fmaxf(atten.y, atten.z));
+            survival = fminf(survival, A.rr_survival);
+            if (g.uniform() > survival) alive = false;
+            else atten = atten * s3(1.0f / survival);
+        }
+    }
+    P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
+    if (alive && bounce + 1 < A.depth) {
+        P.atten[path] = make_float4(atten.x, atten.y, atten.z, FAST ? sc_pdf : 0.0f);
+        P.o[path] = make_float4(p.x, p.y, p.z, __int_as_float((int)(bounce + 1)));
+        P.d[path] = make_float4(nd.x, nd.y, nd.z, d4.w);
+        P.draw[path] = g.draw;
+        cont = true;
+    } else {
+        P.finished[path] = 1;
+    }
+}
+
 template <int M, bool FAST>
 __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A) {
     const GScene& S = *Sp;
@@ -866,138 +1091,7 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
     int path = -1;
     if (i < n) {
         path = P.qmat[(size_t)M * P.qcap + i];
-        GHit h = P.hit[path];
-        float4 o4 = P.o[path], d4 = P.d[path], a4 = P.atten[path];
-        const uint32_t bounce = (uint32_t)__float_as_int(o4.w);
-        const float4 l4 = bounce == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.L[path];
-        const uint32_t sample = (uint32_t)__float_as_int(d4.w);
-        f3 din = mk3(d4.x, d4.y, d4.z);
-        f3 atten = mk3(a4.x, a4.y, a4.z);
-        f3 L = mk3(l4.x, l4.y, l4.z);
-        f3 p = ld3(h.p), nn = ld3(h.n), tg = ld3(h.tg);
-        f2 uv; uv.x = h.uv[0]; uv.y = h.uv[1];
-        bool front = h.front_face != 0;
-        const uint32_t gp = path_pixel(P, (uint32_t)path);
-        const GView& V = A.views[pixel_view(A, gp)];
-        Gen g = path_gen(A, gp - V.pix0, sample, P.draw[path], V.npix);
-        const GMaterial& m = S.materials[h.material];
-        // emission (material.h:182-188, 306-312)
-        f3 em = s3(0.0f);
-        if (M == NR_MAT_PBR && front) em = tex_sample(S, m.tex[3], uv);
-        if (M == NR_MAT_EMISSIVE && front) em = tex_sample(S, m.tex[0], uv) * m.brightness;
-        L = L + atten * em;
-        bool alive = false;
-        f3 nd = din;
-        float sc_pdf = -1.0f;                                                   // pdf of the scattered direction (-1: specular)
-        if (M == NR_MAT_DIELECTRIC) {                                           // material.h:220-244
-            float ri = front ? (1.0f / m.ior) : m.ior;
-            f3 unit = normalize(din);
-            float cos_theta = fminf(dot(-unit, nn), 1.0f);
-            float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
-            bool refl = (double)(ri * sin_theta) > 1.0;
-            if (!refl) {
-                float r0 = (1.0f - ri) / (1.0f + ri);
-                r0 = r0 * r0;
-                float refl_p = r0 + (1.0f - r0) * nrm::powf_((1.0f - cos_theta), 5.0f);
-                refl = refl_p > g.uniform();
-            }
-            nd = refl ? reflect(unit, nn) : refract(unit, nn, ri);
-            atten = atten * ld3(m.tint);
-            alive = true;
-        } else if (M != NR_MAT_EMISSIVE) {
-            ONB uvw;
-            f3 view = s3(0.0f), sn = nn;
-            float alpha = 0.0f, sp = 0.0f;
-            if (M == NR_MAT_LAMBERTIAN) {
-                uvw.build(nn);
-            } else if (M == NR_MAT_PBR) {                                       // material.h:127-150
-                sn = pbr_shading_normal(S, m, nn, tg, uv);
-                f3 albedo = tex_sample(S, m.tex[0], uv);
-                float metallic = tex_sample(S, m.tex[2], uv).x;
-                f3 F0 = s3(0.04f) * (1.0f - metallic) + albedo * metallic;
-                sp = fminf(0.9f, fmaxf(0.1f, fmaxf(F0.x, fmaxf(F0.y, F0.z))));
-                view = normalize(-din);
-                float rough = tex_sample(S, m.tex[1], uv).x;
-                alpha = fmaxf(rough * rough, 1e-3f);
-                uvw.build(sn);
-            }
-            f3 dir;
-            float pdf;
-            if (S.first_light >= 0) {                                          // brdf.cu:16-23
-                const GObject& Lo = S.objects[S.first_light];
-                if (g.rf() < 0.5f) {
-                    if (M == NR_MAT_LAMBERTIAN) dir = uvw.xf(cosine_dir(g));
-                    else if (M == NR_MAT_PBR) dir = pbr_pdf_generate(uvw, view, alpha, sp, g);
-                    else dir = unit_vector(g);
-                } else {
-                    dir = light_random(Lo, p, g);
-                }
-                float mp;
-                if (M == NR_MAT_LAMBERTIAN) mp = cosine_pdf_value(uvw, dir);
-                else if (M == NR_MAT_PBR) mp = pbr_pdf_value(uvw, view, alpha, sp, dir);
-                else mp = 1.0f / NR_PI4;
-                pdf = 0.5f * mp + 0.5f * light_pdf(Lo, p, dir);
-            } else {
-                if (M == NR_MAT_LAMBERTIAN) { dir = uvw.xf(cosine_dir(g)); pdf = cosine_pdf_value(uvw, dir); }
-                else if (M == NR_MAT_PBR) { dir = pbr_pdf_generate(uvw, view, alpha, sp, g); pdf = pbr_pdf_value(uvw, view, alpha, sp, dir); }
-                else { dir = unit_vector(g); pdf = 1.0f / NR_PI4; }
-            }
-            if (NEE_M && marg && bounce + 1 < A.depth) {
-                // drawn after the scattering sample: the scattered direction does
-                // not depend on whether the sky has sampling tables
-                float u1 = g.uniform(), u2 = g.uniform(), ps;
-                f3 ld = sky_cdf_sample(S.sky, marg, u1, u2, ps);
-                if (ps > 0.0f) {
-                    f3 f = mat_eval<M>(S, m, nn, sn, uv, normalize(-din), ld);
-                    if (f.x > 0.0f || f.y > 0.0f || f.z > 0.0f) {
-                        // pdf of the scattering technique in direction ld
-                        float mp;
-                        if (M == NR_MAT_LAMBERTIAN) mp = cosine_pdf_value(uvw, ld);
-                        else if (M == NR_MAT_PBR) mp = pbr_pdf_value(uvw, view, alpha, sp, ld);
-                        else mp = 1.0f / NR_PI4;
-                        if (S.first_light >= 0) mp = 0.5f * mp + 0.5f * light_pdf(S.objects[S.first_light], p, ld);
-                        f3 c = atten * f * sky_sample(S.sky, ld) * (power_heuristic(ps, mp) / ps);
-                        if (c.x > 0.0f || c.y > 0.0f || c.z > 0.0f) {
-                            P.sho[path] = make_float4(p.x, p.y, p.z, 0.0f);
-                            P.shd[path] = make_float4(ld.x, ld.y, ld.z, 0.0f);
-                            P.shc[path] = make_float4(c.x, c.y, c.z, 0.0f);
-                            shadow = true;
-                        }
-                    }
-                }
-            }
-            if (!(pdf <= 0.0f)) {
-                f3 brdf = mat_eval<M>(S, m, nn, sn, uv, normalize(-din), dir);
-                atten = atten * divs(brdf, pdf);
-                nd = dir;
-                alive = true;
-                sc_pdf = pdf;
-            }
-        }
-        if (bounce == 0) {                                                      // trace.cu:67-70 (zero when
-            f3 dv = s3(0.0f), nv = s3(0.0f);                                    // the path ends here)
-            if (alive) { dv = dv + atten; nv = nv + nn; }
-            P.diff[path] = make_float4(dv.x, dv.y, dv.z, 0.0f);
-            P.nrm[path] = make_float4(nv.x, nv.y, nv.z, 0.0f);
-        }
-        if (alive) {
-            if (A.do_rr && bounce >= A.rr_start && bounce + 1 < A.depth) {      // trace.cu:72-77
-                float survival = fmaxf(atten.x, fmaxf(atten.y, atten.z));
-                survival = fminf(survival, A.rr_survival);
-                if (g.uniform() > survival) alive = false;
-                else atten = atten * s3(1.0f / survival);
-            }
-        }
-        P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
-        if (alive && bounce + 1 < A.depth) {
-            P.atten[path] = make_float4(atten.x, atten.y, atten.z, FAST ? sc_pdf : 0.0f);
-            P.o[path] = make_float4(p.x, p.y, p.z, __int_as_float((int)(bounce + 1)));
-            P.d[path] = make_float4(nd.x, nd.y, nd.z, d4.w);
-            P.draw[path] = g.draw;
-            cont = true;
-        } else {
-            P.finished[path] = 1;
-        }
+        shade_one<M, FAST>(S, P, A, path, marg, cont, shadow);
     }
     int slot = block_append<256>(cont, &P.counts[CNT_ACT + out]);
     if (cont) P.qact[(size_t)out * P.qcap + slot] = path;
@@ -1012,6 +1106,18 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
 // (the visibility of scene_closest) adds its contribution to the path's
 // radiance.  Runs after every k_shade of the iteration and before k_regen.
 template <int TY>
+__device__ __forceinline__ void shadow_one(const GScene& S, const PathPtrs& P, int path, Stack& st) {
+    float4 o4 = P.sho[path], d4 = P.shd[path];
+    bool of = false;
+    bool blocked = scene_occluded<TY, NR_SHADOW_LAY>(S, mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), st, of);
+    if (of) atomicOr(P.err, 1);
+    if (!blocked && !of) {
+        float4 c = P.shc[path], l = P.L[path];
+        P.L[path] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+    }
+}
+
+template <int TY>
 __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_shadow(const GScene* __restrict__ Sp, PathPtrs P, StackCfg SC) {
     extern __shared__ int2 smem[];
     const GScene& S = *Sp;
@@ -1019,17 +1125,7 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_shadow(const GScene* __res
     const int n = P.counts[CNT_SHD];
     if ((int)(blockIdx.x * blockDim.x) >= n) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        int path = P.qshd[i];
-        float4 o4 = P.sho[path], d4 = P.shd[path];
-        bool of = false;
-        bool blocked = scene_occluded<TY, true>(S, mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), st, of);
-        if (of) atomicOr(P.err, 1);
-        if (!blocked && !of) {
-            float4 c = P.shc[path], l = P.L[path];
-            P.L[path] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
-        }
-    }
+    if (i < n) shadow_one<TY>(S, P, P.qshd[i], st);
 }
 
 // Per pixel, scanning all pixels in tile order so the new primary rays of a
@@ -1038,11 +1134,23 @@ __global__ void __launch_bounds__(128) NR_TRAV_ATTR k_shadow(const GScene* __res
 // waits), then start the next samples in the slots that became idle.  The
 // occupied slots (samples started and not yet folded, ns - nf) are counted:
 // that number never grows, so the host sizes later traversal grids from it.
+// Folds the ended sample nf of path-array pixel i (in path slot p) into the
+// running means (accumulate_samples_kernel, data.cu:28-40) and frees the slot.
+__device__ __forceinline__ void fold_sample(const PathPtrs& P, const TraceArgs& A, uint32_t i, uint32_t p, uint32_t nf) {
+    float fn = (float)(A.fast ? nf + 1 : nf);                   // fast mode counts pass 0 as the first
+    float a = (fn - 1.0f) / fn, b = 1.0f / fn;
+    fold(P.acc + i, P.L[p], a, b);
+    fold(P.acc + P.cap + i, P.diff[p], a, b);
+    fold(P.acc + 2 * (size_t)P.cap + i, P.nrm[p], a, b);
+    P.finished[p] = 2;
+}
+
 constexpr int REGEN_BLOCK = 512;     // 1.2% faster renders than 1024 or 256 on the 1080p workload
 __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) {
     uint32_t t = A.t0 + blockIdx.x * blockDim.x + threadIdx.x, v = 0, x = 0, y = 0;
     uint32_t started[NR_SLOTS_MAX];
     uint32_t n_start = 0, occ = 0;
+    bool pending = false;
     if (t < A.t1 && tile_pixel(A, t, v, x, y)) {
         const GView& V = A.views[v];
         const uint32_t i = V.pix0 + y * V.cam.W + x;
@@ -1052,12 +1160,7 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) 
         for (int k = 0; k < NR_SLOTS_MAX; k++) {
             const uint32_t p = i + (nf % P.slots) * P.cap;
             if (nf >= ns || P.finished[p] != 1) break;
-            float fn = (float)(A.fast ? nf + 1 : nf);           // fast mode counts pass 0 as the first
-            float a = (fn - 1.0f) / fn, b = 1.0f / fn;
-            fold(P.acc + i, P.L[p], a, b);
-            fold(P.acc + P.cap + i, P.diff[p], a, b);
-            fold(P.acc + 2 * (size_t)P.cap + i, P.nrm[p], a, b);
-            P.finished[p] = 2;
+            fold_sample(P, A, i, p, nf);
             nf++;
         }
 #pragma unroll
@@ -1070,12 +1173,55 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) 
         P.nf[i] = nf;
         P.ns[i] = ns;
         occ = ns - nf;
+        pending = ns < end;
     }
     int pos[NR_SLOTS_MAX];
     block_append_ranks<REGEN_BLOCK>(n_start, &P.counts[CNT_ACT + (1 - A.in)], pos, occ, &P.counts[CNT_OCC]);
+    // pixels that still have samples to start (tail mode waits until there are none)
+    const int pend = __syncthreads_count(pending);
+    if (threadIdx.x == 0 && pend) atomicAdd(&P.counts[CNT_PEND], pend);
 #pragma unroll
     for (int k = 0; k < NR_SLOTS_MAX; k++)
         if ((uint32_t)k < n_start) P.qact[(size_t)(1 - A.in) * P.qcap + pos[k]] = (int32_t)started[k];
+}
+
+// Tail mode.  Once few paths are live, a wavefront iteration costs about the
+// traversal time of its slowest ray plus four launches, however few rays it
+// holds, and the last paths drain over many such iterations.  When every
+// pixel's samples have all been started and few paths are live, k_tail gives
+// each live path one thread that runs it to its end, segment by segment, with
+// the per-path steps the wavefront kernels run (extend_one, shade_one,
+// shadow_one) in their order; the k_regen that follows folds every pixel's
+// samples in sample order, so the AOVs are the same bit for bit.
+template <bool FAST>
+__device__ __forceinline__ void shade_dispatch(const GScene& S, const PathPtrs& P, const TraceArgs& A, int m, int path,
+                                               bool& cont, bool& shadow) {
+    const float* marg = S.sky.marg;
+    switch (m) {
+        case NR_MAT_LAMBERTIAN: shade_one<NR_MAT_LAMBERTIAN, FAST>(S, P, A, path, marg, cont, shadow); break;
+        case NR_MAT_PBR: shade_one<NR_MAT_PBR, FAST>(S, P, A, path, marg, cont, shadow); break;
+        case NR_MAT_DIELECTRIC: shade_one<NR_MAT_DIELECTRIC, FAST>(S, P, A, path, marg, cont, shadow); break;
+        case NR_MAT_EMISSIVE: shade_one<NR_MAT_EMISSIVE, FAST>(S, P, A, path, marg, cont, shadow); break;
+        default: shade_one<NR_MAT_ISOTROPIC, FAST>(S, P, A, path, marg, cont, shadow); break;
+    }
+}
+
+template <int TY, bool FAST>
+__global__ void __launch_bounds__(128) k_tail(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A, StackCfg SC) {
+    extern __shared__ int2 smem[];
+    const GScene& S = *Sp;
+    Stack st = make_stack(smem, S, SC);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.counts[CNT_ACT + A.in]) return;
+    const int p = P.qact[(size_t)A.in * P.qcap + i];
+    while (true) {
+        const int m = extend_one<TY, FAST>(S, P, p, st);
+        if (m == 0) break;                                  // the sky ended the path
+        bool cont = false, shadow = false;
+        shade_dispatch<FAST>(S, P, A, m, p, cont, shadow);
+        if (shadow) shadow_one<TY>(S, P, p, st);
+        if (!cont) break;                                   // shade_one marked it ended
+    }
 }
 
 // ray_depth 0: every pass adds a zero sample (trace.cu:63 runs no bounce).  Pass s
@@ -1297,12 +1443,13 @@ struct nr_ctx {
     int shade_grid = 0;                 // workgroups of 256 for k_shade (grid-stride)
     int2* spill = nullptr;              // traversal stack levels beyond LDS
     size_t spill_elems = 0;
-    int32_t* h_live = nullptr;          // pinned: live-path counts of the last two iterations, per stream group
+    int32_t* h_live = nullptr;          // pinned: occupied slots and pending pixels of the last two iterations, per group
     hipEvent_t ev_count[2] = {nullptr, nullptr};
     // stream groups: disjoint bands of pixels, each a complete wavefront pipeline on
     // its own stream, so one group's traversal tail overlaps another's work
     int render_streams = NR_RENDER_STREAMS;
     int slots = NR_SLOTS_DEFAULT;       // path slots per pixel (nr_set_tuning)
+    int tail_paths = NR_TAIL_PATHS;     // tail mode once a group's live-path bound is at most this (nr_set_tuning)
     hipStream_t gstream[NR_MAX_GROUPS] = {};
     hipEvent_t gev[NR_MAX_GROUPS][2] = {};
     hipEvent_t gev_done[NR_MAX_GROUPS] = {};
@@ -1310,7 +1457,7 @@ struct nr_ctx {
     int64_t last_iterations = 0;
     // timing
     bool timing = false;
-    bool trace_wide = false;    // nr_trace_closest over the 4-wide mesh layout
+    int trace_layout = 0;       // mesh-tree layout of nr_trace_closest (NR_LAY_*)
     std::vector<hipEvent_t> ev;             // per-launch marks
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     float last_ms[8] = {0};
@@ -1567,7 +1714,7 @@ nr_ctx* nr_create(int device) {
         nr_destroy(c);
         return nullptr;
     }
-    if (hipHostMalloc((void**)&c->h_live, sizeof(int32_t) * 2 * NR_MAX_GROUPS, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&c->h_live, sizeof(int32_t) * 4 * NR_MAX_GROUPS, hipHostMallocDefault) != hipSuccess) {
         g_err = "hipHostMalloc failed";
         nr_destroy(c);
         return nullptr;
@@ -1740,6 +1887,7 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
         if (e) return e;
         gm.nodes = dn; gm.tris = dt; gm.idx = di; gm.verts = dv;
         gm.wnodes = nullptr;
+        gm.qnodes = nullptr;
         if (n_nodes > 0) {
             GWNode* dw = nullptr;
             HIPCK(hipMalloc((void**)&dw, sizeof(GWNode) * (size_t)n_nodes));
@@ -1747,6 +1895,12 @@ int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
             hipLaunchKernelGGL(k_wide_nodes, dim3((n_nodes + 255) / 256), dim3(256), 0, c->stream, dn, n_nodes, dw);
             HIPCK(hipGetLastError());
             gm.wnodes = dw;
+            GQNode* dq = nullptr;
+            HIPCK(hipMalloc((void**)&dq, sizeof(GQNode) * (size_t)n_nodes));
+            c->scene_mem.push_back(dq);
+            hipLaunchKernelGGL(k_quant_nodes, dim3((n_nodes + 255) / 256), dim3(256), 0, c->stream, dw, n_nodes, dq);
+            HIPCK(hipGetLastError());
+            gm.qnodes = dq;
         }
         meshes[mi] = gm;
         mesh_nodes[mi] = n_nodes;
@@ -2078,7 +2232,7 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
     const size_t lds = trav_lds_bytes();
     dim3 gS(c->shade_grid);
     size_t ei = 0;
-    float t_ext = 0.0f, t_shade = 0.0f, t_other = 0.0f, t_shadow = 0.0f;
+    float t_ext = 0.0f, t_shade = 0.0f, t_other = 0.0f, t_shadow = 0.0f, t_tail = 0.0f;
     std::vector<std::pair<size_t, int>> marks;   // (event index of start, category)
     auto mark = [&](int cat) {
         if (!c->timing) return;
@@ -2122,7 +2276,7 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
             TraceArgs A;
             StackCfg SC;
             hipStream_t st;
-            int64_t it = 0, bound = 0;
+            int64_t it = 0, bound = 0, pending = 1;
             bool done = false;
         } grp[NR_MAX_GROUPS];
         const size_t lv = c->scene.stack_depth > LDS_STACK_LEVELS ? (size_t)(c->scene.stack_depth - LDS_STACK_LEVELS) : 0;
@@ -2198,7 +2352,7 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
                 Group& gr = grp[g];
                 if (gr.done) continue;
                 const int64_t it = gr.it;
-                int32_t* live = c->h_live + 2 * g;
+                int32_t* live = c->h_live + 4 * g;        // [occupied, pending] of the last two iterations
                 if (it >= 2) {
                     if (G > 1) {
                         const hipError_t q = hipEventQuery(c->gev[g][it & 1]);
@@ -2208,7 +2362,8 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
                         HIPCK(hipEventSynchronize(c->gev[g][it & 1]));
                     }
                     progressed = true;
-                    gr.bound = live[it & 1];
+                    gr.bound = live[2 * (it & 1)];
+                    gr.pending = live[2 * (it & 1) + 1];
                     if (gr.bound == 0) { gr.done = true; active--; continue; }
                 }
                 progressed = true;
@@ -2216,11 +2371,31 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
                 PathPtrs& P = gr.P;
                 TraceArgs& A = gr.A;
                 hipStream_t st = gr.st;
+                const bool other = (c->obj_types & NR_TY_OTHER) != 0;
+                constexpr int TYI = NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH;
+                if (it >= 2 && gr.pending == 0 && gr.bound <= (int64_t)c->tail_paths) {
+                    // tail mode: every sample is started; one thread per live path runs it to its end,
+                    // then k_regen folds them all
+                    mark(5);
+                    const dim3 gt((uint32_t)((gr.bound + 127) / 128));
+                    if (fast) {
+                        if (other) hipLaunchKernelGGL((k_tail<NR_TY_ALL, true>), gt, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                        else hipLaunchKernelGGL((k_tail<TYI, true>), gt, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                    } else {
+                        if (other) hipLaunchKernelGGL((k_tail<NR_TY_ALL, false>), gt, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                        else hipLaunchKernelGGL((k_tail<TYI, false>), gt, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                    }
+                    mark(2);
+                    const uint32_t nt = A.t1 - A.t0;
+                    hipLaunchKernelGGL(k_regen, dim3((nt + REGEN_BLOCK - 1) / REGEN_BLOCK), dim3(REGEN_BLOCK), 0, st, P, A);
+                    HIPCK(hipGetLastError());
+                    gr.done = true;
+                    active--;
+                    continue;
+                }
                 hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, st, P.counts, (int32_t)A.in);
                 mark(0);
                 const dim3 gx((uint32_t)((gr.bound + 127) / 128));
-                const bool other = (c->obj_types & NR_TY_OTHER) != 0;
-                constexpr int TYI = NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH;
                 if (fast) {
                     if (other) hipLaunchKernelGGL((k_extend<NR_TY_ALL, true>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
                     else hipLaunchKernelGGL((k_extend<TYI, true>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
@@ -2242,7 +2417,8 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
                 const uint32_t nt = A.t1 - A.t0;
                 hipLaunchKernelGGL(k_regen, dim3((nt + REGEN_BLOCK - 1) / REGEN_BLOCK), dim3(REGEN_BLOCK), 0, st, P, A);
                 HIPCK(hipGetLastError());
-                HIPCK(hipMemcpyAsync(live + (it & 1), P.counts + CNT_OCC, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                HIPCK(hipMemcpyAsync(live + 2 * (it & 1), P.counts + CNT_OCC, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                     st));
                 HIPCK(hipEventRecord(c->gev[g][it & 1], st));
                 A.in = 1 - A.in;
                 gr.it++;
@@ -2270,6 +2446,7 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
             if (marks[k].second == 0) t_ext += ms;
             else if (marks[k].second == 1) t_shade += ms;
             else if (marks[k].second == 4) t_shadow += ms;
+            else if (marks[k].second == 5) t_tail += ms;
             else t_other += ms;
         }
         float tot = 0.0f;
@@ -2277,7 +2454,11 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
         c->last_ms[0] = tot; c->last_ms[1] = t_ext; c->last_ms[2] = t_shade; c->last_ms[3] = t_other;
         c->last_ms[4] = (float)c->last_iterations;
         c->last_ms[5] = t_shadow;
-        c->n_last = 6;
+        c->last_ms[6] = t_tail;
+        unsigned long long traced = 0;                   // timing mode renders with one stream group
+        HIPCK(hipMemcpy(&traced, c->P.counts + CNT_TRACED, sizeof(traced), hipMemcpyDeviceToHost));
+        c->last_ms[7] = (float)traced;
+        c->n_last = 8;
         int32_t err = 0;
         HIPCK(hipMemcpy(&err, c->P.counts + CNT_ERR, sizeof(int32_t), hipMemcpyDeviceToHost));
         if (err) return fail(NR_ERR_LIMIT, "traversal stack overflow");
@@ -2314,11 +2495,26 @@ float* nr_aov_device_ptr(nr_ctx* c, int view, int layer) {
     return c->views[view].layer[layer];
 }
 
-int nr_set_trace_layout(nr_ctx* c, int wide) {
+int nr_set_trace_layout(nr_ctx* c, int layout) {
     if (!c) return fail(NR_ERR_INVALID, "null context");
-    if (wide != 0 && wide != 1) return fail(NR_ERR_INVALID, "layout must be 0 (binary) or 1 (4-wide)");
-    c->trace_wide = wide == 1;
+    if (layout < NR_LAY_BINARY || layout > NR_LAY_QUANT)
+        return fail(NR_ERR_INVALID, "layout must be 0 (binary), 1 (4-wide) or 2 (quantised 4-wide)");
+    c->trace_layout = layout;
     return NR_OK;
+}
+
+// k_trace_closest for the scene's object types and the context's layout
+static void launch_trace(nr_ctx* c, dim3 g, size_t lds, const float* rays, int n, nr_hit* hits, int32_t* counts,
+                         const StackCfg& SC) {
+    constexpr int TYI = NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH;
+    const dim3 b(128);
+    const bool all = (c->obj_types & NR_TY_OTHER) != 0;
+#define NR_TRACE_LAUNCH(TY_, LAY_) \
+    hipLaunchKernelGGL((k_trace_closest<TY_, LAY_>), g, b, lds, c->stream, c->d_scene, rays, n, hits, counts, SC)
+    if (c->trace_layout == NR_LAY_QUANT) { if (all) NR_TRACE_LAUNCH(NR_TY_ALL, NR_LAY_QUANT); else NR_TRACE_LAUNCH(TYI, NR_LAY_QUANT); }
+    else if (c->trace_layout == NR_LAY_WIDE) { if (all) NR_TRACE_LAUNCH(NR_TY_ALL, NR_LAY_WIDE); else NR_TRACE_LAUNCH(TYI, NR_LAY_WIDE); }
+    else { if (all) NR_TRACE_LAUNCH(NR_TY_ALL, NR_LAY_BINARY); else NR_TRACE_LAUNCH(TYI, NR_LAY_BINARY); }
+#undef NR_TRACE_LAUNCH
 }
 
 int nr_trace_closest_device(nr_ctx* c, const float* d_rays, int32_t n, nr_hit* d_hits) {
@@ -2333,16 +2529,7 @@ int nr_trace_closest_device(nr_ctx* c, const float* d_rays, int32_t n, nr_hit* d
     HIPCK(hipMemsetAsync(c->d_err + CNT_ERR, 0, sizeof(int32_t), c->stream));
     hipEvent_t e0 = c->ev_begin, e1 = c->ev_end;
     if (c->timing) (void)hipEventRecord(e0, c->stream);
-    constexpr int TYI = NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH;
-    const dim3 g((n + 127) / 128), b(128);
-    const size_t lds = trav_lds_bytes();
-    if (c->obj_types & NR_TY_OTHER) {
-        if (c->trace_wide) hipLaunchKernelGGL((k_trace_closest<NR_TY_ALL, true>), g, b, lds, c->stream, c->d_scene, d_rays, n, d_hits, c->d_err, SC);
-        else hipLaunchKernelGGL((k_trace_closest<NR_TY_ALL, false>), g, b, lds, c->stream, c->d_scene, d_rays, n, d_hits, c->d_err, SC);
-    } else {
-        if (c->trace_wide) hipLaunchKernelGGL((k_trace_closest<TYI, true>), g, b, lds, c->stream, c->d_scene, d_rays, n, d_hits, c->d_err, SC);
-        else hipLaunchKernelGGL((k_trace_closest<TYI, false>), g, b, lds, c->stream, c->d_scene, d_rays, n, d_hits, c->d_err, SC);
-    }
+    launch_trace(c, dim3((n + 127) / 128), trav_lds_bytes(), d_rays, n, d_hits, c->d_err, SC);
     HIPCK(hipGetLastError());
     if (c->timing) {
         (void)hipEventRecord(e1, c->stream);
@@ -2376,21 +2563,11 @@ int nr_trace_closest_device_noalloc(nr_ctx* c, const float* d_rays, int32_t n, n
     int32_t* counters = (int32_t*)c->scratch;
     int2* spill = (int2*)((char*)c->scratch + NR_NOALLOC_COUNTER_BYTES);
     hipLaunchKernelGGL(k_clear_err, dim3(1), dim3(64), 0, c->stream, counters + CNT_ERR);
-    constexpr int TYI = NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH;
     const size_t lds = trav_lds_bytes();
     for (size_t off = 0; off < (size_t)n; off += chunk) {
         const int m = (int)std::min(chunk, (size_t)n - off);
         const StackCfg SC{spill, (size_t)m};
-        const dim3 g((m + 127) / 128), b(128);
-        const float* r = d_rays + 6 * off;
-        nr_hit* h = d_hits + off;
-        if (c->obj_types & NR_TY_OTHER) {
-            if (c->trace_wide) hipLaunchKernelGGL((k_trace_closest<NR_TY_ALL, true>), g, b, lds, c->stream, c->d_scene, r, m, h, counters, SC);
-            else hipLaunchKernelGGL((k_trace_closest<NR_TY_ALL, false>), g, b, lds, c->stream, c->d_scene, r, m, h, counters, SC);
-        } else {
-            if (c->trace_wide) hipLaunchKernelGGL((k_trace_closest<TYI, true>), g, b, lds, c->stream, c->d_scene, r, m, h, counters, SC);
-            else hipLaunchKernelGGL((k_trace_closest<TYI, false>), g, b, lds, c->stream, c->d_scene, r, m, h, counters, SC);
-        }
+        launch_trace(c, dim3((m + 127) / 128), lds, d_rays + 6 * off, m, d_hits + off, counters, SC);
         HIPCK(hipGetLastError());
     }
     return NR_OK;
@@ -2710,11 +2887,13 @@ int nr_set_tuning(nr_ctx* c, const nr_tuning* t) {
     if (t->slots < 1 || t->slots > NR_SLOTS_MAX) return fail(NR_ERR_INVALID, "slots outside 1..8");
     if (t->stream_groups < 1 || t->stream_groups > NR_MAX_GROUPS) return fail(NR_ERR_INVALID, "stream groups outside 1..4");
     if (t->shade_wg_per_cu < 1 || t->shade_wg_per_cu > 32) return fail(NR_ERR_INVALID, "shade workgroups per CU outside 1..32");
+    if (t->tail_paths < 0) return fail(NR_ERR_INVALID, "negative tail_paths");
     HIPCK(hipSetDevice(c->device));
     HIPCK(hipStreamSynchronize(c->stream));                 // path state may be reallocated by the next render
     c->slots = t->slots;
     c->render_streams = t->stream_groups;
     c->shade_grid = c->n_cu * t->shade_wg_per_cu;
+    c->tail_paths = t->tail_paths;
     return NR_OK;
 }
 
@@ -2723,6 +2902,7 @@ int nr_get_tuning(nr_ctx* c, nr_tuning* t) {
     t->slots = c->slots;
     t->stream_groups = c->render_streams;
     t->shade_wg_per_cu = c->shade_grid / std::max(1, c->n_cu);
+    t->tail_paths = c->tail_paths;
     return NR_OK;
 }
 
@@ -2751,14 +2931,16 @@ int nr_autotune(nr_ctx* c, int view, const nr_render_desc* d, nr_tuning* out) {
     };
     double best_ms = 0.0;
     if (int e = measure(best, best_ms)) return e;
-    const int slot_c[] = {1, 2, 4, 6, 8}, group_c[] = {1, 2, 3, 4}, shade_c[] = {4, 8, 16};
-    for (int axis = 0; axis < 3; axis++) {
-        const int* cand = axis == 0 ? slot_c : (axis == 1 ? group_c : shade_c);
-        const int n = axis == 0 ? 5 : (axis == 1 ? 4 : 3);
+    const int slot_c[] = {1, 2, 4, 6, 8}, group_c[] = {1, 2, 3, 4}, shade_c[] = {4, 8, 16},
+              tail_c[] = {0, 65536, 262144, 1048576};
+    for (int axis = 0; axis < 4; axis++) {
+        const int* cand = axis == 0 ? slot_c : (axis == 1 ? group_c : (axis == 2 ? shade_c : tail_c));
+        const int n = axis == 0 ? 5 : (axis == 1 ? 4 : (axis == 2 ? 3 : 4));
         nr_tuning base = best;
         for (int k = 0; k < n; k++) {
             nr_tuning t = base;
-            int& field = axis == 0 ? t.slots : (axis == 1 ? t.stream_groups : t.shade_wg_per_cu);
+            int& field = axis == 0 ? t.slots
+                                   : (axis == 1 ? t.stream_groups : (axis == 2 ? t.shade_wg_per_cu : t.tail_paths));
             if (field == cand[k]) continue;
             field = cand[k];
             double ms = 0.0;

@@ -327,37 +327,57 @@ NR_D bool tri_hit(const Ray& r, f3 v0, f3 v1, f3 v2, float tmin, float tmax, flo
     return ok;
 }
 
+// Address spaces of the traversal's memory.  Scene records hold generic
+// pointers, and generic (flat) accesses wait on both the vector-memory and the
+// LDS counters; typed pointers make the compiler emit ds_* for LDS and global_*
+// for device memory, so a stack pop does not wait for node fetches in flight and
+// vice versa.  The traversal loops also copy the table pointers they use into
+// locals first: stack stores could otherwise alias the scene records and force
+// a reload of each pointer on every step.
+#define NR_LDS __attribute__((address_space(3)))
+#define NR_GLB __attribute__((address_space(1)))
+template <typename T>
+NR_D const NR_GLB T* glb(const T* p) { return (const NR_GLB T*)p; }
+// Clang vector types (HIP's float4 / int2 classes have no address-space-qualified
+// member operators)
+typedef float nr_f4 __attribute__((ext_vector_type(4)));
+typedef int nr_i2 __attribute__((ext_vector_type(2)));
+NR_D const NR_GLB nr_f4* glb_f4(const NR_GLB void* p) { return (const NR_GLB nr_f4*)p; }
+
 // Traversal stack: the first lds_cap levels live in LDS (one column per lane,
 // conflict-free), deeper levels spill to a per-thread column of global memory.
 // Entries are (ref, entry t).  cap bounds the total depth.
 struct Stack {
-    int2* lds;          // lds[level * 64 + lane]
-    int2* spill;        // spill[(level - lds_cap) * spill_stride + slot]
+    NR_LDS nr_i2* lds;  // lds[level * 64 + lane]
+    NR_GLB nr_i2* spill;// spill[(level - lds_cap) * spill_stride + slot]
     int lane;
     int lds_cap;
     int cap;
     size_t slot;
     size_t spill_stride;
     NR_D void push(int& sp, int32_t ref, float t) {
-        int2 e = make_int2(ref, __float_as_int(t));
+        nr_i2 e = {ref, __float_as_int(t)};
         if (sp < lds_cap) lds[sp * 64 + lane] = e;
         else if (sp < cap) spill[(size_t)(sp - lds_cap) * spill_stride + slot] = e;
         sp++;
     }
     NR_D void pop(int& sp, int32_t& ref, float& t) {
         sp--;
-        int2 e = sp < lds_cap ? lds[sp * 64 + lane] : spill[(size_t)(sp - lds_cap) * spill_stride + slot];
+        nr_i2 e;
+        if (sp < lds_cap) e = lds[sp * 64 + lane];
+        else e = spill[(size_t)(sp - lds_cap) * spill_stride + slot];
         ref = e.x;
         t = __int_as_float(e.y);
     }
 };
 
-NR_D void load_node(const GNode* nodes, int32_t i, float* lmn, float* lmx, float* rmn, float* rmx, int32_t& l, int32_t& r) {
-    const float4* p = reinterpret_cast<const float4*>(nodes + i);
-    float4 a = p[0];
-    float4 b = p[1];
-    float4 c = p[2];
-    float4 d = p[3];
+NR_D void load_node(const NR_GLB GNode* nodes, int32_t i, float* lmn, float* lmx, float* rmn, float* rmx, int32_t& l,
+                    int32_t& r) {
+    const NR_GLB nr_f4* p = glb_f4(nodes + i);
+    const nr_f4 a = p[0];
+    const nr_f4 b = p[1];
+    const nr_f4 c = p[2];
+    const nr_f4 d = p[3];
     lmn[0] = a.x; lmn[1] = a.y; lmn[2] = a.z; lmx[0] = a.w;
     lmx[1] = b.x; lmx[2] = b.y; rmn[0] = b.z; rmn[1] = b.w;
     rmn[2] = c.x; rmx[0] = c.y; rmx[1] = c.z; rmx[2] = c.w;
@@ -388,6 +408,8 @@ NR_D int32_t mesh_closest(const GMesh& M, const Ray& r, float tmin, float tmax, 
     if (sp0 + 1 > st.cap) { overflow = true; return -1; }
     int32_t ref = M.root_ref;
     float te = box_t(M.root_min, M.root_max, r, tmin, tmax);
+    const NR_GLB GNode* const nodes = glb(M.nodes);
+    const NR_GLB GTri* const tris = glb(M.tris);
     bool active = true;     // the lane still has work
     bool leaf = false;      // the lane holds a leaf to test
     while (true) {
@@ -399,7 +421,7 @@ NR_D int32_t mesh_closest(const GMesh& M, const Ray& r, float tmin, float tmax, 
                 leaf = true;
             } else {
                 float lmn[3], lmx[3], rmn[3], rmx[3]; int32_t l, rr;
-                load_node(M.nodes, ref, lmn, lmx, rmn, rmx, l, rr);
+                load_node(nodes, ref, lmn, lmx, rmn, rmx, l, rr);
                 float tl = box_t(lmn, lmx, r, tmin, tmax);
                 float tr = box_t(rmn, rmx, r, tmin, tmax);
                 bool left_near = tl < tr;
@@ -420,8 +442,8 @@ NR_D int32_t mesh_closest(const GMesh& M, const Ray& r, float tmin, float tmax, 
         if (held != 0ull && (__popcll(held) >= NR_LEAF_BATCH || walking == 0ull)) {
             if (leaf) {
                 int32_t slot = ~ref;
-                const float4* tp = reinterpret_cast<const float4*>(M.tris + slot);
-                float4 a = tp[0], b = tp[1], c = tp[2];
+                const NR_GLB nr_f4* tp = glb_f4(tris + slot);
+                const nr_f4 a = tp[0], b = tp[1], c = tp[2];
                 float t, u, v;
                 bool h = tri_hit(r, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), mk3(b.z, b.w, c.x), tmin, tmax, t, u, v);
                 if (h) {
@@ -451,6 +473,8 @@ NR_D int32_t mesh_closest_w(const GMesh& M, const Ray& r, float tmin, float tmax
     if (sp0 + 1 > st.cap) { overflow = true; return -1; }
     int32_t ref = M.root_ref;
     float te = box_t(M.root_min, M.root_max, r, tmin, tmax);
+    const NR_GLB GWNode* const wnodes = glb(M.wnodes);
+    const NR_GLB GTri* const tris = glb(M.tris);
     bool active = true;
     bool leaf = false;
     while (true) {
@@ -461,8 +485,8 @@ NR_D int32_t mesh_closest_w(const GMesh& M, const Ray& r, float tmin, float tmax
             } else if (ref < 0) {
                 leaf = true;
             } else {
-                const float4* p = reinterpret_cast<const float4*>(M.wnodes + ref);
-                float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3], q4 = p[4], q5 = p[5], q6 = p[6];
+                const NR_GLB nr_f4* p = glb_f4(wnodes + ref);
+                const nr_f4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3], q4 = p[4], q5 = p[5], q6 = p[6];
                 const float lx[4] = {q0.x, q0.y, q0.z, q0.w}, ly[4] = {q1.x, q1.y, q1.z, q1.w};
                 const float lz[4] = {q2.x, q2.y, q2.z, q2.w}, hx[4] = {q3.x, q3.y, q3.z, q3.w};
                 const float hy[4] = {q4.x, q4.y, q4.z, q4.w}, hz[4] = {q5.x, q5.y, q5.z, q5.w};
@@ -503,8 +527,94 @@ NR_D int32_t mesh_closest_w(const GMesh& M, const Ray& r, float tmin, float tmax
         if (held != 0ull && (__popcll(held) >= NR_LEAF_BATCH || walking == 0ull)) {
             if (leaf) {
                 int32_t slot = ~ref;
-                const float4* tp = reinterpret_cast<const float4*>(M.tris + slot);
-                float4 a = tp[0], b = tp[1], cc = tp[2];
+                const NR_GLB nr_f4* tp = glb_f4(tris + slot);
+                const nr_f4 a = tp[0], b = tp[1], cc = tp[2];
+                float tt, u, v;
+                bool h = tri_hit(r, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), mk3(b.z, b.w, cc.x), tmin, tmax, tt, u, v);
+                if (h) {
+                    best = slot; tmax = tt; bt = tt; bu = u; bv = v;
+                }
+                leaf = false;
+                if (ANY && h) active = false;
+                else if (sp > sp0) st.pop(sp, ref, te);
+                else active = false;
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+    }
+    return best;
+}
+
+// The 4-wide traversal over quantised nodes (GQNode): one 64 B fetch per two
+// binary levels instead of three 64 B nodes.  Decoded boxes contain the exact
+// ones, so the closest hit is the binary traversal's (of two triangles hit at
+// exactly the same t either may be reported).  Leaf batching as in mesh_closest.
+NR_D float qbyte(uint32_t w, int k) { return (float)((w >> (8 * k)) & 255u); }
+template <bool ANY = false>
+NR_D int32_t mesh_closest_q(const GMesh& M, const Ray& r, float tmin, float tmax, Stack& st, int sp0,
+                            float& bt, float& bu, float& bv, bool& overflow) {
+    int32_t best = -1;
+    int sp = sp0;
+    if (sp0 + 1 > st.cap) { overflow = true; return -1; }
+    int32_t ref = M.root_ref;
+    float te = box_t(M.root_min, M.root_max, r, tmin, tmax);
+    const NR_GLB GQNode* const qnodes = glb(M.qnodes);
+    const NR_GLB GTri* const tris = glb(M.tris);
+    bool active = true;
+    bool leaf = false;
+    while (true) {
+        if (active && !leaf) {
+            if (!(te < tmax)) {
+                if (sp > sp0) st.pop(sp, ref, te);
+                else active = false;
+            } else if (ref < 0) {
+                leaf = true;
+            } else {
+                const NR_GLB nr_f4* p = glb_f4(qnodes + ref);
+                const nr_f4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+                const uint32_t ex = __float_as_uint(q0.w);
+                const float sx = __uint_as_float((ex & 255u) << 23), sy = __uint_as_float(((ex >> 8) & 255u) << 23),
+                            sz = __uint_as_float(((ex >> 16) & 255u) << 23);
+                const uint32_t lx = __float_as_uint(q1.x), ly = __float_as_uint(q1.y), lz = __float_as_uint(q1.z);
+                const uint32_t hx = __float_as_uint(q1.w), hy = __float_as_uint(q2.x), hz = __float_as_uint(q2.y);
+                int32_t c[4] = {__float_as_int(q2.z), __float_as_int(q2.w), __float_as_int(q3.x), __float_as_int(q3.y)};
+                float t[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float mn[3] = {q0.x + qbyte(lx, k) * sx, q0.y + qbyte(ly, k) * sy, q0.z + qbyte(lz, k) * sz};
+                    const float mx[3] = {q0.x + qbyte(hx, k) * sx, q0.y + qbyte(hy, k) * sy, q0.z + qbyte(hz, k) * sz};
+                    t[k] = c[k] != NR_WIDE_EMPTY ? box_t(mn, mx, r, tmin, tmax) : NR_FMAX;
+                }
+#define NR_CSWAP(a, b)                                                         \
+    if (t[b] < t[a]) {                                                         \
+        float tt = t[a]; t[a] = t[b]; t[b] = tt;                               \
+        int32_t cc = c[a]; c[a] = c[b]; c[b] = cc;                             \
+    }
+                NR_CSWAP(0, 1) NR_CSWAP(2, 3) NR_CSWAP(0, 2) NR_CSWAP(1, 3) NR_CSWAP(1, 2)
+#undef NR_CSWAP
+                const int nh = (t[0] < tmax) + (t[1] < tmax) + (t[2] < tmax) + (t[3] < tmax);
+                if (nh == 0) {
+                    if (sp > sp0) st.pop(sp, ref, te);
+                    else active = false;
+                } else if (sp + nh - 1 > st.cap) {
+                    overflow = true;
+                    active = false;
+                } else {
+                    if (nh > 3) st.push(sp, c[3], t[3]);
+                    if (nh > 2) st.push(sp, c[2], t[2]);
+                    if (nh > 1) st.push(sp, c[1], t[1]);
+                    ref = c[0];
+                    te = t[0];
+                }
+            }
+        }
+        unsigned long long held = __ballot(leaf);
+        unsigned long long walking = __ballot(active && !leaf);
+        if (held != 0ull && (__popcll(held) >= NR_LEAF_BATCH || walking == 0ull)) {
+            if (leaf) {
+                int32_t slot = ~ref;
+                const NR_GLB nr_f4* tp = glb_f4(tris + slot);
+                const nr_f4 a = tp[0], b = tp[1], cc = tp[2];
                 float tt, u, v;
                 bool h = tri_hit(r, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), mk3(b.z, b.w, cc.x), tmin, tmax, tt, u, v);
                 if (h) {
@@ -616,7 +726,12 @@ __device__ NR_GENERIC_ATTR bool object_hit_generic(const GScene& S, const GObjec
 // set that covers it; the code of absent types, and its registers, drop out).
 enum : int { NR_TY_QUAD = 1, NR_TY_SPHERE = 2, NR_TY_MESH = 4, NR_TY_OTHER = 8, NR_TY_ALL = 15 };
 
-template <int TY, bool ANY = false, bool WIDE = false>
+// Mesh-tree layouts a traversal can use: the reference's binary tree (its visit
+// order, what the reference's estimator needs), the 4-wide tree, the quantised
+// 4-wide tree (fast mode).
+enum : int { NR_LAY_BINARY = 0, NR_LAY_WIDE = 1, NR_LAY_QUANT = 2 };
+
+template <int TY, bool ANY = false, int LAY = NR_LAY_BINARY>
 NR_D bool object_t(const GScene& S, const GObject& o, const Ray& r, Stack& st, int sp0, bool& overflow,
                    float& tw, int32_t& slot, float& u, float& v, float& tobj) {
     HRec rec;
@@ -629,8 +744,9 @@ NR_D bool object_t(const GScene& S, const GObject& o, const Ray& r, Stack& st, i
         hrec_clear(rec);
         if ((TY & NR_TY_MESH) && o.type == 4) {
             const GMesh& M = S.meshes[o.mesh];
-            slot = (WIDE && M.wnodes) ? mesh_closest_w<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow)
-                                      : mesh_closest<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow);
+            slot = (LAY == NR_LAY_QUANT && M.qnodes) ? mesh_closest_q<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow)
+                   : (LAY == NR_LAY_WIDE && M.wnodes) ? mesh_closest_w<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow)
+                                                      : mesh_closest<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow);
             h = slot >= 0;
             if (h) mesh_interp(M, slot, orr, tobj, u, v, rec);
         } else if ((TY & NR_TY_QUAD) && o.type == 1) {
@@ -651,14 +767,15 @@ NR_D bool object_t(const GScene& S, const GObject& o, const Ray& r, Stack& st, i
 // index of the closest hit or -1; rec receives the world-space record.  Only
 // (object, t, triangle slot, u, v) stay live during the traversal; the record
 // of the winner is rebuilt at the end by the same deterministic arithmetic.
-// WIDE: meshes are traversed over their 4-wide layout (fast mode)
-template <int TY, bool WIDE = false>
+// LAY: the mesh-tree layout traversed (NR_LAY_*)
+template <int TY, int LAY = NR_LAY_BINARY>
 NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, bool& overflow) {
     float tmin = NR_EPS, tmax = NR_FMAX;
     int32_t best = -1, bslot = -1;
     float bu = 0.0f, bv = 0.0f, btobj = 0.0f;
     int sp = 0;
     if (st.cap < 1 || S.n_objects == 0) return -1;
+    const NR_GLB GNode* const top = glb(S.nodes);
     st.push(sp, S.root_ref, box_t(S.root_min, S.root_max, r, tmin, tmax));
     while (sp > 0) {
         int32_t ref; float te;
@@ -668,7 +785,7 @@ NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, 
             int32_t oi = ~ref;
             float tw, u, v, tobj;
             int32_t slot;
-            if (object_t<TY, false, WIDE>(S, S.objects[oi], r, st, sp, overflow, tw, slot, u, v, tobj)) {
+            if (object_t<TY, false, LAY>(S, S.objects[oi], r, st, sp, overflow, tw, slot, u, v, tobj)) {
                 if (tmin < tw && tw < tmax) {
                     best = oi; bslot = slot; bu = u; bv = v; btobj = tobj;
                     tmax = tw;
@@ -677,7 +794,7 @@ NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, 
             if (overflow) break;
         } else {
             float lmn[3], lmx[3], rmn[3], rmx[3]; int32_t l, rr;
-            load_node(S.nodes, ref, lmn, lmx, rmn, rmx, l, rr);
+            load_node(top, ref, lmn, lmx, rmn, rmx, l, rr);
             float tl = box_t(lmn, lmx, r, tmin, tmax);
             float tr = box_t(rmn, rmx, r, tmin, tmax);
             bool left_near = tl < tr;
@@ -711,11 +828,12 @@ NR_D int32_t scene_closest(const GScene& S, const Ray& r, HRec& rec, Stack& st, 
 // Occlusion query with the visibility of scene_closest: true iff scene_closest
 // would return an object for r (some object's hit lies in (NR_EPS, inf)).  A
 // lane stops at the first such hit; mesh leaves stop at their first triangle.
-template <int TY, bool WIDE = true>
+template <int TY, int LAY = NR_LAY_WIDE>
 NR_D bool scene_occluded(const GScene& S, const Ray& r, Stack& st, bool& overflow) {
     const float tmin = NR_EPS, tmax = NR_FMAX;
     int sp = 0;
     if (st.cap < 1 || S.n_objects == 0) return false;
+    const NR_GLB GNode* const top = glb(S.nodes);
     st.push(sp, S.root_ref, box_t(S.root_min, S.root_max, r, tmin, tmax));
     while (sp > 0) {
         int32_t ref; float te;
@@ -724,13 +842,13 @@ NR_D bool scene_occluded(const GScene& S, const Ray& r, Stack& st, bool& overflo
         if (ref < 0) {
             float tw, u, v, tobj;
             int32_t slot;
-            if (object_t<TY, true, WIDE>(S, S.objects[~ref], r, st, sp, overflow, tw, slot, u, v, tobj) && tmin < tw &&
+            if (object_t<TY, true, LAY>(S, S.objects[~ref], r, st, sp, overflow, tw, slot, u, v, tobj) && tmin < tw &&
                 tw < tmax)
                 return true;
             if (overflow) return false;
         } else {
             float lmn[3], lmx[3], rmn[3], rmx[3]; int32_t l, rr;
-            load_node(S.nodes, ref, lmn, lmx, rmn, rmx, l, rr);
+            load_node(top, ref, lmn, lmx, rmn, rmx, l, rr);
             float tl = box_t(lmn, lmx, r, tmin, tmax);
             float tr = box_t(rmn, rmx, r, tmin, tmax);
             bool left_near = tl < tr;

@@ -3,8 +3,11 @@
 // Layout in HBM (all tables are flat arrays, one allocation each):
 //   GNode    64 B   binary BVH node holding BOTH child boxes + child refs, so one
 //                   interior visit is one 64 B line (4 x dwordx4 loads)
-//   GWNode  128 B   4-wide mesh node for fast mode: the grandchildren of binary
-//                   node i at index i, boxes SoA, one 128 B line (8 x dwordx4)
+//   GWNode  128 B   4-wide mesh node: the grandchildren of binary node i at
+//                   index i, boxes SoA, one 128 B line (8 x dwordx4)
+//   GQNode   64 B   the same 4-wide node with its child boxes quantised to 8 bits
+//                   per plane on a per-node power-of-two grid (fast mode), one
+//                   64 B half line (4 x dwordx4)
 //   GTri     48 B   triangle positions v0,v1,v2 in BVH leaf order + original index
 //   GVertex  48 B   shading attributes (normal, tangent, uv), read once per closest hit
 //   GObject 128 B   object record (transform, type, material, mesh)
@@ -44,6 +47,24 @@ struct alignas(128) GWNode {
 };
 static_assert(sizeof(GWNode) == 128, "GWNode is one 128 B line");
 
+// Quantised 4-wide node (fast mode): child k's box on axis a is
+//   [origin[a] + qlo_k * 2^e_a, origin[a] + qhi_k * 2^e_a]
+// with the 8-bit grid indices of child k in byte k of qlo[a] / qhi[a] and the
+// biased exponent e_a + 127 in byte a of `exps` (so 2^e_a is a float with that
+// exponent field: exact, and q * 2^e_a is exact).  The indices are rounded
+// outward and checked with the same float additions the traversal performs, so
+// every decoded box contains the exact child box: traversals over it visit a
+// superset of the nodes and find the same closest hit.
+struct alignas(64) GQNode {
+    float origin[3];
+    uint32_t exps;
+    uint32_t qlo[3];
+    uint32_t qhi[3];
+    int32_t ref[4];         // binary refs as in GWNode
+    int32_t pad[2];
+};
+static_assert(sizeof(GQNode) == 64, "GQNode is one 64 B half line");
+
 struct alignas(16) GTri {
     float v0[3], v1[3], v2[3];
     int32_t orig;
@@ -59,6 +80,7 @@ struct alignas(16) GVertex {
 struct alignas(16) GMesh {
     const GNode* nodes;
     const GWNode* wnodes;   // same tree, 4 wide (null for a one-triangle mesh)
+    const GQNode* qnodes;   // same tree, 4 wide, quantised boxes (null for a one-triangle mesh)
     const GTri* tris;       // BVH slot order
     const uint32_t* idx;    // 3 vertex indices per BVH slot
     const GVertex* verts;

@@ -75,7 +75,7 @@ def lib() -> C.CDLL:
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
-    if L.nr_abi_version() != 4:
+    if L.nr_abi_version() != 5:
         raise NectarError("ABI version mismatch")
     _lib = L
     return L
@@ -128,19 +128,21 @@ class Context:
     def render(self, view: int, desc: abi.NrRenderDesc):
         _check(lib().nr_render(self._h, view, C.byref(desc)))
 
-    def set_tuning(self, slots: int = 4, stream_groups: int = 2, shade_wg_per_cu: int = 8):
-        _check(lib().nr_set_tuning(self._h, C.byref(abi.NrTuning(slots, stream_groups, shade_wg_per_cu))))
+    def set_tuning(self, slots: int = 4, stream_groups: int = 2, shade_wg_per_cu: int = 8, tail_paths: int = 262144):
+        _check(lib().nr_set_tuning(self._h, C.byref(abi.NrTuning(slots, stream_groups, shade_wg_per_cu, tail_paths))))
 
     def get_tuning(self) -> dict:
         t = abi.NrTuning()
         _check(lib().nr_get_tuning(self._h, C.byref(t)))
-        return {"slots": t.slots, "stream_groups": t.stream_groups, "shade_wg_per_cu": t.shade_wg_per_cu}
+        return {"slots": t.slots, "stream_groups": t.stream_groups, "shade_wg_per_cu": t.shade_wg_per_cu,
+                "tail_paths": t.tail_paths}
 
     def autotune(self, view: int, desc: abi.NrRenderDesc) -> dict:
         """Times candidate tunings on `desc`, keeps the fastest (the view's AOVs are cleared)."""
         t = abi.NrTuning()
         _check(lib().nr_autotune(self._h, view, C.byref(desc), C.byref(t)))
-        return {"slots": t.slots, "stream_groups": t.stream_groups, "shade_wg_per_cu": t.shade_wg_per_cu}
+        return {"slots": t.slots, "stream_groups": t.stream_groups, "shade_wg_per_cu": t.shade_wg_per_cu,
+                "tail_paths": t.tail_paths}
 
     def render_tiles(self, view: int, desc: abi.NrRenderDesc, tile_end: np.ndarray, tile_size: int):
         """nr_render with a per-tile sample limit (rows x cols uint32, row major, rows =
@@ -245,9 +247,10 @@ class Context:
         """Stream groups of later nr_render calls (1..4)."""
         _check(lib().nr_set_render_streams(self._h, n))
 
-    def set_trace_layout(self, wide: bool):
-        """Mesh layout of later trace_closest calls: the binary tree (False) or fast mode's 4-wide one."""
-        _check(lib().nr_set_trace_layout(self._h, int(bool(wide))))
+    def set_trace_layout(self, layout: int):
+        """Mesh layout of later trace_closest calls: 0 the binary tree, 1 the 4-wide tree, 2 the 4-wide
+        tree with quantised child boxes (fast mode's)."""
+        _check(lib().nr_set_trace_layout(self._h, int(layout)))
 
     def set_build_mode(self, mode: int):
         """Mesh tree builder of later uploads: abi.BUILD_HOST, BUILD_DEVICE or BUILD_AUTO."""

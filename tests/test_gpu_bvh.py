@@ -169,11 +169,14 @@ def test_set_transform_equals_fresh_upload(ctx, oracle):
             np.testing.assert_array_equal(got[f][hit], ref[f][hit])
 
 
-@pytest.mark.parametrize("scene", ["icosphere", "mixed", "soup"])
-def test_wide_layout_finds_the_binary_closest_hit(ctx, scene):
-    """Fast mode's 4-wide mesh traversal returns the binary traversal's closest hit: the same object and
-    distance for every ray, and the same triangle except where two triangles are hit at exactly the same
-    distance (an edge or vertex hit, where the visit order decides)."""
+@pytest.mark.parametrize("layout", [1, 2])
+@pytest.mark.parametrize("scene", ["icosphere", "mixed", "soup", "flat"])
+def test_wide_layout_finds_the_binary_closest_hit(ctx, scene, layout):
+    """The 4-wide mesh traversals (plain and with quantised child boxes, the one fast mode uses) return
+    the binary traversal's closest hit: the same object and distance for every ray, and the same
+    triangle except where two triangles are hit at exactly the same distance (an edge or vertex hit,
+    where the visit order decides).  "flat": a mesh in the plane y = 0.25 far from the origin, where
+    the quantisation grid of one axis has a tiny extent and the others large offsets."""
     if scene == "icosphere":
         v, t = scenes.displaced_icosphere(freq=80)
         s = _mesh_scene(v, t)
@@ -181,17 +184,31 @@ def test_wide_layout_finds_the_binary_closest_hit(ctx, scene):
     elif scene == "mixed":
         s, _, _ = scenes.mixed_scene()
         rays = scenes.random_rays(200_000, 4, center=(0, 0.3, 0), spread=2.5)
-    else:
+    elif scene == "soup":
         v, t = _soup(20_000, seed=11)
         s = _mesh_scene(v, t)
         rays = scenes.random_rays(200_000, 5, center=(0, 0, 0), spread=2.0)
+    else:
+        rng = np.random.default_rng(12)
+        g = 120
+        xz = np.stack(np.meshgrid(np.arange(g + 1), np.arange(g + 1), indexing="ij"), -1).reshape(-1, 2)
+        p = np.zeros((xz.shape[0], 3), np.float32)
+        p[:, 0] = 1000.0 + xz[:, 0] * 0.01 + rng.random(xz.shape[0]) * 1e-3
+        p[:, 1] = 0.25 + xz[:, 0] * 1e-5 + rng.random(xz.shape[0]) * 1e-6   # slightly tilted: an exactly
+                                                                              # flat triangle is never hit
+        p[:, 2] = -500.0 + xz[:, 1] * 0.01
+        q = (np.arange(g)[:, None] * (g + 1) + np.arange(g)[None, :]).reshape(-1)
+        t = np.concatenate([np.stack([q, q + 1, q + g + 1], 1), np.stack([q + 1, q + g + 2, q + g + 1], 1)])
+        t = t.astype(np.uint32)
+        s = _mesh_scene(scenes.mesh_vertices(p, t), t)
+        rays = scenes.random_rays(200_000, 6, center=(1000.6, 0.25, -499.4), spread=1.0)
     ctx.upload(s)
     ref = ctx.trace_closest(rays)
-    ctx.set_trace_layout(True)
+    ctx.set_trace_layout(layout)
     try:
         got = ctx.trace_closest(rays)
     finally:
-        ctx.set_trace_layout(False)
+        ctx.set_trace_layout(0)
     np.testing.assert_array_equal(got["object_id"], ref["object_id"])
     hit = ref["object_id"] >= 0
     assert hit.sum() > 10_000

@@ -135,6 +135,21 @@ def test_fast_is_deterministic_and_independent_of_stream_groups(ctx):
     np.testing.assert_array_equal(a, c)
 
 
+@pytest.mark.parametrize("tail", [0, 1 << 30])
+def test_fast_tail_mode_gives_the_same_image(ctx, tail):
+    """The tail kernel runs the fast estimator's per-path steps (next-event estimation and shadow rays
+    included) in the same order: the image is the wavefront's bit for bit."""
+    s, cam = _mixed_hdri_scene(res=56)
+    ctx.set_tuning(4, 2, 8, 262144)
+    ref = _render(ctx, s, cam, 6, abi.MODE_FAST)
+    ctx.set_tuning(3, 2, 8, tail)
+    try:
+        got = _render(ctx, s, cam, 6, abi.MODE_FAST, upload=False)
+    finally:
+        ctx.set_tuning(4, 2, 8, 262144)
+    np.testing.assert_array_equal(got, ref)
+
+
 def test_fast_with_a_simple_sky_runs_without_sky_sampling(ctx):
     """No HDRI tables: fast mode is the reference estimator with the 64-bit subsequence."""
     s, cam, _ = scenes.config1(spp=16)
