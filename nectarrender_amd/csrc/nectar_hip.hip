@@ -849,7 +849,7 @@ __device__ __forceinline__ int extend_one(const GScene& S, const PathPtrs& P, in
             P.diff[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             P.nrm[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
-        f3 Le = sky_sample(S.sky, r.d);
+        f3 Le = sky_sample<FAST>(S.sky, r.d);
         // fast mode: a scattered ray that finds the sky shares it with the
         // shadow rays of next-event estimation (power heuristic)
         if (FAST && S.sky.marg && a4.w > 0.0f) Le = Le * power_heuristic(a4.w, sky_cdf_pdf(S.sky, r.d));
@@ -894,7 +894,7 @@ k_extend(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A, StackCfg SC) {
 }
 
 // material evaluate (material.h:59-67, 152-180, 351-357)
-template <int M>
+template <int M, bool F = false>
 __device__ __forceinline__ f3 mat_eval(const GScene& S, const GMaterial& m, f3 n, f3 sn, f2 uv, f3 view, f3 l) {
     if (M == NR_MAT_LAMBERTIAN) {
         float c = dot(n, normalize(l));
@@ -912,10 +912,10 @@ __device__ __forceinline__ f3 mat_eval(const GScene& S, const GMaterial& m, f3 n
         float n_dot_h = fmaxf(dot(sn, h), 0.0f);
         float v_dot_h = fmaxf(dot(view, h), 0.0f);
         f3 F0 = s3(0.04f) * (1.0f - metallic) + albedo * metallic;
-        f3 F = F0 + (s3(1.0f) - F0) * nrm::powf_(1.0f - v_dot_h, 5.0f);
+        f3 Fr = F0 + (s3(1.0f) - F0) * Tm<F>::pow5(1.0f - v_dot_h);
         float D = ggx_D(n_dot_h, alpha);
         float G = smith_g1(n_dot_l, alpha) * smith_g1(n_dot_v, alpha);
-        f3 spec = F * (D * G / fmaxf(4.0f * n_dot_v * n_dot_l, 1e-4f));
+        f3 spec = Fr * (D * G / fmaxf(4.0f * n_dot_v * n_dot_l, 1e-4f));
         f3 diff = (albedo * (1.0f - metallic)) * (1.0f / NR_PI);
         return (diff + spec) * n_dot_l;
     } else {
@@ -969,7 +969,7 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
         if (!refl) {
             float r0 = (1.0f - ri) / (1.0f + ri);
             r0 = r0 * r0;
-            float refl_p = r0 + (1.0f - r0) * nrm::powf_((1.0f - cos_theta), 5.0f);
+            float refl_p = r0 + (1.0f - r0) * Tm<FAST>::pow5(1.0f - cos_theta);
             refl = refl_p > g.uniform();
         }
         nd = refl ? reflect(unit, nn) : refract(unit, nn, ri);
@@ -997,11 +997,11 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
         if (S.first_light >= 0) {                                          // brdf.cu:16-23
             const GObject& Lo = S.objects[S.first_light];
             if (g.rf() < 0.5f) {
-                if (M == NR_MAT_LAMBERTIAN) dir = uvw.xf(cosine_dir(g));
-                else if (M == NR_MAT_PBR) dir = pbr_pdf_generate(uvw, view, alpha, sp, g);
+                if (M == NR_MAT_LAMBERTIAN) dir = uvw.xf(cosine_dir<FAST>(g));
+                else if (M == NR_MAT_PBR) dir = pbr_pdf_generate<FAST>(uvw, view, alpha, sp, g);
                 else dir = unit_vector(g);
             } else {
-                dir = light_random(Lo, p, g);
+                dir = light_random<FAST>(Lo, p, g);
             }
             float mp;
             if (M == NR_MAT_LAMBERTIAN) mp = cosine_pdf_value(uvw, dir);
@@ -1009,8 +1009,8 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
             else mp = 1.0f / NR_PI4;
             pdf = 0.5f * mp + 0.5f * light_pdf(Lo, p, dir);
         } else {
-            if (M == NR_MAT_LAMBERTIAN) { dir = uvw.xf(cosine_dir(g)); pdf = cosine_pdf_value(uvw, dir); }
-            else if (M == NR_MAT_PBR) { dir = pbr_pdf_generate(uvw, view, alpha, sp, g); pdf = pbr_pdf_value(uvw, view, alpha, sp, dir); }
+            if (M == NR_MAT_LAMBERTIAN) { dir = uvw.xf(cosine_dir<FAST>(g)); pdf = cosine_pdf_value(uvw, dir); }
+            else if (M == NR_MAT_PBR) { dir = pbr_pdf_generate<FAST>(uvw, view, alpha, sp, g); pdf = pbr_pdf_value(uvw, view, alpha, sp, dir); }
             else { dir = unit_vector(g); pdf = 1.0f / NR_PI4; }
         }
         if (NEE_M && marg && bounce + 1 < A.depth) {
@@ -1019,7 +1019,7 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
             float u1 = g.uniform(), u2 = g.uniform(), ps;
             f3 ld = sky_cdf_sample(S.sky, marg, u1, u2, ps);
             if (ps > 0.0f) {
-                f3 f = mat_eval<M>(S, m, nn, sn, uv, normalize(-din), ld);
+                f3 f = mat_eval<M, FAST>(S, m, nn, sn, uv, normalize(-din), ld);
                 if (f.x > 0.0f || f.y > 0.0f || f.z > 0.0f) {
                     // pdf of the scattering technique in direction ld
                     float mp;
@@ -1027,7 +1027,7 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
                     else if (M == NR_MAT_PBR) mp = pbr_pdf_value(uvw, view, alpha, sp, ld);
                     else mp = 1.0f / NR_PI4;
                     if (S.first_light >= 0) mp = 0.5f * mp + 0.5f * light_pdf(S.objects[S.first_light], p, ld);
-                    f3 c = atten * f * sky_sample(S.sky, ld) * (power_heuristic(ps, mp) / ps);
+                    f3 c = atten * f * sky_sample<true>(S.sky, ld) * (power_heuristic(ps, mp) / ps);
                     if (c.x > 0.0f || c.y > 0.0f || c.z > 0.0f) {
                         P.sho[path] = make_float4(p.x, p.y, p.z, 0.0f);
                         P.shd[path] = make_float4(ld.x, ld.y, ld.z, 0.0f);
@@ -1038,7 +1038,7 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
             }
         }
         if (!(pdf <= 0.0f)) {
-            f3 brdf = mat_eval<M>(S, m, nn, sn, uv, normalize(-din), dir);
+            f3 brdf = mat_eval<M, FAST>(S, m, nn, sn, uv, normalize(-din), dir);
             atten = atten * divs(brdf, pdf);
             nd = dir;
             alive = true;

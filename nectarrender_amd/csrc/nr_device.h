@@ -21,6 +21,27 @@
 #define NR_HD __host__ __device__ __forceinline__
 #define NR_D __device__ __forceinline__
 
+// Transcendentals of the estimators.  The reference's estimator takes the
+// deterministic double-evaluated functions of nr_mathfn.h, which the oracle
+// shares (bit-identical AOVs).  Fast mode, checked by convergence rather than
+// per pixel, takes the device library's float functions and x^5 by products.
+template <bool F>
+struct Tm {
+    NR_D static float cos(float x) { return nrm::cosf_(x); }
+    NR_D static float sin(float x) { return nrm::sinf_(x); }
+    NR_D static float atan2(float y, float x) { return nrm::atan2f_(y, x); }
+    NR_D static float acos(float x) { return nrm::acosf_(x); }
+    NR_D static float pow5(float x) { return nrm::powf_(x, 5.0f); }
+};
+template <>
+struct Tm<true> {
+    NR_D static float cos(float x) { return cosf(x); }
+    NR_D static float sin(float x) { return sinf(x); }
+    NR_D static float atan2(float y, float x) { return atan2f(y, x); }
+    NR_D static float acos(float x) { return acosf(x); }
+    NR_D static float pow5(float x) { const float x2 = x * x; return x2 * x2 * x; }
+};
+
 struct f3 { float x, y, z; };
 struct f2 { float x, y; };
 
@@ -190,11 +211,12 @@ struct ONB {
     }
     NR_D f3 xf(f3 p) const { return (u * p.x + v * p.y) + w * p.z; }
 };
+template <bool F = false>
 NR_D f3 cosine_dir(Gen& g) {
     float r1 = g.rf();
     float r2 = g.rf();
     float phi = NR_PI2 * r1;
-    return mk3(nrm::cosf_(phi) * sqrtf(r2), nrm::sinf_(phi) * sqrtf(r2), sqrtf(1.0f - r2));
+    return mk3(Tm<F>::cos(phi) * sqrtf(r2), Tm<F>::sin(phi) * sqrtf(r2), sqrtf(1.0f - r2));
 }
 NR_D f3 unit_vector(Gen& g) {
     while (true) {
@@ -882,6 +904,7 @@ NR_D f3 tex_sample(const GScene& S, const GTex& t, f2 uv) {
 // ---------------------------------------------------------------------------
 // sky (light/include/skylight.h:33-122)
 // ---------------------------------------------------------------------------
+template <bool F = false>
 NR_D f3 sky_sample(const GSky& sk, f3 d) {
     if (sk.type == 1) {
         float a = (float)(0.5 * ((double)d.y + 1.0));
@@ -891,8 +914,8 @@ NR_D f3 sky_sample(const GSky& sk, f3 d) {
     if (sk.type == 2) {
         if (!sk.data) return s3(0.0f);
         f3 dir = mv(sk.rot, normalize(d));
-        float u = nrm::atan2f_(dir.z, dir.x) / NR_PI2 + 0.5f;
-        float v = nrm::acosf_(fminf(fmaxf(dir.y, -1.0f), 1.0f)) / NR_PI;
+        float u = Tm<F>::atan2(dir.z, dir.x) / NR_PI2 + 0.5f;
+        float v = Tm<F>::acos(fminf(fmaxf(dir.y, -1.0f), 1.0f)) / NR_PI;
         int W = sk.W, H = sk.H;
         float fx = u * (float)W - 0.5f;
         float fy = v * (float)H - 0.5f;
@@ -943,6 +966,7 @@ NR_D float pbr_pdf_value(const ONB& uvw, f3 view, float alpha, float sp, f3 d) {
     float spec_pdf = (ggx_D(n_dot_h, alpha) * n_dot_h) / (4.0f * v_dot_h);
     return (1.0f - sp) * diffuse_pdf + sp * spec_pdf;
 }
+template <bool F = false>
 NR_D f3 pbr_pdf_generate(const ONB& uvw, f3 view, float alpha, float sp, Gen& g) { // pdf.h:150-166
     if (g.rf() < sp) {
         float u1 = g.rf();
@@ -950,10 +974,10 @@ NR_D f3 pbr_pdf_generate(const ONB& uvw, f3 view, float alpha, float sp, Gen& g)
         float cos_theta = sqrtf((1.0f - u1) / (1.0f + (alpha * alpha - 1.0f) * u1));
         float sin_theta = sqrtf(fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
         float phi = NR_PI2 * u2;
-        f3 h = uvw.xf(mk3(sin_theta * nrm::cosf_(phi), sin_theta * nrm::sinf_(phi), cos_theta));
+        f3 h = uvw.xf(mk3(sin_theta * Tm<F>::cos(phi), sin_theta * Tm<F>::sin(phi), cos_theta));
         return normalize(h * (2.0f * dot(view, h)) - view);
     }
-    return uvw.xf(cosine_dir(g));
+    return uvw.xf(cosine_dir<F>(g));
 }
 NR_D float cosine_pdf_value(const ONB& uvw, f3 d) {                           // pdf.h:26-29
     float c = dot(normalize(d), uvw.w);
@@ -996,6 +1020,7 @@ NR_D float sphere_light_pdf(const GObject& o, f3 origin, f3 dir) {            //
     float solid_angle = NR_PI2 * (1.0f - cos_theta_max);
     return 1.0f / solid_angle;
 }
+template <bool F = false>
 NR_D f3 sphere_light_random(const GObject& o, f3 origin, Gen& g) {            // shapes.h:78-108
     f3 direction = ld3(o.xf.pos) - origin;
     float dist2 = len2(direction);
@@ -1006,8 +1031,8 @@ NR_D f3 sphere_light_random(const GObject& o, f3 origin, Gen& g) {            //
     float rad_sq = o.radius * o.radius;
     float z = 1.0f + r2 * (sqrtf(1.0f - rad_sq / dist2) - 1.0f);
     float phi = NR_PI2 * r1;
-    float x = nrm::cosf_(phi) * sqrtf(1.0f - z * z);
-    float y = nrm::sinf_(phi) * sqrtf(1.0f - z * z);
+    float x = Tm<F>::cos(phi) * sqrtf(1.0f - z * z);
+    float y = Tm<F>::sin(phi) * sqrtf(1.0f - z * z);
     return uvw.xf(mk3(x, y, z));
 }
 NR_D int light_geom(const GObject& L) {
@@ -1019,10 +1044,11 @@ NR_D float light_pdf(const GObject& L, f3 origin, f3 dir) {
     if (t == 2) return sphere_light_pdf(L, origin, dir);
     return 0.0f;
 }
+template <bool F = false>
 NR_D f3 light_random(const GObject& L, f3 origin, Gen& g) {
     int t = light_geom(L);
     if (t == 1) return quad_light_random(L, origin, g);
-    if (t == 2) return sphere_light_random(L, origin, g);
+    if (t == 2) return sphere_light_random<F>(L, origin, g);
     if (t == 4) return s3(0.0f);
     return mk3(1.0f, 0.0f, 0.0f);
 }
@@ -1106,7 +1132,8 @@ NR_HD Ray screen_space_ray(const GCamera& c, float su, float sv) {
 }
 
 // ---------------------------------------------------------------------------
-// HDRI importance sampling (fast mode).  The sky lookup maps a direction d to
+// HDRI importance sampling (fast mode, with fast mode's float transcendentals).
+// The sky lookup maps a direction d to
 // u = atan2(z, x) / 2pi + 0.5, v = acos(y) / pi after the sky rotation R; the
 // density is piecewise constant over texels in (u, v), so the solid-angle pdf is
 // p(u, v) / (2 pi^2 sin(pi v)).
@@ -1146,8 +1173,8 @@ NR_D f3 sky_cdf_sample(const GSky& sk, const float* marg, float u1, float u2, fl
     const float u = ((float)i + fminf(fmaxf(du, 0.0f), 1.0f)) / (float)W;
     const float v = ((float)j + fminf(fmaxf(dv, 0.0f), 1.0f)) / (float)H;
     const float theta = v * NR_PI, phi = (u - 0.5f) * NR_PI2;
-    const float st = nrm::sinf_(theta), ct = nrm::cosf_(theta);
-    const f3 local = mk3(st * nrm::cosf_(phi), ct, st * nrm::sinf_(phi));
+    const float st = Tm<true>::sin(theta), ct = Tm<true>::cos(theta);
+    const f3 local = mk3(st * Tm<true>::cos(phi), ct, st * Tm<true>::sin(phi));
     const float p_uv = (m1 - m0) * pt * (float)(W * H);
     pdf = st > 1e-7f ? p_uv / (2.0f * NR_PI * NR_PI * st) : 0.0f;
     return mtv(sk.rot, local);
@@ -1156,13 +1183,13 @@ NR_D f3 sky_cdf_sample(const GSky& sk, const float* marg, float u1, float u2, fl
 NR_D float sky_cdf_pdf(const GSky& sk, f3 d) {
     const int H = sk.H, W = sk.W;
     f3 dir = mv(sk.rot, normalize(d));
-    float u = nrm::atan2f_(dir.z, dir.x) / NR_PI2 + 0.5f;
-    float v = nrm::acosf_(fminf(fmaxf(dir.y, -1.0f), 1.0f)) / NR_PI;
+    float u = Tm<true>::atan2(dir.z, dir.x) / NR_PI2 + 0.5f;
+    float v = Tm<true>::acos(fminf(fmaxf(dir.y, -1.0f), 1.0f)) / NR_PI;
     int i = min(max((int)(u * (float)W), 0), W - 1);
     int j = min(max((int)(v * (float)H), 0), H - 1);
     const float pt = reinterpret_cast<const float4*>(sk.alias)[(size_t)j * W + i].z;
     const float p_uv = (sk.marg[j + 1] - sk.marg[j]) * pt * (float)(W * H);
-    const float st = nrm::sinf_(v * NR_PI);
+    const float st = Tm<true>::sin(v * NR_PI);
     return st > 1e-7f ? p_uv / (2.0f * NR_PI * NR_PI * st) : 0.0f;
 }
 
