@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define NR_ABI_VERSION 5
+#define NR_ABI_VERSION 6
 
 enum nr_status {
     NR_OK = 0,
@@ -424,6 +424,23 @@ int nr_mesh_tree(nr_ctx* ctx, int mesh, void* nodes64, int32_t* slot_orig, int32
  * re-uploaded; mesh trees, materials, textures and the sky stay on the device. */
 int nr_object_set_transform(nr_ctx* ctx, int32_t index, const float position[3], const float rotation[3],
                             const float scale[3]);
+
+/* SceneInterface::add_object (csrc/scene/include/interface.h:112-116) -> Scene::add_hittable
+ * (csrc/scene/src/scene.cpp:101-104) and the rebuild it requests, as a partial update: the
+ * object is registered last (object_id = the previous object count, registry.h:113-126).
+ * `add` (may be NULL) lists the images, materials and meshes the object brings, appended to the
+ * uploaded tables (its n_objects must be 0, its sky is ignored): indices continue after the
+ * uploaded entries, and obj->material, obj->mesh and the new materials' texture images index the
+ * combined tables.  Only a new mesh gets a tree; the top-level tree and the object table are
+ * rebuilt (the registry's build_bvh, registry.h:216-234); uploaded meshes, images, materials and
+ * the sky stay on the device.  Hits and renders equal those of nr_scene_upload of the combined
+ * scene.  Invalid input leaves the scene as it was; a device failure leaves no scene. */
+int nr_scene_add_object(nr_ctx* ctx, const nr_object_desc* obj, const nr_scene_desc* add);
+
+/* SceneInterface::swap_skylight / request_skylight_update (interface.h:137-151 ->
+ * Skylight::build in Scene::update, scene.cpp:126-127): replaces the sky (and, for an HDRI,
+ * its interleaved copy and fast mode's sampling tables) and nothing else. */
+int nr_scene_set_sky(nr_ctx* ctx, const nr_sky_desc* sky);
 
 /* Wait for all work of the context.  NR_ERR_LIMIT if the last nr_render* call's
  * traversal overflowed its stack (the flag covers that call only; nr_aov_read

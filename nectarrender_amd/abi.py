@@ -239,31 +239,28 @@ class SceneDesc:
         self.sky = dict(type=SKY_HDRI, data=a, rotation=float(rotation), intensity=float(intensity))
 
     # -- lowering ---------------------------------------------------------
-    def to_c(self) -> NrSceneDesc:
-        keep = []
-        n = len(self.objects)
-        objs = (NrObjectDesc * max(n, 1))()
-        for i, o in enumerate(self.objects):
-            objs[i] = NrObjectDesc(o["type"], o["bound_type"], o["material"], o["mesh"], F3(*o["position"]),
-                                   F3(*o["rotation"]), F3(*o["scale"]), o["radius"], o["density"])
-        mats = (NrMaterialDesc * len(self.materials))()
-        for i, m in enumerate(self.materials):
-            md = NrMaterialDesc()
-            md.type = m.type
-            for k in range(5):
-                t = m.tex[k] if k < len(m.tex) else Texture()
-                md.tex[k] = NrTextureDesc(int(t.image), F3(*t.color), float(t.scale))
-            md.tint = F3(*m.tint)
-            md.ior = m.ior
-            md.brightness = m.brightness
-            mats[i] = md
-        imgs = (NrImageDesc * max(len(self.images), 1))()
-        for i, a in enumerate(self.images):
-            imgs[i] = NrImageDesc(a.shape[0], a.shape[1], a.shape[2], a.ctypes.data_as(C.POINTER(C.c_uint8)))
-        meshes = (NrMeshDesc * max(len(self.meshes), 1))()
-        for i, (v, t) in enumerate(self.meshes):
-            meshes[i] = NrMeshDesc(v.shape[0], t.shape[0], v.ctypes.data_as(C.POINTER(C.c_float)),
-                                   t.ctypes.data_as(C.POINTER(C.c_uint32)))
+    def counts(self):
+        """(objects, images, materials, meshes) registered so far."""
+        return len(self.objects), len(self.images), len(self.materials), len(self.meshes)
+
+    def object_c(self, i: int) -> NrObjectDesc:
+        o = self.objects[i]
+        return NrObjectDesc(o["type"], o["bound_type"], o["material"], o["mesh"], F3(*o["position"]),
+                            F3(*o["rotation"]), F3(*o["scale"]), o["radius"], o["density"])
+
+    @staticmethod
+    def _material_c(m) -> NrMaterialDesc:
+        md = NrMaterialDesc()
+        md.type = m.type
+        for k in range(5):
+            t = m.tex[k] if k < len(m.tex) else Texture()
+            md.tex[k] = NrTextureDesc(int(t.image), F3(*t.color), float(t.scale))
+        md.tint = F3(*m.tint)
+        md.ior = m.ior
+        md.brightness = m.brightness
+        return md
+
+    def sky_c(self) -> NrSkyDesc:
         sky = NrSkyDesc()
         sky.type = self.sky["type"]
         sky.intensity = 1.0
@@ -276,10 +273,47 @@ class SceneDesc:
             sky.intensity = self.sky["intensity"]
             sky.C, sky.H, sky.W = a.shape
             sky.data = a.ctypes.data_as(C.POINTER(C.c_float))
-        d = NrSceneDesc(n, objs, len(self.materials), mats, len(self.images), imgs, len(self.meshes), meshes, sky)
-        keep += [objs, mats, imgs, meshes]
-        self._keep = keep
+        return sky
+
+    def tail_c(self, img0: int, mat0: int, mesh0: int, mat1: int = None, mesh1: int = None) -> NrSceneDesc:
+        """The images, materials [mat0, mat1) and meshes [mesh0, mesh1) registered after an upload,
+        as the additions of nr_scene_add_object (no objects); images from img0 on, as far as
+        the materials up to mat1 use them."""
+        mat1 = len(self.materials) if mat1 is None else mat1
+        mesh1 = len(self.meshes) if mesh1 is None else mesh1
+        img1 = img0
+        for m in self.materials[mat0:mat1]:
+            for t in m.tex:
+                img1 = max(img1, int(t.image) + 1)
+        return self._lower([], self.materials[mat0:mat1], self.images[img0:img1], self.meshes[mesh0:mesh1],
+                           None)
+
+    def _lower(self, objects, materials, images, meshes, sky) -> NrSceneDesc:
+        keep = []
+        n = len(objects)
+        objs = (NrObjectDesc * max(n, 1))()
+        for i in range(n):
+            objs[i] = self.object_c(objects[i])
+        mats = (NrMaterialDesc * max(len(materials), 1))()
+        for i, m in enumerate(materials):
+            mats[i] = self._material_c(m)
+        imgs = (NrImageDesc * max(len(images), 1))()
+        for i, a in enumerate(images):
+            imgs[i] = NrImageDesc(a.shape[0], a.shape[1], a.shape[2], a.ctypes.data_as(C.POINTER(C.c_uint8)))
+        mesh_arr = (NrMeshDesc * max(len(meshes), 1))()
+        for i, (v, t) in enumerate(meshes):
+            mesh_arr[i] = NrMeshDesc(v.shape[0], t.shape[0], v.ctypes.data_as(C.POINTER(C.c_float)),
+                                     t.ctypes.data_as(C.POINTER(C.c_uint32)))
+        d = NrSceneDesc(n, objs, len(materials), mats, len(images), imgs, len(meshes), mesh_arr,
+                        sky if sky is not None else NrSkyDesc())
+        keep += [objs, mats, imgs, mesh_arr]
+        self._keep.append(keep)
         return d
+
+    def to_c(self) -> NrSceneDesc:
+        self._keep = []
+        return self._lower(list(range(len(self.objects))), self.materials, self.images, self.meshes,
+                           self.sky_c())
 
 
 def camera_desc(resolution=(512, 512), position=(0, 0, 0), rotation=(0, 0, 0), spp=16, focal_length=2.0,

@@ -305,54 +305,62 @@ void lower_material(Lowered& L, const Material& m) {
     L.mats.push_back(d);
 }
 
+// One registered object: its own material (material registry, registry.h:29-43),
+// textures and mesh geometry not yet lowered are appended to L's tables.
+void lower_object(Lowered& L, const HitP& h) {
+    nr_object_desc o{};
+    o.type = h->type;
+    o.bound_type = h->bound_type;
+    lower_material(L, h->mat ? *h->mat : default_material());
+    o.material = (int32_t)L.mats.size() - 1;
+    o.mesh = -1;
+    if (h->mesh) {
+        int idx = -1;
+        for (size_t i = 0; i < L.keep_mesh.size(); i++)
+            if (L.keep_mesh[i] == h->mesh) idx = (int)i;
+        if (idx < 0) {
+            L.keep_mesh.push_back(h->mesh);
+            nr_mesh_desc md{};
+            md.n_vertices = (int32_t)h->mesh->data.n_vertices();
+            md.n_triangles = (int32_t)h->mesh->data.n_triangles();
+            md.vertices = h->mesh->data.vertices.data();
+            md.triangles = h->mesh->data.triangles.data();
+            L.meshes.push_back(md);
+            idx = (int)L.meshes.size() - 1;
+        }
+        o.mesh = idx;
+    }
+    for (int k = 0; k < 3; k++) {
+        o.position[k] = h->xform.position.e[k];
+        o.rotation[k] = h->xform.rotation_deg.e[k];
+        o.scale[k] = h->xform.scale.e[k];
+    }
+    o.radius = h->radius;
+    o.density = h->density;
+    L.objs.push_back(o);
+}
+
+nr_sky_desc lower_sky(Lowered& L, const std::shared_ptr<Skylight>& sky) {
+    nr_sky_desc sd{};
+    sd.intensity = 1.0f;
+    if (!sky) return sd;
+    L.keep_sky = sky;
+    sd.type = (int32_t)sky->type;
+    for (int k = 0; k < 3; k++) { sd.start[k] = sky->cfg_simple.start.e[k]; sd.end[k] = sky->cfg_simple.end.e[k]; }
+    sd.rotation = sky->cfg_hdri.rotation;
+    sd.intensity = sky->cfg_hdri.intensity;
+    if (sky->type == SkylightType::HDRI && sky->cfg_hdri.data) {
+        sd.C = sky->cfg_hdri.C; sd.H = sky->cfg_hdri.H; sd.W = sky->cfg_hdri.W;
+        sd.data = sky->cfg_hdri.data->data();
+    }
+    return sd;
+}
+
 std::unique_ptr<Lowered> lower_scene(const Scene& s) {
     auto L = std::make_unique<Lowered>();
     lower_material(*L, default_material());                   // slot 0 (material/registry.h:23-26)
-    for (const HitP& h : s.objects) {
-        nr_object_desc o{};
-        o.type = h->type;
-        o.bound_type = h->bound_type;
-        lower_material(*L, h->mat ? *h->mat : default_material());
-        o.material = (int32_t)L->mats.size() - 1;
-        o.mesh = -1;
-        if (h->mesh) {
-            int idx = -1;
-            for (size_t i = 0; i < L->keep_mesh.size(); i++)
-                if (L->keep_mesh[i] == h->mesh) idx = (int)i;
-            if (idx < 0) {
-                L->keep_mesh.push_back(h->mesh);
-                nr_mesh_desc md{};
-                md.n_vertices = (int32_t)h->mesh->data.n_vertices();
-                md.n_triangles = (int32_t)h->mesh->data.n_triangles();
-                md.vertices = h->mesh->data.vertices.data();
-                md.triangles = h->mesh->data.triangles.data();
-                L->meshes.push_back(md);
-                idx = (int)L->meshes.size() - 1;
-            }
-            o.mesh = idx;
-        }
-        for (int k = 0; k < 3; k++) {
-            o.position[k] = h->xform.position.e[k];
-            o.rotation[k] = h->xform.rotation_deg.e[k];
-            o.scale[k] = h->xform.scale.e[k];
-        }
-        o.radius = h->radius;
-        o.density = h->density;
-        L->objs.push_back(o);
-    }
-    nr_sky_desc sd{};
-    sd.intensity = 1.0f;
-    if (s.sky) {
-        L->keep_sky = s.sky;
-        sd.type = (int32_t)s.sky->type;
-        for (int k = 0; k < 3; k++) { sd.start[k] = s.sky->cfg_simple.start.e[k]; sd.end[k] = s.sky->cfg_simple.end.e[k]; }
-        sd.rotation = s.sky->cfg_hdri.rotation;
-        sd.intensity = s.sky->cfg_hdri.intensity;
-        if (s.sky->type == SkylightType::HDRI && s.sky->cfg_hdri.data) {
-            sd.C = s.sky->cfg_hdri.C; sd.H = s.sky->cfg_hdri.H; sd.W = s.sky->cfg_hdri.W;
-            sd.data = s.sky->cfg_hdri.data->data();
-        }
-    }
+    for (const HitP& h : s.objects) lower_object(*L, h);
+    nr_sky_desc sd = lower_sky(*L, s.sky);
     L->desc.n_objects = (int32_t)L->objs.size();
     L->desc.objects = L->objs.data();
     L->desc.n_materials = (int32_t)L->mats.size();
@@ -756,6 +764,7 @@ struct RenderEngine {
     uint32_t pattern_spp;        // size of the sample pattern built at construction (camera.h:296-305)
     uint32_t sample_idx = 1;
     std::shared_ptr<Scene> scene;
+    std::unique_ptr<Lowered> lowered;   // the descriptors of what is on the device (tables grow with add_object)
     bool has_scene = false;
     std::shared_ptr<RenderLayers> layers;
     std::shared_ptr<TransferStream> stream;
@@ -804,15 +813,44 @@ struct RenderEngine {
         std::unique_lock<std::shared_mutex> lay(ctx->layout_mu);
         check(nr_scene_upload(ctx->h, &L->desc));
         scene = std::make_shared<Scene>(s);
+        lowered = std::move(L);
         has_scene = true;
     }
 
-    // re-upload the engine's own scene after a live edit
-    void upload_scene() {
-        auto L = lower_scene(*scene);
+    // Registers h last (Scene::add_hittable, scene.cpp:101-104) and uploads it with
+    // what it brings (its material, new textures, a new mesh) as a partial update.
+    void add_object(const HitP& h) {
         std::lock_guard<std::mutex> g(ctx->mu);
         std::unique_lock<std::shared_mutex> lay(ctx->layout_mu);
-        check(nr_scene_upload(ctx->h, &L->desc));
+        Lowered& L = *lowered;
+        const size_t i0 = L.imgs.size(), m0 = L.mats.size(), me0 = L.meshes.size();
+        lower_object(L, h);
+        nr_scene_desc add{};
+        add.n_materials = (int32_t)(L.mats.size() - m0);
+        add.materials = L.mats.data() + m0;
+        add.n_images = (int32_t)(L.imgs.size() - i0);
+        add.images = L.imgs.data() + i0;
+        add.n_meshes = (int32_t)(L.meshes.size() - me0);
+        add.meshes = L.meshes.data() + me0;
+        const int rc = nr_scene_add_object(ctx->h, &L.objs.back(), &add);
+        if (rc != NR_OK) {                      // keep the lowered tables equal to the device's
+            L.objs.pop_back();
+            L.mats.resize(m0);
+            L.imgs.resize(i0);
+            L.keep_tex.resize(i0);
+            L.meshes.resize(me0);
+            L.keep_mesh.resize(me0);
+            check(rc);
+        }
+        scene->objects.push_back(h);
+    }
+
+    // uploads the engine's sky alone (Skylight::build in Scene::update, scene.cpp:126-127)
+    void upload_sky() {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        std::unique_lock<std::shared_mutex> lay(ctx->layout_mu);
+        nr_sky_desc sd = lower_sky(*lowered, scene->sky);
+        check(nr_scene_set_sky(ctx->h, &sd));
     }
 
     void push_camera() {
@@ -1054,15 +1092,13 @@ void SceneInterface::set_transform(const Transform& x) {
     e.requests.restart();
 }
 
-void SceneInterface::add_object(const HitP& h) {                             // interface.h:100-105
+void SceneInterface::add_object(const HitP& h) {                             // interface.h:112-116
     RenderEngine& e = *eng;
     if (!e.has_scene) throw std::runtime_error("add_object() called before set_scene()");
     {
         std::lock_guard<std::mutex> g(mu);
-        e.scene->objects.insert(e.scene->objects.begin() + (long)e.scene->n_hittables, h);
-        e.scene->n_hittables++;
+        e.add_object(h);
     }
-    e.upload_scene();
     e.requests.restart();
 }
 
@@ -1078,13 +1114,13 @@ void SceneInterface::swap_skylight(const Skylight& sky) {                     //
         if (!eng->has_scene) throw std::runtime_error("no scene");
         eng->scene->sky = std::make_shared<Skylight>(sky);
     }
-    eng->upload_scene();
+    eng->upload_sky();
     eng->requests.restart();
 }
 
 void SceneInterface::request_skylight_update() {                             // interface.h:140-143
     if (!eng->has_scene) throw std::runtime_error("no scene");
-    eng->upload_scene();
+    eng->upload_sky();
     eng->requests.restart();
 }
 

@@ -102,7 +102,10 @@ constexpr int LDS_STACK_LEVELS = NR_LDS_LEVELS;   // traversal stack levels kept
 #ifndef NR_TRAV_OCC
 #define NR_TRAV_OCC 6
 #endif
-// waves per SIMD of k_extend for quad / sphere / mesh scenes
+// waves per SIMD of k_extend for quad / sphere / mesh scenes: at 7 (72 VGPRs) the
+// traversal loop keeps its state in registers; built for 8 it spilled 120 bytes
+// per lane inside the loop and ran slower on the 1080p workload.  The
+// every-object-type kernels keep the budget of NR_TRAV_OCC.
 #ifndef NR_EXT_WAVES
 #define NR_EXT_WAVES 7
 #endif
@@ -827,9 +830,6 @@ __global__ void k_counts(int32_t* counts, int32_t in) {
     if (t == 3) counts[CNT_PEND] = 0;
 }
 
-// 8 waves per SIMD for the quad / sphere / mesh kernels (64 VGPRs, 20 bytes of
-// scratch per lane: 2% faster than 71 VGPRs at 7 waves on the 1080p workload);
-// the every-object-type kernels keep the 80-VGPR budget of NR_TRAV_OCC
 // One extension segment of a path (trace.cu:19-22 and the closest hit of
 // trace_ray): a miss adds the sky and ends the path, a hit stores its record.
 // Returns the material type of the hit, 0 on a miss.
@@ -1423,8 +1423,11 @@ struct nr_ctx {
     int obj_types = NR_TY_ALL;      // object types of the scene (NR_TY_* bits): picks the traversal kernel
     int build_mode = NR_BUILD_AUTO;
     // host copies kept for introspection and live edits
+    std::vector<GImage> h_imgs;
+    std::vector<GMaterial> h_mats;
     std::vector<GMesh> h_meshes;
     std::vector<int32_t> h_mesh_nodes;      // child-pair nodes per mesh
+    std::vector<nrb::Box> h_mesh_local;     // object-space box per mesh
     std::vector<GObject> reg_objects;       // registration order
     std::vector<nrb::Box> reg_local;        // local (object-space) boxes
     int max_mesh_depth = 0;
@@ -1661,6 +1664,299 @@ static void launch_shade(nr_ctx* c, dim3 gS, hipStream_t st, const PathPtrs& P, 
     if (c->mat_present[NR_MAT_ISOTROPIC]) hipLaunchKernelGGL((k_shade<NR_MAT_ISOTROPIC, FAST>), gS, dim3(256), 0, st, c->d_scene, P, A);
 }
 
+// ---------------------------------------------------------------------------
+// Scene tables.  nr_scene_upload assembles them all; the live edits
+// (nr_object_set_transform, nr_scene_add_object, nr_scene_set_sky) rebuild
+// only the tables an edit changes and re-publish the GScene record.
+// ---------------------------------------------------------------------------
+
+// Frees one allocation of the scene's list: a table that an edit replaced.
+static void scene_release(nr_ctx* c, const void* p) {
+    if (!p) return;
+    auto it = std::find(c->scene_mem.begin(), c->scene_mem.end(), const_cast<void*>(p));
+    if (it == c->scene_mem.end()) return;
+    (void)hipFree(*it);
+    c->scene_mem.erase(it);
+}
+
+static int check_image(const nr_image_desc& im) {
+    if (im.C < 3 || im.H <= 0 || im.W <= 0 || !im.data) return fail(NR_ERR_INVALID, "image needs C >= 3 and data");
+    return NR_OK;
+}
+
+static int check_material(const nr_material_desc& md, int n_images) {
+    if (md.type < 1 || md.type > 5) return fail(NR_ERR_INVALID, "bad material type");
+    for (int k = 0; k < 5; k++)
+        if (md.tex[k].image >= n_images) return fail(NR_ERR_INVALID, "texture image index out of range");
+    return NR_OK;
+}
+
+static GMaterial make_material(const nr_material_desc& md) {
+    GMaterial g;
+    std::memset(&g, 0, sizeof(g));
+    g.type = md.type;
+    for (int k = 0; k < 5; k++) {
+        g.tex[k].image = md.tex[k].image;
+        for (int a = 0; a < 3; a++) g.tex[k].color[a] = md.tex[k].color[a];
+        g.tex[k].scale = md.tex[k].scale;
+    }
+    for (int a = 0; a < 3; a++) g.tint[a] = md.tint[a];
+    g.ior = md.ior;
+    g.brightness = md.brightness;
+    return g;
+}
+
+static int check_mesh(const nr_mesh_desc& md) {
+    if (md.n_vertices < 0 || md.n_triangles < 0) return fail(NR_ERR_INVALID, "bad mesh sizes");
+    if ((md.n_vertices > 0 && !md.vertices) || (md.n_triangles > 0 && !md.triangles))
+        return fail(NR_ERR_INVALID, "mesh arrays missing");
+    for (int64_t k = 0; k < 3 * (int64_t)md.n_triangles; k++)
+        if (md.triangles[k] >= (uint32_t)md.n_vertices) return fail(NR_ERR_INVALID, "triangle index out of range");
+    return NR_OK;
+}
+
+// One mesh on the device: its tree (device builder for large meshes, else the
+// host binned-SAH build, the same node array either way), triangles in leaf
+// order, vertex attributes, and the 4-wide and quantised 4-wide node layouts.
+// `local` receives the mesh's object-space box (mesh.h:218-237).
+static int build_mesh(nr_ctx* c, const nr_mesh_desc& md, GMesh& gm, int32_t& n_nodes, nrb::Box& local) {
+    const bool on_device = md.n_triangles > 1 &&
+                           (c->build_mode == NR_BUILD_DEVICE ||
+                            (c->build_mode == NR_BUILD_AUTO && md.n_triangles >= 4096));
+    gm = GMesh{};
+    n_nodes = 0;
+    GNode* dn = nullptr;
+    GTri* dt = nullptr;
+    uint32_t* di = nullptr;
+    bool built = false;
+    if (on_device) {
+        int e = build_mesh_on_device(c, md, gm, dn, dt, di, n_nodes);
+        if (e == NR_OK) built = true;
+        else if (e != NR_ERR_LIMIT) return e;        // a degenerate cluster the device path refuses: host build
+    }
+    if (!built) {
+        std::vector<nrb::Box> tb(md.n_triangles);
+        for (int t = 0; t < md.n_triangles; t++) {                             // mesh.h:320-336
+            const float* a = md.vertices + 11 * (size_t)md.triangles[3 * t];
+            const float* b = md.vertices + 11 * (size_t)md.triangles[3 * t + 1];
+            const float* cc = md.vertices + 11 * (size_t)md.triangles[3 * t + 2];
+            float mn[3], mx[3];
+            for (int k = 0; k < 3; k++) {
+                mn[k] = fminf(a[k], fminf(b[k], cc[k]));
+                mx[k] = fmaxf(a[k], fmaxf(b[k], cc[k]));
+            }
+            tb[t] = nrb::box_points(mn, mx);
+        }
+        nrb::Tree T;
+        nrb::Builder().build(tb, T);
+        std::vector<GNode> nodes;
+        nrb::to_pairs(T, nodes, gm.root_ref, gm.root_min, gm.root_max);
+        gm.depth = T.depth;
+        std::vector<GTri> tris(md.n_triangles);
+        std::vector<uint32_t> idx(3 * (size_t)md.n_triangles);
+        for (int s = 0; s < md.n_triangles; s++) {
+            int src = T.order[s];
+            GTri& g = tris[s];
+            for (int k = 0; k < 3; k++) {
+                uint32_t vi = md.triangles[3 * src + k];
+                idx[3 * (size_t)s + k] = vi;
+                float* dst = k == 0 ? g.v0 : (k == 1 ? g.v1 : g.v2);
+                for (int a = 0; a < 3; a++) dst[a] = md.vertices[11 * (size_t)vi + a];
+            }
+            g.orig = src;
+            g.pad0 = g.pad1 = 0;
+        }
+        int e = upload(c, nodes.data(), nodes.size(), &dn);
+        if (!e) e = upload(c, tris.data(), tris.size(), &dt);
+        if (!e) e = upload(c, idx.data(), idx.size(), &di);
+        if (e) return e;
+        n_nodes = (int32_t)nodes.size();
+        HIPCK(hipStreamSynchronize(c->stream));          // the host arrays end with this scope
+    }
+    std::vector<GVertex> verts(md.n_vertices);
+    float mn[3] = {1e32f, 1e32f, 1e32f}, mx[3] = {-1e32f, -1e32f, -1e32f};
+    for (int v = 0; v < md.n_vertices; v++) {
+        const float* f = md.vertices + 11 * (size_t)v;
+        GVertex& g = verts[v];
+        for (int a = 0; a < 3; a++) { g.normal[a] = f[3 + a]; g.tangent[a] = f[6 + a]; }
+        g.uv[0] = f[9]; g.uv[1] = f[10];
+        g.pad[0] = g.pad[1] = g.pad[2] = g.pad[3] = 0.0f;
+        for (int a = 0; a < 3; a++) { mn[a] = fminf(mn[a], f[a]); mx[a] = fmaxf(mx[a], f[a]); }
+    }
+    local = md.n_vertices > 0 ? nrb::box_points(mn, mx) : nrb::box_none();     // mesh.h:218-237
+    GVertex* dv;
+    if (int e = upload(c, verts.data(), verts.size(), &dv)) return e;
+    HIPCK(hipStreamSynchronize(c->stream));
+    gm.nodes = dn; gm.tris = dt; gm.idx = di; gm.verts = dv;
+    gm.wnodes = nullptr;
+    gm.qnodes = nullptr;
+    if (n_nodes > 0) {
+        GWNode* dw = nullptr;
+        HIPCK(hipMalloc((void**)&dw, sizeof(GWNode) * (size_t)n_nodes));
+        c->scene_mem.push_back(dw);
+        hipLaunchKernelGGL(k_wide_nodes, dim3((n_nodes + 255) / 256), dim3(256), 0, c->stream, dn, n_nodes, dw);
+        HIPCK(hipGetLastError());
+        gm.wnodes = dw;
+        GQNode* dq = nullptr;
+        HIPCK(hipMalloc((void**)&dq, sizeof(GQNode) * (size_t)n_nodes));
+        c->scene_mem.push_back(dq);
+        hipLaunchKernelGGL(k_quant_nodes, dim3((n_nodes + 255) / 256), dim3(256), 0, c->stream, dw, n_nodes, dq);
+        HIPCK(hipGetLastError());
+        gm.qnodes = dq;
+    }
+    return NR_OK;
+}
+
+static int check_object(const nr_object_desc& od, int n_materials, int n_meshes) {
+    if (od.type < 1 || od.type > 6) return fail(NR_ERR_INVALID, "bad hittable type");
+    int geom = (od.type == NR_HIT_CONSTANT_MEDIUM || od.type == NR_HIT_OBJECT_LIGHT) ? od.bound_type : od.type;
+    if (geom < 1 || geom > 4) return fail(NR_ERR_INVALID, "bound object must be quad, sphere, cube or mesh");
+    if (geom == NR_HIT_MESH && (od.mesh < 0 || od.mesh >= n_meshes)) return fail(NR_ERR_INVALID, "mesh index out of range");
+    if (od.material < 0 || od.material >= n_materials) return fail(NR_ERR_INVALID, "material index out of range");
+    return NR_OK;
+}
+
+// The record of the object registered as `id` (Hittable + its registry entry)
+// and its object-space box `lb`; registers its material type and object type
+// with the context's kernel selection.
+static GObject make_object(nr_ctx* c, const nr_object_desc& od, int32_t id, nrb::Box& lb) {
+    const int geom = (od.type == NR_HIT_CONSTANT_MEDIUM || od.type == NR_HIT_OBJECT_LIGHT) ? od.bound_type : od.type;
+    GObject o;
+    std::memset(&o, 0, sizeof(o));
+    o.xf = host::xform(od.position, od.rotation, od.scale, geom == NR_HIT_SPHERE);
+    o.type = od.type;
+    o.bound_type = od.bound_type;
+    o.material = od.material;
+    o.object_id = id;
+    o.mesh = geom == NR_HIT_MESH ? od.mesh : -1;
+    o.radius = od.radius + 1e-8f;
+    o.neg_inv_density = -1.0f / (od.density + 1e-8f);
+    o.pad[0] = c->h_mats[od.material].type;
+    c->mat_present[c->h_mats[od.material].type] = true;
+    c->obj_types |= od.type == NR_HIT_QUAD ? NR_TY_QUAD : od.type == NR_HIT_SPHERE ? NR_TY_SPHERE
+                  : od.type == NR_HIT_MESH ? NR_TY_MESH : NR_TY_OTHER;
+    if (geom == NR_HIT_QUAD || geom == NR_HIT_CUBE) {
+        float a[3] = {-0.5f, -0.5f, -0.5f}, b[3] = {0.5f, 0.5f, 0.5f};
+        lb = nrb::box_points(a, b);
+    } else if (geom == NR_HIT_SPHERE) {
+        float a[3] = {-o.radius, -o.radius, -o.radius}, b[3] = {o.radius, o.radius, o.radius};
+        lb = nrb::box_points(a, b);
+    } else {
+        lb = c->h_mesh_local[od.mesh];
+    }
+    return o;
+}
+
+// Top level (the registry's build_bvh, registry.h:216-234): the binned-SAH tree
+// over the world boxes of the registered objects in registration order, and
+// the object records in the tree's item order.  in_place: the object count is
+// the one the tables were allocated for, so they are rewritten where they are.
+static int upload_top(nr_ctx* c, GScene& S, bool in_place) {
+    const int32_t n = (int32_t)c->reg_objects.size();
+    std::vector<nrb::Box> wb(n);
+    for (int32_t i = 0; i < n; i++) wb[i] = host::transformed(c->reg_local[i], c->reg_objects[i].xf);
+    nrb::Tree T;
+    nrb::Builder().build(wb, T);
+    std::vector<GNode> tnodes;
+    nrb::to_pairs(T, tnodes, S.root_ref, S.root_min, S.root_max);
+    std::vector<GObject> objs(n);
+    S.first_light = -1;
+    for (int32_t k = 0; k < n; k++) {
+        objs[k] = c->reg_objects[T.order[k]];
+        if (S.first_light < 0 && objs[k].type == NR_HIT_OBJECT_LIGHT) S.first_light = k;   // registry.h:240-244
+    }
+    S.n_objects = n;
+    S.stack_depth = T.depth + mesh_stack_levels(c->max_mesh_depth) + 4;
+    if (in_place) {
+        if (!tnodes.empty())
+            HIPCK(hipMemcpyAsync((void*)S.nodes, tnodes.data(), sizeof(GNode) * tnodes.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        if (n > 0)
+            HIPCK(hipMemcpyAsync((void*)S.objects, objs.data(), sizeof(GObject) * objs.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+    } else {
+        scene_release(c, S.nodes);
+        scene_release(c, S.objects);
+        GNode* dn = nullptr;
+        GObject* dob = nullptr;
+        int e = upload(c, tnodes.data(), tnodes.size(), &dn);
+        if (!e) e = upload(c, objs.data(), objs.size(), &dob);
+        S.nodes = dn;
+        S.objects = dob;
+        if (e) {
+            (void)hipStreamSynchronize(c->stream);    // copies in flight read the host arrays
+            return e;
+        }
+    }
+    HIPCK(hipStreamSynchronize(c->stream));
+    return NR_OK;
+}
+
+static int check_sky(const nr_sky_desc& sky) {
+    if (sky.type < NR_SKY_NULL || sky.type > NR_SKY_HDRI) return fail(NR_ERR_INVALID, "bad sky type");
+    if (sky.type == NR_SKY_HDRI && sky.data && (sky.C < 3 || sky.H <= 0 || sky.W <= 0))
+        return fail(NR_ERR_INVALID, "HDRI needs C >= 3");
+    return NR_OK;
+}
+
+// The skylight (skylight.h:71-122) and, for an HDRI, the map, its interleaved
+// copy and the importance-sampling tables of fast mode.
+static int upload_sky(nr_ctx* c, const nr_sky_desc& sky, GSky& g) {
+    g = GSky{};
+    g.type = sky.type;
+    for (int a = 0; a < 3; a++) { g.start[a] = sky.start[a]; g.end[a] = sky.end[a]; }
+    {
+        host::M3 R = host::ry(host::deg2rad(sky.rotation));
+        std::memcpy(g.rot, R.m, sizeof(g.rot));
+    }
+    g.intensity = sky.intensity;
+    if (sky.type != NR_SKY_HDRI || !sky.data) return NR_OK;
+    g.H = sky.H; g.W = sky.W;
+    float* p;
+    // only the first three planes are read (skylight.h:106-114)
+    if (int e = upload(c, sky.data, (size_t)3 * sky.H * sky.W, &p)) return e;
+    g.data = p;
+    const size_t hw = (size_t)sky.H * sky.W;
+    std::vector<float> rgbx(4 * hw, 0.0f), marg, alias;
+    for (size_t k = 0; k < hw; k++)
+        for (int ch = 0; ch < 3; ch++) rgbx[4 * k + ch] = sky.data[ch * hw + k];
+    float* px = nullptr;
+    int e = upload(c, rgbx.data(), rgbx.size(), &px);
+    g.rgbx = px;
+    if (!e && sky_cdfs(sky.data, sky.H, sky.W, marg, alias)) {
+        float *pm = nullptr, *pa = nullptr;
+        e = upload(c, marg.data(), marg.size(), &pm);
+        if (!e) e = upload(c, alias.data(), alias.size(), &pa);
+        g.marg = pm;
+        g.alias = pa;
+    }
+    (void)hipStreamSynchronize(c->stream);               // the host tables end with this scope
+    return e;
+}
+
+// Writes the GScene record the kernels read (allocated by the first upload).
+static int publish_scene(nr_ctx* c, const GScene& S) {
+    if (!c->d_scene) {
+        GScene* ds;
+        if (int e = upload(c, &S, 1, &ds)) return e;
+        c->d_scene = ds;
+    } else {
+        HIPCK(hipMemcpyAsync(c->d_scene, &S, sizeof(GScene), hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCK(hipStreamSynchronize(c->stream));
+    c->scene = S;
+    return NR_OK;
+}
+
+template <typename T>
+static int replace_table(nr_ctx* c, const std::vector<T>& host_table, const T*& dev) {
+    scene_release(c, dev);
+    T* p = nullptr;
+    if (int e = upload(c, host_table.data(), host_table.size(), &p)) return e;
+    dev = p;
+    return NR_OK;
+}
+
 extern "C" {
 
 int nr_abi_version(void) { return NR_ABI_VERSION; }
@@ -1772,250 +2068,139 @@ void nr_destroy(nr_ctx* c) {
 int nr_scene_upload(nr_ctx* c, const nr_scene_desc* d) {
     if (!c || !d) return fail(NR_ERR_INVALID, "null context or scene");
     HIPCK(hipSetDevice(c->device));
+    if (d->n_materials <= 0) return fail(NR_ERR_INVALID, "scene needs at least the default material");
+    if (d->n_objects < 0 || d->n_images < 0 || d->n_meshes < 0) return fail(NR_ERR_INVALID, "negative table size");
+    for (int i = 0; i < d->n_images; i++)
+        if (int e = check_image(d->images[i])) return e;
+    for (int i = 0; i < d->n_materials; i++)
+        if (int e = check_material(d->materials[i], d->n_images)) return e;
+    for (int i = 0; i < d->n_meshes; i++)
+        if (int e = check_mesh(d->meshes[i])) return e;
+    for (int i = 0; i < d->n_objects; i++)
+        if (int e = check_object(d->objects[i], d->n_materials, d->n_meshes)) return e;
+    if (int e = check_sky(d->sky)) return e;
     HIPCK(hipStreamSynchronize(c->stream));
     free_list(c->scene_mem);
+    c->d_scene = nullptr;
     c->has_scene = false;
     for (auto& b : c->mat_present) b = false;
     c->obj_types = 0;
+    c->h_imgs.clear();
+    c->h_mats.clear();
+    c->h_meshes.clear();
+    c->h_mesh_nodes.clear();
+    c->h_mesh_local.clear();
+    c->reg_objects.clear();
+    c->reg_local.clear();
+    c->max_mesh_depth = 0;
     GScene S{};
-    // images
-    std::vector<GImage> imgs(d->n_images);
     for (int i = 0; i < d->n_images; i++) {
         const nr_image_desc& im = d->images[i];
-        if (im.C < 3 || im.H <= 0 || im.W <= 0 || !im.data) return fail(NR_ERR_INVALID, "image needs C >= 3 and data");
         uint8_t* p = nullptr;
-        int e = upload(c, im.data, (size_t)im.C * im.H * im.W, &p);
-        if (e) return e;
-        imgs[i] = GImage{p, im.C, im.H, im.W, 0};
+        if (int e = upload(c, im.data, (size_t)im.C * im.H * im.W, &p)) return e;
+        c->h_imgs.push_back(GImage{p, im.C, im.H, im.W, 0});
     }
-    // materials
-    if (d->n_materials <= 0) return fail(NR_ERR_INVALID, "scene needs at least the default material");
-    std::vector<GMaterial> mats(d->n_materials);
-    for (int i = 0; i < d->n_materials; i++) {
-        const nr_material_desc& md = d->materials[i];
-        if (md.type < 1 || md.type > 5) return fail(NR_ERR_INVALID, "bad material type");
-        GMaterial& g = mats[i];
-        std::memset(&g, 0, sizeof(g));
-        g.type = md.type;
-        for (int k = 0; k < 5; k++) {
-            if (md.tex[k].image >= d->n_images) return fail(NR_ERR_INVALID, "texture image index out of range");
-            g.tex[k].image = md.tex[k].image;
-            for (int a = 0; a < 3; a++) g.tex[k].color[a] = md.tex[k].color[a];
-            g.tex[k].scale = md.tex[k].scale;
-        }
-        for (int a = 0; a < 3; a++) g.tint[a] = md.tint[a];
-        g.ior = md.ior;
-        g.brightness = md.brightness;
-    }
-    // meshes
-    std::vector<GMesh> meshes(d->n_meshes);
-    std::vector<int32_t> mesh_nodes(d->n_meshes, 0);
-    std::vector<nrb::Box> mesh_local(d->n_meshes);
-    int max_mesh_depth = 0;
+    for (int i = 0; i < d->n_materials; i++) c->h_mats.push_back(make_material(d->materials[i]));
     for (int mi = 0; mi < d->n_meshes; mi++) {
-        const nr_mesh_desc& md = d->meshes[mi];
-        if (md.n_vertices < 0 || md.n_triangles < 0) return fail(NR_ERR_INVALID, "bad mesh sizes");
-        for (int64_t k = 0; k < 3 * (int64_t)md.n_triangles; k++)
-            if (md.triangles[k] >= (uint32_t)md.n_vertices) return fail(NR_ERR_INVALID, "triangle index out of range");
-        const bool on_device = md.n_triangles > 1 &&
-                               (c->build_mode == NR_BUILD_DEVICE ||
-                                (c->build_mode == NR_BUILD_AUTO && md.n_triangles >= 4096));
-        GMesh gm{};
-        int32_t n_nodes = 0;
-        GNode* dn = nullptr;
-        GTri* dt = nullptr;
-        uint32_t* di = nullptr;
-        bool built = false;
-        if (on_device) {
-            int e = build_mesh_on_device(c, md, gm, dn, dt, di, n_nodes);
-            if (e == NR_OK) built = true;
-            else if (e != NR_ERR_LIMIT) return e;        // a degenerate cluster the device path refuses: host build
-        }
-        if (!built) {
-            std::vector<nrb::Box> tb(md.n_triangles);
-            for (int t = 0; t < md.n_triangles; t++) {                         // mesh.h:320-336
-                const float* a = md.vertices + 11 * (size_t)md.triangles[3 * t];
-                const float* b = md.vertices + 11 * (size_t)md.triangles[3 * t + 1];
-                const float* cc = md.vertices + 11 * (size_t)md.triangles[3 * t + 2];
-                float mn[3], mx[3];
-                for (int k = 0; k < 3; k++) {
-                    mn[k] = fminf(a[k], fminf(b[k], cc[k]));
-                    mx[k] = fmaxf(a[k], fmaxf(b[k], cc[k]));
-                }
-                tb[t] = nrb::box_points(mn, mx);
-            }
-            nrb::Tree T;
-            nrb::Builder().build(tb, T);
-            std::vector<GNode> nodes;
-            nrb::to_pairs(T, nodes, gm.root_ref, gm.root_min, gm.root_max);
-            gm.depth = T.depth;
-            std::vector<GTri> tris(md.n_triangles);
-            std::vector<uint32_t> idx(3 * (size_t)md.n_triangles);
-            for (int s = 0; s < md.n_triangles; s++) {
-                int src = T.order[s];
-                GTri& g = tris[s];
-                for (int k = 0; k < 3; k++) {
-                    uint32_t vi = md.triangles[3 * src + k];
-                    idx[3 * (size_t)s + k] = vi;
-                    float* dst = k == 0 ? g.v0 : (k == 1 ? g.v1 : g.v2);
-                    for (int a = 0; a < 3; a++) dst[a] = md.vertices[11 * (size_t)vi + a];
-                }
-                g.orig = src;
-                g.pad0 = g.pad1 = 0;
-            }
-            int e = upload(c, nodes.data(), nodes.size(), &dn);
-            if (!e) e = upload(c, tris.data(), tris.size(), &dt);
-            if (!e) e = upload(c, idx.data(), idx.size(), &di);
-            if (e) return e;
-            n_nodes = (int32_t)nodes.size();
-        }
-        max_mesh_depth = std::max(max_mesh_depth, gm.depth);
-        std::vector<GVertex> verts(md.n_vertices);
-        float mn[3] = {1e32f, 1e32f, 1e32f}, mx[3] = {-1e32f, -1e32f, -1e32f};
-        for (int v = 0; v < md.n_vertices; v++) {
-            const float* f = md.vertices + 11 * (size_t)v;
-            GVertex& g = verts[v];
-            for (int a = 0; a < 3; a++) { g.normal[a] = f[3 + a]; g.tangent[a] = f[6 + a]; }
-            g.uv[0] = f[9]; g.uv[1] = f[10];
-            g.pad[0] = g.pad[1] = g.pad[2] = g.pad[3] = 0.0f;
-            for (int a = 0; a < 3; a++) { mn[a] = fminf(mn[a], f[a]); mx[a] = fmaxf(mx[a], f[a]); }
-        }
-        if (md.n_vertices > 0) mesh_local[mi] = nrb::box_points(mn, mx);        // mesh.h:218-237
-        else mesh_local[mi] = nrb::box_none();
-        GVertex* dv;
-        int e = upload(c, verts.data(), verts.size(), &dv);
-        if (e) return e;
-        gm.nodes = dn; gm.tris = dt; gm.idx = di; gm.verts = dv;
-        gm.wnodes = nullptr;
-        gm.qnodes = nullptr;
-        if (n_nodes > 0) {
-            GWNode* dw = nullptr;
-            HIPCK(hipMalloc((void**)&dw, sizeof(GWNode) * (size_t)n_nodes));
-            c->scene_mem.push_back(dw);
-            hipLaunchKernelGGL(k_wide_nodes, dim3((n_nodes + 255) / 256), dim3(256), 0, c->stream, dn, n_nodes, dw);
-            HIPCK(hipGetLastError());
-            gm.wnodes = dw;
-            GQNode* dq = nullptr;
-            HIPCK(hipMalloc((void**)&dq, sizeof(GQNode) * (size_t)n_nodes));
-            c->scene_mem.push_back(dq);
-            hipLaunchKernelGGL(k_quant_nodes, dim3((n_nodes + 255) / 256), dim3(256), 0, c->stream, dw, n_nodes, dq);
-            HIPCK(hipGetLastError());
-            gm.qnodes = dq;
-        }
-        meshes[mi] = gm;
-        mesh_nodes[mi] = n_nodes;
+        GMesh gm;
+        int32_t n_nodes;
+        nrb::Box local;
+        if (int e = build_mesh(c, d->meshes[mi], gm, n_nodes, local)) return e;
+        c->h_meshes.push_back(gm);
+        c->h_mesh_nodes.push_back(n_nodes);
+        c->h_mesh_local.push_back(local);
+        c->max_mesh_depth = std::max(c->max_mesh_depth, gm.depth);
     }
-    // objects: registration order -> world boxes -> top-level tree
-    int n = d->n_objects;
-    std::vector<GObject> reg(n);
-    std::vector<nrb::Box> wb(n), local(n);
-    for (int i = 0; i < n; i++) {
-        const nr_object_desc& od = d->objects[i];
-        if (od.type < 1 || od.type > 6) return fail(NR_ERR_INVALID, "bad hittable type");
-        int geom = (od.type == NR_HIT_CONSTANT_MEDIUM || od.type == NR_HIT_OBJECT_LIGHT) ? od.bound_type : od.type;
-        if (geom < 1 || geom > 4) return fail(NR_ERR_INVALID, "bound object must be quad, sphere, cube or mesh");
-        if (geom == NR_HIT_MESH && (od.mesh < 0 || od.mesh >= d->n_meshes)) return fail(NR_ERR_INVALID, "mesh index out of range");
-        if (od.material < 0 || od.material >= d->n_materials) return fail(NR_ERR_INVALID, "material index out of range");
-        GObject& o = reg[i];
-        std::memset(&o, 0, sizeof(o));
-        o.xf = host::xform(od.position, od.rotation, od.scale, geom == NR_HIT_SPHERE);
-        o.type = od.type;
-        o.bound_type = od.bound_type;
-        o.material = od.material;
-        o.object_id = i;
-        o.mesh = geom == NR_HIT_MESH ? od.mesh : -1;
-        o.radius = od.radius + 1e-8f;
-        o.neg_inv_density = -1.0f / (od.density + 1e-8f);
-        o.pad[0] = mats[od.material].type;
-        c->mat_present[mats[od.material].type] = true;
-        c->obj_types |= od.type == NR_HIT_QUAD ? NR_TY_QUAD : od.type == NR_HIT_SPHERE ? NR_TY_SPHERE
-                      : od.type == NR_HIT_MESH ? NR_TY_MESH : NR_TY_OTHER;
+    // objects in registration order (hittables then lights, scene.h:31-39)
+    for (int i = 0; i < d->n_objects; i++) {
         nrb::Box lb;
-        if (geom == NR_HIT_QUAD || geom == NR_HIT_CUBE) { float a[3] = {-0.5f, -0.5f, -0.5f}, b[3] = {0.5f, 0.5f, 0.5f}; lb = nrb::box_points(a, b); }
-        else if (geom == NR_HIT_SPHERE) { float a[3] = {-o.radius, -o.radius, -o.radius}, b[3] = {o.radius, o.radius, o.radius}; lb = nrb::box_points(a, b); }
-        else lb = mesh_local[od.mesh];
-        local[i] = lb;
-        wb[i] = host::transformed(lb, o.xf);
+        c->reg_objects.push_back(make_object(c, d->objects[i], i, lb));
+        c->reg_local.push_back(lb);
     }
-    nrb::Tree T;
-    nrb::Builder().build(wb, T);
-    std::vector<GNode> tnodes;
-    nrb::to_pairs(T, tnodes, S.root_ref, S.root_min, S.root_max);
-    std::vector<GObject> objs(n);
-    S.first_light = -1;
-    for (int s = 0; s < n; s++) {
-        objs[s] = reg[T.order[s]];
-        if (S.first_light < 0 && objs[s].type == NR_HIT_OBJECT_LIGHT) S.first_light = s;   // registry.h:240-244
-    }
-    S.n_objects = n;
-    S.stack_depth = T.depth + mesh_stack_levels(max_mesh_depth) + 4;
-    {
-        GNode* dn; GObject* dob; GMesh* dm; GMaterial* dmat; GImage* dimg;
-        int e = upload(c, tnodes.data(), tnodes.size(), &dn);
-        if (!e) e = upload(c, objs.data(), objs.size(), &dob);
-        if (!e) e = upload(c, meshes.data(), meshes.size(), &dm);
-        if (!e) e = upload(c, mats.data(), mats.size(), &dmat);
-        if (!e) e = upload(c, imgs.data(), imgs.size(), &dimg);
-        if (e) return e;
-        S.nodes = dn; S.objects = dob; S.meshes = dm; S.materials = dmat; S.images = dimg;
-    }
-    // sky (skylight.h)
-    S.sky.type = d->sky.type;
-    for (int a = 0; a < 3; a++) { S.sky.start[a] = d->sky.start[a]; S.sky.end[a] = d->sky.end[a]; }
-    {
-        host::M3 R = host::ry(host::deg2rad(d->sky.rotation));
-        std::memcpy(S.sky.rot, R.m, sizeof(S.sky.rot));
-    }
-    S.sky.intensity = d->sky.intensity;
-    S.sky.data = nullptr;
-    S.sky.marg = nullptr;
-    S.sky.alias = nullptr;
-    S.sky.rgbx = nullptr;
-    std::vector<float> sky_marg, sky_alias, sky_rgbx;     // alive until the stream sync below
-    if (d->sky.type == NR_SKY_HDRI && d->sky.data) {
-        if (d->sky.C < 3 || d->sky.H <= 0 || d->sky.W <= 0) return fail(NR_ERR_INVALID, "HDRI needs C >= 3");
-        S.sky.H = d->sky.H; S.sky.W = d->sky.W;
-        float* p;
-        // only the first three planes are read (skylight.h:106-114)
-        int e = upload(c, d->sky.data, (size_t)3 * d->sky.H * d->sky.W, &p);
-        if (e) return e;
-        S.sky.data = p;
-        const size_t hw = (size_t)d->sky.H * d->sky.W;
-        sky_rgbx.assign(4 * hw, 0.0f);
-        for (size_t k = 0; k < hw; k++)
-            for (int ch = 0; ch < 3; ch++) sky_rgbx[4 * k + ch] = d->sky.data[ch * hw + k];
-        float* px;
-        e = upload(c, sky_rgbx.data(), sky_rgbx.size(), &px);
-        if (e) return e;
-        S.sky.rgbx = px;
-        if (sky_cdfs(d->sky.data, d->sky.H, d->sky.W, sky_marg, sky_alias)) {
-            float *pm, *pa;
-            e = upload(c, sky_marg.data(), sky_marg.size(), &pm);
-            if (!e) e = upload(c, sky_alias.data(), sky_alias.size(), &pa);
-            if (e) return e;
-            S.sky.marg = pm;
-            S.sky.alias = pa;
-        }
-    }
+    if (int e = replace_table(c, c->h_imgs, S.images)) return e;
+    if (int e = replace_table(c, c->h_mats, S.materials)) return e;
+    if (int e = replace_table(c, c->h_meshes, S.meshes)) return e;
+    if (int e = upload_top(c, S, false)) return e;
+    if (int e = upload_sky(c, d->sky, S.sky)) return e;
     // cube face transforms (shapes.h:192-223)
     const float fp[6][3] = {{0, 0, 0.5f}, {0, 0, -0.5f}, {0.5f, 0, 0}, {-0.5f, 0, 0}, {0, 0.5f, 0}, {0, -0.5f, 0}};
     const float fr[6][3] = {{90, 0, 0}, {-90, 0, 0}, {0, 0, -90}, {0, 0, 90}, {0, 0, 0}, {180, 0, 0}};
     const float one[3] = {1.0f, 1.0f, 1.0f};
     for (int i = 0; i < 6; i++) S.cube_faces[i] = host::xform(fp[i], fr[i], one, false);
-    {
-        GScene* ds;
-        int e = upload(c, &S, 1, &ds);
-        if (e) return e;
-        c->d_scene = ds;
-    }
-    HIPCK(hipStreamSynchronize(c->stream));
-    c->scene = S;
+    if (int e = publish_scene(c, S)) return e;
     c->has_scene = true;
-    c->h_meshes = meshes;
-    c->h_mesh_nodes = mesh_nodes;
-    c->reg_objects = reg;
-    c->reg_local = local;
-    c->max_mesh_depth = max_mesh_depth;
+    return NR_OK;
+}
+
+int nr_scene_add_object(nr_ctx* c, const nr_object_desc* obj, const nr_scene_desc* add) {
+    if (!c || !obj) return fail(NR_ERR_INVALID, "null argument");
+    if (!c->has_scene) return fail(NR_ERR_STATE, "no scene");
+    const int ni = add ? add->n_images : 0, nm = add ? add->n_materials : 0, nme = add ? add->n_meshes : 0;
+    if (add && add->n_objects != 0) return fail(NR_ERR_INVALID, "the additions list no objects");
+    if (ni < 0 || nm < 0 || nme < 0) return fail(NR_ERR_INVALID, "negative table size");
+    const int n_img = (int)c->h_imgs.size() + ni, n_mat = (int)c->h_mats.size() + nm,
+              n_mesh = (int)c->h_meshes.size() + nme;
+    for (int i = 0; i < ni; i++)
+        if (int e = check_image(add->images[i])) return e;
+    for (int i = 0; i < nm; i++)
+        if (int e = check_material(add->materials[i], n_img)) return e;
+    for (int i = 0; i < nme; i++)
+        if (int e = check_mesh(add->meshes[i])) return e;
+    if (int e = check_object(*obj, n_mat, n_mesh)) return e;
+    HIPCK(hipSetDevice(c->device));
+    HIPCK(hipStreamSynchronize(c->stream));
+    // a device failure below leaves no usable scene (the next call must upload one)
+    c->has_scene = false;
+    GScene S = c->scene;
+    if (ni > 0) {
+        for (int i = 0; i < ni; i++) {
+            const nr_image_desc& im = add->images[i];
+            uint8_t* p = nullptr;
+            if (int e = upload(c, im.data, (size_t)im.C * im.H * im.W, &p)) return e;
+            c->h_imgs.push_back(GImage{p, im.C, im.H, im.W, 0});
+        }
+        if (int e = replace_table(c, c->h_imgs, S.images)) return e;
+    }
+    if (nm > 0) {
+        for (int i = 0; i < nm; i++) c->h_mats.push_back(make_material(add->materials[i]));
+        if (int e = replace_table(c, c->h_mats, S.materials)) return e;
+    }
+    if (nme > 0) {
+        for (int i = 0; i < nme; i++) {
+            GMesh gm;
+            int32_t n_nodes;
+            nrb::Box local;
+            if (int e = build_mesh(c, add->meshes[i], gm, n_nodes, local)) return e;
+            c->h_meshes.push_back(gm);
+            c->h_mesh_nodes.push_back(n_nodes);
+            c->h_mesh_local.push_back(local);
+            c->max_mesh_depth = std::max(c->max_mesh_depth, gm.depth);
+        }
+        if (int e = replace_table(c, c->h_meshes, S.meshes)) return e;
+    }
+    nrb::Box lb;
+    c->reg_objects.push_back(make_object(c, *obj, (int32_t)c->reg_objects.size(), lb));
+    c->reg_local.push_back(lb);
+    if (int e = upload_top(c, S, false)) return e;
+    if (int e = publish_scene(c, S)) return e;
+    c->has_scene = true;
+    return NR_OK;
+}
+
+int nr_scene_set_sky(nr_ctx* c, const nr_sky_desc* sky) {
+    if (!c || !sky) return fail(NR_ERR_INVALID, "null argument");
+    if (!c->has_scene) return fail(NR_ERR_STATE, "no scene");
+    if (int e = check_sky(*sky)) return e;
+    HIPCK(hipSetDevice(c->device));
+    HIPCK(hipStreamSynchronize(c->stream));
+    c->has_scene = false;
+    GScene S = c->scene;
+    const GSky old = S.sky;
+    if (int e = upload_sky(c, *sky, S.sky)) return e;
+    for (const void* p : {(const void*)old.data, (const void*)old.rgbx, (const void*)old.marg, (const void*)old.alias})
+        scene_release(c, p);
+    if (int e = publish_scene(c, S)) return e;
+    c->has_scene = true;
     return NR_OK;
 }
 
@@ -2994,27 +3179,11 @@ int nr_object_set_transform(nr_ctx* c, int32_t index, const float position[3], c
     GObject& o = c->reg_objects[index];
     const int geom = (o.type == NR_HIT_CONSTANT_MEDIUM || o.type == NR_HIT_OBJECT_LIGHT) ? o.bound_type : o.type;
     o.xf = host::xform(position, rotation, scale, geom == NR_HIT_SPHERE);
-    std::vector<nrb::Box> wb(n);
-    for (int32_t i = 0; i < n; i++) wb[i] = host::transformed(c->reg_local[i], c->reg_objects[i].xf);
-    nrb::Tree T;
-    nrb::Builder().build(wb, T);
-    std::vector<GNode> tnodes;
-    GScene S = c->scene;
-    nrb::to_pairs(T, tnodes, S.root_ref, S.root_min, S.root_max);
-    std::vector<GObject> objs(n);
-    S.first_light = -1;
-    for (int32_t k = 0; k < n; k++) {
-        objs[k] = c->reg_objects[T.order[k]];
-        if (S.first_light < 0 && objs[k].type == NR_HIT_OBJECT_LIGHT) S.first_light = k;   // registry.h:240-244
-    }
-    S.stack_depth = T.depth + mesh_stack_levels(c->max_mesh_depth) + 4;
-    // same sizes as the full upload: rewrite the tables in place once the queued work is done
+    // same object count as the tables hold: rewritten in place once the queued work is done
     HIPCK(hipStreamSynchronize(c->stream));
-    if (!tnodes.empty()) HIPCK(hipMemcpy((void*)S.nodes, tnodes.data(), sizeof(GNode) * tnodes.size(), hipMemcpyHostToDevice));
-    HIPCK(hipMemcpy((void*)S.objects, objs.data(), sizeof(GObject) * objs.size(), hipMemcpyHostToDevice));
-    HIPCK(hipMemcpy(c->d_scene, &S, sizeof(GScene), hipMemcpyHostToDevice));
-    c->scene = S;
-    return NR_OK;
+    GScene S = c->scene;
+    if (int e = upload_top(c, S, true)) return e;
+    return publish_scene(c, S);
 }
 
 int nr_sync(nr_ctx* c) {

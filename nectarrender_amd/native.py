@@ -70,12 +70,14 @@ def lib() -> C.CDLL:
                                    C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "nr_object_set_transform": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                               C.POINTER(C.c_float)]),
+        "nr_scene_add_object": (C.c_int, [vp, C.POINTER(abi.NrObjectDesc), C.POINTER(abi.NrSceneDesc)]),
+        "nr_scene_set_sky": (C.c_int, [vp, C.POINTER(abi.NrSkyDesc)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
-    if L.nr_abi_version() != 5:
+    if L.nr_abi_version() != 6:
         raise NectarError("ABI version mismatch")
     _lib = L
     return L
@@ -117,6 +119,27 @@ class Context:
         d = scene.to_c()
         self._scene_keep = scene
         _check(lib().nr_scene_upload(self._h, C.byref(d)))
+        self._uploaded = scene.counts()
+
+    def add_objects(self, scene: abi.SceneDesc):
+        """Uploads the objects registered on `scene` since the last upload (with the images,
+        materials and meshes they bring) by nr_scene_add_object, one partial update each."""
+        if getattr(self, "_scene_keep", None) is not scene:
+            raise NectarError("add_objects needs the scene of the last upload")
+        n_obj, n_img, n_mat, n_mesh = self._uploaded
+        for i in range(n_obj, len(scene.objects)):
+            o = scene.objects[i]
+            need_mat, need_mesh = o["material"] + 1, o["mesh"] + 1
+            add = scene.tail_c(n_img, n_mat, n_mesh, max(n_mat, need_mat), max(n_mesh, need_mesh))
+            obj = scene.object_c(i)
+            _check(lib().nr_scene_add_object(self._h, C.byref(obj), C.byref(add)))
+            n_img, n_mat, n_mesh = add.n_images + n_img, add.n_materials + n_mat, add.n_meshes + n_mesh
+            self._uploaded = (i + 1, n_img, n_mat, n_mesh)
+
+    def set_sky(self, scene: abi.SceneDesc):
+        """Uploads `scene.sky` alone (nr_scene_set_sky)."""
+        sky = scene.sky_c()
+        _check(lib().nr_scene_set_sky(self._h, C.byref(sky)))
 
     def set_camera(self, view: int, cam: abi.NrCameraDesc, seed: int):
         _check(lib().nr_camera_set(self._h, view, C.byref(cam), seed))

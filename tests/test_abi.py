@@ -26,7 +26,7 @@ def test_header_declares_entry_points():
               "nr_aov_read", "nr_post", "nr_last_error", "nr_view_size", "nr_unet_create", "nr_unet_forward",
               "nr_unet_train_step", "nr_unet_gradients", "nr_denoise", "nr_viewport", "nr_pick",
               "nr_selection_mask", "nr_overlay_device", "nr_readback", "nr_set_build_mode", "nr_set_trace_layout", "nr_mesh_tree", "nr_trace_closest_device_noalloc",
-              "nr_object_set_transform", "nr_view_mask_buffer"):
+              "nr_object_set_transform", "nr_view_mask_buffer", "nr_scene_add_object", "nr_scene_set_sky"):
         assert n in names
 
 
@@ -40,7 +40,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_loads_and_reports_version():
     L = C.CDLL(LIB)
-    assert L.nr_abi_version() == 5
+    assert L.nr_abi_version() == 6
     L.nr_last_error.restype = C.c_char_p
     assert isinstance(L.nr_last_error(), bytes)
 
@@ -74,6 +74,8 @@ def test_null_arguments_fail_cleanly_without_device():
     assert L.nr_set_render_streams(None, 2) == 1
     assert L.nr_mesh_tree(None, 0, None, None, None, None, None) == 1
     assert L.nr_object_set_transform(None, 0, None, None, None) == 1
+    assert L.nr_scene_add_object(None, None, None) == 1
+    assert L.nr_scene_set_sky(None, None) == 1
     from nectarrender_amd import denoise
     D = denoise._lib()
     assert D.nr_unet_param_count() == 1016131

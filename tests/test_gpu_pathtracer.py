@@ -243,6 +243,28 @@ def test_scene_interface_set_transform_rerenders(oracle):
     assert x is not None
 
 
+def test_scene_interface_add_object_and_swap_skylight_rerender(oracle):
+    """add_object registers the sphere last (after the light, registry.h:113-126) and uploads only it and
+    its material; swap_skylight uploads only the sky.  Each render equals the oracle's of the edited scene."""
+    e = _engine()
+    si = e.get_scene_interface()
+    si.add_object(Hit.sphere(V3(-0.15, 0.05, 0.25), 0.12, Mat.lambertian(Col(0.2, 0.4, 0.9))))
+    e.render()
+    s, cam, rp = scenes.cornell(spp=SPP, res=RES)
+    s.sphere((-0.15, 0.05, 0.25), 0.12, abi.Material.lambertian((0.2, 0.4, 0.9)))
+    desc = abi.render_desc(1, SPP, rp["depth"], rp["seed"])
+    ref = oracle.render(oracle.OracleScene(s), oracle.OracleCamera(cam, rp["seed"]), desc)
+    assert _same(e.layers().beauty.numpy_float(), ref[0])
+    nodes = si.get_scene_outline().nodes
+    assert len(nodes) == 9 and nodes[-1].type_name == "Sphere"
+    si.swap_skylight(P.light.Skylight.simple(Col(0.9, 0.8, 0.7), Col(0.2, 0.3, 0.5)))
+    e.set_n_samples(SPP)                                  # restarts the pass count (engine.cpp:129-134)
+    e.render()
+    s.sky_simple((0.9, 0.8, 0.7), (0.2, 0.3, 0.5))
+    ref = oracle.render(oracle.OracleScene(s), oracle.OracleCamera(cam, rp["seed"]), desc)
+    assert _same(e.layers().beauty.numpy_float(), ref[0])
+
+
 def test_default_readback_is_denoised():
     """StreamConfig.apply_denoising defaults to on (stream.h:14): the readback is the U-Net output."""
     e = _engine()
