@@ -1,13 +1,16 @@
-// nr_io.cpp — OBJ, PNG, PNM and Radiance HDR readers (see nr_io.h).
+// nr_io.cpp — OBJ, PNG, PNM and Radiance HDR readers; JPEG, BMP and TGA are in
+// nr_image.cpp (see nr_io.h).
 //
 // The formats are decoded from their published specifications (Wavefront OBJ,
 // PNG ISO/IEC 15948 with zlib inflate, Netpbm, Greg Ward's RGBE); the
 // conversions applied after decoding follow what the reference's loaders
 // return: tinyobjloader 2.x with triangulation (quads split along the shorter
-// diagonal) feeding parse_obj_file (mesh.h:39-141), and stb_image 2.30 for
+// diagonal, larger faces by its ear clipping) feeding parse_obj_file
+// (mesh.h:39-141), and stb_image 2.30 for
 // images (8-bit output, 16-bit samples reduced to their high byte, HDR texels
 // as mantissa * 2^(exponent-136)).
 #include "nr_io.h"
+#include "nr_image.h"
 
 #include <zlib.h>
 
@@ -15,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <sstream>
 #include <stdexcept>
 #include <unordered_map>
@@ -74,6 +78,82 @@ float parse_real(const char*& s) {
     char* end = nullptr;
     double d = std::strtod(tok.c_str(), &end);   // correctly rounded, then to float
     return (float)d;
+}
+
+// Five or more corners: the ear clipping tinyobjloader 2.x runs under
+// triangulate = true (its built-in method, all arithmetic in float).  The face
+// is projected onto two coordinate axes chosen from the first corner whose
+// edges are not parallel: dropping x when the x component of their cross
+// product is the largest, else using (x, z), or (x, y) when z is the largest.
+// Starting at corner 0, the triangle (g, g+1, g+2) is cut off when the sign of
+// its turn agrees with the sign of the projected term x_g y_{g+1} - y_g x_{g+1}
+// and no other remaining corner lies inside it (even-odd crossing test);
+// otherwise g advances.  The middle corner of a cut triangle is removed, and
+// the search stops after as many failed tries as corners remain (the rest of a
+// degenerate face is dropped).  The last three corners form the final triangle.
+bool inside_triangle(const float* x, const float* y, float tx, float ty) {
+    bool in = false;
+    for (int i = 0, j = 2; i < 3; j = i++)
+        if (((y[i] > ty) != (y[j] > ty)) && (tx < (x[j] - x[i]) * (ty - y[i]) / (y[j] - y[i]) + x[i])) in = !in;
+    return in;
+}
+
+void triangulate_ngon(const std::vector<Idx>& face, const std::vector<float>& pos, std::vector<Idx>& corners) {
+    const size_t n = face.size();
+    auto valid = [&](int v, size_t c) { return v >= 0 && (size_t)v * 3 + c < pos.size(); };
+    size_t ax0 = 1, ax1 = 2;
+    for (size_t k = 0; k < n; k++) {
+        const int a = face[k].v, b = face[(k + 1) % n].v, c = face[(k + 2) % n].v;
+        if (!valid(a, 2) || !valid(b, 2) || !valid(c, 2)) continue;
+        const float e0x = pos[3 * b] - pos[3 * a], e0y = pos[3 * b + 1] - pos[3 * a + 1],
+                    e0z = pos[3 * b + 2] - pos[3 * a + 2];
+        const float e1x = pos[3 * c] - pos[3 * b], e1y = pos[3 * c + 1] - pos[3 * b + 1],
+                    e1z = pos[3 * c + 2] - pos[3 * b + 2];
+        const float cx = std::fabs(e0y * e1z - e0z * e1y), cy = std::fabs(e0z * e1x - e0x * e1z),
+                    cz = std::fabs(e0x * e1y - e0y * e1x);
+        const float eps = std::numeric_limits<float>::epsilon();
+        if (cx > eps || cy > eps || cz > eps) {
+            if (!(cx > cy && cx > cz)) {
+                ax0 = 0;
+                if (cz > cx && cz > cy) ax1 = 1;
+            }
+            break;
+        }
+    }
+    std::vector<Idx> rem = face;
+    size_t g = 0, tries = n, last_count = n;
+    while (rem.size() > 3 && tries > 0) {
+        const size_t m = rem.size();
+        if (g >= m) g -= m;
+        if (last_count != m) {
+            last_count = m;
+            tries = m;
+        } else {
+            tries--;
+        }
+        Idx ind[3];
+        float x[3], y[3];
+        for (int k = 0; k < 3; k++) {
+            ind[k] = rem[(g + k) % m];
+            const int v = ind[k].v;
+            const bool ok = valid(v, ax0) && valid(v, ax1);
+            x[k] = ok ? pos[3 * (size_t)v + ax0] : 0.0f;
+            y[k] = ok ? pos[3 * (size_t)v + ax1] : 0.0f;
+        }
+        const float turn = (x[1] - x[0]) * (y[2] - y[1]) - (y[1] - y[0]) * (x[2] - x[1]);
+        const float term = (x[0] * y[1] - y[0] * x[1]) * 0.5f;
+        if (turn * term < 0.0f) { g++; continue; }
+        bool blocked = false;
+        for (size_t o = 3; o < m && !blocked; o++) {
+            const int v = rem[(g + o) % m].v;
+            if (!valid(v, ax0) || !valid(v, ax1)) continue;
+            blocked = inside_triangle(x, y, pos[3 * (size_t)v + ax0], pos[3 * (size_t)v + ax1]);
+        }
+        if (blocked) { g++; continue; }
+        corners.insert(corners.end(), {ind[0], ind[1], ind[2]});
+        rem.erase(rem.begin() + (long)((g + 1) % m));
+    }
+    if (rem.size() == 3) corners.insert(corners.end(), {rem[0], rem[1], rem[2]});
 }
 
 }  // namespace
@@ -154,9 +234,7 @@ MeshData parse_obj(const std::string& text, const std::string& name) {
                     corners.insert(corners.end(), {face[0], face[1], face[3], face[1], face[2], face[3]});
                 }
             } else {
-                // five or more corners: fan from the first corner
-                for (size_t k = 1; k + 1 < face.size(); k++)
-                    corners.insert(corners.end(), {face[0], face[k], face[k + 1]});
+                triangulate_ngon(face, pos, corners);
             }
         }
         // o, g, s, usemtl, mtllib, l, p: no geometry for the path tracer
@@ -253,8 +331,6 @@ MeshData load_obj(const std::string& path) {
 // ---------------------------------------------------------------------------
 namespace {
 
-// C = channel count the decoder reports, Cout = channels per pixel in hwc
-struct Raw8 { int C = 0, Cout = 0, H = 0, W = 0; std::vector<uint8_t> hwc; };
 struct RawF { int C = 0, H = 0, W = 0; std::vector<float> hwc; };
 
 uint32_t be32(const uint8_t* p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
@@ -265,9 +341,6 @@ int paeth(int a, int b, int c) {
     if (pb <= pc) return b;
     return c;
 }
-
-// Largest width or height any image reader accepts (stb_image's STBI_MAX_DIMENSIONS).
-constexpr uint32_t MAX_DIM = 1u << 24;
 
 // PNG (ISO/IEC 15948): every colour type / bit depth pair the specification
 // allows, Adam7 interlacing, tRNS transparency.  Pairs the specification
@@ -553,10 +626,16 @@ bool decode_hdr(const std::string& b, RawF& out, std::string& err) {
     return true;
 }
 
+// The formats in the order stb_image tries them (stbi__load_main): those with a
+// signature first, TGA (which has none) last.
 bool decode_any8(const std::string& path, Raw8& r, std::string& err) {
     std::string b = read_file(path);
+    const uint8_t* d = (const uint8_t*)b.data();
     if (b.size() >= 8 && (uint8_t)b[0] == 137 && b[1] == 'P' && b[2] == 'N' && b[3] == 'G') return decode_png(b, r, err);
+    if (is_bmp(d, b.size())) return decode_bmp(d, b.size(), r, err);
+    if (is_jpeg(d, b.size())) return decode_jpeg(d, b.size(), r, err);
     if (b.size() >= 2 && b[0] == 'P' && (b[1] == '5' || b[1] == '6')) return decode_pnm(b, r, err);
+    if (is_tga(d, b.size())) return decode_tga(d, b.size(), r, err);
     err = "unsupported image format";
     return false;
 }

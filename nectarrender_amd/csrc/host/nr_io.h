@@ -41,7 +41,8 @@ struct ImageF {
 };
 
 // load_image_uint8 (csrc/material/include/material/texture.h:8-27).
-// PNG (every colour type and bit depth, Adam7 interlacing), binary PPM/PGM and Radiance .hdr.
+// PNG (every colour type and bit depth, Adam7 interlacing), JPEG (baseline,
+// extended and progressive Huffman), BMP, TGA, binary PPM/PGM and Radiance .hdr.
 Image8 load_image_u8(const std::string& path);
 // load_image_float (texture.h:29-47): Radiance .hdr (RGBE), PFM, and 8-bit
 // formats converted as pow(x/255, 2.2) for colour channels.

@@ -78,6 +78,18 @@ uint8_t* ioref_image_u8(const char* path, int* C, int* H, int* W) {
     return chw;
 }
 
+// stbi_load(path, ..., 0) itself: interleaved H x W x C samples as the library
+// returns them (grey and grey + alpha files included), before the glue above
+uint8_t* ioref_stbi_u8(const char* path, int* C, int* H, int* W) {
+    unsigned char* data = stbi_load(path, W, H, C, 0);
+    if (!data) return nullptr;
+    const size_t n = (size_t)(*W) * (*H) * (*C);
+    uint8_t* out = (uint8_t*)std::malloc(n);
+    std::memcpy(out, data, n);
+    stbi_image_free(data);
+    return out;
+}
+
 // texture.h:29-47
 float* ioref_image_f32(const char* path, int* C, int* H, int* W) {
     float* data = stbi_loadf(path, W, H, C, 0);

@@ -27,6 +27,8 @@ def lib():
         ip = C.POINTER(C.c_int)
         L.ioref_image_u8.restype = C.POINTER(C.c_uint8)
         L.ioref_image_u8.argtypes = [C.c_char_p, ip, ip, ip]
+        L.ioref_stbi_u8.restype = C.POINTER(C.c_uint8)
+        L.ioref_stbi_u8.argtypes = [C.c_char_p, ip, ip, ip]
         L.ioref_image_f32.restype = C.POINTER(C.c_float)
         L.ioref_image_f32.argtypes = [C.c_char_p, ip, ip, ip]
         L.ioref_obj.restype = C.c_int
@@ -46,6 +48,18 @@ def image_u8(path: str):
     out = np.ctypeslib.as_array(p, (n,)).copy().reshape(c.value, h.value, w.value)
     lib().ioref_free(p)
     return out, c.value
+
+
+def stbi_u8(path: str) -> np.ndarray:
+    """stbi_load(path, ..., 0) as the library returns it: H x W x C uint8 (any C)."""
+    c, h, w = C.c_int(), C.c_int(), C.c_int()
+    p = lib().ioref_stbi_u8(path.encode(), C.byref(c), C.byref(h), C.byref(w))
+    if not p:
+        raise RuntimeError("stb could not load " + path)
+    n = c.value * h.value * w.value
+    out = np.ctypeslib.as_array(p, (n,)).copy().reshape(h.value, w.value, c.value)
+    lib().ioref_free(p)
+    return out
 
 
 def image_f32(path: str) -> np.ndarray:

@@ -219,6 +219,35 @@ def _obj_cases(rng):
                 lines.append(f"f {q[0]}/{q[0]} {q[2]}/{q[2]} {q[3]}/{q[3]}")
     cases["noisy_grid"] = "\n".join(lines) + "\n"
     cases["crlf_comments"] = "# header\r\nv 0 0 0\r\nv 1 0 0\r\nv 0 1 0\r\n\r\nf 1 2 3\r\n"
+
+    # faces of five or more corners: tinyobjloader's ear clipping (projection axes chosen from the
+    # first non-degenerate corner, reflex corners and blocked ears skipped)
+    def poly(P, faces, extra=""):
+        return ("".join(f"v {x:.7g} {y:.7g} {z:.7g}\n" for x, y, z in P) + extra
+                + "".join("f " + " ".join(str(i + 1) for i in f) + "\n" for f in faces))
+    cases["concave_pentagon_xy"] = poly([(0, 0, 0), (2, 0, 0), (2, 2, 0), (1, 0.8, 0), (0, 2, 0)], [range(5)])
+    ang = np.linspace(0, 2 * np.pi, 8, endpoint=False)
+    r = np.where(np.arange(8) % 2 == 0, 1.0, 0.35)
+    star = [(r[i] * np.cos(a), 0.2, r[i] * np.sin(a)) for i, a in enumerate(ang)]
+    cases["star_octagon_xz"] = poly(star, [range(8)])
+    cases["star_octagon_xz_clockwise"] = poly(star, [range(7, -1, -1)])
+    cases["hexagon_yz_collinear"] = poly([(0.5, 0, 0), (0.5, 1, 0), (0.5, 2, 0), (0.5, 2, 1), (0.5, 1, 2), (0.5, 0, 1)],
+                                         [range(6)])
+    cases["comb_nonagon"] = poly([(0, 0, 0), (4, 0, 0), (4, 1, 0), (3, 1, 0), (3, 3, 0), (2, 1.2, 0), (1, 3, 0),
+                                  (1, 1, 0), (0, 1, 0)], [range(9)])
+    tilted, faces, base = [], [], 0
+    for k in range(12):
+        n = int(rng.integers(5, 12))
+        a = np.sort(rng.random(n)) * 2 * np.pi
+        pts = np.stack([(0.3 + rng.random(n)) * np.cos(a), (0.3 + rng.random(n)) * np.sin(a), np.zeros(n)], 1)
+        R = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+        tilted += list(pts @ R.T + rng.normal(size=3))
+        idx = list(range(base, base + n))
+        faces.append(idx if k % 2 else idx[::-1])
+        base += n
+    cases["tilted_ngons"] = poly(tilted, faces)
+    cases["ngon_with_uv_normals"] = ("v 0 0 0\nv 2 0 0\nv 2 2 0\nv 1 0.7 0\nv 0 2 0\nvt 0 0\nvt 1 0\nvt 1 1\n"
+                                     "vt 0.5 0.4\nvt 0 1\nvn 0 0 1\nf 1/1/1 2/2/1 3/3/1 4/4/1 5/5/1\n")
     return cases
 
 
@@ -355,3 +384,405 @@ def test_ldr_as_hdri_matches_reference(tmp_path):
         p = str(tmp_path / f"{name}.png")
         write_png(p, rng.integers(0, 256, (5, 8, ch)), ctype)
         np.testing.assert_array_equal(pt.io.load_image_float(p), ioref.image_f32(p), err_msg=name)
+
+
+# ---------------------------------------------------------------------------
+# JPEG, BMP, TGA (stb_image's loaders in the reference)
+# ---------------------------------------------------------------------------
+def _same_as_stb(path):
+    """The product's planes against stb's own output for the same file: the reference's glue
+    (texture.h:8-27) for 3- and 4-channel files, stb's raw samples replicated for grey ones."""
+    got, c = pt.io.load_image_uint8(path)
+    raw = ioref.stbi_u8(path)
+    if raw.shape[2] >= 3:
+        ref, c_ref = ioref.image_u8(path)
+        assert c == c_ref, path
+        np.testing.assert_array_equal(got, ref, err_msg=path)
+    else:
+        assert c == 3 and got.shape == (3,) + raw.shape[:2], path
+        for k in range(3):
+            np.testing.assert_array_equal(got[k], raw[:, :, 0], err_msg=path)
+    return raw
+
+
+def _smooth(rng, h, w, c):
+    y, x = np.mgrid[0:h, 0:w]
+    planes = [127 + 100 * np.sin(x / 5.0 + k) * np.cos(y / 7.0 - k) + rng.normal(0, 20, (h, w)) for k in range(c)]
+    return np.clip(np.stack(planes, -1), 0, 255).astype(np.uint8)
+
+
+@needs_ref
+def test_jpeg_from_libjpeg_matches_reference(tmp_path):
+    """libjpeg's encoder (through PIL): baseline 4:4:4 / 4:2:2 / 4:2:0, progressive (spectral
+    selection + successive approximation), restart markers, RGB without colour transform, CMYK
+    (Adobe), grey baseline and progressive; sizes down to 1 x 1 and widths that leave partial
+    MCUs."""
+    Image = pytest.importorskip("PIL.Image")
+    rng = np.random.default_rng(21)
+    n = 0
+    for h, w in ((23, 37), (1, 1), (8, 8), (17, 9), (3, 70)):
+        a = _smooth(rng, h, w, 3)
+        for kw in (dict(subsampling=0), dict(subsampling=1), dict(subsampling=2), dict(progressive=True),
+                   dict(progressive=True, subsampling=0), dict(restart_marker_blocks=3), dict(keep_rgb=True),
+                   dict(optimize=True, progressive=True, subsampling=1)):
+            p = str(tmp_path / f"c{n}.jpg")
+            Image.fromarray(a).save(p, "JPEG", quality=85, **kw)
+            _same_as_stb(p)
+            n += 1
+        for mode, kw in (("L", {}), ("L", dict(progressive=True)), ("CMYK", {})):
+            im = Image.fromarray(a[:, :, 0]) if mode == "L" else Image.fromarray(a).convert("CMYK")
+            p = str(tmp_path / f"c{n}.jpg")
+            im.save(p, "JPEG", quality=70, **kw)
+            raw = _same_as_stb(p)
+            assert raw.shape[2] == (1 if mode == "L" else 3)
+            n += 1
+
+
+# a baseline JPEG writer for what libjpeg's front end does not produce: any sampling factors,
+# single-component scans, restart intervals, 16-bit quantisation tables, Adobe transforms.
+# Coefficients are written as given (no DCT): both decoders reconstruct the same data.
+_AC_SYMBOLS = [0x00, 0xF0] + [(r << 4) | s for r in range(16) for s in range(1, 11)]
+
+
+def _huff_codes(counts):
+    codes, code = [], 0
+    for L in range(1, 17):
+        for _ in range(counts[L - 1]):
+            codes.append((code, L))
+            code += 1
+        code <<= 1
+    return codes
+
+
+class _BitWriter:
+    def __init__(self):
+        self.out, self.acc, self.n = bytearray(), 0, 0
+
+    def put(self, v, n):
+        for i in range(n - 1, -1, -1):
+            self.acc = (self.acc << 1) | ((v >> i) & 1)
+            self.n += 1
+            if self.n == 8:
+                self.out.append(self.acc)
+                if self.acc == 0xFF:
+                    self.out.append(0)
+                self.acc, self.n = 0, 0
+
+    def flush(self):
+        while self.n:
+            self.put(1, 1)
+
+
+def write_jpeg_coeffs(path, W, H, sampling, coeffs, qtables, restart=0, interleaved=True, ids=None, jfif=True,
+                      adobe=None, fill=False):
+    """sampling: (h, v, tq) per component; coeffs[c]: (rows, cols, 64) quantised coefficients in
+    zigzag order over the component's whole-MCU block grid."""
+    dc_counts = [0, 0, 0, 12] + [0] * 12
+    ac_counts = [0] * 7 + [len(_AC_SYMBOLS)] + [0] * 8
+    dc_code = dict(zip(range(12), _huff_codes(dc_counts)))
+    ac_code = dict(zip(_AC_SYMBOLS, _huff_codes(ac_counts)))
+    nc = len(sampling)
+    hmax, vmax = max(s[0] for s in sampling), max(s[1] for s in sampling)
+    mx, my = -(-W // (8 * hmax)), -(-H // (8 * vmax))
+    seg = lambda m, body: bytes([0xFF, m]) + struct.pack(">H", len(body) + 2) + body    # noqa: E731
+    out = bytearray(b"\xff\xd8")
+    if jfif:
+        out += seg(0xE0, b"JFIF\x00\x01\x01\x00\x00\x01\x00\x01\x00\x00")
+    if adobe is not None:
+        out += seg(0xEE, b"Adobe\x00\x64\x00\x00\x00\x00" + bytes([adobe]))
+    for t, q in qtables.items():
+        q = np.asarray(q).reshape(64)
+        if q.max() > 255:
+            out += seg(0xDB, bytes([0x10 | t]) + b"".join(struct.pack(">H", int(x)) for x in q))
+        else:
+            out += seg(0xDB, bytes([t]) + bytes(int(x) for x in q))
+    ids = ids or list(range(1, nc + 1))
+    out += seg(0xC0, struct.pack(">BHHB", 8, H, W, nc) + b"".join(bytes([ids[c], (h << 4) | v, tq])
+                                                                 for c, (h, v, tq) in enumerate(sampling)))
+    out += seg(0xC4, bytes([0x00]) + bytes(dc_counts) + bytes(range(12)))
+    out += seg(0xC4, bytes([0x10]) + bytes(ac_counts) + bytes(_AC_SYMBOLS))
+    if restart:
+        out += seg(0xDD, struct.pack(">H", restart))
+
+    def block(bw, blk, pred):
+        diff = int(blk[0]) - pred
+        size = abs(diff).bit_length()
+        bw.put(*dc_code[size])
+        if size:
+            bw.put(diff if diff > 0 else diff + (1 << size) - 1, size)
+        run = 0
+        for k in range(1, 64):
+            v = int(blk[k])
+            if v == 0:
+                run += 1
+                continue
+            while run > 15:
+                bw.put(*ac_code[0xF0])
+                run -= 16
+            size = abs(v).bit_length()
+            bw.put(*ac_code[(run << 4) | size])
+            bw.put(v if v > 0 else v + (1 << size) - 1, size)
+            run = 0
+        if run:
+            bw.put(*ac_code[0x00])
+        return int(blk[0])
+
+    def scan(comp_list, units):
+        """units: list of MCUs, each a list of (component, block) in coding order."""
+        nonlocal out
+        out += seg(0xDA, bytes([len(comp_list)]) + b"".join(bytes([ids[c], 0x00]) for c in comp_list) +
+                   bytes([0, 63, 0]))
+        bw, pred, rst = _BitWriter(), [0] * nc, 0
+        for i, mcu in enumerate(units):
+            if restart and i and i % restart == 0:
+                bw.flush()
+                bw.out += bytes([0xFF, 0xD0 + rst % 8])
+                rst += 1
+                pred = [0] * nc
+            for c, blk in mcu:
+                pred[c] = block(bw, blk, pred[c])
+        bw.flush()
+        out += bw.out
+        if fill:
+            out += b"\xff\xff"
+
+    if interleaved:
+        units = []
+        for j in range(my):
+            for i in range(mx):
+                units.append([(c, coeffs[c][j * v + y, i * h + x]) for c, (h, v, _) in enumerate(sampling)
+                              for y in range(v) for x in range(h)])
+        scan(list(range(nc)), units)
+    else:
+        for c, (h, v, _) in enumerate(sampling):
+            cw = -(-(-(-W * h // hmax)) // 8)
+            ch = -(-(-(-H * v // vmax)) // 8)
+            scan([c], [[(c, coeffs[c][y, x])] for y in range(ch) for x in range(cw)])
+    out += b"\xff\xd9"
+    with open(path, "wb") as fh:
+        fh.write(bytes(out))
+
+
+def _random_coeffs(rng, sampling, W, H):
+    hmax, vmax = max(s[0] for s in sampling), max(s[1] for s in sampling)
+    mx, my = -(-W // (8 * hmax)), -(-H // (8 * vmax))
+    out = []
+    for h, v, _ in sampling:
+        c = np.zeros((my * v, mx * h, 64), np.int64)
+        c[..., 0] = rng.integers(-40, 40, c.shape[:2])
+        k = rng.integers(1, 64, c.shape[:2] + (6,))
+        vals = rng.integers(-25, 26, c.shape[:2] + (6,))
+        for t in range(6):
+            np.put_along_axis(c, k[..., t:t + 1], vals[..., t:t + 1], axis=2)
+        c[..., 1:4] += rng.integers(-8, 9, c.shape[:2] + (3,))
+        out.append(c)
+    return out
+
+
+@needs_ref
+def test_jpeg_sampling_scans_and_markers_match_reference(tmp_path):
+    rng = np.random.default_rng(22)
+    q8 = {0: rng.integers(1, 9, 64), 1: rng.integers(1, 12, 64)}
+    q16 = {0: rng.integers(200, 400, 64), 1: rng.integers(1, 9, 64)}
+    cases = [
+        ("4:2:0 odd size", (37, 23), [(2, 2, 0), (1, 1, 1), (1, 1, 1)], {}),
+        ("4:4:0 (vertical only)", (20, 29), [(1, 2, 0), (1, 1, 1), (1, 1, 1)], {}),
+        ("4:1:1 (generic 4x)", (45, 9), [(4, 1, 0), (1, 1, 1), (1, 1, 1)], {}),
+        ("3x3 (generic)", (25, 26), [(3, 3, 0), (1, 1, 1), (1, 1, 1)], {}),
+        ("mixed chroma factors", (33, 18), [(2, 2, 0), (2, 1, 1), (1, 2, 1)], {}),
+        ("4x4 luma, 2x2 chroma", (41, 35), [(4, 4, 0), (2, 2, 1), (2, 2, 1)], {}),
+        ("1 x 1", (1, 1), [(2, 2, 0), (1, 1, 1), (1, 1, 1)], {}),
+        ("restart every 2 MCUs", (50, 30), [(2, 1, 0), (1, 1, 1), (1, 1, 1)], dict(restart=2, fill=True)),
+        ("single-component scans", (27, 19), [(2, 2, 0), (1, 1, 1), (1, 1, 1)], dict(interleaved=False)),
+        ("single-component scans + restarts", (27, 19), [(1, 1, 0), (1, 1, 1), (1, 1, 1)],
+         dict(interleaved=False, restart=5)),
+        ("16-bit quantisation tables", (16, 24), [(1, 1, 0), (1, 1, 1), (1, 1, 1)], dict(q=q16)),
+        ("RGB by component ids", (19, 11), [(1, 1, 0), (1, 1, 0), (1, 1, 0)], dict(ids=[82, 71, 66], jfif=False)),
+        ("RGB by Adobe transform 0", (19, 11), [(1, 1, 0), (1, 1, 0), (1, 1, 0)], dict(adobe=0, jfif=False)),
+        ("YCbCr without JFIF", (19, 11), [(1, 1, 0), (1, 1, 1), (1, 1, 1)], dict(jfif=False)),
+        ("CMYK (Adobe 0)", (21, 13), [(1, 1, 0)] * 4, dict(adobe=0, jfif=False)),
+        ("YCCK (Adobe 2)", (21, 13), [(2, 2, 0), (1, 1, 1), (1, 1, 1), (2, 2, 0)], dict(adobe=2, jfif=False)),
+        ("YCbCr + 4th channel (Adobe 1)", (21, 13), [(1, 1, 0), (1, 1, 1), (1, 1, 1), (1, 1, 0)],
+         dict(adobe=1, jfif=False)),
+        ("grey, 2x2 factors", (30, 17), [(2, 2, 0)], {}),
+    ]
+    for name, (W, H), sampling, kw in cases:
+        kw = dict(kw)
+        q = kw.pop("q", q8)
+        p = str(tmp_path / "t.jpg")
+        write_jpeg_coeffs(p, W, H, sampling, _random_coeffs(rng, sampling, W, H), q, **kw)
+        raw = _same_as_stb(p)
+        assert raw.shape[:2] == (H, W), name
+
+
+def write_bmp(path, pix, bpp, hsz=40, bitfields=None, top_down=False, palette=None, alpha_mask=None):
+    """pix: H x W x 4 RGBA (bpp >= 16) or H x W palette indices (bpp <= 8, palette N x 3 RGB).
+    bitfields: (r, g, b) masks for BI_BITFIELDS, or None for BI_RGB."""
+    pix = np.asarray(pix)
+    H, W = pix.shape[:2]
+    rows = []
+    for y in range(H):
+        r = bytearray()
+        if bpp <= 8:
+            bits = "".join(format(int(v), f"0{bpp}b") for v in pix[y])
+            bits += "0" * (-len(bits) % 8)
+            r += bytes(int(bits[i:i + 8], 2) for i in range(0, len(bits), 8))
+        elif bpp == 24:
+            for R, G, B, _ in pix[y]:
+                r += bytes([B, G, R])
+        else:
+            masks = bitfields or ((0x7C00, 0x3E0, 0x1F) if bpp == 16 else (0xFF0000, 0xFF00, 0xFF))
+            am = alpha_mask if alpha_mask is not None else (0xFF000000 if bpp == 32 else 0)
+            for px in pix[y]:
+                v = 0
+                for m, val in zip(list(masks) + [am], px):
+                    if m:
+                        lo = (m & -m).bit_length() - 1
+                        width = bin(m).count("1")
+                        v |= ((int(val) >> (8 - width)) << lo) & m
+                r += struct.pack("<H" if bpp == 16 else "<I", v)
+        r += bytes(-len(r) % 4)
+        rows.append(bytes(r))
+    data = b"".join(rows if top_down else rows[::-1])
+    pal = b""
+    if palette is not None:
+        pal = b"".join(bytes([b, g, r_]) + (b"" if hsz == 12 else b"\x00") for r_, g, b in np.asarray(palette))
+    compress = 3 if bitfields is not None else 0
+    if hsz == 12:
+        info = struct.pack("<IHHHH", 12, W, H, 1, bpp)
+    else:
+        info = struct.pack("<IiiHHIIiiII", hsz, W, -H if top_down else H, 1, bpp, compress, len(data), 2835,
+                           2835, 0 if palette is None else len(palette), 0)
+        if hsz == 56:
+            info += struct.pack("<4I", 0, 0, 0, 0)
+        elif hsz in (108, 124):
+            m = bitfields or (0, 0, 0)
+            info += struct.pack("<4I", *m, alpha_mask or 0) + struct.pack("<I", 0x73524742) + bytes(48)
+            if hsz == 124:
+                info += bytes(16)
+        elif compress == 3:
+            info += struct.pack("<3I", *bitfields)
+    offset = 14 + len(info) + len(pal)
+    with open(path, "wb") as fh:
+        fh.write(b"BM" + struct.pack("<IHHI", offset + len(data), 0, 0, offset) + info + pal + data)
+
+
+@needs_ref
+def test_bmp_matches_reference(tmp_path):
+    rng = np.random.default_rng(23)
+    rgba = lambda h, w: rng.integers(0, 256, (h, w, 4))                                  # noqa: E731
+    pal = rng.integers(0, 256, (256, 3))
+    cases = [
+        ("24-bit", dict(pix=rgba(5, 7), bpp=24)),
+        ("24-bit top-down", dict(pix=rgba(6, 3), bpp=24, top_down=True)),
+        ("24-bit 1 x 1", dict(pix=rgba(1, 1), bpp=24)),
+        ("32-bit alpha", dict(pix=rgba(4, 5), bpp=32)),
+        ("32-bit zero alpha", dict(pix=np.concatenate([rgba(4, 5)[..., :3], np.zeros((4, 5, 1), int)], 2), bpp=32)),
+        ("16-bit 5:5:5", dict(pix=rgba(5, 9), bpp=16)),
+        ("16-bit 5:6:5 bit fields", dict(pix=rgba(5, 9), bpp=16, bitfields=(0xF800, 0x7E0, 0x1F))),
+        ("32-bit odd bit fields", dict(pix=rgba(3, 4), bpp=32, bitfields=(0xF00000, 0x3F00, 0x7))),
+        ("8-bit palette", dict(pix=rng.integers(0, 256, (6, 11)), bpp=8, palette=pal)),
+        ("4-bit palette", dict(pix=rng.integers(0, 16, (5, 7)), bpp=4, palette=pal[:16])),
+        ("1-bit palette", dict(pix=rng.integers(0, 2, (4, 13)), bpp=1, palette=pal[:2])),
+        ("OS/2 header, 8-bit", dict(pix=rng.integers(0, 256, (3, 5)), bpp=8, hsz=12, palette=pal)),
+        ("OS/2 header, 24-bit", dict(pix=rgba(3, 5), bpp=24, hsz=12)),
+        ("56-byte header", dict(pix=rgba(3, 6), bpp=24, hsz=56)),
+        ("V4 header, 32-bit masks", dict(pix=rgba(4, 4), bpp=32, hsz=108, bitfields=(0xFF0000, 0xFF00, 0xFF),
+                                         alpha_mask=0xFF000000)),
+        ("V5 header, 24-bit", dict(pix=rgba(3, 5), bpp=24, hsz=124)),
+    ]
+    for name, kw in cases:
+        p = str(tmp_path / "t.bmp")
+        write_bmp(p, **kw)
+        raw = _same_as_stb(p)
+        assert raw.shape[0] == np.asarray(kw["pix"]).shape[0], name
+
+
+def write_tga(path, pix, bpp, grey=False, rle=False, top=False, palette=None, pal_bits=24, id_len=0):
+    """pix: H x W x C samples in file order (BGR(A), grey(+alpha), 16-bit words as H x W, or
+    palette indices); palette: entries already in file order (bytes per entry or 16-bit words)."""
+    pix = np.asarray(pix)
+    H, W = pix.shape[:2]
+    kind = 1 if palette is not None else (3 if grey else 2)
+    if rle:
+        kind += 8
+
+    def enc(p):
+        if bpp in (15, 16) and not grey:
+            return struct.pack("<H", int(p))
+        if palette is not None:
+            return struct.pack("<H", int(p)) if bpp == 16 else bytes([int(p)])
+        return bytes(int(x) for x in np.atleast_1d(p))
+
+    order = range(H) if top else range(H - 1, -1, -1)
+    pixels = [enc(pix[y, x]) for y in order for x in range(W)]
+    body = bytearray()
+    if not rle:
+        body = b"".join(pixels)
+    else:
+        i = 0
+        while i < len(pixels):
+            run = 1
+            while i + run < len(pixels) and run < 128 and pixels[i + run] == pixels[i]:
+                run += 1
+            if run >= 2:
+                body += bytes([0x80 | (run - 1)]) + pixels[i]
+                i += run
+            else:
+                n = 1
+                while i + n < len(pixels) and n < 128 and pixels[i + n] != pixels[i + n - 1]:
+                    n += 1
+                body += bytes([n - 1]) + b"".join(pixels[i:i + n])
+                i += n
+    pal = b""
+    if palette is not None:
+        pal = b"".join(struct.pack("<H", int(e)) if pal_bits in (15, 16) else bytes(int(x) for x in e)
+                       for e in palette)
+    head = struct.pack("<BBBHHBHHHHBB", id_len, 1 if palette is not None else 0, kind, 0,
+                       0 if palette is None else len(palette), pal_bits if palette is not None else 0, 0, 0, W, H,
+                       bpp, 0x20 if top else 0)
+    with open(path, "wb") as fh:
+        fh.write(head + bytes(range(id_len)) + pal + bytes(body))
+
+
+@needs_ref
+def test_tga_matches_reference(tmp_path):
+    rng = np.random.default_rng(24)
+    blocky = lambda h, w, c: np.repeat(rng.integers(0, 256, (h, (w + 2) // 3, c)), 3, axis=1)[:, :w]  # noqa: E731
+    words = lambda h, w: np.repeat(rng.integers(0, 1 << 16, (h, (w + 1) // 2)), 2, axis=1)[:, :w]     # noqa: E731
+    cases = [
+        ("24-bit", dict(pix=blocky(5, 7, 3), bpp=24)),
+        ("24-bit top-left origin", dict(pix=blocky(5, 7, 3), bpp=24, top=True)),
+        ("32-bit RLE", dict(pix=blocky(6, 9, 4), bpp=32, rle=True)),
+        ("24-bit RLE top", dict(pix=blocky(4, 10, 3), bpp=24, rle=True, top=True, id_len=5)),
+        ("16-bit 5:5:5", dict(pix=words(4, 5), bpp=16)),
+        ("15-bit RLE", dict(pix=words(4, 6), bpp=15, rle=True)),
+        ("8-bit grey", dict(pix=blocky(5, 6, 1), bpp=8, grey=True)),
+        ("8-bit grey RLE", dict(pix=blocky(5, 6, 1), bpp=8, grey=True, rle=True)),
+        ("16-bit grey + alpha", dict(pix=blocky(3, 4, 2), bpp=16, grey=True)),
+        ("colour-mapped, 24-bit entries", dict(pix=rng.integers(0, 20, (5, 8)), bpp=8,
+                                               palette=rng.integers(0, 256, (20, 3)))),
+        ("colour-mapped RLE, 32-bit entries", dict(pix=np.repeat(rng.integers(0, 9, (4, 4)), 2, axis=1), bpp=8,
+                                                   palette=rng.integers(0, 256, (9, 4)), pal_bits=32, rle=True)),
+        ("colour-mapped, 16-bit entries and indices", dict(pix=rng.integers(0, 30, (3, 7)), bpp=16,
+                                                           palette=rng.integers(0, 1 << 16, 30), pal_bits=16)),
+        ("1 x 1", dict(pix=blocky(1, 1, 3), bpp=24)),
+    ]
+    for name, kw in cases:
+        p = str(tmp_path / "t.tga")
+        write_tga(p, **kw)
+        raw = _same_as_stb(p)
+        assert raw.shape[:2] == np.asarray(kw["pix"]).shape[:2], name
+
+
+def test_unknown_and_truncated_images_rejected(tmp_path):
+    p = tmp_path / "x.bin"
+    p.write_bytes(b"\x00\x07garbage")
+    with pytest.raises(RuntimeError):
+        pt.io.load_image_uint8(str(p))
+    p.write_bytes(b"\xff\xd8\xff\xe0\x00\x10JFIF")          # SOI, then nothing after the APP0 header
+    with pytest.raises(RuntimeError):
+        pt.io.load_image_uint8(str(p))
+    p.write_bytes(b"BM" + bytes(12) + struct.pack("<I", 40) + bytes(8))    # BMP header cut short
+    with pytest.raises(RuntimeError):
+        pt.io.load_image_uint8(str(p))
