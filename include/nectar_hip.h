@@ -62,7 +62,12 @@ enum nr_material_type {
 /* SkylightType, csrc/light/include/skylight.h:131 */
 enum nr_sky_type { NR_SKY_NULL = 0, NR_SKY_SIMPLE = 1, NR_SKY_HDRI = 2 };
 
-/* LayerType, csrc/data/include/data/render_layers.h:11-14 */
+/* LayerType, csrc/data/include/data/render_layers.h:11-14.  A view holds BEAUTY,
+ * DIFFUSE and WORLD_NORMAL (3 x H x W, the reference's default layers) and
+ * DEPTH (1 x H x W): the running mean of the distance from the camera to the
+ * first hit of each sample (0 where the camera ray misses).  The reference
+ * allocates DEPTH but no kernel writes it; it is written here for the
+ * denoiser's input. */
 enum nr_layer {
     NR_LAYER_BEAUTY = 0,
     NR_LAYER_DIFFUSE = 1,
@@ -276,7 +281,7 @@ int nr_render_tiles(nr_ctx* ctx, int view, const nr_render_desc* desc, const uin
 /* Image size of a view (H = resolution[0], W = resolution[1]). */
 int nr_view_size(nr_ctx* ctx, int view, int32_t* H, int32_t* W);
 
-/* Copy one AOV layer (C x H x W fp32) of a view to host memory. */
+/* Copy one AOV layer (C x H x W fp32; C = 1 for DEPTH, 3 otherwise) of a view to host memory. */
 int nr_aov_read(nr_ctx* ctx, int view, int layer, float* host_chw, size_t n_floats);
 
 /* Device pointer of a view's AOV layer (C x H x W fp32), NULL if absent. */

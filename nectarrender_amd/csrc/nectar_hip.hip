@@ -640,6 +640,7 @@ __device__ __forceinline__ void fold(float4* acc, float4 v, float a, float b) {
     m.x = m.x * a + v.x * b;
     m.y = m.y * a + v.y * b;
     m.z = m.z * a + v.z * b;
+    m.w = m.w * a + v.w * b;          // DEPTH (normal layer); 0 for the others
     *acc = m;
 }
 
@@ -765,24 +766,25 @@ __global__ void __launch_bounds__(256) k_quant_nodes(const GWNode* __restrict__ 
 // pixel-interleaved copies (one 16-byte read-modify-write per layer and pixel)
 // and writes them back at the end.
 __global__ void __launch_bounds__(256) k_pack_acc(const float* __restrict__ b, const float* __restrict__ d,
-                                                  const float* __restrict__ n, float4* __restrict__ acc, uint32_t HW,
-                                                  uint32_t cap) {
+                                                  const float* __restrict__ n, const float* __restrict__ z,
+                                                  float4* __restrict__ acc, uint32_t HW, uint32_t cap) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= HW) return;
     acc[i] = make_float4(b[i], b[HW + i], b[2 * (size_t)HW + i], 0.0f);
     acc[cap + i] = make_float4(d[i], d[HW + i], d[2 * (size_t)HW + i], 0.0f);
-    acc[2 * (size_t)cap + i] = make_float4(n[i], n[HW + i], n[2 * (size_t)HW + i], 0.0f);
+    acc[2 * (size_t)cap + i] = make_float4(n[i], n[HW + i], n[2 * (size_t)HW + i], z[i]);
 }
 
 __global__ void __launch_bounds__(256) k_unpack_acc(const float4* __restrict__ acc, float* __restrict__ b,
-                                                    float* __restrict__ d, float* __restrict__ n, uint32_t HW,
-                                                    uint32_t cap) {
+                                                    float* __restrict__ d, float* __restrict__ n,
+                                                    float* __restrict__ dep, uint32_t HW, uint32_t cap) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= HW) return;
     float4 x = acc[i], y = acc[cap + i], z = acc[2 * (size_t)cap + i];
     b[i] = x.x; b[HW + i] = x.y; b[2 * (size_t)HW + i] = x.z;
     d[i] = y.x; d[HW + i] = y.y; d[2 * (size_t)HW + i] = y.z;
     n[i] = z.x; n[HW + i] = z.y; n[2 * (size_t)HW + i] = z.z;
+    dep[i] = z.w;
 }
 
 // First `slots` samples of every pixel (fewer if the range is shorter),
@@ -1049,7 +1051,8 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
         f3 dv = s3(0.0f), nv = s3(0.0f);                                    // the path ends here)
         if (alive) { dv = dv + atten; nv = nv + nn; }
         P.diff[path] = make_float4(dv.x, dv.y, dv.z, 0.0f);
-        P.nrm[path] = make_float4(nv.x, nv.y, nv.z, 0.0f);
+        // the DEPTH AOV rides in .w: distance from the camera to the first hit
+        P.nrm[path] = make_float4(nv.x, nv.y, nv.z, vlength(p - mk3(o4.x, o4.y, o4.z)));
     }
     if (alive) {
         if (A.do_rr && bounce >= A.rr_start && bounce + 1 < A.depth) {      // trace.cu:72-77
@@ -1227,8 +1230,8 @@ __global__ void __launch_bounds__(128) k_tail(const GScene* __restrict__ Sp, Pat
 // ray_depth 0: every pass adds a zero sample (trace.cu:63 runs no bounce).  Pass s
 // is folded as sample number fs; with per-tile limits (tile_end non-null, a view
 // W pixels wide) a pixel whose tile ends at or before pass s keeps its AOVs.
-__global__ void __launch_bounds__(256) k_fold_zero(float* beauty, float* diffuse, float* normal, uint32_t n, uint32_t s,
-                                                   uint32_t fs, uint32_t W, const uint32_t* __restrict__ tile_end,
+__global__ void __launch_bounds__(256) k_fold_zero(float* beauty, float* diffuse, float* normal, float* depth, uint32_t n,
+                                                   uint32_t s, uint32_t fs, uint32_t W, const uint32_t* __restrict__ tile_end,
                                                    uint32_t tile_shift, uint32_t tile_cols) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1238,6 +1241,7 @@ __global__ void __launch_bounds__(256) k_fold_zero(float* beauty, float* diffuse
     float* layers[3] = {beauty, diffuse, normal};
     for (float* l : layers)
         for (int c = 0; c < 3; c++) l[(size_t)c * n + i] = l[(size_t)c * n + i] * a + 0.0f * b;
+    depth[i] = depth[i] * a + 0.0f * b;
 }
 
 struct PostArgs {
@@ -1410,6 +1414,9 @@ struct View {
     uint8_t* h_rb = nullptr;              // pinned staging (3 x H x W)
     size_t sel_cap = 0, rb_cap = 0, den_cap = 0;
 };
+
+// planes of an AOV layer (render_layers.h:178-188: SHADOW and DEPTH have one)
+static inline int layer_channels(int L) { return (L == NR_LAYER_DEPTH || L == NR_LAYER_SHADOW) ? 1 : 3; }
 
 struct nr_ctx {
     int device = 0;
@@ -2239,10 +2246,11 @@ int nr_camera_set(nr_ctx* c, int view, const nr_camera_desc* cd, uint32_t seed) 
     g.n_pattern = cd->samples_per_pixel;
     if (!v.valid || v.H != H || v.W != W) {
         for (auto& l : v.layer) { if (l) HIPCK(hipFree(l)); l = nullptr; }
-        const int layers[3] = {NR_LAYER_BEAUTY, NR_LAYER_DIFFUSE, NR_LAYER_WORLD_NORMAL};
+        const int layers[4] = {NR_LAYER_BEAUTY, NR_LAYER_DIFFUSE, NR_LAYER_WORLD_NORMAL, NR_LAYER_DEPTH};
         for (int L : layers) {
-            HIPCK(hipMalloc((void**)&v.layer[L], sizeof(float) * 3 * (size_t)H * W));
-            HIPCK(hipMemset(v.layer[L], 0, sizeof(float) * 3 * (size_t)H * W));
+            const size_t bytes = sizeof(float) * layer_channels(L) * (size_t)H * W;
+            HIPCK(hipMalloc((void**)&v.layer[L], bytes));
+            HIPCK(hipMemset(v.layer[L], 0, bytes));
         }
     }
     v.H = H; v.W = W;
@@ -2257,8 +2265,8 @@ int nr_aov_reset(nr_ctx* c, int view) {
     HIPCK(hipSetDevice(c->device));
     View& v = c->views[view];
     LayerWrite w(c, v);
-    for (auto* l : v.layer)
-        if (l) HIPCK(hipMemsetAsync(l, 0, sizeof(float) * 3 * (size_t)v.H * v.W, c->stream));
+    for (int L = 0; L < 9; L++)
+        if (v.layer[L]) HIPCK(hipMemsetAsync(v.layer[L], 0, sizeof(float) * layer_channels(L) * (size_t)v.H * v.W, c->stream));
     return NR_OK;
 }
 
@@ -2444,7 +2452,8 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
             const uint32_t n = hv[k].npix;
             for (uint32_t s = d->sample_begin; s < d->sample_end; s++)
                 hipLaunchKernelGGL(k_fold_zero, dim3((n + 255) / 256), dim3(256), 0, c->stream, layer_of(k, NR_LAYER_BEAUTY),
-                                   layer_of(k, NR_LAYER_DIFFUSE), layer_of(k, NR_LAYER_WORLD_NORMAL), n, s,
+                                   layer_of(k, NR_LAYER_DIFFUSE), layer_of(k, NR_LAYER_WORLD_NORMAL),
+                                   layer_of(k, NR_LAYER_DEPTH), n, s,
                                    fast ? s + 1 : s, c->views[ids[k]].W, tl ? tl->d_end : nullptr, tl ? tl->shift : 0u,
                                    tl ? tl->cols : 0u);
         }
@@ -2494,7 +2503,8 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
         for (int k = 0; k < n_views; k++)
             hipLaunchKernelGGL(k_pack_acc, dim3((hv[k].npix + 255) / 256), dim3(256), 0, c->stream,
                                layer_of(k, NR_LAYER_BEAUTY), layer_of(k, NR_LAYER_DIFFUSE),
-                               layer_of(k, NR_LAYER_WORLD_NORMAL), c->P.acc + hv[k].pix0, hv[k].npix, c->P.cap);
+                               layer_of(k, NR_LAYER_WORLD_NORMAL), layer_of(k, NR_LAYER_DEPTH), c->P.acc + hv[k].pix0,
+                               hv[k].npix, c->P.cap);
         if (G > 1) {
             HIPCK(hipEventRecord(c->ev_packed, c->stream));
             for (int g = 0; g < G; g++) HIPCK(hipStreamWaitEvent(grp[g].st, c->ev_packed, 0));
@@ -2618,7 +2628,7 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
         for (int k = 0; k < n_views; k++)
             hipLaunchKernelGGL(k_unpack_acc, dim3((hv[k].npix + 255) / 256), dim3(256), 0, c->stream,
                                c->P.acc + hv[k].pix0, layer_of(k, NR_LAYER_BEAUTY), layer_of(k, NR_LAYER_DIFFUSE),
-                               layer_of(k, NR_LAYER_WORLD_NORMAL), hv[k].npix, c->P.cap);
+                               layer_of(k, NR_LAYER_WORLD_NORMAL), layer_of(k, NR_LAYER_DEPTH), hv[k].npix, c->P.cap);
     }
     HIPCK(hipGetLastError());
     if (c->timing) {
@@ -2662,7 +2672,7 @@ int nr_aov_read(nr_ctx* c, int view, int layer, float* host, size_t n_floats) {
     if (!c || !host || view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_INVALID, "bad view");
     if (layer < 0 || layer > 8 || !c->views[view].layer[layer]) return fail(NR_ERR_INVALID, "layer not allocated");
     View& v = c->views[view];
-    size_t need = 3 * (size_t)v.H * v.W;
+    size_t need = (size_t)layer_channels(layer) * v.H * v.W;
     if (n_floats < need) return fail(NR_ERR_INVALID, "host buffer too small");
     HIPCK(hipSetDevice(c->device));
     HIPCK(hipMemcpyAsync(host, v.layer[layer], need * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -2811,6 +2821,7 @@ int nr_post(nr_ctx* c, int view, const nr_post_desc* d, uint8_t* out8, float* ou
     if (!c || !d || view < 0 || view >= NR_MAX_VIEWS || !c->views[view].valid) return fail(NR_ERR_INVALID, "bad view");
     View& v = c->views[view];
     if (d->layer < 0 || d->layer > 8 || !v.layer[d->layer]) return fail(NR_ERR_INVALID, "post: layer not allocated");
+    if (layer_channels(d->layer) != 3) return fail(NR_ERR_INVALID, "post: the layer has one channel");
     HIPCK(hipSetDevice(c->device));
     uint32_t HW = v.H * v.W;
     PostArgs A = post_args(d);
@@ -2893,6 +2904,10 @@ int nr_viewport(nr_ctx* c, int view, int show_axis_grid) {
     HIPCK(hipMemsetAsync(c->d_err + CNT_ERR, 0, sizeof(int32_t), c->stream));
     {
         LayerWrite w(c, *v);
+        // aovs.overwrite (engine.cpp:89-91) copies every layer of the viewport's
+        // sample AOVs; trace_viewport writes no depth, so DEPTH becomes 0
+        if (v->layer[NR_LAYER_DEPTH])
+            HIPCK(hipMemsetAsync(v->layer[NR_LAYER_DEPTH], 0, sizeof(float) * (size_t)v->H * v->W, c->stream));
         if (c->obj_types & NR_TY_OTHER)
             hipLaunchKernelGGL(k_viewport<NR_TY_ALL>, dim3(nb), dim3(128), trav_lds_bytes(), c->stream, c->d_scene,
                                v->cam, show_axis_grid != 0, lb, ld, ln, c->d_err, SC);
@@ -3005,6 +3020,7 @@ int nr_readback(nr_ctx* c, int view, const nr_readback_desc* d, uint8_t* host) {
     if (!d || !host) return fail(NR_ERR_INVALID, "null argument");
     const int layer = d->post.layer;
     if (layer < 0 || layer > 8 || !v->layer[layer]) return fail(NR_ERR_INVALID, "readback: layer not allocated");
+    if (layer_channels(layer) != 3) return fail(NR_ERR_INVALID, "readback: the layer has one channel");
     if (d->denoiser && (!v->layer[NR_LAYER_DIFFUSE] || !v->layer[NR_LAYER_WORLD_NORMAL]))
         return fail(NR_ERR_STATE, "denoising needs the DIFFUSE and WORLD_NORMAL layers");
     std::lock_guard<std::mutex> one(v->rb_mu);

@@ -198,8 +198,9 @@ class Context:
         _check(lib().nr_sync(self._h))
 
     def read(self, view: int = 0, layer: int = abi.LAYER_BEAUTY) -> np.ndarray:
+        """A layer of the view, (C, H, W) float32: C = 1 for DEPTH, 3 otherwise."""
         H, W = self.views[view]
-        out = np.empty((3, H, W), np.float32)
+        out = np.empty((1 if layer == abi.LAYER_DEPTH else 3, H, W), np.float32)
         _check(lib().nr_aov_read(self._h, view, layer, out.ctypes.data_as(C.POINTER(C.c_float)), out.size))
         return out
 

@@ -1135,7 +1135,7 @@ static Ray camera_ray(const Camera& c, uint32_t x, uint32_t y, uint32_t pixel, u
 // ---------------------------------------------------------------------------
 // path (engine/src/trace.cu:11-79, engine/src/brdf.cu:4-35)
 // ---------------------------------------------------------------------------
-struct PathOut { V3 beauty{0, 0, 0}, diffuse{0, 0, 0}, normal{0, 0, 0}; int64_t segments = 0; };
+struct PathOut { V3 beauty{0, 0, 0}, diffuse{0, 0, 0}, normal{0, 0, 0}; float depth = 0.0f; int64_t segments = 0; };
 
 // sample_cosine_brdf (brdf.cu:4-35): with an ObjectLight in the scene, the 50/50
 // MixturePDF of the material's pdf and lights[0] (pdf.h:262-283), else the material's
@@ -1167,6 +1167,10 @@ static void trace_path(const Scene& S, const Camera& c, const nr_render_desc& d,
             out.beauty = add(out.beauty, mul(atten, sky_sample(S, ray)));
             alive = false;
         } else {
+            if (bounce == 0) {      // DEPTH: distance from the camera to the first hit (no reference writer)
+                const V3 e = sub(rec.p, ray.o);
+                out.depth = std::sqrt(((e.x * e.x) + (e.y * e.y)) + (e.z * e.z));
+            }
             const Material& m = S.materials[rec.material_id];
             Ray r_in = ray;
             out.beauty = add(out.beauty, mul(atten, mat_emitted(S, m, rec)));
@@ -1392,7 +1396,7 @@ int nro_nrooks(uint32_t n, uint32_t seed, float* out) {
 }
 
 int nro_render_pass(const nro_scene* s, const nro_camera* c, const nr_render_desc* d, uint32_t sample_idx,
-                    const int32_t* pixels, int n_pixels, float* beauty, float* diffuse, float* normal,
+                    const int32_t* pixels, int n_pixels, float* beauty, float* diffuse, float* normal, float* depth,
                     int64_t* counts, int threads) {
     const Camera& cam = c->c;
     if (sample_idx >= cam.pattern.size()) return 1;
@@ -1418,6 +1422,7 @@ int nro_render_pass(const nro_scene* s, const nro_camera* c, const nr_render_des
                 if (beauty) { beauty[px] = po.beauty.x; beauty[HW + px] = po.beauty.y; beauty[2 * HW + px] = po.beauty.z; }
                 if (diffuse) { diffuse[px] = po.diffuse.x; diffuse[HW + px] = po.diffuse.y; diffuse[2 * HW + px] = po.diffuse.z; }
                 if (normal) { normal[px] = po.normal.x; normal[HW + px] = po.normal.y; normal[2 * HW + px] = po.normal.z; }
+                if (depth) depth[px] = po.depth;
                 seg += po.segments; inner += cc.inner; tri += cc.tri; obj += cc.obj;
             };
             if (pixels) {
@@ -1444,15 +1449,16 @@ void nro_accumulate(float* avg, const float* sample, size_t n, uint32_t current_
 }
 
 int nro_render(const nro_scene* s, const nro_camera* c, const nr_render_desc* d,
-               float* beauty, float* diffuse, float* normal, int threads) {
+               float* beauty, float* diffuse, float* normal, float* depth, int threads) {
     size_t n = 3 * c->c.H * c->c.W;
-    std::vector<float> sb(n), sd(n), sn(n);
+    std::vector<float> sb(n), sd(n), sn(n), sz(n / 3);
     for (uint32_t si = d->sample_begin; si < d->sample_end; si++) {                      // engine.cpp:42-74
-        int e = nro_render_pass(s, c, d, si, nullptr, 0, sb.data(), sd.data(), sn.data(), nullptr, threads);
+        int e = nro_render_pass(s, c, d, si, nullptr, 0, sb.data(), sd.data(), sn.data(), sz.data(), nullptr, threads);
         if (e) return e;
         if (beauty) nro_accumulate(beauty, sb.data(), n, si);
         if (diffuse) nro_accumulate(diffuse, sd.data(), n, si);
         if (normal) nro_accumulate(normal, sn.data(), n, si);
+        if (depth) nro_accumulate(depth, sz.data(), n / 3, si);
     }
     return 0;
 }

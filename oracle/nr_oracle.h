@@ -40,18 +40,19 @@ int nro_nrooks(uint32_t n, uint32_t seed, float* out2n);
 
 /* One sample pass: writes the per-pass sample values (what trace_full_kernel
  * adds into the cleared sample AOVs) for the listed pixels (NULL = all) into
- * full-size planar CHW buffers (3 x H x W each; NULL buffers are skipped).
+ * full-size planar CHW buffers (3 x H x W each, depth 1 x H x W: the distance
+ * from the camera to the first hit, 0 on a miss; NULL buffers are skipped).
  * counts (optional, 5 values): segments, inner nodes, triangle tests,
  * top-level objects, paths. */
 int nro_render_pass(const nro_scene* s, const nro_camera* c, const nr_render_desc* d,
                     uint32_t sample_idx, const int32_t* pixels, int n_pixels,
-                    float* beauty, float* diffuse, float* normal,
+                    float* beauty, float* diffuse, float* normal, float* depth,
                     int64_t* counts, int threads);
 
 /* Full RenderEngine::render loop: passes d->sample_begin .. d->sample_end-1,
  * each folded into the running means (accumulate_samples_kernel). */
 int nro_render(const nro_scene* s, const nro_camera* c, const nr_render_desc* d,
-               float* beauty, float* diffuse, float* normal, int threads);
+               float* beauty, float* diffuse, float* normal, float* depth, int threads);
 
 /* accumulate_samples_kernel on n floats */
 void nro_accumulate(float* avg, const float* sample, size_t n, uint32_t current_sample);

@@ -43,8 +43,8 @@ def lib():
         "nro_camera_ray": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, fp]),
         "nro_nrooks": (C.c_int, [C.c_uint32, C.c_uint32, fp]),
         "nro_render_pass": (C.c_int, [vp, vp, C.POINTER(abi.NrRenderDesc), C.c_uint32, C.POINTER(C.c_int32),
-                                      C.c_int, fp, fp, fp, C.POINTER(C.c_int64), C.c_int]),
-        "nro_render": (C.c_int, [vp, vp, C.POINTER(abi.NrRenderDesc), fp, fp, fp, C.c_int]),
+                                      C.c_int, fp, fp, fp, fp, C.POINTER(C.c_int64), C.c_int]),
+        "nro_render": (C.c_int, [vp, vp, C.POINTER(abi.NrRenderDesc), fp, fp, fp, fp, C.c_int]),
         "nro_accumulate": (None, [fp, fp, C.c_size_t, C.c_uint32]),
         "nro_post": (C.c_int, [fp, C.c_int, C.c_int, C.POINTER(abi.NrPostDesc), fp, C.POINTER(C.c_uint8)]),
         "nro_philox_block": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
@@ -201,30 +201,34 @@ class OracleCamera:
 
 
 def render_pass(scene: OracleScene, cam: OracleCamera, desc: abi.NrRenderDesc, s: int, pixels=None, threads=0,
-                counts=False):
-    n = cam.H * cam.W
+                counts=False, depth=False):
+    """One sample pass: (beauty, diffuse, normal[, depth][, counts]); depth is 1 x H x W."""
     b = np.zeros((3, cam.H, cam.W), np.float32)
     d = np.zeros_like(b)
     nn = np.zeros_like(b)
+    z = np.zeros((1, cam.H, cam.W), np.float32)
     px = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
     cnt = np.zeros(5, np.int64)
     rc = lib().nro_render_pass(scene.h, cam.h, C.byref(desc), s,
                                None if px is None else px.ctypes.data_as(C.POINTER(C.c_int32)),
-                               0 if px is None else px.size, _fp(b), _fp(d), _fp(nn),
+                               0 if px is None else px.size, _fp(b), _fp(d), _fp(nn), _fp(z),
                                cnt.ctypes.data_as(C.POINTER(C.c_int64)) if counts else None, threads)
     if rc:
         raise RuntimeError("oracle render pass failed")
-    return (b, d, nn, cnt) if counts else (b, d, nn)
+    out = (b, d, nn) + ((z,) if depth else ())
+    return out + (cnt,) if counts else out
 
 
-def render(scene: OracleScene, cam: OracleCamera, desc: abi.NrRenderDesc, threads=0):
+def render(scene: OracleScene, cam: OracleCamera, desc: abi.NrRenderDesc, threads=0, depth=False):
+    """Running means of the passes: (beauty, diffuse, normal[, depth 1 x H x W])."""
     b = np.zeros((3, cam.H, cam.W), np.float32)
     d = np.zeros_like(b)
     nn = np.zeros_like(b)
-    rc = lib().nro_render(scene.h, cam.h, C.byref(desc), _fp(b), _fp(d), _fp(nn), threads)
+    z = np.zeros((1, cam.H, cam.W), np.float32)
+    rc = lib().nro_render(scene.h, cam.h, C.byref(desc), _fp(b), _fp(d), _fp(nn), _fp(z), threads)
     if rc:
         raise RuntimeError("oracle render failed")
-    return b, d, nn
+    return (b, d, nn, z) if depth else (b, d, nn)
 
 
 def viewport(scene: OracleScene, cam: OracleCamera, show_axis_grid: bool, threads=0):

@@ -90,8 +90,8 @@ def _render_both(ctx, oracle, scene, cam, rp, s0, s1):
     ctx.set_camera(0, cam, rp["seed"])
     ctx.reset(0)
     ctx.render(0, desc)
-    g = [ctx.read(0, L) for L in (abi.LAYER_BEAUTY, abi.LAYER_DIFFUSE, abi.LAYER_WORLD_NORMAL)]
-    o = oracle.render(oracle.OracleScene(scene), oracle.OracleCamera(cam, rp["seed"]), desc)
+    g = [ctx.read(0, L) for L in (abi.LAYER_BEAUTY, abi.LAYER_DIFFUSE, abi.LAYER_WORLD_NORMAL, abi.LAYER_DEPTH)]
+    o = oracle.render(oracle.OracleScene(scene), oracle.OracleCamera(cam, rp["seed"]), desc, depth=True)
     return g, o
 
 
@@ -110,7 +110,7 @@ def test_render_parity(ctx, oracle, name):
     else:
         s, cam, rp = scenes.mixed_scene(spp=5, res=128)
     g, o = _render_both(ctx, oracle, s, cam, rp, 1, 5)
-    for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
+    for k, name_ in enumerate(("beauty", "diffuse", "world_normal", "depth")):
         _assert_same(g[k], o[k], name_)
 
 
@@ -119,7 +119,7 @@ def test_render_parity_c2_geometry(ctx, oracle):
     s, cam, rp = scenes.config2(spp=3)
     cam.resolution[0], cam.resolution[1] = 90, 160
     g, o = _render_both(ctx, oracle, s, cam, rp, 1, 3)
-    for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
+    for k, name_ in enumerate(("beauty", "diffuse", "world_normal", "depth")):
         _assert_same(g[k], o[k], name_)
 
 
@@ -139,9 +139,9 @@ def test_render_parity_c3_views(ctx, oracle):
         ctx.render(v, d)
     for v, cam in enumerate(cams):
         d = abi.render_desc(1, 3, rp["depth"], rp["seed"] + v)
-        o = oracle.render(osc, oracle.OracleCamera(cam, rp["seed"] + v), d)
-        g = [ctx.read(v, L) for L in (abi.LAYER_BEAUTY, abi.LAYER_DIFFUSE, abi.LAYER_WORLD_NORMAL)]
-        for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
+        o = oracle.render(osc, oracle.OracleCamera(cam, rp["seed"] + v), d, depth=True)
+        g = [ctx.read(v, L) for L in (abi.LAYER_BEAUTY, abi.LAYER_DIFFUSE, abi.LAYER_WORLD_NORMAL, abi.LAYER_DEPTH)]
+        for k, name_ in enumerate(("beauty", "diffuse", "world_normal", "depth")):
             _assert_same(g[k], o[k], f"view {v} {name_}")
     assert desc.ray_depth == 16
 
@@ -157,7 +157,7 @@ def test_render_same_for_any_stream_group_count(ctx, oracle, groups):
         g, o = _render_both(ctx, oracle, s, cam, rp, 1, 4)
     finally:
         ctx.set_render_streams(2)
-    for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
+    for k, name_ in enumerate(("beauty", "diffuse", "world_normal", "depth")):
         _assert_same(g[k], o[k], name_)
 
 
@@ -173,7 +173,7 @@ def test_render_views_in_one_pipeline_equals_one_at_a_time(ctx, mode):
         cam.samples_per_pixel = 5
         ctx.set_camera(v, cam, rp["seed"] + v)
     d = abi.render_desc(p0, 5, rp["depth"], rp["seed"], mode=mode)
-    layers = (abi.LAYER_BEAUTY, abi.LAYER_DIFFUSE, abi.LAYER_WORLD_NORMAL)
+    layers = (abi.LAYER_BEAUTY, abi.LAYER_DIFFUSE, abi.LAYER_WORLD_NORMAL, abi.LAYER_DEPTH)
     single = []
     for v in range(len(cams)):
         ctx.reset(v)
@@ -220,7 +220,7 @@ def test_render_same_for_any_tuning(ctx, oracle, tuning, scene):
         g, o = _render_both(ctx, oracle, s, cam, rp, 1, 5)
     finally:
         ctx.set_tuning(4, 2, 8)
-    for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
+    for k, name_ in enumerate(("beauty", "diffuse", "world_normal", "depth")):
         _assert_same(g[k], o[k], f"{tuning} {name_}")
 
 
@@ -250,7 +250,7 @@ def test_scene_without_objects(ctx, oracle):
     cam = abi.camera_desc((20, 30), (0, 0.5, 3.0), (-5, 0, 0), 5, 2.0, 3.0, 0.0, 2.0, 1.0)
     rp = {"depth": 8, "seed": 3}
     g, o = _render_both(ctx, oracle, s, cam, rp, 1, 4)
-    for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
+    for k, name_ in enumerate(("beauty", "diffuse", "world_normal", "depth")):
         _assert_same(g[k], o[k], name_)
     ctx.reset(0)
     ctx.render(0, abi.render_desc(0, 4, rp["depth"], rp["seed"], mode=abi.MODE_FAST))
@@ -266,6 +266,6 @@ def test_degenerate_view_sizes(ctx, oracle, hw):
     s, cam, rp = scenes.mixed_scene(spp=4, res=16)
     cam.resolution[0], cam.resolution[1] = hw
     g, o = _render_both(ctx, oracle, s, cam, rp, 1, 3)
-    for k, name_ in enumerate(("beauty", "diffuse", "world_normal")):
-        assert g[k].shape == (3, hw[0], hw[1])
+    for k, name_ in enumerate(("beauty", "diffuse", "world_normal", "depth")):
+        assert g[k].shape == (1 if name_ == "depth" else 3, hw[0], hw[1])
         _assert_same(g[k], o[k], name_)

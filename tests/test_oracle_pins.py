@@ -652,3 +652,34 @@ def test_c2_mesh_tree_digest(oracle):
     got, n = c2_tree_digest(oracle)
     assert n == 1_003_520
     assert got == want[0]
+
+
+def test_depth_is_the_distance_to_the_first_hit(oracle):
+    """DEPTH (no writer in the reference; written for the denoiser): a pinhole camera at the centre of a
+    sphere of radius R sees the sphere at distance R (less the hit point's EPS offset) from every pixel,
+    whatever the jitter; rays that
+    leave through a gap in the walls see the sky and give 0; the running mean of constant samples stays
+    constant."""
+    R = 2.5
+    s = abi.SceneDesc()
+    s.sphere((0.3, -0.2, 0.1), R, abi.Material.lambertian((0.5, 0.5, 0.5)))
+    s.sky_simple((1, 1, 1), (1, 1, 1))
+    cam = abi.camera_desc((12, 20), (0.3, -0.2, 0.1), (10, 25, 0), 4, 1.5, 3.0, 0.0, 2.0, 1.0)
+    osc, oc = oracle.OracleScene(s), oracle.OracleCamera(cam, 7)
+    b, d, n, z = oracle.render(osc, oc, abi.render_desc(1, 4, 3, 7), depth=True)
+    assert z.shape == (1, 12, 20)
+    # the hit point is the reference's: the surface point moved EPS = 1e-4 along the (inward) normal
+    np.testing.assert_allclose(z, R - 1e-4, atol=2e-6)
+    # an empty scene: every camera ray misses
+    e = abi.SceneDesc()
+    e.sky_simple((1, 1, 1), (1, 1, 1))
+    _, _, _, z0 = oracle.render(oracle.OracleScene(e), oc, abi.render_desc(1, 4, 3, 7), depth=True)
+    assert not z0.any()
+    # a quad at distance 4 straight ahead: depth x cos(angle to the axis) = 4 per sample
+    q = abi.SceneDesc()
+    q.quad((0, 0, -4), (90, 0, 0), (100, 1, 100), abi.Material.lambertian((0.5, 0.5, 0.5)))
+    camq = abi.camera_desc((9, 9), (0, 0, 0), (0, 0, 0), 2, 2.0, 3.0, 0.0, 2.0, 1.0)
+    zq = oracle.render_pass(oracle.OracleScene(q), oracle.OracleCamera(camq, 1), abi.render_desc(1, 2, 2, 1), 1,
+                            depth=True)[3][0]
+    assert (zq >= 4.0 * (1 - 1e-6)).all() and (zq < 4.0 * 1.5).all()
+    assert zq[4, 4] < zq[0, 0] and zq[4, 4] < zq[8, 8]          # off-axis pixels are farther
