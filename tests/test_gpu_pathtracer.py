@@ -45,7 +45,7 @@ def _camera():
 def oracle_cornell(oracle):
     s, cam, rp = scenes.cornell(spp=SPP, res=RES)
     desc = abi.render_desc(1, SPP, rp["depth"], rp["seed"])
-    return oracle.render(oracle.OracleScene(s), oracle.OracleCamera(cam, rp["seed"]), desc)
+    return oracle.render(oracle.OracleScene(s), oracle.OracleCamera(cam, rp["seed"]), desc, depth=True)
 
 
 def _engine(seed=42, depth=5):
@@ -68,7 +68,10 @@ def test_render_matches_oracle(oracle_cornell):
     u8 = L.beauty.numpy()
     ref = np.clip(np.nan_to_num(oracle_cornell[0], nan=0.0), 0, 1) * np.float32(255)
     np.testing.assert_array_equal(u8, ref.astype(np.uint8))
-    assert not L.depth.is_enabled() and L.beauty.shape() == [3, RES, RES]
+    assert L.beauty.shape() == [3, RES, RES]
+    # the engine's layers also carry DEPTH (the reference allocates it without writing it)
+    assert L.depth.is_enabled() and L.depth.shape() == [1, RES, RES]
+    assert _same(L.depth.numpy_float(), oracle_cornell[3])
     # a second render() has nothing left to do (sample index stays at n)
     e.render()
     assert _same(e.layers().beauty.numpy_float(), oracle_cornell[0])
