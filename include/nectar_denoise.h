@@ -7,9 +7,12 @@
  * RenderEngine::RenderEngine, csrc/engine/src/engine.cpp:25-28).  OIDN's
  * network is not available; this is a U-Net of its own, trained on this
  * library's path-traced images, with BF16 MFMA convolutions (FP32
- * accumulation) for inference and training.
+ * accumulation) for inference and training.  Besides OIDN's inputs it takes
+ * the DEPTH AOV (render_layers.h:11-14; distance to the first hit, 0 where
+ * the camera ray misses).
  *
- * Network: inputs log1p(beauty), albedo, normal (9 channels, padded to 16);
+ * Network: inputs log1p(beauty), log1p(albedo), the normal clamped to
+ * [-1, 1] and log1p(depth) (10 channels, padded to 16; NaN reads as 0);
  * five levels of 32/48/64/96/128 channels, two 3x3 convolutions + ReLU per
  * level and side, 2x2 average pooling, nearest 2x upsampling, skip
  * concatenation [encoder, upsampled]; a 3x3 output convolution adds a
@@ -48,23 +51,26 @@ int nr_unet_set_params(nr_unet* net, const float* params, size_t n);
 int nr_unet_get_params(nr_unet* net, float* params, size_t n);
 
 /* Denoise N images of H x W (any size; padded internally to multiples of 16).
- * d_beauty, d_albedo, d_normal, d_out: device planar fp32, N x 3 x H x W. */
+ * d_beauty, d_albedo, d_normal, d_out: device planar fp32, N x 3 x H x W;
+ * d_depth: device fp32, N x 1 x H x W. */
 int nr_unet_forward(nr_unet* net, const float* d_beauty, const float* d_albedo, const float* d_normal,
-                    int32_t N, int32_t H, int32_t W, float* d_out);
+                    const float* d_depth, int32_t N, int32_t H, int32_t W, float* d_out);
 
 /* One training step on N crops of H x W (multiples of 16): forward, L1 loss
  * on log1p radiance against d_target, backward, Adam (beta 0.9/0.999, eps
  * 1e-8) with learning rate lr.  *loss receives the mean loss before the update. */
 int nr_unet_train_step(nr_unet* net, const float* d_beauty, const float* d_albedo, const float* d_normal,
-                       const float* d_target, int32_t N, int32_t H, int32_t W, float lr, float* loss);
+                       const float* d_depth, const float* d_target, int32_t N, int32_t H, int32_t W, float lr,
+                       float* loss);
 
 /* Loss and its gradient with respect to every parameter (host fp32,
  * nr_unet_param_count() values), without updating: the check entry point. */
 int nr_unet_gradients(nr_unet* net, const float* d_beauty, const float* d_albedo, const float* d_normal,
-                      const float* d_target, int32_t N, int32_t H, int32_t W, float* grad, float* loss);
+                      const float* d_depth, const float* d_target, int32_t N, int32_t H, int32_t W, float* grad,
+                      float* loss);
 
-/* Denoise a view's BEAUTY layer using its DIFFUSE and WORLD_NORMAL layers
- * into d_out (device planar fp32, 3 x H x W).  Waits for the view's render. */
+/* Denoise a view's BEAUTY layer using its DIFFUSE, WORLD_NORMAL and DEPTH
+ * layers into d_out (device planar fp32, 3 x H x W).  Waits for the view's render. */
 int nr_denoise(nr_ctx* ctx, int view, nr_unet* net, float* d_out);
 
 /* GPU time (ms) of the last nr_unet_forward / nr_denoise and the FLOPs it

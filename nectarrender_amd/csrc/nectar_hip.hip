@@ -3048,7 +3048,8 @@ int nr_readback(nr_ctx* c, int view, const nr_readback_desc* d, uint8_t* host) {
             // the U-Net runs on its own stream and returns when done
             HIPCK(hipEventSynchronize(c->ev_layers));
             int rc = nr_unet_forward(static_cast<nr_unet*>(d->denoiser), src, v->layer[NR_LAYER_DIFFUSE],
-                                     v->layer[NR_LAYER_WORLD_NORMAL], 1, (int32_t)v->H, (int32_t)v->W, v->d_rb_den);
+                                     v->layer[NR_LAYER_WORLD_NORMAL], v->layer[NR_LAYER_DEPTH], 1, (int32_t)v->H,
+                                     (int32_t)v->W, v->d_rb_den);
             if (rc != NR_OK) return rc;
             (void)hipSetDevice(c->device);
             src = v->d_rb_den;

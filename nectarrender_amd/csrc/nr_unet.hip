@@ -19,6 +19,7 @@
 #include <cstring>
 #include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/nectar_denoise.h"
@@ -28,6 +29,7 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
 typedef float f32x4 __attribute__((__vector_size__(4 * sizeof(float))));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));   // 16 bytes as one register quad
 
 int fail(int code, const std::string& m) { nr_error_slot() = m; return code; }
 #define UCK(expr)                                                                         \
@@ -61,18 +63,74 @@ struct View {             // element (pixel p, channel c) at p[pix * cs + off + 
 };
 
 // ---------------------------------------------------------------------------
+// network inputs
+// ---------------------------------------------------------------------------
+// Planar fp32 render layers of N images of H0 x W0: beauty, albedo, normal
+// (N x 3 x H0 x W0) and depth (N x 1 x H0 x W0).
+struct Inputs {
+    const float *b, *al, *nm, *dp;
+    int H0, W0;
+};
+
+// The 16 network input channels of pixel (n, y, x) (zero outside the image):
+// log1p(beauty), log1p(albedo), the clamped normal, log1p(depth), 6 zeros.
+// NaN samples become 0 (fmaxf returns the other operand).  Split into the ten
+// loads (from pixel 0 where the pixel lies outside, so loads never branch) and
+// the transform, so a loader can have every load in flight before it transforms.
+__device__ inline bool input_load(const Inputs& in, int n, int y, int x, float raw[10]) {
+    const bool inside = y >= 0 && x >= 0 && y < in.H0 && x < in.W0;
+    const size_t hw = (size_t)in.H0 * in.W0, pix = inside ? (size_t)y * in.W0 + x : 0;
+    const size_t base = (size_t)(inside ? n : 0) * 3 * hw + pix;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        raw[c] = in.b[base + c * hw];
+        raw[3 + c] = in.al[base + c * hw];
+        raw[6 + c] = in.nm[base + c * hw];
+    }
+    raw[9] = in.dp[(size_t)(inside ? n : 0) * hw + pix];
+    return inside;
+}
+__device__ inline void input_transform(const float raw[10], bool inside, float f[16]) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) f[k] = 0.0f;
+    if (!inside) return;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        f[c] = log1pf(fmaxf(raw[c], 0.0f));
+        f[3 + c] = log1pf(fmaxf(raw[3 + c], 0.0f));
+        const float nv = raw[6 + c];
+        f[6 + c] = (nv == nv) ? fminf(fmaxf(nv, -1.0f), 1.0f) : 0.0f;
+    }
+    f[9] = log1pf(fmaxf(raw[9], 0.0f));
+}
+__device__ inline void input_pixel(const Inputs& in, int n, int y, int x, float f[16]) {
+    float raw[10];
+    const bool inside = input_load(in, n, y, x, raw);
+    input_transform(raw, inside, f);
+}
+
+// ---------------------------------------------------------------------------
 // convolution (forward and input gradient)
 // ---------------------------------------------------------------------------
-// LDS-tiled variant: a workgroup computes 4 rows x 64 pixels x (16 TM) output
-// channels.  For each chunk of 32 input channels the 6 x 66-pixel halo tile is
-// staged in LDS once and feeds all 9 taps (each input element is read from
-// memory ~1.5 times instead of 9); weights are [rows][tap][channels padded to
-// 32], so one K step of 32 is one tap of one chunk.  The tile is stored
-// [row][channel group][column]: the 16 lanes that share a channel group read 16
-// consecutive 16-byte columns, one contiguous 256-byte run with no bank conflict.
+// LDS-tiled: a workgroup (ROWS waves) computes ROWS rows x 64 pixels x (16 TM)
+// output channels.  For each chunk of 32 input channels the (ROWS + 2) x 66-pixel
+// halo tile and the chunk's weights of the workgroup's output channels are
+// staged in LDS once and feed all 9 taps.  With several chunks (MC) the next
+// chunk's tile and weights are loaded into registers while this chunk's MFMAs
+// run, so a workgroup waits on memory once per chunk, not once per tap; a
+// single-chunk layer holds no such registers (more waves per SIMD).  The tile is
+// stored [row][8-channel group][column]: the 16 lanes that share a channel group
+// read 16 consecutive 16-byte columns, one contiguous 256-byte run with no bank
+// conflict.  The weights are stored [tap][8-channel group][output channel], so
+// the 16 lanes that share a channel group read 16 consecutive rows, again one
+// contiguous 256-byte run.  Global weights are [rows][9][c32].
+enum { LD_PLAIN = 0, LD_UP = 1, LD_INPUT = 2 };      // tile loader
+enum { EP_PLAIN = 0, EP_POOL = 1, EP_OUT = 2 };      // epilogue
+
 struct ConvLdsArgs {
     const uint16_t* in; int in_cs, in_off, cin_avail;   // channels present in the input tensor
-    const uint16_t* up; int up_cs, up_off, up_c0;      // channels >= up_c0: nearest 2x upsample of a (H/2 x W/2) tensor
+    const uint16_t* up; int up_cs, up_off, up_c0;      // LD_UP: channels >= up_c0 from the nearest 2x upsample of an (H/2 x W/2) tensor
+    Inputs src;                                        // LD_INPUT: the planar render layers; EP_OUT: beauty for the residual
     int c32;                                           // padded channels per tap of the weights
     int N, H, W;
     const uint16_t* w;                                 // [rows][9][c32]
@@ -80,9 +138,10 @@ struct ConvLdsArgs {
     int cout;
     int relu;
     uint16_t* out; int out_cs, out_off;
-    uint16_t* pool; int pool_cs, pool_off;             // also store the 2x2 average of the stored values (H, W even)
+    uint16_t* pool; int pool_cs, pool_off;             // EP_POOL: also the 2x2 average of the stored values (H, W even)
     const uint16_t* mask; int mask_cs, mask_off;
-    float* out_f32; const float* resid;
+    float* out_f32;                                    // EP_OUT: log-space result (N x H x W x 4), else
+    float* out_final;                                  // EP_OUT: expm1(max(y, 0)), planar N x 3 x H0 x W0
 };
 
 // value of the lane next to this one (lane ^ 1), through DPP quad_perm [1,0,3,2]
@@ -90,19 +149,30 @@ __device__ inline float lane_xor1(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
 }
 
-constexpr int CL_TILE_W = 64, CL_TILE_H = 4;
+constexpr int CL_TILE_W = 64;
 
-// UP: the loader reads channels >= up_c0 from the upsampled tensor; POOL: the
-// fused-pool epilogue (forward encoder output).  Both are compile-time so the
-// plain convolutions carry neither's registers.
-template <int TM, bool UP, bool POOL>
-__global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
-    __shared__ uint4 tile[CL_TILE_H + 2][4][CL_TILE_W + 2];   // [row][8-channel group][column]
+// LD: which tile loader (LD_UP: the decoder's skip + upsampled channels; LD_INPUT:
+// the network input computed from the render layers, one chunk).  EP: the
+// epilogue (EP_POOL: the encoder's fused 2x2 average pool; EP_OUT: the output
+// convolution, residual added).  ROWS: output rows (waves) per workgroup.  MC:
+// more than one chunk (prefetch the next one).  All compile-time, so each variant
+// carries only its own registers.
+// Waves per SIMD the LDS allows (two 8-wave or three 4-wave workgroups per CU); the
+// register budget is held to it, except that 48 output channels per workgroup get two
+// waves' worth of registers (they would spill at three).
+template <int TM, int LD, int EP, int ROWS, bool MC>
+__global__ void __launch_bounds__(64 * ROWS) __attribute__((amdgpu_waves_per_eu(ROWS == 8 ? 4 : TM == 3 ? 2 : 3)))
+k_conv_lds(ConvLdsArgs a) {
+    constexpr int NT = 64 * ROWS;                                // threads
+    __shared__ u32x4 tile[ROWS + 2][4][CL_TILE_W + 2];           // [row][8-channel group][column]
+    constexpr int NW = 9 * 16 * TM * 4;                          // 16-byte weight entries per chunk
+    constexpr int NWL = (NW + NT - 1) / NT;
+    __shared__ u32x4 wl[NW];                                     // [tap][8-channel group][output channel]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, kq = lane >> 4;
     const int x0 = blockIdx.x * CL_TILE_W;
-    const int rowblocks = (a.H + CL_TILE_H - 1) / CL_TILE_H;
-    const int n = blockIdx.y / rowblocks, y0 = (blockIdx.y - n * rowblocks) * CL_TILE_H;
+    const int rowblocks = (a.H + ROWS - 1) / ROWS;
+    const int n = blockIdx.y / rowblocks, y0 = (blockIdx.y - n * rowblocks) * ROWS;
     const int y = y0 + wave;
     const int cobase = blockIdx.z * 16 * TM;
     f32x4 acc[TM][4];
@@ -110,60 +180,200 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
     for (int m = 0; m < TM; m++)
 #pragma unroll
         for (int t = 0; t < 4; t++) acc[m][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int nchunks = a.c32 >> 5;
-    constexpr int NV = (CL_TILE_H + 2) * (CL_TILE_W + 2) * 4;
-    for (int c = 0; c < nchunks; c++) {
-        // all of this thread's tile loads in flight at once, then the LDS stores
-        constexpr int NLD = (NV + 255) / 256;
-        uint4 val[NLD];
+    // this lane's output-channel biases, loaded with the first chunk
+    float bias[TM][4];
+#pragma unroll
+    for (int m = 0; m < TM; m++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int co = cobase + m * 16 + kq * 4 + j;
+            bias[m][j] = (a.bias && co < a.cout) ? a.bias[co] : 0.0f;
+        }
+    const int nchunks = MC ? (a.c32 >> 5) : 1;
+    constexpr int NV = (ROWS + 2) * (CL_TILE_W + 2) * 4;        // 16-byte tile entries
+    constexpr int NLD = (NV + NT - 1) / NT;
+    // LD_UP chunks made only of upsampled channels: the coarse tile, in the fine tile's memory
+    constexpr int CROWS = ROWS / 2 + 2, CCOLS = CL_TILE_W / 2 + 2;
+    constexpr int NVC = CROWS * CCOLS * 4;
+    constexpr int NLDC = (NVC + NT - 1) / NT;
+    static_assert(CROWS * CCOLS <= (ROWS + 2) * (CL_TILE_W + 2), "coarse tile fits the fine one");
+    auto ctile = reinterpret_cast<u32x4 (*)[4][CCOLS]>(&tile[0][0][0]);
+    constexpr int NP = (ROWS + 2) * (CL_TILE_W + 2);             // tile pixels
+    constexpr int NPI = LD == LD_INPUT ? (NP + NT - 1) / NT : 1;  // LD_INPUT: tile pixels per thread
+    u32x4 val[LD == LD_INPUT ? 1 : NLD], wv[NWL];
+    u32x4 cval[LD == LD_UP ? NLDC : 1];                          // LD_UP: a coarse chunk's loads
+    float raw[NPI][10];                                          // LD_INPUT: the render-layer values
+    uint32_t okm = 0u;                                           // val[k] / raw[k] lies inside the image
+    // EP_OUT: the residual's beauty values of this lane's pixels (used by lanes kq = 0)
+    float resid[EP == EP_OUT ? 4 : 1][3];
+    if constexpr (EP == EP_OUT) {
+        const Inputs& sr = a.src;
+        const size_t hw0 = (size_t)sr.H0 * sr.W0;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int x = x0 + t * 16 + r;
+            const bool inside = y < sr.H0 && x < sr.W0;
+            const size_t q = inside ? (size_t)n * 3 * hw0 + (size_t)y * sr.W0 + x : 0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) resid[t][j] = sr.b[q + j * hw0];
+        }
+    }
+    // A chunk's weights: global -> registers (wv) -> LDS (wl).  Every load and store is
+    // unconditional: a thread past entry NW loads and stores entry NW - 1 again, the same
+    // value to the same place as its owner.
+    auto load_w = [&](int c) {
+#pragma unroll
+        for (int k = 0; k < NWL; k++) {
+            const int v = min((int)threadIdx.x + k * NT, NW - 1);
+            const int q = v & 3, rt = v >> 2;
+            const int row = rt / 9, tap = rt - row * 9;
+            wv[k] = *reinterpret_cast<const u32x4*>(a.w + ((size_t)(cobase + row) * 9 + tap) * a.c32 + c * 32 + q * 8);
+        }
+    };
+    auto store_w = [&]() {
+#pragma unroll
+        for (int k = 0; k < NWL; k++) {
+            const int v = min((int)threadIdx.x + k * NT, NW - 1);
+            const int q = v & 3, rt = v >> 2;
+            const int row = rt / 9, tap = rt - row * 9;
+            wl[(tap * 4 + q) * 16 * TM + row] = wv[k];
+        }
+    };
+    // The fine tile of chunk c (input-tensor channels; LD_UP: channels from up_c0 on read
+    // the coarse tensor at (y/2, x/2)).  Branch-free: entries outside the image (or past
+    // the tile) load the tensor's first 16 bytes and are stored as zeros (okm), so no load
+    // waits under divergent control flow.
+    auto load_fine = [&](int c) {
 #pragma unroll
         for (int k = 0; k < NLD; k++) {
-            const int v = threadIdx.x + k * 256;
+            const int v = threadIdx.x + k * NT;
             const int g = v & 3, px = v >> 2;
             const int row = px / (CL_TILE_W + 2), col = px - row * (CL_TILE_W + 2);
             const int yy = y0 - 1 + row, xx = x0 - 1 + col, ch = c * 32 + g * 8;
-            val[k] = make_uint4(0u, 0u, 0u, 0u);
-            if (v < NV && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && ch < a.cin_avail) {
-                const uint16_t* src =
-                    (UP && ch >= a.up_c0)
-                        ? a.up + ((size_t)(n * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.up_cs + a.up_off + (ch - a.up_c0)
-                        : a.in + ((size_t)(n * a.H + yy) * a.W + xx) * a.in_cs + a.in_off + ch;
-                val[k] = *reinterpret_cast<const uint4*>(src);
-            }
+            const bool ok = v < NV && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W && ch < a.cin_avail;
+            const uint16_t* src =
+                !ok ? a.in
+                    : (LD == LD_UP && ch >= a.up_c0)
+                          ? a.up + ((size_t)(n * (a.H >> 1) + (yy >> 1)) * (a.W >> 1) + (xx >> 1)) * a.up_cs + a.up_off + (ch - a.up_c0)
+                          : a.in + ((size_t)(n * a.H + yy) * a.W + xx) * a.in_cs + a.in_off + ch;
+            val[k] = *reinterpret_cast<const u32x4*>(src);
+            okm = ok ? (okm | (1u << k)) : (okm & ~(1u << k));
         }
-        __syncthreads();
+    };
+    auto store_fine = [&]() {
 #pragma unroll
         for (int k = 0; k < NLD; k++) {
-            const int v = threadIdx.x + k * 256;
+            const int v = threadIdx.x + k * NT;
             const int g = v & 3, px = v >> 2;
             const int row = px / (CL_TILE_W + 2), col = px - row * (CL_TILE_W + 2);
-            if (v < NV) tile[row][g][col] = val[k];
+            if (v < NV) tile[row][g][col] = ((okm >> k) & 1u) ? val[k] : u32x4{0u, 0u, 0u, 0u};
         }
-        __syncthreads();
+    };
+    // LD_UP, a chunk made only of upsampled channels: its (ROWS/2 + 2) x 34 coarse tile
+    // (a quarter of the loads), in the fine tile's memory
+    auto load_coarse = [&](int c) {
+#pragma unroll
+        for (int k = 0; k < NLDC; k++) {
+            const int v = threadIdx.x + k * NT;
+            const int g = v & 3, cp = v >> 2;
+            const int crow = cp / CCOLS, ccol = cp - crow * CCOLS;
+            const int cy = (y0 >> 1) - 1 + crow, cx = (x0 >> 1) - 1 + ccol, ch = c * 32 + g * 8;
+            const bool ok = v < NVC && cy >= 0 && cy < (a.H >> 1) && cx >= 0 && cx < (a.W >> 1) && ch < a.cin_avail;
+            const uint16_t* src =
+                !ok ? a.in : a.up + ((size_t)(n * (a.H >> 1) + cy) * (a.W >> 1) + cx) * a.up_cs + a.up_off + (ch - a.up_c0);
+            cval[k] = *reinterpret_cast<const u32x4*>(src);
+            okm = ok ? (okm | (1u << k)) : (okm & ~(1u << k));
+        }
+    };
+    auto store_coarse = [&]() {
+#pragma unroll
+        for (int k = 0; k < NLDC; k++) {
+            const int v = threadIdx.x + k * NT;
+            const int g = v & 3, cp = v >> 2;
+            const int crow = cp / CCOLS, ccol = cp - crow * CCOLS;
+            if (v < NVC) ctile[crow][g][ccol] = ((okm >> k) & 1u) ? cval[k] : u32x4{0u, 0u, 0u, 0u};
+        }
+    };
+    // LD_INPUT (one chunk): the render-layer values of this thread's tile pixels, then
+    // the network input channels (groups 0-1; groups 2-3 are zero)
+    auto load_input = [&]() {
+#pragma unroll
+        for (int k = 0; k < NPI; k++) {
+            const int px = min((int)threadIdx.x + k * NT, NP - 1);
+            const int row = px / (CL_TILE_W + 2), col = px - row * (CL_TILE_W + 2);
+            const int yy = y0 - 1 + row, xx = x0 - 1 + col;
+            const bool ok = yy < a.H && xx < a.W && input_load(a.src, n, yy, xx, raw[k]);
+            okm = ok ? (okm | (1u << k)) : (okm & ~(1u << k));
+        }
+    };
+    auto store_input = [&]() {
+#pragma unroll
+        for (int k = 0; k < NPI; k++) {
+            const int px = threadIdx.x + k * NT;
+            if (px >= NP) continue;
+            const int row = px / (CL_TILE_W + 2), col = px - row * (CL_TILE_W + 2);
+            float f[16];
+            input_transform(raw[k], (okm >> k) & 1u, f);
+            tile[row][0][col] = u32x4{pk_bf16(f[0], f[1]), pk_bf16(f[2], f[3]), pk_bf16(f[4], f[5]), pk_bf16(f[6], f[7])};
+            tile[row][1][col] = u32x4{pk_bf16(f[8], f[9]), pk_bf16(f[10], f[11]), pk_bf16(f[12], f[13]),
+                                      pk_bf16(f[14], f[15])};
+            tile[row][2][col] = u32x4{0u, 0u, 0u, 0u};
+            tile[row][3][col] = u32x4{0u, 0u, 0u, 0u};
+        }
+    };
+    // The 9 taps of one staged chunk.  A fine tile position (row, col) of a coarse
+    // chunk is the coarse tile's ((row + 1) / 2, (col + 1) / 2).
+    auto mfma_chunk = [&](auto coarse_tag) {
+        constexpr bool CO = decltype(coarse_tag)::value;
 #pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap - 3 * (tap / 3);
             bf16x8 b[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++) b[t] = __builtin_bit_cast(bf16x8, tile[wave + dy][kq][t * 16 + r + dx]);
+            for (int t = 0; t < 4; t++) {
+                if constexpr (CO) b[t] = __builtin_bit_cast(bf16x8, ctile[(wave + dy + 1) >> 1][kq][(t * 16 + r + dx + 1) >> 1]);
+                else b[t] = __builtin_bit_cast(bf16x8, tile[wave + dy][kq][t * 16 + r + dx]);
+            }
 #pragma unroll
             for (int m = 0; m < TM; m++) {
-                const int co = cobase + m * 16 + r;
-                uint4 wv = *reinterpret_cast<const uint4*>(a.w + ((size_t)co * 9 + tap) * a.c32 + c * 32 + kq * 8);
-                bf16x8 af = __builtin_bit_cast(bf16x8, wv);
+                const bf16x8 af = __builtin_bit_cast(bf16x8, wl[(tap * 4 + kq) * 16 * TM + m * 16 + r]);
 #pragma unroll
                 for (int t = 0; t < 4; t++)
                     acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b[t], acc[m][t], 0, 0, 0);
             }
         }
+    };
+    auto run = [&](auto coarse_tag, int cb, int ce) {
+        constexpr bool CO = decltype(coarse_tag)::value;
+        for (int c = cb; c < ce; c++) {
+            load_w(c);
+            if constexpr (LD == LD_INPUT) load_input();
+            else if constexpr (CO) load_coarse(c);
+            else load_fine(c);
+            if (c > 0) __syncthreads();        // every wave is done with the previous chunk
+            store_w();
+            if constexpr (LD == LD_INPUT) store_input();
+            else if constexpr (CO) store_coarse();
+            else store_fine();
+            __syncthreads();
+            mfma_chunk(coarse_tag);
+        }
+    };
+    if constexpr (LD == LD_UP) {
+        // the chunks holding any skip channel, then those made only of upsampled ones
+        const int nfine = min(nchunks, (a.up_c0 + 31) >> 5);
+        run(std::false_type{}, 0, nfine);
+        run(std::true_type{}, nfine, nchunks);
+    } else {
+        run(std::false_type{}, 0, nchunks);
     }
-    if constexpr (POOL) {
+    if constexpr (EP == EP_POOL) {
         // Forward encoder output: bias + ReLU, bf16 store, and the 2x2 average pool of
         // the stored values.  The horizontal pair is lanes r, r^1; the vertical pair is
         // waves 2i, 2i+1 (tile rows are 4-aligned), exchanged through the (no longer
         // needed) input tile.  H and W are even, so a pair is entirely in or out.
         const bool rowv = y < a.H;
-        float* xch = reinterpret_cast<float*>(&tile[0][0][0]);   // [2][16][32] floats per m
+        float* xch = reinterpret_cast<float*>(&tile[0][0][0]);   // [ROWS/2][16][32] floats per m
 #pragma unroll
         for (int m = 0; m < TM; m++) {
             const int co0 = cobase + m * 16 + kq * 4;
@@ -174,7 +384,7 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
                 float v[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    float xv = acc[m][t][j] + a.bias[co0 + j];
+                    float xv = acc[m][t][j] + bias[m][j];
                     v[j] = a.relu ? fmaxf(xv, 0.0f) : xv;
                 }
                 uint2 pk;
@@ -214,6 +424,32 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
         return;
     }
     if (y >= a.H) return;
+    if constexpr (EP == EP_OUT) {
+        // output convolution (channels 0-2 of the first 16-row tile, lanes kq = 0): the
+        // residual log1p(beauty) added in fp32; either the log-space value for the loss
+        // (out_f32, padded dims, 4-stride) or the final radiance expm1(max(y, 0)) into the
+        // planar image of the unpadded size
+        if (kq != 0) return;
+        const Inputs& s = a.src;
+        const size_t hw0 = (size_t)s.H0 * s.W0;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int x = x0 + t * 16 + r;
+            if (x >= a.W) continue;
+            const bool inside = y < s.H0 && x < s.W0;
+            const size_t p = ((size_t)n * a.H + y) * a.W + x;
+            const size_t q = (size_t)n * 3 * hw0 + (size_t)y * s.W0 + x;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const float res = inside ? log1pf(fmaxf(resid[t][j], 0.0f)) : 0.0f;
+                const float v = acc[0][t][j] + bias[0][j] + res;
+                if (a.out_f32) a.out_f32[p * 4 + j] = v;
+                else if (inside) a.out_final[q + j * hw0] = expm1f(fmaxf(v, 0.0f));
+            }
+            if (a.out_f32) a.out_f32[p * 4 + 3] = 0.0f;
+        }
+        return;
+    }
 #pragma unroll
     for (int m = 0; m < TM; m++)
 #pragma unroll
@@ -226,27 +462,20 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvLdsArgs a) {
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int co = co0 + j;
-                float xv = acc[m][t][j] + ((a.bias && co < a.cout) ? a.bias[co] : 0.0f);
+                float xv = acc[m][t][j] + bias[m][j];
                 if (a.relu) xv = fmaxf(xv, 0.0f);
                 if (a.mask && co < a.cout && !(bf2f(a.mask[p * a.mask_cs + a.mask_off + co]) > 0.0f)) xv = 0.0f;
                 v[j] = xv;
             }
-            if (a.out_f32) {
-                for (int j = 0; j < 4; j++) {
-                    const int co = co0 + j;
-                    if (co < a.cout && co < 4) a.out_f32[p * 4 + co] = v[j] + a.resid[p * 4 + co];
-                }
+            uint16_t* o = a.out + p * a.out_cs + a.out_off + co0;
+            if (co0 + 3 < a.cout) {
+                uint2 pk;
+                pk.x = pk_bf16(v[0], v[1]);
+                pk.y = pk_bf16(v[2], v[3]);
+                *reinterpret_cast<uint2*>(o) = pk;
             } else {
-                uint16_t* o = a.out + p * a.out_cs + a.out_off + co0;
-                if (co0 + 3 < a.cout) {
-                    uint2 pk;
-                    pk.x = pk_bf16(v[0], v[1]);
-                    pk.y = pk_bf16(v[2], v[3]);
-                    *reinterpret_cast<uint2*>(o) = pk;
-                } else {
-                    for (int j = 0; j < 4; j++)
-                        if (co0 + j < a.cout) o[j] = f2bf(v[j]);
-                }
+                for (int j = 0; j < 4; j++)
+                    if (co0 + j < a.cout) o[j] = f2bf(v[j]);
             }
         }
 }
@@ -389,31 +618,20 @@ __global__ void k_up_bwd(View g, View mask, View d, int C, int N, int Ho, int Wo
     st8(d.p + q * d.cs + d.off + c, o);
 }
 
-// planar N x 3 x H x W inputs -> X0 (16 channels, padded dims Hp x Wp) and log1p beauty (4-stride f32)
-__global__ void k_prep(const float* beauty, const float* albedo, const float* normal, int N, int H, int W, int Hp,
-                       int Wp, uint16_t* x0, float* xlog) {
+// the network input of N images (padded dims Hp x Wp) as the 16-channel bf16 X0; the
+// first convolution of the inference forward computes the same values in its loader
+__global__ void k_prep(Inputs in, int N, int Hp, int Wp, uint16_t* x0) {
     const size_t total = (size_t)N * Hp * Wp;
     size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= total) return;
     const int x = (int)(p % Wp), y = (int)((p / Wp) % Hp), n = (int)(p / ((size_t)Wp * Hp));
     float f[16];
-    for (int k = 0; k < 16; k++) f[k] = 0.0f;
-    if (x < W && y < H) {
-        const size_t hw = (size_t)H * W, base = (size_t)n * 3 * hw + (size_t)y * W + x;
-        for (int c = 0; c < 3; c++) {
-            float b = beauty[base + c * hw];
-            f[c] = log1pf(fmaxf(b, 0.0f));          // NaN samples become 0
-            f[3 + c] = log1pf(fmaxf(albedo[base + c * hw], 0.0f));
-            float nv = normal[base + c * hw];
-            f[6 + c] = (nv == nv) ? fminf(fmaxf(nv, -1.0f), 1.0f) : 0.0f;
-        }
-    }
+    input_pixel(in, n, y, x, f);
     uint32_t pk[8];
     for (int k = 0; k < 8; k++) pk[k] = pk_bf16(f[2 * k], f[2 * k + 1]);
     uint4* o = reinterpret_cast<uint4*>(x0 + p * 16);
     o[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
     o[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
-    for (int c = 0; c < 4; c++) xlog[p * 4 + c] = c < 3 ? f[c] : 0.0f;
 }
 
 __global__ void k_target(const float* t, int N, int H, int W, float* tlog) {
@@ -452,16 +670,6 @@ __global__ void __launch_bounds__(256) k_loss(const float* y, const float* t, in
         __syncthreads();
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
-}
-
-__global__ void k_output(const float* y, int N, int H, int W, int Hp, int Wp, float* out) {
-    const size_t total = (size_t)N * H * W;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int x = (int)(i % W), yy = (int)((i / W) % H), n = (int)(i / ((size_t)W * H));
-    const size_t p = ((size_t)n * Hp + yy) * Wp + x;
-    const size_t hw = (size_t)H * W, base = (size_t)n * 3 * hw + (size_t)yy * W + x;
-    for (int c = 0; c < 3; c++) out[base + c * hw] = expm1f(fmaxf(y[p * 4 + c], 0.0f));
 }
 
 __global__ void k_adam(float* prm, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
@@ -507,7 +715,7 @@ inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b
 // ---------------------------------------------------------------------------
 constexpr int LV = 5;
 constexpr int CH[LV] = {32, 48, 64, 96, 128};
-constexpr int IN_CH = 9, IN_PAD = 16;
+constexpr int IN_CH = 10, IN_PAD = 16;
 
 struct Layer {
     int cin, cout;            // actual channels of the parameters
@@ -570,7 +778,7 @@ struct nr_unet {
     // X: level input, A: first conv, E: encoder output (skip), DA/DO: decoder convs
     uint16_t *X[LV] = {}, *A[LV] = {}, *E[LV] = {}, *DA[LV] = {}, *DO[LV] = {};
     uint16_t *gX[LV] = {}, *gA[LV] = {}, *gE[LV] = {}, *gD[LV] = {}, *gDA[LV] = {}, *gDO[LV] = {}, *gY = nullptr;
-    float *xlog = nullptr, *y = nullptr, *tlog = nullptr, *partial = nullptr;
+    float *y = nullptr, *tlog = nullptr, *partial = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float last_ms = 0.0f;
     double last_flops = 0.0;
@@ -618,7 +826,6 @@ int ensure_shape(nr_unet* u, int N, int H, int W) {
     }
     const size_t P0 = (size_t)N * H * W;
     want((void**)&u->gY, P0 * 16 * 2);
-    want((void**)&u->xlog, P0 * 4 * 4);
     want((void**)&u->y, P0 * 4 * 4);
     want((void**)&u->tlog, P0 * 4 * 4);
     want((void**)&u->partial, (P0 / 256 + 1) * 4);
@@ -636,16 +843,38 @@ int ensure_shape(nr_unet* u, int N, int H, int W) {
 
 double conv_flops(const Layer& l, size_t P) { return 2.0 * P * 9.0 * l.cin * l.cout; }
 
-// `up` (may be null): input channels from cin_groups*8 on are the nearest 2x
-// upsample of this H/2 x W/2 tensor.  `pool` (may be null): also write the 2x2
-// average of the (ReLU, bf16) output there.
+// What a convolution reads besides its input tensor and writes besides its output.
+struct ConvOpt {
+    View up{nullptr, 0, 0};        // input channels from cin_groups*8 on: the nearest 2x upsample of this H/2 x W/2 tensor
+    int up_groups = 0;
+    View pool{nullptr, 0, 0};      // also write the 2x2 average of the (ReLU, bf16) output here
+    View mask{nullptr, 0, 0};      // input gradient: zero where the forward activation is not positive
+    const Inputs* input = nullptr; // first layer: compute the tile from these render layers
+    const Inputs* resid = nullptr; // output layer: add log1p(beauty) of these layers; write out_f32 or out_final
+    float* out_f32 = nullptr;
+    float* out_final = nullptr;
+};
+
+template <int TM, int LD, int EP>
+void launch_conv_shape(int rows, bool mc, dim3 grid, hipStream_t st, const ConvLdsArgs& a) {
+    constexpr bool single = LD == LD_INPUT || EP == EP_OUT;     // one chunk by construction
+    if (rows == 8) {
+        if (mc && !single) hipLaunchKernelGGL((k_conv_lds<TM, LD, EP, 8, !single>), grid, dim3(512), 0, st, a);
+        else hipLaunchKernelGGL((k_conv_lds<TM, LD, EP, 8, false>), grid, dim3(512), 0, st, a);
+    } else {
+        if (mc && !single) hipLaunchKernelGGL((k_conv_lds<TM, LD, EP, 4, !single>), grid, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_conv_lds<TM, LD, EP, 4, false>), grid, dim3(256), 0, st, a);
+    }
+}
+
 void launch_conv(nr_unet* u, const Layer& l, bool backward, View in, int cin_groups, int N, int H, int W, View out,
-                 int cout, bool relu, const float* bias, View mask, float* out_f32, const float* resid,
-                 View up = View{nullptr, 0, 0}, int up_groups = 0, View pool = View{nullptr, 0, 0}) {
+                 int cout, bool relu, const float* bias, const ConvOpt& o = ConvOpt{}) {
     ConvLdsArgs a{};
-    a.in = in.p; a.in_cs = in.cs; a.in_off = in.off; a.cin_avail = (cin_groups + up_groups) * 8;
-    a.up = up.p; a.up_cs = up.cs; a.up_off = up.off; a.up_c0 = cin_groups * 8;
-    a.pool = pool.p; a.pool_cs = pool.cs; a.pool_off = pool.off;
+    a.in = in.p; a.in_cs = in.cs; a.in_off = in.off; a.cin_avail = (cin_groups + o.up_groups) * 8;
+    a.up = o.up.p; a.up_cs = o.up.cs; a.up_off = o.up.off; a.up_c0 = cin_groups * 8;
+    a.pool = o.pool.p; a.pool_cs = o.pool.cs; a.pool_off = o.pool.off;
+    if (o.input) a.src = *o.input;
+    if (o.resid) a.src = *o.resid;
     a.c32 = backward ? l.bc32 : l.fc32;
     a.N = N; a.H = H; a.W = W;
     a.w = backward ? l.wb2 : l.wf2;
@@ -653,11 +882,16 @@ void launch_conv(nr_unet* u, const Layer& l, bool backward, View in, int cin_gro
     a.cout = cout;
     a.relu = relu;
     a.out = out.p; a.out_cs = out.cs; a.out_off = out.off;
-    a.mask = mask.p; a.mask_cs = mask.cs; a.mask_off = mask.off;
-    a.out_f32 = out_f32; a.resid = resid;
+    a.mask = o.mask.p; a.mask_cs = o.mask.cs; a.mask_off = o.mask.off;
+    a.out_f32 = o.out_f32; a.out_final = o.out_final;
     const int cpad = up16(cout);
-    int TM = (cpad % 64 == 0) ? 4 : (cpad % 48 == 0) ? 3 : (cpad % 32 == 0) ? 2 : 1;
-    const int64_t tiles = (int64_t)((W + CL_TILE_W - 1) / CL_TILE_W) * (N * ((H + CL_TILE_H - 1) / CL_TILE_H));
+    int TM = (cpad % 48 == 0) ? 3 : (cpad % 32 == 0) ? 2 : 1;
+    if (o.resid) TM = 1;                       // three output channels: one 16-row tile
+    const int64_t cols = (W + CL_TILE_W - 1) / CL_TILE_W;
+    // 8 rows (waves) per workgroup where the layer still has at least 2048 workgroups
+    // (full resolution: fewer halo rows per output row, more waves per CU); else 4
+    const int rows = cols * N * ((H + 7) / 8) * (cpad / (16 * TM)) >= 2048 ? 8 : 4;
+    const int64_t tiles = cols * N * ((H + rows - 1) / rows);
     // coarse levels have few pixel tiles: fewer output channels per workgroup
     // (same arithmetic per channel) until the launch has about 4 workgroups per CU
     while (TM > 1 && tiles * (cpad / (16 * TM)) < 1024) {
@@ -665,22 +899,29 @@ void launch_conv(nr_unet* u, const Layer& l, bool backward, View in, int cin_gro
         while (t > 1 && (cpad / 16) % t != 0) t--;
         TM = t;
     }
-    dim3 grid((W + CL_TILE_W - 1) / CL_TILE_W, N * ((H + CL_TILE_H - 1) / CL_TILE_H), cpad / (16 * TM));
-    dim3 block(256);
-    const int mode = a.pool ? 2 : a.up ? 1 : 0;
-#define NR_CONV_CASE(T)                                                                                   \
-    case T:                                                                                               \
-        if (mode == 2) hipLaunchKernelGGL((k_conv_lds<T, false, true>), grid, block, 0, u->stream, a);     \
-        else if (mode == 1) hipLaunchKernelGGL((k_conv_lds<T, true, false>), grid, block, 0, u->stream, a); \
-        else hipLaunchKernelGGL((k_conv_lds<T, false, false>), grid, block, 0, u->stream, a);              \
+    const bool mc = a.c32 > 32;
+    dim3 grid((unsigned)cols, (unsigned)(N * ((H + rows - 1) / rows)), cpad / (16 * TM));
+    hipStream_t st = u->stream;
+    if (o.resid) {
+        launch_conv_shape<1, LD_PLAIN, EP_OUT>(rows, false, grid, st, a);
+    } else if (o.input) {
+        if (TM == 2) launch_conv_shape<2, LD_INPUT, EP_PLAIN>(rows, false, grid, st, a);
+        else launch_conv_shape<1, LD_INPUT, EP_PLAIN>(rows, false, grid, st, a);
+    } else {
+        const int mode = a.pool ? 2 : a.up ? 1 : 0;
+#define NR_CONV_CASE(T)                                                                \
+    case T:                                                                            \
+        if (mode == 2) launch_conv_shape<T, LD_PLAIN, EP_POOL>(rows, mc, grid, st, a);  \
+        else if (mode == 1) launch_conv_shape<T, LD_UP, EP_PLAIN>(rows, mc, grid, st, a); \
+        else launch_conv_shape<T, LD_PLAIN, EP_PLAIN>(rows, mc, grid, st, a);          \
         break;
-    switch (TM) {
-        NR_CONV_CASE(4)
-        NR_CONV_CASE(3)
-        NR_CONV_CASE(2)
-        default: NR_CONV_CASE(1)
-    }
+        switch (TM) {
+            NR_CONV_CASE(3)
+            NR_CONV_CASE(2)
+            default: NR_CONV_CASE(1)
+        }
 #undef NR_CONV_CASE
+    }
 }
 
 void launch_wgrad(nr_unet* u, const Layer& l, View x, int cin, View g, int cout, int N, int H, int W,
@@ -707,43 +948,55 @@ void launch_wgrad(nr_unet* u, const Layer& l, View x, int cin, View g, int cout,
                        u->grad + l.b_off);
 }
 
-void forward(nr_unet* u, int N, int H, int W) {
+// The forward pass.  Training (`in` null): the first layer reads X0, which k_prep
+// wrote for the weight gradient, and the output layer writes the log-space result
+// to y for the loss.  Inference (`in` given): the first layer computes its input
+// tile from the render layers and the output layer writes the final radiance to
+// `final`.  Both read the same bf16 inputs and compute the same values.
+void forward(nr_unet* u, int N, int H, int W, const Inputs& src, bool fused_input, float* final_out) {
     Net& n = u->net;
-    const View none{nullptr, 0, 0};
     for (int l = 0; l < LV; l++) {
         const int h = H >> l, w = W >> l;
         const int cin_t = l == 0 ? IN_PAD : CH[l - 1];
         View x{u->X[l], cin_t, 0}, a{u->A[l], CH[l], 0}, e{u->E[l], CH[l], 0};
-        View xn = l < LV - 1 ? View{u->X[l + 1], CH[l], 0} : none;
         Layer& la = n.enc(l, 0);
         Layer& lb = n.enc(l, 1);
-        launch_conv(u, la, false, x, cin_t / 8, N, h, w, a, CH[l], true, u->prm + la.b_off, none, nullptr, nullptr);
-        launch_conv(u, lb, false, a, CH[l] / 8, N, h, w, e, CH[l], true, u->prm + lb.b_off, none, nullptr, nullptr,
-                    none, 0, xn);
+        ConvOpt oa;
+        if (l == 0 && fused_input) oa.input = &src;
+        launch_conv(u, la, false, x, cin_t / 8, N, h, w, a, CH[l], true, u->prm + la.b_off, oa);
+        ConvOpt ob;
+        if (l < LV - 1) ob.pool = View{u->X[l + 1], CH[l], 0};
+        launch_conv(u, lb, false, a, CH[l] / 8, N, h, w, e, CH[l], true, u->prm + lb.b_off, ob);
     }
     for (int l = LV - 2; l >= 0; l--) {
         const int h = H >> l, w = W >> l;
-        View coarse = l == LV - 2 ? View{u->E[LV - 1], CH[l + 1], 0} : View{u->DO[l + 1], CH[l + 1], 0};
         Layer& la = n.dec(l, 0);
         Layer& lb = n.dec(l, 1);
+        ConvOpt oa;
+        oa.up = l == LV - 2 ? View{u->E[LV - 1], CH[l + 1], 0} : View{u->DO[l + 1], CH[l + 1], 0};
+        oa.up_groups = CH[l + 1] / 8;
         launch_conv(u, la, false, View{u->E[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->DA[l], CH[l], 0}, CH[l], true,
-                    u->prm + la.b_off, none, nullptr, nullptr, coarse, CH[l + 1] / 8);
+                    u->prm + la.b_off, oa);
         launch_conv(u, lb, false, View{u->DA[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->DO[l], CH[l], 0}, CH[l], true,
-                    u->prm + lb.b_off, none, nullptr, nullptr);
+                    u->prm + lb.b_off);
     }
     Layer& lo = n.out();
-    launch_conv(u, lo, false, View{u->DO[0], CH[0], 0}, CH[0] / 8, N, H, W, none, 3, false, u->prm + lo.b_off, none,
-                u->y, u->xlog);
+    ConvOpt oo;
+    oo.resid = &src;
+    if (final_out) oo.out_final = final_out;
+    else oo.out_f32 = u->y;
+    launch_conv(u, lo, false, View{u->DO[0], CH[0], 0}, CH[0] / 8, N, H, W, View{nullptr, 0, 0}, 3, false,
+                u->prm + lo.b_off, oo);
 }
 
 void backward(nr_unet* u, int N, int H, int W) {
     Net& n = u->net;
-    const View none{nullptr, 0, 0};
+    auto masked = [](View m) { ConvOpt o; o.mask = m; return o; };
     Layer& lo = n.out();
     // output layer
     launch_wgrad(u, lo, View{u->DO[0], CH[0], 0}, CH[0], View{u->gY, 16, 0}, 3, N, H, W);
-    launch_conv(u, lo, true, View{u->gY, 16, 0}, 2, N, H, W, View{u->gDO[0], CH[0], 0}, CH[0], false, nullptr,
-                View{u->DO[0], CH[0], 0}, nullptr, nullptr);
+    launch_conv(u, lo, true, View{u->gY, 16, 0}, 2, N, H, W, View{u->gDO[0], CH[0], 0}, CH[0], false,
+                    nullptr, masked(View{u->DO[0], CH[0], 0}));
     // decoder, finest level first
     for (int l = 0; l < LV - 1; l++) {
         const int h = H >> l, w = W >> l;
@@ -752,11 +1005,11 @@ void backward(nr_unet* u, int N, int H, int W) {
         Layer& lb = n.dec(l, 1);
         launch_wgrad(u, lb, View{u->DA[l], CH[l], 0}, CH[l], View{u->gDO[l], CH[l], 0}, CH[l], N, h, w);
         launch_conv(u, lb, true, View{u->gDO[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->gDA[l], CH[l], 0}, CH[l], false,
-                    nullptr, View{u->DA[l], CH[l], 0}, nullptr, nullptr);
+                    nullptr, masked(View{u->DA[l], CH[l], 0}));
         View coarse = l == LV - 2 ? View{u->E[LV - 1], CH[l + 1], 0} : View{u->DO[l + 1], CH[l + 1], 0};
         launch_wgrad(u, la, View{u->E[l], CH[l], 0}, cd, View{u->gDA[l], CH[l], 0}, CH[l], N, h, w, coarse, CH[l]);
         launch_conv(u, la, true, View{u->gDA[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->gD[l], cd, 0}, cd, false,
-                    nullptr, none, nullptr, nullptr);
+                    nullptr);
         // upsampled half of the concatenation -> the coarser decoder output (or the bottleneck)
         View dst = l == LV - 2 ? View{u->gE[LV - 1], CH[l + 1], 0} : View{u->gDO[l + 1], CH[l + 1], 0};
         size_t tot = (size_t)N * (h >> 1) * (w >> 1) * (CH[l + 1] / 8);
@@ -779,11 +1032,11 @@ void backward(nr_unet* u, int N, int H, int W) {
         }
         launch_wgrad(u, lb, View{u->A[l], CH[l], 0}, CH[l], View{u->gE[l], CH[l], 0}, CH[l], N, h, w);
         launch_conv(u, lb, true, View{u->gE[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->gA[l], CH[l], 0}, CH[l], false,
-                    nullptr, View{u->A[l], CH[l], 0}, nullptr, nullptr);
+                    nullptr, masked(View{u->A[l], CH[l], 0}));
         launch_wgrad(u, la, View{u->X[l], cin_t, 0}, cin, View{u->gA[l], CH[l], 0}, CH[l], N, h, w);
         if (l > 0)
             launch_conv(u, la, true, View{u->gA[l], CH[l], 0}, CH[l] / 8, N, h, w, View{u->gX[l], cin_t, 0}, cin, false,
-                        nullptr, none, nullptr, nullptr);
+                    nullptr);
     }
 }
 
@@ -804,17 +1057,16 @@ int check_dims(int N, int H, int W) {
     return NR_OK;
 }
 
-int train_common(nr_unet* u, const float* b, const float* al, const float* nm, const float* t, int N, int H, int W,
-                 float* loss) {
-    if (!u || !b || !al || !nm || !t) return fail(NR_ERR_INVALID, "null argument");
+int train_common(nr_unet* u, const Inputs& in, const float* t, int N, int H, int W, float* loss) {
+    if (!u || !in.b || !in.al || !in.nm || !in.dp || !t) return fail(NR_ERR_INVALID, "null argument");
     if (int e = check_dims(N, H, W)) return e;
     if (H % 16 || W % 16) return fail(NR_ERR_INVALID, "training crops must be multiples of 16");
     UCK(hipSetDevice(u->device));
     if (int e = ensure_shape(u, N, H, W)) return e;
     const size_t P = (size_t)N * H * W;
-    hipLaunchKernelGGL(k_prep, dim3(blocks(P, 256)), dim3(256), 0, u->stream, b, al, nm, N, H, W, H, W, u->X[0], u->xlog);
+    hipLaunchKernelGGL(k_prep, dim3(blocks(P, 256)), dim3(256), 0, u->stream, in, N, H, W, u->X[0]);
     hipLaunchKernelGGL(k_target, dim3(blocks(P, 256)), dim3(256), 0, u->stream, t, N, H, W, u->tlog);
-    forward(u, N, H, W);
+    forward(u, N, H, W, in, false, nullptr);
     const unsigned nb = blocks(P, 256);
     hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, u->stream, u->y, u->tlog, (int)P, 1.0f / (3.0f * (float)P), u->gY,
                        u->partial);
@@ -906,20 +1158,16 @@ int nr_unet_get_params(nr_unet* u, float* p, size_t n) {
     return NR_OK;
 }
 
-int nr_unet_forward(nr_unet* u, const float* b, const float* al, const float* nm, int32_t N, int32_t H, int32_t W,
-                    float* out) {
-    if (!u || !b || !al || !nm || !out) return fail(NR_ERR_INVALID, "null argument");
+int nr_unet_forward(nr_unet* u, const float* b, const float* al, const float* nm, const float* dp, int32_t N, int32_t H,
+                    int32_t W, float* out) {
+    if (!u || !b || !al || !nm || !dp || !out) return fail(NR_ERR_INVALID, "null argument");
     if (int e = check_dims(N, H, W)) return e;
     UCK(hipSetDevice(u->device));
     const int Hp = up16(H), Wp = up16(W);
     if (int e = ensure_shape(u, N, Hp, Wp)) return e;
     const size_t P = (size_t)N * Hp * Wp;
     UCK(hipEventRecord(u->e0, u->stream));
-    hipLaunchKernelGGL(k_prep, dim3(blocks(P, 256)), dim3(256), 0, u->stream, b, al, nm, N, H, W, Hp, Wp, u->X[0],
-                       u->xlog);
-    forward(u, N, Hp, Wp);
-    const size_t Po = (size_t)N * H * W;
-    hipLaunchKernelGGL(k_output, dim3(blocks(Po, 256)), dim3(256), 0, u->stream, u->y, N, H, W, Hp, Wp, out);
+    forward(u, N, Hp, Wp, Inputs{b, al, nm, dp, H, W}, true, out);
     UCK(hipGetLastError());
     UCK(hipEventRecord(u->e1, u->stream));
     UCK(hipEventSynchronize(u->e1));
@@ -928,17 +1176,17 @@ int nr_unet_forward(nr_unet* u, const float* b, const float* al, const float* nm
     return NR_OK;
 }
 
-int nr_unet_gradients(nr_unet* u, const float* b, const float* al, const float* nm, const float* t, int32_t N,
-                      int32_t H, int32_t W, float* grad, float* loss) {
+int nr_unet_gradients(nr_unet* u, const float* b, const float* al, const float* nm, const float* dp, const float* t,
+                      int32_t N, int32_t H, int32_t W, float* grad, float* loss) {
     if (!grad) return fail(NR_ERR_INVALID, "null argument");
-    if (int e = train_common(u, b, al, nm, t, N, H, W, loss)) return e;
+    if (int e = train_common(u, Inputs{b, al, nm, dp, H, W}, t, N, H, W, loss)) return e;
     UCK(hipMemcpy(grad, u->grad, u->net.n_params * 4, hipMemcpyDeviceToHost));
     return NR_OK;
 }
 
-int nr_unet_train_step(nr_unet* u, const float* b, const float* al, const float* nm, const float* t, int32_t N,
-                       int32_t H, int32_t W, float lr, float* loss) {
-    if (int e = train_common(u, b, al, nm, t, N, H, W, loss)) return e;
+int nr_unet_train_step(nr_unet* u, const float* b, const float* al, const float* nm, const float* dp, const float* t,
+                       int32_t N, int32_t H, int32_t W, float lr, float* loss) {
+    if (int e = train_common(u, Inputs{b, al, nm, dp, H, W}, t, N, H, W, loss)) return e;
     u->adam_t++;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.0f - std::pow(b1, (float)u->adam_t), c2 = 1.0f - std::pow(b2, (float)u->adam_t);
@@ -955,11 +1203,12 @@ int nr_denoise(nr_ctx* ctx, int view, nr_unet* u, float* out) {
     const float* b = nr_aov_device_ptr(ctx, view, NR_LAYER_BEAUTY);
     const float* al = nr_aov_device_ptr(ctx, view, NR_LAYER_DIFFUSE);
     const float* nm = nr_aov_device_ptr(ctx, view, NR_LAYER_WORLD_NORMAL);
-    if (!b || !al || !nm) return fail(NR_ERR_STATE, "view has no beauty/diffuse/world-normal layers");
+    const float* dp = nr_aov_device_ptr(ctx, view, NR_LAYER_DEPTH);
+    if (!b || !al || !nm || !dp) return fail(NR_ERR_STATE, "view has no beauty/diffuse/world-normal/depth layers");
     int32_t H = 0, W = 0;
     if (nr_view_size(ctx, view, &H, &W) != NR_OK) return NR_ERR_STATE;
     if (nr_sync(ctx) != NR_OK) return NR_ERR_HIP;
-    return nr_unet_forward(u, b, al, nm, 1, H, W, out);
+    return nr_unet_forward(u, b, al, nm, dp, 1, H, W, out);
 }
 
 int nr_unet_last_stats(nr_unet* u, float* ms, double* flops) {

@@ -27,10 +27,10 @@ def _lib():
             "nr_unet_destroy": (None, [vp]),
             "nr_unet_set_params": (C.c_int, [vp, C.POINTER(C.c_float), C.c_size_t]),
             "nr_unet_get_params": (C.c_int, [vp, C.POINTER(C.c_float), C.c_size_t]),
-            "nr_unet_forward": (C.c_int, [vp, fp, fp, fp, C.c_int32, C.c_int32, C.c_int32, fp]),
-            "nr_unet_train_step": (C.c_int, [vp, fp, fp, fp, fp, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+            "nr_unet_forward": (C.c_int, [vp, fp, fp, fp, fp, C.c_int32, C.c_int32, C.c_int32, fp]),
+            "nr_unet_train_step": (C.c_int, [vp, fp, fp, fp, fp, fp, C.c_int32, C.c_int32, C.c_int32, C.c_float,
                                              C.POINTER(C.c_float)]),
-            "nr_unet_gradients": (C.c_int, [vp, fp, fp, fp, fp, C.c_int32, C.c_int32, C.c_int32,
+            "nr_unet_gradients": (C.c_int, [vp, fp, fp, fp, fp, fp, C.c_int32, C.c_int32, C.c_int32,
                                             C.POINTER(C.c_float), C.POINTER(C.c_float)]),
             "nr_denoise": (C.c_int, [vp, C.c_int, vp, fp]),
             "nr_unet_last_stats": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_double)]),
@@ -99,34 +99,42 @@ class UNet:
         np.save(path, self.get_params())
 
     # compute ---------------------------------------------------------------
-    def forward(self, beauty, albedo, normal, out=None):
-        """(N,3,H,W) float32 CUDA tensors -> denoised radiance (N,3,H,W)."""
+    def forward(self, beauty, albedo, normal, depth, out=None):
+        """(N,3,H,W) beauty, albedo, normal and (N,1,H,W) depth, float32 CUDA tensors ->
+        denoised radiance (N,3,H,W)."""
         import torch
         N, _, H, W = beauty.shape
         if out is None:
             out = torch.empty_like(beauty)
-        _check(_lib().nr_unet_forward(self._h, _dev(beauty), _dev(albedo, beauty.shape), _dev(normal, beauty.shape),
-                                      N, H, W, _dev(out, beauty.shape)))
+        _check(_lib().nr_unet_forward(self._h, *self._inputs(beauty, albedo, normal, depth), N, H, W,
+                                      _dev(out, beauty.shape)))
         return out
 
-    def train_step(self, beauty, albedo, normal, target, lr=1e-3) -> float:
+    def train_step(self, beauty, albedo, normal, depth, target, lr=1e-3) -> float:
         N, _, H, W = beauty.shape
         loss = C.c_float()
-        _check(_lib().nr_unet_train_step(self._h, _dev(beauty), _dev(albedo, beauty.shape), _dev(normal, beauty.shape),
+        _check(_lib().nr_unet_train_step(self._h, *self._inputs(beauty, albedo, normal, depth),
                                          _dev(target, beauty.shape), N, H, W, float(lr), C.byref(loss)))
         return loss.value
 
-    def gradients(self, beauty, albedo, normal, target):
+    def gradients(self, beauty, albedo, normal, depth, target):
         N, _, H, W = beauty.shape
         g = np.empty(self.n_params, np.float32)
         loss = C.c_float()
-        _check(_lib().nr_unet_gradients(self._h, _dev(beauty), _dev(albedo, beauty.shape), _dev(normal, beauty.shape),
+        _check(_lib().nr_unet_gradients(self._h, *self._inputs(beauty, albedo, normal, depth),
                                         _dev(target, beauty.shape), N, H, W, g.ctypes.data_as(C.POINTER(C.c_float)),
                                         C.byref(loss)))
         return loss.value, g
 
+    @staticmethod
+    def _inputs(beauty, albedo, normal, depth):
+        N, _, H, W = beauty.shape
+        return (_dev(beauty, (N, 3, H, W)), _dev(albedo, (N, 3, H, W)), _dev(normal, (N, 3, H, W)),
+                _dev(depth, (N, 1, H, W)))
+
     def denoise_view(self, ctx: native.Context, view: int, out):
-        """Denoise a render context view's beauty (uses its DIFFUSE and WORLD_NORMAL layers) into `out` (3,H,W)."""
+        """Denoise a render context view's beauty (uses its DIFFUSE, WORLD_NORMAL and DEPTH layers) into
+        `out` (3,H,W)."""
         H, W = ctx.views[view]
         _check(_lib().nr_denoise(ctx._h, view, self._h, _dev(out, (3, H, W))))
         return out
@@ -139,20 +147,22 @@ class UNet:
 
 def _time_forward(H: int, W: int, reps: int) -> dict:
     """Median wall time (HIP events inside the library) of one U-Net forward over an H x W
-    frame of synthetic inputs (beauty 0..2, albedo 0..1, normals -1..1), shipped weights."""
+    frame of synthetic inputs (beauty 0..2, albedo 0..1, normals -1..1, depth 0..10), shipped
+    weights."""
     import torch
     net = UNet(0).load()
     g = torch.Generator(device="cuda").manual_seed(0)
     b = torch.rand((1, 3, H, W), device="cuda", generator=g) * 2
     a = torch.rand((1, 3, H, W), device="cuda", generator=g)
     n = torch.rand((1, 3, H, W), device="cuda", generator=g) * 2 - 1
+    d = torch.rand((1, 1, H, W), device="cuda", generator=g) * 10
     out = torch.empty_like(b)
     for _ in range(3):
-        net.forward(b, a, n, out)
+        net.forward(b, a, n, d, out)
     torch.cuda.synchronize()
     ms = []
     for _ in range(reps):
-        net.forward(b, a, n, out)
+        net.forward(b, a, n, d, out)
         ms.append(net.last_stats()[0])
     flops = net.last_stats()[1]
     med = sorted(ms)[len(ms) // 2]

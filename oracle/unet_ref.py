@@ -1,7 +1,7 @@
 """CPU float64 restatement of the denoiser network (test infrastructure only).
 
 Mirrors include/nectar_denoise.h: inputs log1p(beauty), log1p(albedo),
-clamped normal (9 channels); five levels of 32/48/64/96/128 channels with two
+clamped normal, log1p(depth) (10 channels); five levels of 32/48/64/96/128 channels with two
 3x3 convolutions + ReLU per level and side, 2x2 average pooling, nearest 2x
 upsampling, concatenation [encoder, upsampled], a 3x3 output convolution
 added to log1p(beauty); loss = mean |y - log1p(target)| over pixels and RGB.
@@ -19,7 +19,7 @@ import torch
 import torch.nn.functional as F
 
 CH = (32, 48, 64, 96, 128)
-IN_CH = 9
+IN_CH = 10
 
 
 def layer_shapes():
@@ -66,14 +66,17 @@ def _rb(x, on):
     return x.to(torch.bfloat16).to(torch.float64) if on else x
 
 
-def inputs(beauty, albedo, normal, bf16=False):
-    """(N,3,H,W) arrays -> network input (N,9,H,W) and log1p(beauty)."""
+def inputs(beauty, albedo, normal, depth, bf16=False):
+    """(N,3,H,W) beauty, albedo, normal and (N,1,H,W) depth -> network input (N,10,H,W) and
+    log1p(beauty)."""
     b = torch.as_tensor(beauty, dtype=torch.float64)
     a = torch.as_tensor(albedo, dtype=torch.float64)
     n = torch.as_tensor(normal, dtype=torch.float64)
+    z = torch.as_tensor(depth, dtype=torch.float64)
     xl = torch.log1p(torch.nan_to_num(b, nan=0.0).clamp(min=0))
     x = torch.cat([xl, torch.log1p(torch.nan_to_num(a, nan=0.0).clamp(min=0)),
-                   torch.nan_to_num(n, nan=0.0).clamp(-1, 1)], 1)
+                   torch.nan_to_num(n, nan=0.0).clamp(-1, 1),
+                   torch.log1p(torch.nan_to_num(z, nan=0.0).clamp(min=0))], 1)
     # the GPU keeps log1p(beauty) in fp32 for the residual and rounds the network input to bf16
     return _rb(x, bf16), xl.to(torch.float32).to(torch.float64) if bf16 else xl
 
@@ -100,9 +103,9 @@ def forward(layers, x, xl, bf16=False):
     return conv(d, k, relu=False) + xl
 
 
-def loss_and_grads(flat, beauty, albedo, normal, target, bf16=False):
+def loss_and_grads(flat, beauty, albedo, normal, depth, target, bf16=False):
     layers = [(w.clone().requires_grad_(True), b.clone().requires_grad_(True)) for w, b in unpack(flat)]
-    x, xl = inputs(beauty, albedo, normal, bf16)
+    x, xl = inputs(beauty, albedo, normal, depth, bf16)
     y = forward(layers, x, xl, bf16)
     t = torch.log1p(torch.as_tensor(target, dtype=torch.float64).clamp(min=0))
     loss = (y - t).abs().mean()
@@ -111,8 +114,9 @@ def loss_and_grads(flat, beauty, albedo, normal, target, bf16=False):
     return float(loss.detach()), g.detach().numpy(), y.detach().numpy()
 
 
-def denoise(flat, beauty, albedo, normal, bf16=False):
+def denoise(flat, beauty, albedo, normal, depth, bf16=False):
     with torch.no_grad():
-        x, xl = inputs(beauty, albedo, normal, bf16)
+        x, xl = inputs(beauty, albedo, normal, depth, bf16)
         y = forward(unpack(flat), x, xl, bf16)
     return torch.expm1(y.clamp(min=0)).numpy(), y.numpy()
+

@@ -78,8 +78,9 @@ def test_null_arguments_fail_cleanly_without_device():
     assert L.nr_scene_set_sky(None, None) == 1
     from nectarrender_amd import denoise
     D = denoise._lib()
-    assert D.nr_unet_param_count() == 1016131
-    assert D.nr_unet_forward(None, None, None, None, 1, 16, 16, None) == 1
+    assert D.nr_unet_param_count() == 1016419
+    assert D.nr_unet_forward(None, None, None, None, None, 1, 16, 16, None) == 1
+    assert D.nr_unet_train_step(None, None, None, None, None, None, 1, 16, 16, 1e-3, None) == 1
     assert D.nr_denoise(None, 0, None, None) == 1
     assert not D.nr_unet_create(0, 0) or os.path.exists("/dev/kfd")
 
