@@ -200,7 +200,8 @@ def main():
     ap.add_argument("--streams", type=int, default=0,
                     help="stream groups per render (1..4; default: the library's default)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default tuning")
-    ap.add_argument("--tuning", default="", help="slots,groups,shade,tail: set this tuning instead of autotuning")
+    ap.add_argument("--tuning", default="",
+                    help="slots,groups,shade,tail[,graph iterations]: set this tuning instead of autotuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (a kernel of known traffic for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define NR_ABI_VERSION 6
+#define NR_ABI_VERSION 7
 
 enum nr_status {
     NR_OK = 0,
@@ -401,13 +401,21 @@ typedef struct nr_tuning {
                                  (default 262144; 0 = never), one kernel finishes each remaining
                                  pixel with one thread, instead of wavefront iterations that each
                                  last as long as their slowest ray */
+    int32_t graph_iterations; /* 0 (default): the host launches every kernel of every streaming
+                                 iteration, sizing the traversal grids from the live-path count;
+                                 2..16, even: each stream group captures that many consecutive
+                                 iterations once per render call as a HIP graph (grids of the
+                                 group's full path capacity, surplus workgroups exit at once) and
+                                 replays it, so the host issues one launch per that many
+                                 iterations.  Timing mode (nr_set_timing) always uses the host loop. */
 } nr_tuning;
 int nr_set_tuning(nr_ctx* ctx, const nr_tuning* tuning);
 int nr_get_tuning(nr_ctx* ctx, nr_tuning* tuning);
 
 /* The project's autotuner: renders `desc` on `view` under candidate tunings
  * (coordinate descent over slots {1,2,4,6,8}, stream groups {1..4}, shade
- * workgroups per CU {4,8,16}, tail paths {0, 64k, 256k, 1M}), keeps the fastest,
+ * workgroups per CU {4,8,16}, tail paths {0, 64k, 256k, 1M}, graph iterations
+ * {0, 2, 8}), keeps the fastest,
  * writes it to *tuning (may be NULL) and clears the view's AOVs.  A desc of a few passes of the real
  * workload is enough. */
 int nr_autotune(nr_ctx* ctx, int view, const nr_render_desc* desc, nr_tuning* tuning);

@@ -70,7 +70,7 @@ class NrCameraDesc(C.Structure):
 class NrTuning(C.Structure):
     """nr_tuning: launch tunings of nr_render (results do not depend on them)."""
     _fields_ = [("slots", C.c_int32), ("stream_groups", C.c_int32), ("shade_wg_per_cu", C.c_int32),
-                ("tail_paths", C.c_int32)]
+                ("tail_paths", C.c_int32), ("graph_iterations", C.c_int32)]
 
 
 class NrRenderDesc(C.Structure):

@@ -116,12 +116,14 @@ constexpr int LDS_STACK_LEVELS = NR_LDS_LEVELS;   // traversal stack levels kept
 #endif
 
 // Mesh-tree layouts of the fast-mode traversals (NR_LAY_*): the reference's
-// estimator always walks the binary tree in the reference's order.
+// estimator always walks the binary tree in the reference's order.  Shadow rays
+// over the quantised nodes: k_shadow 24.87 -> 24.64 ms per 64 passes of the 1080p
+// workload against the full-precision 4-wide nodes.
 #ifndef NR_EXT_FAST_LAY
 #define NR_EXT_FAST_LAY NR_LAY_QUANT
 #endif
 #ifndef NR_SHADOW_LAY
-#define NR_SHADOW_LAY NR_LAY_WIDE
+#define NR_SHADOW_LAY NR_LAY_QUANT
 #endif
 
 struct StackCfg {
@@ -1460,9 +1462,14 @@ struct nr_ctx {
     int render_streams = NR_RENDER_STREAMS;
     int slots = NR_SLOTS_DEFAULT;       // path slots per pixel (nr_set_tuning)
     int tail_paths = NR_TAIL_PATHS;     // tail mode once a group's live-path bound is at most this (nr_set_tuning)
+    int graph_iters = 0;                // streaming iterations per captured HIP graph (0: host-launched; nr_set_tuning)
     hipStream_t gstream[NR_MAX_GROUPS] = {};
     hipEvent_t gev[NR_MAX_GROUPS][2] = {};
     hipEvent_t gev_done[NR_MAX_GROUPS] = {};
+    // per stream group: the last captured streaming-iteration graph and the launch
+    // arguments it was captured with (reused while they do not change)
+    hipGraphExec_t gexec[NR_MAX_GROUPS] = {};
+    std::vector<uint8_t> gkey[NR_MAX_GROUPS];
     hipEvent_t ev_packed = nullptr;
     int64_t last_iterations = 0;
     // timing
@@ -2062,6 +2069,7 @@ void nr_destroy(nr_ctx* c) {
     if (c->ev_layers) (void)hipEventDestroy(c->ev_layers);
     for (int g = 0; g < NR_MAX_GROUPS; g++) {
         if (c->gstream[g]) { (void)hipStreamSynchronize(c->gstream[g]); (void)hipStreamDestroy(c->gstream[g]); }
+        if (c->gexec[g]) (void)hipGraphExecDestroy(c->gexec[g]);
         for (auto& ev : c->gev[g]) if (ev) (void)hipEventDestroy(ev);
         if (c->gev_done[g]) (void)hipEventDestroy(c->gev_done[g]);
     }
@@ -2471,6 +2479,8 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
             StackCfg SC;
             hipStream_t st;
             int64_t it = 0, bound = 0, pending = 1;
+            int64_t iters = 0;      // streaming iterations enqueued (a graph replay enqueues several)
+            int64_t cap = 0;        // path slots of the group: the graphs' traversal grids
             bool done = false;
         } grp[NR_MAX_GROUPS];
         const size_t lv = c->scene.stack_depth > LDS_STACK_LEVELS ? (size_t)(c->scene.stack_depth - LDS_STACK_LEVELS) : 0;
@@ -2497,6 +2507,7 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
                 spill_off += lv * cap_g;
                 gr.st = G == 1 ? c->stream : c->gstream[g];
                 gr.bound = (int64_t)(T1 - T0) * 64 * m;
+                gr.cap = cap_g;
             }
         }
         mark(2);
@@ -2537,6 +2548,89 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
                 }
             }
         };
+        const bool other = (c->obj_types & NR_TY_OTHER) != 0;
+        constexpr int TYI = NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH;
+        // One streaming iteration of a group: clear the counters it appends to, trace,
+        // shade each material present, trace the shadow rays (fast mode), fold and start
+        // samples.  `bound` sizes the traversal grids (workgroups past the live count
+        // exit at once).
+        auto enqueue_iteration = [&](Group& gr, const TraceArgs& A, int64_t bound) {
+            const PathPtrs& P = gr.P;
+            hipStream_t st = gr.st;
+            hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, st, P.counts, (int32_t)A.in);
+            mark(0);
+            const dim3 gx((uint32_t)((bound + 127) / 128));
+            if (fast) {
+                if (other) hipLaunchKernelGGL((k_extend<NR_TY_ALL, true>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                else hipLaunchKernelGGL((k_extend<TYI, true>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+            } else {
+                if (other) hipLaunchKernelGGL((k_extend<NR_TY_ALL, false>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                else hipLaunchKernelGGL((k_extend<TYI, false>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+            }
+            mark(1);
+            // the shading grid-stride loops need no more workgroups than live paths
+            const dim3 gSi((uint32_t)std::max<int64_t>(1, std::min<int64_t>(gS.x, (bound + 255) / 256)));
+            if (fast) launch_shade<true>(c, gSi, st, P, A);
+            else launch_shade<false>(c, gSi, st, P, A);
+            if (fast && c->scene.sky.marg) {
+                mark(4);
+                if (other) hipLaunchKernelGGL(k_shadow<NR_TY_ALL>, gx, dim3(128), lds, st, c->d_scene, P, gr.SC);
+                else hipLaunchKernelGGL(k_shadow<TYI>, gx, dim3(128), lds, st, c->d_scene, P, gr.SC);
+            }
+            mark(2);
+            const uint32_t nt = A.t1 - A.t0;
+            hipLaunchKernelGGL(k_regen, dim3((nt + REGEN_BLOCK - 1) / REGEN_BLOCK), dim3(REGEN_BLOCK), 0, st, P, A);
+        };
+        // Graph mode: group g's graph of graph_iters consecutive iterations with grids of
+        // the group's capacity, captured on its stream (its launch arguments are the
+        // cache key: a later call with the same arguments replays the same graph).
+        const bool use_graph = c->graph_iters > 0 && !c->timing;
+        auto group_graph = [&](int g, Group& gr) -> int {
+            struct Key {
+                PathPtrs P;
+                TraceArgs A;
+                StackCfg SC;
+                int64_t cap;
+                const void* scene;
+                hipStream_t st;
+                int32_t iters, shade, fast, other, marg;
+                uint8_t mats[NR_MAT_TYPES];
+            } k;
+            std::memset(&k, 0, sizeof k);         // padding bytes take part in the comparison
+            k.P = gr.P; k.A = gr.A; k.SC = gr.SC; k.cap = gr.cap; k.scene = c->d_scene; k.st = gr.st;
+            k.iters = c->graph_iters; k.shade = (int32_t)gS.x; k.fast = fast; k.other = other;
+            k.marg = c->scene.sky.marg != nullptr;
+            for (int t = 0; t < NR_MAT_TYPES; t++) k.mats[t] = c->mat_present[t] ? 1 : 0;
+            const uint8_t* kb = reinterpret_cast<const uint8_t*>(&k);
+            if (c->gexec[g] && c->gkey[g].size() == sizeof k && std::memcmp(c->gkey[g].data(), kb, sizeof k) == 0)
+                return NR_OK;
+            if (c->gexec[g]) {                    // not running: the call began by waiting for the last one
+                (void)hipGraphExecDestroy(c->gexec[g]);
+                c->gexec[g] = nullptr;
+                c->gkey[g].clear();
+            }
+            HIPCK(hipStreamBeginCapture(gr.st, hipStreamCaptureModeThreadLocal));
+            TraceArgs Ak = gr.A;
+            for (int i = 0; i < c->graph_iters; i++) {
+                enqueue_iteration(gr, Ak, gr.cap);
+                Ak.in = 1 - Ak.in;
+            }
+            const hipError_t le = hipGetLastError();
+            hipGraph_t graph = nullptr;
+            const hipError_t ce = hipStreamEndCapture(gr.st, &graph);
+            if (le != hipSuccess || ce != hipSuccess || !graph) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return fail(NR_ERR_HIP, std::string("capturing the streaming iterations: ") +
+                                            hipGetErrorString(le != hipSuccess ? le : ce));
+            }
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) return fail(NR_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+            c->gexec[g] = ex;
+            c->gkey[g].assign(kb, kb + sizeof k);
+            return NR_OK;
+        };
         const int64_t max_iters = (int64_t)(d->sample_end - d->sample_begin) * d->ray_depth + 2;
         int active = G;
         {
@@ -2562,12 +2656,12 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
                     if (gr.bound == 0) { gr.done = true; active--; continue; }
                 }
                 progressed = true;
-                if (it > max_iters) return fail(NR_ERR_STATE, "streaming loop did not drain");
+                // (a graph replay runs graph_iters iterations, and the drain is seen two replays late)
+                if (gr.iters > max_iters + (use_graph ? 3 * c->graph_iters : 0))
+                    return fail(NR_ERR_STATE, "streaming loop did not drain");
                 PathPtrs& P = gr.P;
                 TraceArgs& A = gr.A;
                 hipStream_t st = gr.st;
-                const bool other = (c->obj_types & NR_TY_OTHER) != 0;
-                constexpr int TYI = NR_TY_QUAD | NR_TY_SPHERE | NR_TY_MESH;
                 if (it >= 2 && gr.pending == 0 && gr.bound <= (int64_t)c->tail_paths) {
                     // tail mode: every sample is started; one thread per live path runs it to its end,
                     // then k_regen folds them all
@@ -2588,40 +2682,25 @@ static int render_impl(nr_ctx* c, const int32_t* view_ids, int n_views, const nr
                     active--;
                     continue;
                 }
-                hipLaunchKernelGGL(k_counts, dim3(1), dim3(64), 0, st, P.counts, (int32_t)A.in);
-                mark(0);
-                const dim3 gx((uint32_t)((gr.bound + 127) / 128));
-                if (fast) {
-                    if (other) hipLaunchKernelGGL((k_extend<NR_TY_ALL, true>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
-                    else hipLaunchKernelGGL((k_extend<TYI, true>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                if (use_graph) {
+                    if (int e = group_graph(g, gr)) return e;
+                    HIPCK(hipGraphLaunch(c->gexec[g], st));
+                    gr.iters += c->graph_iters;     // an even count: the replay leaves A.in as it was
                 } else {
-                    if (other) hipLaunchKernelGGL((k_extend<NR_TY_ALL, false>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
-                    else hipLaunchKernelGGL((k_extend<TYI, false>), gx, dim3(128), lds, st, c->d_scene, P, A, gr.SC);
+                    enqueue_iteration(gr, A, gr.bound);
+                    HIPCK(hipGetLastError());
+                    A.in = 1 - A.in;
+                    gr.iters++;
                 }
-                mark(1);
-                // the shading grid-stride loops need no more workgroups than live paths
-                const dim3 gSi((uint32_t)std::max<int64_t>(1, std::min<int64_t>(gS.x, (gr.bound + 255) / 256)));
-                if (fast) launch_shade<true>(c, gSi, st, P, A);
-                else launch_shade<false>(c, gSi, st, P, A);
-                if (fast && c->scene.sky.marg) {
-                    mark(4);
-                    if (other) hipLaunchKernelGGL(k_shadow<NR_TY_ALL>, gx, dim3(128), lds, st, c->d_scene, P, gr.SC);
-                    else hipLaunchKernelGGL(k_shadow<TYI>, gx, dim3(128), lds, st, c->d_scene, P, gr.SC);
-                }
-                mark(2);
-                const uint32_t nt = A.t1 - A.t0;
-                hipLaunchKernelGGL(k_regen, dim3((nt + REGEN_BLOCK - 1) / REGEN_BLOCK), dim3(REGEN_BLOCK), 0, st, P, A);
-                HIPCK(hipGetLastError());
                 HIPCK(hipMemcpyAsync(live + 2 * (it & 1), P.counts + CNT_OCC, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
                                      st));
                 HIPCK(hipEventRecord(c->gev[g][it & 1], st));
-                A.in = 1 - A.in;
                 gr.it++;
             }
             if (!progressed) std::this_thread::yield();
         }
         c->last_iterations = 0;
-        for (int g = 0; g < G; g++) c->last_iterations += grp[g].it;
+        for (int g = 0; g < G; g++) c->last_iterations += grp[g].iters;
         }   // GroupJoin: the context stream waits for every group
         mark(2);
         auto w = write_guards();
@@ -3090,12 +3169,15 @@ int nr_set_tuning(nr_ctx* c, const nr_tuning* t) {
     if (t->stream_groups < 1 || t->stream_groups > NR_MAX_GROUPS) return fail(NR_ERR_INVALID, "stream groups outside 1..4");
     if (t->shade_wg_per_cu < 1 || t->shade_wg_per_cu > 32) return fail(NR_ERR_INVALID, "shade workgroups per CU outside 1..32");
     if (t->tail_paths < 0) return fail(NR_ERR_INVALID, "negative tail_paths");
+    if (t->graph_iterations != 0 && (t->graph_iterations < 2 || t->graph_iterations > 16 || (t->graph_iterations & 1)))
+        return fail(NR_ERR_INVALID, "graph_iterations must be 0 or even in 2..16");
     HIPCK(hipSetDevice(c->device));
     HIPCK(hipStreamSynchronize(c->stream));                 // path state may be reallocated by the next render
     c->slots = t->slots;
     c->render_streams = t->stream_groups;
     c->shade_grid = c->n_cu * t->shade_wg_per_cu;
     c->tail_paths = t->tail_paths;
+    c->graph_iters = t->graph_iterations;
     return NR_OK;
 }
 
@@ -3105,6 +3187,7 @@ int nr_get_tuning(nr_ctx* c, nr_tuning* t) {
     t->stream_groups = c->render_streams;
     t->shade_wg_per_cu = c->shade_grid / std::max(1, c->n_cu);
     t->tail_paths = c->tail_paths;
+    t->graph_iterations = c->graph_iters;
     return NR_OK;
 }
 
@@ -3134,15 +3217,17 @@ int nr_autotune(nr_ctx* c, int view, const nr_render_desc* d, nr_tuning* out) {
     double best_ms = 0.0;
     if (int e = measure(best, best_ms)) return e;
     const int slot_c[] = {1, 2, 4, 6, 8}, group_c[] = {1, 2, 3, 4}, shade_c[] = {4, 8, 16},
-              tail_c[] = {0, 65536, 262144, 1048576};
-    for (int axis = 0; axis < 4; axis++) {
-        const int* cand = axis == 0 ? slot_c : (axis == 1 ? group_c : (axis == 2 ? shade_c : tail_c));
-        const int n = axis == 0 ? 5 : (axis == 1 ? 4 : (axis == 2 ? 3 : 4));
+              tail_c[] = {0, 65536, 262144, 1048576}, graph_c[] = {0, 2, 8};
+    const int* cands[5] = {slot_c, group_c, shade_c, tail_c, graph_c};
+    const int n_cand[5] = {5, 4, 3, 4, 3};
+    for (int axis = 0; axis < 5; axis++) {
+        const int* cand = cands[axis];
+        const int n = n_cand[axis];
         nr_tuning base = best;
         for (int k = 0; k < n; k++) {
             nr_tuning t = base;
-            int& field = axis == 0 ? t.slots
-                                   : (axis == 1 ? t.stream_groups : (axis == 2 ? t.shade_wg_per_cu : t.tail_paths));
+            int32_t* fields[5] = {&t.slots, &t.stream_groups, &t.shade_wg_per_cu, &t.tail_paths, &t.graph_iterations};
+            int32_t& field = *fields[axis];
             if (field == cand[k]) continue;
             field = cand[k];
             double ms = 0.0;

@@ -77,7 +77,7 @@ def lib() -> C.CDLL:
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
-    if L.nr_abi_version() != 6:
+    if L.nr_abi_version() != 7:
         raise NectarError("ABI version mismatch")
     _lib = L
     return L
@@ -151,21 +151,26 @@ class Context:
     def render(self, view: int, desc: abi.NrRenderDesc):
         _check(lib().nr_render(self._h, view, C.byref(desc)))
 
-    def set_tuning(self, slots: int = 4, stream_groups: int = 2, shade_wg_per_cu: int = 8, tail_paths: int = 262144):
-        _check(lib().nr_set_tuning(self._h, C.byref(abi.NrTuning(slots, stream_groups, shade_wg_per_cu, tail_paths))))
+    def set_tuning(self, slots: int = 4, stream_groups: int = 2, shade_wg_per_cu: int = 8, tail_paths: int = 262144,
+                   graph_iterations: int = 0):
+        _check(lib().nr_set_tuning(self._h, C.byref(abi.NrTuning(slots, stream_groups, shade_wg_per_cu, tail_paths,
+                                                                   graph_iterations))))
+
+    @staticmethod
+    def _tuning_dict(t: abi.NrTuning) -> dict:
+        return {"slots": t.slots, "stream_groups": t.stream_groups, "shade_wg_per_cu": t.shade_wg_per_cu,
+                "tail_paths": t.tail_paths, "graph_iterations": t.graph_iterations}
 
     def get_tuning(self) -> dict:
         t = abi.NrTuning()
         _check(lib().nr_get_tuning(self._h, C.byref(t)))
-        return {"slots": t.slots, "stream_groups": t.stream_groups, "shade_wg_per_cu": t.shade_wg_per_cu,
-                "tail_paths": t.tail_paths}
+        return self._tuning_dict(t)
 
     def autotune(self, view: int, desc: abi.NrRenderDesc) -> dict:
         """Times candidate tunings on `desc`, keeps the fastest (the view's AOVs are cleared)."""
         t = abi.NrTuning()
         _check(lib().nr_autotune(self._h, view, C.byref(desc), C.byref(t)))
-        return {"slots": t.slots, "stream_groups": t.stream_groups, "shade_wg_per_cu": t.shade_wg_per_cu,
-                "tail_paths": t.tail_paths}
+        return self._tuning_dict(t)
 
     def render_tiles(self, view: int, desc: abi.NrRenderDesc, tile_end: np.ndarray, tile_size: int):
         """nr_render with a per-tile sample limit (rows x cols uint32, row major, rows =

@@ -94,6 +94,8 @@ def curves(ctx, abi, torch, cam, rp, ref_disp, unet=None, plain_ks=(1, 2, 4, 8, 
             d = _display(ctx, abi, torch, ctx.read(0), H, W)
             plain.append((K, t_render, rmse(d, ref_disp)))
         if unet is not None and K in denoise_ks:
+            if not den:                            # first call sizes the U-Net's buffers: not timed
+                unet.denoise_view(ctx, 0, out)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             unet.denoise_view(ctx, 0, out)

@@ -71,7 +71,8 @@ def render_dataset(args, torch):
     def work(w):
         try:
             ctx = native.Context(0)
-            ctx.set_tuning(slots=8, stream_groups=2, shade_wg_per_cu=8)   # small images: more samples in flight
+            # small images: more samples in flight
+            ctx.set_tuning(slots=8, stream_groups=args.stream_groups, shade_wg_per_cu=8)
             for i in range(w, args.scenes, args.workers):
                 if args.render_budget > 0 and time.time() - t0 > args.render_budget:
                     break
@@ -115,6 +116,7 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--hdri-mesh", type=float, default=0.3, help="fraction of config2-style scenes")
     ap.add_argument("--workers", type=int, default=4, help="render contexts running concurrently")
+    ap.add_argument("--stream-groups", type=int, default=2, help="stream groups of each render context")
     ap.add_argument("--render-budget", type=float, default=0.0,
                     help="stop rendering scenes after this many seconds (0: render all)")
     ap.add_argument("--train-budget", type=float, default=0.0,

@@ -40,7 +40,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_loads_and_reports_version():
     L = C.CDLL(LIB)
-    assert L.nr_abi_version() == 6
+    assert L.nr_abi_version() == 7
     L.nr_last_error.restype = C.c_char_p
     assert isinstance(L.nr_last_error(), bytes)
 

@@ -195,13 +195,14 @@ def test_render_views_in_one_pipeline_equals_one_at_a_time(ctx, mode):
 
 
 @pytest.mark.parametrize("tuning", [(1, 1, 8, 0), (2, 2, 4, 262144), (8, 3, 16, 1 << 30), (6, 4, 2, 100),
-                                    (4, 2, 8, 0), (4, 1, 8, 1 << 30)])
+                                    (4, 2, 8, 0), (4, 1, 8, 1 << 30), (4, 2, 8, 262144, 2), (8, 1, 16, 0, 8),
+                                    (2, 3, 4, 100, 16)])
 @pytest.mark.parametrize("scene", ["mixed", "config1", "icosphere"])
 def test_render_same_for_any_tuning(ctx, oracle, tuning, scene):
-    """Path slots per pixel, stream groups, the shading grid and the tail-mode threshold change only how
-    work is scheduled: the AOVs equal the oracle's for every tuning, on scenes of every object type, of
-    quads and spheres, and of a mesh.  A threshold of 2^30 hands every pixel to the tail kernel after the
-    first two iterations; 0 never uses it."""
+    """Path slots per pixel, stream groups, the shading grid, the tail-mode threshold and graph-captured
+    iterations change only how work is scheduled: the AOVs equal the oracle's for every tuning, on scenes
+    of every object type, of quads and spheres, and of a mesh.  A threshold of 2^30 hands every pixel to
+    the tail kernel after the first two iterations; 0 never uses it."""
     if scene == "mixed":
         s, cam, rp = scenes.mixed_scene(spp=5, res=72)
     elif scene == "config1":
@@ -232,6 +233,7 @@ def test_autotune_picks_a_valid_tuning_and_clears_the_view(ctx, oracle):
     try:
         assert 1 <= t["slots"] <= 8 and 1 <= t["stream_groups"] <= 4 and 1 <= t["shade_wg_per_cu"] <= 32
         assert t["tail_paths"] in (0, 65536, 262144, 1048576)
+        assert t["graph_iterations"] in (0, 2, 8)
         assert ctx.get_tuning() == t
         assert not ctx.read(0).any()
         g, o = _render_both(ctx, oracle, s, cam, rp, 1, 8)
