@@ -414,8 +414,8 @@ int nr_get_tuning(nr_ctx* ctx, nr_tuning* tuning);
 
 /* The project's autotuner: renders `desc` on `view` under candidate tunings
  * (coordinate descent over slots {1,2,4,6,8}, stream groups {1..4}, shade
- * workgroups per CU {4,8,16}, tail paths {0, 64k, 256k, 1M}, graph iterations
- * {0, 2, 8}), keeps the fastest,
+ * workgroups per CU {4,8,16}, tail paths {0, 64k, 256k, 1M, 2M, 4M}, graph
+ * iterations {0, 2, 8}), keeps the fastest,
  * writes it to *tuning (may be NULL) and clears the view's AOVs.  A desc of a few passes of the real
  * workload is enough. */
 int nr_autotune(nr_ctx* ctx, int view, const nr_render_desc* desc, nr_tuning* tuning);

@@ -3217,9 +3217,9 @@ int nr_autotune(nr_ctx* c, int view, const nr_render_desc* d, nr_tuning* out) {
     double best_ms = 0.0;
     if (int e = measure(best, best_ms)) return e;
     const int slot_c[] = {1, 2, 4, 6, 8}, group_c[] = {1, 2, 3, 4}, shade_c[] = {4, 8, 16},
-              tail_c[] = {0, 65536, 262144, 1048576}, graph_c[] = {0, 2, 8};
+              tail_c[] = {0, 65536, 262144, 1048576, 2097152, 4194304}, graph_c[] = {0, 2, 8};
     const int* cands[5] = {slot_c, group_c, shade_c, tail_c, graph_c};
-    const int n_cand[5] = {5, 4, 3, 4, 3};
+    const int n_cand[5] = {5, 4, 3, 6, 3};
     for (int axis = 0; axis < 5; axis++) {
         const int* cand = cands[axis];
         const int n = n_cand[axis];
