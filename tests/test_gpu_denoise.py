@@ -25,6 +25,7 @@ def torch_cuda():
 def net():
     from nectarrender_amd import denoise
     n = denoise.UNet(0, seed=3)
+    n.p0 = n.get_params()                      # every test perturbs these, whatever ran before it
     yield n
     n.close()
 
@@ -45,9 +46,8 @@ def _cuda(torch, *xs):
 
 
 def _params(net, rng):
-    p = net.get_params()
     # larger than the initial output layer so every path of the network matters
-    p = (p + rng.normal(size=p.shape).astype(np.float32) * 0.01).astype(np.float32)
+    p = (net.p0 + rng.normal(size=net.p0.shape).astype(np.float32) * 0.01).astype(np.float32)
     net.set_params(p)
     return p
 
@@ -66,6 +66,22 @@ def test_forward_matches_checker(torch_cuda, net, shape):
     err = np.abs(got_y - np.maximum(y[:, :, :H, :W], 0))
     assert err.max() < 2e-2, err.max()
     assert np.isfinite(out).all()
+
+
+def test_forward_at_1080p_matches_checker(torch_cuda, net):
+    """The benchmark frame's shape (1080 x 1920, padded to 1088 x 1920): the full-resolution
+    layers run 8-row workgroups and the first and last convolutions read the render layers and
+    write the planar image themselves, a configuration only frames of about a megapixel reach."""
+    rng = np.random.default_rng(14)
+    p = _params(net, rng)
+    H, W = 1080, 1920
+    b, a, n, d, _ = _batch(rng, 1, H, W)
+    out = net.forward(*_cuda(torch_cuda, b, a, n, d)).cpu().numpy()
+    pad = lambda x: np.pad(x, ((0, 0), (0, 0), (0, 8), (0, 0)))
+    _, y = U.denoise(p, pad(b), pad(a), pad(n), pad(d), bf16=True)
+    err = np.abs(np.log1p(out) - np.maximum(y[:, :, :H, :W], 0))
+    assert np.isfinite(out).all()
+    assert err.max() < 2e-2, err.max()
 
 
 def _check_gradients(g, ref_g):
