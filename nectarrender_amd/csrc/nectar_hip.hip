@@ -103,11 +103,23 @@ constexpr int LDS_STACK_LEVELS = NR_LDS_LEVELS;   // traversal stack levels kept
 #define NR_TRAV_OCC 6
 #endif
 // waves per SIMD of k_extend for quad / sphere / mesh scenes: at 7 (72 VGPRs) the
-// traversal loop keeps its state in registers; built for 8 it spilled 120 bytes
-// per lane inside the loop and ran slower on the 1080p workload.  The
+// binary-tree traversal loop keeps its state in registers; built for 8 it spilled
+// 120 bytes per lane inside the loop and ran slower on the 1080p workload.  The
+// fast-mode kernel (quantised nodes, folded box test) at 6: 80 VGPRs and 12 bytes
+// of spill outside the loop; at 7 it spilled 64 bytes and reloaded per-ray terms
+// inside the loop (1080p MIS workload 1185 -> 1055 Mpaths/s), at 5 1163.  The
 // every-object-type kernels keep the budget of NR_TRAV_OCC.
 #ifndef NR_EXT_WAVES
 #define NR_EXT_WAVES 7
+#endif
+#ifndef NR_EXT_FAST_WAVES
+#define NR_EXT_FAST_WAVES 6
+#endif
+// waves per SIMD of k_shadow for quad / sphere / mesh scenes: 7 (72 VGPRs, 16
+// bytes of spill outside the loop) against 6 (75 VGPRs): 1080p MIS workload
+// 1183 -> 1187 Mpaths/s.
+#ifndef NR_SHADOW_WAVES
+#define NR_SHADOW_WAVES 7
 #endif
 #if NR_TRAV_OCC > 0
 #define NR_TRAV_ATTR __attribute__((amdgpu_waves_per_eu(NR_TRAV_OCC, 8)))
@@ -716,12 +728,21 @@ __global__ void __launch_bounds__(256) k_wide_nodes(const GNode* __restrict__ no
 // 2^e (e >= -126) with origin + 255 * 2^e >= the union's upper bound; each child
 // plane is rounded outward to the grid and checked with the float additions
 // the traversal decodes with (origin + q * 2^e, q * 2^e exact), stepping q
-// outward until the decoded box contains the exact one.
+// outward until the decoded box contains the exact one.  A child box of zero
+// thickness on some axis is left out (ref NR_WIDE_EMPTY): its triangles all lie
+// in that axis plane, and the reference's slab test (AABB::hit_t, empty when
+// tmax <= tmin) never enters such a box for a ray crossing the plane, while a ray
+// in the plane is parallel to every one of its triangles, which the triangle
+// test rejects.
 __device__ __forceinline__ float qdecode(float origin, uint32_t q, float s) { return origin + (float)q * s; }
 __global__ void __launch_bounds__(256) k_quant_nodes(const GWNode* __restrict__ wn, int32_t n, GQNode* __restrict__ out) {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const GWNode w = wn[i];
+    GWNode w = wn[i];
+    for (int k = 0; k < 4; k++)
+        if (w.ref[k] != NR_WIDE_EMPTY &&
+            (w.lo[0][k] == w.hi[0][k] || w.lo[1][k] == w.hi[1][k] || w.lo[2][k] == w.hi[2][k]))
+            w.ref[k] = NR_WIDE_EMPTY;
     GQNode q;
     uint32_t exps = 0;
     for (int a = 0; a < 3; a++) {
@@ -878,7 +899,7 @@ __device__ __forceinline__ int extend_one(const GScene& S, const PathPtrs& P, in
 }
 
 template <int TY, bool FAST>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(TY == NR_TY_ALL ? (NR_TRAV_OCC > 0 ? NR_TRAV_OCC : 1) : NR_EXT_WAVES, 8)))
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(TY == NR_TY_ALL ? (NR_TRAV_OCC > 0 ? NR_TRAV_OCC : 1) : FAST ? NR_EXT_FAST_WAVES : NR_EXT_WAVES, 8)))
 k_extend(const GScene* __restrict__ Sp, PathPtrs P, TraceArgs A, StackCfg SC) {
     extern __shared__ int2 smem[];
     const GScene& S = *Sp;
@@ -1123,7 +1144,9 @@ __device__ __forceinline__ void shadow_one(const GScene& S, const PathPtrs& P, i
 }
 
 template <int TY>
-__global__ void __launch_bounds__(128) NR_TRAV_ATTR k_shadow(const GScene* __restrict__ Sp, PathPtrs P, StackCfg SC) {
+__global__ void __launch_bounds__(128)
+__attribute__((amdgpu_waves_per_eu(TY == NR_TY_ALL ? (NR_TRAV_OCC > 0 ? NR_TRAV_OCC : 1) : NR_SHADOW_WAVES, 8)))
+k_shadow(const GScene* __restrict__ Sp, PathPtrs P, StackCfg SC) {
     extern __shared__ int2 smem[];
     const GScene& S = *Sp;
     Stack st = make_stack(smem, S, SC);
@@ -3232,7 +3255,7 @@ int nr_autotune(nr_ctx* c, int view, const nr_render_desc* d, nr_tuning* out) {
             field = cand[k];
             double ms = 0.0;
             if (int e = measure(t, ms)) return e;
-            if (ms < best_ms) { best_ms = ms; best = t; }
+            if (ms < 0.99 * best_ms) { best_ms = ms; best = t; }   // 1%: above one timing's noise
         }
     }
     if (int e = nr_set_tuning(c, &best)) return e;

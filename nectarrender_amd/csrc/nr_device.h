@@ -568,9 +568,23 @@ NR_D int32_t mesh_closest_w(const GMesh& M, const Ray& r, float tmin, float tmax
 }
 
 // The 4-wide traversal over quantised nodes (GQNode): one 64 B fetch per two
-// binary levels instead of three 64 B nodes.  Decoded boxes contain the exact
-// ones, so the closest hit is the binary traversal's (of two triangles hit at
-// exactly the same t either may be reported).  Leaf batching as in mesh_closest.
+// binary levels instead of three 64 B nodes.  Leaf batching as in mesh_closest.
+//
+// Box test: the slab distance of plane code q on axis a is (origin + q s - o) inv
+// = fma(q, s inv, (origin - o) inv), one FMA per plane after a few per-node terms;
+// the entry plane is the lower one where inv >= 0, the upper one otherwise.  Each
+// slab is widened by m = 2^-18 (|(origin - o) inv| + |origin inv| + 255 |s inv|),
+// at least eight times the rounding of this form, of the decoded planes (which
+// contain the exact child boxes) and of box_t's own (P - o) inv, so every box
+// that box_t accepts on the exact child box is accepted here with an entry
+// distance no larger: the closest hit is the binary traversal's (of two
+// triangles hit at the same t either may be reported).  An axis with |inv| > 2^40
+// or |o| > 2^80 does not constrain the box: its terms are NaN, which fmaxf and
+// fminf (IEEE maxNum / minNum, v_max_f32 / v_min_f32) pass over, and every other
+// term stays finite.  The one difference from the binary traversal: a ray lying
+// exactly in a box's plane, with a zero direction component, makes box_t compute
+// (P - o) * inf = NaN and, for +0, reject a box the ray touches; here that axis
+// is left out, so such a ray may find a hit the binary traversal misses.
 NR_D float qbyte(uint32_t w, int k) { return (float)((w >> (8 * k)) & 255u); }
 template <bool ANY = false>
 NR_D int32_t mesh_closest_q(const GMesh& M, const Ray& r, float tmin, float tmax, Stack& st, int sp0,
@@ -582,6 +596,16 @@ NR_D int32_t mesh_closest_q(const GMesh& M, const Ray& r, float tmin, float tmax
     float te = box_t(M.root_min, M.root_max, r, tmin, tmax);
     const NR_GLB GQNode* const qnodes = glb(M.qnodes);
     const NR_GLB GTri* const tris = glb(M.tris);
+    // per-ray terms of the box test
+    float ic[3];
+    bool neg[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float iv = comp(r.inv, a), o = comp(r.o, a);
+        const bool ok = fabsf(iv) <= 0x1p40f && fabsf(o) <= 0x1p80f;
+        ic[a] = ok ? iv : __builtin_nanf("");
+        neg[a] = iv < 0.0f;
+    }
     bool active = true;
     bool leaf = false;
     while (true) {
@@ -595,17 +619,36 @@ NR_D int32_t mesh_closest_q(const GMesh& M, const Ray& r, float tmin, float tmax
                 const NR_GLB nr_f4* p = glb_f4(qnodes + ref);
                 const nr_f4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
                 const uint32_t ex = __float_as_uint(q0.w);
-                const float sx = __uint_as_float((ex & 255u) << 23), sy = __uint_as_float(((ex >> 8) & 255u) << 23),
-                            sz = __uint_as_float(((ex >> 16) & 255u) << 23);
-                const uint32_t lx = __float_as_uint(q1.x), ly = __float_as_uint(q1.y), lz = __float_as_uint(q1.z);
-                const uint32_t hx = __float_as_uint(q1.w), hy = __float_as_uint(q2.x), hz = __float_as_uint(q2.y);
+                const float org[3] = {q0.x, q0.y, q0.z};
+                const uint32_t wlo[3] = {__float_as_uint(q1.x), __float_as_uint(q1.y), __float_as_uint(q1.z)};
+                const uint32_t whi[3] = {__float_as_uint(q1.w), __float_as_uint(q2.x), __float_as_uint(q2.y)};
                 int32_t c[4] = {__float_as_int(q2.z), __float_as_int(q2.w), __float_as_int(q3.x), __float_as_int(q3.y)};
                 float t[4];
+                float Ae[3], Ax[3], B[3];
+                uint32_t we[3], wx[3];
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    const float sc = __uint_as_float(((ex >> (8 * a)) & 255u) << 23);
+                    const float A = (org[a] - comp(r.o, a)) * ic[a];
+                    B[a] = sc * ic[a];
+                    const float m =
+                        __builtin_fmaf(fabsf(B[a]), 255.0f, fabsf(A) + fabsf(org[a] * ic[a])) * 0x1p-18f;
+                    Ae[a] = A - m;
+                    Ax[a] = A + m;
+                    we[a] = neg[a] ? whi[a] : wlo[a];
+                    wx[a] = neg[a] ? wlo[a] : whi[a];
+                }
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const float mn[3] = {q0.x + qbyte(lx, k) * sx, q0.y + qbyte(ly, k) * sy, q0.z + qbyte(lz, k) * sz};
-                    const float mx[3] = {q0.x + qbyte(hx, k) * sx, q0.y + qbyte(hy, k) * sy, q0.z + qbyte(hz, k) * sz};
-                    t[k] = c[k] != NR_WIDE_EMPTY ? box_t(mn, mx, r, tmin, tmax) : NR_FMAX;
+                    const float e0 = __builtin_fmaf(qbyte(we[0], k), B[0], Ae[0]);
+                    const float e1 = __builtin_fmaf(qbyte(we[1], k), B[1], Ae[1]);
+                    const float e2 = __builtin_fmaf(qbyte(we[2], k), B[2], Ae[2]);
+                    const float x0 = __builtin_fmaf(qbyte(wx[0], k), B[0], Ax[0]);
+                    const float x1 = __builtin_fmaf(qbyte(wx[1], k), B[1], Ax[1]);
+                    const float x2 = __builtin_fmaf(qbyte(wx[2], k), B[2], Ax[2]);
+                    const float t0 = fmaxf(fmaxf(e0, e1), fmaxf(e2, tmin));
+                    const float t1 = fminf(fminf(x0, x1), fminf(x2, tmax));
+                    t[k] = (t0 <= t1 && c[k] != NR_WIDE_EMPTY) ? t0 : NR_FMAX;
                 }
 #define NR_CSWAP(a, b)                                                         \
     if (t[b] < t[a]) {                                                         \
