@@ -415,11 +415,11 @@ int nr_get_tuning(nr_ctx* ctx, nr_tuning* tuning);
 /* The project's autotuner: renders `desc` on `view` under candidate tunings
  * (coordinate descent over slots {1,2,4,6,8}, stream groups {1..4}, shade
  * workgroups per CU {4,8,16}, tail paths {0, 64k, 256k, 1M, 2M, 4M}, graph
- * iterations {0, 2, 8}), keeps the fastest (a candidate replaces the current
- * one only when it is at least 1% faster, so one timing's noise does not swap
- * settings that render equally fast), writes it to *tuning (may be NULL) and
- * clears the view's AOVs.  A desc of a few passes of the real workload is
- * enough. */
+ * iterations {0, 2, 8}), each timed as the faster of two renders after a
+ * one-pass warmup, keeps the fastest (a candidate replaces the current one only
+ * when it is at least 1% faster, so timing noise does not swap settings that
+ * render equally fast), writes it to *tuning (may be NULL) and clears the view's
+ * AOVs.  A desc of a few passes of the real workload is enough. */
 int nr_autotune(nr_ctx* ctx, int view, const nr_render_desc* desc, nr_tuning* tuning);
 
 /* Mesh tree builder for later nr_scene_upload calls (enum nr_build_mode). */

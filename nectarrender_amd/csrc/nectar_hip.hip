@@ -3229,12 +3229,15 @@ int nr_autotune(nr_ctx* c, int view, const nr_render_desc* d, nr_tuning* out) {
         if (int e = nr_aov_reset(c, view)) return e;
         if (int e = nr_render(c, view, &w)) return e;
         HIPCK(hipStreamSynchronize(c->stream));
-        if (int e = nr_aov_reset(c, view)) return e;
-        HIPCK(hipStreamSynchronize(c->stream));
-        auto t0 = std::chrono::steady_clock::now();
-        if (int e = nr_render(c, view, d)) return e;
-        HIPCK(hipStreamSynchronize(c->stream));
-        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        ms = 1e30;
+        for (int rep = 0; rep < 2; rep++) {      // the faster of two timings
+            if (int e = nr_aov_reset(c, view)) return e;
+            HIPCK(hipStreamSynchronize(c->stream));
+            auto t0 = std::chrono::steady_clock::now();
+            if (int e = nr_render(c, view, d)) return e;
+            HIPCK(hipStreamSynchronize(c->stream));
+            ms = std::min(ms, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
         return NR_OK;
     };
     double best_ms = 0.0;
@@ -3255,7 +3258,7 @@ int nr_autotune(nr_ctx* c, int view, const nr_render_desc* d, nr_tuning* out) {
             field = cand[k];
             double ms = 0.0;
             if (int e = measure(t, ms)) return e;
-            if (ms < 0.99 * best_ms) { best_ms = ms; best = t; }   // 1%: above one timing's noise
+            if (ms < 0.99 * best_ms) { best_ms = ms; best = t; }   // 1%: above the timings' noise
         }
     }
     if (int e = nr_set_tuning(c, &best)) return e;
