@@ -110,6 +110,32 @@ NR_HD float box_t(const float* mn, const float* mx, const Ray& r, float tmin, fl
     return tmax <= tmin ? NR_FMAX : tmin;
 }
 
+// box_t for a ray whose inverse direction components are all finite and nonzero:
+// then no slab distance is NaN ((P - o) * inv is finite or +-inf), so the selects
+// are min / max and the result is box_t's bit for bit.
+NR_HD float box_t_fin(const float* mn, const float* mx, const Ray& r, float tmin, float tmax) {
+    float lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float o = comp(r.o, a), iv = comp(r.inv, a);
+        const float t0 = (mn[a] - o) * iv, t1 = (mx[a] - o) * iv;
+        lo[a] = fminf(t0, t1);
+        hi[a] = fmaxf(t0, t1);
+    }
+    const float t0 = fmaxf(fmaxf(lo[0], lo[1]), fmaxf(lo[2], tmin));
+    const float t1 = fminf(fminf(hi[0], hi[1]), fminf(hi[2], tmax));
+    return t1 <= t0 ? NR_FMAX : t0;
+}
+NR_HD bool inv_finite(const Ray& r) {
+    bool f = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float m = fabsf(comp(r.inv, a));
+        f = f && m < INFINITY && m > 0.0f;
+    }
+    return f;
+}
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 per draw (core/random.h:50-55 with rocRAND's init semantics):
 // key = seed, counter = (draw/4, 0, subsequence, 0), word draw%4,
@@ -422,9 +448,9 @@ NR_D void load_node(const NR_GLB GNode* nodes, int32_t i, float* lmn, float* lmx
 #ifndef NR_LEAF_BATCH
 #define NR_LEAF_BATCH 1
 #endif
-template <bool ANY = false>
-NR_D int32_t mesh_closest(const GMesh& M, const Ray& r, float tmin, float tmax, Stack& st, int sp0,
-                          float& bt, float& bu, float& bv, bool& overflow) {
+template <bool ANY, bool FIN>
+NR_D int32_t mesh_closest_t(const GMesh& M, const Ray& r, float tmin, float tmax, Stack& st, int sp0,
+                            float& bt, float& bu, float& bv, bool& overflow) {
     int32_t best = -1;
     int sp = sp0;
     if (sp0 + 1 > st.cap) { overflow = true; return -1; }
@@ -444,8 +470,8 @@ NR_D int32_t mesh_closest(const GMesh& M, const Ray& r, float tmin, float tmax, 
             } else {
                 float lmn[3], lmx[3], rmn[3], rmx[3]; int32_t l, rr;
                 load_node(nodes, ref, lmn, lmx, rmn, rmx, l, rr);
-                float tl = box_t(lmn, lmx, r, tmin, tmax);
-                float tr = box_t(rmn, rmx, r, tmin, tmax);
+                float tl = FIN ? box_t_fin(lmn, lmx, r, tmin, tmax) : box_t(lmn, lmx, r, tmin, tmax);
+                float tr = FIN ? box_t_fin(rmn, rmx, r, tmin, tmax) : box_t(rmn, rmx, r, tmin, tmax);
                 bool left_near = tl < tr;
                 int32_t near_i = left_near ? l : rr, far_i = left_near ? rr : l;
                 float near_t = left_near ? tl : tr, far_t = left_near ? tr : tl;
@@ -480,6 +506,15 @@ NR_D int32_t mesh_closest(const GMesh& M, const Ray& r, float tmin, float tmax, 
         if (__ballot(active) == 0ull) break;
     }
     return best;
+}
+
+// The binary traversal; a wave whose rays all have finite, nonzero inverse
+// directions (every ray but axis-parallel ones) runs the min / max slab test.
+template <bool ANY = false>
+NR_D int32_t mesh_closest(const GMesh& M, const Ray& r, float tmin, float tmax, Stack& st, int sp0,
+                          float& bt, float& bu, float& bv, bool& overflow) {
+    if (__ballot(!inv_finite(r)) == 0ull) return mesh_closest_t<ANY, true>(M, r, tmin, tmax, st, sp0, bt, bu, bv, overflow);
+    return mesh_closest_t<ANY, false>(M, r, tmin, tmax, st, sp0, bt, bu, bv, overflow);
 }
 
 // The same closest-hit query over the 4-wide layout (fast mode): one 128 B
@@ -811,7 +846,8 @@ NR_D bool object_t(const GScene& S, const GObject& o, const Ray& r, Stack& st, i
             const GMesh& M = S.meshes[o.mesh];
             slot = (LAY == NR_LAY_QUANT && M.qnodes) ? mesh_closest_q<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow)
                    : (LAY == NR_LAY_WIDE && M.wnodes) ? mesh_closest_w<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow)
-                                                      : mesh_closest<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow);
+                                                      : LAY == NR_LAY_BINARY ? mesh_closest<ANY>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow)
+                   : mesh_closest_t<ANY, false>(M, orr, NR_EPS, NR_FMAX, st, sp0, tobj, u, v, overflow);
             h = slot >= 0;
             if (h) mesh_interp(M, slot, orr, tobj, u, v, rec);
         } else if ((TY & NR_TY_QUAD) && o.type == 1) {
