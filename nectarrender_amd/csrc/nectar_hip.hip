@@ -147,7 +147,7 @@ struct StackCfg {
 // carries the path's bounce index and d.w the sample index it is tracing (as
 // integer bits).  Queues hold path ids; their stride is qcap = slots * padded pixels.
 struct PathPtrs {
-    float4* o;
+    float4* o;          // o[2 p] origin and d[2 p] = o[2 p + 1] direction of path p
     float4* d;
     float4* atten;
     float4* L;
@@ -164,7 +164,7 @@ struct PathPtrs {
     float4* acc;        // running means during a render: beauty [0, cap), diffuse [cap, 2 cap), normal [2 cap, 3 cap)
     // fast mode: one pending shadow ray per path (origin, direction, MIS-weighted
     // contribution added to L if the ray reaches the sky), queued in qshd
-    float4* sho;
+    float4* sho;        // interleaved like o / d
     float4* shd;
     float4* shc;
     int32_t* qshd;
@@ -637,8 +637,8 @@ __device__ __forceinline__ uint32_t start_path(const GView& V, PathPtrs& P, uint
     uint32_t p = V.pix0 + i + (s % P.slots) * P.cap;
     Gen g = path_gen(A, i, s, 0u, V.npix);
     Ray r = camera_ray(cam, x, y, i, s, A.seed, g);
-    P.o[p] = make_float4(r.o.x, r.o.y, r.o.z, __int_as_float(0));
-    P.d[p] = make_float4(r.d.x, r.d.y, r.d.z, __int_as_float((int)s));
+    P.o[2 * (size_t)p] = make_float4(r.o.x, r.o.y, r.o.z, __int_as_float(0));
+    P.d[2 * (size_t)p] = make_float4(r.d.x, r.d.y, r.d.z, __int_as_float((int)s));
     P.atten[p] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
     // L, diff and nrm are not cleared: the first segment of the path writes
     // all three (radiance counted from zero at bounce 0)
@@ -860,7 +860,7 @@ __global__ void k_counts(int32_t* counts, int32_t in) {
 // Returns the material type of the hit, 0 on a miss.
 template <int TY, bool FAST>
 __device__ __forceinline__ int extend_one(const GScene& S, const PathPtrs& P, int path, Stack& st) {
-    float4 o4 = P.o[path], d4 = P.d[path];
+    float4 o4 = P.o[2 * (size_t)path], d4 = P.d[2 * (size_t)path];
     Ray r = mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
     HRec rec;
     hrec_clear(rec);
@@ -963,7 +963,7 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
                                           const float* marg, bool& cont, bool& shadow) {
     constexpr bool NEE_M = FAST && (M == NR_MAT_LAMBERTIAN || M == NR_MAT_PBR || M == NR_MAT_ISOTROPIC);
     GHit h = P.hit[path];
-    float4 o4 = P.o[path], d4 = P.d[path], a4 = P.atten[path];
+    float4 o4 = P.o[2 * (size_t)path], d4 = P.d[2 * (size_t)path], a4 = P.atten[path];
     const uint32_t bounce = (uint32_t)__float_as_int(o4.w);
     const float4 l4 = bounce == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.L[path];
     const uint32_t sample = (uint32_t)__float_as_int(d4.w);
@@ -1054,8 +1054,8 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
                     if (S.first_light >= 0) mp = 0.5f * mp + 0.5f * light_pdf(S.objects[S.first_light], p, ld);
                     f3 c = atten * f * sky_sample<true>(S.sky, ld) * (power_heuristic(ps, mp) / ps);
                     if (c.x > 0.0f || c.y > 0.0f || c.z > 0.0f) {
-                        P.sho[path] = make_float4(p.x, p.y, p.z, 0.0f);
-                        P.shd[path] = make_float4(ld.x, ld.y, ld.z, 0.0f);
+                        P.sho[2 * (size_t)path] = make_float4(p.x, p.y, p.z, 0.0f);
+                        P.shd[2 * (size_t)path] = make_float4(ld.x, ld.y, ld.z, 0.0f);
                         P.shc[path] = make_float4(c.x, c.y, c.z, 0.0f);
                         shadow = true;
                     }
@@ -1088,8 +1088,8 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
     P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
     if (alive && bounce + 1 < A.depth) {
         P.atten[path] = make_float4(atten.x, atten.y, atten.z, FAST ? sc_pdf : 0.0f);
-        P.o[path] = make_float4(p.x, p.y, p.z, __int_as_float((int)(bounce + 1)));
-        P.d[path] = make_float4(nd.x, nd.y, nd.z, d4.w);
+        P.o[2 * (size_t)path] = make_float4(p.x, p.y, p.z, __int_as_float((int)(bounce + 1)));
+        P.d[2 * (size_t)path] = make_float4(nd.x, nd.y, nd.z, d4.w);
         P.draw[path] = g.draw;
         cont = true;
     } else {
@@ -1133,7 +1133,7 @@ __global__ void __launch_bounds__(256) k_shade(const GScene* __restrict__ Sp, Pa
 // radiance.  Runs after every k_shade of the iteration and before k_regen.
 template <int TY>
 __device__ __forceinline__ void shadow_one(const GScene& S, const PathPtrs& P, int path, Stack& st) {
-    float4 o4 = P.sho[path], d4 = P.shd[path];
+    float4 o4 = P.sho[2 * (size_t)path], d4 = P.shd[2 * (size_t)path];
     bool of = false;
     bool blocked = scene_occluded<TY, NR_SHADOW_LAY>(S, mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), st, of);
     if (of) atomicOr(P.err, 1);
@@ -2318,8 +2318,9 @@ static int ensure_paths(nr_ctx* c, uint32_t n, uint32_t npad) {
     };
     const uint32_t slots = (uint32_t)c->slots;
     const size_t np = (size_t)slots * n;         // path slots
-    int e = al((void**)&P.o, sizeof(float4) * np);
-    if (!e) e = al((void**)&P.d, sizeof(float4) * np);
+    // origin and direction of a path side by side (one 32-byte record, read together)
+    int e = al((void**)&P.o, sizeof(float4) * 2 * np);
+    P.d = P.o + 1;
     if (!e) e = al((void**)&P.atten, sizeof(float4) * np);
     if (!e) e = al((void**)&P.L, sizeof(float4) * np);
     if (!e) e = al((void**)&P.diff, sizeof(float4) * np);
@@ -2334,8 +2335,8 @@ static int ensure_paths(nr_ctx* c, uint32_t n, uint32_t npad) {
     if (!e) e = al((void**)&P.nf, sizeof(uint32_t) * (size_t)n);
     if (!e) e = al((void**)&P.ns, sizeof(uint32_t) * (size_t)n);
     if (!e) e = al((void**)&P.acc, sizeof(float4) * 3 * (size_t)n);
-    if (!e) e = al((void**)&P.sho, sizeof(float4) * np);
-    if (!e) e = al((void**)&P.shd, sizeof(float4) * np);
+    if (!e) e = al((void**)&P.sho, sizeof(float4) * 2 * np);   // shadow ray origin, direction side by side
+    if (!e) P.shd = P.sho + 1;
     if (!e) e = al((void**)&P.shc, sizeof(float4) * np);
     if (!e) e = al((void**)&P.qshd, sizeof(int32_t) * nq);
     if (e) { free_list(c->path_mem); return e; }
