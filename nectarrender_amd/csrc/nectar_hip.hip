@@ -149,7 +149,8 @@ struct StackCfg {
 struct PathPtrs {
     float4* o;          // o[2 p] origin and d[2 p] = o[2 p + 1] direction of path p
     float4* d;
-    float4* atten;
+    float4* atten;      // one 64-byte record per path p: atten[4 p] throughput, L[4 p] = atten[4 p + 1]
+                        // radiance, diff[4 p] = atten[4 p + 2] albedo, nrm[4 p] = atten[4 p + 3] normal
     float4* L;
     float4* diff;
     float4* nrm;
@@ -639,7 +640,7 @@ __device__ __forceinline__ uint32_t start_path(const GView& V, PathPtrs& P, uint
     Ray r = camera_ray(cam, x, y, i, s, A.seed, g);
     P.o[2 * (size_t)p] = make_float4(r.o.x, r.o.y, r.o.z, __int_as_float(0));
     P.d[2 * (size_t)p] = make_float4(r.d.x, r.d.y, r.d.z, __int_as_float((int)s));
-    P.atten[p] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
+    P.atten[4 * (size_t)p] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
     // L, diff and nrm are not cleared: the first segment of the path writes
     // all three (radiance counted from zero at bounce 0)
     P.draw[p] = g.draw;
@@ -869,17 +870,17 @@ __device__ __forceinline__ int extend_one(const GScene& S, const PathPtrs& P, in
     if (of) atomicOr(P.err, 1);
     if (oi < 0) {                                                               // trace.cu:19-22
         const bool first = __float_as_int(o4.w) == 0;
-        float4 a4 = P.atten[path], l4 = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.L[path];
+        float4 a4 = P.atten[4 * (size_t)path], l4 = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.L[4 * (size_t)path];
         if (first) {                                                            // no hit: zero AOVs
-            P.diff[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            P.nrm[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            P.diff[4 * (size_t)path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            P.nrm[4 * (size_t)path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         f3 Le = sky_sample<FAST>(S.sky, r.d);
         // fast mode: a scattered ray that finds the sky shares it with the
         // shadow rays of next-event estimation (power heuristic)
         if (FAST && S.sky.marg && a4.w > 0.0f) Le = Le * power_heuristic(a4.w, sky_cdf_pdf(S.sky, r.d));
         f3 L = mk3(l4.x, l4.y, l4.z) + mk3(a4.x, a4.y, a4.z) * Le;
-        P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
+        P.L[4 * (size_t)path] = make_float4(L.x, L.y, L.z, 0.0f);
         P.finished[path] = 1;
         return 0;
     }
@@ -963,9 +964,9 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
                                           const float* marg, bool& cont, bool& shadow) {
     constexpr bool NEE_M = FAST && (M == NR_MAT_LAMBERTIAN || M == NR_MAT_PBR || M == NR_MAT_ISOTROPIC);
     GHit h = P.hit[path];
-    float4 o4 = P.o[2 * (size_t)path], d4 = P.d[2 * (size_t)path], a4 = P.atten[path];
+    float4 o4 = P.o[2 * (size_t)path], d4 = P.d[2 * (size_t)path], a4 = P.atten[4 * (size_t)path];
     const uint32_t bounce = (uint32_t)__float_as_int(o4.w);
-    const float4 l4 = bounce == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.L[path];
+    const float4 l4 = bounce == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.L[4 * (size_t)path];
     const uint32_t sample = (uint32_t)__float_as_int(d4.w);
     f3 din = mk3(d4.x, d4.y, d4.z);
     f3 atten = mk3(a4.x, a4.y, a4.z);
@@ -1073,9 +1074,9 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
     if (bounce == 0) {                                                      // trace.cu:67-70 (zero when
         f3 dv = s3(0.0f), nv = s3(0.0f);                                    // the path ends here)
         if (alive) { dv = dv + atten; nv = nv + nn; }
-        P.diff[path] = make_float4(dv.x, dv.y, dv.z, 0.0f);
+        P.diff[4 * (size_t)path] = make_float4(dv.x, dv.y, dv.z, 0.0f);
         // the DEPTH AOV rides in .w: distance from the camera to the first hit
-        P.nrm[path] = make_float4(nv.x, nv.y, nv.z, vlength(p - mk3(o4.x, o4.y, o4.z)));
+        P.nrm[4 * (size_t)path] = make_float4(nv.x, nv.y, nv.z, vlength(p - mk3(o4.x, o4.y, o4.z)));
     }
     if (alive) {
         if (A.do_rr && bounce >= A.rr_start && bounce + 1 < A.depth) {      // trace.cu:72-77
@@ -1085,9 +1086,9 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
             else atten = atten * s3(1.0f / survival);
         }
     }
-    P.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
+    P.L[4 * (size_t)path] = make_float4(L.x, L.y, L.z, 0.0f);
     if (alive && bounce + 1 < A.depth) {
-        P.atten[path] = make_float4(atten.x, atten.y, atten.z, FAST ? sc_pdf : 0.0f);
+        P.atten[4 * (size_t)path] = make_float4(atten.x, atten.y, atten.z, FAST ? sc_pdf : 0.0f);
         P.o[2 * (size_t)path] = make_float4(p.x, p.y, p.z, __int_as_float((int)(bounce + 1)));
         P.d[2 * (size_t)path] = make_float4(nd.x, nd.y, nd.z, d4.w);
         P.draw[path] = g.draw;
@@ -1138,8 +1139,8 @@ __device__ __forceinline__ void shadow_one(const GScene& S, const PathPtrs& P, i
     bool blocked = scene_occluded<TY, NR_SHADOW_LAY>(S, mkray(mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z)), st, of);
     if (of) atomicOr(P.err, 1);
     if (!blocked && !of) {
-        float4 c = P.shc[path], l = P.L[path];
-        P.L[path] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+        float4 c = P.shc[path], l = P.L[4 * (size_t)path];
+        P.L[4 * (size_t)path] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
     }
 }
 
@@ -1167,9 +1168,9 @@ k_shadow(const GScene* __restrict__ Sp, PathPtrs P, StackCfg SC) {
 __device__ __forceinline__ void fold_sample(const PathPtrs& P, const TraceArgs& A, uint32_t i, uint32_t p, uint32_t nf) {
     float fn = (float)(A.fast ? nf + 1 : nf);                   // fast mode counts pass 0 as the first
     float a = (fn - 1.0f) / fn, b = 1.0f / fn;
-    fold(P.acc + i, P.L[p], a, b);
-    fold(P.acc + P.cap + i, P.diff[p], a, b);
-    fold(P.acc + 2 * (size_t)P.cap + i, P.nrm[p], a, b);
+    fold(P.acc + i, P.L[4 * (size_t)p], a, b);
+    fold(P.acc + P.cap + i, P.diff[4 * (size_t)p], a, b);
+    fold(P.acc + 2 * (size_t)P.cap + i, P.nrm[4 * (size_t)p], a, b);
     P.finished[p] = 2;
 }
 
@@ -2321,10 +2322,8 @@ static int ensure_paths(nr_ctx* c, uint32_t n, uint32_t npad) {
     // origin and direction of a path side by side (one 32-byte record, read together)
     int e = al((void**)&P.o, sizeof(float4) * 2 * np);
     P.d = P.o + 1;
-    if (!e) e = al((void**)&P.atten, sizeof(float4) * np);
-    if (!e) e = al((void**)&P.L, sizeof(float4) * np);
-    if (!e) e = al((void**)&P.diff, sizeof(float4) * np);
-    if (!e) e = al((void**)&P.nrm, sizeof(float4) * np);
+    if (!e) e = al((void**)&P.atten, sizeof(float4) * 4 * np);   // throughput, radiance, albedo, normal
+    if (!e) { P.L = P.atten + 1; P.diff = P.atten + 2; P.nrm = P.atten + 3; }
     if (!e) e = al((void**)&P.draw, sizeof(uint32_t) * np);
     if (!e) e = al((void**)&P.hit, sizeof(GHit) * np);
     const size_t nq = (size_t)slots * npad;      // entries per queue
