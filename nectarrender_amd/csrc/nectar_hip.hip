@@ -154,7 +154,6 @@ struct PathPtrs {
     float4* L;
     float4* diff;
     float4* nrm;
-    uint32_t* draw;
     GHit* hit;
     int32_t* qact;      // 2 * qcap
     int32_t* qmat;      // NR_MAT_TYPES * qcap
@@ -641,9 +640,10 @@ __device__ __forceinline__ uint32_t start_path(const GView& V, PathPtrs& P, uint
     P.o[2 * (size_t)p] = make_float4(r.o.x, r.o.y, r.o.z, __int_as_float(0));
     P.d[2 * (size_t)p] = make_float4(r.d.x, r.d.y, r.d.z, __int_as_float((int)s));
     P.atten[4 * (size_t)p] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
-    // L, diff and nrm are not cleared: the first segment of the path writes
-    // all three (radiance counted from zero at bounce 0)
-    P.draw[p] = g.draw;
+    // L's radiance, diff and nrm are not cleared: the first segment of the path
+    // writes all three (radiance counted from zero at bounce 0).  L.w carries the
+    // path's Philox draw counter.
+    P.L[4 * (size_t)p] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(g.draw));
     P.finished[p] = 0;
     return p;
 }
@@ -966,7 +966,8 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
     GHit h = P.hit[path];
     float4 o4 = P.o[2 * (size_t)path], d4 = P.d[2 * (size_t)path], a4 = P.atten[4 * (size_t)path];
     const uint32_t bounce = (uint32_t)__float_as_int(o4.w);
-    const float4 l4 = bounce == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : P.L[4 * (size_t)path];
+    const float4 lr = P.L[4 * (size_t)path];       // w: the draw counter
+    const float4 l4 = bounce == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : lr;
     const uint32_t sample = (uint32_t)__float_as_int(d4.w);
     f3 din = mk3(d4.x, d4.y, d4.z);
     f3 atten = mk3(a4.x, a4.y, a4.z);
@@ -976,7 +977,7 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
     bool front = h.front_face != 0;
     const uint32_t gp = path_pixel(P, (uint32_t)path);
     const GView& V = A.views[pixel_view(A, gp)];
-    Gen g = path_gen(A, gp - V.pix0, sample, P.draw[path], V.npix);
+    Gen g = path_gen(A, gp - V.pix0, sample, __float_as_uint(lr.w), V.npix);
     const GMaterial& m = S.materials[h.material];
     // emission (material.h:182-188, 306-312)
     f3 em = s3(0.0f);
@@ -1086,12 +1087,11 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
             else atten = atten * s3(1.0f / survival);
         }
     }
-    P.L[4 * (size_t)path] = make_float4(L.x, L.y, L.z, 0.0f);
+    P.L[4 * (size_t)path] = make_float4(L.x, L.y, L.z, __uint_as_float(g.draw));
     if (alive && bounce + 1 < A.depth) {
         P.atten[4 * (size_t)path] = make_float4(atten.x, atten.y, atten.z, FAST ? sc_pdf : 0.0f);
         P.o[2 * (size_t)path] = make_float4(p.x, p.y, p.z, __int_as_float((int)(bounce + 1)));
         P.d[2 * (size_t)path] = make_float4(nd.x, nd.y, nd.z, d4.w);
-        P.draw[path] = g.draw;
         cont = true;
     } else {
         P.finished[path] = 1;
@@ -1140,7 +1140,7 @@ __device__ __forceinline__ void shadow_one(const GScene& S, const PathPtrs& P, i
     if (of) atomicOr(P.err, 1);
     if (!blocked && !of) {
         float4 c = P.shc[path], l = P.L[4 * (size_t)path];
-        P.L[4 * (size_t)path] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+        P.L[4 * (size_t)path] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, l.w);
     }
 }
 
@@ -1168,7 +1168,8 @@ k_shadow(const GScene* __restrict__ Sp, PathPtrs P, StackCfg SC) {
 __device__ __forceinline__ void fold_sample(const PathPtrs& P, const TraceArgs& A, uint32_t i, uint32_t p, uint32_t nf) {
     float fn = (float)(A.fast ? nf + 1 : nf);                   // fast mode counts pass 0 as the first
     float a = (fn - 1.0f) / fn, b = 1.0f / fn;
-    fold(P.acc + i, P.L[4 * (size_t)p], a, b);
+    const float4 lp = P.L[4 * (size_t)p];
+    fold(P.acc + i, make_float4(lp.x, lp.y, lp.z, 0.0f), a, b);
     fold(P.acc + P.cap + i, P.diff[4 * (size_t)p], a, b);
     fold(P.acc + 2 * (size_t)P.cap + i, P.nrm[4 * (size_t)p], a, b);
     P.finished[p] = 2;
@@ -2324,7 +2325,6 @@ static int ensure_paths(nr_ctx* c, uint32_t n, uint32_t npad) {
     P.d = P.o + 1;
     if (!e) e = al((void**)&P.atten, sizeof(float4) * 4 * np);   // throughput, radiance, albedo, normal
     if (!e) { P.L = P.atten + 1; P.diff = P.atten + 2; P.nrm = P.atten + 3; }
-    if (!e) e = al((void**)&P.draw, sizeof(uint32_t) * np);
     if (!e) e = al((void**)&P.hit, sizeof(GHit) * np);
     const size_t nq = (size_t)slots * npad;      // entries per queue
     if (!e) e = al((void**)&P.qact, sizeof(int32_t) * 2 * nq);
