@@ -53,7 +53,7 @@ TRAVERSAL_BYTES_PER_SEGMENT = (TRAVERSAL_BYTES_PER_PATH - 24.0) / 1.7945
 # factor a 1 GiB device-to-device copy gives) + WRITE_SIZE under the tuning the autotuner picks
 # (8 slots, 2 groups, 16 shading workgroups per CU, tail below 262,144 paths):
 # profiles/k_extend_c2_1080p.md.
-TRAVERSAL_HBM_BYTES_PER_SEGMENT = {0: 482.7, 1: 475.5}  # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
+TRAVERSAL_HBM_BYTES_PER_SEGMENT = {0: 519.8, 1: 493.2}  # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
 HBM_PEAK_GBS = 8000.0
 
 
