@@ -241,10 +241,11 @@ def main():
     elif args.streams:
         ctx.set_render_streams(args.streams)
     elif not args.no_autotune:
-        # the library's autotuner on 8 MIS passes of this workload: path slots per pixel, stream groups,
-        # shading grid (the result does not depend on them)
-        ctx.set_camera(0, with_spp(cam, 8), rp["seed"])
-        tuning = ctx.autotune(0, abi.render_desc(0, 8, rp["depth"], rp["seed"], mode=abi.MODE_FAST))
+        # the library's autotuner on the K MIS passes the timed region renders (path slots per pixel
+        # above the pass count cannot show their effect): path slots, stream groups, shading grid,
+        # tail threshold, graph capture (the result does not depend on them)
+        ctx.set_camera(0, with_spp(cam, K), rp["seed"])
+        tuning = ctx.autotune(0, abi.render_desc(0, K, rp["depth"], rp["seed"], mode=abi.MODE_FAST))
 
     def measure(mode):
         """K timed passes (fast mode: 0..K-1 of a K-sample pattern; the reference's estimator: 1..K of

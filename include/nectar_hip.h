@@ -393,7 +393,7 @@ int nr_set_render_streams(nr_ctx* ctx, int n);
  * are folded in order with the same random numbers).  No counterpart in the
  * reference (one CUDA thread per pixel, launch shape fixed). */
 typedef struct nr_tuning {
-    int32_t slots;            /* path slots per pixel, 1..8 (default 4): samples of a pixel in flight;
+    int32_t slots;            /* path slots per pixel, 1..32 (default 4): samples of a pixel in flight;
                                  the path state and queues hold slots x pixels entries */
     int32_t stream_groups;    /* 1..4 (default 2), as nr_set_render_streams */
     int32_t shade_wg_per_cu;  /* k_shade grid: 256-thread workgroups per CU, 1..32 (default 8) */
@@ -413,13 +413,15 @@ int nr_set_tuning(nr_ctx* ctx, const nr_tuning* tuning);
 int nr_get_tuning(nr_ctx* ctx, nr_tuning* tuning);
 
 /* The project's autotuner: renders `desc` on `view` under candidate tunings
- * (coordinate descent over slots {1,2,4,6,8}, stream groups {1..4}, shade
+ * (coordinate descent over slots {1,2,4,8,16,24,32}, skipping slot counts whose
+ * path state would take more than 60% of the device's memory, stream groups {1..4}, shade
  * workgroups per CU {4,8,16}, tail paths {0, 64k, 256k, 1M, 2M, 4M}, graph
  * iterations {0, 2, 8}), each timed as the faster of two renders after a
  * one-pass warmup, keeps the fastest (a candidate replaces the current one only
  * when it is at least 1% faster, so timing noise does not swap settings that
  * render equally fast), writes it to *tuning (may be NULL) and clears the view's
- * AOVs.  A desc of a few passes of the real workload is enough. */
+ * AOVs.  Give it the passes of the real renders: path slots above the desc's
+ * pass count cannot show their effect. */
 int nr_autotune(nr_ctx* ctx, int view, const nr_render_desc* desc, nr_tuning* tuning);
 
 /* Mesh tree builder for later nr_scene_upload calls (enum nr_build_mode). */

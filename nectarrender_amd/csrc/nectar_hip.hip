@@ -76,9 +76,13 @@ enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_SHD = 9, CNT_OCC = 10, CNT_PEN
 // sample s of pixel i runs in slot s % slots (path id i + (s % slots) * cap),
 // so up to `slots` consecutive samples of a pixel are in flight and pixels with
 // long paths do not hold back the whole view (1080p workload, 64 samples: 266
-// iterations with one slot, 146 with 2, 86 with 4; 4 is the fastest for both
-// estimators there).
-#define NR_SLOTS_MAX 8
+// iterations with one slot, 146 with 2, 86 with 4).  With 32, half of a 64-sample
+// render's paths are live at once (about 35 GB of path state and traversal stack at
+// 1080p) and the rays of one pixel's samples, neighbours in every queue, share their
+// tree nodes in cache: 1080p MIS 1284 (8 slots) -> 1368 (16) -> 1470 Mpaths/s (32).
+#ifndef NR_SLOTS_MAX
+#define NR_SLOTS_MAX 32
+#endif
 #define NR_SLOTS_DEFAULT 4
 // Live paths (upper bound) of a stream group below which the group switches
 // to tail mode (k_tail)
@@ -3190,7 +3194,7 @@ int nr_set_render_streams(nr_ctx* c, int n) {
 
 int nr_set_tuning(nr_ctx* c, const nr_tuning* t) {
     if (!c || !t) return fail(NR_ERR_INVALID, "null argument");
-    if (t->slots < 1 || t->slots > NR_SLOTS_MAX) return fail(NR_ERR_INVALID, "slots outside 1..8");
+    if (t->slots < 1 || t->slots > NR_SLOTS_MAX) return fail(NR_ERR_INVALID, "slots outside 1..32");
     if (t->stream_groups < 1 || t->stream_groups > NR_MAX_GROUPS) return fail(NR_ERR_INVALID, "stream groups outside 1..4");
     if (t->shade_wg_per_cu < 1 || t->shade_wg_per_cu > 32) return fail(NR_ERR_INVALID, "shade workgroups per CU outside 1..32");
     if (t->tail_paths < 0) return fail(NR_ERR_INVALID, "negative tail_paths");
@@ -3244,10 +3248,20 @@ int nr_autotune(nr_ctx* c, int view, const nr_render_desc* d, nr_tuning* out) {
     };
     double best_ms = 0.0;
     if (int e = measure(best, best_ms)) return e;
-    const int slot_c[] = {1, 2, 4, 6, 8}, group_c[] = {1, 2, 3, 4}, shade_c[] = {4, 8, 16},
+    // device memory one path slot per pixel of the view takes: path state (origin /
+    // direction, throughput / radiance / albedo / normal, hit, shadow ray, state byte),
+    // its queue entries and its traversal-stack spill columns
+    size_t dev_free = 0, dev_total = 0;
+    HIPCK(hipMemGetInfo(&dev_free, &dev_total));
+    const View& vw = c->views[view];
+    const double npad = (double)((vw.W + 7) / 8) * (double)((vw.H + 7) / 8) * 64.0;
+    const int lv = std::max(0, c->scene.stack_depth - LDS_STACK_LEVELS);
+    const double slot_bytes =
+        npad * (double)(2 * 16 + 4 * 16 + sizeof(GHit) + NR_SH_STRIDE * 16 + 1 + (2 + NR_MAT_TYPES + 1) * 4 + 8 * lv);
+    const int slot_c[] = {1, 2, 4, 8, 16, 24, 32}, group_c[] = {1, 2, 3, 4}, shade_c[] = {4, 8, 16},
               tail_c[] = {0, 65536, 262144, 1048576, 2097152, 4194304}, graph_c[] = {0, 2, 8};
     const int* cands[5] = {slot_c, group_c, shade_c, tail_c, graph_c};
-    const int n_cand[5] = {5, 4, 3, 6, 3};
+    const int n_cand[5] = {7, 4, 3, 6, 3};
     for (int axis = 0; axis < 5; axis++) {
         const int* cand = cands[axis];
         const int n = n_cand[axis];
@@ -3258,6 +3272,7 @@ int nr_autotune(nr_ctx* c, int view, const nr_render_desc* d, nr_tuning* out) {
             int32_t& field = *fields[axis];
             if (field == cand[k]) continue;
             field = cand[k];
+            if (axis == 0 && (double)t.slots * slot_bytes > 0.6 * (double)dev_total) continue;   // too big a view
             double ms = 0.0;
             if (int e = measure(t, ms)) return e;
             if (ms < 0.99 * best_ms) { best_ms = ms; best = t; }   // 1%: above the timings' noise
