@@ -2,7 +2,7 @@
 
 Uploads `scenes.config2()` (1920 x 1080, 1,003,520-triangle icosphere, HDRI with a 1e4 sun,
 depth 12), sets a K-sample pattern and renders passes 0..K-1 with the MIS estimator under
-each (stream groups, tail threshold) tuning, nine times, timing from `nr_render` to the end of
+each (stream groups, tail threshold) tuning with `--slots` path slots (8), nine times, timing from `nr_render` to the end of
 a stream synchronisation; prints the median and the minimum, then renders three more times
 under the last tuning (the dispatches a kernel trace of this command ends with).
 
@@ -19,6 +19,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--samples", type=int, default=1)
     ap.add_argument("--reps", type=int, default=9)
+    ap.add_argument("--slots", type=int, default=8, help="path slots per pixel of every tuning")
     args = ap.parse_args(argv)
     import torch
     torch.cuda.init()
@@ -40,11 +41,11 @@ def main(argv=None):
 
     for groups in (1, 2):
         for tail in (262144, 4194304):
-            ctx.set_tuning(8, groups, 16, tail, 0)
+            ctx.set_tuning(args.slots, groups, 16, tail, 0)
             ts = sorted(render() for _ in range(args.reps))
-            print(f"samples {args.samples}, groups {groups}, tail {tail}: median {ts[len(ts) // 2] * 1e3:.3f} ms, "
+            print(f"samples {args.samples}, slots {args.slots}, groups {groups}, tail {tail}: median {ts[len(ts) // 2] * 1e3:.3f} ms, "
                   f"min {ts[0] * 1e3:.3f} ms", flush=True)
-    ctx.set_tuning(8, 2, 16, 262144, 0)
+    ctx.set_tuning(args.slots, 2, 16, 262144, 0)
     for _ in range(3):
         render()
     ctx.close()
