@@ -393,7 +393,7 @@ int nr_set_render_streams(nr_ctx* ctx, int n);
  * are folded in order with the same random numbers).  No counterpart in the
  * reference (one CUDA thread per pixel, launch shape fixed). */
 typedef struct nr_tuning {
-    int32_t slots;            /* path slots per pixel, 1..32 (default 4): samples of a pixel in flight;
+    int32_t slots;            /* path slots per pixel, 1..64 (default 4): samples of a pixel in flight;
                                  the path state and queues hold slots x pixels entries */
     int32_t stream_groups;    /* 1..4 (default 2), as nr_set_render_streams */
     int32_t shade_wg_per_cu;  /* k_shade grid: 256-thread workgroups per CU, 1..32 (default 8) */
@@ -413,7 +413,7 @@ int nr_set_tuning(nr_ctx* ctx, const nr_tuning* tuning);
 int nr_get_tuning(nr_ctx* ctx, nr_tuning* tuning);
 
 /* The project's autotuner: renders `desc` on `view` under candidate tunings
- * (coordinate descent over slots {1,2,4,8,16,24,32}, skipping slot counts whose
+ * (coordinate descent over slots {1,2,4,8,16,32,48,64}, skipping slot counts whose
  * path state would take more than 60% of the device's memory, stream groups {1..4}, shade
  * workgroups per CU {4,8,16}, tail paths {0, 64k, 256k, 1M, 2M, 4M}, graph
  * iterations {0, 2, 8}), each timed as the faster of two renders after a

@@ -76,12 +76,12 @@ enum { CNT_ACT = 0, CNT_MAT = 2, CNT_ERR = 8, CNT_SHD = 9, CNT_OCC = 10, CNT_PEN
 // sample s of pixel i runs in slot s % slots (path id i + (s % slots) * cap),
 // so up to `slots` consecutive samples of a pixel are in flight and pixels with
 // long paths do not hold back the whole view (1080p workload, 64 samples: 266
-// iterations with one slot, 146 with 2, 86 with 4).  With 32, half of a 64-sample
-// render's paths are live at once (about 35 GB of path state and traversal stack at
-// 1080p) and the rays of one pixel's samples, neighbours in every queue, share their
-// tree nodes in cache: 1080p MIS 1284 (8 slots) -> 1368 (16) -> 1470 Mpaths/s (32).
+// iterations with one slot, 146 with 2, 86 with 4).  With 64, every sample of a
+// 64-sample render is live at once (about 70 GB of path state and traversal stack at
+// 1080p), so each traversal launch holds more rays and its end-of-launch drain is paid
+// less often: 1080p MIS 1284 (8 slots) -> 1368 (16) -> 1460 (32) -> 1611 Mpaths/s (64).
 #ifndef NR_SLOTS_MAX
-#define NR_SLOTS_MAX 32
+#define NR_SLOTS_MAX 64
 #endif
 #define NR_SLOTS_DEFAULT 4
 // Live paths (upper bound) of a stream group below which the group switches
@@ -253,14 +253,17 @@ __device__ __forceinline__ int block_append(bool pred, int32_t* counter) {
 }
 
 // workgroup-level append of `cnt` (< NR_SLOTS_MAX + 1) items per thread with
-// one atomic per workgroup: item k of a thread lands at pos[k]; the
-// workgroup's items are contiguous, ordered by rank k, then by thread.  When
-// `tally` is given, the workgroup's sum of `t` is added to it (one more
-// atomic).  Same-address device atomics serialise, so one per workgroup
-// instead of one per rank matters for launches of thousands of workgroups.
-template <int BLOCK>
-__device__ __forceinline__ void block_append_ranks(uint32_t cnt, int32_t* counter, int* pos, uint32_t t,
-                                                   int32_t* tally) {
+// one atomic per workgroup: item k of a thread lands at the position passed to
+// emit(k, pos); the workgroup's items are contiguous, ordered by rank k, then by
+// thread.  When `tally` is given, the workgroup's sum of `t` is added to it (one
+// more atomic).  Same-address device atomics serialise, so one per workgroup
+// instead of one per rank matters for launches of thousands of workgroups.  The
+// per-rank ballots are taken again when the positions are handed out, so no
+// per-rank state is held in registers (the rank loops end at the wave's largest
+// count).
+template <int BLOCK, class Emit>
+__device__ __forceinline__ void block_append_ranks(uint32_t cnt, int32_t* counter, uint32_t t, int32_t* tally,
+                                                   Emit&& emit) {
     constexpr int NW = BLOCK / 64;
     constexpr int NE = NR_SLOTS_MAX * NW;              // (rank, wave) entries, rank-major
     constexpr int PER = (NE + 63) / 64;
@@ -269,13 +272,16 @@ __device__ __forceinline__ void block_append_ranks(uint32_t cnt, int32_t* counte
     __shared__ int tsum[NW];
     const int wave = threadIdx.x >> 6, lane = __lane_id();
     const unsigned long long below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
-    unsigned long long m[NR_SLOTS_MAX];
     int tw = 0;
-#pragma unroll
     for (int k = 0; k < NR_SLOTS_MAX; k++) {
-        m[k] = __ballot(cnt > (uint32_t)k);
-        tw += __popcll(__ballot(t > (uint32_t)k));
-        if (lane == 0) rbase[k * NW + wave] = __popcll(m[k]);
+        const unsigned long long mk = __ballot(cnt > (uint32_t)k), tk = __ballot(t > (uint32_t)k);
+        if (lane == 0) rbase[k * NW + wave] = __popcll(mk);
+        tw += __popcll(tk);
+        if ((mk | tk) == 0ull) {                       // wave-uniform: the later ranks are empty
+            if (lane == 0)
+                for (int r = k + 1; r < NR_SLOTS_MAX; r++) rbase[r * NW + wave] = 0;
+            break;
+        }
     }
     if (lane == 0) tsum[wave] = tw;
     __syncthreads();
@@ -311,8 +317,11 @@ __device__ __forceinline__ void block_append_ranks(uint32_t cnt, int32_t* counte
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NR_SLOTS_MAX; k++) pos[k] = rbase[k * NW + wave] + __popcll(m[k] & below);
+    for (int k = 0; k < NR_SLOTS_MAX; k++) {
+        const unsigned long long mk = __ballot(cnt > (uint32_t)k);
+        if (mk == 0ull) break;                          // wave-uniform
+        if (cnt > (uint32_t)k) emit(k, rbase[k * NW + wave] + __popcll(mk & below));
+    }
 }
 
 // workgroup-level append of each thread's item to queue `key` (1 <= key < NQ,
@@ -833,14 +842,11 @@ __global__ void __launch_bounds__(256) k_start(PathPtrs P, uint32_t s, TraceArgs
         P.nf[g] = s;
         P.ns[g] = s + m;
     }
-    int pos[NR_SLOTS_MAX];
-    block_append_ranks<256>(m, &P.counts[CNT_ACT + 0], pos, 0u, nullptr);
-#pragma unroll
-    for (uint32_t k = 0; k < NR_SLOTS_MAX; k++) {
-        if (!valid || k >= P.slots) break;
-        if (k < m) P.qact[pos[k]] = (int32_t)start_path(A.views[v], P, x, y, s + k, A);
-        else P.finished[g + ((s + k) % P.slots) * P.cap] = 2;
-    }
+    block_append_ranks<256>(m, &P.counts[CNT_ACT + 0], 0u, nullptr, [&](int k, int pos) {
+        P.qact[pos] = (int32_t)start_path(A.views[v], P, x, y, s + (uint32_t)k, A);
+    });
+    if (valid)
+        for (uint32_t k = m; k < P.slots; k++) P.finished[g + ((s + k) % P.slots) * P.cap] = 2;
 }
 
 // every counter of a group but the overflow flag (shared by the groups; cleared
@@ -1185,12 +1191,11 @@ __device__ __forceinline__ void fold_sample(const PathPtrs& P, const TraceArgs& 
 constexpr int REGEN_BLOCK = 512;     // 1.2% faster renders than 1024 or 256 on the 1080p workload
 __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) {
     uint32_t t = A.t0 + blockIdx.x * blockDim.x + threadIdx.x, v = 0, x = 0, y = 0;
-    uint32_t started[NR_SLOTS_MAX];
-    uint32_t n_start = 0, occ = 0;
+    uint32_t n_start = 0, occ = 0, i = 0, ns0 = 0;
     bool pending = false;
     if (t < A.t1 && tile_pixel(A, t, v, x, y)) {
         const GView& V = A.views[v];
-        const uint32_t i = V.pix0 + y * V.cam.W + x;
+        i = V.pix0 + y * V.cam.W + x;
         const uint32_t end = pixel_end(A, x, y);
         uint32_t nf = P.nf[i], ns = P.ns[i];
 #pragma unroll
@@ -1200,10 +1205,10 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) 
             fold_sample(P, A, i, p, nf);
             nf++;
         }
-#pragma unroll
+        ns0 = ns;
         for (int k = 0; k < NR_SLOTS_MAX; k++) {
             if (ns >= end || ns - nf >= P.slots) break;         // slot ns % slots is idle when ns - nf < slots
-            started[k] = start_path(V, P, x, y, ns, A);
+            start_path(V, P, x, y, ns, A);
             n_start = k + 1;
             ns++;
         }
@@ -1212,14 +1217,14 @@ __global__ void __launch_bounds__(REGEN_BLOCK) k_regen(PathPtrs P, TraceArgs A) 
         occ = ns - nf;
         pending = ns < end;
     }
-    int pos[NR_SLOTS_MAX];
-    block_append_ranks<REGEN_BLOCK>(n_start, &P.counts[CNT_ACT + (1 - A.in)], pos, occ, &P.counts[CNT_OCC]);
+    // queue entries: the path of sample ns0 + k is pixel i's slot (ns0 + k) % slots
+    block_append_ranks<REGEN_BLOCK>(n_start, &P.counts[CNT_ACT + (1 - A.in)], occ, &P.counts[CNT_OCC],
+                                    [&](int k, int pos) {
+        P.qact[(size_t)(1 - A.in) * P.qcap + pos] = (int32_t)(i + ((ns0 + (uint32_t)k) % P.slots) * P.cap);
+    });
     // pixels that still have samples to start (tail mode waits until there are none)
     const int pend = __syncthreads_count(pending);
     if (threadIdx.x == 0 && pend) atomicAdd(&P.counts[CNT_PEND], pend);
-#pragma unroll
-    for (int k = 0; k < NR_SLOTS_MAX; k++)
-        if ((uint32_t)k < n_start) P.qact[(size_t)(1 - A.in) * P.qcap + pos[k]] = (int32_t)started[k];
 }
 
 // Tail mode.  Once few paths are live, a wavefront iteration costs about the
@@ -3194,7 +3199,7 @@ int nr_set_render_streams(nr_ctx* c, int n) {
 
 int nr_set_tuning(nr_ctx* c, const nr_tuning* t) {
     if (!c || !t) return fail(NR_ERR_INVALID, "null argument");
-    if (t->slots < 1 || t->slots > NR_SLOTS_MAX) return fail(NR_ERR_INVALID, "slots outside 1..32");
+    if (t->slots < 1 || t->slots > NR_SLOTS_MAX) return fail(NR_ERR_INVALID, "slots outside 1..64");
     if (t->stream_groups < 1 || t->stream_groups > NR_MAX_GROUPS) return fail(NR_ERR_INVALID, "stream groups outside 1..4");
     if (t->shade_wg_per_cu < 1 || t->shade_wg_per_cu > 32) return fail(NR_ERR_INVALID, "shade workgroups per CU outside 1..32");
     if (t->tail_paths < 0) return fail(NR_ERR_INVALID, "negative tail_paths");
@@ -3258,10 +3263,10 @@ int nr_autotune(nr_ctx* c, int view, const nr_render_desc* d, nr_tuning* out) {
     const int lv = std::max(0, c->scene.stack_depth - LDS_STACK_LEVELS);
     const double slot_bytes =
         npad * (double)(2 * 16 + 4 * 16 + sizeof(GHit) + NR_SH_STRIDE * 16 + 1 + (2 + NR_MAT_TYPES + 1) * 4 + 8 * lv);
-    const int slot_c[] = {1, 2, 4, 8, 16, 24, 32}, group_c[] = {1, 2, 3, 4}, shade_c[] = {4, 8, 16},
+    const int slot_c[] = {1, 2, 4, 8, 16, 32, 48, 64}, group_c[] = {1, 2, 3, 4}, shade_c[] = {4, 8, 16},
               tail_c[] = {0, 65536, 262144, 1048576, 2097152, 4194304}, graph_c[] = {0, 2, 8};
     const int* cands[5] = {slot_c, group_c, shade_c, tail_c, graph_c};
-    const int n_cand[5] = {7, 4, 3, 6, 3};
+    const int n_cand[5] = {8, 4, 3, 6, 3};
     for (int axis = 0; axis < 5; axis++) {
         const int* cand = cands[axis];
         const int n = n_cand[axis];

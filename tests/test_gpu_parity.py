@@ -196,7 +196,8 @@ def test_render_views_in_one_pipeline_equals_one_at_a_time(ctx, mode):
 
 @pytest.mark.parametrize("tuning", [(1, 1, 8, 0), (2, 2, 4, 262144), (8, 3, 16, 1 << 30), (6, 4, 2, 100),
                                     (4, 2, 8, 0), (4, 1, 8, 1 << 30), (4, 2, 8, 262144, 2), (8, 1, 16, 0, 8),
-                                    (2, 3, 4, 100, 16), (32, 2, 16, 262144), (24, 3, 8, 0, 2), (16, 1, 4, 1 << 30)])
+                                    (2, 3, 4, 100, 16), (32, 2, 16, 262144), (24, 3, 8, 0, 2), (16, 1, 4, 1 << 30),
+                                    (64, 2, 8, 262144), (48, 1, 8, 0)])
 @pytest.mark.parametrize("scene", ["mixed", "config1", "icosphere"])
 def test_render_same_for_any_tuning(ctx, oracle, tuning, scene):
     """Path slots per pixel, stream groups, the shading grid, the tail-mode threshold and graph-captured
@@ -231,7 +232,7 @@ def test_autotune_picks_a_valid_tuning_and_clears_the_view(ctx, oracle):
     ctx.set_camera(0, cam, rp["seed"])
     t = ctx.autotune(0, abi.render_desc(1, 8, rp["depth"], rp["seed"]))
     try:
-        assert t["slots"] in (1, 2, 4, 8, 16, 24, 32) and 1 <= t["stream_groups"] <= 4
+        assert t["slots"] in (1, 2, 4, 8, 16, 32, 48, 64) and 1 <= t["stream_groups"] <= 4
         assert 1 <= t["shade_wg_per_cu"] <= 32
         assert t["tail_paths"] in (0, 65536, 262144, 1048576, 2097152, 4194304)
         assert t["graph_iterations"] in (0, 2, 8)
