@@ -51,9 +51,9 @@ TRAVERSAL_BYTES_PER_SEGMENT = (TRAVERSAL_BYTES_PER_PATH - 24.0) / 1.7945
 # decision's probability), so the same model applies to its k_extend.
 # HBM bytes k_extend moves per traced ray on this workload, from rocprofv3 FETCH_SIZE (scaled by the
 # factor a 1 GiB device-to-device copy gives) + WRITE_SIZE over the 64 passes under the tuning the
-# autotuner picks (32 slots, 2 groups, 8 shading workgroups per CU, tail below 262,144 paths):
+# autotuner picks (64 slots, 2 groups, 8 shading workgroups per CU, tail below 1,048,576 paths):
 # profiles/k_extend_c2_1080p.md.
-TRAVERSAL_HBM_BYTES_PER_SEGMENT = {0: 546.0, 1: 517.5}  # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
+TRAVERSAL_HBM_BYTES_PER_SEGMENT = {0: 483.0, 1: 461.3}  # keyed by estimator (abi.MODE_PARITY, MODE_FAST)
 HBM_PEAK_GBS = 8000.0
 
 
