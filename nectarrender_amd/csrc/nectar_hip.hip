@@ -903,11 +903,7 @@ __device__ __forceinline__ int extend_one(const GScene& S, const PathPtrs& P, in
     h.n[0] = rec.n.x; h.n[1] = rec.n.y; h.n[2] = rec.n.z;
     h.tg[0] = rec.tg.x; h.tg[1] = rec.tg.y; h.tg[2] = rec.tg.z;
     h.uv[0] = rec.uv.x; h.uv[1] = rec.uv.y;
-    h.material = ob.material;
-    h.front_face = rec.front ? 1 : 0;
-    h.object_id = ob.object_id;
-    h.prim = rec.prim;
-    h.t = rec.t;
+    h.matf = (uint32_t)ob.material | (rec.front ? 0x80000000u : 0u);
     P.hit[path] = h;
     return ob.pad[0];
 }
@@ -987,11 +983,11 @@ __device__ __forceinline__ void shade_one(const GScene& S, const PathPtrs& P, co
     f3 L = mk3(l4.x, l4.y, l4.z);
     f3 p = ld3(h.p), nn = ld3(h.n), tg = ld3(h.tg);
     f2 uv; uv.x = h.uv[0]; uv.y = h.uv[1];
-    bool front = h.front_face != 0;
+    bool front = (h.matf >> 31) != 0;
     const uint32_t gp = path_pixel(P, (uint32_t)path);
     const GView& V = A.views[pixel_view(A, gp)];
     Gen g = path_gen(A, gp - V.pix0, sample, __float_as_uint(lr.w), V.npix);
-    const GMaterial& m = S.materials[h.material];
+    const GMaterial& m = S.materials[h.matf & 0x7FFFFFFFu];
     // emission (material.h:182-188, 306-312)
     f3 em = s3(0.0f);
     if (M == NR_MAT_PBR && front) em = tex_sample(S, m.tex[3], uv);

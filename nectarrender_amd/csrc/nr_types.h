@@ -171,15 +171,14 @@ struct GCamera {
     uint32_t n_pattern;
 };
 
-// closest-hit record written by the traversal stage for the shading stage (64 B)
+// closest-hit record written by the traversal stage for the shading stage (48 B:
+// what shading reads; the front-face flag in bit 31 of `matf`, the material index
+// in the rest)
 struct alignas(16) GHit {
     float p[3];
     float n[3];
     float tg[3];
     float uv[2];
-    int32_t material;
-    int32_t front_face;
-    int32_t object_id;
-    int32_t prim;
-    float t;
+    uint32_t matf;
 };
+static_assert(sizeof(GHit) == 48, "GHit is three 16-byte words");
